@@ -1,0 +1,203 @@
+/*
+ * mdt.h - C ABI of the MI355X-native metadynamics CV-and-bias path.
+ *
+ * Drop-in boundary for the Metad-toy LAMMPS plugin's hot path.  The plugin's
+ * FixMetadynamics / CV / Gaussian_Hill_Base C++ interfaces are replaced by
+ * this plain-pointer C ABI; a LAMMPS-side fix (or any MD engine) binds
+ * positions and full neighbour lists, hands over the `fix metad` keyword list
+ * unchanged, and calls mdt_fix_post_force once per step.  Positions,
+ * neighbour lists, selections, per-atom order parameters, bias grids and
+ * forces stay in device memory; per step the host sees only a few doubles.
+ *
+ * Conventions
+ *   - Every function returning int returns MDT_OK (0) or a negative status;
+ *     the message is available from mdt_last_error(ctx).  Nothing here calls
+ *     exit(), MPI or any collective.
+ *   - One HIP stream per context.  A context is not thread-safe.
+ *   - Positions are FP64, 3 per atom, locals first then ghost images (the
+ *     LAMMPS Atom::x layout).  Neighbour lists are CSR over local atoms with
+ *     entries indexing [0, n_all); list order is preserved (it defines
+ *     N-nearest tie-breaking).
+ *   - "walker" indexes an independent system; a context holds n_walkers of
+ *     them, all advanced together by one batched launch sequence per step.
+ *
+ * Reference paths below are relative to the Metad-toy source tree.
+ */
+#ifndef MDT_H
+#define MDT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDT_OK 0
+#define MDT_EINVAL (-22)   /* bad argument or bad fix keyword list        */
+#define MDT_ENOMEM (-12)   /* device or host allocation failed            */
+#define MDT_EHIP (-5)      /* a HIP runtime call failed                   */
+#define MDT_ENOTSUP (-95)  /* a feature the reference has but this build
+                              does not provide (message names it)        */
+#define MDT_ESTATE (-71)   /* call out of order (e.g. step before bind)   */
+
+typedef struct mdt_ctx mdt_ctx;
+typedef struct mdt_fix mdt_fix;
+typedef struct mdt_system mdt_system;
+
+/* ------------------------------------------------------------------------
+ * Context
+ * ---------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` holding `n_walkers` independent
+ * systems.  Replaces the per-rank GPU state the reference keeps in CV/env
+ * objects (src/cvs/CVs_Steinhardt.cu:189-228, include/utils/zqc_CVs_tools.h:33-229). */
+int mdt_create(mdt_ctx** ctx, int device, int n_walkers);
+void mdt_destroy(mdt_ctx* ctx);
+const char* mdt_last_error(const mdt_ctx* ctx);
+/* Library build tag ("mdt <version> gfx950"). */
+const char* mdt_version(void);
+
+/* ------------------------------------------------------------------------
+ * Inputs from the MD engine (replaces the host flattening in
+ * Steinhardt_env::refresh_lmpbox/get_env, src/cvs/CVs_Steinhardt.cu:391-566,
+ * and the NeighHub list hand-off, src/core/neigh_hub.cu:191-231).
+ * ---------------------------------------------------------------------- */
+
+/* Orthorhombic periodic box of one walker.  Each edge must exceed twice the
+ * list cutoff so an atom sees at most one image of another inside it. */
+int mdt_set_box(mdt_ctx* ctx, int walker, const double lo[3], const double hi[3],
+                const int periodic[3]);
+
+/* Positions x[n_all*3] (locals then ghosts).  on_device != 0 means `x` is a
+ * device pointer.  The data are copied; call again (or mdt_update_positions)
+ * whenever they change. */
+int mdt_bind_positions(mdt_ctx* ctx, int walker, const double* x, int64_t n_local,
+                       int64_t n_all, int on_device);
+
+/* Full neighbour list of every local atom: offsets[n_center+1], idx[offsets[n]],
+ * entries in [0, n_all); ghost_owner[n_all-n_local] maps each ghost image to
+ * its local atom (LAMMPS: atom->map(tag[j])).  LAMMPS NEIGHMASK bits must
+ * already be stripped.  Each list may hold at most 128 entries. */
+int mdt_bind_neighbors(mdt_ctx* ctx, int walker, const int32_t* offsets, const int32_t* idx,
+                       int64_t n_center, const int32_t* ghost_owner, int on_device);
+
+/* Optional group membership of local atoms (1 = in group; default: all).
+ * Replaces the mask/groupbit compaction of refresh_lmpbox (:391-425). */
+int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group);
+
+/* Convenience: box, positions and lists of a host-built mdt_system. */
+int mdt_bind_system(mdt_ctx* ctx, int walker, const mdt_system* sys);
+
+/* New positions for an already bound walker (same n_all, same lists). */
+int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_device);
+
+/* ------------------------------------------------------------------------
+ * fix metad
+ * ---------------------------------------------------------------------- */
+
+/* Parse the `fix ID group metad ...` keyword list (argv = the words after
+ * "metad") and build CVs, symbols, dimensions and the bias for every walker.
+ * Keywords: GAUSSIAN PACE RECORD CV_dim TEMP CAL SYMBOL|SIMBOL DIM
+ * Gaussian_Hill_type METAD_RESTART WT, plus BOLTZ <kB> and HILLS_FILE <path>.
+ * Replaces FixMetadynamics::FixMetadynamics (src/core/fix_crystallize.cu:32-302),
+ * CVFactory::create (src/core/CVFactory.cu:8-30) and Steinhardt::create
+ * (src/cvs/CVs_Steinhardt.cu:33-161). */
+int mdt_fix_create(mdt_ctx* ctx, int argc, const char* const* argv, mdt_fix** fix);
+void mdt_fix_destroy(mdt_fix* fix);
+
+/* First-run set-up: allocate the bias, open/replay HILLS (METAD_RESTART 1).
+ * Replaces FixMetadynamics::init (:343-378), GH_t0_uniformGrid::init_set_mode
+ * and init_hills (src/grids/GH_t0_uniformGrid.cu:73-192).  Called implicitly
+ * by the first step if omitted. */
+int mdt_fix_init(mdt_fix* fix);
+
+/* One MD step for every walker: CVs, hill deposit on step % PACE == 0 (before
+ * the force, as the reference does), bias value/gradient, chain rule and
+ * f += F_bias for local atoms.  `f` holds n_local*3 doubles per walker,
+ * walkers concatenated in order; on_device selects a device or host pointer;
+ * f == NULL accumulates into the context's own device force buffer.
+ * Replaces FixMetadynamics::post_force (src/core/fix_crystallize.cu:386-472). */
+int mdt_fix_post_force(mdt_fix* fix, int64_t step, double* f, int on_device);
+
+/* `nsteps` consecutive steps step0..step0+nsteps-1 into the context's device
+ * force buffer with one graph launch per step and no host synchronisation
+ * (standalone/batched driver).  Results are complete after mdt_fix_sync. */
+int mdt_fix_run(mdt_fix* fix, int64_t step0, int64_t nsteps);
+int mdt_fix_sync(mdt_fix* fix);
+
+/* Run `nsteps` steps with plain launches and a HIP event pair around every
+ * kernel on the context stream; ms[k] = mean duration of kernel k (names
+ * comma-separated in `names`).  Measurement aid with no reference
+ * counterpart (the reference times nothing below the LAMMPS Modify total). */
+int mdt_fix_profile(mdt_fix* fix, int64_t step0, int64_t nsteps, char* names, int names_cap,
+                    double* ms, int ms_cap, int* nkern);
+
+/* Bias energy V_b of a walker at the last step (compute_scalar, :484-487). */
+int mdt_fix_scalar(mdt_fix* fix, int walker, double* value);
+/* Dimension value s_n (compute_vector, :489-494). */
+int mdt_fix_vector(mdt_fix* fix, int walker, int n, double* value);
+/* Value of a CV function, e.g. "Q6.AVE" (STEIN_QL::compute_cv_AVE,
+ * src/cvs/CVs_Steinhardt.cu:824-839), at the last step. */
+int mdt_fix_cv_value(mdt_fix* fix, int walker, const char* cal_func, double* value);
+/* dV/dv for a CV function at the last step (the weight the chain rule gives
+ * STEIN_QL::bias_force_AVE, src/grids/DimSet.cu:83-97). */
+int mdt_fix_cv_weight(mdt_fix* fix, int walker, const char* cal_func, double* value);
+
+/* Per-atom export "CAL.prop" for n_local atoms of a walker (Fix::extract
+ * "colvar_peratom:<CAL>.<prop>", :592-613; props: stein_q, local_q, nsel,
+ * count_rc). */
+int mdt_fix_extract_peratom(mdt_fix* fix, int walker, const char* key, double* out);
+
+/* Literal N-nearest selection of a CAL's environment at the current
+ * positions, in the reference slot order (get_environment_Steinhardt_Q,
+ * src/cvs/CVs_Steinhardt.cu:1324-1406): sel_idx[n_local*N] (-1 = empty),
+ * count_rc[n_local], n_sel[n_local].  Any pointer may be NULL. */
+int mdt_fix_selection(mdt_fix* fix, int walker, const char* cal_name, int32_t* sel_idx,
+                      int32_t* count_rc, int32_t* n_sel);
+
+/* Copy the context's device force buffer of a walker to host (n_local*3). */
+int mdt_fix_get_forces(mdt_fix* fix, int walker, double* f_host);
+/* Zero the context's device force buffer. */
+int mdt_fix_zero_forces(mdt_fix* fix);
+
+/* Bias of a walker evaluated at np points s[np*D] on the device:
+ * V[np] and dV/ds[np*D] (either may be NULL).  Grid: Catmull-Rom as
+ * get_bias_energy/get_dVdcv (src/grids/GH_t0_uniformGrid.cu:312-461);
+ * hills sum: exact FP64 sum of Gaussians. */
+int mdt_fix_bias_eval(mdt_fix* fix, int walker, int64_t np, const double* s, double* V,
+                      double* dVds);
+/* Deposit nh hills (centres s[nh*D], heights h[nh], isotropic sigma) into a
+ * walker's bias as add_to_grid does (:226-282). */
+int mdt_fix_bias_add_hills(mdt_fix* fix, int walker, int64_t nh, const double* s,
+                           const double* h, double sigma);
+/* Copy a walker's dense grid (row-major, last dimension fastest). */
+int mdt_fix_grid(mdt_fix* fix, int walker, double* out, int64_t n);
+/* Number of hills deposited so far by a walker and a copy of them:
+ * rec[nh*(D+3)] = (step, s..., height, sigma) per hill. */
+int mdt_fix_hills(mdt_fix* fix, int walker, int64_t* nh, double* rec, int64_t cap);
+
+/* ------------------------------------------------------------------------
+ * Synthetic systems (host; the standalone driver's stand-in for LAMMPS
+ * create_atoms + neighbor build)
+ * ---------------------------------------------------------------------- */
+int mdt_system_create(mdt_system** out);
+void mdt_system_destroy(mdt_system* s);
+const char* mdt_system_error(const mdt_system* s);
+int mdt_system_set_box(mdt_system* s, const double lo[3], const double hi[3]);
+void mdt_system_get_box(const mdt_system* s, double lo[3], double hi[3]);
+int mdt_system_add_fcc(mdt_system* s, int nx, int ny, int nz, double a, double sigma_below,
+                       double sigma_above, double z_split, uint64_t seed);
+int mdt_system_add_uniform(mdt_system* s, int64_t n, double z0, double z1, uint64_t seed);
+int mdt_system_set_positions(mdt_system* s, const double* x, int64_t n);
+int mdt_system_build(mdt_system* s, double list_cutoff);
+int64_t mdt_system_n_local(const mdt_system* s);
+int64_t mdt_system_n_all(const mdt_system* s);
+int64_t mdt_system_nnz(const mdt_system* s);
+int mdt_system_copy(const mdt_system* s, double* x_all, int32_t* offsets, int32_t* idx,
+                    int32_t* ghost_owner);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDT_H */

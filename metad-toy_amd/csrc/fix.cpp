@@ -1,0 +1,1590 @@
+// Host side of the C ABI: context, input binding, the `fix metad` keyword
+// parser, DIM expression compiler, HILLS I/O and the per-step launch
+// sequence (plain launches for the drop-in path, one captured HIP graph per
+// step for the standalone/batched driver).
+//
+// Mirrors of the reference interface (paths relative to the Metad-toy tree):
+//   FixMetadynamics ctor / init / post_force   src/core/fix_crystallize.cu:32-472
+//   Steinhardt::create (STEINH keyword list)   src/cvs/CVs_Steinhardt.cu:33-161
+//   MetaDimensionManager (SYMBOL, DIM)         src/grids/DimSet.cu:28-97
+//   GH_t0_uniformGrid (grid, HILLS, restart)   src/grids/GH_t0_uniformGrid.cu:73-220
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "mdt.h"
+
+using namespace mdt;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+struct Err : std::exception {
+    int code;
+    std::string msg;
+    Err(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+std::string fmt(const char* f, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, f);
+    vsnprintf(buf, sizeof(buf), f, ap);
+    va_end(ap);
+    return buf;
+}
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw Err(MDT_EHIP, fmt("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                    __FILE__, __LINE__));                                 \
+    } while (0)
+
+template <class T>
+struct DBuf {  // device buffer
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Err(MDT_ENOMEM, fmt("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e)));
+        }
+        n = count;
+    }
+    // blocking: the host staging vectors are usually freed right after
+    void upload(const T* h, size_t count, hipStream_t st, size_t off = 0) {
+        if (!count) return;
+        HIPCHK(hipMemcpyAsync(p + off, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+};
+
+double parse_num(const char* s, const char* what) {
+    char* end = nullptr;
+    double v = strtod(s, &end);
+    if (!s[0] || (end && *end)) throw Err(MDT_EINVAL, fmt("expected a number for %s, got '%s'", what, s));
+    return v;
+}
+long long parse_int(const char* s, const char* what) {
+    char* end = nullptr;
+    long long v = strtoll(s, &end, 10);
+    if (!s[0] || (end && *end)) throw Err(MDT_EINVAL, fmt("expected an integer for %s, got '%s'", what, s));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Expression compiler: infix -> reverse Polish over symbols (DIM expressions;
+// the reference compiles them with exprtk, src/grids/DimSet.cu:54-66).
+// Supported: numbers, symbols, pi, + - * / ^, unary -, parentheses and
+// sin cos tan asin acos atan sinh cosh tanh exp log log10 sqrt abs pow min max.
+// ---------------------------------------------------------------------------
+struct ExprCompiler {
+    const std::string& src;
+    const std::vector<std::string>& syms;
+    size_t pos = 0;
+    ExprProg prog{};
+    ExprCompiler(const std::string& s, const std::vector<std::string>& y) : src(s), syms(y) { prog.n = 0; }
+
+    void emit(int op, double c = 0) {
+        if (prog.n >= kMaxProg) throw Err(MDT_EINVAL, "DIM expression too long");
+        prog.op[prog.n] = op;
+        prog.c[prog.n] = c;
+        prog.n++;
+    }
+    void ws() { while (pos < src.size() && isspace((unsigned char)src[pos])) ++pos; }
+    bool eat(char ch) { ws(); if (pos < src.size() && src[pos] == ch) { ++pos; return true; } return false; }
+    [[noreturn]] void fail(const char* why) {
+        throw Err(MDT_EINVAL, fmt("DIM expression '%s': %s at column %zu", src.c_str(), why, pos + 1));
+    }
+    void expr() {  // sum
+        term();
+        for (;;) {
+            if (eat('+')) { term(); emit(OP_ADD); }
+            else if (eat('-')) { term(); emit(OP_SUB); }
+            else break;
+        }
+    }
+    void term() {
+        unary();
+        for (;;) {
+            if (eat('*')) { unary(); emit(OP_MUL); }
+            else if (eat('/')) { unary(); emit(OP_DIV); }
+            else break;
+        }
+    }
+    void unary() {
+        if (eat('-')) { unary(); emit(OP_NEG); return; }
+        if (eat('+')) { unary(); return; }
+        power();
+    }
+    void power() {
+        primary();
+        if (eat('^')) { unary(); emit(OP_POW); }
+    }
+    void primary() {
+        ws();
+        if (pos >= src.size()) fail("unexpected end");
+        char ch = src[pos];
+        if (ch == '(') { ++pos; expr(); if (!eat(')')) fail("missing ')'"); return; }
+        if (isdigit((unsigned char)ch) || ch == '.') {
+            const char* b = src.c_str() + pos;
+            char* e = nullptr;
+            double v = strtod(b, &e);
+            if (e == b) fail("bad number");
+            pos += (size_t)(e - b);
+            emit(OP_CONST, v);
+            return;
+        }
+        if (isalpha((unsigned char)ch) || ch == '_') {
+            size_t b = pos;
+            while (pos < src.size() && (isalnum((unsigned char)src[pos]) || src[pos] == '_')) ++pos;
+            std::string name = src.substr(b, pos - b);
+            static const std::map<std::string, int> f1 = {
+                {"sin", OP_SIN}, {"cos", OP_COS}, {"tan", OP_TAN}, {"exp", OP_EXP}, {"log", OP_LOG},
+                {"sqrt", OP_SQRT}, {"abs", OP_ABS}, {"tanh", OP_TANH}, {"asin", OP_ASIN},
+                {"acos", OP_ACOS}, {"atan", OP_ATAN}, {"sinh", OP_SINH}, {"cosh", OP_COSH},
+                {"log10", OP_LOG10}};
+            static const std::map<std::string, int> f2 = {{"pow", OP_POW}, {"min", OP_MIN}, {"max", OP_MAX}};
+            ws();
+            if (pos < src.size() && src[pos] == '(') {
+                ++pos;
+                auto a = f1.find(name);
+                if (a != f1.end()) { expr(); if (!eat(')')) fail("missing ')'"); emit(a->second); return; }
+                auto b2 = f2.find(name);
+                if (b2 != f2.end()) {
+                    expr(); if (!eat(',')) fail("expected ','");
+                    expr(); if (!eat(')')) fail("missing ')'");
+                    emit(b2->second);
+                    return;
+                }
+                fail("unknown function");
+            }
+            if (name == "pi") { emit(OP_CONST, 3.14159265358979323846); return; }
+            for (size_t k = 0; k < syms.size(); ++k)
+                if (syms[k] == name) { emit(OP_SYM, (double)k); return; }
+            throw Err(MDT_EINVAL, fmt("DIM expression '%s': unknown symbol '%s'", src.c_str(), name.c_str()));
+        }
+        fail("unexpected character");
+    }
+    ExprProg compile() {
+        expr();
+        ws();
+        if (pos != src.size()) fail("trailing characters");
+        int depth = 0, maxd = 0;
+        for (int k = 0; k < prog.n; ++k) {
+            int op = prog.op[k];
+            if (op == OP_CONST || op == OP_SYM) depth++;
+            else if ((op >= OP_ADD && op <= OP_POW) || op == OP_MIN || op == OP_MAX) depth--;
+            maxd = std::max(maxd, depth);
+        }
+        if (maxd > kStack) throw Err(MDT_EINVAL, "DIM expression nests too deeply");
+        return prog;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// walkers and context
+// ---------------------------------------------------------------------------
+struct WalkerIn {
+    bool has_box = false, has_x = false, has_nl = false;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int64_t n_local = 0, n_all = 0, nnz = 0;
+    std::vector<double> x;
+    std::vector<int32_t> off, idx, owner;
+    std::vector<uint8_t> group;
+    // after commit
+    int64_t atom_off = 0, ghost_off = 0, nnz_off = 0;
+    int32_t tile_off = 0, ntiles = 0;
+};
+
+}  // namespace
+
+struct mdt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<WalkerIn> w;
+    std::string err;
+    bool committed = false;
+    uint64_t generation = 0;   // bumps on every commit (invalidates graphs)
+    int64_t total_local = 0, total_ghost = 0, total_nnz = 0;
+    bool any_group = false;
+    DBuf<double> x;
+    DBuf<int32_t> off, idx, owner;
+    DBuf<uint8_t> revpos, group;
+    DBuf<Tile> tiles;
+    DBuf<double> f;
+    std::vector<Tile> h_tiles;
+    std::vector<int32_t> tile_begin;
+    mdt_fix* fix = nullptr;
+
+    Geo geo() const {
+        Geo g{};
+        g.x = x.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p; g.revpos = revpos.p;
+        g.group = nullptr;
+        g.tiles = tiles.p; g.ntiles = (int32_t)h_tiles.size(); g.n_local_total = (int32_t)total_local;
+        return g;
+    }
+    WalkerIn& walker(int k) {
+        if (k < 0 || k >= (int)w.size()) throw Err(MDT_EINVAL, fmt("walker %d out of range [0,%zu)", k, w.size()));
+        return w[k];
+    }
+    void commit();
+};
+
+void mdt_ctx::commit() {
+    int64_t tl = 0, tg = 0, tn = 0;
+    for (size_t k = 0; k < w.size(); ++k) {
+        WalkerIn& a = w[k];
+        if (!a.has_box || !a.has_x || !a.has_nl)
+            throw Err(MDT_ESTATE, fmt("walker %zu needs box, positions and neighbour lists before the first step", k));
+        if ((int64_t)a.off.size() != a.n_local + 1)
+            throw Err(MDT_EINVAL, fmt("walker %zu: neighbour list has %zu centers, expected n_local=%lld",
+                                      k, a.off.size() - 1, (long long)a.n_local));
+        for (int64_t i = 0; i < a.n_local; ++i) {
+            int64_t c = (int64_t)a.off[i + 1] - a.off[i];
+            if (c < 0 || c > kMaxList)
+                throw Err(MDT_ENOTSUP, fmt("walker %zu atom %lld has %lld list entries; at most %d are supported",
+                                           k, (long long)i, (long long)c, kMaxList));
+        }
+        a.atom_off = tl; a.ghost_off = tg; a.nnz_off = tn;
+        tl += a.n_local; tg += a.n_all - a.n_local; tn += a.nnz;
+    }
+    if (tl + tg > INT32_MAX || tn > INT32_MAX) throw Err(MDT_ENOTSUP, "more than 2^31 atoms or list entries");
+    total_local = tl; total_ghost = tg; total_nnz = tn;
+    std::vector<double> hx((size_t)(tl + tg) * 3);
+    std::vector<int32_t> hoff((size_t)tl + 1), hidx((size_t)tn), hown((size_t)std::max<int64_t>(tg, 1));
+    std::vector<uint8_t> hgroup;
+    any_group = false;
+    for (auto& a : w) any_group |= !a.group.empty();
+    if (any_group) hgroup.assign((size_t)tl, 1);
+    h_tiles.clear();
+    tile_begin.assign(w.size() + 1, 0);
+    for (size_t k = 0; k < w.size(); ++k) {
+        WalkerIn& a = w[k];
+        std::memcpy(&hx[3 * a.atom_off], a.x.data(), sizeof(double) * 3 * a.n_local);
+        std::memcpy(&hx[3 * (tl + a.ghost_off)], a.x.data() + 3 * a.n_local, sizeof(double) * 3 * (a.n_all - a.n_local));
+        for (int64_t i = 0; i < a.n_local; ++i) hoff[a.atom_off + i] = (int32_t)(a.nnz_off + a.off[i]);
+        for (int64_t e = 0; e < a.nnz; ++e) {
+            int32_t j = a.idx[e];
+            if (j < 0 || j >= a.n_all)
+                throw Err(MDT_EINVAL, fmt("walker %zu: list entry %lld = %d outside [0,%lld)", k, (long long)e, j, (long long)a.n_all));
+            hidx[a.nnz_off + e] = j < a.n_local ? (int32_t)(a.atom_off + j) : (int32_t)(tl + a.ghost_off + (j - a.n_local));
+        }
+        for (int64_t g = 0; g < a.n_all - a.n_local; ++g) {
+            int32_t o = a.owner[g];
+            if (o < 0 || o >= a.n_local) throw Err(MDT_EINVAL, fmt("walker %zu: ghost %lld has owner %d outside [0,n_local)", k, (long long)g, o));
+            hown[a.ghost_off + g] = (int32_t)(a.atom_off + o);
+        }
+        if (!a.group.empty()) std::memcpy(&hgroup[a.atom_off], a.group.data(), a.n_local);
+        a.tile_off = (int32_t)h_tiles.size();
+        for (int64_t s = 0; s < a.n_local; s += kBlock) {
+            Tile t{};
+            t.start = (int32_t)(a.atom_off + s);
+            t.count = (int32_t)std::min<int64_t>(kBlock, a.n_local - s);
+            t.walker = (int32_t)k;
+            h_tiles.push_back(t);
+        }
+        a.ntiles = (int32_t)h_tiles.size() - a.tile_off;
+        tile_begin[k + 1] = (int32_t)h_tiles.size();
+    }
+    hoff[tl] = (int32_t)tn;
+    HIPCHK(hipSetDevice(device));
+    x.alloc(hx.size()); x.upload(hx.data(), hx.size(), stream);
+    off.alloc(hoff.size()); off.upload(hoff.data(), hoff.size(), stream);
+    idx.alloc(std::max<size_t>(hidx.size(), 1)); idx.upload(hidx.data(), hidx.size(), stream);
+    owner.alloc(hown.size()); owner.upload(hown.data(), tg, stream);
+    revpos.alloc(std::max<size_t>(hidx.size(), 1));
+    if (any_group) { group.alloc(hgroup.size()); group.upload(hgroup.data(), hgroup.size(), stream); }
+    else group.release();
+    tiles.alloc(std::max<size_t>(h_tiles.size(), 1)); tiles.upload(h_tiles.data(), h_tiles.size(), stream);
+    f.alloc((size_t)std::max<int64_t>(tl, 1) * 3);
+    HIPCHK(launch_fill(f.p, tl * 3, 0.0, stream));
+    HIPCHK(launch_revpos(geo(), tl, revpos.p, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    committed = true;
+    generation++;
+}
+
+// ---------------------------------------------------------------------------
+// fix
+// ---------------------------------------------------------------------------
+namespace {
+
+enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2 };
+
+struct EnvH {
+    double rc = 0;
+    int N = 0;
+    std::string group;
+    int lmask = 0;
+    bool biased = false;
+    EnvDev dev{};
+    DBuf<float> rec, ql, extra;
+    DBuf<int32_t> nsel, cnt;
+    DBuf<uint64_t> selmask;
+    DBuf<double> inv_ng;
+};
+
+struct CvH {
+    std::string name;
+    CvKind kind = CV_Q;
+    int l = 0;
+    int env = -1;
+    int slot = -1;          // value slot
+    int psum = -1;          // partial slot (Q, LD)
+    bool biased = false;
+    // LD
+    DBuf<float> lq, bq;
+    // SK
+    std::vector<double> kvec;   // nk*3
+    DBuf<double> kd, krev, apart, amp;
+    DBuf<float> ampf;
+    DBuf<int32_t> chunk, chunk_begin, natoms, abegin;
+    int nchunk = 0, max_atoms = 0;
+};
+
+struct SymH {
+    std::string name;
+    int cv = -1;
+    std::string func;
+};
+
+}  // namespace
+
+struct mdt_fix {
+    mdt_ctx* ctx = nullptr;
+    // parsed keywords
+    double sigma = 0.05, height0 = 0.1, biasf = 10.0, temp = 300.0, boltz = 8.617343e-5;
+    int pace = 100, cv_dim = 1, hill_type = 0;
+    bool restart = false, wt = false;
+    std::string hills_file = "HILLS";
+    std::vector<double> sigmas;                 // per-dimension (extension)
+    std::vector<double> lo, hi;
+    std::vector<int> nb;
+    std::vector<std::string> dim_expr;
+    std::vector<ExprProg> h_progs;
+    std::vector<EnvH*> envs;
+    std::vector<CvH*> cvs;
+    std::vector<SymH> syms;
+    // device state
+    bool inited = false;
+    uint64_t built_gen = 0;
+    int nslot_sum = 0;
+    DBuf<double> partial, direct, grid, hills, state, wcv;
+    DBuf<int64_t> nhills, step;
+    DBuf<int32_t> tile_begin, mlp_ok;
+    DBuf<ExprProg> progs;
+    DBuf<float> mlp;
+    ReduceDev red{};
+    int64_t hills_cap = 0;
+    std::vector<int64_t> hills_written;   // per walker, hills already in the HILLS file
+    std::vector<int64_t> hills_bound;     // per walker upper bound of deposited hills
+    std::vector<FILE*> hfiles;
+    // graph
+    hipGraphExec_t gexec = nullptr;
+    uint64_t graph_gen = 0;
+    // host staging
+    double* pin = nullptr;        // pinned scratch for small transfers
+    size_t pin_n = 0;
+    std::vector<double> h_state;
+    bool state_valid = false;
+
+    ~mdt_fix() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (FILE* fp : hfiles) if (fp) fclose(fp);
+        for (auto* e : envs) delete e;
+        for (auto* c : cvs) delete c;
+        if (pin) (void)hipHostFree(pin);
+    }
+    int D() const { return cv_dim; }
+    int nw() const { return (int)ctx->w.size(); }
+};
+
+namespace {
+
+void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
+    int i = 0;
+    auto need = [&](int k, const char* what) {
+        if (i + k >= argc) throw Err(MDT_EINVAL, fmt("Error: %s", what));
+    };
+    bool have_dim_kw = false;
+    std::map<int, std::pair<std::string, std::vector<double>>> dims;
+    while (i < argc) {
+        std::string kw = argv[i];
+        if (kw == "GAUSSIAN") {
+            need(3, "GAUSSIAN command requires 3 arguments: sigma, height, biasf.");
+            fx->sigma = parse_num(argv[i + 1], "GAUSSIAN sigma");
+            fx->height0 = parse_num(argv[i + 2], "GAUSSIAN height");
+            fx->biasf = parse_num(argv[i + 3], "GAUSSIAN biasf");
+            i += 4;
+        } else if (kw == "PACE") {
+            need(1, "PACE command requires 1 argument: integer timesteps.");
+            fx->pace = (int)parse_int(argv[i + 1], "PACE");
+            i += 2;
+        } else if (kw == "RECORD") {
+            // accepted for compatibility; the reference opens the file and never writes it
+            int a = i + 1;
+            while (a < argc) {
+                if (!strcmp(argv[a], "FILE_NAME") || !strcmp(argv[a], "REC_PACE")) {
+                    if (a + 1 >= argc) throw Err(MDT_EINVAL, fmt("Error: '%s' keyword requires a value", argv[a]));
+                    a += 2;
+                } else break;
+            }
+            i = a;
+        } else if (kw == "CV_dim") {
+            need(1, "CV_dim command requires 1 argument.");
+            fx->cv_dim = (int)parse_int(argv[i + 1], "CV_dim");
+            if (fx->cv_dim < 1 || fx->cv_dim > kMaxDim)
+                throw Err(MDT_ENOTSUP, fmt("CV_dim %d: only 1-%d dimensions are supported", fx->cv_dim, kMaxDim));
+            have_dim_kw = true;
+            i += 2;
+        } else if (kw == "TEMP") {
+            need(1, "TEMP command requires 1 argument: target temperature.");
+            fx->temp = parse_num(argv[i + 1], "TEMP");
+            i += 2;
+        } else if (kw == "BOLTZ") {
+            need(1, "BOLTZ requires 1 argument (Boltzmann constant in energy units per K).");
+            fx->boltz = parse_num(argv[i + 1], "BOLTZ");
+            i += 2;
+        } else if (kw == "HILLS_FILE") {
+            need(1, "HILLS_FILE requires a path (NONE disables the file).");
+            fx->hills_file = argv[i + 1];
+            i += 2;
+        } else if (kw == "SIGMAS") {
+            // per-dimension hill widths for the hills-sum and learned biases
+            int d = fx->cv_dim;
+            need(d, "SIGMAS requires CV_dim values.");
+            fx->sigmas.clear();
+            for (int k = 0; k < d; ++k) fx->sigmas.push_back(parse_num(argv[i + 1 + k], "SIGMAS"));
+            i += 1 + d;
+        } else if (kw == "CAL") {
+            need(3, "CAL requires NAME <name> <type> ...");
+            if (strcmp(argv[i + 1], "NAME")) throw Err(MDT_EINVAL, "Error: CAL requires NAME keyword.");
+            std::string name = argv[i + 2], type = argv[i + 3];
+            for (auto* c : fx->cvs)
+                if (c->name == name) throw Err(MDT_EINVAL, fmt("Error: CAL name '%s' is duplicated.", name.c_str()));
+            i += 4;
+            if (type == "STEINH") {
+                if (i + 2 >= argc)
+                    throw Err(MDT_EINVAL, "Error: STEINH command requires \"STEINH <Q/L/LD> <3/4/6> <group> \".");
+                std::string qt = argv[i];
+                int l = (int)parse_int(argv[i + 1], "STEINH order");
+                std::string group = argv[i + 2];
+                i += 3;
+                if (l != 3 && l != 4 && l != 6 && l != 8 && l != 12)
+                    throw Err(MDT_EINVAL, "Error: Steinhardt order L must be 3, 4, 6, 8, or 12.");
+                if (l == 8 || l == 12) throw Err(MDT_ENOTSUP, fmt("STEINH order %d is accepted by the reference parser but has no implementation there either", l));
+                double rc = 4.0, eps = 1e-6;
+                int N = 12;
+                while (i < argc) {
+                    std::string k2 = argv[i];
+                    if (k2 == "cutoff_r") { need(1, "'cutoff_r' keyword requires a value"); rc = parse_num(argv[i + 1], "cutoff_r"); i += 2; }
+                    else if (k2 == "cutoff_Natoms") { need(1, "'cutoff_Natoms' keyword requires an integer"); N = (int)parse_int(argv[i + 1], "cutoff_Natoms"); i += 2; }
+                    else if (k2 == "cutoff_eps") { need(1, "'cutoff_eps' keyword requires a value"); eps = parse_num(argv[i + 1], "cutoff_eps"); i += 2; }
+                    else if (k2 == "d_block_size") { need(1, "'d_block_size' keyword requires an integer");
+                        if (parse_int(argv[i + 1], "d_block_size") <= 0) throw Err(MDT_EINVAL, "Error: 'd_block_size' must be > 0");
+                        i += 2; }
+                    else if (k2 == "SW_FUNC_r" || k2 == "SW_FUNC_cv")
+                        throw Err(MDT_ENOTSUP, fmt("%s: switching-function STEINH variants are not provided", k2.c_str()));
+                    else break;
+                }
+                (void)eps;
+                if (qt == "L") throw Err(MDT_ENOTSUP, "STEINH L (switching-weighted local Q) is not provided; use STEINH LD (Lechner-Dellago over the N nearest)");
+                if (qt != "Q" && qt != "LD") throw Err(MDT_EINVAL, "Error: Steinhardt type must be 'Q', 'L' or 'LD'.");
+                if (N < 1 || N > kMaxSlots) throw Err(MDT_ENOTSUP, fmt("cutoff_Natoms %d outside [1,%d]", N, kMaxSlots));
+                if (!(rc > 0)) throw Err(MDT_EINVAL, "cutoff_r must be positive");
+                int ei = -1;
+                for (size_t k = 0; k < fx->envs.size(); ++k)
+                    if (fx->envs[k]->rc == rc && fx->envs[k]->N == N && fx->envs[k]->group == group) ei = (int)k;
+                if (ei < 0) {
+                    EnvH* e = new EnvH();
+                    e->rc = rc; e->N = N; e->group = group;
+                    fx->envs.push_back(e);
+                    ei = (int)fx->envs.size() - 1;
+                }
+                fx->envs[ei]->lmask |= l == 3 ? kL3 : l == 4 ? kL4 : kL6;
+                CvH* c = new CvH();
+                c->name = name; c->kind = qt == "Q" ? CV_Q : CV_LD; c->l = l; c->env = ei;
+                fx->cvs.push_back(c);
+            } else if (type == "SK") {
+                // SK <group> [FCC111 a] [KVEC kx ky kz]... [NRAND n KMIN a KMAX b SEED s]
+                if (i >= argc) throw Err(MDT_EINVAL, "SK requires a group");
+                std::string group = argv[i++];
+                if (group != "all") throw Err(MDT_ENOTSUP, "SK supports group all only");
+                CvH* c = new CvH();
+                c->name = name; c->kind = CV_SK;
+                long long nrand = 0, seed = 12345;
+                double kmin = 1.5, kmax = 3.5;
+                while (i < argc) {
+                    std::string k2 = argv[i];
+                    if (k2 == "FCC111") {
+                        need(1, "FCC111 requires the lattice constant");
+                        double a = parse_num(argv[i + 1], "FCC111");
+                        double k0 = 2.0 * 3.14159265358979323846 / a;
+                        for (int sx = -1; sx <= 1; sx += 2)
+                            for (int sy = -1; sy <= 1; sy += 2)
+                                for (int sz = -1; sz <= 1; sz += 2) {
+                                    c->kvec.push_back(sx * k0); c->kvec.push_back(sy * k0); c->kvec.push_back(sz * k0);
+                                }
+                        i += 2;
+                    } else if (k2 == "KVEC") {
+                        need(3, "KVEC requires 3 components");
+                        for (int d = 0; d < 3; ++d) c->kvec.push_back(parse_num(argv[i + 1 + d], "KVEC"));
+                        i += 4;
+                    } else if (k2 == "NRAND") { need(1, "NRAND requires a count"); nrand = parse_int(argv[i + 1], "NRAND"); i += 2; }
+                    else if (k2 == "KMIN") { need(1, "KMIN requires a value"); kmin = parse_num(argv[i + 1], "KMIN"); i += 2; }
+                    else if (k2 == "KMAX") { need(1, "KMAX requires a value"); kmax = parse_num(argv[i + 1], "KMAX"); i += 2; }
+                    else if (k2 == "SEED") { need(1, "SEED requires an integer"); seed = parse_int(argv[i + 1], "SEED"); i += 2; }
+                    else break;
+                }
+                // seeded isotropic random k: |k| ~ U[kmin,kmax], direction uniform on the sphere
+                uint64_t st = (uint64_t)seed;
+                auto next = [&]() {
+                    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+                    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                    z ^= z >> 31;
+                    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+                };
+                for (long long q = 0; q < nrand; ++q) {
+                    double km = kmin + (kmax - kmin) * next();
+                    double cz = 2.0 * next() - 1.0, ph = 2.0 * 3.14159265358979323846 * next();
+                    double sz = std::sqrt(std::max(0.0, 1.0 - cz * cz));
+                    c->kvec.push_back(km * sz * std::cos(ph));
+                    c->kvec.push_back(km * sz * std::sin(ph));
+                    c->kvec.push_back(km * cz);
+                }
+                if (c->kvec.empty()) { delete c; throw Err(MDT_EINVAL, "SK needs at least one k vector"); }
+                fx->cvs.push_back(c);
+            } else if (type == "SW_FUNC") {
+                throw Err(MDT_ENOTSUP, "CAL ... SW_FUNC: switching functions are not provided");
+            } else if (type == "STRU_FACTOR" || type == "WEIGHT_CHEM" || type == "DISTANCE") {
+                throw Err(MDT_ENOTSUP, fmt("CV type %s is outside this library's scope", type.c_str()));
+            } else {
+                throw Err(MDT_EINVAL, fmt("Metad-toy Error: Unknown or unregistered object type '%s' specified in CAL '%s'.",
+                                          type.c_str(), name.c_str()));
+            }
+        } else if (kw == "SYMBOL" || kw == "SIMBOL") {
+            need(2, "SYMBOL requires 2 arguments: SYMBOL <symbol_name> <cv_method>.");
+            std::string target = argv[i + 2];
+            size_t dot = target.find('.');
+            SymH s;
+            s.name = argv[i + 1];
+            std::string cname = dot == std::string::npos ? target : target.substr(0, dot);
+            s.func = dot == std::string::npos ? "value" : target.substr(dot + 1);
+            for (size_t k = 0; k < fx->cvs.size(); ++k) if (fx->cvs[k]->name == cname) s.cv = (int)k;
+            if (s.cv < 0) throw Err(MDT_EINVAL, fmt("SYMBOL %s: CAL '%s' is not defined", s.name.c_str(), cname.c_str()));
+            if (s.func != "AVE") throw Err(MDT_EINVAL, fmt("We can't find the func %s.", s.func.c_str()));
+            for (auto& o : fx->syms) if (o.name == s.name) throw Err(MDT_EINVAL, fmt("SYMBOL %s defined twice", s.name.c_str()));
+            if ((int)fx->syms.size() >= kMaxSym) throw Err(MDT_ENOTSUP, fmt("at most %d symbols", kMaxSym));
+            fx->syms.push_back(s);
+            i += 3;
+        } else if (kw == "DIM") {
+            need(5, "DIM command requires 5 arguments: index, lower, upper, bins, cv_expr.");
+            int d = (int)parse_int(argv[i + 1], "DIM index") - 1;
+            if (d < 0 || d >= fx->cv_dim) throw Err(MDT_EINVAL, "Error: DIM index out of range or unsupported dimension.");
+            double lo = parse_num(argv[i + 2], "DIM lower"), hi = parse_num(argv[i + 3], "DIM upper");
+            int nbins = (int)parse_int(argv[i + 4], "DIM bins");
+            dims[d] = {argv[i + 5], {lo, hi, (double)nbins}};
+            i += 6;
+        } else if (kw == "Gaussian_Hill_type") {
+            need(1, "Gaussian_Hill_type requires 1 argument.");
+            fx->hill_type = (int)parse_int(argv[i + 1], "Gaussian_Hill_type");
+            i += 2;
+        } else if (kw == "METAD_RESTART") {
+            need(1, "METAD_RESTART command requires 1 argument: 0 or 1.");
+            fx->restart = parse_int(argv[i + 1], "METAD_RESTART") != 0;
+            i += 2;
+        } else if (kw == "WT") {
+            need(1, "WT requires 1 argument: 0 or 1.");
+            fx->wt = parse_int(argv[i + 1], "WT") != 0;
+            i += 2;
+        } else {
+            throw Err(MDT_EINVAL, fmt("Error: Unknown keyword in fix metadynamics command: %s", argv[i]));
+        }
+    }
+    (void)have_dim_kw;
+    for (int d = 0; d < fx->cv_dim; ++d) {
+        auto it = dims.find(d);
+        if (it == dims.end()) throw Err(MDT_EINVAL, fmt("CV_dim is %d but DIM %d is missing", fx->cv_dim, d + 1));
+        fx->dim_expr.push_back(it->second.first);
+        fx->lo.push_back(it->second.second[0]);
+        fx->hi.push_back(it->second.second[1]);
+        fx->nb.push_back((int)it->second.second[2]);
+        if (!(fx->hi[d] > fx->lo[d])) throw Err(MDT_EINVAL, fmt("DIM %d: upper bound must exceed lower", d + 1));
+    }
+    if (fx->hill_type == 1) throw Err(MDT_ENOTSUP, "Gaussian_Hill_type 1 (sparse hash grid) is not provided; use 0 (grid), 2 (exact hills sum) or 3 (learned bias)");
+    if (fx->hill_type != 0 && fx->hill_type != 2 && fx->hill_type != 3)
+        throw Err(MDT_EINVAL, fmt("Gaussian_Hill_type %d unknown", fx->hill_type));
+    if (fx->hill_type == 0) {
+        if (fx->cv_dim > 2) throw Err(MDT_ENOTSUP, "the uniform grid supports 1 or 2 dimensions; use Gaussian_Hill_type 2 or 3 for 3-D");
+        for (int d = 0; d < fx->cv_dim; ++d)
+            if (fx->nb[d] < 4) throw Err(MDT_EINVAL, "DIM bins must be >= 4 for Catmull-Rom interpolation");
+    }
+    if (fx->hill_type == 3 && fx->cv_dim != 3) throw Err(MDT_ENOTSUP, "the learned bias is built for CV_dim 3");
+    if (fx->sigmas.empty()) fx->sigmas.assign(fx->cv_dim, fx->sigma);
+    if (fx->syms.empty()) throw Err(MDT_EINVAL, "fix metad needs at least one SYMBOL");
+    if ((int)fx->cvs.size() > kMaxSlotCV) throw Err(MDT_ENOTSUP, fmt("at most %d CALs", kMaxSlotCV));
+    // compile the DIM expressions against the symbol table (DimSet.cu:54-66)
+    std::vector<std::string> names;
+    for (auto& s : fx->syms) names.push_back(s.name);
+    fx->h_progs.clear();
+    for (int d = 0; d < fx->cv_dim; ++d) fx->h_progs.push_back(ExprCompiler(fx->dim_expr[d], names).compile());
+}
+
+bool prog_uses(const ExprProg& p, int sym) {
+    for (int k = 0; k < p.n; ++k) if (p.op[k] == OP_SYM && (int)p.c[k] == sym) return true;
+    return false;
+}
+
+void open_hills(mdt_fix* fx);
+
+void fix_build(mdt_fix* fx) {
+    mdt_ctx* ctx = fx->ctx;
+    if (!ctx->committed) ctx->commit();
+    const int nw = fx->nw();
+    const int64_t NT = ctx->total_local;
+    hipStream_t st = ctx->stream;
+    const std::vector<ExprProg>& progs = fx->h_progs;
+    // slots
+    int nsum = 0;
+    for (size_t c = 0; c < fx->cvs.size(); ++c) fx->cvs[c]->slot = (int)c;
+    for (auto* c : fx->cvs) c->biased = false;
+    for (size_t q = 0; q < fx->syms.size(); ++q) {
+        bool used = false;
+        for (auto& p : progs) used |= prog_uses(p, (int)q);
+        if (used) fx->cvs[fx->syms[q].cv]->biased = true;
+    }
+    // group counts per walker
+    std::vector<double> ng_all(nw), ng_grp(nw);
+    for (int k = 0; k < nw; ++k) {
+        WalkerIn& a = ctx->w[k];
+        ng_all[k] = (double)a.n_local;
+        double g = 0;
+        if (!a.group.empty()) for (auto v : a.group) g += v ? 1 : 0; else g = (double)a.n_local;
+        ng_grp[k] = g;
+    }
+    // environments
+    for (auto* e : fx->envs) {
+        const int lm = e->lmask;
+        const int RS = rec_stride(lm), NL = rec_nl(lm);
+        e->rec.alloc((size_t)NT * RS);
+        e->ql.alloc((size_t)NT * NL);
+        e->nsel.alloc(NT); e->cnt.alloc(NT); e->selmask.alloc((size_t)NT * 2);
+        e->inv_ng.alloc(nw);
+        std::vector<double> inv(nw);
+        const bool all = e->group == "all";
+        if (!all && !ctx->any_group) throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", e->group.c_str()));
+        for (int k = 0; k < nw; ++k) {
+            double ng = all ? ng_all[k] : ng_grp[k];
+            inv[k] = ng > 0 ? 1.0 / ng : 0.0;
+        }
+        e->inv_ng.upload(inv.data(), nw, st);
+        EnvDev& d = e->dev;
+        d = EnvDev{};
+        d.rc2 = e->rc * e->rc; d.N = e->N; d.lmask = lm;
+        d.rec = e->rec.p; d.ql = e->ql.p; d.nsel = e->nsel.p; d.cnt_rc = e->cnt.p; d.selmask = e->selmask.p;
+        d.inv_ng = e->inv_ng.p;
+        int li = 0;
+        for (int l : {3, 4, 6}) {
+            int bit = l == 3 ? kL3 : l == 4 ? kL4 : kL6;
+            if (!(lm & bit)) continue;
+            d.slot_ql[li] = nsum++;
+            d.wslot[li] = -1;
+            for (auto* c : fx->cvs)
+                if (c->kind == CV_Q && fx->envs[c->env] == e && c->l == l) {
+                    c->psum = d.slot_ql[li];
+                    if (c->biased) {
+                        if (d.wslot[li] >= 0) throw Err(MDT_ENOTSUP, "two biased CALs with identical STEINH Q definitions");
+                        d.wslot[li] = c->slot;
+                    }
+                }
+            li++;
+        }
+        e->biased = false;
+        for (int k = 0; k < NL; ++k) e->biased |= d.wslot[k] >= 0;
+        bool ld_biased = false;
+        for (auto* c : fx->cvs) if (c->kind == CV_LD && fx->envs[c->env] == e && c->biased) ld_biased = true;
+        if (ld_biased) {
+            e->extra.alloc((size_t)NT * RS);
+            HIPCHK(launch_fill_f(e->extra.p, (int64_t)NT * RS, 0.0f, st));
+            d.extra = e->extra.p;
+            e->biased = true;
+        } else {
+            e->extra.release();
+            d.extra = nullptr;
+        }
+    }
+    for (auto* c : fx->cvs) {
+        if (c->kind == CV_LD) {
+            c->psum = nsum++;
+            const int W = 2 * c->l + 1, BS = (W + 3) / 4 * 4;
+            c->lq.alloc(NT);
+            c->bq.alloc((size_t)NT * BS);
+        } else if (c->kind == CV_SK) {
+            const int nk = (int)(c->kvec.size() / 3);
+            std::vector<double> krev(c->kvec.size());
+            for (size_t q = 0; q < krev.size(); ++q) krev[q] = c->kvec[q] / (2.0 * 3.14159265358979323846);
+            c->kd.alloc(c->kvec.size()); c->kd.upload(c->kvec.data(), c->kvec.size(), st);
+            c->krev.alloc(krev.size()); c->krev.upload(krev.data(), krev.size(), st);
+            std::vector<int32_t> ch, cb(nw + 1, 0), na(nw), ab(nw);
+            const int CH = 8192;
+            c->max_atoms = 0;
+            for (int k = 0; k < nw; ++k) {
+                WalkerIn& a = ctx->w[k];
+                na[k] = (int32_t)a.n_local; ab[k] = (int32_t)a.atom_off;
+                c->max_atoms = std::max<int>(c->max_atoms, (int)a.n_local);
+                for (int64_t s = 0; s < a.n_local; s += CH) {
+                    ch.push_back((int32_t)(a.atom_off + s));
+                    ch.push_back((int32_t)std::min<int64_t>(CH, a.n_local - s));
+                    ch.push_back(k);
+                }
+                cb[k + 1] = (int32_t)(ch.size() / 3);
+            }
+            c->nchunk = (int)(ch.size() / 3);
+            c->chunk.alloc(ch.size()); c->chunk.upload(ch.data(), ch.size(), st);
+            c->chunk_begin.alloc(cb.size()); c->chunk_begin.upload(cb.data(), cb.size(), st);
+            c->natoms.alloc(nw); c->natoms.upload(na.data(), nw, st);
+            c->abegin.alloc(nw); c->abegin.upload(ab.data(), nw, st);
+            c->apart.alloc((size_t)c->nchunk * nk * 2);
+            c->amp.alloc((size_t)nw * nk * 2);
+        }
+    }
+    fx->nslot_sum = nsum;
+    const int ntiles = (int)ctx->h_tiles.size();
+    fx->partial.alloc((size_t)std::max(1, nsum) * std::max(1, ntiles));
+    fx->direct.alloc((size_t)nw * kMaxSlotCV);
+    HIPCHK(launch_fill(fx->direct.p, (int64_t)nw * kMaxSlotCV, 0.0, st));
+    fx->state.alloc((size_t)nw * kStStride);
+    HIPCHK(launch_fill(fx->state.p, (int64_t)nw * kStStride, 0.0, st));
+    fx->wcv.alloc((size_t)nw * kMaxSlotCV);
+    HIPCHK(launch_fill(fx->wcv.p, (int64_t)nw * kMaxSlotCV, 0.0, st));
+    fx->tile_begin.alloc(nw + 1); fx->tile_begin.upload(ctx->tile_begin.data(), nw + 1, st);
+    fx->progs.alloc(progs.size()); fx->progs.upload(progs.data(), progs.size(), st);
+    if (!fx->inited) {
+        // bias storage (first run only, like init_set_mode)
+        const int D = fx->D();
+        int64_t gsz = 1;
+        if (fx->hill_type == 0) for (int d = 0; d < D; ++d) gsz *= fx->nb[d];
+        else gsz = 1;
+        fx->grid.alloc((size_t)nw * gsz);
+        HIPCHK(launch_fill(fx->grid.p, (int64_t)nw * gsz, 0.0, st));
+        fx->hills_cap = 4096;
+        fx->hills.alloc((size_t)nw * fx->hills_cap * (D + 3));
+        fx->nhills.alloc(nw);
+        std::vector<int64_t> z(nw, 0);
+        fx->nhills.upload(z.data(), nw, st);
+        fx->step.alloc(nw);
+        fx->step.upload(z.data(), nw, st);
+        fx->mlp_ok.alloc(nw);
+        std::vector<int32_t> zi(nw, 0);
+        fx->mlp_ok.upload(zi.data(), nw, st);
+        fx->hills_written.assign(nw, 0);
+        fx->hills_bound.assign(nw, 0);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    // reduce descriptor
+    ReduceDev& r = fx->red;
+    r = ReduceDev{};
+    r.n_walkers = nw; r.nslot_sum = nsum; r.ntiles = ntiles; r.tiles = ctx->tiles.p;
+    r.tile_begin = fx->tile_begin.p; r.partial = fx->partial.p;
+    r.direct = fx->direct.p;
+    r.ncv = (int)fx->cvs.size();
+    // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
+    for (size_t c = 0; c < fx->cvs.size(); ++c) r.cv_src[c] = fx->cvs[c]->kind == CV_SK ? -1 : fx->cvs[c]->psum;
+    r.inv_ng = fx->envs.empty() ? nullptr : fx->envs[0]->inv_ng.p;
+    for (auto* e : fx->envs)
+        if (e->group != fx->envs[0]->group) throw Err(MDT_ENOTSUP, "all STEINH CALs must use the same group");
+    r.nsym = (int)fx->syms.size();
+    for (size_t q = 0; q < fx->syms.size(); ++q) r.sym_slot[q] = fx->cvs[fx->syms[q].cv]->slot;
+    r.D = fx->D();
+    r.prog = fx->progs.p;
+    r.kind = fx->hill_type == 0 ? BIAS_GRID : fx->hill_type == 2 ? BIAS_HILLS : BIAS_MLP;
+    r.pace = fx->pace;
+    for (int d = 0; d < fx->D(); ++d) {
+        r.lo[d] = fx->lo[d]; r.hi[d] = fx->hi[d]; r.nb[d] = fx->nb[d];
+        double dcv = (fx->hi[d] - fx->lo[d]) / fx->nb[d];
+        r.radius[d] = (int)std::ceil(4.0 * fx->sigma / dcv) + 1;
+        r.sigma[d] = fx->sigmas[d];
+    }
+    if (r.kind == BIAS_GRID) for (int d = 0; d < fx->D(); ++d) r.sigma[d] = fx->sigma;
+    r.grid_size = 1;
+    if (r.kind == BIAS_GRID) for (int d = 0; d < fx->D(); ++d) r.grid_size *= fx->nb[d];
+    r.grid = fx->grid.p;
+    r.h0 = fx->height0; r.biasf = fx->biasf; r.kT = fx->temp * fx->boltz; r.wt = fx->wt ? 1 : 0;
+    r.hills = fx->hills.p; r.nhills = fx->nhills.p; r.hills_cap = fx->hills_cap;
+    r.step = fx->step.p; r.state = fx->state.p; r.wcv = fx->wcv.p;
+    r.mlp = fx->mlp.p; r.mlp_ok = fx->mlp_ok.p;
+    for (auto* e : fx->envs) e->dev.partial = fx->partial.p, e->dev.wcv = fx->wcv.p;
+    if (!fx->pin) {
+        fx->pin_n = 1 << 16;
+        HIPCHK(hipHostMalloc((void**)&fx->pin, fx->pin_n * sizeof(double), hipHostMallocDefault));
+    }
+    fx->built_gen = ctx->generation;
+    if (!fx->inited) {
+        fx->inited = true;
+        open_hills(fx);
+    }
+}
+
+// Restart from HILLS: "step cv... h sigma" per line, '#' comments
+// (GH_t0_uniformGrid.cu:100-192); replayed hills use the file's h and sigma.
+void replay_hills(mdt_fix* fx, int w, const std::string& path) {
+    FILE* fp = fopen(path.c_str(), "r");
+    if (!fp) return;
+    const int D = fx->D();
+    std::vector<double> s, h;
+    std::vector<double> sig;
+    char line[4096];
+    while (fgets(line, sizeof(line), fp)) {
+        if (line[0] == '#' || line[0] == '\n' || line[0] == '\r') continue;
+        char* p = line;
+        char* q = nullptr;
+        strtoll(p, &q, 10);
+        p = q;
+        double cv[kMaxDim];
+        for (int d = 0; d < D; ++d) { cv[d] = strtod(p, &q); p = q; }
+        double hh = strtod(p, &q); p = q;
+        double sg = strtod(p, &q);
+        for (int d = 0; d < D; ++d) s.push_back(cv[d]);
+        h.push_back(hh);
+        sig.push_back(sg);
+    }
+    fclose(fp);
+    if (h.empty()) return;
+    hipStream_t st = fx->ctx->stream;
+    DBuf<double> ds, dh;
+    // hills with different sigma are deposited in runs of equal sigma
+    size_t a = 0;
+    while (a < h.size()) {
+        size_t b = a;
+        while (b < h.size() && sig[b] == sig[a]) ++b;
+        ds.alloc((b - a) * D); ds.upload(&s[a * D], (b - a) * D, st);
+        dh.alloc(b - a); dh.upload(&h[a], b - a, st);
+        HIPCHK(launch_bias_add(fx->red, w, (int64_t)(b - a), ds.p, dh.p, sig[a], st));
+        HIPCHK(hipStreamSynchronize(st));
+        a = b;
+    }
+    fx->hills_bound[w] += (int64_t)h.size();
+}
+
+std::string hills_path(mdt_fix* fx, int w) {
+    if (fx->nw() == 1) return fx->hills_file;
+    return fx->hills_file + "." + std::to_string(w);
+}
+
+void open_hills(mdt_fix* fx) {
+    const int nw = fx->nw();
+    fx->hfiles.assign(nw, nullptr);
+    if (fx->hills_file == "NONE") return;
+    for (int w = 0; w < nw; ++w) {
+        std::string p = hills_path(fx, w);
+        bool exists = false;
+        if (fx->restart) {
+            FILE* t = fopen(p.c_str(), "r");
+            if (t) { fseek(t, 0, SEEK_END); exists = ftell(t) > 0; fclose(t); }
+        }
+        if (exists) {
+            replay_hills(fx, w, p);
+            // replayed hills are not re-written
+            int64_t nh = 0;
+            HIPCHK(hipMemcpy(&nh, fx->nhills.p + w, sizeof(int64_t), hipMemcpyDeviceToHost));
+            fx->hills_written[w] = nh;
+            fx->hfiles[w] = fopen(p.c_str(), "a");
+        } else {
+            fx->hfiles[w] = fopen(p.c_str(), "w");
+            if (fx->hfiles[w]) {
+                if (fx->D() == 2) fprintf(fx->hfiles[w], "# step cv1 cv2 height sigma\n");
+                if (fx->D() == 1) fprintf(fx->hfiles[w], "# step cv_values height sigma\n");
+            }
+        }
+        if (!fx->hfiles[w]) throw Err(MDT_EINVAL, fmt("Cannot open HILLS file %s for writing/appending.", p.c_str()));
+    }
+}
+
+// grow the device hill log so that `extra` more hills fit for every walker
+void ensure_hills(mdt_fix* fx, int64_t extra) {
+    int64_t need = 0;
+    for (auto b : fx->hills_bound) need = std::max(need, b + extra);
+    if (need <= fx->hills_cap) return;
+    int64_t cap = fx->hills_cap;
+    while (cap < need) cap *= 2;
+    const int D = fx->D(), nw = fx->nw();
+    DBuf<double> nh;
+    nh.alloc((size_t)nw * cap * (D + 3));
+    for (int w = 0; w < nw; ++w)
+        HIPCHK(hipMemcpyAsync(nh.p + (size_t)w * cap * (D + 3), fx->hills.p + (size_t)w * fx->hills_cap * (D + 3),
+                              sizeof(double) * fx->hills_cap * (D + 3), hipMemcpyDeviceToDevice, fx->ctx->stream));
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    std::swap(fx->hills.p, nh.p);
+    std::swap(fx->hills.n, nh.n);
+    fx->hills_cap = cap;
+    fx->red.hills = fx->hills.p;
+    fx->red.hills_cap = cap;
+    fx->graph_gen = 0;   // invalidate the captured graph
+}
+
+// write hills deposited since the last flush to the HILLS files
+void flush_hills(mdt_fix* fx) {
+    const int nw = fx->nw(), D = fx->D();
+    std::vector<int64_t> nh(nw);
+    HIPCHK(hipMemcpyAsync(nh.data(), fx->nhills.p, sizeof(int64_t) * nw, hipMemcpyDeviceToHost, fx->ctx->stream));
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    for (int w = 0; w < nw; ++w) {
+        fx->hills_bound[w] = nh[w];
+        int64_t a = fx->hills_written[w];
+        if (nh[w] <= a) continue;
+        std::vector<double> buf((size_t)(nh[w] - a) * (D + 3));
+        HIPCHK(hipMemcpy(buf.data(), fx->hills.p + ((size_t)w * fx->hills_cap + a) * (D + 3),
+                         sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        FILE* fp = fx->hfiles.empty() ? nullptr : fx->hfiles[w];
+        if (fp) {
+            for (int64_t k = 0; k < nh[w] - a; ++k) {
+                const double* rec = &buf[(size_t)k * (D + 3)];
+                fprintf(fp, "%ld", (long)rec[0]);
+                for (int d = 0; d < D; ++d) fprintf(fp, " %.16g", rec[1 + d]);
+                if (fx->hill_type == 0 || D == 1)
+                    fprintf(fp, " %.16g %.16g\n", rec[1 + D], rec[2 + D]);
+                else {
+                    fprintf(fp, " %.16g", rec[1 + D]);
+                    for (int d = 0; d < D; ++d) fprintf(fp, " %.16g", fx->sigmas[d]);
+                    fprintf(fp, "\n");
+                }
+            }
+            fflush(fp);
+        }
+        fx->hills_written[w] = nh[w];
+    }
+}
+
+// Optional per-launch timing (HIP events on the context stream).
+struct Timer {
+    bool on = false;
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    std::vector<int64_t> count;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<int> pending_id;
+    int id_of(const std::string& n) {
+        for (size_t k = 0; k < names.size(); ++k) if (names[k] == n) return (int)k;
+        names.push_back(n); ms.push_back(0.0); count.push_back(0);
+        return (int)names.size() - 1;
+    }
+    template <class F>
+    void run(const char* name, hipStream_t st, F&& fn) {
+        if (!on) { fn(); return; }
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, st));
+        fn();
+        HIPCHK(hipEventRecord(b, st));
+        pending.push_back({a, b});
+        pending_id.push_back(id_of(name));
+    }
+    void collect() {
+        for (size_t k = 0; k < pending.size(); ++k) {
+            HIPCHK(hipEventSynchronize(pending[k].second));
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, pending[k].first, pending[k].second));
+            ms[pending_id[k]] += t;
+            count[pending_id[k]] += 1;
+            (void)hipEventDestroy(pending[k].first);
+            (void)hipEventDestroy(pending[k].second);
+        }
+        pending.clear();
+        pending_id.clear();
+    }
+};
+
+void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
+    static Timer off_timer;
+    Timer& T = tm ? *tm : off_timer;
+    mdt_ctx* ctx = fx->ctx;
+    hipStream_t st = ctx->stream;
+    Geo g = ctx->geo();
+    for (auto* e : fx->envs) {
+        Geo ge = g;
+        ge.group = e->group == "all" ? nullptr : ctx->group.p;
+        T.run("k_env", st, [&] { HIPCHK(launch_env(ge, e->dev, st)); });
+    }
+    for (auto* c : fx->cvs) {
+        if (c->kind == CV_LD) {
+            EnvH* e = fx->envs[c->env];
+            Geo ge = g;
+            ge.group = e->group == "all" ? nullptr : ctx->group.p;
+            LdDev d{};
+            d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = c->biased ? c->slot : -1;
+            T.run("k_ld", st, [&] { HIPCHK(launch_ld(ge, e->dev, d, st)); });
+        } else if (c->kind == CV_SK) {
+            SkDev s{};
+            s.nk = (int)(c->kvec.size() / 3);
+            s.k = c->krev.p; s.kvec = c->kd.p; s.nchunk = c->nchunk; s.chunk = c->chunk.p;
+            s.apart = c->apart.p; s.amp = c->amp.p; s.chunk_begin = c->chunk_begin.p;
+            s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
+            s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
+            T.run("k_sk_amp", st, [&] { HIPCHK(launch_sk_amp(g, s, st)); });
+        }
+    }
+    T.run("k_reduce", st, [&] { HIPCHK(launch_reduce(fx->red, st)); });
+    for (auto* c : fx->cvs) {
+        if (c->kind == CV_LD && c->biased) {
+            EnvH* e = fx->envs[c->env];
+            LdDev d{};
+            d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = c->slot;
+            T.run("k_ld_coeff", st, [&] { HIPCHK(launch_ld_coeff(g, e->dev, d, st)); });
+        }
+    }
+    for (auto* e : fx->envs)
+        if (e->biased) T.run("k_force", st, [&] { HIPCHK(launch_force(g, e->dev, f, st)); });
+    for (auto* c : fx->cvs) {
+        if (c->kind == CV_SK && c->biased) {
+            SkDev s{};
+            s.nk = (int)(c->kvec.size() / 3);
+            s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p;
+            s.wslot = c->slot; s.wcv = fx->wcv.p; s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p;
+            s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
+            T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(g, s, f, st)); });
+        }
+    }
+}
+
+void use_device(mdt_ctx* ctx) {
+    if (ctx->device < 0) throw Err(MDT_ESTATE, "parse-only context (device -1) cannot run steps");
+    HIPCHK(hipSetDevice(ctx->device));
+}
+
+void ensure_built(mdt_fix* fx) {
+    if (fx->ctx->device < 0) throw Err(MDT_ESTATE, "parse-only context (device -1) cannot run steps");
+    if (!fx->ctx->committed || !fx->inited || fx->built_gen != fx->ctx->generation) fix_build(fx);
+}
+
+void set_steps(mdt_fix* fx, int64_t step) {
+    const int nw = fx->nw();
+    std::vector<int64_t> s(nw, step);
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    HIPCHK(hipMemcpy(fx->step.p, s.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice));
+}
+
+void fetch_state(mdt_fix* fx) {
+    const int nw = fx->nw();
+    fx->h_state.resize((size_t)nw * kStStride);
+    HIPCHK(hipMemcpyAsync(fx->h_state.data(), fx->state.p, sizeof(double) * fx->h_state.size(), hipMemcpyDeviceToHost, fx->ctx->stream));
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    fx->state_valid = true;
+}
+
+int fail(mdt_ctx* ctx, const Err& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+}
+
+#define API_BEGIN(ctxp) mdt_ctx* ctx__ = (ctxp); try {
+#define API_END                                              \
+    }                                                        \
+    catch (const Err& e) { return fail(ctx__, e); }          \
+    catch (const std::exception& e) { return fail(ctx__, Err(MDT_EINVAL, e.what())); } \
+    return MDT_OK;
+
+}  // namespace
+
+extern "C" {
+
+const char* mdt_version(void) { return "mdt 0.1 gfx950"; }
+
+int mdt_create(mdt_ctx** out, int device, int n_walkers) {
+    if (!out) return MDT_EINVAL;
+    *out = nullptr;
+    if (n_walkers < 1) return MDT_EINVAL;
+    if (device == -1) {
+        // parse-only context: keyword lists can be validated on a host without a GPU
+        mdt_ctx* c = new mdt_ctx();
+        c->device = -1;
+        c->w.resize(n_walkers);
+        *out = c;
+        return MDT_OK;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MDT_EHIP;
+    if (device < 0 || device >= ndev) return MDT_EINVAL;
+    mdt_ctx* c = new mdt_ctx();
+    c->device = device;
+    c->w.resize(n_walkers);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MDT_EHIP;
+    }
+    *out = c;
+    return MDT_OK;
+}
+
+void mdt_destroy(mdt_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->fix) { ctx->fix->ctx = ctx; }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* mdt_last_error(const mdt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mdt_set_box(mdt_ctx* ctx, int walker, const double lo[3], const double hi[3], const int periodic[3]) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    for (int d = 0; d < 3; ++d) {
+        if (!(hi[d] > lo[d])) throw Err(MDT_EINVAL, "box hi must exceed lo");
+        if (periodic && !periodic[d]) throw Err(MDT_ENOTSUP, "non-periodic boxes: pass ghost-free lists; only periodic boxes are supported");
+        a.lo[d] = lo[d]; a.hi[d] = hi[d];
+    }
+    a.has_box = true;
+    ctx->committed = false;
+    API_END
+}
+
+static void copy_in(void* dst, const void* src, size_t bytes, int on_device) {
+    if (!bytes) return;
+    if (on_device) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    else std::memcpy(dst, src, bytes);
+}
+
+int mdt_bind_positions(mdt_ctx* ctx, int walker, const double* x, int64_t n_local, int64_t n_all, int on_device) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (n_local < 0 || n_all < n_local || (!x && n_all)) throw Err(MDT_EINVAL, "bad position arguments");
+    if (ctx->committed && a.has_x && a.n_local == n_local && a.n_all == n_all) {
+        a.x.resize((size_t)n_all * 3);
+        copy_in(a.x.data(), x, sizeof(double) * 3 * n_all, on_device);
+        return mdt_update_positions(ctx, walker, x, on_device);
+    }
+    a.n_local = n_local; a.n_all = n_all;
+    a.x.resize((size_t)n_all * 3);
+    copy_in(a.x.data(), x, sizeof(double) * 3 * n_all, on_device);
+    a.has_x = true;
+    ctx->committed = false;
+    API_END
+}
+
+int mdt_bind_neighbors(mdt_ctx* ctx, int walker, const int32_t* offsets, const int32_t* idx, int64_t n_center,
+                       const int32_t* ghost_owner, int on_device) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "bind positions before neighbour lists");
+    if (n_center != a.n_local) throw Err(MDT_EINVAL, "neighbour lists must cover every local atom (n_center == n_local)");
+    a.off.resize((size_t)n_center + 1);
+    copy_in(a.off.data(), offsets, sizeof(int32_t) * (n_center + 1), on_device);
+    if (a.off[0] != 0) throw Err(MDT_EINVAL, "offsets[0] must be 0");
+    for (int64_t i = 0; i < n_center; ++i)
+        if (a.off[i + 1] < a.off[i]) throw Err(MDT_EINVAL, "offsets must be non-decreasing");
+    a.nnz = a.off[n_center];
+    a.idx.resize((size_t)a.nnz);
+    copy_in(a.idx.data(), idx, sizeof(int32_t) * a.nnz, on_device);
+    a.owner.resize((size_t)(a.n_all - a.n_local));
+    copy_in(a.owner.data(), ghost_owner, sizeof(int32_t) * (a.n_all - a.n_local), on_device);
+    a.has_nl = true;
+    ctx->committed = false;
+    API_END
+}
+
+int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "bind positions before the group");
+    if (in_group) a.group.assign(in_group, in_group + a.n_local);
+    else a.group.clear();
+    ctx->committed = false;
+    API_END
+}
+
+int mdt_bind_system(mdt_ctx* ctx, int walker, const mdt_system* sys) {
+    API_BEGIN(ctx)
+    if (!sys) throw Err(MDT_EINVAL, "null system");
+    const int64_t nl = mdt_system_n_local(sys), na = mdt_system_n_all(sys), nz = mdt_system_nnz(sys);
+    if (na < nl || na == 0) throw Err(MDT_ESTATE, "system not built (mdt_system_build)");
+    double lo[3], hi[3];
+    mdt_system_get_box(sys, lo, hi);
+    int per[3] = {1, 1, 1};
+    int rc = mdt_set_box(ctx, walker, lo, hi, per);
+    if (rc) return rc;
+    std::vector<double> x((size_t)na * 3);
+    std::vector<int32_t> off((size_t)nl + 1), idx((size_t)nz), own((size_t)(na - nl));
+    mdt_system_copy(sys, x.data(), off.data(), idx.data(), own.data());
+    rc = mdt_bind_positions(ctx, walker, x.data(), nl, na, 0);
+    if (rc) return rc;
+    rc = mdt_bind_neighbors(ctx, walker, off.data(), idx.data(), nl, own.data(), 0);
+    if (rc) return rc;
+    API_END
+}
+
+int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_device) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "positions not bound");
+    if (!ctx->committed) {
+        copy_in(a.x.data(), x, sizeof(double) * 3 * a.n_all, on_device);
+        return MDT_OK;
+    }
+    hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * a.atom_off, x, sizeof(double) * 3 * a.n_local, k, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), x + 3 * a.n_local,
+                          sizeof(double) * 3 * (a.n_all - a.n_local), k, ctx->stream));
+    if (!on_device) HIPCHK(hipStreamSynchronize(ctx->stream));   // pageable source must stay valid
+    API_END
+}
+
+int mdt_fix_create(mdt_ctx* ctx, int argc, const char* const* argv, mdt_fix** out) {
+    API_BEGIN(ctx)
+    if (!out) throw Err(MDT_EINVAL, "null output");
+    *out = nullptr;
+    std::unique_ptr<mdt_fix> fx(new mdt_fix());
+    fx->ctx = ctx;
+    parse_fix(fx.get(), argc, argv);
+    *out = fx.release();
+    ctx->fix = *out;
+    API_END
+}
+
+void mdt_fix_destroy(mdt_fix* fx) {
+    if (!fx) return;
+    if (fx->ctx && fx->inited) {
+        try { flush_hills(fx); } catch (...) {}
+    }
+    if (fx->ctx && fx->ctx->fix == fx) fx->ctx->fix = nullptr;
+    delete fx;
+}
+
+int mdt_fix_init(mdt_fix* fx) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    use_device(fx->ctx);
+    ensure_built(fx);
+    API_END
+}
+
+int mdt_fix_post_force(mdt_fix* fx, int64_t step, double* f, int on_device) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    ensure_built(fx);
+    ensure_hills(fx, 1);
+    for (auto& b : fx->hills_bound) b += 1;
+    set_steps(fx, step);
+    double* fd = ctx->f.p;
+    if (f && on_device) fd = f;
+    else if (f) HIPCHK(launch_fill(ctx->f.p, ctx->total_local * 3, 0.0, ctx->stream));
+    enqueue_step(fx, fd);
+    if (f && !on_device) {
+        std::vector<double> tmp((size_t)ctx->total_local * 3);
+        HIPCHK(hipMemcpyAsync(tmp.data(), ctx->f.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (size_t k = 0; k < tmp.size(); ++k) f[k] += tmp[k];
+    }
+    fetch_state(fx);
+    flush_hills(fx);
+    API_END
+}
+
+int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    ensure_built(fx);
+    const int64_t ndep = fx->pace > 0 ? nsteps / fx->pace + 1 : 0;
+    ensure_hills(fx, ndep);
+    for (auto& b : fx->hills_bound) b += ndep;
+    set_steps(fx, step0);
+    if (!fx->gexec || fx->graph_gen != ctx->generation + 1) {
+        if (fx->gexec) { (void)hipGraphExecDestroy(fx->gexec); fx->gexec = nullptr; }
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_step(fx, ctx->f.p);
+        } catch (...) {
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &dummy);
+            if (dummy) (void)hipGraphDestroy(dummy);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
+        hipError_t e = hipGraphInstantiate(&fx->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) throw Err(MDT_EHIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e)));
+        fx->graph_gen = ctx->generation + 1;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(fx->gexec, ctx->stream));
+    fx->state_valid = false;
+    API_END
+}
+
+int mdt_fix_profile(mdt_fix* fx, int64_t step0, int64_t nsteps, char* names, int names_cap,
+                    double* ms, int ms_cap, int* nkern) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    ensure_built(fx);
+    const int64_t ndep = fx->pace > 0 ? nsteps / fx->pace + 1 : 0;
+    ensure_hills(fx, ndep);
+    for (auto& b : fx->hills_bound) b += ndep;
+    set_steps(fx, step0);
+    Timer T;
+    T.on = true;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        enqueue_step(fx, ctx->f.p, &T);
+        T.collect();
+    }
+    std::string all;
+    for (size_t k = 0; k < T.names.size(); ++k) {
+        if (k) all += ",";
+        all += T.names[k];
+        if ((int)k < ms_cap) ms[k] = T.count[k] ? T.ms[k] / (double)T.count[k] : 0.0;
+    }
+    if (names && names_cap > 0) {
+        strncpy(names, all.c_str(), (size_t)names_cap - 1);
+        names[names_cap - 1] = 0;
+    }
+    if (nkern) *nkern = (int)T.names.size();
+    fx->state_valid = false;
+    API_END
+}
+
+int mdt_fix_sync(mdt_fix* fx) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    fetch_state(fx);
+    flush_hills(fx);
+    API_END
+}
+
+static const double* state_of(mdt_fix* fx, int w) {
+    if (!fx->inited) throw Err(MDT_ESTATE, "no step has run yet");
+    if (w < 0 || w >= fx->nw()) throw Err(MDT_EINVAL, "walker out of range");
+    if (!fx->state_valid) fetch_state(fx);
+    return fx->h_state.data() + (size_t)w * kStStride;
+}
+
+int mdt_fix_scalar(mdt_fix* fx, int walker, double* value) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    *value = state_of(fx, walker)[kStV];
+    API_END
+}
+
+int mdt_fix_vector(mdt_fix* fx, int walker, int n, double* value) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    if (n < 0 || n >= fx->D()) { *value = 0.0; return MDT_OK; }
+    *value = state_of(fx, walker)[kStS + n];
+    API_END
+}
+
+static int find_cv(mdt_fix* fx, const char* cal_func) {
+    std::string t = cal_func ? cal_func : "";
+    size_t dot = t.find('.');
+    std::string name = t.substr(0, dot);
+    std::string func = dot == std::string::npos ? "AVE" : t.substr(dot + 1);
+    if (func != "AVE") throw Err(MDT_EINVAL, fmt("We can't find the func %s.", func.c_str()));
+    for (size_t k = 0; k < fx->cvs.size(); ++k) if (fx->cvs[k]->name == name) return (int)k;
+    throw Err(MDT_EINVAL, fmt("no CAL named '%s'", name.c_str()));
+}
+
+int mdt_fix_cv_value(mdt_fix* fx, int walker, const char* cal_func, double* value) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    int c = find_cv(fx, cal_func);
+    *value = state_of(fx, walker)[kStCv + fx->cvs[c]->slot];
+    API_END
+}
+
+int mdt_fix_cv_weight(mdt_fix* fx, int walker, const char* cal_func, double* value) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    int c = find_cv(fx, cal_func);
+    state_of(fx, walker);
+    double w = 0;
+    HIPCHK(hipMemcpy(&w, fx->wcv.p + (size_t)walker * kMaxSlotCV + fx->cvs[c]->slot, sizeof(double), hipMemcpyDeviceToHost));
+    *value = w;
+    API_END
+}
+
+int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* out) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    if (!fx->inited) throw Err(MDT_ESTATE, "no step has run yet");
+    mdt_ctx* ctx = fx->ctx;
+    WalkerIn& a = ctx->walker(walker);
+    std::string k = key ? key : "";
+    size_t dot = k.find('.');
+    if (dot == std::string::npos) throw Err(MDT_EINVAL, "key must be CAL.prop");
+    std::string name = k.substr(0, dot), prop = k.substr(dot + 1);
+    CvH* c = nullptr;
+    for (auto* q : fx->cvs) if (q->name == name) c = q;
+    if (!c) throw Err(MDT_EINVAL, fmt("no CAL named '%s'", name.c_str()));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int64_t n = a.n_local, o = a.atom_off;
+    if (c->kind == CV_SK) throw Err(MDT_EINVAL, "SK has no per-atom properties");
+    EnvH* e = fx->envs[c->env];
+    if (prop == "stein_q" || (prop == "local_q" && c->kind == CV_LD)) {
+        std::vector<float> v((size_t)n);
+        if (prop == "stein_q") {
+            int li = rec_li(e->lmask, c->l);
+            HIPCHK(hipMemcpy(v.data(), e->ql.p + (size_t)li * ctx->total_local + o, sizeof(float) * n, hipMemcpyDeviceToHost));
+        } else {
+            HIPCHK(hipMemcpy(v.data(), c->lq.p + o, sizeof(float) * n, hipMemcpyDeviceToHost));
+        }
+        for (int64_t i = 0; i < n; ++i) out[i] = v[i];
+    } else if (prop == "nsel" || prop == "count_rc") {
+        std::vector<int32_t> v((size_t)n);
+        HIPCHK(hipMemcpy(v.data(), (prop == "nsel" ? e->nsel.p : e->cnt.p) + o, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) out[i] = v[i];
+    } else if (prop == "qlm") {
+        // q_lm of degree l: 2l+1 floats per atom (Re q_l0, Re/Im q_lm m>=1)
+        const int RS = rec_stride(e->lmask), O = rec_off(e->lmask, c->l), W = 2 * c->l + 1;
+        std::vector<float> v((size_t)n * RS);
+        HIPCHK(hipMemcpy(v.data(), e->rec.p + (size_t)o * RS, sizeof(float) * v.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) for (int q = 0; q < W; ++q) out[i * W + q] = v[(size_t)i * RS + O + q];
+    } else {
+        throw Err(MDT_EINVAL, fmt("unknown per-atom property '%s'", prop.c_str()));
+    }
+    API_END
+}
+
+int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* sel_idx, int32_t* count_rc, int32_t* n_sel) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    ensure_built(fx);
+    WalkerIn& a = ctx->walker(walker);
+    CvH* c = nullptr;
+    for (auto* q : fx->cvs) if (q->name == (cal_name ? cal_name : "")) c = q;
+    if (!c || c->kind == CV_SK) throw Err(MDT_EINVAL, "selection export needs a STEINH CAL name");
+    EnvH* e = fx->envs[c->env];
+    const int N = e->N;
+    const int64_t n = a.n_local;
+    DBuf<int32_t> si, cr, ns;
+    si.alloc((size_t)std::max<int64_t>(n, 1) * N); cr.alloc(std::max<int64_t>(n, 1)); ns.alloc(std::max<int64_t>(n, 1));
+    Geo g = ctx->geo();
+    g.group = e->group == "all" ? nullptr : ctx->group.p;
+    HIPCHK(launch_select_literal(g, (int32_t)a.atom_off, (int32_t)n, e->rc * e->rc, N, si.p, cr.p, ns.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (sel_idx) {
+        std::vector<int32_t> v((size_t)n * N);
+        HIPCHK(hipMemcpy(v.data(), si.p, sizeof(int32_t) * v.size(), hipMemcpyDeviceToHost));
+        for (auto& j : v) {
+            if (j < 0) continue;
+            if (j < ctx->total_local) j = (int32_t)(j - a.atom_off);
+            else j = (int32_t)(a.n_local + (j - ctx->total_local - a.ghost_off));
+        }
+        std::memcpy(sel_idx, v.data(), sizeof(int32_t) * v.size());
+    }
+    if (count_rc) HIPCHK(hipMemcpy(count_rc, cr.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (n_sel) HIPCHK(hipMemcpy(n_sel, ns.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fix_get_forces(mdt_fix* fx, int walker, double* f_host) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    WalkerIn& a = ctx->walker(walker);
+    if (!ctx->committed) throw Err(MDT_ESTATE, "no forces yet");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(f_host, ctx->f.p + 3 * a.atom_off, sizeof(double) * 3 * a.n_local, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fix_zero_forces(mdt_fix* fx) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    if (!ctx->committed) ctx->commit();
+    HIPCHK(launch_fill(ctx->f.p, ctx->total_local * 3, 0.0, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int mdt_fix_bias_eval(mdt_fix* fx, int walker, int64_t np, const double* s, double* V, double* dVds) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    ensure_built(fx);
+    ctx->walker(walker);
+    const int D = fx->D();
+    DBuf<double> ds, dv, dd;
+    ds.alloc((size_t)std::max<int64_t>(np, 1) * D); dv.alloc(std::max<int64_t>(np, 1)); dd.alloc((size_t)std::max<int64_t>(np, 1) * D);
+    ds.upload(s, (size_t)np * D, ctx->stream);
+    HIPCHK(launch_bias_eval(fx->red, walker, np, ds.p, dv.p, dd.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (V) HIPCHK(hipMemcpy(V, dv.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+    if (dVds) HIPCHK(hipMemcpy(dVds, dd.p, sizeof(double) * np * D, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fix_bias_add_hills(mdt_fix* fx, int walker, int64_t nh, const double* s, const double* h, double sigma) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    ensure_built(fx);
+    ctx->walker(walker);
+    if (fx->hill_type != 0) { ensure_hills(fx, nh); fx->hills_bound[walker] += nh; }
+    const int D = fx->D();
+    DBuf<double> ds, dh;
+    ds.alloc((size_t)std::max<int64_t>(nh, 1) * D); dh.alloc(std::max<int64_t>(nh, 1));
+    ds.upload(s, (size_t)nh * D, ctx->stream);
+    dh.upload(h, (size_t)nh, ctx->stream);
+    HIPCHK(launch_bias_add(fx->red, walker, nh, ds.p, dh.p, sigma, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (fx->hill_type != 0) {
+        int64_t n = 0;
+        HIPCHK(hipMemcpy(&n, fx->nhills.p + walker, sizeof(int64_t), hipMemcpyDeviceToHost));
+        fx->hills_written[walker] = n;   // externally supplied hills are not echoed to HILLS
+    }
+    API_END
+}
+
+int mdt_fix_grid(mdt_fix* fx, int walker, double* out, int64_t n) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    ensure_built(fx);
+    if (fx->hill_type != 0) throw Err(MDT_EINVAL, "this bias has no grid");
+    if (n != fx->red.grid_size) throw Err(MDT_EINVAL, fmt("grid has %lld bins", (long long)fx->red.grid_size));
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    HIPCHK(hipMemcpy(out, fx->grid.p + (size_t)walker * fx->red.grid_size, sizeof(double) * n, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fix_hills(mdt_fix* fx, int walker, int64_t* nh, double* rec, int64_t cap) {
+    API_BEGIN(fx ? fx->ctx : nullptr)
+    ensure_built(fx);
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    int64_t n = 0;
+    HIPCHK(hipMemcpy(&n, fx->nhills.p + walker, sizeof(int64_t), hipMemcpyDeviceToHost));
+    *nh = n;
+    const int D = fx->D();
+    if (rec && cap > 0) {
+        int64_t m = std::min(n, cap);
+        HIPCHK(hipMemcpy(rec, fx->hills.p + (size_t)walker * fx->hills_cap * (D + 3), sizeof(double) * m * (D + 3), hipMemcpyDeviceToHost));
+    }
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,1221 @@
+// gfx950 kernels of the metadynamics CV-and-bias path.
+//
+// Per MD step (one walker or many, atoms of all walkers concatenated):
+//   k_env        N-nearest selection (bit-exact FP64 r^2, reference insertion
+//                semantics when the cap binds) fused with FP32 Y_lm / q_lm /
+//                q_l for every degree on the environment, FP64 per-tile sums.
+//   k_ld         Lechner-Dellago local Q_l from neighbours' q_lm.
+//   k_sk_amp     structure-factor amplitudes A(k) (FP64 phase, FP64 sums).
+//   k_reduce     per walker: FP64 CV sums, DIM expressions with exact
+//                derivatives, hill deposit (well-tempered), bias value and
+//                gradient, chain-rule weights dV/dv.
+//   k_ld_coeff   two-hop coefficients of the local-Q bias.
+//   k_force      exact bias forces by a deterministic gather over each atom's
+//                own selected pairs and the reverse pairs (no atomics).
+//   k_sk_force   structure-factor bias forces.
+//
+// Reference counterparts (paths relative to the Metad-toy source tree):
+//   selection  src/cvs/CVs_Steinhardt.cu:1324-1406
+//   q_lm/q_l   src/cvs/CVs_Steinhardt.cu:1410-1532, include/CVsets/CV_Steinhardt_math.h:981-1215
+//   gradient   src/cvs/CVs_Steinhardt.cu:1538-1698 (replaced: exact gradient)
+//   grid       src/grids/GH_t0_uniformGrid.cu:194-461
+//   chain rule src/grids/DimSet.cu:68-97
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <utility>
+
+#include "engine.h"
+
+namespace mdt {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Compile-time tables for Y_lm(n) = c_lm(n_z) (n_x + i n_y)^m with
+// c_lm(u) = N_lm (-1)^m d^m P_l(u)/du^m (orthonormal, Condon-Shortley phase:
+// the same functions as the reference closed forms).
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+constexpr double cfact(int n) { double f = 1; for (int i = 2; i <= n; ++i) f *= i; return f; }
+constexpr double csqrt(double v) {
+    double r = v > 1 ? v : 1.0;
+    for (int it = 0; it < 200; ++it) r = 0.5 * (r + v / r);
+    return r;
+}
+struct YTab { double c[7][7][8]; };
+constexpr YTab make_ytab() {
+    YTab t{};
+    for (int l = 0; l <= 6; ++l) {
+        double P[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; 2 * k <= l; ++k) {
+            double sg = (k % 2) ? -1.0 : 1.0;
+            double v = sg * cfact(l) / (cfact(k) * cfact(l - k)) *
+                       cfact(2 * l - 2 * k) / (cfact(l) * cfact(l - 2 * k));
+            double p2 = 1; for (int q = 0; q < l; ++q) p2 *= 2.0;
+            P[l - 2 * k] += v / p2;
+        }
+        for (int m = 0; m <= l; ++m) {
+            double D[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int k = 0; k <= l; ++k) D[k] = P[k];
+            for (int d = 0; d < m; ++d) {
+                for (int k = 0; k < 7; ++k) D[k] = D[k + 1] * (k + 1);
+                D[7] = 0;
+            }
+            double N = csqrt((2 * l + 1) / (4 * kPi) * cfact(l - m) / cfact(l + m));
+            double sg = (m % 2) ? -1.0 : 1.0;
+            for (int k = 0; k <= l; ++k) t.c[l][m][k] = sg * N * D[k];
+        }
+    }
+    return t;
+}
+constexpr YTab kY = make_ytab();
+
+// coefficient of u^K of: W=0 c_lm, W=1 mu_m*m*c_lm, W=2 mu_m*c'_lm
+template <int L, int M, int W, int K>
+struct YC {
+    static constexpr double mu = M ? 2.0 : 1.0;
+    static constexpr double raw =
+        W == 0 ? kY.c[L][M][K]
+      : W == 1 ? mu * M * kY.c[L][M][K]
+               : (K + 1 <= L ? mu * (K + 1) * kY.c[L][M][K + 1] : 0.0);
+    static constexpr float v = (float)raw;
+};
+// Horner in v = u^2 over k = K, K+2, ..., TOP
+template <int L, int M, int W, int K, int TOP>
+struct HornerU2 {
+    static __device__ __forceinline__ float run(float v) {
+        if constexpr (K >= TOP) return YC<L, M, W, TOP>::v;
+        else return __builtin_fmaf(v, HornerU2<L, M, W, K + 2, TOP>::run(v), YC<L, M, W, K>::v);
+    }
+};
+// polynomial of degree DEG with parity DEG%2 evaluated at u (u2 = u*u)
+template <int L, int M, int W>
+__device__ __forceinline__ float ypoly(float u, float u2) {
+    constexpr int DEG = (W == 2) ? (L - M - 1) : (L - M);
+    if constexpr (DEG < 0) {
+        return 0.0f;
+    } else {
+        float h = HornerU2<L, M, W, DEG % 2, DEG>::run(u2);
+        if constexpr (DEG % 2) return h * u; else return h;
+    }
+}
+
+template <int... Is, class F>
+__device__ __forceinline__ void sfor_impl(std::integer_sequence<int, Is...>, F&& f) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void sfor(F&& f) { sfor_impl(std::make_integer_sequence<int, N>{}, f); }
+
+// powers z^m = (nx + i ny)^m, m = 0..LMAX
+template <int LMAX>
+__device__ __forceinline__ void zpow(float nx, float ny, float* zr, float* zi) {
+    zr[0] = 1.0f; zi[0] = 0.0f;
+    zr[1] = nx; zi[1] = ny;
+    #pragma unroll
+    for (int m = 2; m <= LMAX; ++m) {
+        zr[m] = __builtin_fmaf(zr[m - 1], nx, -zi[m - 1] * ny);
+        zi[m] = __builtin_fmaf(zr[m - 1], ny, zi[m - 1] * nx);
+    }
+}
+
+constexpr int lmax_of(int lmask) { return (lmask & kL6) ? 6 : (lmask & kL4) ? 4 : 3; }
+
+// q[off..off+2L] += Y_l (packed: Re q_l0, then Re/Im for m = 1..L)
+template <int L>
+__device__ __forceinline__ void acc_ylm(float u, float u2, const float* zr, const float* zi, float* q) {
+    q[0] += ypoly<L, 0, 0>(u, u2);
+    sfor<L>([&](auto mm) {
+        constexpr int M = decltype(mm)::value + 1;
+        float c = ypoly<L, M, 0>(u, u2);
+        q[2 * M - 1] = __builtin_fmaf(c, zr[M], q[2 * M - 1]);
+        q[2 * M] = __builtin_fmaf(c, zi[M], q[2 * M]);
+    });
+}
+
+// G += grad_n of sum_m mu_m Re(conj(K_m) c_lm(n_z) z^m)   (K packed as the record)
+template <int L>
+__device__ __forceinline__ void acc_grad(float u, float u2, const float* zr, const float* zi,
+                                         const float* K, float& gx, float& gy, float& gz) {
+    gz = __builtin_fmaf(ypoly<L, 0, 2>(u, u2), K[0], gz);
+    sfor<L>([&](auto mm) {
+        constexpr int M = decltype(mm)::value + 1;
+        const float kr = K[2 * M - 1], ki = K[2 * M];
+        // conj(K) z^(M-1)
+        float tr = __builtin_fmaf(kr, zr[M - 1], ki * zi[M - 1]);
+        float ti = __builtin_fmaf(kr, zi[M - 1], -ki * zr[M - 1]);
+        float a = ypoly<L, M, 1>(u, u2);          // mu m c_lm
+        gx = __builtin_fmaf(a, tr, gx);
+        gy = __builtin_fmaf(-a, ti, gy);
+        float b = ypoly<L, M, 2>(u, u2);          // mu c'_lm
+        gz = __builtin_fmaf(b, __builtin_fmaf(kr, zr[M], ki * zi[M]), gz);
+    });
+}
+
+__device__ __forceinline__ int owner_of(const Geo& g, int j) {
+    return j < g.n_local_total ? j : g.owner[j - g.n_local_total];
+}
+
+// tile index with XCD-contiguous ranges: workgroup b runs on XCD b % 8
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    int per = nblocks >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+inline int xcd_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide FP64 sum of NV values per thread in a fixed order; result on thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double* v, double* sh /* NV*4 */) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    #pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k * 4 + wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        #pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = (sh[k * 4 + 0] + sh[k * 4 + 1]) + (sh[k * 4 + 2] + sh[k * 4 + 3]);
+    }
+}
+
+// FP64 pair vector and r^2 exactly as the reference evaluates it without
+// contraction: r2 = (dx*dx + dy*dy) + dz*dz.
+__device__ __forceinline__ double pair_r2(const double* __restrict__ x, int j, double xi0, double xi1,
+                                          double xi2, double& d0, double& d1, double& d2) {
+#pragma clang fp contract(off)
+    d0 = x[3 * j] - xi0;
+    d1 = x[3 * j + 1] - xi1;
+    d2 = x[3 * j + 2] - xi2;
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse positions: for list entry (k, p) -> j with owner i, the position q
+// in i's list of the image of k at displacement -(x_j - x_k).  Computed once
+// per neighbour-list binding.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_revpos(Geo g, int64_t n, uint8_t* __restrict__ revpos) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int beg = g.off[k], end = g.off[k + 1];
+    const double xk0 = g.x[3 * k], xk1 = g.x[3 * k + 1], xk2 = g.x[3 * k + 2];
+    for (int e = beg; e < end; ++e) {
+        const int j = g.idx[e];
+        const int i = owner_of(g, j);
+        const double d0 = g.x[3 * j] - xk0, d1 = g.x[3 * j + 1] - xk1, d2 = g.x[3 * j + 2] - xk2;
+        const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+        int best = 0xFF;
+        double bd = 1e-8;
+        const int b2 = g.off[i], e2 = g.off[i + 1];
+        for (int q = b2; q < e2; ++q) {
+            const int t = g.idx[q];
+            if (owner_of(g, t) != (int)k) continue;
+            double s0 = g.x[3 * t] - xi0 + d0, s1 = g.x[3 * t + 1] - xi1 + d1, s2 = g.x[3 * t + 2] - xi2 + d2;
+            double dd = s0 * s0 + s1 * s1 + s2 * s2;
+            if (dd < bd) { bd = dd; best = q - b2; }
+        }
+        revpos[e] = (uint8_t)best;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_env: selection + q_lm for every degree in LMASK.  One thread per atom.
+// Phase A scans the list with FP64 r^2; if the number within rc does not
+// exceed N every such candidate is kept (what the reference insertion keeps
+// then), otherwise the reference insertion is replayed in registers to find
+// the kept set.  Phase B accumulates Y_lm in FP32 over the kept set.
+// ---------------------------------------------------------------------------
+template <int NMAX>
+__device__ __forceinline__ void literal_insert(const Geo& g, int beg, int c, double xi0, double xi1,
+                                               double xi2, double rc2, int N, double* sr2, int* sp) {
+    #pragma unroll
+    for (int s = 0; s < NMAX; ++s) { sr2[s] = __builtin_inf(); sp[s] = -1; }
+    for (int p = 0; p < c; ++p) {
+        double d0, d1, d2;
+        double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+        if ((r2 > rc2) || (r2 < 1e-12)) continue;
+        double r = r2;
+        int q = p;
+        #pragma unroll
+        for (int s = 0; s < NMAX; ++s) {
+            if (s < N) {
+                const bool sw = sr2[s] > r;
+                const double tr = sr2[s];
+                const int tq = sp[s];
+                sr2[s] = sw ? r : tr;
+                sp[s] = sw ? q : tq;
+                r = sw ? tr : r;
+                q = sw ? tq : q;
+            }
+        }
+    }
+}
+
+template <int NMAX, int LMASK>
+__global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
+    constexpr int NL = rec_nl(LMASK);
+    constexpr int QF = rec_qfloats(LMASK);
+    constexpr int RS = rec_stride(LMASK);
+    constexpr int LMAX = lmax_of(LMASK);
+    __shared__ double sh[NL * 4];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    double part[NL];
+    #pragma unroll
+    for (int k = 0; k < NL; ++k) part[k] = 0.0;
+
+    if ((int)threadIdx.x < tl.count) {
+        const int i = tl.start + threadIdx.x;
+        const bool center = g.group == nullptr || g.group[i];
+        const int beg = g.off[i];
+        const int c = center ? g.off[i + 1] - beg : 0;
+        const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+        const double rc2 = e.rc2;
+        const int N = e.N;
+        uint64_t m0 = 0, m1 = 0;
+        int cnt = 0;
+        for (int p = 0; p < c; ++p) {
+            double d0, d1, d2;
+            double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+            if ((r2 > rc2) || (r2 < 1e-12)) continue;
+            ++cnt;
+            if (p < 64) m0 |= 1ull << p; else m1 |= 1ull << (p - 64);
+        }
+        if (cnt > N) {
+            double sr2[NMAX];
+            int sp[NMAX];
+            literal_insert<NMAX>(g, beg, c, xi0, xi1, xi2, rc2, N, sr2, sp);
+            m0 = 0; m1 = 0;
+            #pragma unroll
+            for (int s = 0; s < NMAX; ++s) {
+                if (s < N && sp[s] >= 0) {
+                    if (sp[s] < 64) m0 |= 1ull << sp[s]; else m1 |= 1ull << (sp[s] - 64);
+                }
+            }
+        }
+        const int n = cnt < N ? cnt : N;
+        float q[QF];
+        #pragma unroll
+        for (int k = 0; k < QF; ++k) q[k] = 0.0f;
+        uint64_t a0 = m0, a1 = m1;
+        while (a0 | a1) {
+            int p;
+            if (a0) { p = __builtin_ctzll(a0); a0 &= a0 - 1; }
+            else    { p = 64 + __builtin_ctzll(a1); a1 &= a1 - 1; }
+            double d0, d1, d2;
+            double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+            const float ir = __builtin_amdgcn_rsqf((float)r2);
+            const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
+            const float u2 = u * u;
+            float zr[LMAX + 1], zi[LMAX + 1];
+            zpow<LMAX>(nx, ny, zr, zi);
+            if constexpr (LMASK & kL3) acc_ylm<3>(u, u2, zr, zi, q + rec_off(LMASK, 3));
+            if constexpr (LMASK & kL4) acc_ylm<4>(u, u2, zr, zi, q + rec_off(LMASK, 4));
+            if constexpr (LMASK & kL6) acc_ylm<6>(u, u2, zr, zi, q + rec_off(LMASK, 6));
+        }
+        const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
+        #pragma unroll
+        for (int k = 0; k < QF; ++k) q[k] *= inv_n;
+        float out[RS];
+        #pragma unroll
+        for (int k = 0; k < RS; ++k) out[k] = k < QF ? q[k] : 0.0f;
+        const double inv_ng = e.inv_ng[tl.walker];
+        auto finish = [&](auto ll) {
+            constexpr int L = decltype(ll)::value;
+            if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
+                constexpr int o = rec_off(LMASK, L);
+                constexpr int li = rec_li(LMASK, L);
+                float s2 = q[o] * q[o];
+                #pragma unroll
+                for (int m = 1; m <= L; ++m) s2 += 2.0f * (q[o + 2 * m - 1] * q[o + 2 * m - 1] + q[o + 2 * m] * q[o + 2 * m]);
+                constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
+                const float ql = __builtin_sqrtf(s2 * P);
+                out[QF + li] = (ql > 0.0f && n > 0) ? (float)(P * inv_ng) / (ql * (float)n) : 0.0f;
+                e.ql[(int64_t)li * g.n_local_total + i] = ql;
+                part[li] = center ? (double)ql : 0.0;
+            }
+        };
+        finish(std::integral_constant<int, 3>{});
+        finish(std::integral_constant<int, 4>{});
+        finish(std::integral_constant<int, 6>{});
+        float4* dst = reinterpret_cast<float4*>(e.rec + (int64_t)i * RS);
+        #pragma unroll
+        for (int k = 0; k < RS / 4; ++k) dst[k] = make_float4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+        e.nsel[i] = n;
+        e.cnt_rc[i] = cnt;
+        reinterpret_cast<ulonglong2*>(e.selmask)[i] = make_ulonglong2(m0, m1);
+    }
+    block_sum<NL>(part, sh);
+    if (threadIdx.x == 0) {
+        #pragma unroll
+        for (int k = 0; k < NL; ++k) e.partial[(int64_t)e.slot_ql[k] * g.ntiles + t] = part[k];
+    }
+}
+
+// Literal selection export in the reference slot order (tests, per-atom export).
+template <int NMAX>
+__global__ __launch_bounds__(kBlock) void k_select_literal(Geo g, int first, int count, double rc2, int N,
+                                                           int32_t* sel_idx, int32_t* cnt_rc, int32_t* nsel) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= count) return;
+    const int i = first + a;
+    const bool center = g.group == nullptr || g.group[i];
+    const int beg = g.off[i];
+    const int c = center ? g.off[i + 1] - beg : 0;
+    const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+    double sr2[NMAX];
+    int sp[NMAX];
+    literal_insert<NMAX>(g, beg, c, xi0, xi1, xi2, rc2, N, sr2, sp);
+    int cnt = 0;
+    for (int p = 0; p < c; ++p) {
+        double d0, d1, d2;
+        double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+        if ((r2 > rc2) || (r2 < 1e-12)) continue;
+        ++cnt;
+    }
+    #pragma unroll
+    for (int s = 0; s < NMAX; ++s)
+        if (s < N) sel_idx[(int64_t)a * N + s] = sp[s] >= 0 ? g.idx[beg + sp[s]] : -1;
+    cnt_rc[a] = cnt;
+    nsel[a] = cnt < N ? cnt : N;
+}
+
+// ---------------------------------------------------------------------------
+// Lechner-Dellago local q: qbar_i = (q_i + sum_{j in S(i)} q_owner(j)) / (n_i + 1).
+// Writes lq, bq = P inv_ng / (lq (n+1)) * qbar, and per-tile sums of lq.
+// ---------------------------------------------------------------------------
+template <int L, int LMASK>
+__global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
+    constexpr int RS = rec_stride(LMASK);
+    constexpr int O = rec_off(LMASK, L);
+    constexpr int W = 2 * L + 1;
+    constexpr int BS = (W + 3) / 4 * 4;
+    __shared__ double sh[4];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    double part[1] = {0.0};
+    if ((int)threadIdx.x < tl.count) {
+        const int i = tl.start + threadIdx.x;
+        const bool center = g.group == nullptr || g.group[i];
+        float qb[W];
+        const float* ri = e.rec + (int64_t)i * RS + O;
+        #pragma unroll
+        for (int k = 0; k < W; ++k) qb[k] = ri[k];
+        const ulonglong2 mk = reinterpret_cast<const ulonglong2*>(e.selmask)[i];
+        uint64_t a0 = mk.x, a1 = mk.y;
+        const int beg = g.off[i];
+        while (a0 | a1) {
+            int p;
+            if (a0) { p = __builtin_ctzll(a0); a0 &= a0 - 1; }
+            else    { p = 64 + __builtin_ctzll(a1); a1 &= a1 - 1; }
+            const int o = owner_of(g, g.idx[beg + p]);
+            const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)o * RS);
+            float r[RS];
+            #pragma unroll
+            for (int k = 0; k < RS / 4; ++k) {
+                float4 v = rp[k];
+                r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
+            }
+            #pragma unroll
+            for (int k = 0; k < W; ++k) qb[k] += r[O + k];
+        }
+        const int n = e.nsel[i];
+        const float inv = 1.0f / (float)(n + 1);
+        float s2 = 0.0f;
+        #pragma unroll
+        for (int k = 0; k < W; ++k) {
+            qb[k] *= inv;
+            s2 += (k == 0 ? 1.0f : 2.0f) * qb[k] * qb[k];
+        }
+        constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
+        const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
+        const float beta = lq > 0.0f ? (float)(P * e.inv_ng[tl.walker]) / (lq * (float)(n + 1)) : 0.0f;
+        d.lq[i] = lq;
+        float* bq = d.bq + (int64_t)i * BS;
+        #pragma unroll
+        for (int k = 0; k < BS; ++k) bq[k] = k < W ? beta * qb[k] : 0.0f;
+        part[0] = (double)lq;
+    }
+    block_sum<1>(part, sh);
+    if (threadIdx.x == 0) e.partial[(int64_t)d.slot * g.ntiles + t] = part[0];
+}
+
+// extra_i[l block] = w / n_i * (bq_i + sum_{i' : i in S(i')} bq_i')
+template <int L, int LMASK>
+__global__ __launch_bounds__(kBlock) void k_ld_coeff(Geo g, EnvDev e, LdDev d) {
+    constexpr int RS = rec_stride(LMASK);
+    constexpr int O = rec_off(LMASK, L);
+    constexpr int W = 2 * L + 1;
+    constexpr int BS = (W + 3) / 4 * 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    if ((int)threadIdx.x >= tl.count) return;
+    const int i = tl.start + threadIdx.x;
+    const double w = e.wcv[(int64_t)tl.walker * kMaxSlotCV + d.wslot];
+    float C[BS];
+    const float4* bi = reinterpret_cast<const float4*>(d.bq + (int64_t)i * BS);
+    #pragma unroll
+    for (int k = 0; k < BS / 4; ++k) { float4 v = bi[k]; C[4*k] = v.x; C[4*k+1] = v.y; C[4*k+2] = v.z; C[4*k+3] = v.w; }
+    const int beg = g.off[i], c = g.off[i + 1] - beg;
+    for (int p = 0; p < c; ++p) {
+        const int rp = g.revpos[beg + p];
+        if (rp == 0xFF) continue;
+        const int o = owner_of(g, g.idx[beg + p]);
+        const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[o];
+        const bool rev = rp < 64 ? ((mo.x >> rp) & 1ull) : ((mo.y >> (rp - 64)) & 1ull);
+        if (!rev) continue;
+        const float4* bo = reinterpret_cast<const float4*>(d.bq + (int64_t)o * BS);
+        #pragma unroll
+        for (int k = 0; k < BS / 4; ++k) { float4 v = bo[k]; C[4*k] += v.x; C[4*k+1] += v.y; C[4*k+2] += v.z; C[4*k+3] += v.w; }
+    }
+    const int n = e.nsel[i];
+    const float s = n > 0 ? (float)w / (float)n : 0.0f;
+    float* ex = e.extra + (int64_t)i * RS + O;
+    #pragma unroll
+    for (int k = 0; k < W; ++k) ex[k] = s * C[k];
+}
+
+// ---------------------------------------------------------------------------
+// k_force: F_k = sum_{j in S(k)} g_kj - sum_{(i,j): owner(j)=k, j in S(i)} g_ij,
+// g_ij = sum_lm mu_m Re(conj(K_ilm) grad Y_lm(x_j - x_i)),
+// K_i = w_l a_il q_il (+ extra_i).  Gather over k's own list: every candidate
+// that is selected by k (own) or selects k (rev, via the reverse position)
+// contributes once, with the two coefficient sets added before the contraction.
+// ---------------------------------------------------------------------------
+template <int LMASK, bool EXTRA>
+__device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* wl, float* K) {
+    constexpr int RS = rec_stride(LMASK);
+    constexpr int QF = rec_qfloats(LMASK);
+    const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)a * RS);
+    float r[RS];
+    #pragma unroll
+    for (int k = 0; k < RS / 4; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
+    float x[RS];
+    if constexpr (EXTRA) {
+        const float4* xp = reinterpret_cast<const float4*>(e.extra + (int64_t)a * RS);
+        #pragma unroll
+        for (int k = 0; k < RS / 4; ++k) { float4 v = xp[k]; x[4*k] = v.x; x[4*k+1] = v.y; x[4*k+2] = v.z; x[4*k+3] = v.w; }
+    }
+    auto one = [&](auto ll) {
+        constexpr int L = decltype(ll)::value;
+        if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
+            constexpr int o = rec_off(LMASK, L);
+            constexpr int li = rec_li(LMASK, L);
+            const float s = wl[li] * r[QF + li];
+            #pragma unroll
+            for (int k = 0; k < 2 * L + 1; ++k) {
+                float v = s * r[o + k];
+                if constexpr (EXTRA) v += x[o + k];
+                K[o + k] = v;
+            }
+        }
+    };
+    one(std::integral_constant<int, 3>{});
+    one(std::integral_constant<int, 4>{});
+    one(std::integral_constant<int, 6>{});
+}
+
+template <int LMASK, bool EXTRA>
+__global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __restrict__ f) {
+    constexpr int QF = rec_qfloats(LMASK);
+    constexpr int NL = rec_nl(LMASK);
+    constexpr int LMAX = lmax_of(LMASK);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    if ((int)threadIdx.x >= tl.count) return;
+    const int i = tl.start + threadIdx.x;
+    float wl[NL];
+    #pragma unroll
+    for (int k = 0; k < NL; ++k)
+        wl[k] = e.wslot[k] >= 0 ? (float)e.wcv[(int64_t)tl.walker * kMaxSlotCV + e.wslot[k]] : 0.0f;
+    float Ki[QF];
+    load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
+    const ulonglong2 mk = reinterpret_cast<const ulonglong2*>(e.selmask)[i];
+    const int beg = g.off[i], c = g.off[i + 1] - beg;
+    const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
+    for (int p = 0; p < c; ++p) {
+        const bool own = p < 64 ? ((mk.x >> p) & 1ull) : ((mk.y >> (p - 64)) & 1ull);
+        const int rp = g.revpos[beg + p];
+        const int j = g.idx[beg + p];
+        bool rev = false;
+        int o = 0;
+        if (rp != 0xFF) {
+            o = owner_of(g, j);
+            const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[o];
+            rev = rp < 64 ? ((mo.x >> rp) & 1ull) : ((mo.y >> (rp - 64)) & 1ull);
+        }
+        if (!(own || rev)) continue;
+        float C[QF];
+        #pragma unroll
+        for (int k = 0; k < QF; ++k) C[k] = own ? Ki[k] : 0.0f;
+        if (rev) {
+            float Ko[QF];
+            load_coeff<LMASK, EXTRA>(e, o, wl, Ko);
+            // reverse pair enters with (-1)^l
+            if constexpr (LMASK & kL3) {
+                #pragma unroll
+                for (int k = 0; k < 7; ++k) C[rec_off(LMASK, 3) + k] -= Ko[rec_off(LMASK, 3) + k];
+            }
+            if constexpr (LMASK & kL4) {
+                #pragma unroll
+                for (int k = 0; k < 9; ++k) C[rec_off(LMASK, 4) + k] += Ko[rec_off(LMASK, 4) + k];
+            }
+            if constexpr (LMASK & kL6) {
+                #pragma unroll
+                for (int k = 0; k < 13; ++k) C[rec_off(LMASK, 6) + k] += Ko[rec_off(LMASK, 6) + k];
+            }
+        }
+        double d0, d1, d2;
+        double r2 = pair_r2(g.x, j, xi0, xi1, xi2, d0, d1, d2);
+        const float ir = __builtin_amdgcn_rsqf((float)r2);
+        const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
+        const float u2 = u * u;
+        float zr[LMAX + 1], zi[LMAX + 1];
+        zpow<LMAX>(nx, ny, zr, zi);
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        if constexpr (LMASK & kL3) acc_grad<3>(u, u2, zr, zi, C + rec_off(LMASK, 3), gx, gy, gz);
+        if constexpr (LMASK & kL4) acc_grad<4>(u, u2, zr, zi, C + rec_off(LMASK, 4), gx, gy, gz);
+        if constexpr (LMASK & kL6) acc_grad<6>(u, u2, zr, zi, C + rec_off(LMASK, 6), gx, gy, gz);
+        const float dot = gx * nx + gy * ny + gz * u;
+        Fx = __builtin_fmaf(ir, __builtin_fmaf(-dot, nx, gx), Fx);
+        Fy = __builtin_fmaf(ir, __builtin_fmaf(-dot, ny, gy), Fy);
+        Fz = __builtin_fmaf(ir, __builtin_fmaf(-dot, u, gz), Fz);
+    }
+    f[3 * (int64_t)i] += (double)Fx;
+    f[3 * (int64_t)i + 1] += (double)Fy;
+    f[3 * (int64_t)i + 2] += (double)Fz;
+}
+
+// ---------------------------------------------------------------------------
+// Bias: uniform grid (Catmull-Rom at bin centres, clamp) and exact hills sum.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cr_value(double p0, double p1, double p2, double p3, double x) {
+    return p1 + 0.5 * x * ((p2 - p0) + x * (2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3) +
+                           x * x * (-p0 + 3.0 * p1 - 3.0 * p2 + p3));
+}
+__device__ __forceinline__ double cr_deriv(double p0, double p1, double p2, double p3, double x) {
+    return 0.5 * ((p2 - p0) + 2.0 * x * (2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3) +
+                  3.0 * x * x * (-p0 + 3.0 * p1 - 3.0 * p2 + p3));
+}
+__device__ __forceinline__ void center_frac_clamp(double cv, double lo, double h, int nb, int& i, double& f) {
+    double u = (cv - lo) / h - 0.5;
+    i = (int)floor(u);
+    f = u - i;
+    if (i < 1) { i = 1; f = 0.0; }
+    if (i > nb - 3) { i = nb - 3; f = 1.0; }
+}
+
+// grid value and gradient at s (one thread)
+__device__ double grid_eval(const ReduceDev& r, const double* grid, const double* s, double* dV) {
+#pragma clang fp contract(off)
+    if (r.D == 1) {
+        double h = (r.hi[0] - r.lo[0]) / r.nb[0];
+        int i; double x;
+        center_frac_clamp(s[0], r.lo[0], h, r.nb[0], i, x);
+        double p0 = grid[i - 1], p1 = grid[i], p2 = grid[i + 1], p3 = grid[i + 2];
+        double d = cr_deriv(p0, p1, p2, p3, x) / h;
+        if (isnan(d)) d = 0.0;
+        dV[0] = d;
+        return cr_value(p0, p1, p2, p3, x);
+    }
+    double h0 = (r.hi[0] - r.lo[0]) / r.nb[0], h1 = (r.hi[1] - r.lo[1]) / r.nb[1];
+    int i0, j0; double dx, dy;
+    center_frac_clamp(s[0], r.lo[0], h0, r.nb[0], i0, dx);
+    center_frac_clamp(s[1], r.lo[1], h1, r.nb[1], j0, dy);
+    const int n1 = r.nb[1];
+    double col[4], dcol[4];
+    for (int t = 0; t < 4; ++t) {
+        int jj = j0 - 1 + t;
+        double p0 = grid[(int64_t)(i0 - 1) * n1 + jj], p1 = grid[(int64_t)i0 * n1 + jj];
+        double p2 = grid[(int64_t)(i0 + 1) * n1 + jj], p3 = grid[(int64_t)(i0 + 2) * n1 + jj];
+        col[t] = cr_value(p0, p1, p2, p3, dx);
+        dcol[t] = cr_deriv(p0, p1, p2, p3, dx);
+    }
+    dV[0] = cr_value(dcol[0], dcol[1], dcol[2], dcol[3], dy) / h0;
+    dV[1] = cr_deriv(col[0], col[1], col[2], col[3], dy) / h1;
+    if (isnan(dV[0]) || isnan(dV[1])) { dV[0] = 0.0; dV[1] = 0.0; }
+    return cr_value(col[0], col[1], col[2], col[3], dy);
+}
+
+// Deposit one hill into a grid with the block's threads (add_to_grid).
+__device__ void grid_deposit_block(const ReduceDev& r, double* grid, const double* s, double w, double sig) {
+#pragma clang fp contract(off)
+    int lower[2] = {0, 0}, upper[2] = {0, 0};
+    for (int k = 0; k < r.D; ++k) {
+        double dcv = (r.hi[k] - r.lo[k]) / r.nb[k];
+        int R = (int)ceil(4.0 * sig / dcv) + 1;
+        int c = (int)((s[k] - r.lo[k]) / dcv);
+        lower[k] = c - R > 0 ? c - R : 0;
+        upper[k] = c + R < r.nb[k] - 1 ? c + R : r.nb[k] - 1;
+    }
+    if (r.D == 1) {
+        for (int g = lower[0] + threadIdx.x; g <= upper[0]; g += blockDim.x) {
+            double ctr = r.lo[0] + (g + 0.5) * (r.hi[0] - r.lo[0]) / r.nb[0];
+            double dx = s[0] - ctr;
+            grid[g] += w * exp(-0.5 * (dx * dx) / (sig * sig));
+        }
+    } else {
+        const int n0 = upper[0] - lower[0] + 1, n1 = upper[1] - lower[1] + 1;
+        for (int q = threadIdx.x; q < n0 * n1; q += blockDim.x) {
+            if (n0 <= 0 || n1 <= 0) break;
+            const int i = lower[0] + q / n1, j = lower[1] + q % n1;
+            double d0 = s[0] - (r.lo[0] + (i + 0.5) * (r.hi[0] - r.lo[0]) / r.nb[0]);
+            double d1 = s[1] - (r.lo[1] + (j + 0.5) * (r.hi[1] - r.lo[1]) / r.nb[1]);
+            grid[(int64_t)i * r.nb[1] + j] += w * exp(-0.5 * (d0 * d0 + d1 * d1) / (sig * sig));
+        }
+    }
+}
+
+// Exact FP64 hills sum at s, block-cooperative; V and dV valid on thread 0.
+__device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh) {
+    const double* H = r.hills + (int64_t)w * r.hills_cap * (r.D + 3);
+    const int64_t nh = r.nhills[w];
+    double acc[4] = {0, 0, 0, 0};
+    for (int64_t h = threadIdx.x; h < nh; h += blockDim.x) {
+        const double* rec = H + h * (r.D + 3);
+        double ex = 0, dd[kMaxDim];
+        for (int d = 0; d < r.D; ++d) {
+            dd[d] = s[d] - rec[1 + d];
+            ex += dd[d] * dd[d] / (2.0 * r.sigma[d] * r.sigma[d]);
+        }
+        double t = rec[1 + r.D] * exp(-ex);
+        acc[0] += t;
+        for (int d = 0; d < r.D; ++d) acc[1 + d] -= t * dd[d] / (r.sigma[d] * r.sigma[d]);
+    }
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0) for (int d = 0; d < r.D; ++d) dV[d] = acc[1 + d];
+    __syncthreads();
+    return acc[0];
+}
+
+// learned bias: 3 -> 64 -> 64 -> 1 tanh network, packed FP32 weights
+// [W1 64x3][b1 64][W2 64x64][b2 64][W3 64][b3]; thread 0 evaluates.
+constexpr int kH = 64;
+__device__ double mlp_eval(const ReduceDev& r, const double* s, double* dV) {
+    const float* W1 = r.mlp;
+    const float* b1 = W1 + kH * 3;
+    const float* W2 = b1 + kH;
+    const float* b2 = W2 + kH * kH;
+    const float* W3 = b2 + kH;
+    const float b3 = W3[kH];
+    float in[3];
+    for (int d = 0; d < 3; ++d) in[d] = (float)((s[d] - r.mlp_shift_in[d]) * r.mlp_scale_in[d]);
+    float h1[kH], h2[kH], g2[kH];
+    for (int a = 0; a < kH; ++a) {
+        float z = b1[a];
+        for (int d = 0; d < 3; ++d) z += W1[a * 3 + d] * in[d];
+        h1[a] = tanhf(z);
+    }
+    float v = b3;
+    for (int a = 0; a < kH; ++a) {
+        float z = b2[a];
+        for (int b = 0; b < kH; ++b) z += W2[a * kH + b] * h1[b];
+        h2[a] = tanhf(z);
+        v += W3[a] * h2[a];
+        g2[a] = W3[a] * (1.0f - h2[a] * h2[a]);
+    }
+    float gi[3] = {0, 0, 0};
+    for (int b = 0; b < kH; ++b) {
+        float t = 0;
+        for (int a = 0; a < kH; ++a) t += g2[a] * W2[a * kH + b];
+        t *= (1.0f - h1[b] * h1[b]);
+        for (int d = 0; d < 3; ++d) gi[d] += t * W1[b * 3 + d];
+    }
+    for (int d = 0; d < 3; ++d) dV[d] = (double)gi[d] * r.mlp_scale_in[d] / r.mlp_scale_out;
+    return (double)v / r.mlp_scale_out + r.mlp_shift_out;
+}
+
+// V(s) and dV/ds of walker w's bias, block-cooperative; valid on thread 0.
+__device__ double bias_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh) {
+    if (r.kind == BIAS_GRID) {
+        double v = 0.0;
+        if (threadIdx.x == 0) v = grid_eval(r, r.grid + (int64_t)w * r.grid_size, s, dV);
+        return v;
+    }
+    if (r.kind == BIAS_MLP && r.mlp_ok[w]) {
+        double v = 0.0;
+        if (threadIdx.x == 0) v = mlp_eval(r, s, dV);
+        return v;
+    }
+    return hills_eval_block(r, w, s, dV, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Expression evaluation with forward-mode derivatives over the symbols.
+// ---------------------------------------------------------------------------
+struct Dual { double v; double d[kMaxSym]; };
+
+__device__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double* val, double* dval) {
+    Dual st[kStack];
+    int sp = 0;
+    for (int k = 0; k < p.n; ++k) {
+        const int op = p.op[k];
+        if (op == OP_CONST || op == OP_SYM) {
+            Dual& a = st[sp++];
+            for (int q = 0; q < kMaxSym; ++q) a.d[q] = 0.0;
+            if (op == OP_CONST) a.v = p.c[k];
+            else { int q = (int)p.c[k]; a.v = sym[q]; a.d[q] = 1.0; }
+            continue;
+        }
+        if (op >= OP_ADD && op <= OP_POW || op == OP_MIN || op == OP_MAX) {
+            Dual b = st[--sp];
+            Dual& a = st[sp - 1];
+            double av = a.v, bv = b.v, da = 0, db = 0, rv = 0;
+            switch (op) {
+                case OP_ADD: rv = av + bv; da = 1; db = 1; break;
+                case OP_SUB: rv = av - bv; da = 1; db = -1; break;
+                case OP_MUL: rv = av * bv; da = bv; db = av; break;
+                case OP_DIV: rv = av / bv; da = 1.0 / bv; db = -av / (bv * bv); break;
+                case OP_POW: rv = pow(av, bv); da = bv * pow(av, bv - 1.0);
+                             db = av > 0 ? rv * log(av) : 0.0; break;
+                case OP_MIN: rv = av <= bv ? av : bv; da = av <= bv; db = !(av <= bv); break;
+                case OP_MAX: rv = av >= bv ? av : bv; da = av >= bv; db = !(av >= bv); break;
+            }
+            for (int q = 0; q < kMaxSym; ++q) a.d[q] = da * a.d[q] + db * b.d[q];
+            a.v = rv;
+            continue;
+        }
+        Dual& a = st[sp - 1];
+        double x = a.v, rv = 0, dx = 0;
+        switch (op) {
+            case OP_NEG: rv = -x; dx = -1; break;
+            case OP_SIN: rv = sin(x); dx = cos(x); break;
+            case OP_COS: rv = cos(x); dx = -sin(x); break;
+            case OP_TAN: rv = tan(x); dx = 1.0 + rv * rv; break;
+            case OP_EXP: rv = exp(x); dx = rv; break;
+            case OP_LOG: rv = log(x); dx = 1.0 / x; break;
+            case OP_LOG10: rv = log10(x); dx = 1.0 / (x * 2.302585092994046); break;
+            case OP_SQRT: rv = sqrt(x); dx = 0.5 / rv; break;
+            case OP_ABS: rv = fabs(x); dx = x >= 0 ? 1.0 : -1.0; break;
+            case OP_TANH: rv = tanh(x); dx = 1.0 - rv * rv; break;
+            case OP_ASIN: rv = asin(x); dx = 1.0 / sqrt(1.0 - x * x); break;
+            case OP_ACOS: rv = acos(x); dx = -1.0 / sqrt(1.0 - x * x); break;
+            case OP_ATAN: rv = atan(x); dx = 1.0 / (1.0 + x * x); break;
+            case OP_SINH: rv = sinh(x); dx = cosh(x); break;
+            case OP_COSH: rv = cosh(x); dx = sinh(x); break;
+        }
+        for (int q = 0; q < kMaxSym; ++q) a.d[q] *= dx;
+        a.v = rv;
+    }
+    *val = st[0].v;
+    for (int q = 0; q < nsym; ++q) dval[q] = st[0].d[q];
+}
+
+// ---------------------------------------------------------------------------
+// k_reduce: one workgroup per walker.  Post-force order of
+// FixMetadynamics::post_force (src/core/fix_crystallize.cu:409-450):
+// CVs -> expression -> deposit on step % pace == 0 (WT height from the bias
+// before the deposit) -> V and dV/ds after the deposit -> chain rule.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceDev r) {
+    const int w = blockIdx.x;
+    __shared__ double sh[64];
+    __shared__ double cv[kMaxSlotCV];
+    __shared__ double s[kMaxDim];
+    __shared__ double dsdv[kMaxDim * kMaxSym];
+    __shared__ double bc[8];
+    const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
+    // 1. CV values (deterministic FP64 tree over tiles)
+    for (int c = 0; c < r.ncv; ++c) {
+        double v[1] = {0.0};
+        const int src = r.cv_src[c];
+        if (src >= 0) {
+            for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v[0] += r.partial[(int64_t)src * r.ntiles + t];
+        }
+        block_sum<1>(v, sh);
+        if (threadIdx.x == 0) cv[c] = src >= 0 ? v[0] * r.inv_ng[w] : r.direct[(int64_t)w * kMaxSlotCV + c];
+        __syncthreads();
+    }
+    // 2. dimensions
+    if (threadIdx.x == 0) {
+        double sym[kMaxSym];
+        for (int q = 0; q < r.nsym; ++q) sym[q] = cv[r.sym_slot[q]];
+        for (int d = 0; d < r.D; ++d) expr_eval(r.prog[d], r.nsym, sym, &s[d], &dsdv[d * kMaxSym]);
+    }
+    __syncthreads();
+    // 3. deposit
+    const int64_t step = r.step[w];
+    const bool dep = r.pace > 0 && (step % r.pace) == 0;
+    double dV[kMaxDim] = {0, 0, 0};
+    if (dep) {
+        double Vb = 0.0;
+        if (r.wt) Vb = bias_eval_block(r, w, s, dV, sh);
+        if (threadIdx.x == 0) bc[0] = r.h0 * exp(-(Vb) / (r.kT * (r.biasf - 1.0)));
+        __syncthreads();
+        const double hgt = bc[0];
+        if (r.kind == BIAS_GRID) grid_deposit_block(r, r.grid + (int64_t)w * r.grid_size, s, hgt, r.sigma[0]);
+        if (threadIdx.x == 0) {
+            int64_t k = r.nhills[w];
+            if (k < r.hills_cap) {
+                double* rec = r.hills + ((int64_t)w * r.hills_cap + k) * (r.D + 3);
+                rec[0] = (double)step;
+                for (int d = 0; d < r.D; ++d) rec[1 + d] = s[d];
+                rec[1 + r.D] = hgt;
+                rec[2 + r.D] = r.sigma[0];
+                r.nhills[w] = k + 1;
+            }
+        }
+        __syncthreads();
+    }
+    // 4. bias value and gradient after the deposit
+    const double V = bias_eval_block(r, w, s, dV, sh);
+    double Vx = 0.0;
+    if (r.kind == BIAS_MLP) {
+        double dx[kMaxDim];
+        Vx = hills_eval_block(r, w, s, dx, sh);
+    }
+    if (threadIdx.x == 0) {
+        double* st = r.state + (int64_t)w * kStStride;
+        for (int c = 0; c < r.ncv; ++c) st[kStCv + c] = cv[c];
+        for (int d = 0; d < r.D; ++d) { st[kStS + d] = s[d]; st[kStDV + d] = dV[d]; }
+        st[kStV] = V;
+        st[kStHill] = dep ? bc[0] : 0.0;
+        st[kStStep] = (double)step;
+        st[kStVexact] = Vx;
+        // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv
+        double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
+        for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = 0.0;
+        for (int q = 0; q < r.nsym; ++q) {
+            double acc = 0.0;
+            for (int d = 0; d < r.D; ++d) acc += dV[d] * dsdv[d * kMaxSym + q];
+            wc[r.sym_slot[q]] += acc;
+        }
+        r.step[w] = step + 1;
+    }
+}
+
+__global__ void k_bias_eval(ReduceDev r, int w, int64_t np, const double* s, double* V, double* dV) {
+    __shared__ double sh[64];
+    for (int64_t p = blockIdx.x; p < np; p += gridDim.x) {
+        double sp[kMaxDim];
+        for (int d = 0; d < r.D; ++d) sp[d] = s[p * r.D + d];
+        double g[kMaxDim] = {0, 0, 0};
+        double v = bias_eval_block(r, w, sp, g, sh);
+        if (threadIdx.x == 0) {
+            if (V) V[p] = v;
+            if (dV) for (int d = 0; d < r.D; ++d) dV[p * r.D + d] = g[d];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_bias_add(ReduceDev r, int w, int64_t nh, const double* s, const double* h, double sig) {
+    for (int64_t k = 0; k < nh; ++k) {
+        double sp[kMaxDim];
+        for (int d = 0; d < r.D; ++d) sp[d] = s[k * r.D + d];
+        if (r.kind == BIAS_GRID) {
+            grid_deposit_block(r, r.grid + (int64_t)w * r.grid_size, sp, h[k], sig);
+        } else if (threadIdx.x == 0) {
+            int64_t q = r.nhills[w];
+            if (q < r.hills_cap) {
+                double* rec = r.hills + ((int64_t)w * r.hills_cap + q) * (r.D + 3);
+                rec[0] = -1.0;
+                for (int d = 0; d < r.D; ++d) rec[1 + d] = sp[d];
+                rec[1 + r.D] = h[k];
+                rec[2 + r.D] = sig;
+                r.nhills[w] = q + 1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill(double* p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f(float* p, int64_t n, float v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Structure factor.  k is stored in revolutions per Angstrom (k / 2pi) so the
+// FP64 phase reduces to its fractional part before the FP32 sin/cos.
+// ---------------------------------------------------------------------------
+constexpr int kSkTile = 16;   // k vectors per workgroup in the amplitude pass
+
+__device__ __forceinline__ void sincos_rev(double ph, float& s, float& c) {
+    const double fr = ph - rint(ph);                 // in [-0.5, 0.5]
+    const float a = (float)fr * 6.28318530717958647692f;
+    __sincosf(a, &s, &c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
+    __shared__ double sh[kSkTile * 2 * 4];
+    __shared__ double kt[kSkTile * 3];
+    const int kb = blockIdx.x * kSkTile;
+    const int ch = blockIdx.y;
+    const int start = sk.chunk[3 * ch], count = sk.chunk[3 * ch + 1];
+    if (threadIdx.x < kSkTile * 3) {
+        int q = kb + threadIdx.x / 3;
+        kt[threadIdx.x] = q < sk.nk ? sk.k[3 * q + threadIdx.x % 3] : 0.0;
+    }
+    __syncthreads();
+    double acc[kSkTile * 2];
+    #pragma unroll
+    for (int q = 0; q < kSkTile * 2; ++q) acc[q] = 0.0;
+    for (int a = threadIdx.x; a < count; a += blockDim.x) {
+        const int i = start + a;
+        if (g.group && !g.group[i]) continue;
+        const double x0 = g.x[3 * i], x1 = g.x[3 * i + 1], x2 = g.x[3 * i + 2];
+        #pragma unroll
+        for (int q = 0; q < kSkTile; ++q) {
+            const double ph = x0 * kt[3 * q] + x1 * kt[3 * q + 1] + x2 * kt[3 * q + 2];
+            float sn, cs;
+            sincos_rev(ph, sn, cs);
+            acc[2 * q] += (double)cs;
+            acc[2 * q + 1] += (double)sn;
+        }
+    }
+    block_sum<kSkTile * 2>(acc, sh);
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < kSkTile; ++q) {
+            if (kb + q >= sk.nk) break;
+            sk.apart[((int64_t)ch * sk.nk + kb + q) * 2] = acc[2 * q];
+            sk.apart[((int64_t)ch * sk.nk + kb + q) * 2 + 1] = acc[2 * q + 1];
+        }
+    }
+}
+
+// per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S
+__global__ __launch_bounds__(kBlock) void k_sk_sum(SkDev sk) {
+    __shared__ double sh[4];
+    const int w = blockIdx.x;
+    const int c0 = sk.chunk_begin[w], c1 = sk.chunk_begin[w + 1];
+    const double N = (double)sk.n_atoms[w];
+    double v[1] = {0.0};
+    for (int q = threadIdx.x; q < sk.nk; q += blockDim.x) {
+        double re = 0, im = 0;
+        for (int c = c0; c < c1; ++c) {
+            re += sk.apart[((int64_t)c * sk.nk + q) * 2];
+            im += sk.apart[((int64_t)c * sk.nk + q) * 2 + 1];
+        }
+        sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
+        sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
+        v[0] += (re * re + im * im) / N;
+    }
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) sk.direct[(int64_t)w * kMaxSlotCV + sk.slot] = v[0] / sk.nk;
+}
+
+// F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j})
+constexpr int kSkFT = 256;
+__global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f) {
+    __shared__ double kt[kSkFT * 3];
+    __shared__ float kv[kSkFT * 3];
+    __shared__ float am[kSkFT * 2];
+    const int w = blockIdx.y;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = sk.n_atoms[w];
+    const double wt = sk.wcv[(int64_t)w * kMaxSlotCV + sk.wslot];
+    const float scale = (float)(wt * 2.0 / ((double)n * sk.nk));
+    const int i = sk.atom_begin[w] + a;
+    const bool act = a < n && (g.group == nullptr || g.group[i]);
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (act) { x0 = g.x[3 * i]; x1 = g.x[3 * i + 1]; x2 = g.x[3 * i + 2]; }
+    float gx = 0, gy = 0, gz = 0;
+    for (int kb = 0; kb < sk.nk; kb += kSkFT) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < kSkFT; q += blockDim.x) {
+            const int qq = kb + q;
+            const bool ok = qq < sk.nk;
+            for (int d = 0; d < 3; ++d) {
+                kt[3 * q + d] = ok ? sk.k[3 * qq + d] : 0.0;
+                kv[3 * q + d] = ok ? (float)sk.kvec[3 * qq + d] : 0.0f;
+            }
+            am[2 * q] = ok ? (float)sk.amp[((int64_t)w * sk.nk + qq) * 2] : 0.0f;
+            am[2 * q + 1] = ok ? (float)sk.amp[((int64_t)w * sk.nk + qq) * 2 + 1] : 0.0f;
+        }
+        __syncthreads();
+        if (act) {
+            const int lim = sk.nk - kb < kSkFT ? sk.nk - kb : kSkFT;
+            for (int q = 0; q < lim; ++q) {
+                const double ph = x0 * kt[3 * q] + x1 * kt[3 * q + 1] + x2 * kt[3 * q + 2];
+                float sn, cs;
+                sincos_rev(ph, sn, cs);
+                const float imv = __builtin_fmaf(am[2 * q], sn, -am[2 * q + 1] * cs);
+                gx = __builtin_fmaf(kv[3 * q], imv, gx);
+                gy = __builtin_fmaf(kv[3 * q + 1], imv, gy);
+                gz = __builtin_fmaf(kv[3 * q + 2], imv, gz);
+            }
+        }
+    }
+    if (act) {
+        f[3 * (int64_t)i] += (double)(scale * gx);
+        f[3 * (int64_t)i + 1] += (double)(scale * gy);
+        f[3 * (int64_t)i + 2] += (double)(scale * gz);
+    }
+}
+
+template <int NMAX>
+hipError_t env_dispatch_l(const Geo& g, const EnvDev& e, dim3 grid, hipStream_t st) {
+    switch (e.lmask) {
+        case kL6:             hipLaunchKernelGGL((k_env<NMAX, kL6>), grid, dim3(kBlock), 0, st, g, e); break;
+        case kL4:             hipLaunchKernelGGL((k_env<NMAX, kL4>), grid, dim3(kBlock), 0, st, g, e); break;
+        case kL4 | kL6:       hipLaunchKernelGGL((k_env<NMAX, kL4 | kL6>), grid, dim3(kBlock), 0, st, g, e); break;
+        case kL3:             hipLaunchKernelGGL((k_env<NMAX, kL3>), grid, dim3(kBlock), 0, st, g, e); break;
+        case kL3 | kL4:       hipLaunchKernelGGL((k_env<NMAX, kL3 | kL4>), grid, dim3(kBlock), 0, st, g, e); break;
+        case kL3 | kL6:       hipLaunchKernelGGL((k_env<NMAX, kL3 | kL6>), grid, dim3(kBlock), 0, st, g, e); break;
+        case kL3 | kL4 | kL6: hipLaunchKernelGGL((k_env<NMAX, kL3 | kL4 | kL6>), grid, dim3(kBlock), 0, st, g, e); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_revpos(const Geo& g, int64_t n, uint8_t* revpos, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_revpos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, revpos);
+    return hipGetLastError();
+}
+
+hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st) {
+    if (g.ntiles == 0) return hipSuccess;
+    dim3 grid(xcd_grid(g.ntiles));
+    if (e.N <= 12) return env_dispatch_l<12>(g, e, grid, st);
+    if (e.N <= 16) return env_dispatch_l<16>(g, e, grid, st);
+    if (e.N <= 18) return env_dispatch_l<18>(g, e, grid, st);
+    if (e.N <= 24) return env_dispatch_l<24>(g, e, grid, st);
+    if (e.N <= 32) return env_dispatch_l<32>(g, e, grid, st);
+    if (e.N <= 64) return env_dispatch_l<64>(g, e, grid, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_select_literal(const Geo& g, int32_t first, int32_t count, double rc2, int N,
+                                 int32_t* sel_idx, int32_t* cnt_rc, int32_t* nsel, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    dim3 grid((count + kBlock - 1) / kBlock);
+#define MDT_SEL(NM) hipLaunchKernelGGL(k_select_literal<NM>, grid, dim3(kBlock), 0, st, g, first, count, rc2, N, sel_idx, cnt_rc, nsel)
+    if (N <= 12) MDT_SEL(12);
+    else if (N <= 16) MDT_SEL(16);
+    else if (N <= 18) MDT_SEL(18);
+    else if (N <= 24) MDT_SEL(24);
+    else if (N <= 32) MDT_SEL(32);
+    else if (N <= 64) MDT_SEL(64);
+    else return hipErrorInvalidValue;
+#undef MDT_SEL
+    return hipGetLastError();
+}
+
+template <int L>
+static hipError_t ld_dispatch(const Geo& g, const EnvDev& e, const LdDev& d, dim3 grid, hipStream_t st, bool coeff) {
+#define MDT_LD(LM) do { if (coeff) hipLaunchKernelGGL((k_ld_coeff<L, LM>), grid, dim3(kBlock), 0, st, g, e, d); \
+                        else hipLaunchKernelGGL((k_ld<L, LM>), grid, dim3(kBlock), 0, st, g, e, d); } while (0)
+    switch (e.lmask) {
+        case kL6: if constexpr (L == 6) MDT_LD(kL6); else return hipErrorInvalidValue; break;
+        case kL4: if constexpr (L == 4) MDT_LD(kL4); else return hipErrorInvalidValue; break;
+        case kL3: if constexpr (L == 3) MDT_LD(kL3); else return hipErrorInvalidValue; break;
+        case kL4 | kL6: if constexpr (L != 3) MDT_LD(kL4 | kL6); else return hipErrorInvalidValue; break;
+        case kL3 | kL4: if constexpr (L != 6) MDT_LD(kL3 | kL4); else return hipErrorInvalidValue; break;
+        case kL3 | kL6: if constexpr (L != 4) MDT_LD(kL3 | kL6); else return hipErrorInvalidValue; break;
+        case kL3 | kL4 | kL6: MDT_LD(kL3 | kL4 | kL6); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef MDT_LD
+    return hipGetLastError();
+}
+
+hipError_t launch_ld(const Geo& g, const EnvDev& e, const LdDev& d, hipStream_t st) {
+    if (g.ntiles == 0) return hipSuccess;
+    dim3 grid(xcd_grid(g.ntiles));
+    if (d.l == 6) return ld_dispatch<6>(g, e, d, grid, st, false);
+    if (d.l == 4) return ld_dispatch<4>(g, e, d, grid, st, false);
+    if (d.l == 3) return ld_dispatch<3>(g, e, d, grid, st, false);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_ld_coeff(const Geo& g, const EnvDev& e, const LdDev& d, hipStream_t st) {
+    if (g.ntiles == 0) return hipSuccess;
+    dim3 grid(xcd_grid(g.ntiles));
+    if (d.l == 6) return ld_dispatch<6>(g, e, d, grid, st, true);
+    if (d.l == 4) return ld_dispatch<4>(g, e, d, grid, st, true);
+    if (d.l == 3) return ld_dispatch<3>(g, e, d, grid, st, true);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st) {
+    if (g.ntiles == 0) return hipSuccess;
+    dim3 grid(xcd_grid(g.ntiles));
+    const bool ex = e.extra != nullptr;
+#define MDT_F(LM) do { if (ex) hipLaunchKernelGGL((k_force<LM, true>), grid, dim3(kBlock), 0, st, g, e, f); \
+                       else hipLaunchKernelGGL((k_force<LM, false>), grid, dim3(kBlock), 0, st, g, e, f); } while (0)
+    switch (e.lmask) {
+        case kL6: MDT_F(kL6); break;
+        case kL4: MDT_F(kL4); break;
+        case kL4 | kL6: MDT_F(kL4 | kL6); break;
+        case kL3: MDT_F(kL3); break;
+        case kL3 | kL4: MDT_F(kL3 | kL4); break;
+        case kL3 | kL6: MDT_F(kL3 | kL6); break;
+        case kL3 | kL4 | kL6: MDT_F(kL3 | kL4 | kL6); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef MDT_F
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const ReduceDev& r, hipStream_t st) {
+    if (r.n_walkers == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce, dim3(r.n_walkers), dim3(kBlock), 0, st, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    int64_t b = (n + 255) / 256; if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)b), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_f(float* p, int64_t n, float v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    int64_t b = (n + 255) / 256; if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_fill_f, dim3((unsigned)b), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_bias_eval(const ReduceDev& r, int w, int64_t np, const double* s, double* V,
+                            double* dV, hipStream_t st) {
+    if (np == 0) return hipSuccess;
+    int64_t b = np < 1024 ? np : 1024;
+    hipLaunchKernelGGL(k_bias_eval, dim3((unsigned)b), dim3(kBlock), 0, st, r, w, np, s, V, dV);
+    return hipGetLastError();
+}
+
+hipError_t launch_bias_add(const ReduceDev& r, int w, int64_t nh, const double* s, const double* h,
+                           double sigma, hipStream_t st) {
+    if (nh == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bias_add, dim3(1), dim3(kBlock), 0, st, r, w, nh, s, h, sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
+    dim3 grid((s.nk + kSkTile - 1) / kSkTile, s.nchunk);
+    hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sk_sum, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t st) {
+    // grid.y = walkers, grid.x covers the largest walker
+    dim3 grid((s.max_atoms + kBlock - 1) / kBlock, s.n_walkers);
+    hipLaunchKernelGGL(k_sk_force, grid, dim3(kBlock), 0, st, g, s, f);
+    return hipGetLastError();
+}
+
+}  // namespace mdt

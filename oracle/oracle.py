@@ -1,0 +1,252 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the cpu_baseline leg of bench.py, never by the shipped library.  Every function
+here calls the C restatement; numpy is used only to marshal arrays.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+_L = None
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-B", "-C", HERE], check=True, capture_output=True)
+
+
+def lib():
+    global _L
+    if _L is None:
+        build()
+        _L = C.CDLL(LIB)
+        P = C.c_void_p
+        I, D = C.c_int, C.c_double
+        for name, res, args in [
+            ("orc_ylm", None, [I, D, D, D, P]),
+            ("orc_ylm_poly", None, [I, D, D, D, P]),
+            ("orc_ylm_grad", None, [I, D, D, D, P]),
+            ("orc_ylm_coef", None, [P]),
+            ("orc_select", None, [I, P, P, P, P, D, I, D, P, P, P, P, P]),
+            ("orc_steinhardt", None, [I, I, I, P, P, P, P]),
+            ("orc_kahan_mean", D, [I, P, P, D]),
+            ("orc_ld_local", None, [I, I, I, I, P, P, P, P, P, P]),
+            ("orc_grad_from_coeff", None, [I, I, I, I, P, P, P, P, P, P]),
+            ("orc_q_coeff", None, [I, I, P, P, D, D, P]),
+            ("orc_ld_coeff", None, [I, I, I, I, P, P, P, P, P, D, D, P]),
+            ("orc_sk", D, [I, P, I, P, P, P]),
+            ("orc_sk_grad", None, [I, P, I, P, P, D, P]),
+            ("orc_grid_radius", I, [D, D, D, I]),
+            ("orc_grid_deposit", None, [I, P, P, P, P, P, D, D]),
+            ("orc_grid_eval", D, [I, P, P, P, P, P, P]),
+            ("orc_wt_height", D, [I, P, P, P, P, P, D, D, D, D, I]),
+            ("orc_hills_sum", None, [I, I, P, P, P, I, P, P, P]),
+            ("orc_mlp_forward", None, [I, I, P, P, P, P, P, D, I, P, P, P]),
+            ("orc_mlp_loss_grad", D, [I, I, P, P, P, P, P, D, I, P, P, P, P, P, P, P, P]),
+        ]:
+            f = getattr(_L, name)
+            f.restype = res
+            f.argtypes = args
+    return _L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- Y_lm
+def ylm(l, d):
+    out = np.zeros(2 * (l + 1))
+    lib().orc_ylm(l, float(d[0]), float(d[1]), float(d[2]), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def ylm_poly(l, d):
+    out = np.zeros(2 * (l + 1))
+    lib().orc_ylm_poly(l, float(d[0]), float(d[1]), float(d[2]), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def ylm_grad(l, d):
+    """(l+1, 3) complex: d Y_lm / d r_a."""
+    out = np.zeros(6 * (l + 1))
+    lib().orc_ylm_grad(l, float(d[0]), float(d[1]), float(d[2]), _p(out))
+    o = out.reshape(l + 1, 3, 2)
+    return o[..., 0] + 1j * o[..., 1]
+
+
+# ---------------------------------------------------------------- selection / Steinhardt
+class Env:
+    """Selection of every local atom (centers = all locals, or a group)."""
+
+    def __init__(self, x_all, off, idx, owner, n_local, rc, N, group=None):
+        self.x = _f64(x_all).reshape(-1, 3)
+        self.off, self.idx, self.owner = _i32(off), _i32(idx), _i32(owner)
+        self.n_local, self.rc, self.N = n_local, rc, N
+        if group is None:
+            centers = np.arange(n_local, dtype=np.int32)
+        else:
+            centers = np.nonzero(np.asarray(group))[0].astype(np.int32)
+        self.centers = centers
+        nc = centers.size
+        # per-center CSR (the oracle takes lists in center order)
+        if group is None:
+            offs, cidx = self.off, self.idx
+        else:
+            offs = np.zeros(nc + 1, np.int32)
+            lst = []
+            for k, i in enumerate(centers):
+                seg = self.idx[self.off[i]:self.off[i + 1]]
+                lst.append(seg)
+                offs[k + 1] = offs[k] + seg.size
+            cidx = np.concatenate(lst).astype(np.int32) if lst else np.zeros(0, np.int32)
+        self.sel_idx = np.zeros((nc, N), np.int32)
+        self.sel_pos = np.zeros((nc, N), np.int32)
+        self.sel_d = np.zeros((nc, N, 4))
+        self.cnt = np.zeros(nc, np.int32)
+        self.nsel = np.zeros(nc, np.int32)
+        lo = self.x.min(axis=0)
+        L = np.ptp(self.x[:n_local], axis=0)
+        box_sum2 = float(np.sum(L) ** 2)
+        lib().orc_select(nc, _p(centers), _p(offs), _p(cidx), _p(self.x), rc * rc, N, box_sum2,
+                         _p(self.sel_idx), _p(self.sel_pos), _p(self.sel_d), _p(self.cnt), _p(self.nsel))
+        self._full = group is None
+
+    def owner_of(self, j):
+        j = np.asarray(j)
+        return np.where(j < self.n_local, j, self.owner[np.clip(j - self.n_local, 0, None)])
+
+    def steinhardt(self, l):
+        nc = self.centers.size
+        q = np.zeros(nc * 2 * (l + 1))
+        ql = np.zeros(nc)
+        lib().orc_steinhardt(l, nc, self.N, _p(self.sel_d), _p(self.nsel), _p(q), _p(ql))
+        return q.reshape(nc, 2 * (l + 1)), ql
+
+    def q_ave(self, l):
+        _, ql = self.steinhardt(l)
+        return lib().orc_kahan_mean(ql.size, _p(ql), None, float(ql.size))
+
+    def ld(self, l):
+        assert self._full, "local Q oracle runs with centers == all local atoms"
+        q, _ = self.steinhardt(l)
+        qb = np.zeros_like(q)
+        lq = np.zeros(self.n_local)
+        lib().orc_ld_local(l, self.n_local, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
+                           _p(self.nsel), _p(q), _p(qb), _p(lq))
+        return qb, lq
+
+    def dQ_dx(self, l, w=1.0):
+        """w * d Q_l.AVE / dx for local atoms (n_local, 3)."""
+        assert self._full
+        q, ql = self.steinhardt(l)
+        K = np.zeros_like(q)
+        lib().orc_q_coeff(l, self.n_local, _p(q), _p(ql), float(self.n_local), w, _p(K))
+        out = np.zeros((self.n_local, 3))
+        lib().orc_grad_from_coeff(l, self.n_local, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
+                                  _p(self.sel_d), _p(self.nsel), _p(K), _p(out))
+        return out
+
+    def dLQ_dx(self, l, w=1.0):
+        qb, lq = self.ld(l)
+        K = np.zeros_like(qb)
+        lib().orc_ld_coeff(l, self.n_local, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
+                           _p(self.nsel), _p(qb), _p(lq), float(self.n_local), w, _p(K))
+        out = np.zeros((self.n_local, 3))
+        lib().orc_grad_from_coeff(l, self.n_local, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
+                                  _p(self.sel_d), _p(self.nsel), _p(K), _p(out))
+        return out
+
+
+def kahan_mean(v):
+    v = _f64(v)
+    return lib().orc_kahan_mean(v.size, _p(v), None, float(v.size))
+
+
+# ---------------------------------------------------------------- structure factor
+def sk(x, k):
+    x, k = _f64(x).reshape(-1, 3), _f64(k).reshape(-1, 3)
+    A = np.zeros(2 * k.shape[0])
+    S = np.zeros(k.shape[0])
+    v = lib().orc_sk(x.shape[0], _p(x), k.shape[0], _p(k), _p(A), _p(S))
+    return v, A[0::2] + 1j * A[1::2], S
+
+
+def sk_grad(x, k, w=1.0):
+    x, k = _f64(x).reshape(-1, 3), _f64(k).reshape(-1, 3)
+    _, A, _ = sk(x, k)
+    Ai = np.zeros(2 * k.shape[0])
+    Ai[0::2], Ai[1::2] = A.real, A.imag
+    out = np.zeros_like(x)
+    lib().orc_sk_grad(x.shape[0], _p(x), k.shape[0], _p(k), _p(Ai), w, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- grid / hills
+class Grid:
+    def __init__(self, lo, hi, nb):
+        self.lo, self.hi = _f64(lo), _f64(hi)
+        self.nb = _i32(nb)
+        self.D = self.lo.size
+        self.g = np.zeros(int(np.prod(self.nb)))
+
+    def deposit(self, s, w, sigma):
+        s = _f64(s)
+        lib().orc_grid_deposit(self.D, _p(self.lo), _p(self.hi), _p(self.nb), _p(self.g), _p(s), w, sigma)
+
+    def eval(self, s):
+        s = _f64(s)
+        dV = np.zeros(self.D)
+        V = lib().orc_grid_eval(self.D, _p(self.lo), _p(self.hi), _p(self.nb), _p(self.g), _p(s), _p(dV))
+        return V, dV
+
+    def wt_height(self, s, h0, biasf, T, kB, wt):
+        s = _f64(s)
+        return lib().orc_wt_height(self.D, _p(self.lo), _p(self.hi), _p(self.nb), _p(self.g), _p(s),
+                                   h0, biasf, T, kB, int(wt))
+
+
+def hills_sum(centers, heights, sigma, s):
+    c = _f64(centers)
+    D = c.shape[1]
+    h, sg, s = _f64(heights), _f64(sigma), _f64(np.atleast_2d(s))
+    V = np.zeros(s.shape[0])
+    dV = np.zeros_like(s)
+    lib().orc_hills_sum(D, h.size, _p(c), _p(h), _p(sg), s.shape[0], _p(s), _p(V), _p(dV))
+    return V, dV
+
+
+# ---------------------------------------------------------------- MLP
+def mlp_forward(params, s):
+    W1, b1, W2, b2, W3, b3 = [_f64(p) for p in params]
+    s = _f64(np.atleast_2d(s))
+    H, Din = W1.shape
+    V = np.zeros(s.shape[0])
+    dV = np.zeros_like(s)
+    lib().orc_mlp_forward(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(b3), s.shape[0], _p(s),
+                          _p(V), _p(dV))
+    return V, dV
+
+
+def mlp_loss_grad(params, s, y):
+    W1, b1, W2, b2, W3, b3 = [_f64(p) for p in params]
+    s, y = _f64(np.atleast_2d(s)), _f64(y)
+    H, Din = W1.shape
+    g = [np.zeros_like(W1), np.zeros_like(b1), np.zeros_like(W2), np.zeros_like(b2), np.zeros_like(W3)]
+    gb3 = np.zeros(1)
+    loss = lib().orc_mlp_loss_grad(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(b3), s.shape[0],
+                                   _p(s), _p(y), _p(g[0]), _p(g[1]), _p(g[2]), _p(g[3]), _p(g[4]), _p(gb3))
+    return loss, g + [gb3[0]]

@@ -1,0 +1,212 @@
+"""GPU parity of the HIP path against the FP64 oracle (run on an MI355X with -m gpu).
+
+Bars (written in each test):
+  selection (indices and slot order)  bit-exact
+  per-atom q_l and CV values          1e-5 relative
+  forces                              1e-4 relative (max |F - F_ref| / max |F_ref|)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import Mover, fcc_system, liquid_system, q46_fix, rel_force_err
+
+pytestmark = pytest.mark.gpu
+
+CV_TOL = 1e-5
+F_TOL = 1e-4
+
+
+def _ctx(mdt, system):
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, system)
+    return ctx
+
+
+def _fix_q(mdt, ctx, N=18, rc=3.9, extra=""):
+    return mdt.Fix(ctx, q46_fix(extra=extra, N=N, rc=rc))
+
+
+@pytest.mark.parametrize("case", ["fcc_jitter", "fcc_perfect", "liquid_capped"])
+def test_selection_bit_exact(mdt, orc, case):
+    if case == "fcc_jitter":
+        s, N, rc = fcc_system(mdt, 10, sigma=0.1, seed=11), 18, 3.9
+    elif case == "fcc_perfect":
+        s, N, rc = fcc_system(mdt, 6, sigma=0.0), 8, 3.9          # ties at the cap
+    else:
+        s, N, rc = liquid_system(mdt, 3000, density=0.05, seed=5), 12, 3.9   # cap binds often
+    print(f"[selection] case={case} n={s.n_local} N={N}", flush=True)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    ctx = _ctx(mdt, s)
+    fix = _fix_q(mdt, ctx, N=N, rc=rc)
+    fix.post_force(0)
+    sel, cnt, ns = fix.selection("Q6", n, N)
+    env = orc.Env(x, off, idx, own, n, rc, N)
+    assert np.array_equal(cnt, env.cnt)
+    assert np.array_equal(ns, env.nsel)
+    assert np.array_equal(sel, env.sel_idx), "slot order differs"
+    # the fused hot path keeps the same set
+    assert np.array_equal(fix.extract("Q6.nsel", n).astype(np.int32), env.nsel)
+    assert np.array_equal(fix.extract("Q6.count_rc", n).astype(np.int32), env.cnt)
+    if case == "liquid_capped":
+        assert (env.cnt > N).sum() > 100
+
+
+def test_q_values_and_forces_c1(mdt, orc):
+    """4,000-atom jittered FCC (the CPU reference configuration), Q6+Q4 on v1+v2."""
+    s = fcc_system(mdt, 10, sigma=0.1, seed=1)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    print(f"[q/forces] n={n}", flush=True)
+    ctx = _ctx(mdt, s)
+    fix = _fix_q(mdt, ctx)
+    # put hills around the current CV so dV/ds is nonzero
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    s0 = env.q_ave(6) + env.q_ave(4)
+    fix.init()
+    fix.add_hills(np.array([[s0 - 0.004], [s0 + 0.011]]), np.array([0.005, 0.003]), 0.01)
+    f = np.zeros((n, 3))
+    fix.post_force(1, f)     # step 1: no deposit with PACE 10
+    for l, name in ((6, "Q6"), (4, "Q4")):
+        q_ref = env.steinhardt(l)[1]
+        q = fix.extract(f"{name}.stein_q", n)
+        assert np.abs(q - q_ref).max() <= CV_TOL * np.abs(q_ref).max()
+        assert abs(fix.cv(f"{name}.AVE") - env.q_ave(l)) <= CV_TOL * env.q_ave(l)
+    w6, w4 = fix.cv_weight("Q6.AVE"), fix.cv_weight("Q4.AVE")
+    assert w6 != 0.0 and w6 == w4
+    F_ref = -(env.dQ_dx(6, w6) + env.dQ_dx(4, w4))
+    err = rel_force_err(f, F_ref)
+    print(f"force rel err {err:.3e}")
+    assert err <= F_TOL
+    # translation invariance of the exact gradient
+    assert np.abs(f.sum(axis=0)).max() <= 1e-4 * np.abs(f).max() * np.sqrt(n)
+
+
+def test_forces_when_cap_binds(mdt, orc):
+    """Asymmetric selections (N cap binds): reverse pairs must still be exact."""
+    s = liquid_system(mdt, 3000, density=0.05, seed=9)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, q46_fix(N=12, lo=0.0, hi=1.5, nb=1500))
+    env = orc.Env(x, off, idx, own, n, 3.9, 12)
+    s0 = env.q_ave(6) + env.q_ave(4)
+    fix.init()
+    fix.add_hills(np.array([[s0 + 0.006]]), np.array([0.005]), 0.01)
+    f = np.zeros((n, 3))
+    fix.post_force(1, f)
+    w = fix.cv_weight("Q6.AVE")
+    F_ref = -(env.dQ_dx(6, w) + env.dQ_dx(4, w))
+    assert rel_force_err(f, F_ref) <= F_TOL
+
+
+def test_ld_local_q(mdt, orc):
+    """Lechner-Dellago local Q6 value and forces (biased alone)."""
+    s = fcc_system(mdt, 8, sigma=0.3, seed=4)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE "
+                       "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                       "CAL NAME LQ6 STEINH LD 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                       "CV_dim 1 SYMBOL v1 LQ6.AVE DIM 1 0 1 1000 v1 WT 0")
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    qb, lq = env.ld(6)
+    fix.init()
+    fix.add_hills(np.array([[lq.mean() - 0.005]]), np.array([0.005]), 0.01)
+    f = np.zeros((n, 3))
+    fix.post_force(1, f)
+    assert abs(fix.cv("LQ6.AVE") - lq.mean()) <= CV_TOL * lq.mean()
+    lq_gpu = fix.extract("LQ6.local_q", n)
+    assert np.abs(lq_gpu - lq).max() <= CV_TOL * lq.max()
+    w = fix.cv_weight("LQ6.AVE")
+    assert w != 0.0
+    F_ref = -env.dLQ_dx(6, w)
+    assert rel_force_err(f, F_ref) <= F_TOL
+
+
+def test_lockstep_grid_c1(mdt, orc):
+    """Lock-step replay: moving atoms, hills every 10 steps (PACE 10), grid, V and forces each step."""
+    s = fcc_system(mdt, 10, sigma=0.1, seed=21)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE "
+                       "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                       "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0")
+    mv = Mover(x, n, own)
+    rng = np.random.default_rng(0)
+    g = orc.Grid([0.0], [1.0], [1000])
+    xl = x[:n].copy()
+    worst = 0.0
+    for step in range(0, 60):
+        xa = mv.set_local(xl)
+        ctx.update_positions(0, xa)
+        f = np.zeros((n, 3))
+        fix.post_force(step, f)
+        env = orc.Env(xa, off, idx, own, n, 3.9, 18)
+        q6 = env.q_ave(6)
+        assert abs(fix.cv("Q6.AVE") - q6) <= CV_TOL * q6
+        if step % 10 == 0:
+            g.deposit([q6], g.wt_height([q6], 0.005, 10.0, 300.0, 8.617343e-5, 0), 0.01)
+        V, dV = g.eval([q6])
+        assert abs(fix.scalar() - V) <= 1e-5 * max(abs(V), 1e-3)
+        F_ref = -env.dQ_dx(6, dV[0])
+        if np.abs(F_ref).max() > 0:
+            e = rel_force_err(f, F_ref)
+            worst = max(worst, e)
+            assert e <= F_TOL, (step, e)
+        xl = xl + rng.normal(scale=0.005, size=xl.shape)
+    grid = fix.grid(1000)
+    assert np.abs(grid - g.g).max() <= 1e-6 * max(np.abs(g.g).max(), 1e-12)
+    print(f"lock-step worst force rel err {worst:.3e}")
+
+
+def test_graph_run_matches_post_force(mdt, orc):
+    s = fcc_system(mdt, 8, sigma=0.2, seed=2)
+    n = s.n_local
+    res = []
+    for mode in ("post", "graph"):
+        ctx = _ctx(mdt, s)
+        fix = mdt.Fix(ctx, q46_fix(pace=3, wt=1))
+        fix.zero_forces()
+        if mode == "post":
+            for st in range(12):
+                fix.post_force(st)
+        else:
+            fix.run(0, 12)
+            fix.sync()
+        res.append((fix.forces(n), fix.scalar(), fix.cv("Q6.AVE"), fix.hills(1)))
+        fix.close()
+        ctx.close()
+    (f0, v0, q0, h0), (f1, v1, q1, h1) = res
+    assert np.array_equal(f0, f1)
+    assert v0 == v1 and q0 == q1
+    assert np.array_equal(h0, h1) and h0.shape[0] == 4
+
+
+def test_hills_file_and_restart(mdt, orc, tmp_path):
+    s = fcc_system(mdt, 6, sigma=0.2, seed=3)
+    n = s.n_local
+    path = str(tmp_path / "HILLS")
+    args = (f"GAUSSIAN 0.01 0.005 10 PACE 2 HILLS_FILE {path} "
+            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CV_dim 1 SIMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 1 TEMP 300")
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, args + " METAD_RESTART 0")
+    for st in range(10):
+        fix.post_force(st)
+    g1 = fix.grid(1000)
+    fix.close()
+    lines = [l for l in open(path) if not l.startswith("#")]
+    assert len(lines) == 5
+    first = lines[0].split()
+    assert first[0] == "0" and abs(float(first[2]) - 0.005) < 1e-15 and float(first[3]) == 0.01
+    # restart replays the file into a fresh grid
+    ctx2 = _ctx(mdt, s)
+    fix2 = mdt.Fix(ctx2, args + " METAD_RESTART 1")
+    fix2.init()
+    g2 = fix2.grid(1000)
+    assert np.abs(g1 - g2).max() <= 1e-12 * np.abs(g1).max()

@@ -1,0 +1,146 @@
+"""CPU tests of the host side: system builder, neighbour lists, C-ABI exports, fix parser."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import fcc_system, liquid_system
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(mdt):
+    hdr = open(os.path.join(ROOT, "include", "mdt.h")).read()
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(mdt_\w+)\s*\(", hdr, re.M))
+    assert len(names) > 40
+    L = ctypes.CDLL(mdt.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert mdt.lib().mdt_version().decode().startswith("mdt")
+
+
+def test_fcc_generator_counts_and_box(mdt):
+    s = mdt.System().add_fcc(3, 4, 5, 5.26)
+    lo, hi = s.box()
+    assert np.allclose(hi - lo, [3 * 5.26, 4 * 5.26, 5 * 5.26])
+    assert s.n_local == 4 * 60
+    x = s.arrays()[0]
+    assert x.min() >= 0 and np.all(x < hi)
+
+
+def test_jitter_is_seeded_and_split(mdt):
+    a = mdt.System().add_fcc(6, 6, 6, 5.26, 0.6, 0.1, 3 * 5.26, seed=4).arrays()[0]
+    b = mdt.System().add_fcc(6, 6, 6, 5.26, 0.6, 0.1, 3 * 5.26, seed=4).arrays()[0]
+    p = mdt.System().add_fcc(6, 6, 6, 5.26).arrays()[0]
+    assert np.array_equal(a, b)
+    L = 6 * 5.26
+    d = (a - p + L / 2) % L - L / 2
+    low = p[:, 2] < 3 * 5.26
+    assert 0.5 < d[low].std() < 0.7 and 0.08 < d[~low].std() < 0.12
+
+
+@pytest.mark.parametrize("kind", ["fcc", "liquid"])
+def test_neighbour_list_bruteforce(mdt, kind):
+    s = fcc_system(mdt, 4, sigma=0.3, seed=2) if kind == "fcc" else liquid_system(mdt, 800, seed=1)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    lo, hi = s.box()
+    L = hi - lo
+    rg = 5.9
+    # ghosts are exact images of their owners and lie within rg of the box
+    shift = x[n:] - x[own]
+    assert np.allclose(shift / L - np.round(shift / L), 0, atol=1e-9)
+    assert np.all(x[n:] >= lo - rg) and np.all(x[n:] < hi + rg)
+    # every local pair within rg under minimum image appears exactly once per center
+    xl = x[:n]
+    for i in range(0, n, 7):
+        d = xl - xl[i]
+        d -= L * np.round(d / L)
+        r2 = np.sum(d * d, axis=1)
+        expect = sorted(j for j in np.nonzero(r2 <= rg * rg)[0] if j != i)
+        got = idx[off[i]:off[i + 1]]
+        owners = np.where(got < n, got, own[np.clip(got - n, 0, None)])
+        assert sorted(owners.tolist()) == expect
+        dd = x[got] - x[i]
+        assert np.all(np.sum(dd * dd, axis=1) <= rg * rg)
+
+
+def test_box_too_small_is_rejected(mdt):
+    s = mdt.System().add_fcc(2, 2, 2, 5.26)
+    with pytest.raises(mdt.MdtError):
+        s.build(5.9)
+
+
+GOOD = ("GAUSSIAN 0.01 0.005 10 PACE 10 "
+        "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+        "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
+        "CV_dim 1 SIMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE DIM 1 0 1 1000 \"v1+v2\" METAD_RESTART 0 WT 0")
+
+
+def test_fix_parser_accepts_reference_syntax(mdt):
+    ctx = mdt.Context(-1, 1)
+    fix = mdt.Fix(ctx, GOOD)
+    fix.close()
+    # README example, RECORD block included
+    mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10.0 PACE 10 RECORD FILE_NAME rec.log REC_PACE 1 "
+                 "CAL NAME Q6 STEINH Q 6 metad_group cutoff_r 3.9 cutoff_Natoms 18 "
+                 "CAL NAME Q4 STEINH Q 4 metad_group cutoff_r 3.9 cutoff_Natoms 18 "
+                 "CV_dim 1 SIMBOL v1 Q6.AVE SIMBOL v2 Q4.AVE DIM 1 0 1 1000 v1+v2 METAD_RESTART 1 WT 0").close()
+    # expressions with functions and constants compile
+    mdt.Fix(ctx, GOOD.replace('"v1+v2"', '"sqrt(v1^2+v2^2)*cos(pi/4)+max(v1,v2)-exp(-v1)"')).close()
+
+
+@pytest.mark.parametrize("bad,code", [
+    ("GAUSSIAN 0.01 0.005", -22),
+    ("PACE", -22),
+    ("FOO 1", -22),
+    ("CAL NAME Q6 STEINH Q 5 all", -22),
+    ("CAL NAME Q6 STEINH Q 8 all", -95),
+    ("CAL NAME Q6 STEINH L 6 all", -95),
+    ("CAL NAME Q6 STEINH Q 6 all SW_FUNC_r sw", -95),
+    ("CAL NAME Q6 DISTANCE 1 2", -95),
+    ("CAL NAME Q6 NOPE 1 2", -22),
+])
+def test_fix_parser_errors(mdt, bad, code):
+    ctx = mdt.Context(-1, 1)
+    with pytest.raises(mdt.MdtError) as e:
+        mdt.Fix(ctx, bad)
+    assert e.value.code == code
+
+
+def test_fix_semantic_errors(mdt):
+    ctx = mdt.Context(-1, 1)
+    base = "GAUSSIAN 0.01 0.005 10 CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+    for tail, frag in [
+        ("CV_dim 1 SYMBOL v1 Q7.AVE DIM 1 0 1 100 v1", "not defined"),
+        ("CV_dim 1 SYMBOL v1 Q6.MEAN DIM 1 0 1 100 v1", "find the func"),
+        ("CV_dim 1 SYMBOL v1 Q6.AVE", "DIM 1 is missing"),
+        ("CV_dim 2 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1", "DIM 2 is missing"),
+        ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 2 0 1 100 v1", "out of range"),
+        ("CV_dim 3 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 DIM 2 0 1 100 v1 DIM 3 0 1 100 v1", "grid supports"),
+        ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 Gaussian_Hill_type 1", "sparse hash"),
+        ("CAL NAME Q6 STEINH Q 4 all CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1", "duplicated"),
+    ]:
+        with pytest.raises(mdt.MdtError) as e:
+            mdt.Fix(ctx, base + tail)
+        assert frag in str(e.value), (tail, str(e.value))
+
+
+def test_parse_only_context_refuses_steps(mdt):
+    ctx = mdt.Context(-1, 1)
+    fix = mdt.Fix(ctx, GOOD)
+    with pytest.raises(mdt.MdtError) as e:
+        fix.init()
+    assert "parse-only" in str(e.value)
+
+
+@pytest.mark.parametrize("expr,frag", [("v1+v9", "unknown symbol"), ("v1+", "unexpected end"),
+                                       ("(v1", "missing ')'"), ("foo(v1)", "unknown function"),
+                                       ("v1 v2", "trailing")])
+def test_expression_errors(mdt, expr, frag):
+    ctx = mdt.Context(-1, 1)
+    with pytest.raises(mdt.MdtError) as e:
+        mdt.Fix(ctx, GOOD.replace('"v1+v2"', f'"{expr}"'))
+    assert frag in str(e.value)

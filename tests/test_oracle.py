@@ -1,0 +1,384 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each oracle function is checked against something other than its own formula:
+an independent library (scipy), a closed form, a hand-derived known answer,
+finite differences or an invariant.
+"""
+import numpy as np
+import pytest
+from scipy.special import sph_harm_y
+
+from helpers import fcc_system
+
+
+# ------------------------------------------------------------------ Y_lm
+@pytest.mark.parametrize("l", [3, 4, 6])
+def test_ylm_matches_scipy(orc, l):
+    rng = np.random.default_rng(100 + l)
+    for _ in range(50):
+        d = rng.normal(size=3) * rng.uniform(0.5, 4.0)
+        r = np.linalg.norm(d)
+        th, ph = np.arccos(d[2] / r), np.arctan2(d[1], d[0])
+        ref = np.array([sph_harm_y(l, m, th, ph) for m in range(l + 1)])
+        assert np.abs(orc.ylm(l, d) - ref).max() < 1e-13
+        # the polynomial form c_lm(u) (x + i y)^m used by the kernels is the same function
+        assert np.abs(orc.ylm_poly(l, d) - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("l", [3, 4, 6])
+def test_ylm_sum_rule(orc, l):
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        Y = orc.ylm(l, rng.normal(size=3))
+        tot = abs(Y[0]) ** 2 + 2 * np.sum(np.abs(Y[1:]) ** 2)
+        assert abs(tot - (2 * l + 1) / (4 * np.pi)) < 1e-13
+
+
+@pytest.mark.parametrize("l", [3, 4, 6])
+def test_ylm_gradient_fd(orc, l):
+    rng = np.random.default_rng(11)
+    h = 1e-6
+    for _ in range(10):
+        d = rng.normal(size=3) * 3.0
+        g = orc.ylm_grad(l, d)
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            fd = (orc.ylm(l, d + e) - orc.ylm(l, d - e)) / (2 * h)
+            assert np.abs(fd - g[:, a]).max() < 1e-7
+
+
+# ------------------------------------------------------------------ q_l known answers
+def _shell_q(orc, l, vecs):
+    N = len(vecs)
+    sel_d = np.zeros((1, N, 4))
+    sel_d[0, :, :3] = vecs
+    sel_d[0, :, 3] = np.sum(np.asarray(vecs) ** 2, axis=1)
+    q = np.zeros(2 * (l + 1))
+    ql = np.zeros(1)
+    orc.lib().orc_steinhardt(l, 1, N, sel_d.ctypes.data, np.array([N], np.int32).ctypes.data,
+                             q.ctypes.data, ql.ctypes.data)
+    return ql[0]
+
+
+def _fcc12():
+    v = []
+    for a in (-1, 1):
+        for b in (-1, 1):
+            v += [(a, b, 0), (a, 0, b), (0, a, b)]
+    return np.array(v, float) / 2
+
+
+def _bcc8():
+    return np.array([(a, b, c) for a in (-1, 1) for b in (-1, 1) for c in (-1, 1)], float) / 2
+
+
+def _sc6():
+    return np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)], float)
+
+
+def _hcp12():
+    a = 1.0
+    c = np.sqrt(8.0 / 3.0) * a
+    v = [(a * np.cos(k * np.pi / 3), a * np.sin(k * np.pi / 3), 0.0) for k in range(6)]
+    for z in (c / 2, -c / 2):
+        for k in range(3):
+            ang = np.pi / 6 + k * 2 * np.pi / 3
+            v.append((a / np.sqrt(3) * np.cos(ang), a / np.sqrt(3) * np.sin(ang), z))
+    return np.array(v)
+
+
+@pytest.mark.parametrize("shell,q4,q6", [
+    (_fcc12, 0.190941, 0.574524),
+    (_bcc8, 0.509175, 0.628539),
+    (_sc6, 0.763763, 0.353553),
+    (_hcp12, 0.097222, 0.484762),
+])
+def test_q_known_answers(orc, shell, q4, q6):
+    v = shell()
+    assert abs(_shell_q(orc, 4, v) - q4) < 1e-6
+    assert abs(_shell_q(orc, 6, v) - q6) < 1e-6
+
+
+def test_perfect_fcc_system(mdt, orc):
+    s = fcc_system(mdt, 5, sigma=0.0)
+    x, off, idx, own = s.arrays()
+    env = orc.Env(x, off, idx, own, s.n_local, 3.9, 18)
+    assert np.all(env.nsel == 12)
+    assert abs(env.q_ave(6) - 0.574524) < 1e-6 and abs(env.q_ave(4) - 0.190941) < 1e-6
+    # local Q of a perfect lattice equals Q (all neighbours identical)
+    _, lq = env.ld(6)
+    assert np.abs(lq - 0.5745242597140697).max() < 1e-12
+
+
+# ------------------------------------------------------------------ selection
+def _select(orc, r2_list, N, rc=10.0):
+    """One center at the origin with neighbours on the x axis at the given r^2, in list order."""
+    n = len(r2_list)
+    x = np.zeros((n + 1, 3))
+    x[1:, 0] = np.sqrt(np.asarray(r2_list, float))
+    off = np.array([0, n], np.int32)
+    idx = np.arange(1, n + 1, dtype=np.int32)
+    sel_idx = np.zeros(N, np.int32)
+    sel_pos = np.zeros(N, np.int32)
+    sel_d = np.zeros(N * 4)
+    cnt = np.zeros(1, np.int32)
+    ns = np.zeros(1, np.int32)
+    orc.lib().orc_select(1, np.array([0], np.int32).ctypes.data, off.ctypes.data, idx.ctypes.data, x.ctypes.data,
+                         rc * rc, N, 1e6, sel_idx.ctypes.data, sel_pos.ctypes.data, sel_d.ctypes.data,
+                         cnt.ctypes.data, ns.ctypes.data)
+    return sel_pos.tolist(), cnt[0], ns[0]
+
+
+def test_selection_known_answers(orc):
+    # below the cap with distinct r^2: ascending r^2, everything kept
+    pos, cnt, ns = _select(orc, [4.0, 1.0, 9.0, 2.25], 6)
+    assert pos == [1, 3, 0, 2, -1, -1] and cnt == 4 and ns == 4
+    # above the cap, distinct: the N smallest
+    pos, cnt, ns = _select(orc, [4.0, 1.0, 9.0, 2.25, 0.25], 3)
+    assert pos == [4, 1, 3] and cnt == 5 and ns == 3
+    # ties, worked by hand from the strict '>' carry rule of the reference:
+    # a(1), b(1), c(0.25): after a,b -> [a,b]; c displaces a, a passes b (1 > 1 false)
+    # and lands behind it -> [c, b, a].  The equal run rotates.
+    pos, _, _ = _select(orc, [1.0, 1.0, 0.25], 3)
+    assert pos == [2, 1, 0]
+    # same with a cap of 2: the rotated 'a' falls off the end
+    pos, cnt, ns = _select(orc, [1.0, 1.0, 0.25], 2)
+    assert pos == [2, 1] and cnt == 3 and ns == 2
+    # an entry equal to the current maximum of a full buffer is dropped
+    pos, _, _ = _select(orc, [1.0, 2.0, 2.0], 2)
+    assert pos == [0, 1]
+    # cutoff and self-distance filters
+    pos, cnt, _ = _select(orc, [0.0, 1.0, 100.0 + 1e-9], 4)
+    assert cnt == 1 and pos[0] == 1
+
+
+def test_selection_bruteforce_no_ties(orc):
+    """Without ties the kept set is the N smallest r^2 in ascending order."""
+    rng = np.random.default_rng(3)
+    for _ in range(30):
+        r2 = rng.uniform(0.5, 150.0, size=rng.integers(1, 40))
+        N = int(rng.integers(1, 20))
+        pos, cnt, ns = _select(orc, r2, N)
+        keep = [k for k in range(len(r2)) if r2[k] <= 100.0]
+        order = sorted(keep, key=lambda k: r2[k])[:N]
+        assert cnt == len(keep) and ns == min(len(keep), N)
+        assert pos[:ns] == order
+
+
+# ------------------------------------------------------------------ gradients
+def _move_all(x_all, n, owner, k, a, h):
+    y = x_all.copy()
+    y[k, a] += h
+    gh = np.nonzero(owner == k)[0] + n
+    y[gh, a] += h
+    return y
+
+
+@pytest.mark.parametrize("kind", ["Q6", "Q4", "LQ6", "Q3"])
+def test_gradient_finite_differences(mdt, orc, kind):
+    s = fcc_system(mdt, 3, sigma=0.25, seed=5)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    l = int(kind[-1])
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    g = env.dLQ_dx(l) if kind.startswith("LQ") else env.dQ_dx(l)
+
+    def cv(y):
+        e = orc.Env(y, off, idx, own, n, 3.9, 18)
+        return e.ld(l)[1].mean() if kind.startswith("LQ") else e.q_ave(l)
+
+    h = 1e-6
+    for k in (0, 7, 50, 101):
+        for a in range(3):
+            fd = (cv(_move_all(x, n, own, k, a, h)) - cv(_move_all(x, n, own, k, a, -h))) / (2 * h)
+            assert abs(fd - g[k, a]) <= 1e-6 * np.abs(g).max() + 1e-9
+    # translation invariance
+    assert np.abs(g.sum(axis=0)).max() < 1e-12 * n
+
+
+def test_gradient_with_binding_cap(mdt, orc):
+    """N smaller than the shell: asymmetric selections, exact gradient still matches FD."""
+    s = fcc_system(mdt, 3, sigma=0.25, seed=8)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    env = orc.Env(x, off, idx, own, n, 3.9, 6)
+    assert (env.cnt > 6).sum() > n // 2
+    g = env.dQ_dx(6)
+    h = 1e-6
+    for k in (3, 40, 77):
+        for a in range(3):
+            p = orc.Env(_move_all(x, n, own, k, a, h), off, idx, own, n, 3.9, 6)
+            m = orc.Env(_move_all(x, n, own, k, a, -h), off, idx, own, n, 3.9, 6)
+            if not (np.array_equal(p.sel_idx, env.sel_idx) and np.array_equal(m.sel_idx, env.sel_idx)):
+                continue   # selection changed within +-h: the CV is not differentiable there
+            fd = (p.q_ave(6) - m.q_ave(6)) / (2 * h)
+            assert abs(fd - g[k, a]) <= 1e-6 * np.abs(g).max() + 1e-9
+
+
+# ------------------------------------------------------------------ grid
+def test_grid_single_hill_closed_form(orc):
+    g = orc.Grid([0.0], [1.0], [1000])
+    g.deposit([0.3217], 0.005, 0.01)
+    ctr = (np.arange(1000) + 0.5) / 1000
+    full = 0.005 * np.exp(-0.5 * (0.3217 - ctr) ** 2 / 0.01 ** 2)
+    R = int(np.ceil(4 * 0.01 / 0.001)) + 1
+    assert R == 41
+    c = int(0.3217 / 0.001)
+    win = np.zeros(1000, bool)
+    win[max(0, c - R):min(999, c + R) + 1] = True
+    assert np.abs(g.g[win] - full[win]).max() < 1e-15
+    assert np.all(g.g[~win] == 0)
+    # truncation bound h exp(-(R-1)^2 dx^2 / (2 sigma^2))
+    assert full[~win].max() <= 0.005 * np.exp(-((R - 1) * 0.001) ** 2 / (2 * 0.01 ** 2))
+
+
+def test_catmull_rom_exact_on_quadratics(orc):
+    for D in (1, 2):
+        lo, hi, nb = [0.0] * D, [2.0] * D, [40] * D
+        g = orc.Grid(lo, hi, nb)
+        c = (np.arange(40) + 0.5) * 0.05
+        if D == 1:
+            g.g[:] = 0.3 + 1.7 * c - 0.9 * c ** 2
+        else:
+            X, Y = np.meshgrid(c, c, indexing="ij")
+            g.g[:] = (0.3 + 1.7 * X - 0.9 * X ** 2 + 0.4 * Y - 0.2 * Y ** 2 + 0.5 * X * Y).ravel()
+        rng = np.random.default_rng(D)
+        for _ in range(20):
+            s = rng.uniform(0.2, 1.8, size=D)
+            V, dV = g.eval(s)
+            if D == 1:
+                x0 = s[0]
+                assert abs(V - (0.3 + 1.7 * x0 - 0.9 * x0 ** 2)) < 1e-12
+                assert abs(dV[0] - (1.7 - 1.8 * x0)) < 1e-10
+            else:
+                x0, y0 = s
+                assert abs(V - (0.3 + 1.7 * x0 - 0.9 * x0 ** 2 + 0.4 * y0 - 0.2 * y0 ** 2 + 0.5 * x0 * y0)) < 1e-12
+                assert abs(dV[0] - (1.7 - 1.8 * x0 + 0.5 * y0)) < 1e-10
+                assert abs(dV[1] - (0.4 - 0.4 * y0 + 0.5 * x0)) < 1e-10
+
+
+def test_grid_clamp_and_constant(orc):
+    g = orc.Grid([0.0], [1.0], [100])
+    g.g[:] = 0.7
+    for s in (-1.0, 0.0, 0.004, 0.5, 0.99, 2.0):
+        V, dV = g.eval([s])
+        assert abs(V - 0.7) < 1e-15 and abs(dV[0]) < 1e-12
+    g.g[:] = np.arange(100.0)
+    # below lo + 1.5 dx the index clamps to 1 with f = 0: value = g[1]
+    assert g.eval([0.0])[0] == 1.0
+    # above hi - 2.5 dx it clamps to nb-3 with f = 1: value = g[nb-2]
+    assert g.eval([1.0])[0] == 98.0
+
+
+def test_wt_height(orc):
+    g = orc.Grid([0.0], [1.0], [1000])
+    s = [0.4]
+    assert g.wt_height(s, 0.005, 10.0, 300.0, 8.617343e-5, 1) == 0.005
+    g.deposit(s, 0.005, 0.01)
+    V = g.eval(s)[0]
+    kT = 300.0 * 8.617343e-5
+    assert abs(g.wt_height(s, 0.005, 10.0, 300.0, 8.617343e-5, 1) - 0.005 * np.exp(-V / (kT * 9.0))) < 1e-18
+    assert g.wt_height(s, 0.005, 10.0, 300.0, 8.617343e-5, 0) == 0.005
+
+
+def test_reference_hills_fixture(orc):
+    """The reference's own HILLS file (238 1-D hills, h 0.005, sigma 0.01) on the config-1 grid."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "HILLS_reference_test_cvset")
+    rows = np.loadtxt(path, comments="#")
+    assert rows.shape == (238, 4)
+    g = orc.Grid([0.0], [1.0], [1000])
+    for r in rows:
+        g.deposit([r[1]], r[2], r[3])
+    ctr = (np.arange(1000) + 0.5) / 1000
+    full = np.sum(rows[:, 2, None] * np.exp(-0.5 * (rows[:, 1, None] - ctr) ** 2 / rows[:, 3, None] ** 2), axis=0)
+    # truncated windows differ from the full sum by at most 238 tails of h e^-8
+    assert np.abs(g.g - full).max() <= 238 * 0.005 * np.exp(-8.0)
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "grid_reference_hills.npy"))
+    assert np.array_equal(g.g, gold)
+
+
+# ------------------------------------------------------------------ hills sum
+def test_hills_sum_fd_and_grid(orc):
+    rng = np.random.default_rng(4)
+    c = rng.uniform(0.3, 0.7, size=(50, 3))
+    h = rng.uniform(0.001, 0.005, size=50)
+    sg = np.array([0.01, 0.02, 0.5])
+    s = rng.uniform(0.3, 0.7, size=(5, 3))
+    V, dV = orc.hills_sum(c, h, sg, s)
+    for p in range(5):
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = 1e-6
+            fd = (orc.hills_sum(c, h, sg, s[p] + e)[0][0] - orc.hills_sum(c, h, sg, s[p] - e)[0][0]) / 2e-6
+            assert abs(fd - dV[p, a]) < 1e-6 * max(1.0, abs(dV[p, a]))
+    # 1-D: equals the grid at a bin centre when every hill lies well inside the window
+    g = orc.Grid([0.0], [1.0], [1000])
+    c1 = np.array([[0.4], [0.41]])
+    for k in range(2):
+        g.deposit(c1[k], 0.005, 0.01)
+    Vs, _ = orc.hills_sum(c1, [0.005, 0.005], [0.01], [[0.4055]])
+    assert abs(Vs[0] - g.g[405]) < 1e-15
+
+
+# ------------------------------------------------------------------ structure factor
+def test_sk_lattice_and_symmetry(mdt, orc):
+    s = fcc_system(mdt, 4, sigma=0.0)
+    x = s.arrays()[0][: s.n_local]
+    a = 5.26
+    k111 = 2 * np.pi / a * np.array([[1.0, 1.0, 1.0]])
+    v, A, S = orc.sk(x, k111)
+    assert abs(abs(A[0]) - s.n_local) < 1e-8 * s.n_local
+    assert abs(S[0] - s.n_local) < 1e-8 * s.n_local
+    rng = np.random.default_rng(0)
+    xr = rng.uniform(0, 20, size=(300, 3))
+    k = rng.normal(size=(6, 3))
+    _, _, S1 = orc.sk(xr, k)
+    _, _, S2 = orc.sk(xr, -k)
+    assert np.abs(S1 - S2).max() < 1e-10
+
+
+def test_sk_gradient_fd(orc):
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0, 20, size=(200, 3))
+    k = rng.normal(size=(10, 3)) * 2.0
+    g = orc.sk_grad(x, k)
+    h = 1e-6
+    for j in (0, 17, 150):
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[j, a] += h
+            xm[j, a] -= h
+            fd = (orc.sk(xp, k)[0] - orc.sk(xm, k)[0]) / (2 * h)
+            assert abs(fd - g[j, a]) < 1e-6 * max(1.0, np.abs(g).max())
+
+
+# ------------------------------------------------------------------ MLP
+def _mlp_params(seed, H=64):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(size=(H, 3)) * 0.5, rng.normal(size=H) * 0.1, rng.normal(size=(H, H)) / 8,
+            rng.normal(size=H) * 0.1, rng.normal(size=H) / 8, 0.03]
+
+
+def test_mlp_matches_torch_autograd(orc):
+    torch = pytest.importorskip("torch")
+    p = _mlp_params(2)
+    rng = np.random.default_rng(3)
+    s = rng.uniform(-1, 1, size=(16, 3))
+    y = rng.uniform(-1, 1, size=16)
+    V, dV = orc.mlp_forward(p, s)
+    loss, grads = orc.mlp_loss_grad(p, s, y)
+    tp = [torch.tensor(np.asarray(q, dtype=np.float64), requires_grad=True) for q in p]
+    ts = torch.tensor(s, requires_grad=True)
+    h1 = torch.tanh(ts @ tp[0].T + tp[1])
+    h2 = torch.tanh(h1 @ tp[2].T + tp[3])
+    out = h2 @ tp[4] + tp[5]
+    assert np.abs(out.detach().numpy() - V).max() < 1e-12
+    (gs,) = torch.autograd.grad(out.sum(), ts, retain_graph=True)
+    assert np.abs(gs.numpy() - dV).max() < 1e-12
+    L = torch.mean((out - torch.tensor(y)) ** 2)
+    tg = torch.autograd.grad(L, tp)
+    assert abs(L.item() - loss) < 1e-12
+    for a, b in zip(tg, grads):
+        assert np.abs(a.numpy() - np.asarray(b)).max() < 1e-12
