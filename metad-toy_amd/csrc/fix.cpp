@@ -244,7 +244,7 @@ struct mdt_ctx {
     DBuf<double> f;
     std::vector<Tile> h_tiles;
     std::vector<int32_t> tile_begin;
-    mdt_fix* fix = nullptr;
+    std::vector<mdt_fix*> fixes;   // fixes created on this context (detached on destroy)
 
     Geo geo() const {
         Geo g{};
@@ -382,6 +382,7 @@ struct SymH {
 
 struct mdt_fix {
     mdt_ctx* ctx = nullptr;
+    int device = 0;
     // parsed keywords
     double sigma = 0.05, height0 = 0.1, biasf = 10.0, temp = 300.0, boltz = 8.617343e-5;
     int pace = 100, cv_dim = 1, hill_type = 0;
@@ -1109,6 +1110,7 @@ int fail(mdt_ctx* ctx, const Err& e) {
 }
 
 #define API_BEGIN(ctxp) mdt_ctx* ctx__ = (ctxp); try {
+#define FIX_BEGIN(fxp) if (!(fxp) || !(fxp)->ctx) return MDT_ESTATE; API_BEGIN((fxp)->ctx)
 #define API_END                                              \
     }                                                        \
     catch (const Err& e) { return fail(ctx__, e); }          \
@@ -1151,7 +1153,10 @@ void mdt_destroy(mdt_ctx* ctx) {
     if (!ctx) return;
     if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->fix) { ctx->fix->ctx = ctx; }
+    for (mdt_fix* f : ctx->fixes) {
+        if (ctx->stream) { try { flush_hills(f); } catch (...) {} }
+        f->ctx = nullptr;
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1268,29 +1273,32 @@ int mdt_fix_create(mdt_ctx* ctx, int argc, const char* const* argv, mdt_fix** ou
     std::unique_ptr<mdt_fix> fx(new mdt_fix());
     fx->ctx = ctx;
     parse_fix(fx.get(), argc, argv);
+    fx->device = ctx->device;
     *out = fx.release();
-    ctx->fix = *out;
+    ctx->fixes.push_back(*out);
     API_END
 }
 
 void mdt_fix_destroy(mdt_fix* fx) {
     if (!fx) return;
-    if (fx->ctx && fx->inited) {
-        try { flush_hills(fx); } catch (...) {}
+    if (fx->ctx) {
+        if (fx->inited) { try { flush_hills(fx); } catch (...) {} }
+        auto& v = fx->ctx->fixes;
+        v.erase(std::remove(v.begin(), v.end(), fx), v.end());
     }
-    if (fx->ctx && fx->ctx->fix == fx) fx->ctx->fix = nullptr;
+    if (fx->device >= 0) (void)hipSetDevice(fx->device);
     delete fx;
 }
 
 int mdt_fix_init(mdt_fix* fx) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     use_device(fx->ctx);
     ensure_built(fx);
     API_END
 }
 
 int mdt_fix_post_force(mdt_fix* fx, int64_t step, double* f, int on_device) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1313,7 +1321,7 @@ int mdt_fix_post_force(mdt_fix* fx, int64_t step, double* f, int on_device) {
 }
 
 int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1346,7 +1354,7 @@ int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
 
 int mdt_fix_profile(mdt_fix* fx, int64_t step0, int64_t nsteps, char* names, int names_cap,
                     double* ms, int ms_cap, int* nkern) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1376,7 +1384,7 @@ int mdt_fix_profile(mdt_fix* fx, int64_t step0, int64_t nsteps, char* names, int
 }
 
 int mdt_fix_sync(mdt_fix* fx) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     HIPCHK(hipStreamSynchronize(fx->ctx->stream));
     fetch_state(fx);
     flush_hills(fx);
@@ -1391,13 +1399,13 @@ static const double* state_of(mdt_fix* fx, int w) {
 }
 
 int mdt_fix_scalar(mdt_fix* fx, int walker, double* value) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     *value = state_of(fx, walker)[kStV];
     API_END
 }
 
 int mdt_fix_vector(mdt_fix* fx, int walker, int n, double* value) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     if (n < 0 || n >= fx->D()) { *value = 0.0; return MDT_OK; }
     *value = state_of(fx, walker)[kStS + n];
     API_END
@@ -1414,14 +1422,14 @@ static int find_cv(mdt_fix* fx, const char* cal_func) {
 }
 
 int mdt_fix_cv_value(mdt_fix* fx, int walker, const char* cal_func, double* value) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     int c = find_cv(fx, cal_func);
     *value = state_of(fx, walker)[kStCv + fx->cvs[c]->slot];
     API_END
 }
 
 int mdt_fix_cv_weight(mdt_fix* fx, int walker, const char* cal_func, double* value) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     int c = find_cv(fx, cal_func);
     state_of(fx, walker);
     double w = 0;
@@ -1431,7 +1439,7 @@ int mdt_fix_cv_weight(mdt_fix* fx, int walker, const char* cal_func, double* val
 }
 
 int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* out) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     if (!fx->inited) throw Err(MDT_ESTATE, "no step has run yet");
     mdt_ctx* ctx = fx->ctx;
     WalkerIn& a = ctx->walker(walker);
@@ -1472,7 +1480,7 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
 }
 
 int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* sel_idx, int32_t* count_rc, int32_t* n_sel) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1505,7 +1513,7 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
 }
 
 int mdt_fix_get_forces(mdt_fix* fx, int walker, double* f_host) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     WalkerIn& a = ctx->walker(walker);
     if (!ctx->committed) throw Err(MDT_ESTATE, "no forces yet");
@@ -1515,7 +1523,7 @@ int mdt_fix_get_forces(mdt_fix* fx, int walker, double* f_host) {
 }
 
 int mdt_fix_zero_forces(mdt_fix* fx) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     if (!ctx->committed) ctx->commit();
     HIPCHK(launch_fill(ctx->f.p, ctx->total_local * 3, 0.0, ctx->stream));
@@ -1524,7 +1532,7 @@ int mdt_fix_zero_forces(mdt_fix* fx) {
 }
 
 int mdt_fix_bias_eval(mdt_fix* fx, int walker, int64_t np, const double* s, double* V, double* dVds) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1541,7 +1549,7 @@ int mdt_fix_bias_eval(mdt_fix* fx, int walker, int64_t np, const double* s, doub
 }
 
 int mdt_fix_bias_add_hills(mdt_fix* fx, int walker, int64_t nh, const double* s, const double* h, double sigma) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1563,7 +1571,7 @@ int mdt_fix_bias_add_hills(mdt_fix* fx, int walker, int64_t nh, const double* s,
 }
 
 int mdt_fix_grid(mdt_fix* fx, int walker, double* out, int64_t n) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     ensure_built(fx);
     if (fx->hill_type != 0) throw Err(MDT_EINVAL, "this bias has no grid");
     if (n != fx->red.grid_size) throw Err(MDT_EINVAL, fmt("grid has %lld bins", (long long)fx->red.grid_size));
@@ -1573,7 +1581,7 @@ int mdt_fix_grid(mdt_fix* fx, int walker, double* out, int64_t n) {
 }
 
 int mdt_fix_hills(mdt_fix* fx, int walker, int64_t* nh, double* rec, int64_t cap) {
-    API_BEGIN(fx ? fx->ctx : nullptr)
+    FIX_BEGIN(fx)
     ensure_built(fx);
     HIPCHK(hipStreamSynchronize(fx->ctx->stream));
     int64_t n = 0;

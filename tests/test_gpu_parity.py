@@ -128,7 +128,13 @@ def test_ld_local_q(mdt, orc):
 
 
 def test_lockstep_grid_c1(mdt, orc):
-    """Lock-step replay: moving atoms, hills every 10 steps (PACE 10), grid, V and forces each step."""
+    """Lock-step replay: moving atoms, hills every 10 steps (PACE 10), grid, V, dV/ds, forces each step.
+
+    The oracle keeps its own grid fed with its own CVs.  Forces are compared with the
+    oracle's exact gradient contracted with the GPU's chain weight (1e-4 relative); the
+    weight itself is compared with the oracle's dV/ds on its natural scale h/sigma
+    (near a hill maximum dV/ds passes through zero, where a relative test is meaningless).
+    """
     s = fcc_system(mdt, 10, sigma=0.1, seed=21)
     x, off, idx, own = s.arrays()
     n = s.n_local
@@ -152,15 +158,16 @@ def test_lockstep_grid_c1(mdt, orc):
         if step % 10 == 0:
             g.deposit([q6], g.wt_height([q6], 0.005, 10.0, 300.0, 8.617343e-5, 0), 0.01)
         V, dV = g.eval([q6])
-        assert abs(fix.scalar() - V) <= 1e-5 * max(abs(V), 1e-3)
-        F_ref = -env.dQ_dx(6, dV[0])
-        if np.abs(F_ref).max() > 0:
-            e = rel_force_err(f, F_ref)
-            worst = max(worst, e)
-            assert e <= F_TOL, (step, e)
+        assert abs(fix.scalar() - V) <= 1e-4 * abs(V)
+        w = fix.cv_weight("Q6.AVE")
+        assert abs(w - dV[0]) <= 1e-4 * 0.005 / 0.01, (step, w, dV[0])
+        F_ref = -env.dQ_dx(6, w)
+        e = rel_force_err(f, F_ref)
+        worst = max(worst, e)
+        assert e <= F_TOL, (step, e)
         xl = xl + rng.normal(scale=0.005, size=xl.shape)
     grid = fix.grid(1000)
-    assert np.abs(grid - g.g).max() <= 1e-6 * max(np.abs(g.g).max(), 1e-12)
+    assert np.abs(grid - g.g).max() <= 1e-4 * np.abs(g.g).max()
     print(f"lock-step worst force rel err {worst:.3e}")
 
 
