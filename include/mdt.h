@@ -157,6 +157,12 @@ int mdt_fix_selection(mdt_fix* fix, int walker, const char* cal_name, int32_t* s
 
 /* Copy the context's device force buffer of a walker to host (n_local*3). */
 int mdt_fix_get_forces(mdt_fix* fix, int walker, double* f_host);
+/* Virial of the bias forces held in the context's device force buffer
+ * (f == NULL or host-f steps), sum_i F_i (x) unwrap(x_i) over a walker's local
+ * atoms as v[6] = xx yy zz xy xz yz, the tally of FixMetadynamics::post_force
+ * (src/core/fix_crystallize.cu:453-469).  image: LAMMPS image flags, 3 ints
+ * per local atom, NULL = all zero. */
+int mdt_fix_virial(mdt_fix* fix, int walker, const int32_t* image, double v[6]);
 /* Zero the context's device force buffer. */
 int mdt_fix_zero_forces(mdt_fix* fix);
 
@@ -175,6 +181,36 @@ int mdt_fix_grid(mdt_fix* fix, int walker, double* out, int64_t n);
 /* Number of hills deposited so far by a walker and a copy of them:
  * rec[nh*(D+3)] = (step, s..., height, sigma) per hill. */
 int mdt_fix_hills(mdt_fix* fix, int walker, int64_t* nh, double* rec, int64_t cap);
+
+/* ------------------------------------------------------------------------
+ * Learned bias (Gaussian_Hill_type 3): no reference counterpart.  The
+ * network represents the hills deposited up to its last fit; hills deposited
+ * since are summed exactly, and the exact FP64 sum replaces the network
+ * whenever its held-out max |dV| exceeds MLP_TOL kT.
+ * ---------------------------------------------------------------------- */
+
+/* Fit the network of a walker to the hills deposited so far now
+ * (asynchronous; runs by itself every MLP_REFIT steps). */
+int mdt_fix_mlp_refit(mdt_fix* fix, int walker);
+/* Held-out max |dV| of the last fit, whether the network is in use, number of
+ * fits and of fallbacks, hills the last fit was trained on. */
+int mdt_fix_mlp_status(mdt_fix* fix, int walker, double* err, int32_t* ok, int64_t* fits,
+                       int64_t* fallbacks, int64_t* n_fit);
+/* Parameters (live != 0: network in use; 0: last fitted) as 4481 floats
+ * W1[64][3] b1[64] W2[64][64] b2[64] W3[64] b3, and norm[8] = input shift[3],
+ * input scale[3], output scale, output shift. */
+int mdt_fix_mlp_params(mdt_fix* fix, int walker, int live, float* params, double* norm);
+
+/* Standalone network on a context's device (no fix needed): params and norm
+ * as returned by mdt_fix_mlp_params.  mdt_mlp_forward: V at np points s[np*3]
+ * on the BF16 MFMA inference kernel.  mdt_mlp_loss_grad: gradient of
+ * L = (1/np) sum_p (y(x_p) - v_p / out_scale)^2 with respect to the 4481
+ * parameters (y = network output before the output scale), one pass of the
+ * MFMA training kernel without an update. */
+int mdt_mlp_forward(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s,
+                    double* V);
+int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s,
+                      const double* v, float* grad);
 
 /* ------------------------------------------------------------------------
  * Synthetic systems (host; the standalone driver's stand-in for LAMMPS

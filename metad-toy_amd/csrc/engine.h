@@ -7,6 +7,8 @@
 namespace mdt {
 
 constexpr int kBlock = 256;        // threads per workgroup for per-atom kernels
+constexpr int kLanes = 4;          // lanes per atom in the per-atom kernels
+constexpr int kTile = kBlock / kLanes;   // atoms per tile (one workgroup)
 constexpr int kMaxList = 128;      // max neighbour-list length per atom (2 x 64-bit masks)
 constexpr int kMaxSlots = 64;      // max cutoff_Natoms
 constexpr int kMaxSlotCV = 16;     // CV value slots per walker
@@ -18,31 +20,33 @@ constexpr int kStack = 16;
 // Bit set of Steinhardt degrees an environment computes.
 constexpr int kL3 = 1, kL4 = 2, kL6 = 4;
 
-// Per-atom record: for each l present (3, 4, 6 in that order) 2l+1 floats
-// (Re q_l0, then Re/Im q_lm for m = 1..l), then one scale a_l per l,
-// padded to a multiple of 4 floats.
+// Per-atom record: for each l present (3, 4, 6 in that order) a 16-byte aligned
+// block of 2l+1 floats (Re q_l0, then Re/Im q_lm for m = 1..l) followed by the
+// scale a_l = 4pi/(2l+1) / (N_group q_l n); blocks are padded to 4 floats and the
+// record to 64 bytes so a neighbour gather is a few aligned vector loads.
 __host__ __device__ constexpr int rec_nl(int lmask) {
     return ((lmask & kL3) ? 1 : 0) + ((lmask & kL4) ? 1 : 0) + ((lmask & kL6) ? 1 : 0);
 }
-__host__ __device__ constexpr int rec_qfloats(int lmask) {
-    return ((lmask & kL3) ? 7 : 0) + ((lmask & kL4) ? 9 : 0) + ((lmask & kL6) ? 13 : 0);
+__host__ __device__ constexpr int rec_blk(int l) { return (2 * l + 2 + 3) / 4 * 4; }
+__host__ __device__ constexpr int rec_used(int lmask) {
+    return ((lmask & kL3) ? rec_blk(3) : 0) + ((lmask & kL4) ? rec_blk(4) : 0) + ((lmask & kL6) ? rec_blk(6) : 0);
 }
-__host__ __device__ constexpr int rec_stride(int lmask) {
-    return (rec_qfloats(lmask) + rec_nl(lmask) + 3) / 4 * 4;
-}
+__host__ __device__ constexpr int rec_stride(int lmask) { return (rec_used(lmask) + 15) / 16 * 16; }
 // offset of degree l's block inside the record
 __host__ __device__ constexpr int rec_off(int lmask, int l) {
     return l == 3 ? 0
-         : l == 4 ? ((lmask & kL3) ? 7 : 0)
-                  : ((lmask & kL3) ? 7 : 0) + ((lmask & kL4) ? 9 : 0);
+         : l == 4 ? ((lmask & kL3) ? rec_blk(3) : 0)
+                  : ((lmask & kL3) ? rec_blk(3) : 0) + ((lmask & kL4) ? rec_blk(4) : 0);
 }
+// slot of the scale a_l
+__host__ __device__ constexpr int rec_a(int lmask, int l) { return rec_off(lmask, l) + 2 * l + 1; }
 // index (0..nl-1) of degree l among the present ones
 __host__ __device__ constexpr int rec_li(int lmask, int l) {
     return l == 3 ? 0 : l == 4 ? ((lmask & kL3) ? 1 : 0)
                                : ((lmask & kL3) ? 1 : 0) + ((lmask & kL4) ? 1 : 0);
 }
 
-// A tile is a run of <= kBlock consecutive local atoms of one walker.
+// A tile is a run of <= kTile consecutive local atoms of one walker.
 struct Tile {
     int32_t start;
     int32_t count;
@@ -52,11 +56,13 @@ struct Tile {
 
 // Geometry shared by all per-atom kernels (device pointers).
 struct Geo {
-    const double* x;          // (n_local_total + n_ghost_total) * 3
+    const double4* x4;        // (n_local_total + n_ghost_total), xyz + pad, FP64
     const int32_t* off;       // n_local_total + 1 (global CSR offsets)
     const int32_t* idx;       // global all-space indices
     const int32_t* owner;     // ghost -> local index (indexed by j - n_local_total)
-    const uint8_t* revpos;    // per CSR entry: position of the reverse entry, 0xFF = none
+    const int32_t* eown;      // per CSR entry: local owner of idx[e]
+    const uint8_t* revpos;    // per CSR entry: position of the reverse entry (the image of this
+                              // center in the owner's list), 0xFF = none
     const uint8_t* group;     // per local atom, NULL = all
     const Tile* tiles;
     int32_t ntiles;
@@ -72,7 +78,7 @@ struct EnvDev {
     float* ql;                // nl * n_local_total (SoA per degree)
     int32_t* nsel;            // n_local_total
     int32_t* cnt_rc;          // n_local_total
-    uint64_t* selmask;        // n_local_total * 2
+    uint64_t* selmask;        // n_local_total * 4: selected mask, within-rc (slack) mask
     float* extra;             // optional n_local_total * rec_stride: extra coefficients
     double* partial;          // [slot][tile]
     int32_t slot_ql[3];       // partial slot per present degree (order 3,4,6)
@@ -104,6 +110,39 @@ struct ExprProg {
 
 // Bias representations.
 enum BiasKind : int32_t { BIAS_GRID = 0, BIAS_HILLS = 2, BIAS_MLP = 3 };
+
+// Learned bias network 3 -> kMlpH -> kMlpH -> 1 (tanh), FP32 master parameters
+// (row-major W1[a][d], b1, W2[a][b] with z2[a] = sum_b W2[a][b] h1[b], b2, W3, b3),
+// a BF16 copy of W2 for the MFMA products, and the input/output normalisation.
+constexpr int kMlpH = 64;
+constexpr int kW1 = 0, kB1 = kW1 + kMlpH * 3, kW2 = kB1 + kMlpH, kB2 = kW2 + kMlpH * kMlpH,
+              kW3 = kB2 + kMlpH, kB3 = kW3 + kMlpH, kMlpP = kB3 + 1;
+struct MlpNet {
+    float p[kMlpP + 3];
+    short w2bf[kMlpH * kMlpH];      // BF16(W2)
+    short w2lo[kMlpH * kMlpH];      // BF16(W2 - BF16(W2)): hi + lo carries 16 mantissa bits
+    double in_shift[3], in_scale[3], out_scale, out_shift;
+    int64_t n_fit;            // hills represented by the network; later hills are summed exactly
+};
+struct MlpDev {
+    int32_t D;
+    int64_t hills_cap;
+    const double* hills;
+    const int64_t* nhills;
+    double sigma[3];
+    double spread;            // sampling width around a hill, in sigmas
+    MlpNet* live;             // per walker, used by the step
+    MlpNet* stage;            // per walker, being fitted
+    float* adam_m;
+    float* adam_v;
+    int64_t* adam_t;
+    int32_t* ok;              // per walker: 1 = live network trusted
+    double* err;              // per walker: last held-out max |dV|
+    int64_t* fits;
+    int64_t* fallbacks;
+    double* box_lo_out;       // per walker [3]: box of the hill centres
+    double* box_hi_out;
+};
 
 // Everything the per-walker reduce/bias kernel needs (device pointers).
 struct ReduceDev {
@@ -140,9 +179,9 @@ struct ReduceDev {
     double* state;            // per walker [kStateStride]
     double* wcv;              // per walker [kMaxSlotCV]
     // learned bias
-    const float* mlp;         // packed network (see kernels.hip), NULL unless kind == BIAS_MLP
+    const MlpNet* mlp;        // per walker live network, NULL unless kind == BIAS_MLP
     const int32_t* mlp_ok;    // per walker: 1 = network trusted, 0 = use exact sum
-    double mlp_scale_in[kMaxDim], mlp_shift_in[kMaxDim], mlp_scale_out, mlp_shift_out;
+    const double* mlp_err;    // per walker: held-out max |dV| of the last fit
 };
 
 // state layout per walker
@@ -152,11 +191,15 @@ constexpr int kStV = kStS + kMaxDim;           // V after deposit
 constexpr int kStDV = kStV + 1;                // [kMaxDim] dV/ds
 constexpr int kStHill = kStDV + kMaxDim;       // height deposited this step (0 if none)
 constexpr int kStStep = kStHill + 1;           // step of this state
-constexpr int kStVexact = kStStep + 1;         // exact hills-sum V (learned-bias check)
+constexpr int kStMlpErr = kStStep + 1;         // learned bias: held-out max |dV| of the live fit
+constexpr int kStMlpOk = kStMlpErr + 1;        // learned bias: 1 if the network is in use
 constexpr int kStStride = 32;
 
 // ---- launchers (kernels.hip) ----
-hipError_t launch_revpos(const Geo& g, int64_t n_local_total, uint8_t* revpos, hipStream_t st);
+hipError_t launch_topology(const Geo& g, int64_t n_local_total, int32_t* eown, uint8_t* revpos,
+                           hipStream_t st);
+hipError_t launch_pack_x4(const double* x, double4* x4, int64_t n, hipStream_t st);
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, hipStream_t st);
 hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st);
 hipError_t launch_select_literal(const Geo& g, int32_t first, int32_t count, double rc2, int N,
                                  int32_t* sel_idx, int32_t* cnt_rc, int32_t* nsel, hipStream_t st);
@@ -165,11 +208,25 @@ hipError_t launch_ld_coeff(const Geo& g, const EnvDev& e, const LdDev& d, hipStr
 hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st);
 hipError_t launch_reduce(const ReduceDev& r, hipStream_t st);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
+hipError_t launch_virial(const double4* x4, const double* f, const int32_t* image, int first, int n,
+                         const double L[3], double* out, hipStream_t st);
 hipError_t launch_fill_f(float* p, int64_t n, float v, hipStream_t st);
 hipError_t launch_bias_eval(const ReduceDev& r, int walker, int64_t np, const double* s,
                             double* V, double* dV, hipStream_t st);
 hipError_t launch_bias_add(const ReduceDev& r, int walker, int64_t nh, const double* s,
                            const double* h, double sigma, hipStream_t st);
+
+// learned bias (mlp.hip)
+hipError_t mlp_data(const MlpDev& m, int w, int64_t n, uint64_t seed, float4* s, double* v, hipStream_t st);
+hipError_t mlp_box(const MlpDev& m, int w, hipStream_t st);
+hipError_t mlp_stats(const MlpDev& m, int w, int64_t n, const double* v, hipStream_t st);
+hipError_t mlp_train_step(const MlpDev& m, int w, int64_t first, int64_t count, int64_t n_total, const float4* s,
+                          const double* v, float* gpart, float lr, float* gsum, hipStream_t st);
+hipError_t mlp_forward(const MlpNet* net, int64_t n, const float4* s, const double* vref, double* vout,
+                       double* blk_err, hipStream_t st);
+hipError_t mlp_commit(const MlpDev& m, int w, int nblk, const double* blk_err, double tol, hipStream_t st);
+hipError_t mlp_copy_net(const MlpNet* src, MlpNet* dst, hipStream_t st);
+hipError_t mlp_init(MlpNet* net, uint64_t seed, hipStream_t st);
 
 // Structure factor
 struct SkDev {

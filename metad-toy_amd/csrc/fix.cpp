@@ -238,7 +238,8 @@ struct mdt_ctx {
     int64_t total_local = 0, total_ghost = 0, total_nnz = 0;
     bool any_group = false;
     DBuf<double> x;
-    DBuf<int32_t> off, idx, owner;
+    DBuf<double4> x4;
+    DBuf<int32_t> off, idx, owner, eown;
     DBuf<uint8_t> revpos, group;
     DBuf<Tile> tiles;
     DBuf<double> f;
@@ -248,7 +249,8 @@ struct mdt_ctx {
 
     Geo geo() const {
         Geo g{};
-        g.x = x.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p; g.revpos = revpos.p;
+        g.x4 = x4.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p;
+        g.eown = eown.p; g.revpos = revpos.p;
         g.group = nullptr;
         g.tiles = tiles.p; g.ntiles = (int32_t)h_tiles.size(); g.n_local_total = (int32_t)total_local;
         return g;
@@ -306,10 +308,10 @@ void mdt_ctx::commit() {
         }
         if (!a.group.empty()) std::memcpy(&hgroup[a.atom_off], a.group.data(), a.n_local);
         a.tile_off = (int32_t)h_tiles.size();
-        for (int64_t s = 0; s < a.n_local; s += kBlock) {
+        for (int64_t s = 0; s < a.n_local; s += kTile) {
             Tile t{};
             t.start = (int32_t)(a.atom_off + s);
-            t.count = (int32_t)std::min<int64_t>(kBlock, a.n_local - s);
+            t.count = (int32_t)std::min<int64_t>(kTile, a.n_local - s);
             t.walker = (int32_t)k;
             h_tiles.push_back(t);
         }
@@ -322,13 +324,16 @@ void mdt_ctx::commit() {
     off.alloc(hoff.size()); off.upload(hoff.data(), hoff.size(), stream);
     idx.alloc(std::max<size_t>(hidx.size(), 1)); idx.upload(hidx.data(), hidx.size(), stream);
     owner.alloc(hown.size()); owner.upload(hown.data(), tg, stream);
+    x4.alloc((size_t)(tl + tg));
+    HIPCHK(launch_pack_x4(x.p, x4.p, tl + tg, stream));
+    eown.alloc(std::max<size_t>(hidx.size(), 1));
     revpos.alloc(std::max<size_t>(hidx.size(), 1));
     if (any_group) { group.alloc(hgroup.size()); group.upload(hgroup.data(), hgroup.size(), stream); }
     else group.release();
     tiles.alloc(std::max<size_t>(h_tiles.size(), 1)); tiles.upload(h_tiles.data(), h_tiles.size(), stream);
     f.alloc((size_t)std::max<int64_t>(tl, 1) * 3);
     HIPCHK(launch_fill(f.p, tl * 3, 0.0, stream));
-    HIPCHK(launch_revpos(geo(), tl, revpos.p, stream));
+    HIPCHK(launch_topology(geo(), tl, eown.p, revpos.p, stream));
     HIPCHK(hipStreamSynchronize(stream));
     committed = true;
     generation++;
@@ -389,6 +394,12 @@ struct mdt_fix {
     bool restart = false, wt = false;
     std::string hills_file = "HILLS";
     std::vector<double> sigmas;                 // per-dimension (extension)
+    // learned bias (Gaussian_Hill_type 3)
+    int64_t mlp_refit = 500, mlp_points = 1000000, mlp_holdout = 65536, mlp_batch = 4096;
+    int mlp_epochs = 8, mlp_epochs_first = 40;
+    double mlp_lr = 1e-2, mlp_tol = 0.1, mlp_spread = 2.0;
+    uint64_t mlp_seed = 20241203;
+    int64_t mlp_round = 0;
     std::vector<double> lo, hi;
     std::vector<int> nb;
     std::vector<std::string> dim_expr;
@@ -404,7 +415,13 @@ struct mdt_fix {
     DBuf<int64_t> nhills, step;
     DBuf<int32_t> tile_begin, mlp_ok;
     DBuf<ExprProg> progs;
-    DBuf<float> mlp;
+    DBuf<MlpNet> mlp_live, mlp_stage;
+    DBuf<float> mlp_m, mlp_v, mlp_gpart;
+    DBuf<int64_t> mlp_t, mlp_fits, mlp_fallbacks;
+    DBuf<double> mlp_err, mlp_box_lo, mlp_box_hi, mlp_vdata, mlp_blkerr;
+    DBuf<float4> mlp_sdata;
+    std::vector<int> mlp_has_fit;
+    MlpDev mdev{};
     ReduceDev red{};
     int64_t hills_cap = 0;
     std::vector<int64_t> hills_written;   // per walker, hills already in the HILLS file
@@ -479,6 +496,25 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
         } else if (kw == "HILLS_FILE") {
             need(1, "HILLS_FILE requires a path (NONE disables the file).");
             fx->hills_file = argv[i + 1];
+            i += 2;
+        } else if (kw.rfind("MLP_", 0) == 0) {
+            // learned-bias options (Gaussian_Hill_type 3)
+            need(1, "MLP_* options take one value");
+            const char* v = argv[i + 1];
+            if (kw == "MLP_REFIT") fx->mlp_refit = parse_int(v, kw.c_str());
+            else if (kw == "MLP_POINTS") fx->mlp_points = parse_int(v, kw.c_str());
+            else if (kw == "MLP_HOLDOUT") fx->mlp_holdout = parse_int(v, kw.c_str());
+            else if (kw == "MLP_BATCH") fx->mlp_batch = parse_int(v, kw.c_str());
+            else if (kw == "MLP_EPOCHS") fx->mlp_epochs = (int)parse_int(v, kw.c_str());
+            else if (kw == "MLP_EPOCHS_FIRST") fx->mlp_epochs_first = (int)parse_int(v, kw.c_str());
+            else if (kw == "MLP_LR") fx->mlp_lr = parse_num(v, kw.c_str());
+            else if (kw == "MLP_TOL") fx->mlp_tol = parse_num(v, kw.c_str());
+            else if (kw == "MLP_SPREAD") fx->mlp_spread = parse_num(v, kw.c_str());
+            else if (kw == "MLP_SEED") fx->mlp_seed = (uint64_t)parse_int(v, kw.c_str());
+            else throw Err(MDT_EINVAL, fmt("Error: Unknown keyword in fix metadynamics command: %s", kw.c_str()));
+            if (fx->mlp_refit < 1 || fx->mlp_points < 64 || fx->mlp_holdout < 64 || fx->mlp_batch < 64 ||
+                fx->mlp_epochs < 0 || fx->mlp_epochs_first < 0 || !(fx->mlp_lr >= 0) || !(fx->mlp_tol >= 0))
+                throw Err(MDT_EINVAL, fmt("%s: value out of range", kw.c_str()));
             i += 2;
         } else if (kw == "SIGMAS") {
             // per-dimension hill widths for the hills-sum and learned biases
@@ -700,7 +736,7 @@ void fix_build(mdt_fix* fx) {
         const int RS = rec_stride(lm), NL = rec_nl(lm);
         e->rec.alloc((size_t)NT * RS);
         e->ql.alloc((size_t)NT * NL);
-        e->nsel.alloc(NT); e->cnt.alloc(NT); e->selmask.alloc((size_t)NT * 2);
+        e->nsel.alloc(NT); e->cnt.alloc(NT); e->selmask.alloc((size_t)NT * 4);
         e->inv_ng.alloc(nw);
         std::vector<double> inv(nw);
         const bool all = e->group == "all";
@@ -809,6 +845,22 @@ void fix_build(mdt_fix* fx) {
         fx->mlp_ok.alloc(nw);
         std::vector<int32_t> zi(nw, 0);
         fx->mlp_ok.upload(zi.data(), nw, st);
+        if (fx->hill_type == 3) {
+            fx->mlp_live.alloc(nw); fx->mlp_stage.alloc(nw);
+            fx->mlp_m.alloc((size_t)nw * kMlpP); fx->mlp_v.alloc((size_t)nw * kMlpP);
+            fx->mlp_t.alloc(nw); fx->mlp_fits.alloc(nw); fx->mlp_fallbacks.alloc(nw);
+            fx->mlp_err.alloc(nw); fx->mlp_box_lo.alloc((size_t)3 * nw); fx->mlp_box_hi.alloc((size_t)3 * nw);
+            fx->mlp_t.upload(z.data(), nw, st); fx->mlp_fits.upload(z.data(), nw, st);
+            fx->mlp_fallbacks.upload(z.data(), nw, st);
+            HIPCHK(launch_fill(fx->mlp_err.p, nw, 0.0, st));
+            const int64_t ntot = fx->mlp_points + fx->mlp_holdout;
+            fx->mlp_sdata.alloc((size_t)ntot);
+            fx->mlp_vdata.alloc((size_t)ntot);
+            fx->mlp_gpart.alloc((size_t)((fx->mlp_batch + 63) / 64) * kMlpP);
+            fx->mlp_blkerr.alloc((size_t)((fx->mlp_holdout + 63) / 64));
+            for (int w = 0; w < nw; ++w) HIPCHK(mlp_init(fx->mlp_live.p + w, fx->mlp_seed + 977 * w, st));
+            fx->mlp_has_fit.assign(nw, 0);
+        }
         fx->hills_written.assign(nw, 0);
         fx->hills_bound.assign(nw, 0);
         HIPCHK(hipStreamSynchronize(st));
@@ -844,7 +896,18 @@ void fix_build(mdt_fix* fx) {
     r.h0 = fx->height0; r.biasf = fx->biasf; r.kT = fx->temp * fx->boltz; r.wt = fx->wt ? 1 : 0;
     r.hills = fx->hills.p; r.nhills = fx->nhills.p; r.hills_cap = fx->hills_cap;
     r.step = fx->step.p; r.state = fx->state.p; r.wcv = fx->wcv.p;
-    r.mlp = fx->mlp.p; r.mlp_ok = fx->mlp_ok.p;
+    r.mlp = fx->mlp_live.p; r.mlp_ok = fx->mlp_ok.p; r.mlp_err = fx->mlp_err.p;
+    if (fx->hill_type == 3) {
+        MlpDev& m = fx->mdev;
+        m = MlpDev{};
+        m.D = fx->D(); m.hills_cap = fx->hills_cap; m.hills = fx->hills.p; m.nhills = fx->nhills.p;
+        for (int d = 0; d < 3; ++d) m.sigma[d] = fx->sigmas[d];
+        m.spread = fx->mlp_spread;
+        m.live = fx->mlp_live.p; m.stage = fx->mlp_stage.p;
+        m.adam_m = fx->mlp_m.p; m.adam_v = fx->mlp_v.p; m.adam_t = fx->mlp_t.p;
+        m.ok = fx->mlp_ok.p; m.err = fx->mlp_err.p; m.fits = fx->mlp_fits.p; m.fallbacks = fx->mlp_fallbacks.p;
+        m.box_lo_out = fx->mlp_box_lo.p; m.box_hi_out = fx->mlp_box_hi.p;
+    }
     for (auto* e : fx->envs) e->dev.partial = fx->partial.p, e->dev.wcv = fx->wcv.p;
     if (!fx->pin) {
         fx->pin_n = 1 << 16;
@@ -951,6 +1014,8 @@ void ensure_hills(mdt_fix* fx, int64_t extra) {
     fx->hills_cap = cap;
     fx->red.hills = fx->hills.p;
     fx->red.hills_cap = cap;
+    fx->mdev.hills = fx->hills.p;
+    fx->mdev.hills_cap = cap;
     fx->graph_gen = 0;   // invalidate the captured graph
 }
 
@@ -985,6 +1050,45 @@ void flush_hills(mdt_fix* fx) {
         }
         fx->hills_written[w] = nh[w];
     }
+}
+
+// Fit the learned bias of walker w to the hills deposited so far (asynchronous,
+// on the context stream): sample + exact FP64 labels, Adam epochs on the MFMA
+// kernels, held-out max |dV| gate against MLP_TOL kT.
+void mlp_refit(mdt_fix* fx, int w) {
+    hipStream_t st = fx->ctx->stream;
+    const MlpDev& m = fx->mdev;
+    MlpNet* stage = fx->mlp_stage.p + w;
+    const bool first = !fx->mlp_has_fit[w];
+    if (first) HIPCHK(mlp_init(stage, fx->mlp_seed + 977 * w, st));
+    else HIPCHK(mlp_copy_net(fx->mlp_live.p + w, stage, st));
+    HIPCHK(launch_fill_f(fx->mlp_m.p + (size_t)w * kMlpP, kMlpP, 0.0f, st));
+    HIPCHK(launch_fill_f(fx->mlp_v.p + (size_t)w * kMlpP, kMlpP, 0.0f, st));
+    HIPCHK(mlp_box(m, w, st));
+    const uint64_t seed = fx->mlp_seed * 0x9E3779B97F4A7C15ull + (uint64_t)(fx->mlp_round++) * 7919 + w;
+    const int64_t M = fx->mlp_points, Hn = fx->mlp_holdout;
+    HIPCHK(mlp_data(m, w, M, seed, fx->mlp_sdata.p, fx->mlp_vdata.p, st));
+    HIPCHK(mlp_data(m, w, Hn, seed ^ 0xA5A5A5A5A5A5A5A5ull, fx->mlp_sdata.p + M, fx->mlp_vdata.p + M, st));
+    HIPCHK(mlp_stats(m, w, M, fx->mlp_vdata.p, st));
+    const int epochs = first ? fx->mlp_epochs_first : fx->mlp_epochs;
+    const int64_t B = fx->mlp_batch;
+    int64_t it = 0;
+    const int64_t nit = epochs * ((M + B - 1) / B);
+    for (int e = 0; e < epochs; ++e)
+        for (int64_t b0 = 0; b0 < M; b0 += B, ++it) {
+            // cosine decay of the learning rate over the fit
+            const double lr = fx->mlp_lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.141592653589793 * (double)it / (double)std::max<int64_t>(nit, 1))));
+            HIPCHK(mlp_train_step(m, w, b0, std::min(B, M - b0), M, fx->mlp_sdata.p, fx->mlp_vdata.p,
+                                  fx->mlp_gpart.p, (float)lr, nullptr, st));
+        }
+    const int nblk = (int)((Hn + 63) / 64);
+    HIPCHK(mlp_forward(stage, Hn, fx->mlp_sdata.p + M, fx->mlp_vdata.p + M, nullptr, fx->mlp_blkerr.p, st));
+    HIPCHK(mlp_commit(m, w, nblk, fx->mlp_blkerr.p, fx->mlp_tol * fx->temp * fx->boltz, st));
+    fx->mlp_has_fit[w] = 1;
+}
+
+void mlp_refit_all(mdt_fix* fx) {
+    for (int w = 0; w < fx->nw(); ++w) mlp_refit(fx, w);
 }
 
 // Optional per-launch timing (HIP events on the context stream).
@@ -1262,6 +1366,9 @@ int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_devic
     HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * a.atom_off, x, sizeof(double) * 3 * a.n_local, k, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), x + 3 * a.n_local,
                           sizeof(double) * 3 * (a.n_all - a.n_local), k, ctx->stream));
+    HIPCHK(launch_pack_x4(ctx->x.p + 3 * a.atom_off, ctx->x4.p + a.atom_off, a.n_local, ctx->stream));
+    HIPCHK(launch_pack_x4(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), ctx->x4.p + ctx->total_local + a.ghost_off,
+                          a.n_all - a.n_local, ctx->stream));
     if (!on_device) HIPCHK(hipStreamSynchronize(ctx->stream));   // pageable source must stay valid
     API_END
 }
@@ -1309,6 +1416,7 @@ int mdt_fix_post_force(mdt_fix* fx, int64_t step, double* f, int on_device) {
     if (f && on_device) fd = f;
     else if (f) HIPCHK(launch_fill(ctx->f.p, ctx->total_local * 3, 0.0, ctx->stream));
     enqueue_step(fx, fd);
+    if (fx->hill_type == 3 && (step + 1) % fx->mlp_refit == 0) mlp_refit_all(fx);
     if (f && !on_device) {
         std::vector<double> tmp((size_t)ctx->total_local * 3);
         HIPCHK(hipMemcpyAsync(tmp.data(), ctx->f.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -1347,7 +1455,10 @@ int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
         if (e != hipSuccess) throw Err(MDT_EHIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e)));
         fx->graph_gen = ctx->generation + 1;
     }
-    for (int64_t s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(fx->gexec, ctx->stream));
+    for (int64_t s = 0; s < nsteps; ++s) {
+        HIPCHK(hipGraphLaunch(fx->gexec, ctx->stream));
+        if (fx->hill_type == 3 && (step0 + s + 1) % fx->mlp_refit == 0) mlp_refit_all(fx);
+    }
     fx->state_valid = false;
     API_END
 }
@@ -1469,7 +1580,7 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
         for (int64_t i = 0; i < n; ++i) out[i] = v[i];
     } else if (prop == "qlm") {
         // q_lm of degree l: 2l+1 floats per atom (Re q_l0, Re/Im q_lm m>=1)
-        const int RS = rec_stride(e->lmask), O = rec_off(e->lmask, c->l), W = 2 * c->l + 1;
+        const int RS = rec_stride(e->lmask), O = rec_off(e->lmask, c->l), W = 2 * c->l + 1;  // q block of l
         std::vector<float> v((size_t)n * RS);
         HIPCHK(hipMemcpy(v.data(), e->rec.p + (size_t)o * RS, sizeof(float) * v.size(), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) for (int q = 0; q < W; ++q) out[i * W + q] = v[(size_t)i * RS + O + q];
@@ -1519,6 +1630,149 @@ int mdt_fix_get_forces(mdt_fix* fx, int walker, double* f_host) {
     if (!ctx->committed) throw Err(MDT_ESTATE, "no forces yet");
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(f_host, ctx->f.p + 3 * a.atom_off, sizeof(double) * 3 * a.n_local, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fix_mlp_refit(mdt_fix* fx, int walker) {
+    FIX_BEGIN(fx)
+    use_device(fx->ctx);
+    ensure_built(fx);
+    if (fx->hill_type != 3) throw Err(MDT_EINVAL, "no learned bias (Gaussian_Hill_type 3)");
+    fx->ctx->walker(walker);
+    mlp_refit(fx, walker);
+    fx->state_valid = false;
+    API_END
+}
+
+int mdt_fix_mlp_status(mdt_fix* fx, int walker, double* err, int32_t* ok, int64_t* fits, int64_t* fallbacks,
+                       int64_t* n_fit) {
+    FIX_BEGIN(fx)
+    use_device(fx->ctx);
+    if (fx->hill_type != 3 || !fx->inited) throw Err(MDT_EINVAL, "no learned bias (Gaussian_Hill_type 3)");
+    fx->ctx->walker(walker);
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    if (err) HIPCHK(hipMemcpy(err, fx->mlp_err.p + walker, sizeof(double), hipMemcpyDeviceToHost));
+    if (ok) HIPCHK(hipMemcpy(ok, fx->mlp_ok.p + walker, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (fits) HIPCHK(hipMemcpy(fits, fx->mlp_fits.p + walker, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (fallbacks) HIPCHK(hipMemcpy(fallbacks, fx->mlp_fallbacks.p + walker, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (n_fit) {
+        MlpNet h;
+        HIPCHK(hipMemcpy(&h, fx->mlp_stage.p + walker, sizeof(MlpNet), hipMemcpyDeviceToHost));
+        *n_fit = h.n_fit;
+    }
+    API_END
+}
+
+int mdt_fix_mlp_params(mdt_fix* fx, int walker, int live, float* params, double* norm) {
+    FIX_BEGIN(fx)
+    use_device(fx->ctx);
+    if (fx->hill_type != 3 || !fx->inited) throw Err(MDT_EINVAL, "no learned bias (Gaussian_Hill_type 3)");
+    fx->ctx->walker(walker);
+    HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    MlpNet h;
+    HIPCHK(hipMemcpy(&h, (live ? fx->mlp_live.p : fx->mlp_stage.p) + walker, sizeof(MlpNet), hipMemcpyDeviceToHost));
+    if (params) std::memcpy(params, h.p, sizeof(float) * kMlpP);
+    if (norm) {
+        for (int d = 0; d < 3; ++d) { norm[d] = h.in_shift[d]; norm[3 + d] = h.in_scale[d]; }
+        norm[6] = h.out_scale; norm[7] = h.out_shift;
+    }
+    API_END
+}
+
+// standalone network: params (4481 floats) + norm[8] -> a device MlpNet
+static void upload_net(mdt_ctx* ctx, const float* params, const double* norm, DBuf<MlpNet>& net) {
+    MlpNet h{};
+    std::memcpy(h.p, params, sizeof(float) * kMlpP);
+    auto bf = [](float v) {   // round to nearest even BF16, as the device does
+        uint32_t u;
+        std::memcpy(&u, &v, 4);
+        uint32_t lsb = (u >> 16) & 1u;
+        u += 0x7FFFu + lsb;
+        return (uint16_t)(u >> 16);
+    };
+    for (int k = 0; k < kMlpH * kMlpH; ++k) {
+        const uint16_t hi = bf(params[kW2 + k]);
+        uint32_t hb = (uint32_t)hi << 16;
+        float hv;
+        std::memcpy(&hv, &hb, 4);
+        h.w2bf[k] = (short)hi;
+        h.w2lo[k] = (short)bf(params[kW2 + k] - hv);
+    }
+    for (int d = 0; d < 3; ++d) { h.in_shift[d] = norm[d]; h.in_scale[d] = norm[3 + d]; }
+    h.out_scale = norm[6]; h.out_shift = norm[7];
+    h.n_fit = 0;
+    net.alloc(1);
+    net.upload(&h, 1, ctx->stream);
+}
+
+static void upload_points(mdt_ctx* ctx, int64_t np, const double* s, DBuf<float4>& ds) {
+    std::vector<float4> h((size_t)std::max<int64_t>(np, 1));
+    for (int64_t p = 0; p < np; ++p) h[p] = make_float4((float)s[3 * p], (float)s[3 * p + 1], (float)s[3 * p + 2], 0.0f);
+    ds.alloc(h.size());
+    ds.upload(h.data(), h.size(), ctx->stream);
+}
+
+int mdt_mlp_forward(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s, double* V) {
+    API_BEGIN(ctx)
+    use_device(ctx);
+    DBuf<MlpNet> net;
+    upload_net(ctx, params, norm, net);
+    DBuf<float4> ds;
+    upload_points(ctx, np, s, ds);
+    DBuf<double> dv;
+    dv.alloc((size_t)std::max<int64_t>(np, 1));
+    HIPCHK(mlp_forward(net.p, np, ds.p, nullptr, dv.p, nullptr, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(V, dv.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s,
+                      const double* v, float* grad) {
+    API_BEGIN(ctx)
+    use_device(ctx);
+    DBuf<MlpNet> net;
+    upload_net(ctx, params, norm, net);
+    DBuf<float4> ds;
+    upload_points(ctx, np, s, ds);
+    DBuf<double> dv;
+    dv.alloc((size_t)std::max<int64_t>(np, 1));
+    dv.upload(v, (size_t)np, ctx->stream);
+    const int nb = (int)((np + 63) / 64);
+    DBuf<float> gpart, gsum, am, av;
+    gpart.alloc((size_t)std::max(nb, 1) * kMlpP);
+    gsum.alloc(kMlpP); am.alloc(kMlpP); av.alloc(kMlpP);
+    DBuf<int64_t> at;
+    at.alloc(1);
+    int64_t zero = 0;
+    at.upload(&zero, 1, ctx->stream);
+    MlpDev m{};
+    m.D = 3; m.stage = net.p; m.live = net.p; m.adam_m = am.p; m.adam_v = av.p; m.adam_t = at.p;
+    HIPCHK(mlp_train_step(m, 0, 0, np, np, ds.p, dv.p, gpart.p, 0.0f, gsum.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(grad, gsum.p, sizeof(float) * kMlpP, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fix_virial(mdt_fix* fx, int walker, const int32_t* image, double v[6]) {
+    FIX_BEGIN(fx)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    WalkerIn& a = ctx->walker(walker);
+    if (!ctx->committed) throw Err(MDT_ESTATE, "no forces yet");
+    const int n = (int)a.n_local;
+    const int nb = (n + kBlock - 1) / kBlock;
+    DBuf<double> part;
+    part.alloc((size_t)std::max(nb, 1) * 6);
+    DBuf<int32_t> img;
+    if (image) { img.alloc((size_t)std::max(n, 1) * 3); img.upload(image, (size_t)n * 3, ctx->stream); }
+    const double L[3] = {a.hi[0] - a.lo[0], a.hi[1] - a.lo[1], a.hi[2] - a.lo[2]};
+    HIPCHK(launch_virial(ctx->x4.p, ctx->f.p, image ? img.p : nullptr, (int)a.atom_off, n, L, part.p, ctx->stream));
+    std::vector<double> h((size_t)std::max(nb, 1) * 6);
+    HIPCHK(hipMemcpyAsync(h.data(), part.p, sizeof(double) * nb * 6, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 6; ++k) v[k] = 0.0;
+    for (int b = 0; b < nb; ++b) for (int k = 0; k < 6; ++k) v[k] += h[(size_t)b * 6 + k];
     API_END
 }
 

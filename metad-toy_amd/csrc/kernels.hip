@@ -170,67 +170,131 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block-wide FP64 sum of NV values per thread in a fixed order; result on thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double* v, double* sh /* NV*4 */) {
+// NW = waves per block (blockDim.x / 64); sh holds NV * NW doubles.
+template <int NV, int NW = 4>
+__device__ __forceinline__ void block_sum(double* v, double* sh) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     #pragma unroll
     for (int k = 0; k < NV; ++k) {
         double s = wave_sum(v[k]);
-        if (lane == 0) sh[k * 4 + wv] = s;
+        if (lane == 0) sh[k * NW + wv] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = (sh[k * 4 + 0] + sh[k * 4 + 1]) + (sh[k * 4 + 2] + sh[k * 4 + 3]);
+        for (int k = 0; k < NV; ++k) {
+            double a = 0.0;
+            #pragma unroll
+            for (int q = 0; q < NW; ++q) a += sh[k * NW + q];
+            v[k] = a;
+        }
     }
 }
 
 // FP64 pair vector and r^2 exactly as the reference evaluates it without
 // contraction: r2 = (dx*dx + dy*dy) + dz*dz.
-__device__ __forceinline__ double pair_r2(const double* __restrict__ x, int j, double xi0, double xi1,
+__device__ __forceinline__ double pair_r2(const double4* __restrict__ x4, int j, double xi0, double xi1,
                                           double xi2, double& d0, double& d1, double& d2) {
 #pragma clang fp contract(off)
-    d0 = x[3 * j] - xi0;
-    d1 = x[3 * j + 1] - xi1;
-    d2 = x[3 * j + 2] - xi2;
+    const double4 p = x4[j];
+    d0 = p.x - xi0;
+    d1 = p.y - xi1;
+    d2 = p.z - xi2;
     return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
 // ---------------------------------------------------------------------------
-// Reverse positions: for list entry (k, p) -> j with owner i, the position q
-// in i's list of the image of k at displacement -(x_j - x_k).  Computed once
-// per neighbour-list binding.
+// List topology, once per neighbour-list binding: for entry e = (k -> j) the
+// local owner i of j, and the reverse position: where in i's list the image
+// of k sits at displacement -(x_j - x_k).  An atom sees one image of another
+// inside the list cutoff when every box edge exceeds twice the cutoff, so the
+// reverse entry is unique.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_revpos(Geo g, int64_t n, uint8_t* __restrict__ revpos) {
+__global__ __launch_bounds__(256) void k_topology(Geo g, int64_t n, int32_t* __restrict__ eown,
+                                                   uint8_t* __restrict__ revpos) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int beg = g.off[k], end = g.off[k + 1];
-    const double xk0 = g.x[3 * k], xk1 = g.x[3 * k + 1], xk2 = g.x[3 * k + 2];
+    const double4 xk = g.x4[k];
     for (int e = beg; e < end; ++e) {
         const int j = g.idx[e];
         const int i = owner_of(g, j);
-        const double d0 = g.x[3 * j] - xk0, d1 = g.x[3 * j + 1] - xk1, d2 = g.x[3 * j + 2] - xk2;
-        const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+        const double4 xj = g.x4[j];
+        const double d0 = xj.x - xk.x, d1 = xj.y - xk.y, d2 = xj.z - xk.z;
+        const double4 xi = g.x4[i];
         int best = 0xFF;
         double bd = 1e-8;
         const int b2 = g.off[i], e2 = g.off[i + 1];
         for (int q = b2; q < e2; ++q) {
             const int t = g.idx[q];
             if (owner_of(g, t) != (int)k) continue;
-            double s0 = g.x[3 * t] - xi0 + d0, s1 = g.x[3 * t + 1] - xi1 + d1, s2 = g.x[3 * t + 2] - xi2 + d2;
+            const double4 xt = g.x4[t];
+            double s0 = xt.x - xi.x + d0, s1 = xt.y - xi.y + d1, s2 = xt.z - xi.z + d2;
             double dd = s0 * s0 + s1 * s1 + s2 * s2;
             if (dd < bd) { bd = dd; best = q - b2; }
         }
+        eown[e] = i;
         revpos[e] = (uint8_t)best;
     }
 }
 
+__global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x4[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+}
+
 // ---------------------------------------------------------------------------
-// k_env: selection + q_lm for every degree in LMASK.  One thread per atom.
-// Phase A scans the list with FP64 r^2; if the number within rc does not
+// 128-bit candidate masks and lane-group helpers.  The per-atom kernels run
+// kLanes adjacent lanes per atom; lane l takes the set bits of rank l, l+kLanes,
+// ...  Group sums use a fixed xor butterfly, so every lane of a group holds the
+// bitwise identical result and the outcome does not depend on scheduling.
+// ---------------------------------------------------------------------------
+struct M128 { uint64_t lo, hi; };
+__device__ __forceinline__ bool mbit(const M128& m, int p) {
+    return p < 64 ? ((m.lo >> p) & 1ull) : ((m.hi >> (p - 64)) & 1ull);
+}
+__device__ __forceinline__ void mset(M128& m, int p) {
+    if (p < 64) m.lo |= 1ull << p; else m.hi |= 1ull << (p - 64);
+}
+__device__ __forceinline__ bool mempty(const M128& m) { return (m.lo | m.hi) == 0ull; }
+__device__ __forceinline__ int mpop(M128& m) {
+    if (m.lo) { int p = __builtin_ctzll(m.lo); m.lo &= m.lo - 1; return p; }
+    int p = 64 + __builtin_ctzll(m.hi); m.hi &= m.hi - 1; return p;
+}
+__device__ __forceinline__ void mdrop(M128& m, int k) {
+    for (int t = 0; t < k; ++t) { if (m.lo) m.lo &= m.lo - 1; else m.hi &= m.hi - 1; }
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int o) {
+    return (uint64_t)__shfl_xor((long long)v, o, 64);
+}
+__device__ __forceinline__ M128 grp_or(M128 m) {
+    #pragma unroll
+    for (int o = 1; o < kLanes; o <<= 1) { m.lo |= shfl_xor_u64(m.lo, o); m.hi |= shfl_xor_u64(m.hi, o); }
+    return m;
+}
+__device__ __forceinline__ float grp_sum(float v) {
+    #pragma unroll
+    for (int o = 1; o < kLanes; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int grp_sum_i(int v) {
+    #pragma unroll
+    for (int o = 1; o < kLanes; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ M128 grp_bcast0(M128 m) {
+    const int src = (int)(threadIdx.x & ~(unsigned)(kLanes - 1)) & 63;
+    m.lo = (uint64_t)__shfl((long long)m.lo, src, 64);
+    m.hi = (uint64_t)__shfl((long long)m.hi, src, 64);
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// k_env: selection + q_lm for every degree in LMASK, kLanes lanes per atom.
+// Phase A scans the list with FP64 r^2; when the number within rc does not
 // exceed N every such candidate is kept (what the reference insertion keeps
-// then), otherwise the reference insertion is replayed in registers to find
-// the kept set.  Phase B accumulates Y_lm in FP32 over the kept set.
+// then), otherwise lane 0 replays the reference insertion to find the kept
+// set.  Phase B accumulates Y_lm in FP32 over the kept set.
 // ---------------------------------------------------------------------------
 template <int NMAX>
 __device__ __forceinline__ void literal_insert(const Geo& g, int beg, int c, double xi0, double xi1,
@@ -239,7 +303,7 @@ __device__ __forceinline__ void literal_insert(const Geo& g, int beg, int c, dou
     for (int s = 0; s < NMAX; ++s) { sr2[s] = __builtin_inf(); sp[s] = -1; }
     for (int p = 0; p < c; ++p) {
         double d0, d1, d2;
-        double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+        double r2 = pair_r2(g.x4, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
         if ((r2 > rc2) || (r2 < 1e-12)) continue;
         double r = r2;
         int q = p;
@@ -258,100 +322,114 @@ __device__ __forceinline__ void literal_insert(const Geo& g, int beg, int c, dou
     }
 }
 
+// candidates with r^2 <= rc^2 (1 + kRcSlack) in this atom's frame: a superset of
+// every pair that either side of a pair can select (frames differ by rounding)
+constexpr double kRcSlack = 1e-9;
+
 template <int NMAX, int LMASK>
 __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     constexpr int NL = rec_nl(LMASK);
-    constexpr int QF = rec_qfloats(LMASK);
+    constexpr int QF = rec_used(LMASK);
     constexpr int RS = rec_stride(LMASK);
     constexpr int LMAX = lmax_of(LMASK);
     __shared__ double sh[NL * 4];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     double part[NL];
     #pragma unroll
     for (int k = 0; k < NL; ++k) part[k] = 0.0;
 
-    if ((int)threadIdx.x < tl.count) {
-        const int i = tl.start + threadIdx.x;
-        const bool center = g.group == nullptr || g.group[i];
-        const int beg = g.off[i];
-        const int c = center ? g.off[i + 1] - beg : 0;
-        const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
-        const double rc2 = e.rc2;
-        const int N = e.N;
-        uint64_t m0 = 0, m1 = 0;
-        int cnt = 0;
-        for (int p = 0; p < c; ++p) {
-            double d0, d1, d2;
-            double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
-            if ((r2 > rc2) || (r2 < 1e-12)) continue;
-            ++cnt;
-            if (p < 64) m0 |= 1ull << p; else m1 |= 1ull << (p - 64);
-        }
-        if (cnt > N) {
+    const bool center = valid && (g.group == nullptr || g.group[i]);
+    const int beg = g.off[i];
+    const int c = center ? g.off[i + 1] - beg : 0;
+    const double4 xi = g.x4[i];
+    const double rc2 = e.rc2, rc2s = e.rc2 * (1.0 + kRcSlack);
+    const int N = e.N;
+    M128 own{0, 0}, rcm{0, 0};
+    int cnt = 0;
+    for (int p = lane; p < c; p += kLanes) {
+        double d0, d1, d2;
+        double r2 = pair_r2(g.x4, g.idx[beg + p], xi.x, xi.y, xi.z, d0, d1, d2);
+        if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
+        if ((r2 > rc2) || (r2 < 1e-12)) continue;
+        ++cnt;
+        mset(own, p);
+    }
+    cnt = grp_sum_i(cnt);
+    own = grp_or(own);
+    rcm = grp_or(rcm);
+    if (cnt > N) {                                   // uniform within the group
+        if (lane == 0) {
             double sr2[NMAX];
             int sp[NMAX];
-            literal_insert<NMAX>(g, beg, c, xi0, xi1, xi2, rc2, N, sr2, sp);
-            m0 = 0; m1 = 0;
+            literal_insert<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
+            own = M128{0, 0};
             #pragma unroll
-            for (int s = 0; s < NMAX; ++s) {
-                if (s < N && sp[s] >= 0) {
-                    if (sp[s] < 64) m0 |= 1ull << sp[s]; else m1 |= 1ull << (sp[s] - 64);
-                }
-            }
+            for (int s = 0; s < NMAX; ++s)
+                if (s < N && sp[s] >= 0) mset(own, sp[s]);
         }
-        const int n = cnt < N ? cnt : N;
-        float q[QF];
-        #pragma unroll
-        for (int k = 0; k < QF; ++k) q[k] = 0.0f;
-        uint64_t a0 = m0, a1 = m1;
-        while (a0 | a1) {
-            int p;
-            if (a0) { p = __builtin_ctzll(a0); a0 &= a0 - 1; }
-            else    { p = 64 + __builtin_ctzll(a1); a1 &= a1 - 1; }
-            double d0, d1, d2;
-            double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
-            const float ir = __builtin_amdgcn_rsqf((float)r2);
-            const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
-            const float u2 = u * u;
-            float zr[LMAX + 1], zi[LMAX + 1];
-            zpow<LMAX>(nx, ny, zr, zi);
-            if constexpr (LMASK & kL3) acc_ylm<3>(u, u2, zr, zi, q + rec_off(LMASK, 3));
-            if constexpr (LMASK & kL4) acc_ylm<4>(u, u2, zr, zi, q + rec_off(LMASK, 4));
-            if constexpr (LMASK & kL6) acc_ylm<6>(u, u2, zr, zi, q + rec_off(LMASK, 6));
+        own = grp_bcast0(own);
+    }
+    const int n = cnt < N ? cnt : N;
+    float q[QF];
+    #pragma unroll
+    for (int k = 0; k < QF; ++k) q[k] = 0.0f;
+    M128 it = own;
+    mdrop(it, lane);
+    while (!mempty(it)) {
+        const int p = mpop(it);
+        double d0, d1, d2;
+        double r2 = pair_r2(g.x4, g.idx[beg + p], xi.x, xi.y, xi.z, d0, d1, d2);
+        const float ir = __builtin_amdgcn_rsqf((float)r2);
+        const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
+        const float u2 = u * u;
+        float zr[LMAX + 1], zi[LMAX + 1];
+        zpow<LMAX>(nx, ny, zr, zi);
+        if constexpr (LMASK & kL3) acc_ylm<3>(u, u2, zr, zi, q + rec_off(LMASK, 3));
+        if constexpr (LMASK & kL4) acc_ylm<4>(u, u2, zr, zi, q + rec_off(LMASK, 4));
+        if constexpr (LMASK & kL6) acc_ylm<6>(u, u2, zr, zi, q + rec_off(LMASK, 6));
+        mdrop(it, kLanes - 1);
+    }
+    const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
+    #pragma unroll
+    for (int k = 0; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
+    float out[RS];
+    #pragma unroll
+    for (int k = 0; k < RS; ++k) out[k] = k < QF ? q[k] : 0.0f;
+    const double inv_ng = e.inv_ng[tl.walker];
+    auto finish = [&](auto ll) {
+        constexpr int L = decltype(ll)::value;
+        if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
+            constexpr int o = rec_off(LMASK, L);
+            constexpr int li = rec_li(LMASK, L);
+            float s2 = q[o] * q[o];
+            #pragma unroll
+            for (int m = 1; m <= L; ++m) s2 += 2.0f * (q[o + 2 * m - 1] * q[o + 2 * m - 1] + q[o + 2 * m] * q[o + 2 * m]);
+            constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
+            const float ql = __builtin_sqrtf(s2 * P);
+            out[rec_a(LMASK, L)] = (ql > 0.0f && n > 0) ? (float)(P * inv_ng) / (ql * (float)n) : 0.0f;
+            if (valid && lane == 0) e.ql[(int64_t)li * g.n_local_total + i] = ql;
+            part[li] = (center && lane == 0) ? (double)ql : 0.0;
         }
-        const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
-        #pragma unroll
-        for (int k = 0; k < QF; ++k) q[k] *= inv_n;
-        float out[RS];
-        #pragma unroll
-        for (int k = 0; k < RS; ++k) out[k] = k < QF ? q[k] : 0.0f;
-        const double inv_ng = e.inv_ng[tl.walker];
-        auto finish = [&](auto ll) {
-            constexpr int L = decltype(ll)::value;
-            if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
-                constexpr int o = rec_off(LMASK, L);
-                constexpr int li = rec_li(LMASK, L);
-                float s2 = q[o] * q[o];
-                #pragma unroll
-                for (int m = 1; m <= L; ++m) s2 += 2.0f * (q[o + 2 * m - 1] * q[o + 2 * m - 1] + q[o + 2 * m] * q[o + 2 * m]);
-                constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
-                const float ql = __builtin_sqrtf(s2 * P);
-                out[QF + li] = (ql > 0.0f && n > 0) ? (float)(P * inv_ng) / (ql * (float)n) : 0.0f;
-                e.ql[(int64_t)li * g.n_local_total + i] = ql;
-                part[li] = center ? (double)ql : 0.0;
-            }
-        };
-        finish(std::integral_constant<int, 3>{});
-        finish(std::integral_constant<int, 4>{});
-        finish(std::integral_constant<int, 6>{});
+    };
+    finish(std::integral_constant<int, 3>{});
+    finish(std::integral_constant<int, 4>{});
+    finish(std::integral_constant<int, 6>{});
+    if (valid) {
         float4* dst = reinterpret_cast<float4*>(e.rec + (int64_t)i * RS);
         #pragma unroll
-        for (int k = 0; k < RS / 4; ++k) dst[k] = make_float4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
-        e.nsel[i] = n;
-        e.cnt_rc[i] = cnt;
-        reinterpret_cast<ulonglong2*>(e.selmask)[i] = make_ulonglong2(m0, m1);
+        for (int k = 0; k < RS / 4; ++k)
+            if (k % kLanes == lane) dst[k] = make_float4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+        if (lane == 0) {
+            e.nsel[i] = n;
+            e.cnt_rc[i] = cnt;
+            reinterpret_cast<ulonglong2*>(e.selmask)[2 * i] = make_ulonglong2(own.lo, own.hi);
+            reinterpret_cast<ulonglong2*>(e.selmask)[2 * i + 1] = make_ulonglong2(rcm.lo, rcm.hi);
+        }
     }
     block_sum<NL>(part, sh);
     if (threadIdx.x == 0) {
@@ -370,14 +448,14 @@ __global__ __launch_bounds__(kBlock) void k_select_literal(Geo g, int first, int
     const bool center = g.group == nullptr || g.group[i];
     const int beg = g.off[i];
     const int c = center ? g.off[i + 1] - beg : 0;
-    const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+    const double4 xi = g.x4[i];
     double sr2[NMAX];
     int sp[NMAX];
-    literal_insert<NMAX>(g, beg, c, xi0, xi1, xi2, rc2, N, sr2, sp);
+    literal_insert<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
     int cnt = 0;
     for (int p = 0; p < c; ++p) {
         double d0, d1, d2;
-        double r2 = pair_r2(g.x, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+        double r2 = pair_r2(g.x4, g.idx[beg + p], xi.x, xi.y, xi.z, d0, d1, d2);
         if ((r2 > rc2) || (r2 < 1e-12)) continue;
         ++cnt;
     }
@@ -386,6 +464,24 @@ __global__ __launch_bounds__(kBlock) void k_select_literal(Geo g, int first, int
         if (s < N) sel_idx[(int64_t)a * N + s] = sp[s] >= 0 ? g.idx[beg + sp[s]] : -1;
     cnt_rc[a] = cnt;
     nsel[a] = cnt < N ? cnt : N;
+}
+
+// Reverse-selected candidates of atom i: the candidate's owner selects i.
+// Checked only inside the slack-widened rc mask, distributed over the lanes.
+__device__ __forceinline__ M128 reverse_mask(const Geo& g, const EnvDev& e, int beg, const M128& rcm, int lane) {
+    M128 rev{0, 0};
+    M128 it = rcm;
+    mdrop(it, lane);
+    while (!mempty(it)) {
+        const int p = mpop(it);
+        const int rp = g.revpos[beg + p];
+        if (rp != 0xFF) {
+            const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * g.eown[beg + p]];
+            if (rp < 64 ? ((mo.x >> rp) & 1ull) : ((mo.y >> (rp - 64)) & 1ull)) mset(rev, p);
+        }
+        mdrop(it, kLanes - 1);
+    }
+    return grp_or(rev);
 }
 
 // ---------------------------------------------------------------------------
@@ -402,43 +498,44 @@ __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     double part[1] = {0.0};
-    if ((int)threadIdx.x < tl.count) {
-        const int i = tl.start + threadIdx.x;
-        const bool center = g.group == nullptr || g.group[i];
-        float qb[W];
-        const float* ri = e.rec + (int64_t)i * RS + O;
+    const bool center = valid && (g.group == nullptr || g.group[i]);
+    float qb[W];
+    #pragma unroll
+    for (int k = 0; k < W; ++k) qb[k] = 0.0f;
+    const ulonglong2 mk = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i];
+    M128 it{valid ? mk.x : 0ull, valid ? mk.y : 0ull};
+    const int beg = g.off[i];
+    mdrop(it, lane);
+    while (!mempty(it)) {
+        const int p = mpop(it);
+        const float4* ro = reinterpret_cast<const float4*>(e.rec + (int64_t)g.eown[beg + p] * RS + O);
         #pragma unroll
-        for (int k = 0; k < W; ++k) qb[k] = ri[k];
-        const ulonglong2 mk = reinterpret_cast<const ulonglong2*>(e.selmask)[i];
-        uint64_t a0 = mk.x, a1 = mk.y;
-        const int beg = g.off[i];
-        while (a0 | a1) {
-            int p;
-            if (a0) { p = __builtin_ctzll(a0); a0 &= a0 - 1; }
-            else    { p = 64 + __builtin_ctzll(a1); a1 &= a1 - 1; }
-            const int o = owner_of(g, g.idx[beg + p]);
-            const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)o * RS);
-            float r[RS];
-            #pragma unroll
-            for (int k = 0; k < RS / 4; ++k) {
-                float4 v = rp[k];
-                r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
-            }
-            #pragma unroll
-            for (int k = 0; k < W; ++k) qb[k] += r[O + k];
+        for (int k = 0; k < rec_blk(L) / 4; ++k) {
+            const float4 v = ro[k];
+            if (4 * k < W) qb[4 * k] += v.x;
+            if (4 * k + 1 < W) qb[4 * k + 1] += v.y;
+            if (4 * k + 2 < W) qb[4 * k + 2] += v.z;
+            if (4 * k + 3 < W) qb[4 * k + 3] += v.w;
         }
-        const int n = e.nsel[i];
-        const float inv = 1.0f / (float)(n + 1);
-        float s2 = 0.0f;
-        #pragma unroll
-        for (int k = 0; k < W; ++k) {
-            qb[k] *= inv;
-            s2 += (k == 0 ? 1.0f : 2.0f) * qb[k] * qb[k];
-        }
-        constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
-        const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
-        const float beta = lq > 0.0f ? (float)(P * e.inv_ng[tl.walker]) / (lq * (float)(n + 1)) : 0.0f;
+        mdrop(it, kLanes - 1);
+    }
+    const float* ri = e.rec + (int64_t)i * RS + O;
+    const int n = e.nsel[i];
+    const float inv = 1.0f / (float)(n + 1);
+    float s2 = 0.0f;
+    #pragma unroll
+    for (int k = 0; k < W; ++k) {
+        qb[k] = (grp_sum(qb[k]) + ri[k]) * inv;
+        s2 += (k == 0 ? 1.0f : 2.0f) * qb[k] * qb[k];
+    }
+    constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
+    const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
+    const float beta = lq > 0.0f ? (float)(P * e.inv_ng[tl.walker]) / (lq * (float)(n + 1)) : 0.0f;
+    if (valid && lane == 0) {
         d.lq[i] = lq;
         float* bq = d.bq + (int64_t)i * BS;
         #pragma unroll
@@ -459,59 +556,69 @@ __global__ __launch_bounds__(kBlock) void k_ld_coeff(Geo g, EnvDev e, LdDev d) {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
-    if ((int)threadIdx.x >= tl.count) return;
-    const int i = tl.start + threadIdx.x;
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     const double w = e.wcv[(int64_t)tl.walker * kMaxSlotCV + d.wslot];
+    const ulonglong2 mr = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i + 1];
+    const int beg = g.off[i];
+    M128 rcm{valid ? mr.x : 0ull, valid ? mr.y : 0ull};
+    M128 rev = reverse_mask(g, e, beg, rcm, lane);
     float C[BS];
-    const float4* bi = reinterpret_cast<const float4*>(d.bq + (int64_t)i * BS);
     #pragma unroll
-    for (int k = 0; k < BS / 4; ++k) { float4 v = bi[k]; C[4*k] = v.x; C[4*k+1] = v.y; C[4*k+2] = v.z; C[4*k+3] = v.w; }
-    const int beg = g.off[i], c = g.off[i + 1] - beg;
-    for (int p = 0; p < c; ++p) {
-        const int rp = g.revpos[beg + p];
-        if (rp == 0xFF) continue;
-        const int o = owner_of(g, g.idx[beg + p]);
-        const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[o];
-        const bool rev = rp < 64 ? ((mo.x >> rp) & 1ull) : ((mo.y >> (rp - 64)) & 1ull);
-        if (!rev) continue;
-        const float4* bo = reinterpret_cast<const float4*>(d.bq + (int64_t)o * BS);
+    for (int k = 0; k < BS; ++k) C[k] = 0.0f;
+    mdrop(rev, lane);
+    while (!mempty(rev)) {
+        const int p = mpop(rev);
+        const float4* bo = reinterpret_cast<const float4*>(d.bq + (int64_t)g.eown[beg + p] * BS);
         #pragma unroll
         for (int k = 0; k < BS / 4; ++k) { float4 v = bo[k]; C[4*k] += v.x; C[4*k+1] += v.y; C[4*k+2] += v.z; C[4*k+3] += v.w; }
+        mdrop(rev, kLanes - 1);
     }
+    const float4* bi = reinterpret_cast<const float4*>(d.bq + (int64_t)i * BS);
+    float B[BS];
+    #pragma unroll
+    for (int k = 0; k < BS / 4; ++k) { float4 v = bi[k]; B[4*k] = v.x; B[4*k+1] = v.y; B[4*k+2] = v.z; B[4*k+3] = v.w; }
     const int n = e.nsel[i];
     const float s = n > 0 ? (float)w / (float)n : 0.0f;
     float* ex = e.extra + (int64_t)i * RS + O;
     #pragma unroll
-    for (int k = 0; k < W; ++k) ex[k] = s * C[k];
+    for (int k = 0; k < W; ++k) {
+        const float v = s * (B[k] + grp_sum(C[k]));
+        if (valid && lane == 0) ex[k] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // k_force: F_k = sum_{j in S(k)} g_kj - sum_{(i,j): owner(j)=k, j in S(i)} g_ij,
 // g_ij = sum_lm mu_m Re(conj(K_ilm) grad Y_lm(x_j - x_i)),
 // K_i = w_l a_il q_il (+ extra_i).  Gather over k's own list: every candidate
-// that is selected by k (own) or selects k (rev, via the reverse position)
-// contributes once, with the two coefficient sets added before the contraction.
+// that k selects (own) or whose owner selects k (rev) contributes once, with
+// the two coefficient sets added before the contraction.  kLanes lanes per
+// atom share the active pairs round-robin.
 // ---------------------------------------------------------------------------
 template <int LMASK, bool EXTRA>
 __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* wl, float* K) {
     constexpr int RS = rec_stride(LMASK);
-    constexpr int QF = rec_qfloats(LMASK);
+    constexpr int NV = rec_used(LMASK) / 4;     // float4s actually used
     const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)a * RS);
-    float r[RS];
+    float r[NV * 4];
     #pragma unroll
-    for (int k = 0; k < RS / 4; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
-    float x[RS];
+    for (int k = 0; k < NV; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
+    float x[NV * 4];
     if constexpr (EXTRA) {
         const float4* xp = reinterpret_cast<const float4*>(e.extra + (int64_t)a * RS);
         #pragma unroll
-        for (int k = 0; k < RS / 4; ++k) { float4 v = xp[k]; x[4*k] = v.x; x[4*k+1] = v.y; x[4*k+2] = v.z; x[4*k+3] = v.w; }
+        for (int k = 0; k < NV; ++k) { float4 v = xp[k]; x[4*k] = v.x; x[4*k+1] = v.y; x[4*k+2] = v.z; x[4*k+3] = v.w; }
     }
+    #pragma unroll
+    for (int k = 0; k < NV * 4; ++k) K[k] = 0.0f;
     auto one = [&](auto ll) {
         constexpr int L = decltype(ll)::value;
         if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
             constexpr int o = rec_off(LMASK, L);
             constexpr int li = rec_li(LMASK, L);
-            const float s = wl[li] * r[QF + li];
+            const float s = wl[li] * r[rec_a(LMASK, L)];
             #pragma unroll
             for (int k = 0; k < 2 * L + 1; ++k) {
                 float v = s * r[o + k];
@@ -527,42 +634,41 @@ __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* 
 
 template <int LMASK, bool EXTRA>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __restrict__ f) {
-    constexpr int QF = rec_qfloats(LMASK);
+    constexpr int QF = rec_used(LMASK);
     constexpr int NL = rec_nl(LMASK);
     constexpr int LMAX = lmax_of(LMASK);
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
-    if ((int)threadIdx.x >= tl.count) return;
-    const int i = tl.start + threadIdx.x;
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     float wl[NL];
     #pragma unroll
     for (int k = 0; k < NL; ++k)
         wl[k] = e.wslot[k] >= 0 ? (float)e.wcv[(int64_t)tl.walker * kMaxSlotCV + e.wslot[k]] : 0.0f;
+    const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i];
+    const ulonglong2 mr = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i + 1];
+    const M128 own{valid ? mo.x : 0ull, valid ? mo.y : 0ull};
+    const M128 rcm{valid ? mr.x : 0ull, valid ? mr.y : 0ull};
+    const int beg = g.off[i];
+    const M128 rev = reverse_mask(g, e, beg, rcm, lane);
     float Ki[QF];
     load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
-    const ulonglong2 mk = reinterpret_cast<const ulonglong2*>(e.selmask)[i];
-    const int beg = g.off[i], c = g.off[i + 1] - beg;
-    const double xi0 = g.x[3 * i], xi1 = g.x[3 * i + 1], xi2 = g.x[3 * i + 2];
+    const double4 xi = g.x4[i];
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
-    for (int p = 0; p < c; ++p) {
-        const bool own = p < 64 ? ((mk.x >> p) & 1ull) : ((mk.y >> (p - 64)) & 1ull);
-        const int rp = g.revpos[beg + p];
+    M128 it{own.lo | rev.lo, own.hi | rev.hi};
+    mdrop(it, lane);
+    while (!mempty(it)) {
+        const int p = mpop(it);
+        const bool is_own = mbit(own, p), is_rev = mbit(rev, p);
         const int j = g.idx[beg + p];
-        bool rev = false;
-        int o = 0;
-        if (rp != 0xFF) {
-            o = owner_of(g, j);
-            const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[o];
-            rev = rp < 64 ? ((mo.x >> rp) & 1ull) : ((mo.y >> (rp - 64)) & 1ull);
-        }
-        if (!(own || rev)) continue;
         float C[QF];
         #pragma unroll
-        for (int k = 0; k < QF; ++k) C[k] = own ? Ki[k] : 0.0f;
-        if (rev) {
+        for (int k = 0; k < QF; ++k) C[k] = is_own ? Ki[k] : 0.0f;
+        if (is_rev) {
             float Ko[QF];
-            load_coeff<LMASK, EXTRA>(e, o, wl, Ko);
+            load_coeff<LMASK, EXTRA>(e, g.eown[beg + p], wl, Ko);
             // reverse pair enters with (-1)^l
             if constexpr (LMASK & kL3) {
                 #pragma unroll
@@ -578,7 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
             }
         }
         double d0, d1, d2;
-        double r2 = pair_r2(g.x, j, xi0, xi1, xi2, d0, d1, d2);
+        double r2 = pair_r2(g.x4, j, xi.x, xi.y, xi.z, d0, d1, d2);
         const float ir = __builtin_amdgcn_rsqf((float)r2);
         const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
         const float u2 = u * u;
@@ -592,10 +698,14 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
         Fx = __builtin_fmaf(ir, __builtin_fmaf(-dot, nx, gx), Fx);
         Fy = __builtin_fmaf(ir, __builtin_fmaf(-dot, ny, gy), Fy);
         Fz = __builtin_fmaf(ir, __builtin_fmaf(-dot, u, gz), Fz);
+        mdrop(it, kLanes - 1);
     }
-    f[3 * (int64_t)i] += (double)Fx;
-    f[3 * (int64_t)i + 1] += (double)Fy;
-    f[3 * (int64_t)i + 2] += (double)Fz;
+    Fx = grp_sum(Fx); Fy = grp_sum(Fy); Fz = grp_sum(Fz);
+    if (valid && lane == 0) {
+        f[3 * (int64_t)i] += (double)Fx;
+        f[3 * (int64_t)i + 1] += (double)Fy;
+        f[3 * (int64_t)i + 2] += (double)Fz;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -679,65 +789,83 @@ __device__ void grid_deposit_block(const ReduceDev& r, double* grid, const doubl
 }
 
 // Exact FP64 hills sum at s, block-cooperative; V and dV valid on thread 0.
-__device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh) {
+template <int NW>
+__device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh,
+                                   int64_t h_first = 0) {
     const double* H = r.hills + (int64_t)w * r.hills_cap * (r.D + 3);
     const int64_t nh = r.nhills[w];
     double acc[4] = {0, 0, 0, 0};
-    for (int64_t h = threadIdx.x; h < nh; h += blockDim.x) {
+    for (int64_t h = h_first + threadIdx.x; h < nh; h += blockDim.x) {
         const double* rec = H + h * (r.D + 3);
-        double ex = 0, dd[kMaxDim];
-        for (int d = 0; d < r.D; ++d) {
+        double ex = 0, dd[kMaxDim] = {0, 0, 0};
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) {
             dd[d] = s[d] - rec[1 + d];
             ex += dd[d] * dd[d] / (2.0 * r.sigma[d] * r.sigma[d]);
         }
         double t = rec[1 + r.D] * exp(-ex);
         acc[0] += t;
-        for (int d = 0; d < r.D; ++d) acc[1 + d] -= t * dd[d] / (r.sigma[d] * r.sigma[d]);
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc[1 + d] -= t * dd[d] / (r.sigma[d] * r.sigma[d]);
     }
-    block_sum<4>(acc, sh);
-    if (threadIdx.x == 0) for (int d = 0; d < r.D; ++d) dV[d] = acc[1 + d];
+    block_sum<4, NW>(acc, sh);
+    if (threadIdx.x == 0) {
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) dV[d] = acc[1 + d];
+    }
     __syncthreads();
     return acc[0];
 }
 
-// learned bias: 3 -> 64 -> 64 -> 1 tanh network, packed FP32 weights
-// [W1 64x3][b1 64][W2 64x64][b2 64][W3 64][b3]; thread 0 evaluates.
-constexpr int kH = 64;
-__device__ double mlp_eval(const ReduceDev& r, const double* s, double* dV) {
-    const float* W1 = r.mlp;
-    const float* b1 = W1 + kH * 3;
-    const float* W2 = b1 + kH;
-    const float* b2 = W2 + kH * kH;
-    const float* W3 = b2 + kH;
-    const float b3 = W3[kH];
-    float in[3];
-    for (int d = 0; d < 3; ++d) in[d] = (float)((s[d] - r.mlp_shift_in[d]) * r.mlp_scale_in[d]);
-    float h1[kH], h2[kH], g2[kH];
-    for (int a = 0; a < kH; ++a) {
-        float z = b1[a];
-        for (int d = 0; d < 3; ++d) z += W1[a * 3 + d] * in[d];
-        h1[a] = tanhf(z);
+// Learned bias network at one point, block-cooperative (threads 0..63 are the
+// hidden units), FP32 master weights (the MFMA kernels carry W2 as BF16 hi + lo,
+// the same function to ~1e-5).  V and dV/ds valid on thread 0.
+__device__ double mlp_eval_block(const MlpNet* net, const double* s, double* dV) {
+    __shared__ float h1s[kMlpH], g2s[kMlpH], ys[kMlpH], gis[3][kMlpH];
+    const int a = threadIdx.x;
+    float x[3];
+    #pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = (float)((s[d] - net->in_shift[d]) * net->in_scale[d]);
+    if (a < kMlpH) {
+        float z = net->p[kB1 + a];
+        #pragma unroll
+        for (int d = 0; d < 3; ++d) z = __builtin_fmaf(net->p[kW1 + 3 * a + d], x[d], z);
+        h1s[a] = tanhf(z);
     }
-    float v = b3;
-    for (int a = 0; a < kH; ++a) {
-        float z = b2[a];
-        for (int b = 0; b < kH; ++b) z += W2[a * kH + b] * h1[b];
-        h2[a] = tanhf(z);
-        v += W3[a] * h2[a];
-        g2[a] = W3[a] * (1.0f - h2[a] * h2[a]);
+    __syncthreads();
+    if (a < kMlpH) {
+        float z = net->p[kB2 + a];
+        for (int b = 0; b < kMlpH; ++b) z = __builtin_fmaf(net->p[kW2 + a * kMlpH + b], h1s[b], z);
+        const float h = tanhf(z);
+        ys[a] = net->p[kW3 + a] * h;
+        g2s[a] = net->p[kW3 + a] * (1.0f - h * h);
     }
-    float gi[3] = {0, 0, 0};
-    for (int b = 0; b < kH; ++b) {
-        float t = 0;
-        for (int a = 0; a < kH; ++a) t += g2[a] * W2[a * kH + b];
-        t *= (1.0f - h1[b] * h1[b]);
-        for (int d = 0; d < 3; ++d) gi[d] += t * W1[b * 3 + d];
+    __syncthreads();
+    if (a < kMlpH) {
+        float t = 0.0f;
+        for (int c = 0; c < kMlpH; ++c) t = __builtin_fmaf(g2s[c], net->p[kW2 + c * kMlpH + a], t);
+        t *= 1.0f - h1s[a] * h1s[a];
+        #pragma unroll
+        for (int d = 0; d < 3; ++d) gis[d][a] = t * net->p[kW1 + 3 * a + d];
     }
-    for (int d = 0; d < 3; ++d) dV[d] = (double)gi[d] * r.mlp_scale_in[d] / r.mlp_scale_out;
-    return (double)v / r.mlp_scale_out + r.mlp_shift_out;
+    __syncthreads();
+    double V = 0.0;
+    if (a == 0) {
+        float y = net->p[kB3];
+        for (int c = 0; c < kMlpH; ++c) y += ys[c];
+        for (int d = 0; d < 3; ++d) {
+            float gsum = 0.0f;
+            for (int c = 0; c < kMlpH; ++c) gsum += gis[d][c];
+            dV[d] = (double)gsum * net->in_scale[d] * net->out_scale;
+        }
+        V = (double)y * net->out_scale + net->out_shift;
+    }
+    __syncthreads();
+    return V;
 }
 
 // V(s) and dV/ds of walker w's bias, block-cooperative; valid on thread 0.
+template <int NW>
 __device__ double bias_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh) {
     if (r.kind == BIAS_GRID) {
         double v = 0.0;
@@ -745,11 +873,18 @@ __device__ double bias_eval_block(const ReduceDev& r, int w, const double* s, do
         return v;
     }
     if (r.kind == BIAS_MLP && r.mlp_ok[w]) {
-        double v = 0.0;
-        if (threadIdx.x == 0) v = mlp_eval(r, s, dV);
-        return v;
+        // network for the fitted hills + exact sum of the hills deposited since
+        const MlpNet* net = r.mlp + w;
+        double dn[kMaxDim] = {0, 0, 0};
+        const double vn = mlp_eval_block(net, s, dn);
+        const double vh = hills_eval_block<NW>(r, w, s, dV, sh, net->n_fit);
+        if (threadIdx.x == 0) {
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) dV[d] += dn[d];
+        }
+        return vn + vh;
     }
-    return hills_eval_block(r, w, s, dV, sh);
+    return hills_eval_block<NW>(r, w, s, dV, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -757,8 +892,8 @@ __device__ double bias_eval_block(const ReduceDev& r, int w, const double* s, do
 // ---------------------------------------------------------------------------
 struct Dual { double v; double d[kMaxSym]; };
 
-__device__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double* val, double* dval) {
-    Dual st[kStack];
+// st: kStack entries of scratch (LDS in k_reduce: an interpreter stack is dynamically indexed)
+__device__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double* val, double* dval, Dual* st) {
     int sp = 0;
     for (int k = 0; k < p.n; ++k) {
         const int op = p.op[k];
@@ -769,8 +904,8 @@ __device__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double
             else { int q = (int)p.c[k]; a.v = sym[q]; a.d[q] = 1.0; }
             continue;
         }
-        if (op >= OP_ADD && op <= OP_POW || op == OP_MIN || op == OP_MAX) {
-            Dual b = st[--sp];
+        if ((op >= OP_ADD && op <= OP_POW) || op == OP_MIN || op == OP_MAX) {
+            const Dual& b = st[--sp];
             Dual& a = st[sp - 1];
             double av = a.v, bv = b.v, da = 0, db = 0, rv = 0;
             switch (op) {
@@ -819,30 +954,52 @@ __device__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double
 // CVs -> expression -> deposit on step % pace == 0 (WT height from the bias
 // before the deposit) -> V and dV/ds after the deposit -> chain rule.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_reduce(ReduceDev r) {
+constexpr int kRedThreads = 1024;
+__global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     const int w = blockIdx.x;
-    __shared__ double sh[64];
+    __shared__ double sh[kMaxSlotCV * (kRedThreads / 64)];
+    __shared__ ExprProg progs[kMaxDim];
     __shared__ double cv[kMaxSlotCV];
     __shared__ double s[kMaxDim];
     __shared__ double dsdv[kMaxDim * kMaxSym];
     __shared__ double bc[8];
+    __shared__ Dual stack[kStack];
+    __shared__ double symv[kMaxSym];
     const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
-    // 1. CV values (deterministic FP64 tree over tiles)
-    for (int c = 0; c < r.ncv; ++c) {
-        double v[1] = {0.0};
-        const int src = r.cv_src[c];
-        if (src >= 0) {
-            for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v[0] += r.partial[(int64_t)src * r.ntiles + t];
-        }
-        block_sum<1>(v, sh);
-        if (threadIdx.x == 0) cv[c] = src >= 0 ? v[0] * r.inv_ng[w] : r.direct[(int64_t)w * kMaxSlotCV + c];
-        __syncthreads();
+    // expression programs to LDS (read serially by thread 0 below)
+    {
+        const int nwords = r.D * (int)(sizeof(ExprProg) / 4);
+        const int* src = reinterpret_cast<const int*>(r.prog);
+        int* dst = reinterpret_cast<int*>(progs);
+        for (int k = threadIdx.x; k < nwords; k += blockDim.x) dst[k] = src[k];
     }
-    // 2. dimensions
+    // 1. CV values: one deterministic FP64 tree over tiles for every slot,
+    //    four independent loads in flight per thread and slot
+    double v[kMaxSlotCV];
+    #pragma unroll
+    for (int c = 0; c < kMaxSlotCV; ++c) v[c] = 0.0;
+    constexpr int U = 4;
+    for (int t = t0 + threadIdx.x; t < t1; t += U * kRedThreads) {
+        #pragma unroll
+        for (int c = 0; c < kMaxSlotCV; ++c) {
+            const int src = c < r.ncv ? r.cv_src[c] : -1;
+            if (src >= 0) {
+                const double* pp = r.partial + (int64_t)src * r.ntiles;
+                double x[U];
+                #pragma unroll
+                for (int u = 0; u < U; ++u) x[u] = (t + u * kRedThreads < t1) ? pp[t + u * kRedThreads] : 0.0;
+                v[c] += (x[0] + x[1]) + (x[2] + x[3]);
+            }
+        }
+    }
+    block_sum<kMaxSlotCV, kRedThreads / 64>(v, sh);
     if (threadIdx.x == 0) {
-        double sym[kMaxSym];
-        for (int q = 0; q < r.nsym; ++q) sym[q] = cv[r.sym_slot[q]];
-        for (int d = 0; d < r.D; ++d) expr_eval(r.prog[d], r.nsym, sym, &s[d], &dsdv[d * kMaxSym]);
+        #pragma unroll
+        for (int c = 0; c < kMaxSlotCV; ++c)
+            if (c < r.ncv) cv[c] = r.cv_src[c] >= 0 ? v[c] * r.inv_ng[w] : r.direct[(int64_t)w * kMaxSlotCV + c];
+        // 2. dimensions
+        for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
+        for (int d = 0; d < r.D; ++d) expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
     }
     __syncthreads();
     // 3. deposit
@@ -851,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceDev r) {
     double dV[kMaxDim] = {0, 0, 0};
     if (dep) {
         double Vb = 0.0;
-        if (r.wt) Vb = bias_eval_block(r, w, s, dV, sh);
+        if (r.wt) Vb = bias_eval_block<kRedThreads / 64>(r, w, s, dV, sh);
         if (threadIdx.x == 0) bc[0] = r.h0 * exp(-(Vb) / (r.kT * (r.biasf - 1.0)));
         __syncthreads();
         const double hgt = bc[0];
@@ -870,26 +1027,24 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceDev r) {
         __syncthreads();
     }
     // 4. bias value and gradient after the deposit
-    const double V = bias_eval_block(r, w, s, dV, sh);
-    double Vx = 0.0;
-    if (r.kind == BIAS_MLP) {
-        double dx[kMaxDim];
-        Vx = hills_eval_block(r, w, s, dx, sh);
-    }
+    const double V = bias_eval_block<kRedThreads / 64>(r, w, s, dV, sh);
     if (threadIdx.x == 0) {
         double* st = r.state + (int64_t)w * kStStride;
         for (int c = 0; c < r.ncv; ++c) st[kStCv + c] = cv[c];
-        for (int d = 0; d < r.D; ++d) { st[kStS + d] = s[d]; st[kStDV + d] = dV[d]; }
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { st[kStS + d] = s[d]; st[kStDV + d] = dV[d]; }
         st[kStV] = V;
         st[kStHill] = dep ? bc[0] : 0.0;
         st[kStStep] = (double)step;
-        st[kStVexact] = Vx;
+        st[kStMlpErr] = r.kind == BIAS_MLP ? r.mlp_err[w] : 0.0;
+        st[kStMlpOk] = r.kind == BIAS_MLP ? (double)r.mlp_ok[w] : 0.0;
         // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv
         double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
         for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = 0.0;
         for (int q = 0; q < r.nsym; ++q) {
             double acc = 0.0;
-            for (int d = 0; d < r.D; ++d) acc += dV[d] * dsdv[d * kMaxSym + q];
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc += dV[d] * dsdv[d * kMaxSym + q];
             wc[r.sym_slot[q]] += acc;
         }
         r.step[w] = step + 1;
@@ -902,7 +1057,7 @@ __global__ void k_bias_eval(ReduceDev r, int w, int64_t np, const double* s, dou
         double sp[kMaxDim];
         for (int d = 0; d < r.D; ++d) sp[d] = s[p * r.D + d];
         double g[kMaxDim] = {0, 0, 0};
-        double v = bias_eval_block(r, w, sp, g, sh);
+        double v = bias_eval_block<4>(r, w, sp, g, sh);
         if (threadIdx.x == 0) {
             if (V) V[p] = v;
             if (dV) for (int d = 0; d < r.D; ++d) dV[p * r.D + d] = g[d];
@@ -930,6 +1085,26 @@ __global__ void k_bias_add(ReduceDev r, int w, int64_t nh, const double* s, cons
         }
         __syncthreads();
     }
+}
+
+// Virial partials of a force buffer: sum_i F_i (x) unwrap(x_i), components
+// xx yy zz xy xz yz (the tally of FixMetadynamics::post_force).
+__global__ __launch_bounds__(kBlock) void k_virial(const double4* __restrict__ x4, const double* __restrict__ f,
+                                                   const int32_t* __restrict__ image, int first, int n,
+                                                   double L0, double L1, double L2, double* __restrict__ out) {
+    __shared__ double sh[6 * 4];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (a < n) {
+        const int i = first + a;
+        const double4 p = x4[i];
+        double u0 = p.x, u1 = p.y, u2 = p.z;
+        if (image) { u0 += image[3 * a] * L0; u1 += image[3 * a + 1] * L1; u2 += image[3 * a + 2] * L2; }
+        const double fx = f[3 * (int64_t)i], fy = f[3 * (int64_t)i + 1], fz = f[3 * (int64_t)i + 2];
+        v[0] = fx * u0; v[1] = fy * u1; v[2] = fz * u2; v[3] = fx * u1; v[4] = fx * u2; v[5] = fy * u2;
+    }
+    block_sum<6>(v, sh);
+    if (threadIdx.x == 0) for (int k = 0; k < 6; ++k) out[6 * blockIdx.x + k] = v[k];
 }
 
 __global__ void k_fill(double* p, int64_t n, double v) {
@@ -968,7 +1143,8 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
     for (int a = threadIdx.x; a < count; a += blockDim.x) {
         const int i = start + a;
         if (g.group && !g.group[i]) continue;
-        const double x0 = g.x[3 * i], x1 = g.x[3 * i + 1], x2 = g.x[3 * i + 2];
+        const double4 xv = g.x4[i];
+        const double x0 = xv.x, x1 = xv.y, x2 = xv.z;
         #pragma unroll
         for (int q = 0; q < kSkTile; ++q) {
             const double ph = x0 * kt[3 * q] + x1 * kt[3 * q + 1] + x2 * kt[3 * q + 2];
@@ -1023,7 +1199,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
     const int i = sk.atom_begin[w] + a;
     const bool act = a < n && (g.group == nullptr || g.group[i]);
     double x0 = 0, x1 = 0, x2 = 0;
-    if (act) { x0 = g.x[3 * i]; x1 = g.x[3 * i + 1]; x2 = g.x[3 * i + 2]; }
+    if (act) { const double4 xv = g.x4[i]; x0 = xv.x; x1 = xv.y; x2 = xv.z; }
     float gx = 0, gy = 0, gz = 0;
     for (int kb = 0; kb < sk.nk; kb += kSkFT) {
         __syncthreads();
@@ -1078,9 +1254,26 @@ hipError_t env_dispatch_l(const Geo& g, const EnvDev& e, dim3 grid, hipStream_t 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_revpos(const Geo& g, int64_t n, uint8_t* revpos, hipStream_t st) {
+hipError_t launch_topology(const Geo& g, int64_t n, int32_t* eown, uint8_t* revpos, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_revpos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, revpos);
+    hipLaunchKernelGGL(k_topology, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, eown, revpos);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_x4(const double* x, double4* x4, int64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    int64_t b = (n + 255) / 256; if (b > 8192) b = 8192;
+    hipLaunchKernelGGL(k_pack_x4, dim3((unsigned)b), dim3(256), 0, st, x, x4, n);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    int64_t b = (n + 255) / 256; if (b > 8192) b = 8192;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)b), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 
@@ -1170,7 +1363,15 @@ hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st
 
 hipError_t launch_reduce(const ReduceDev& r, hipStream_t st) {
     if (r.n_walkers == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3(r.n_walkers), dim3(kBlock), 0, st, r);
+    hipLaunchKernelGGL(k_reduce, dim3(r.n_walkers), dim3(kRedThreads), 0, st, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_virial(const double4* x4, const double* f, const int32_t* image, int first, int n,
+                         const double L[3], double* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_virial, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, x4, f, image, first, n,
+                       L[0], L[1], L[2], out);
     return hipGetLastError();
 }
 

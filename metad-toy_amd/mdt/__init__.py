@@ -65,10 +65,16 @@ def lib():
         "mdt_fix_selection": (I, [P, I, C.c_char_p, ip, ip, ip]),
         "mdt_fix_get_forces": (I, [P, I, dp]),
         "mdt_fix_zero_forces": (I, [P]),
+        "mdt_fix_virial": (I, [P, I, ip, dp]),
+        "mdt_fix_mlp_refit": (I, [P, I]),
+        "mdt_fix_mlp_status": (I, [P, I, dp, ip, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)]),
+        "mdt_fix_mlp_params": (I, [P, I, I, C.POINTER(C.c_float), dp]),
         "mdt_fix_bias_eval": (I, [P, I, I64, dp, dp, dp]),
         "mdt_fix_bias_add_hills": (I, [P, I, I64, dp, dp, D]),
         "mdt_fix_grid": (I, [P, I, dp, I64]),
         "mdt_fix_hills": (I, [P, I, C.POINTER(I64), dp, I64]),
+        "mdt_mlp_forward": (I, [P, C.POINTER(C.c_float), dp, I64, dp, dp]),
+        "mdt_mlp_loss_grad": (I, [P, C.POINTER(C.c_float), dp, I64, dp, dp, C.POINTER(C.c_float)]),
         "mdt_system_create": (I, [C.POINTER(P)]),
         "mdt_system_destroy": (None, [P]),
         "mdt_system_error": (C.c_char_p, [P]),
@@ -214,6 +220,31 @@ class Context:
         self._chk(lib().mdt_update_positions(self._h, walker, x.ctypes.data, 0))
 
 
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(ctx, params, norm, s):
+    """Network V at points s (n,3) on the MFMA inference kernel."""
+    p = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float64)
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    V = np.zeros(s.shape[0])
+    ctx._chk(lib().mdt_mlp_forward(ctx._h, _fp(p), _dp(nm), s.shape[0], _dp(s), _dp(V)))
+    return V
+
+
+def mlp_loss_grad(ctx, params, norm, s, v):
+    """Gradient of (1/n) sum (y - v/out_scale)^2 over the 4481 parameters (MFMA training kernel)."""
+    p = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float64)
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, np.float64)
+    g = np.zeros(4481, np.float32)
+    ctx._chk(lib().mdt_mlp_loss_grad(ctx._h, _fp(p), _dp(nm), s.shape[0], _dp(s), _dp(v), _fp(g)))
+    return g
+
+
 class Fix:
     """``fix metad`` on every walker of a context."""
 
@@ -299,6 +330,30 @@ class Fix:
         f = np.zeros((n_local, 3))
         self._chk(lib().mdt_fix_get_forces(self._h, walker, _dp(f)))
         return f
+
+    def virial(self, walker=0, image=None):
+        v = np.zeros(6)
+        img = None if image is None else np.ascontiguousarray(image, np.int32)
+        self._chk(lib().mdt_fix_virial(self._h, walker, _ip(img), _dp(v)))
+        return v
+
+    def mlp_refit(self, walker=0):
+        self._chk(lib().mdt_fix_mlp_refit(self._h, walker))
+
+    def mlp_status(self, walker=0):
+        err = C.c_double()
+        ok = C.c_int32()
+        fits, fb, nfit = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(lib().mdt_fix_mlp_status(self._h, walker, C.byref(err), C.byref(ok), C.byref(fits),
+                                           C.byref(fb), C.byref(nfit)))
+        return {"err": err.value, "ok": ok.value, "fits": fits.value, "fallbacks": fb.value, "n_fit": nfit.value}
+
+    def mlp_params(self, walker=0, live=True):
+        p = np.zeros(4481, np.float32)
+        norm = np.zeros(8)
+        self._chk(lib().mdt_fix_mlp_params(self._h, walker, 1 if live else 0,
+                                           p.ctypes.data_as(C.POINTER(C.c_float)), _dp(norm)))
+        return p, norm
 
     def zero_forces(self):
         self._chk(lib().mdt_fix_zero_forces(self._h))

@@ -82,6 +82,14 @@ def test_q_values_and_forces_c1(mdt, orc):
     assert err <= F_TOL
     # translation invariance of the exact gradient
     assert np.abs(f.sum(axis=0)).max() <= 1e-4 * np.abs(f).max() * np.sqrt(n)
+    # virial of the bias forces, the reference tally sum F (x) unwrap(x)
+    img = np.random.default_rng(5).integers(-2, 3, size=(n, 3)).astype(np.int32)
+    L = s.box()[1] - s.box()[0]
+    u = x[:n] + img * L
+    ref = np.array([f[:, 0] @ u[:, 0], f[:, 1] @ u[:, 1], f[:, 2] @ u[:, 2],
+                    f[:, 0] @ u[:, 1], f[:, 0] @ u[:, 2], f[:, 1] @ u[:, 2]])
+    vir = fix.virial(0, img)
+    assert np.abs(vir - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
 def test_forces_when_cap_binds(mdt, orc):
