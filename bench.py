@@ -1,20 +1,23 @@
-"""Benchmark: atom-steps/s of the Q6+Q4 metadynamics CV+bias+force path at 256,000 atoms.
+"""Benchmark of the metadynamics CV+bias+force path on one MI355X.
 
-Workload (config 2): 40x40x40 FCC cells of Ar (a = 5.26 A, 256,000 atoms), lower half
-jittered by sigma = 0.6 A ("melted"), upper half by 0.1 A.  CVs Q6.AVE, Q4.AVE and the
-Lechner-Dellago local Q6 (N = 18 nearest within 3.9 A, list cutoff 5.9 A); DIM 1 on
-"v1+v2" over [0,2] with 2000 bins, well-tempered (biasf 10), sigma 0.01, height 0.005,
-PACE 10.  A step = selection + q_lm + local Q + FP64 reductions + hill deposit/bias
-evaluation + chain rule + exact forces for every atom, one HIP graph launch per step.
-Inputs are resident in HBM before the timed region; MD integration and neighbour-list
-builds are outside the path and not timed.
+Default (the headline metric, config 2): atom-steps/s of Q6+Q4 CV+bias forces at 256,000
+atoms.  40x40x40 FCC cells of Ar (a = 5.26 A), lower half jittered by sigma = 0.6 A
+("melted"), upper half by 0.1 A; CVs Q6.AVE, Q4.AVE and the Lechner-Dellago local Q6
+(N = 18 nearest within 3.9 A, list cutoff 5.9 A); DIM 1 on "v1+v2" over [0,2] with 2000
+bins, well-tempered (biasf 10), sigma 0.01, height 0.005, PACE 10.  A step = selection +
+q_lm + local Q + FP64 reductions + hill deposit/bias evaluation + chain rule + exact
+forces, one HIP graph launch per step.  Inputs are resident in HBM before the timed
+region; MD integration and neighbour-list builds are outside the path and not timed.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other workloads (each prints its own JSON line): --config c1 (4,000-atom CPU reference
+case), c3 (500,000-atom structure factor), c4 (64 walkers), c5 (3-D learned bias vs exact
+hills sum).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -25,49 +28,321 @@ PKG = os.path.join(ROOT, "metad-toy_amd")
 sys.path.insert(0, PKG)
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# algorithmic work per selected pair for {Q4, Q6} (FMA = 2 FLOP, transcendental = 1)
-FLOP_FWD_PAIR = 261.0
-FLOP_GRAD_PAIR = 903.0
-FLOP_LD_FWD_PAIR = 28.0
-
-FIX_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE "
-            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
-            "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
-            "CAL NAME LQ6 STEINH LD 6 all cutoff_r 3.9 cutoff_Natoms 18 "
-            "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE SYMBOL v3 LQ6.AVE "
-            "DIM 1 0 2 2000 v1+v2 WT 1")
-NCELL, A, RC, RG = 40, 5.26, 3.9, 5.9
+HBM_PEAK_GBS = 8000.0
+# algorithmic work (FMA = 2 FLOP, transcendental = 1)
+FLOP_FWD_PAIR = 261.0        # q_lm of {Q4, Q6} per selected pair
+FLOP_GRAD_PAIR = 903.0       # gradient of {Q4, Q6} per selected pair
+FLOP_LD_FWD_PAIR = 28.0      # local Q6 average per selected pair
+FLOP_SK = 24.0               # structure factor per (atom, k): 10 forward + 14 gradient
+KB = 8.617343e-5
+A, RC, RG = 5.26, 3.9, 5.9
+METRIC = "atom-steps/s, Q6+Q4 CV+bias forces @256k atoms; NN bias vs hills @<=0.1 kT"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_system(mdt):
-    L = NCELL * A
-    s = mdt.System().add_fcc(NCELL, NCELL, NCELL, A, 0.6, 0.1, L / 2.0, seed=2024)
-    s.build(RG)
-    return s
+def steinh(name, l, kind="Q", N=18):
+    return f"CAL NAME {name} STEINH {kind} {l} all cutoff_r {RC} cutoff_Natoms {N} "
 
 
-def cpu_baseline(system, budget_s=30.0):
-    """FP64 oracle on the same 256k-atom system, one full step (1 host core)."""
-    from oracle import oracle as orc
-    x, off, idx, own = system.arrays()
-    n = system.n_local
+C2_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE " + steinh("Q6", 6) + steinh("Q4", 4) +
+           steinh("LQ6", 6, "LD") + "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE SYMBOL v3 LQ6.AVE "
+           "DIM 1 0 2 2000 v1+v2 WT 1")
+
+
+def c2_system(mdt):
+    n = 40
+    s = mdt.System().add_fcc(n, n, n, A, 0.6, 0.1, n * A / 2.0, seed=2024)
+    return s.build(RG)
+
+
+def timed_run(fix, step0, steps):
+    fix.sync()
     t0 = time.perf_counter()
-    env = orc.Env(x, off, idx, own, n, RC, 18)
-    q6 = env.q_ave(6)
-    q4 = env.q_ave(4)
-    _, lq = env.ld(6)
-    F = -(env.dQ_dx(6, 1.0) + env.dQ_dx(4, 1.0))
-    dt = time.perf_counter() - t0
-    del F, lq, q6, q4
-    return {"value": n * 1.0 / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 full step of the same {n}-atom workload (selection, Q6, Q4, LQ6, exact "
-                      f"Q6+Q4 forces) by the FP64 C++ oracle, {dt:.1f} s"}
+    fix.run(step0, steps)
+    fix.sync()
+    return time.perf_counter() - t0
+
+
+def oracle_steps(fn, budget_s):
+    """Run fn() until budget_s of CPU time is used (at least once); returns (count, seconds)."""
+    k, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        k += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            return k, dt
+
+
+def steinhardt_roofline(fix, system, kms):
+    """Roofline of the dominant per-atom kernel from the algorithmic work per launch."""
+    n = system.n_local
+    nsel = fix.extract("Q6.nsel", n)
+    pairs = float(nsel.sum())
+    x, off, idx, own = system.arrays()
+    c = np.diff(off).astype(np.float64)
+    n_ghost = x.shape[0] - n
+    bytes_sel = float(np.sum(24 + 4 * (c + 1) + 4 * nsel + 4) + 24 * n_ghost)
+    work = {"k_env": FLOP_FWD_PAIR * pairs, "k_force": FLOP_GRAD_PAIR * pairs, "k_ld": FLOP_LD_FWD_PAIR * pairs}
+    dom = max((k for k in kms if k in work), key=lambda k: kms[k])
+    t = kms[dom] * 1e-3
+    rate = work[dom] / t / 1e12
+    per = {"k_env": FLOP_FWD_PAIR, "k_force": FLOP_GRAD_PAIR, "k_ld": FLOP_LD_FWD_PAIR}[dom]
+    roof = {"kernel": dom, "bound": "fp32", "achieved": rate, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": rate / FP32_PEAK_TFLOPS, "traffic": None,
+            "algorithmic": f"{per:.0f} FLOP x {pairs:.0f} selected pairs per launch", "mean_ms": kms[dom],
+            "selection_hbm_GBs": bytes_sel / (kms["k_env"] * 1e-3) / 1e9 if "k_env" in kms else None,
+            "selection_bytes": bytes_sel}
+    tp = os.path.join(ROOT, "profiles", f"traffic_{dom}_c2.json")
+    if os.path.exists(tp):
+        roof["traffic"] = json.load(open(tp)).get("bytes_per_launch")
+    return roof, pairs
+
+
+def line(value, steps, warmup, ms, dtype, config, roof, cpu, metric=METRIC, unit="atom-steps/s"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic (seeded lattices / uniform liquids, see config)",
+            "config": config, "roofline": roof, "cpu_baseline": cpu}
+
+
+# ------------------------------------------------------------------ config 2 (headline)
+def run_c2(mdt, args):
+    log("config 2: building 256k-atom system")
+    system = c2_system(mdt)
+    n = system.n_local
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, system)
+    fix = mdt.Fix(ctx, C2_ARGS)
+    fix.init()
+    fix.run(0, args.warmup)
+    dt = timed_run(fix, args.warmup, args.steps)
+    value = n * args.steps / dt
+    log(f"c2: {dt * 1e6 / args.steps:.1f} us/step, {value:.3e} atom-steps/s")
+    kms = fix.profile(args.warmup + args.steps, args.profile_steps)
+    log("kernel mean ms: " + json.dumps(kms))
+    roof, pairs = steinhardt_roofline(fix, system, kms)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        x, off, idx, own = system.arrays()
+
+        def one():
+            env = orc.Env(x, off, idx, own, n, RC, 18)
+            env.q_ave(6), env.q_ave(4), env.ld(6)
+            -(env.dQ_dx(6, 1.0) + env.dQ_dx(4, 1.0))
+        k, t = oracle_steps(one, args.cpu_budget)
+        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{k} full step(s) of the same {n}-atom workload (selection, Q6, Q4, LQ6, exact "
+                         f"Q6+Q4 forces) by the FP64 C++ oracle on 1 core, {t:.1f} s"}
+    cfg = {"workload": "config 2: 256,000-atom FCC Ar half melted, Q6.AVE+Q4.AVE+LD-LQ6 (N=18, rc=3.9), "
+                       "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms}
+    return line(value, args.steps, args.warmup, dt * 1e3 / args.steps, "f32 pair math, f64 sums/grid", cfg, roof, cpu)
+
+
+# ------------------------------------------------------------------ config 1
+def run_c1(mdt, args):
+    s = mdt.System().add_fcc(10, 10, 10, A, 0.1, 0.1, 0.0, seed=1).build(RG)
+    n = s.n_local
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE TEMP 300 " + steinh("Q6", 6) +
+                  "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0")
+    fix.init()
+    fix.run(0, args.warmup)
+    steps = 10000
+    dt = timed_run(fix, args.warmup, steps)
+    value = n * steps / dt
+    kms = fix.profile(args.warmup + steps, args.profile_steps)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        x, off, idx, own = s.arrays()
+
+        def one():
+            env = orc.Env(x, off, idx, own, n, RC, 18)
+            env.q_ave(6)
+            env.dQ_dx(6, 1.0)
+        k, t = oracle_steps(one, min(args.cpu_budget, 10.0))
+        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{k} steps of the 4,000-atom Q6 case (selection, Q6.AVE, exact forces), FP64 oracle, {t:.1f} s"}
+    cfg = {"workload": "config 1: 4,000-atom FCC Ar (sigma 0.1 A), Q6.AVE on [0,1]x1000, PACE 10, WT 0, "
+                       "10,000 steps", "atoms": n, "kernels_ms": kms}
+    return line(value, steps, args.warmup, dt * 1e3 / steps, "f32 pair math, f64 sums/grid", cfg, None, cpu,
+                metric="atom-steps/s, config 1 (4,000 atoms, Q6.AVE)")
+
+
+# ------------------------------------------------------------------ config 3
+def sk_kvecs(nrand=2048, seed=31):
+    k0 = 2 * np.pi / A
+    ks = [(sx * k0, sy * k0, sz * k0) for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+    return ks, f"FCC111 {A} NRAND {nrand} KMIN 1.5 KMAX 3.5 SEED {seed}"
+
+
+def run_c3(mdt, args):
+    L = 52 * A
+    s = mdt.System().set_box([0, 0, 0], [L, L, L]).add_fcc(52, 52, 26, A, 0.0, 0.0, 0.0, seed=3)
+    n_fcc = 4 * 52 * 52 * 26
+    s.add_uniform(500000 - n_fcc, L / 2, L, seed=4).build(RG)
+    n = s.n_local
+    _, kspec = sk_kvecs()
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    fix = mdt.Fix(ctx, f"GAUSSIAN 0.5 0.005 10 PACE 10 HILLS_FILE NONE CAL NAME SK SK all {kspec} "
+                       "CV_dim 1 SYMBOL v1 SK.AVE DIM 1 0 50 1000 v1 WT 0")
+    fix.init()
+    fix.run(0, 2)
+    steps = max(5, args.steps // 100)
+    dt = timed_run(fix, 2, steps)
+    value = n * steps / dt
+    kms = fix.profile(2 + steps, 3)
+    nk = 2056
+    fl = FLOP_SK * n * nk
+    tk = (kms.get("k_sk_amp", 0) + kms.get("k_sk_force", 0)) * 1e-3
+    roof = {"kernel": "k_sk_amp+k_sk_force", "bound": "fp32", "achieved": fl / tk / 1e12, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": fl / tk / 1e12 / FP32_PEAK_TFLOPS, "traffic": None,
+            "algorithmic": f"24 FLOP x {n} atoms x {nk} k per step"}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        x = s.arrays()[0][:n]
+        ks = np.array(sk_kvecs()[0])[:8]
+        t0 = time.perf_counter()
+        orc.sk(x, ks)
+        orc.sk_grad(x, ks)
+        t = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        # scale the 8-vector sample to the full 2,056-vector step
+        cpu = {"value": n / (t * nk / 8), "unit": "atom-steps/s", "cores": cores, "kind": "port",
+               "sample": f"S(k) and its gradient for the 8 (111) vectors of the 500,000-atom slab (FP64 oracle, "
+                         f"OpenMP), {t:.1f} s, scaled by 2056/8 to a full step"}
+    cfg = {"workload": "config 3: 500,000-atom FCC/liquid slab, SK.AVE over 8 (111) + 2048 random k, "
+                       "grid [0,50]x1000", "atoms": n, "k_vectors": nk, "cv_value": fix.cv("SK.AVE"),
+           "kernels_ms": kms}
+    return line(value, steps, 2, dt * 1e3 / steps, "f64 phase, f32 sincos, f64 sums", cfg, roof, cpu,
+                metric="atom-steps/s, config 3 (500k atoms, S(k) over 2,056 k)")
+
+
+# ------------------------------------------------------------------ config 4
+def c4_systems(mdt, nw=64, seed=44):
+    rng = np.random.default_rng(seed)
+    out = []
+    for w in range(nw):
+        nt = int(rng.integers(2000, 64001))
+        if rng.random() < 0.5:
+            m = max(4, int(round((nt / 4) ** (1 / 3))))
+            s = mdt.System().add_fcc(m, m, m, A, 0.1, 0.1, 0.0, seed=1000 + w)
+        else:
+            L = (nt / 0.021) ** (1 / 3)
+            s = mdt.System().set_box([0, 0, 0], [L, L, L]).add_uniform(nt, 0.0, L, seed=2000 + w)
+        out.append(s.build(RG))
+    return out
+
+
+C4_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE " + steinh("Q4", 4) + steinh("Q6", 6) +
+           "CV_dim 2 SYMBOL v1 Q4.AVE SYMBOL v2 Q6.AVE DIM 1 0 1 200 v1 DIM 2 0 1 200 v2 WT 1")
+
+
+def run_c4(mdt, args):
+    systems = c4_systems(mdt)
+    ntot = sum(s.n_local for s in systems)
+    ctx = mdt.Context(0, len(systems))
+    for w, s in enumerate(systems):
+        ctx.bind_system(w, s)
+    fix = mdt.Fix(ctx, C4_ARGS)
+    fix.init()
+    fix.run(0, args.warmup)
+    steps = max(100, args.steps // 4)
+    dt = timed_run(fix, args.warmup, steps)
+    value = ntot * steps / dt
+    kms = fix.profile(args.warmup + steps, args.profile_steps)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        s0 = systems[0]
+        x, off, idx, own = s0.arrays()
+
+        def one():
+            env = orc.Env(x, off, idx, own, s0.n_local, RC, 18)
+            env.q_ave(4), env.q_ave(6)
+            -(env.dQ_dx(4, 1.0) + env.dQ_dx(6, 1.0))
+        k, t = oracle_steps(one, min(args.cpu_budget, 10.0))
+        cpu = {"value": s0.n_local * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{k} steps of walker 0 ({s0.n_local} atoms, Q4+Q6 and forces), FP64 oracle, {t:.1f} s; "
+                         "the oracle runs walkers one after another"}
+    cfg = {"workload": "config 4: 64 walkers, 2k-64k atoms each (FCC sigma 0.1 or liquid 0.021/A^3), "
+                       "2-D grids 200x200 on (Q4.AVE, Q6.AVE), WT, one graph launch per step",
+           "atoms_total": ntot, "walkers": len(systems), "kernels_ms": kms}
+    return line(value, steps, args.warmup, dt * 1e3 / steps, "f32 pair math, f64 sums/grid", cfg, None, cpu,
+                metric="atom-steps/s, config 4 (64 walkers, one graph launch per step)")
+
+
+# ------------------------------------------------------------------ config 5
+def wt_hills(s0, nh, kt, biasf=10.0, h0=0.005, seed=5):
+    """Hills of a well-tempered run diffusing around s0: centres spread 2 sigma, heights
+    h0 exp(-V(c)/(kT (biasf-1))) with V the sum of the earlier hills (FP64)."""
+    sig = np.array([0.01, 0.01, 0.5])
+    rng = np.random.default_rng(seed)
+    c = s0 + rng.normal(size=(nh, 3)) * 2.0 * sig
+    h = np.zeros(nh)
+    for k in range(nh):
+        V = float(np.sum(h[:k] * np.exp(-0.5 * np.sum(((c[:k] - c[k]) / sig) ** 2, axis=1)))) if k else 0.0
+        h[k] = h0 * np.exp(-V / (kt * (biasf - 1.0)))
+    return c, h
+
+
+def run_c5(mdt, args):
+    system = c2_system(mdt)
+    n = system.n_local
+    kspec = f"FCC111 {A}"
+    kt = 300.0 * KB
+    base = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE SIGMAS 0.01 0.01 0.5 " + steinh("Q4", 4) +
+            steinh("Q6", 6) + f"CAL NAME S111 SK all {kspec} CV_dim 3 SYMBOL v1 Q4.AVE SYMBOL v2 Q6.AVE "
+            "SYMBOL v3 S111.AVE DIM 1 0 1 100 v1 DIM 2 0 1 100 v2 DIM 3 0 200000 100 v3 WT 1 ")
+    res = {}
+    steps = 2000
+    nh0 = 19800
+    hist = None
+    for kind in ("hills", "mlp"):
+        ctx = mdt.Context(0, 1)
+        ctx.bind_system(0, system)
+        extra = "Gaussian_Hill_type 2" if kind == "hills" else ("Gaussian_Hill_type 3 MLP_REFIT 500 "
+                                                               "MLP_POINTS 1000000 MLP_HOLDOUT 65536")
+        fix = mdt.Fix(ctx, base + extra)
+        fix.init()
+        fix.post_force(0)
+        if hist is None:
+            s0 = np.array([fix.vector(k) for k in range(3)])
+            # late-run state: 19,800 hills already deposited around the visited CV region
+            hist = wt_hills(s0, nh0, kt)
+        fix.add_hills(hist[0], hist[1], 0.01)
+        t_fit = None
+        if kind == "mlp":
+            t0 = time.perf_counter()
+            fix.mlp_refit()
+            fix.sync()
+            t_fit = time.perf_counter() - t0
+        fix.run(1, 20)
+        dt = timed_run(fix, 21, steps)
+        r = {"atom_steps_per_s": n * steps / dt, "us_per_step": dt * 1e6 / steps, "hills_end": int(fix.hills(3).shape[0])}
+        if kind == "mlp":
+            st = fix.mlp_status()
+            r.update({"first_fit_s": t_fit, "heldout_max_dV_kT": st["err"] / kt, "network_in_use": st["ok"],
+                      "fits": st["fits"], "fallbacks": st["fallbacks"]})
+        res[kind] = r
+        log(f"c5 {kind}: {r}")
+        fix.close()
+        ctx.close()
+    cfg = {"workload": "config 5: 256,000-atom config-2 system, 3-D CV (Q4.AVE, Q6.AVE, S(k_111)), "
+                       "sigma 0.01/0.01/0.5, h 0.005, WT 10; 19,800 hills pre-deposited, 2,000 timed steps "
+                       "(+200 hills), learned bias refit every 500 steps on 1M points", "atoms": n, "results": res}
+    return line(res["mlp"]["atom_steps_per_s"], steps, 20, res["mlp"]["us_per_step"] * 1e-3,
+                "bf16 MFMA network (f32 accum), f64 labels", cfg, None, None,
+                metric="atom-steps/s, config 5 (learned bias; exact hills-sum result in config.results)")
 
 
 def main():
@@ -76,92 +351,19 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
-
     import mdt
     if not os.path.exists(mdt.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
-
-    log("building 256k-atom system")
-    system = build_system(mdt)
-    n = system.n_local
-    ctx = mdt.Context(0, 1)
-    ctx.bind_system(0, system)
-    fix = mdt.Fix(ctx, FIX_ARGS)
-    fix.init()
-    log(f"warmup {args.warmup}")
-    fix.run(0, args.warmup)
-    fix.sync()
-    # timed region: K graph launches bracketed by stream synchronisation
-    step0 = args.warmup
-    fix.sync()
-    t0 = time.perf_counter()
-    fix.run(step0, args.steps)
-    fix.sync()
-    t1 = time.perf_counter()
-    dt = t1 - t0
-    ms_per_step = dt * 1e3 / args.steps
-    value = n * args.steps / dt
-    log(f"timed {args.steps} steps: {ms_per_step * 1e3:.1f} us/step, {value:.3e} atom-steps/s")
-
-    # per-kernel durations (HIP events on the context stream, plain launches)
-    kms = fix.profile(step0 + args.steps, args.profile_steps)
-    log("kernel mean ms: " + json.dumps(kms))
-    nsel = fix.extract("Q6.nsel", n)
-    pairs = float(nsel.sum())
-    x, off, idx, own = system.arrays()
-    c = np.diff(off).astype(np.float64)
-    n_ghost = x.shape[0] - n
-    bytes_sel = float(np.sum(24 + 4 * (c + 1) + 4 * nsel + 4) + 24 * n_ghost)
-    work = {
-        "k_env": ("mfma" if False else "fp32", FLOP_FWD_PAIR * pairs, bytes_sel),
-        "k_force": ("fp32", FLOP_GRAD_PAIR * pairs, None),
-        "k_ld": ("fp32", FLOP_LD_FWD_PAIR * pairs, None),
-    }
-    dom = max((k for k in kms if k in work), key=lambda k: kms[k])
-    t = kms[dom] * 1e-3
-    kind, flop, byt = work[dom]
-    fl_rate = flop / t / 1e12
-    roof = {"kernel": dom, "bound": "fp32-valu", "achieved": fl_rate, "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": fl_rate / FP32_PEAK_TFLOPS, "traffic": None,
-            "algorithmic": f"{FLOP_GRAD_PAIR if dom == 'k_force' else FLOP_FWD_PAIR:.0f} FLOP x {pairs:.0f} selected pairs",
-            "mean_ms": kms[dom]}
-    if byt is not None:
-        roof["hbm_achieved_GBs"] = byt / t / 1e9
-    tp = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
-    if os.path.exists(tp):
-        roof["traffic"] = json.load(open(tp)).get("bytes_per_launch")
-
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline (oracle, 1 core)")
-        cpu = cpu_baseline(system)
-
-    out = {
-        "metric": "atom-steps/s, Q6+Q4 CV+bias forces @256k atoms; NN bias vs hills @<=0.1 kT",
-        "value": value,
-        "unit": "atom-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 pair math, f64 sums/grid",
-        "data": "synthetic (seeded FCC, half jittered 0.6 A, half 0.1 A)",
-        "config": {"workload": "config 2: 256,000-atom FCC Ar, Q6.AVE+Q4.AVE (+LQ6) N=18 rc=3.9, "
-                               "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs,
-                   "kernels_ms": kms},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
+    out = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.config](mdt, args)
     print(json.dumps(out), flush=True)
 
 

@@ -104,8 +104,11 @@ enum ExprOp : int32_t {
 };
 struct ExprProg {
     int32_t n;
+    int32_t linear;           // 1: s = lin0 + sum_q lin[q] v_q (evaluated without the interpreter)
     int32_t op[kMaxProg];
     double c[kMaxProg];       // constant (OP_CONST) or symbol index (OP_SYM)
+    double lin0;
+    double lin[kMaxSym];
 };
 
 // Bias representations.
@@ -229,6 +232,12 @@ hipError_t mlp_copy_net(const MlpNet* src, MlpNet* dst, hipStream_t st);
 hipError_t mlp_init(MlpNet* net, uint64_t seed, hipStream_t st);
 
 // Structure factor
+struct SkK {                  // per walker and k vector, rebuilt every step
+    double kr[3];             // k / 2pi (revolutions per length)
+    float kv[3];              // k
+    float ar, ai;             // A(k) (FP32 copy for the gradient)
+    float pad[3];
+};
 struct SkDev {
     int32_t nk;
     const double* k;          // nk * 3 (k / 2pi, revolutions per length)
@@ -237,6 +246,7 @@ struct SkDev {
     const int32_t* chunk;     // per chunk: start, count, walker (3 ints)
     double* apart;            // [chunk][nk][2]
     double* amp;              // per walker [nk][2]
+    SkK* kpack;               // per walker [nk]
     float* ampf;              // per walker [nk][2] scaled for the gradient
     const int32_t* chunk_begin;  // per walker first chunk (n_walkers + 1)
     double* direct;           // per walker [kMaxSlotCV]

@@ -372,6 +372,7 @@ struct CvH {
     // SK
     std::vector<double> kvec;   // nk*3
     DBuf<double> kd, krev, apart, amp;
+    DBuf<SkK> kpack;
     DBuf<float> ampf;
     DBuf<int32_t> chunk, chunk_begin, natoms, abegin;
     int nchunk = 0, max_atoms = 0;
@@ -449,6 +450,8 @@ struct mdt_fix {
 
 namespace {
 
+void affine_form(ExprProg& p);
+
 void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
     int i = 0;
     auto need = [&](int k, const char* what) {
@@ -517,12 +520,19 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 throw Err(MDT_EINVAL, fmt("%s: value out of range", kw.c_str()));
             i += 2;
         } else if (kw == "SIGMAS") {
-            // per-dimension hill widths for the hills-sum and learned biases
-            int d = fx->cv_dim;
-            need(d, "SIGMAS requires CV_dim values.");
+            // per-dimension hill widths for the hills-sum and learned biases: the numbers
+            // that follow (one per dimension, checked against CV_dim after parsing)
             fx->sigmas.clear();
-            for (int k = 0; k < d; ++k) fx->sigmas.push_back(parse_num(argv[i + 1 + k], "SIGMAS"));
-            i += 1 + d;
+            int k = i + 1;
+            while (k < argc && fx->sigmas.size() < (size_t)kMaxDim) {
+                char* end = nullptr;
+                double v = strtod(argv[k], &end);
+                if (!argv[k][0] || (end && *end)) break;
+                fx->sigmas.push_back(v);
+                ++k;
+            }
+            if (fx->sigmas.empty()) throw Err(MDT_EINVAL, "SIGMAS requires one value per dimension.");
+            i = k;
         } else if (kw == "CAL") {
             need(3, "CAL requires NAME <name> <type> ...");
             if (strcmp(argv[i + 1], "NAME")) throw Err(MDT_EINVAL, "Error: CAL requires NAME keyword.");
@@ -689,13 +699,74 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
     }
     if (fx->hill_type == 3 && fx->cv_dim != 3) throw Err(MDT_ENOTSUP, "the learned bias is built for CV_dim 3");
     if (fx->sigmas.empty()) fx->sigmas.assign(fx->cv_dim, fx->sigma);
+    if ((int)fx->sigmas.size() != fx->cv_dim)
+        throw Err(MDT_EINVAL, fmt("SIGMAS has %zu values but CV_dim is %d", fx->sigmas.size(), fx->cv_dim));
+    for (double v : fx->sigmas) if (!(v > 0)) throw Err(MDT_EINVAL, "SIGMAS must be positive");
     if (fx->syms.empty()) throw Err(MDT_EINVAL, "fix metad needs at least one SYMBOL");
     if ((int)fx->cvs.size() > kMaxSlotCV) throw Err(MDT_ENOTSUP, fmt("at most %d CALs", kMaxSlotCV));
     // compile the DIM expressions against the symbol table (DimSet.cu:54-66)
     std::vector<std::string> names;
     for (auto& s : fx->syms) names.push_back(s.name);
     fx->h_progs.clear();
-    for (int d = 0; d < fx->cv_dim; ++d) fx->h_progs.push_back(ExprCompiler(fx->dim_expr[d], names).compile());
+    for (int d = 0; d < fx->cv_dim; ++d) {
+        ExprProg pg = ExprCompiler(fx->dim_expr[d], names).compile();
+        affine_form(pg);
+        fx->h_progs.push_back(pg);
+    }
+}
+
+// Affine reading of a program: when every operation keeps the value of the form
+// c0 + sum_q c_q v_q the program is evaluated on the device as that sum.
+void affine_form(ExprProg& p) {
+    struct Aff { double c0; double c[kMaxSym]; bool cst; };
+    std::vector<Aff> st;
+    p.linear = 0;
+    auto cst = [](const Aff& a) { return a.cst; };
+    for (int k = 0; k < p.n; ++k) {
+        const int op = p.op[k];
+        if (op == OP_CONST || op == OP_SYM) {
+            Aff a{};
+            if (op == OP_CONST) { a.c0 = p.c[k]; a.cst = true; }
+            else { a.c[(int)p.c[k]] = 1.0; a.cst = false; }
+            st.push_back(a);
+            continue;
+        }
+        if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV) {
+            Aff b = st.back(); st.pop_back();
+            Aff& a = st.back();
+            if (op == OP_ADD || op == OP_SUB) {
+                const double sg = op == OP_ADD ? 1.0 : -1.0;
+                a.c0 += sg * b.c0;
+                for (int q = 0; q < kMaxSym; ++q) a.c[q] += sg * b.c[q];
+                a.cst = a.cst && b.cst;
+            } else if (op == OP_MUL && (cst(a) || cst(b))) {
+                const Aff& v = cst(a) ? b : a;
+                const double f = cst(a) ? a.c0 : b.c0;
+                Aff r{};
+                r.c0 = v.c0 * f;
+                for (int q = 0; q < kMaxSym; ++q) r.c[q] = v.c[q] * f;
+                r.cst = v.cst;
+                a = r;
+            } else if (op == OP_DIV && cst(b) && b.c0 != 0.0) {
+                a.c0 /= b.c0;
+                for (int q = 0; q < kMaxSym; ++q) a.c[q] /= b.c0;
+            } else {
+                return;   // not affine
+            }
+            continue;
+        }
+        if (op == OP_NEG) {
+            Aff& a = st.back();
+            a.c0 = -a.c0;
+            for (int q = 0; q < kMaxSym; ++q) a.c[q] = -a.c[q];
+            continue;
+        }
+        return;           // functions, powers, min/max: use the interpreter
+    }
+    if (st.size() != 1) return;
+    p.linear = 1;
+    p.lin0 = st[0].c0;
+    for (int q = 0; q < kMaxSym; ++q) p.lin[q] = st[0].c[q];
 }
 
 bool prog_uses(const ExprProg& p, int sym) {
@@ -794,7 +865,7 @@ void fix_build(mdt_fix* fx) {
             c->kd.alloc(c->kvec.size()); c->kd.upload(c->kvec.data(), c->kvec.size(), st);
             c->krev.alloc(krev.size()); c->krev.upload(krev.data(), krev.size(), st);
             std::vector<int32_t> ch, cb(nw + 1, 0), na(nw), ab(nw);
-            const int CH = 8192;
+            const int CH = 16384;   // atoms per chunk of the amplitude pass
             c->max_atoms = 0;
             for (int k = 0; k < nw; ++k) {
                 WalkerIn& a = ctx->w[k];
@@ -814,6 +885,7 @@ void fix_build(mdt_fix* fx) {
             c->abegin.alloc(nw); c->abegin.upload(ab.data(), nw, st);
             c->apart.alloc((size_t)c->nchunk * nk * 2);
             c->amp.alloc((size_t)nw * nk * 2);
+            c->kpack.alloc((size_t)nw * nk);
         }
     }
     fx->nslot_sum = nsum;
@@ -1154,7 +1226,7 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             SkDev s{};
             s.nk = (int)(c->kvec.size() / 3);
             s.k = c->krev.p; s.kvec = c->kd.p; s.nchunk = c->nchunk; s.chunk = c->chunk.p;
-            s.apart = c->apart.p; s.amp = c->amp.p; s.chunk_begin = c->chunk_begin.p;
+            s.apart = c->apart.p; s.amp = c->amp.p; s.kpack = c->kpack.p; s.chunk_begin = c->chunk_begin.p;
             s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
             s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             T.run("k_sk_amp", st, [&] { HIPCHK(launch_sk_amp(g, s, st)); });
@@ -1175,7 +1247,7 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         if (c->kind == CV_SK && c->biased) {
             SkDev s{};
             s.nk = (int)(c->kvec.size() / 3);
-            s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p;
+            s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p; s.kpack = c->kpack.p;
             s.wslot = c->slot; s.wcv = fx->wcv.p; s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p;
             s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(g, s, f, st)); });

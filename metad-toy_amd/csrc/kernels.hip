@@ -999,7 +999,15 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
             if (c < r.ncv) cv[c] = r.cv_src[c] >= 0 ? v[c] * r.inv_ng[w] : r.direct[(int64_t)w * kMaxSlotCV + c];
         // 2. dimensions
         for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
-        for (int d = 0; d < r.D; ++d) expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
+        for (int d = 0; d < r.D; ++d) {
+            if (progs[d].linear) {
+                double v = progs[d].lin0;
+                for (int q = 0; q < r.nsym; ++q) { v += progs[d].lin[q] * symv[q]; dsdv[d * kMaxSym + q] = progs[d].lin[q]; }
+                s[d] = v;
+            } else {
+                expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
+            }
+        }
     }
     __syncthreads();
     // 3. deposit
@@ -1118,7 +1126,8 @@ __global__ void k_fill_f(float* p, int64_t n, float v) {
 // Structure factor.  k is stored in revolutions per Angstrom (k / 2pi) so the
 // FP64 phase reduces to its fractional part before the FP32 sin/cos.
 // ---------------------------------------------------------------------------
-constexpr int kSkTile = 16;   // k vectors per workgroup in the amplitude pass
+constexpr int kSkTile = 8;    // k vectors per workgroup in the amplitude pass
+constexpr int kSkChunk = 16384;   // atoms per chunk (host side uses the same value)
 
 __device__ __forceinline__ void sincos_rev(double ph, float& s, float& c) {
     const double fr = ph - rint(ph);                 // in [-0.5, 0.5]
@@ -1126,17 +1135,19 @@ __device__ __forceinline__ void sincos_rev(double ph, float& s, float& c) {
     __sincosf(a, &s, &c);
 }
 
+// A(k) partial sums: workgroup (k tile, atom chunk); k vectors are uniform per
+// workgroup and live in scalar registers.
 __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
     __shared__ double sh[kSkTile * 2 * 4];
-    __shared__ double kt[kSkTile * 3];
     const int kb = blockIdx.x * kSkTile;
     const int ch = blockIdx.y;
     const int start = sk.chunk[3 * ch], count = sk.chunk[3 * ch + 1];
-    if (threadIdx.x < kSkTile * 3) {
-        int q = kb + threadIdx.x / 3;
-        kt[threadIdx.x] = q < sk.nk ? sk.k[3 * q + threadIdx.x % 3] : 0.0;
+    double kx[kSkTile], ky[kSkTile], kz[kSkTile];
+    #pragma unroll
+    for (int q = 0; q < kSkTile; ++q) {
+        const int qq = kb + q < sk.nk ? kb + q : sk.nk - 1;
+        kx[q] = sk.k[3 * qq]; ky[q] = sk.k[3 * qq + 1]; kz[q] = sk.k[3 * qq + 2];
     }
-    __syncthreads();
     double acc[kSkTile * 2];
     #pragma unroll
     for (int q = 0; q < kSkTile * 2; ++q) acc[q] = 0.0;
@@ -1144,10 +1155,9 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
         const int i = start + a;
         if (g.group && !g.group[i]) continue;
         const double4 xv = g.x4[i];
-        const double x0 = xv.x, x1 = xv.y, x2 = xv.z;
         #pragma unroll
         for (int q = 0; q < kSkTile; ++q) {
-            const double ph = x0 * kt[3 * q] + x1 * kt[3 * q + 1] + x2 * kt[3 * q + 2];
+            const double ph = xv.x * kx[q] + xv.y * ky[q] + xv.z * kz[q];
             float sn, cs;
             sincos_rev(ph, sn, cs);
             acc[2 * q] += (double)cs;
@@ -1164,7 +1174,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
     }
 }
 
-// per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S
+// per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S, packed k data
 __global__ __launch_bounds__(kBlock) void k_sk_sum(SkDev sk) {
     __shared__ double sh[4];
     const int w = blockIdx.x;
@@ -1179,53 +1189,41 @@ __global__ __launch_bounds__(kBlock) void k_sk_sum(SkDev sk) {
         }
         sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
         sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
+        SkK kp;
+        for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
+        kp.ar = (float)re; kp.ai = (float)im;
+        kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
+        sk.kpack[(int64_t)w * sk.nk + q] = kp;
         v[0] += (re * re + im * im) / N;
     }
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) sk.direct[(int64_t)w * kMaxSlotCV + sk.slot] = v[0] / sk.nk;
 }
 
-// F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j})
-constexpr int kSkFT = 256;
+// F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j}); one atom per thread, the k loop
+// index is uniform so the packed k data load through the scalar path
 __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f) {
-    __shared__ double kt[kSkFT * 3];
-    __shared__ float kv[kSkFT * 3];
-    __shared__ float am[kSkFT * 2];
     const int w = blockIdx.y;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = sk.n_atoms[w];
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
     const double wt = sk.wcv[(int64_t)w * kMaxSlotCV + sk.wslot];
     const float scale = (float)(wt * 2.0 / ((double)n * sk.nk));
-    const int i = sk.atom_begin[w] + a;
+    const int i = sk.atom_begin[w] + (a < n ? a : 0);
     const bool act = a < n && (g.group == nullptr || g.group[i]);
-    double x0 = 0, x1 = 0, x2 = 0;
-    if (act) { const double4 xv = g.x4[i]; x0 = xv.x; x1 = xv.y; x2 = xv.z; }
+    const double4 xv = g.x4[i];
+    const SkK* kp = sk.kpack + (int64_t)w * sk.nk;
     float gx = 0, gy = 0, gz = 0;
-    for (int kb = 0; kb < sk.nk; kb += kSkFT) {
-        __syncthreads();
-        for (int q = threadIdx.x; q < kSkFT; q += blockDim.x) {
-            const int qq = kb + q;
-            const bool ok = qq < sk.nk;
-            for (int d = 0; d < 3; ++d) {
-                kt[3 * q + d] = ok ? sk.k[3 * qq + d] : 0.0;
-                kv[3 * q + d] = ok ? (float)sk.kvec[3 * qq + d] : 0.0f;
-            }
-            am[2 * q] = ok ? (float)sk.amp[((int64_t)w * sk.nk + qq) * 2] : 0.0f;
-            am[2 * q + 1] = ok ? (float)sk.amp[((int64_t)w * sk.nk + qq) * 2 + 1] : 0.0f;
-        }
-        __syncthreads();
-        if (act) {
-            const int lim = sk.nk - kb < kSkFT ? sk.nk - kb : kSkFT;
-            for (int q = 0; q < lim; ++q) {
-                const double ph = x0 * kt[3 * q] + x1 * kt[3 * q + 1] + x2 * kt[3 * q + 2];
-                float sn, cs;
-                sincos_rev(ph, sn, cs);
-                const float imv = __builtin_fmaf(am[2 * q], sn, -am[2 * q + 1] * cs);
-                gx = __builtin_fmaf(kv[3 * q], imv, gx);
-                gy = __builtin_fmaf(kv[3 * q + 1], imv, gy);
-                gz = __builtin_fmaf(kv[3 * q + 2], imv, gz);
-            }
-        }
+    #pragma unroll 4
+    for (int q = 0; q < sk.nk; ++q) {
+        const SkK k = kp[q];
+        const double ph = xv.x * k.kr[0] + xv.y * k.kr[1] + xv.z * k.kr[2];
+        float sn, cs;
+        sincos_rev(ph, sn, cs);
+        const float imv = __builtin_fmaf(k.ar, sn, -k.ai * cs);
+        gx = __builtin_fmaf(k.kv[0], imv, gx);
+        gy = __builtin_fmaf(k.kv[1], imv, gy);
+        gz = __builtin_fmaf(k.kv[2], imv, gz);
     }
     if (act) {
         f[3 * (int64_t)i] += (double)(scale * gx);

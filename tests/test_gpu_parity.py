@@ -225,3 +225,28 @@ def test_hills_file_and_restart(mdt, orc, tmp_path):
     fix2.init()
     g2 = fix2.grid(1000)
     assert np.abs(g1 - g2).max() <= 1e-12 * np.abs(g1).max()
+
+
+@pytest.mark.parametrize("expr", ["v1*v2+0.5*exp(v1)", "sqrt(v1^2+v2^2)", "2*v1-v2/3+0.25"])
+def test_dim_expression_and_chain_rule(mdt, orc, expr):
+    """DIM expressions (interpreted or affine) and exact ds/dv in the chain rule (DimSet.cu:83-97)."""
+    s = fcc_system(mdt, 6, sigma=0.2, seed=6)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    v1, v2 = env.q_ave(6), env.q_ave(4)
+    f = {"v1*v2+0.5*exp(v1)": (lambda a, b: a * b + 0.5 * np.exp(a), lambda a, b: (b + 0.5 * np.exp(a), a)),
+         "sqrt(v1^2+v2^2)": (lambda a, b: np.sqrt(a * a + b * b),
+                             lambda a, b: (a / np.sqrt(a * a + b * b), b / np.sqrt(a * a + b * b))),
+         "2*v1-v2/3+0.25": (lambda a, b: 2 * a - b / 3 + 0.25, lambda a, b: (2.0, -1.0 / 3.0))}[expr]
+    s0 = f[0](v1, v2)
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, q46_fix(expr=f'"{expr}"', lo=0.0, hi=3.0, nb=3000))
+    fix.init()
+    fix.add_hills(np.array([[s0 + 0.007]]), np.array([0.005]), 0.01)
+    fix.post_force(1)
+    assert abs(fix.vector(0) - s0) <= 1e-5 * abs(s0)
+    V, dV = fix.bias_eval(np.array([[fix.vector(0)]]))
+    d1, d2 = f[1](v1, v2)
+    assert abs(fix.cv_weight("Q6.AVE") - dV[0, 0] * d1) <= 1e-4 * abs(dV[0, 0] * d1)
+    assert abs(fix.cv_weight("Q4.AVE") - dV[0, 0] * d2) <= 1e-4 * abs(dV[0, 0] * d2)
