@@ -345,13 +345,38 @@ def run_c5(mdt, args):
                 metric="atom-steps/s, config 5 (learned bias; exact hills-sum result in config.results)")
 
 
+def run_copy(mdt, args):
+    """Plain device copy of a known size: the calibration for HBM byte counters and the
+    achievable copy bandwidth of the chip (reads 1 GiB, writes 1 GiB per launch)."""
+    import torch
+    n = 1 << 28
+    src = torch.full((n,), 1.0, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    steps = max(5, args.steps // 100)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    nbytes = 2 * 4 * n
+    return {"metric": "copy GB/s (read+write)", "value": nbytes / dt / 1e9, "unit": "GB/s", "n_gpus": 1,
+            "steps": steps, "warmup": 3, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "device copy of 2^28 float32 (1 GiB read + 1 GiB written per launch)",
+                       "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--profile-steps", type=int, default=50)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"])
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -363,7 +388,8 @@ def main():
     if not os.path.exists(mdt.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
-    out = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.config](mdt, args)
+    out = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5,
+           "copy": run_copy}[args.config](mdt, args)
     print(json.dumps(out), flush=True)
 
 

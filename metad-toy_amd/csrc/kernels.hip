@@ -892,8 +892,9 @@ __device__ double bias_eval_block(const ReduceDev& r, int w, const double* s, do
 // ---------------------------------------------------------------------------
 struct Dual { double v; double d[kMaxSym]; };
 
-// st: kStack entries of scratch (LDS in k_reduce: an interpreter stack is dynamically indexed)
-__device__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double* val, double* dval, Dual* st) {
+// st: kStack entries of scratch (LDS in k_reduce: an interpreter stack is dynamically indexed).
+// Kept out of line: its FP64 math constants would otherwise occupy the caller's registers.
+__device__ __noinline__ void expr_eval(const ExprProg& p, int nsym, const double* sym, double* val, double* dval, Dual* st) {
     int sp = 0;
     for (int k = 0; k < p.n; ++k) {
         const int op = p.op[k];
@@ -973,30 +974,39 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         int* dst = reinterpret_cast<int*>(progs);
         for (int k = threadIdx.x; k < nwords; k += blockDim.x) dst[k] = src[k];
     }
-    // 1. CV values: one deterministic FP64 tree over tiles for every slot,
-    //    four independent loads in flight per thread and slot
-    double v[kMaxSlotCV];
-    #pragma unroll
-    for (int c = 0; c < kMaxSlotCV; ++c) v[c] = 0.0;
-    constexpr int U = 4;
-    for (int t = t0 + threadIdx.x; t < t1; t += U * kRedThreads) {
-        #pragma unroll
-        for (int c = 0; c < kMaxSlotCV; ++c) {
-            const int src = c < r.ncv ? r.cv_src[c] : -1;
-            if (src >= 0) {
-                const double* pp = r.partial + (int64_t)src * r.ntiles;
-                double x[U];
+    // 1. CV values: one deterministic FP64 tree over tiles per slot (each thread
+    //    sums its tiles in a fixed order, then waves, then the wave totals in
+    //    order).  Slots go one after another so only U loads per thread are live:
+    //    a 1024-thread block has 128 VGPRs per lane.
+    constexpr int U = 8;
+    constexpr int NW = kRedThreads / 64;
+    const int lane64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < r.ncv; ++c) {
+        const int src = r.cv_src[c];
+        if (src < 0) continue;
+        const double* pp = r.partial + (int64_t)src * r.ntiles;
+        double acc = 0.0;
+        for (int t = t0 + threadIdx.x; t < t1; t += U * kRedThreads) {
+            double x[U];
+            #pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = (t + u * kRedThreads < t1) ? pp[t + u * kRedThreads] : 0.0;
+            acc += ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+        }
+        acc = wave_sum(acc);
+        if (lane64 == 0) sh[c * NW + wv] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < r.ncv; ++c) {
+            if (r.cv_src[c] >= 0) {
+                double a = 0.0;
                 #pragma unroll
-                for (int u = 0; u < U; ++u) x[u] = (t + u * kRedThreads < t1) ? pp[t + u * kRedThreads] : 0.0;
-                v[c] += (x[0] + x[1]) + (x[2] + x[3]);
+                for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
+                cv[c] = a * r.inv_ng[w];
+            } else {
+                cv[c] = r.direct[(int64_t)w * kMaxSlotCV + c];
             }
         }
-    }
-    block_sum<kMaxSlotCV, kRedThreads / 64>(v, sh);
-    if (threadIdx.x == 0) {
-        #pragma unroll
-        for (int c = 0; c < kMaxSlotCV; ++c)
-            if (c < r.ncv) cv[c] = r.cv_src[c] >= 0 ? v[c] * r.inv_ng[w] : r.direct[(int64_t)w * kMaxSlotCV + c];
         // 2. dimensions
         for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
         for (int d = 0; d < r.D; ++d) {
@@ -1059,16 +1069,19 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     }
 }
 
-__global__ void k_bias_eval(ReduceDev r, int w, int64_t np, const double* s, double* V, double* dV) {
+__global__ __launch_bounds__(kBlock) void k_bias_eval(ReduceDev r, int w, int64_t np, const double* s,
+                                                      double* V, double* dV) {
     __shared__ double sh[64];
     for (int64_t p = blockIdx.x; p < np; p += gridDim.x) {
-        double sp[kMaxDim];
-        for (int d = 0; d < r.D; ++d) sp[d] = s[p * r.D + d];
+        double sp[kMaxDim] = {0, 0, 0};
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) sp[d] = s[p * r.D + d];
         double g[kMaxDim] = {0, 0, 0};
-        double v = bias_eval_block<4>(r, w, sp, g, sh);
+        double v = bias_eval_block<kBlock / 64>(r, w, sp, g, sh);
         if (threadIdx.x == 0) {
             if (V) V[p] = v;
-            if (dV) for (int d = 0; d < r.D; ++d) dV[p * r.D + d] = g[d];
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (dV && d < r.D) dV[p * r.D + d] = g[d];
         }
         __syncthreads();
     }
