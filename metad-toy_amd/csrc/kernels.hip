@@ -257,12 +257,21 @@ __device__ __forceinline__ void mset(M128& m, int p) {
     if (p < 64) m.lo |= 1ull << p; else m.hi |= 1ull << (p - 64);
 }
 __device__ __forceinline__ bool mempty(const M128& m) { return (m.lo | m.hi) == 0ull; }
+__device__ __forceinline__ int mpop_count(const M128& m) { return __popcll(m.lo) + __popcll(m.hi); }
 __device__ __forceinline__ int mpop(M128& m) {
     if (m.lo) { int p = __builtin_ctzll(m.lo); m.lo &= m.lo - 1; return p; }
     int p = 64 + __builtin_ctzll(m.hi); m.hi &= m.hi - 1; return p;
 }
 __device__ __forceinline__ void mdrop(M128& m, int k) {
     for (int t = 0; t < k; ++t) { if (m.lo) m.lo &= m.lo - 1; else m.hi &= m.hi - 1; }
+}
+// next set bit of this lane's round-robin share (-1 when done); `it` must have
+// had mdrop(it, lane) applied once before the first call
+__device__ __forceinline__ int mnext(M128& it) {
+    if (mempty(it)) return -1;
+    const int p = mpop(it);
+    mdrop(it, kLanes - 1);
+    return p;
 }
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int o) {
     return (uint64_t)__shfl_xor((long long)v, o, 64);
@@ -351,13 +360,30 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     const int N = e.N;
     M128 own{0, 0}, rcm{0, 0};
     int cnt = 0;
-    for (int p = lane; p < c; p += kLanes) {
-        double d0, d1, d2;
-        double r2 = pair_r2(g.x4, g.idx[beg + p], xi.x, xi.y, xi.z, d0, d1, d2);
-        if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
-        if ((r2 > rc2) || (r2 < 1e-12)) continue;
-        ++cnt;
-        mset(own, p);
+    // scan in batches of kScan entries per lane: all list loads of a batch, then
+    // all position loads, so a batch costs two memory round trips
+    constexpr int kScan = 4;
+    for (int p0 = lane; p0 < c; p0 += kScan * kLanes) {
+        int jj[kScan];
+        #pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+            const int p = p0 + u * kLanes;
+            jj[u] = g.idx[beg + (p < c ? p : p0)];
+        }
+        double4 xx[kScan];
+        #pragma unroll
+        for (int u = 0; u < kScan; ++u) xx[u] = g.x4[jj[u]];
+        #pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+            const int p = p0 + u * kLanes;
+            if (p < c) {
+#pragma clang fp contract(off)
+                const double d0 = xx[u].x - xi.x, d1 = xx[u].y - xi.y, d2 = xx[u].z - xi.z;
+                const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
+                if (!((r2 > rc2) || (r2 < 1e-12))) { ++cnt; mset(own, p); }
+            }
+        }
     }
     cnt = grp_sum_i(cnt);
     own = grp_or(own);
@@ -380,10 +406,21 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     for (int k = 0; k < QF; ++k) q[k] = 0.0f;
     M128 it = own;
     mdrop(it, lane);
-    while (!mempty(it)) {
-        const int p = mpop(it);
-        double d0, d1, d2;
-        double r2 = pair_r2(g.x4, g.idx[beg + p], xi.x, xi.y, xi.z, d0, d1, d2);
+    // two-stage prefetch: the next pair's position is in flight while this one is evaluated
+    int p = mnext(it);
+    double4 xc = g.x4[p >= 0 ? g.idx[beg + p] : i];
+    int pn = mnext(it);
+    int jn = pn >= 0 ? g.idx[beg + pn] : i;
+    while (p >= 0) {
+        const double4 xn = g.x4[jn];
+        const int pnn = mnext(it);
+        const int jnn = pnn >= 0 ? g.idx[beg + pnn] : i;
+        double d0, d1, d2, r2;
+        {
+#pragma clang fp contract(off)
+            d0 = xc.x - xi.x; d1 = xc.y - xi.y; d2 = xc.z - xi.z;
+            r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        }
         const float ir = __builtin_amdgcn_rsqf((float)r2);
         const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
         const float u2 = u * u;
@@ -392,14 +429,18 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         if constexpr (LMASK & kL3) acc_ylm<3>(u, u2, zr, zi, q + rec_off(LMASK, 3));
         if constexpr (LMASK & kL4) acc_ylm<4>(u, u2, zr, zi, q + rec_off(LMASK, 4));
         if constexpr (LMASK & kL6) acc_ylm<6>(u, u2, zr, zi, q + rec_off(LMASK, 6));
-        mdrop(it, kLanes - 1);
+        p = pn; xc = xn; pn = pnn; jn = jnn;
     }
     const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
     #pragma unroll
-    for (int k = 0; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
+    for (int k = kRecMask; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
     float out[RS];
     #pragma unroll
     for (int k = 0; k < RS; ++k) out[k] = k < QF ? q[k] : 0.0f;
+    out[0] = __uint_as_float((uint32_t)own.lo);
+    out[1] = __uint_as_float((uint32_t)(own.lo >> 32));
+    out[2] = __uint_as_float((uint32_t)own.hi);
+    out[3] = __uint_as_float((uint32_t)(own.hi >> 32));
     const double inv_ng = e.inv_ng[tl.walker];
     auto finish = [&](auto ll) {
         constexpr int L = decltype(ll)::value;
@@ -490,10 +531,16 @@ __device__ __forceinline__ M128 reverse_mask(const Geo& g, const EnvDev& e, int 
 // ---------------------------------------------------------------------------
 template <int L, int LMASK>
 __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
+    // The kLanes lanes of an atom walk its selected neighbours together; lane t
+    // owns the float4 slice t of the degree-L block, so each neighbour's block is
+    // one cache line touched by one load instruction of the group.
     constexpr int RS = rec_stride(LMASK);
     constexpr int O = rec_off(LMASK, L);
     constexpr int W = 2 * L + 1;
     constexpr int BS = (W + 3) / 4 * 4;
+    constexpr int NB = BS / 4;                 // float4 slices of the block
+    static_assert(NB <= kLanes, "a degree block must fit one float4 per lane");
+    static_assert(O % 4 == 0 && O + BS <= RS, "degree block must be float4 aligned inside the record");
     __shared__ double sh[4];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
@@ -503,43 +550,52 @@ __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
     const int i = tl.start + (valid ? a : 0);
     double part[1] = {0.0};
     const bool center = valid && (g.group == nullptr || g.group[i]);
-    float qb[W];
-    #pragma unroll
-    for (int k = 0; k < W; ++k) qb[k] = 0.0f;
+    const int sl = lane < NB ? lane : 0;       // lanes beyond the block repeat slice 0 and drop it
+    const bool mine = lane < NB;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const ulonglong2 mk = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i];
     M128 it{valid ? mk.x : 0ull, valid ? mk.y : 0ull};
     const int beg = g.off[i];
-    mdrop(it, lane);
+    // batches of kLdBatch neighbours: all owner indices, then all blocks in flight
+    constexpr int kLdBatch = 8;
     while (!mempty(it)) {
-        const int p = mpop(it);
-        const float4* ro = reinterpret_cast<const float4*>(e.rec + (int64_t)g.eown[beg + p] * RS + O);
+        int oo[kLdBatch];
         #pragma unroll
-        for (int k = 0; k < rec_blk(L) / 4; ++k) {
-            const float4 v = ro[k];
-            if (4 * k < W) qb[4 * k] += v.x;
-            if (4 * k + 1 < W) qb[4 * k + 1] += v.y;
-            if (4 * k + 2 < W) qb[4 * k + 2] += v.z;
-            if (4 * k + 3 < W) qb[4 * k + 3] += v.w;
+        for (int u = 0; u < kLdBatch; ++u) {
+            const int p = mempty(it) ? -1 : mpop(it);
+            oo[u] = p >= 0 ? g.eown[beg + p] : -1;
         }
-        mdrop(it, kLanes - 1);
+        float4 vv[kLdBatch];
+        #pragma unroll
+        for (int u = 0; u < kLdBatch; ++u)
+            vv[u] = oo[u] >= 0 ? reinterpret_cast<const float4*>(e.rec + (int64_t)oo[u] * RS + O)[sl]
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        #pragma unroll
+        for (int u = 0; u < kLdBatch; ++u) { acc.x += vv[u].x; acc.y += vv[u].y; acc.z += vv[u].z; acc.w += vv[u].w; }
     }
-    const float* ri = e.rec + (int64_t)i * RS + O;
+    const float4 vi = reinterpret_cast<const float4*>(e.rec + (int64_t)i * RS + O)[sl];
     const int n = e.nsel[i];
     const float inv = 1.0f / (float)(n + 1);
+    float qb[4] = {(acc.x + vi.x) * inv, (acc.y + vi.y) * inv, (acc.z + vi.z) * inv, (acc.w + vi.w) * inv};
     float s2 = 0.0f;
     #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        qb[k] = (grp_sum(qb[k]) + ri[k]) * inv;
-        s2 += (k == 0 ? 1.0f : 2.0f) * qb[k] * qb[k];
+    for (int k = 0; k < 4; ++k) {
+        const int comp = 4 * sl + k;
+        const float wgt = !mine || comp >= W ? 0.0f : (comp == 0 ? 1.0f : 2.0f);
+        s2 += wgt * qb[k] * qb[k];
     }
+    s2 = grp_sum(s2);
     constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
     const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
     const float beta = lq > 0.0f ? (float)(P * e.inv_ng[tl.walker]) / (lq * (float)(n + 1)) : 0.0f;
+    if (valid && mine) {
+        float o4[4];
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) o4[k] = 4 * sl + k < W ? beta * qb[k] : 0.0f;
+        reinterpret_cast<float4*>(d.bq + (int64_t)i * BS)[sl] = make_float4(o4[0], o4[1], o4[2], o4[3]);
+    }
     if (valid && lane == 0) {
         d.lq[i] = lq;
-        float* bq = d.bq + (int64_t)i * BS;
-        #pragma unroll
-        for (int k = 0; k < BS; ++k) bq[k] = k < W ? beta * qb[k] : 0.0f;
         part[0] = (double)lq;
     }
     block_sum<1>(part, sh);
@@ -597,19 +653,18 @@ __global__ __launch_bounds__(kBlock) void k_ld_coeff(Geo g, EnvDev e, LdDev d) {
 // the two coefficient sets added before the contraction.  kLanes lanes per
 // atom share the active pairs round-robin.
 // ---------------------------------------------------------------------------
+// K from a record already in registers (mask quad ignored)
 template <int LMASK, bool EXTRA>
-__device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* wl, float* K) {
+__device__ __forceinline__ void coeff_from_record(const EnvDev& e, int a, const float* wl, const float* r, float* K) {
     constexpr int RS = rec_stride(LMASK);
-    constexpr int NV = rec_used(LMASK) / 4;     // float4s actually used
-    const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)a * RS);
-    float r[NV * 4];
-    #pragma unroll
-    for (int k = 0; k < NV; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
+    constexpr int NV = rec_used(LMASK) / 4;
     float x[NV * 4];
     if constexpr (EXTRA) {
         const float4* xp = reinterpret_cast<const float4*>(e.extra + (int64_t)a * RS);
         #pragma unroll
-        for (int k = 0; k < NV; ++k) { float4 v = xp[k]; x[4*k] = v.x; x[4*k+1] = v.y; x[4*k+2] = v.z; x[4*k+3] = v.w; }
+        for (int k = 0; k < kRecMask; ++k) x[k] = 0.0f;
+        #pragma unroll
+        for (int k = 1; k < NV; ++k) { float4 v = xp[k]; x[4*k] = v.x; x[4*k+1] = v.y; x[4*k+2] = v.z; x[4*k+3] = v.w; }
     }
     #pragma unroll
     for (int k = 0; k < NV * 4; ++k) K[k] = 0.0f;
@@ -633,10 +688,51 @@ __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* 
 }
 
 template <int LMASK, bool EXTRA>
+__device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* wl, float* K) {
+    constexpr int RS = rec_stride(LMASK);
+    constexpr int NV = rec_used(LMASK) / 4;     // float4s actually used (the first holds the mask)
+    const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)a * RS);
+    float r[NV * 4];
+    #pragma unroll
+    for (int k = 0; k < kRecMask; ++k) r[k] = 0.0f;
+    #pragma unroll
+    for (int k = 1; k < NV; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
+    coeff_from_record<LMASK, EXTRA>(e, a, wl, r, K);
+}
+
+// Orders one wave's LDS writes and reads when lanes exchange data through LDS:
+// the hardware completes a wave's LDS accesses in issue order, and the fences
+// keep the compiler from moving a lane's accesses across the exchange point
+// (it cannot see that other lanes read what this lane writes).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Quad broadcast: every lane of an aligned group of four gets lane r's value.
+template <int R>
+__device__ __forceinline__ int quad_bcast(int v) {
+    return __builtin_amdgcn_mov_dpp(v, R | (R << 2) | (R << 4) | (R << 6), 0xF, 0xF, false);
+}
+
+// LDS staging for the cooperative candidate loads: per lane group one 64-byte
+// quad of each of the four candidate records (padded by one float4 so records and
+// groups fall on distinct banks) and the four positions.
+struct ForceStage {
+    float4 rec[kBlock / kLanes][kLanes][kLanes + 1];
+    double4 pos[kBlock / kLanes][kLanes];
+};
+
+template <int LMASK, bool EXTRA>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __restrict__ f) {
     constexpr int QF = rec_used(LMASK);
     constexpr int NL = rec_nl(LMASK);
     constexpr int LMAX = lmax_of(LMASK);
+    constexpr int RS = rec_stride(LMASK);
+    constexpr int NQ = RS / 16;                // 64-byte quads per record
+    constexpr int NV = QF / 4;                 // float4 of the record that are used
+    __shared__ ForceStage stg;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
@@ -652,53 +748,95 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     const M128 own{valid ? mo.x : 0ull, valid ? mo.y : 0ull};
     const M128 rcm{valid ? mr.x : 0ull, valid ? mr.y : 0ull};
     const int beg = g.off[i];
-    const M128 rev = reverse_mask(g, e, beg, rcm, lane);
     float Ki[QF];
     load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
     const double4 xi = g.x4[i];
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
-    M128 it{own.lo | rev.lo, own.hi | rev.hi};
+    // Rounds over the slack-widened rc candidates, lane l taking the set bits of
+    // rank l, l+4, ...  In each round the group's four candidate records and
+    // positions are fetched cooperatively (every load instruction of the group
+    // touches one line), staged in LDS, and each lane evaluates its own pair.
+    const int nrounds = (mpop_count(rcm) + kLanes - 1) / kLanes;   // uniform within the group
+    float4 (*srec)[kLanes + 1] = stg.rec[a];
+    double4* spos = stg.pos[a];
+    M128 it = rcm;
     mdrop(it, lane);
-    while (!mempty(it)) {
-        const int p = mpop(it);
-        const bool is_own = mbit(own, p), is_rev = mbit(rev, p);
-        const int j = g.idx[beg + p];
-        float C[QF];
-        #pragma unroll
-        for (int k = 0; k < QF; ++k) C[k] = is_own ? Ki[k] : 0.0f;
-        if (is_rev) {
-            float Ko[QF];
-            load_coeff<LMASK, EXTRA>(e, g.eown[beg + p], wl, Ko);
-            // reverse pair enters with (-1)^l
-            if constexpr (LMASK & kL3) {
+    for (int rd = 0; rd < nrounds; ++rd) {
+        const int p = mnext(it);
+        const int rp = p >= 0 ? g.revpos[beg + p] : 0xFF;
+        const int o = p >= 0 ? g.eown[beg + p] : i;
+        const int j = p >= 0 ? g.idx[beg + p] : i;
+        const int o0 = quad_bcast<0>(o), o1 = quad_bcast<1>(o), o2 = quad_bcast<2>(o), o3 = quad_bcast<3>(o);
+        // positions: lanes 0,1 fetch the halves of candidate 2h, lanes 2,3 of candidate 2h+1
+        const int jh0 = __builtin_amdgcn_mov_dpp(j, 0 | (0 << 2) | (1 << 4) | (1 << 6), 0xF, 0xF, false);
+        const int jh1 = __builtin_amdgcn_mov_dpp(j, 2 | (2 << 2) | (3 << 4) | (3 << 6), 0xF, 0xF, false);
+        const float4 w0 = reinterpret_cast<const float4*>(g.x4 + jh0)[lane & 1];
+        const float4 w1 = reinterpret_cast<const float4*>(g.x4 + jh1)[lane & 1];
+        // records one quad at a time: lane l fetches float4 l of the quad of each
+        // of the four records, then reads back its own candidate's quad
+        float rr[NV * 4];
+        sfor<NQ>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            const float4 v0 = reinterpret_cast<const float4*>(e.rec + (int64_t)o0 * RS)[4 * q + lane];
+            const float4 v1 = reinterpret_cast<const float4*>(e.rec + (int64_t)o1 * RS)[4 * q + lane];
+            const float4 v2 = reinterpret_cast<const float4*>(e.rec + (int64_t)o2 * RS)[4 * q + lane];
+            const float4 v3 = reinterpret_cast<const float4*>(e.rec + (int64_t)o3 * RS)[4 * q + lane];
+            wave_lds_sync();
+            srec[0][lane] = v0; srec[1][lane] = v1; srec[2][lane] = v2; srec[3][lane] = v3;
+            wave_lds_sync();
+            sfor<4>([&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                if constexpr (4 * q + k < NV) {
+                    const float4 u4 = srec[lane][k];
+                    rr[16 * q + 4 * k] = u4.x; rr[16 * q + 4 * k + 1] = u4.y;
+                    rr[16 * q + 4 * k + 2] = u4.z; rr[16 * q + 4 * k + 3] = u4.w;
+                }
+            });
+        });
+        wave_lds_sync();
+        reinterpret_cast<float4*>(spos + (lane >> 1))[lane & 1] = w0;
+        reinterpret_cast<float4*>(spos + 2 + (lane >> 1))[lane & 1] = w1;
+        wave_lds_sync();
+        const double4 xj = spos[lane];
+        if (p >= 0) {
+            const uint64_t mlo = (uint64_t)__float_as_uint(rr[0]) | ((uint64_t)__float_as_uint(rr[1]) << 32);
+            const uint64_t mhi = (uint64_t)__float_as_uint(rr[2]) | ((uint64_t)__float_as_uint(rr[3]) << 32);
+            const bool is_own = mbit(own, p);
+            const bool is_rev = rp != 0xFF && (rp < 64 ? ((mlo >> rp) & 1ull) : ((mhi >> (rp - 64)) & 1ull));
+            if (is_own || is_rev) {
+                float Ko[QF];
+                coeff_from_record<LMASK, EXTRA>(e, o, wl, rr, Ko);
+                float C[QF];
                 #pragma unroll
-                for (int k = 0; k < 7; ++k) C[rec_off(LMASK, 3) + k] -= Ko[rec_off(LMASK, 3) + k];
-            }
-            if constexpr (LMASK & kL4) {
-                #pragma unroll
-                for (int k = 0; k < 9; ++k) C[rec_off(LMASK, 4) + k] += Ko[rec_off(LMASK, 4) + k];
-            }
-            if constexpr (LMASK & kL6) {
-                #pragma unroll
-                for (int k = 0; k < 13; ++k) C[rec_off(LMASK, 6) + k] += Ko[rec_off(LMASK, 6) + k];
+                for (int k = kRecMask; k < QF; ++k) C[k] = (is_own ? Ki[k] : 0.0f) + (is_rev ? Ko[k] : 0.0f);
+                // reverse pair enters with (-1)^l
+                if constexpr (LMASK & kL3) {
+                    #pragma unroll
+                    for (int k = 0; k < 7; ++k)
+                        C[rec_off(LMASK, 3) + k] = (is_own ? Ki[rec_off(LMASK, 3) + k] : 0.0f) -
+                                                   (is_rev ? Ko[rec_off(LMASK, 3) + k] : 0.0f);
+                }
+                double d0, d1, d2, r2;
+                {
+#pragma clang fp contract(off)
+                    d0 = xj.x - xi.x; d1 = xj.y - xi.y; d2 = xj.z - xi.z;
+                    r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                }
+                const float ir = __builtin_amdgcn_rsqf((float)r2);
+                const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
+                const float u2 = u * u;
+                float zr[LMAX + 1], zi[LMAX + 1];
+                zpow<LMAX>(nx, ny, zr, zi);
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                if constexpr (LMASK & kL3) acc_grad<3>(u, u2, zr, zi, C + rec_off(LMASK, 3), gx, gy, gz);
+                if constexpr (LMASK & kL4) acc_grad<4>(u, u2, zr, zi, C + rec_off(LMASK, 4), gx, gy, gz);
+                if constexpr (LMASK & kL6) acc_grad<6>(u, u2, zr, zi, C + rec_off(LMASK, 6), gx, gy, gz);
+                const float dot = gx * nx + gy * ny + gz * u;
+                Fx = __builtin_fmaf(ir, __builtin_fmaf(-dot, nx, gx), Fx);
+                Fy = __builtin_fmaf(ir, __builtin_fmaf(-dot, ny, gy), Fy);
+                Fz = __builtin_fmaf(ir, __builtin_fmaf(-dot, u, gz), Fz);
             }
         }
-        double d0, d1, d2;
-        double r2 = pair_r2(g.x4, j, xi.x, xi.y, xi.z, d0, d1, d2);
-        const float ir = __builtin_amdgcn_rsqf((float)r2);
-        const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
-        const float u2 = u * u;
-        float zr[LMAX + 1], zi[LMAX + 1];
-        zpow<LMAX>(nx, ny, zr, zi);
-        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        if constexpr (LMASK & kL3) acc_grad<3>(u, u2, zr, zi, C + rec_off(LMASK, 3), gx, gy, gz);
-        if constexpr (LMASK & kL4) acc_grad<4>(u, u2, zr, zi, C + rec_off(LMASK, 4), gx, gy, gz);
-        if constexpr (LMASK & kL6) acc_grad<6>(u, u2, zr, zi, C + rec_off(LMASK, 6), gx, gy, gz);
-        const float dot = gx * nx + gy * ny + gz * u;
-        Fx = __builtin_fmaf(ir, __builtin_fmaf(-dot, nx, gx), Fx);
-        Fy = __builtin_fmaf(ir, __builtin_fmaf(-dot, ny, gy), Fy);
-        Fz = __builtin_fmaf(ir, __builtin_fmaf(-dot, u, gz), Fz);
-        mdrop(it, kLanes - 1);
     }
     Fx = grp_sum(Fx); Fy = grp_sum(Fy); Fz = grp_sum(Fz);
     if (valid && lane == 0) {
