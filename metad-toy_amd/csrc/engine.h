@@ -61,6 +61,7 @@ struct Tile {
 // Geometry shared by all per-atom kernels (device pointers).
 struct Geo {
     const double4* x4;        // (n_local_total + n_ghost_total), xyz + pad, FP64
+    const float4* x4f;        // the same positions rounded to FP32 (list-scan prefilter)
     const int32_t* off;       // n_local_total + 1 (global CSR offsets)
     const int32_t* idx;       // global all-space indices
     const int32_t* owner;     // ghost -> local index (indexed by j - n_local_total)
@@ -205,7 +206,7 @@ constexpr int kStStride = 32;
 // ---- launchers (kernels.hip) ----
 hipError_t launch_topology(const Geo& g, int64_t n_local_total, int32_t* eown, uint8_t* revpos,
                            hipStream_t st);
-hipError_t launch_pack_x4(const double* x, double4* x4, int64_t n, hipStream_t st);
+hipError_t launch_pack_x4(const double* x, double4* x4, float4* x4f, int64_t n, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, hipStream_t st);
 hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st);
 hipError_t launch_select_literal(const Geo& g, int32_t first, int32_t count, double rc2, int N,

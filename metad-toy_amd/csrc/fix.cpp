@@ -239,6 +239,7 @@ struct mdt_ctx {
     bool any_group = false;
     DBuf<double> x;
     DBuf<double4> x4;
+    DBuf<float4> x4f;
     DBuf<int32_t> off, idx, owner, eown;
     DBuf<uint8_t> revpos, group;
     DBuf<Tile> tiles;
@@ -249,7 +250,7 @@ struct mdt_ctx {
 
     Geo geo() const {
         Geo g{};
-        g.x4 = x4.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p;
+        g.x4 = x4.p; g.x4f = x4f.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p;
         g.eown = eown.p; g.revpos = revpos.p;
         g.group = nullptr;
         g.tiles = tiles.p; g.ntiles = (int32_t)h_tiles.size(); g.n_local_total = (int32_t)total_local;
@@ -325,7 +326,8 @@ void mdt_ctx::commit() {
     idx.alloc(std::max<size_t>(hidx.size(), 1)); idx.upload(hidx.data(), hidx.size(), stream);
     owner.alloc(hown.size()); owner.upload(hown.data(), tg, stream);
     x4.alloc((size_t)(tl + tg));
-    HIPCHK(launch_pack_x4(x.p, x4.p, tl + tg, stream));
+    x4f.alloc((size_t)(tl + tg));
+    HIPCHK(launch_pack_x4(x.p, x4.p, x4f.p, tl + tg, stream));
     eown.alloc(std::max<size_t>(hidx.size(), 1));
     revpos.alloc(std::max<size_t>(hidx.size(), 1));
     if (any_group) { group.alloc(hgroup.size()); group.upload(hgroup.data(), hgroup.size(), stream); }
@@ -1438,9 +1440,9 @@ int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_devic
     HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * a.atom_off, x, sizeof(double) * 3 * a.n_local, k, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), x + 3 * a.n_local,
                           sizeof(double) * 3 * (a.n_all - a.n_local), k, ctx->stream));
-    HIPCHK(launch_pack_x4(ctx->x.p + 3 * a.atom_off, ctx->x4.p + a.atom_off, a.n_local, ctx->stream));
+    HIPCHK(launch_pack_x4(ctx->x.p + 3 * a.atom_off, ctx->x4.p + a.atom_off, ctx->x4f.p + a.atom_off, a.n_local, ctx->stream));
     HIPCHK(launch_pack_x4(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), ctx->x4.p + ctx->total_local + a.ghost_off,
-                          a.n_all - a.n_local, ctx->stream));
+                          ctx->x4f.p + ctx->total_local + a.ghost_off, a.n_all - a.n_local, ctx->stream));
     if (!on_device) HIPCHK(hipStreamSynchronize(ctx->stream));   // pageable source must stay valid
     API_END
 }

@@ -238,9 +238,13 @@ __global__ __launch_bounds__(256) void k_topology(Geo g, int64_t n, int32_t* __r
     }
 }
 
-__global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x4[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+__global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4, float4* __restrict__ x4f,
+                          int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = x[3 * i], b = x[3 * i + 1], c = x[3 * i + 2];
+        x4[i] = make_double4(a, b, c, 0.0);
+        x4f[i] = make_float4((float)a, (float)b, (float)c, 0.0f);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -360,9 +364,25 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     const int N = e.N;
     M128 own{0, 0}, rcm{0, 0};
     int cnt = 0;
-    // scan in batches of kScan entries per lane: all list loads of a batch, then
-    // all position loads, so a batch costs two memory round trips
+    // Scan in batches of kScan entries per lane (all list loads of a batch, then all
+    // position loads).  The FP32 positions decide every candidate whose FP32 r^2 is
+    // farther from the cutoffs than its rounding error can reach; the rest (and
+    // near-coincident pairs) are decided with the exact FP64 r^2, so the masks are
+    // the ones the FP64 scan gives.  Error bound for |coordinates| <= X and
+    // |d| <= rc: each FP32 difference is within del = 2uX + u|d| of the exact one,
+    // so |r2_32 - r2| <= 2 sqrt(3) rc del + 3 del^2 + 3u rc^2 (u = 2^-24); the
+    // margin is twice that.
     constexpr int kScan = 4;
+    const float4 xif = g.x4f[i];
+    float margin;
+    {
+        constexpr float u = 5.9604645e-8f;
+        const float rcf = (float)__builtin_sqrt(rc2s);
+        const float X = fmaxf(fmaxf(fabsf(xif.x), fabsf(xif.y)), fabsf(xif.z)) + rcf + 1.0f;
+        const float del = 2.0f * u * X + u * rcf * 1.01f;
+        margin = 2.0f * (3.4641016f * rcf * del + 3.0f * del * del + 3.0f * u * (float)rc2s) + 1e-6f;
+    }
+    const float r2lo = (float)rc2 - margin, r2hi = (float)rc2s + margin;
     for (int p0 = lane; p0 < c; p0 += kScan * kLanes) {
         int jj[kScan];
         #pragma unroll
@@ -370,18 +390,23 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
             const int p = p0 + u * kLanes;
             jj[u] = g.idx[beg + (p < c ? p : p0)];
         }
-        double4 xx[kScan];
+        float4 xx[kScan];
         #pragma unroll
-        for (int u = 0; u < kScan; ++u) xx[u] = g.x4[jj[u]];
+        for (int u = 0; u < kScan; ++u) xx[u] = g.x4f[jj[u]];
         #pragma unroll
         for (int u = 0; u < kScan; ++u) {
             const int p = p0 + u * kLanes;
             if (p < c) {
-#pragma clang fp contract(off)
-                const double d0 = xx[u].x - xi.x, d1 = xx[u].y - xi.y, d2 = xx[u].z - xi.z;
-                const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
-                if (!((r2 > rc2) || (r2 < 1e-12))) { ++cnt; mset(own, p); }
+                const float e0 = xx[u].x - xif.x, e1 = xx[u].y - xif.y, e2 = xx[u].z - xif.z;
+                const float r2f = __builtin_fmaf(e2, e2, __builtin_fmaf(e1, e1, e0 * e0));
+                if (r2f < r2lo && r2f > 1e-2f) {
+                    mset(rcm, p); ++cnt; mset(own, p);
+                } else if (!(r2f > r2hi)) {
+                    double d0, d1, d2;
+                    const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
+                    if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
+                    if (!((r2 > rc2) || (r2 < 1e-12))) { ++cnt; mset(own, p); }
+                }
             }
         }
     }
@@ -1409,10 +1434,10 @@ hipError_t launch_topology(const Geo& g, int64_t n, int32_t* eown, uint8_t* revp
     return hipGetLastError();
 }
 
-hipError_t launch_pack_x4(const double* x, double4* x4, int64_t n, hipStream_t st) {
+hipError_t launch_pack_x4(const double* x, double4* x4, float4* x4f, int64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     int64_t b = (n + 255) / 256; if (b > 8192) b = 8192;
-    hipLaunchKernelGGL(k_pack_x4, dim3((unsigned)b), dim3(256), 0, st, x, x4, n);
+    hipLaunchKernelGGL(k_pack_x4, dim3((unsigned)b), dim3(256), 0, st, x, x4, x4f, n);
     return hipGetLastError();
 }
 

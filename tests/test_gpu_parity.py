@@ -28,12 +28,22 @@ def _fix_q(mdt, ctx, N=18, rc=3.9, extra=""):
     return mdt.Fix(ctx, q46_fix(extra=extra, N=N, rc=rc))
 
 
-@pytest.mark.parametrize("case", ["fcc_jitter", "fcc_perfect", "liquid_capped"])
+@pytest.mark.parametrize("case", ["fcc_jitter", "fcc_perfect", "liquid_capped", "fcc_at_cutoff", "far_box"])
 def test_selection_bit_exact(mdt, orc, case):
     if case == "fcc_jitter":
         s, N, rc = fcc_system(mdt, 10, sigma=0.1, seed=11), 18, 3.9
     elif case == "fcc_perfect":
         s, N, rc = fcc_system(mdt, 6, sigma=0.0), 8, 3.9          # ties at the cap
+    elif case == "fcc_at_cutoff":
+        # rc equal to the nearest-neighbour distance: every first-shell r^2 sits on rc^2
+        # within rounding, so the in/out decisions are those of the exact FP64 r^2
+        s, N, rc = fcc_system(mdt, 6, sigma=0.0), 18, 5.26 / np.sqrt(2.0)
+    elif case == "far_box":
+        # coordinates near 5,000 A: the FP32 rounding margin of the list scan is wide
+        L = 8 * 5.26
+        s = mdt.System().set_box([5000.0] * 3, [5000.0 + L] * 3).add_fcc(8, 8, 8, 5.26, 0.1, 0.1, 0.0, seed=13)
+        s.build(5.9)
+        N, rc = 18, 3.9
     else:
         s, N, rc = liquid_system(mdt, 3000, density=0.05, seed=5), 12, 3.9   # cap binds often
     print(f"[selection] case={case} n={s.n_local} N={N}", flush=True)
@@ -52,6 +62,8 @@ def test_selection_bit_exact(mdt, orc, case):
     assert np.array_equal(fix.extract("Q6.count_rc", n).astype(np.int32), env.cnt)
     if case == "liquid_capped":
         assert (env.cnt > N).sum() > 100
+    if case == "fcc_at_cutoff":
+        assert env.cnt.min() < env.cnt.max() <= 12     # the boundary splits the first shell
 
 
 def test_q_values_and_forces_c1(mdt, orc):
