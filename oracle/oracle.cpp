@@ -325,9 +325,10 @@ void orc_ld_local(int l, int n_center, int N, int n_local, const int* ghost_owne
 // Used with K = w * (4pi/(2l+1)) q_cm / (Ng q_l(c)) for Q_l.AVE and with the
 // two-hop coefficient of orc_ld_coeff for the local form.
 // ---------------------------------------------------------------------------
+// centers[c] is the local atom of center c (NULL: center c is atom c).
 void orc_grad_from_coeff(int l, int n_center, int N, int n_local, const int* ghost_owner,
                          const int* sel_idx, const double* sel_d, const int* n_sel,
-                         const double* K, double* dVdx) {
+                         const double* K, double* dVdx, const int* centers) {
     const int W = 2 * (l + 1);
     std::vector<double> dY(6 * (l + 1));
     for (int c = 0; c < n_center; ++c) {
@@ -344,10 +345,11 @@ void orc_grad_from_coeff(int l, int n_center, int N, int n_local, const int* gho
                     G[a] += mu * (k[2 * m] * dY[m * 6 + 2 * a] + k[2 * m + 1] * dY[m * 6 + 2 * a + 1]);
             }
             int o = owner_of(sel_idx[(size_t)c * N + s], n_local, ghost_owner);
+            const size_t ci = centers ? (size_t)centers[c] : (size_t)c;
             for (int a = 0; a < 3; ++a) {
                 G[a] /= (double)n;
                 dVdx[3 * (size_t)o + a] += G[a];
-                dVdx[3 * (size_t)c + a] -= G[a];
+                dVdx[3 * ci + a] -= G[a];
             }
         }
     }

@@ -36,7 +36,7 @@ def lib():
             ("orc_steinhardt", None, [I, I, I, P, P, P, P]),
             ("orc_kahan_mean", D, [I, P, P, D]),
             ("orc_ld_local", None, [I, I, I, I, P, P, P, P, P, P]),
-            ("orc_grad_from_coeff", None, [I, I, I, I, P, P, P, P, P, P]),
+            ("orc_grad_from_coeff", None, [I, I, I, I, P, P, P, P, P, P, P]),
             ("orc_q_coeff", None, [I, I, P, P, D, D, P]),
             ("orc_ld_coeff", None, [I, I, I, I, P, P, P, P, P, D, D, P]),
             ("orc_sk", D, [I, P, I, P, P, P]),
@@ -150,14 +150,16 @@ class Env:
         return qb, lq
 
     def dQ_dx(self, l, w=1.0):
-        """w * d Q_l.AVE / dx for local atoms (n_local, 3)."""
-        assert self._full
+        """w * d Q_l.AVE / dx for local atoms (n_local, 3); Q_l.AVE averages over the
+        centers (all local atoms, or the group), and the gradient covers every atom."""
         q, ql = self.steinhardt(l)
+        nc = self.centers.size
         K = np.zeros_like(q)
-        lib().orc_q_coeff(l, self.n_local, _p(q), _p(ql), float(self.n_local), w, _p(K))
+        lib().orc_q_coeff(l, nc, _p(q), _p(ql), float(nc), w, _p(K))
         out = np.zeros((self.n_local, 3))
-        lib().orc_grad_from_coeff(l, self.n_local, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
-                                  _p(self.sel_d), _p(self.nsel), _p(K), _p(out))
+        lib().orc_grad_from_coeff(l, nc, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
+                                  _p(self.sel_d), _p(self.nsel), _p(K), _p(out),
+                                  None if self._full else _p(self.centers))
         return out
 
     def dLQ_dx(self, l, w=1.0):
@@ -167,7 +169,7 @@ class Env:
                            _p(self.nsel), _p(qb), _p(lq), float(self.n_local), w, _p(K))
         out = np.zeros((self.n_local, 3))
         lib().orc_grad_from_coeff(l, self.n_local, self.N, self.n_local, _p(self.owner), _p(self.sel_idx),
-                                  _p(self.sel_d), _p(self.nsel), _p(K), _p(out))
+                                  _p(self.sel_d), _p(self.nsel), _p(K), _p(out), None)
         return out
 
 

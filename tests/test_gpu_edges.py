@@ -1,0 +1,160 @@
+"""GPU parity at the edges: group-restricted CALs, isolated atoms, long lists with a large
+cap, odd degree (Q3) forces, and walkers from empty to 4,000 atoms (run with -m gpu).
+
+Bars as in test_gpu_parity.py: selection bit-exact, CV 1e-5 relative, forces 1e-4
+relative (max |F - F_ref| / max |F_ref|).
+"""
+import numpy as np
+import pytest
+
+from helpers import fcc_system, liquid_system, rel_force_err
+
+pytestmark = pytest.mark.gpu
+
+CV_TOL = 1e-5
+F_TOL = 1e-4
+
+
+def _one(mdt, system, args, group=None):
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, system)
+    if group is not None:
+        ctx.bind_group(0, group)
+    return ctx, mdt.Fix(ctx, args)
+
+
+def _biased_step(fix, s_center, n):
+    """One step with a hill placed next to the CV value, so dV/ds is well away from zero."""
+    fix.init()
+    fix.add_hills(np.array([[s_center]]), np.array([0.005]), 0.01)
+    f = np.zeros((n, 3))
+    fix.post_force(1, f)
+    return f
+
+
+def test_group_restricted_cal(mdt, orc):
+    """STEINH on a group: centers are the group's atoms, Q6.AVE divides by the group size,
+    neighbours are any atoms, and forces act on group atoms only (the reference applies
+    the bias to group atoms, CVs_Steinhardt.cu:841-868)."""
+    s = fcc_system(mdt, 8, sigma=0.1, seed=41)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    grp = (x[:n, 2] < x[:n, 2].mean()).astype(np.uint8)
+    print(f"[group] n={n} in group={int(grp.sum())}", flush=True)
+    env = orc.Env(x, off, idx, own, n, 3.9, 18, group=grp)
+    q_ref = env.q_ave(6)
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                            "CAL NAME Q6 STEINH Q 6 lower cutoff_r 3.9 cutoff_Natoms 18 "
+                            "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0", group=grp)
+    f = _biased_step(fix, q_ref + 0.004, n)
+    assert abs(fix.cv("Q6.AVE") - q_ref) <= CV_TOL * q_ref
+    w = fix.cv_weight("Q6.AVE")
+    assert w != 0.0
+    F_ref = -env.dQ_dx(6, w)
+    inside = grp.astype(bool)
+    assert rel_force_err(f[inside], F_ref[inside]) <= F_TOL
+    assert np.all(f[~inside] == 0.0)
+
+
+def test_isolated_atoms(mdt, orc):
+    """A dilute gas: many atoms have no neighbour within rc (q_l = 0, no force), and count
+    in the average as the reference counts them."""
+    s = liquid_system(mdt, 3000, density=0.004, seed=42)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    assert (env.nsel == 0).sum() > n // 4
+    q_ref = env.q_ave(6)
+    print(f"[isolated] n={n} empty={(env.nsel == 0).sum()} Q6={q_ref:.6f}", flush=True)
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                            "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0")
+    f = _biased_step(fix, q_ref + 0.004, n)
+    assert abs(fix.cv("Q6.AVE") - q_ref) <= CV_TOL * q_ref
+    assert np.array_equal(fix.extract("Q6.nsel", n).astype(np.int32), env.nsel)
+    w = fix.cv_weight("Q6.AVE")
+    F_ref = -env.dQ_dx(6, w)
+    assert rel_force_err(f, F_ref) <= F_TOL
+    assert np.all(f[env.nsel + np.bincount(env.owner_of(env.sel_idx[env.sel_idx >= 0]), minlength=n) == 0] == 0.0)
+
+
+def test_long_lists_large_cap(mdt, orc):
+    """Lists longer than 64 entries (both halves of the 128-bit masks) and a cap of 40
+    within 5 A that binds for many atoms: selection bit-exact, CVs and forces."""
+    s = liquid_system(mdt, 4000, density=0.09, seed=43, rg=6.2)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    c = np.diff(off)
+    N, rc = 40, 5.0
+    env = orc.Env(x, off, idx, own, n, rc, N)
+    print(f"[long] n={n} list max={c.max()} mean={c.mean():.1f} capped={(env.cnt > N).sum()}", flush=True)
+    assert c.max() > 64 and c.max() <= 128
+    assert (env.cnt > N).sum() > 50
+    args = ("GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+            f"CAL NAME Q6 STEINH Q 6 all cutoff_r {rc} cutoff_Natoms {N} "
+            f"CAL NAME Q4 STEINH Q 4 all cutoff_r {rc} cutoff_Natoms {N} "
+            "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE DIM 1 0 2 2000 v1+v2 WT 0")
+    ctx, fix = _one(mdt, s, args)
+    fix.init()
+    fix.post_force(0)
+    sel, cnt, ns = fix.selection("Q6", n, N)
+    assert np.array_equal(sel, env.sel_idx)
+    assert np.array_equal(fix.extract("Q6.nsel", n).astype(np.int32), env.nsel)
+    q6, q4 = env.q_ave(6), env.q_ave(4)
+    f = np.zeros((n, 3))
+    fix.add_hills(np.array([[q6 + q4 + 0.004]]), np.array([0.005]), 0.01)
+    fix.post_force(1, f)
+    assert abs(fix.cv("Q6.AVE") - q6) <= CV_TOL * q6
+    assert abs(fix.cv("Q4.AVE") - q4) <= CV_TOL * q4
+    w6, w4 = fix.cv_weight("Q6.AVE"), fix.cv_weight("Q4.AVE")
+    F_ref = -(env.dQ_dx(6, w6) + env.dQ_dx(4, w4))
+    assert rel_force_err(f, F_ref) <= F_TOL
+
+
+def test_odd_degree_forces(mdt, orc):
+    """Q3 (odd l: the reverse pair enters with -1) together with Q6 on one environment."""
+    s = fcc_system(mdt, 6, sigma=0.15, seed=44)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    q3, q6 = env.q_ave(3), env.q_ave(6)
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                            "CAL NAME Q3 STEINH Q 3 all cutoff_r 3.9 cutoff_Natoms 18 "
+                            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                            "CV_dim 1 SYMBOL v1 Q3.AVE SYMBOL v2 Q6.AVE DIM 1 0 2 2000 v1+v2 WT 0")
+    f = _biased_step(fix, q3 + q6 + 0.004, n)
+    assert abs(fix.cv("Q3.AVE") - q3) <= CV_TOL * q3
+    w3, w6 = fix.cv_weight("Q3.AVE"), fix.cv_weight("Q6.AVE")
+    assert w3 != 0.0
+    F_ref = -(env.dQ_dx(3, w3) + env.dQ_dx(6, w6))
+    assert rel_force_err(f, F_ref) <= F_TOL
+
+
+def test_walkers_from_tiny_to_large(mdt, orc):
+    """Walkers of 108, 4,000 and 500 atoms in one context: each matches the oracle on its own."""
+    systems = [fcc_system(mdt, 3, sigma=0.1, seed=45), fcc_system(mdt, 10, sigma=0.1, seed=46),
+               fcc_system(mdt, 5, sigma=0.12, seed=47)]
+    ctx = mdt.Context(0, len(systems))
+    for w, s in enumerate(systems):
+        ctx.bind_system(w, s)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                       "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                       "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0")
+    fix.init()
+    envs = []
+    for w, s in enumerate(systems):
+        x, off, idx, own = s.arrays()
+        env = orc.Env(x, off, idx, own, s.n_local, 3.9, 18)
+        envs.append(env)
+        fix.add_hills(np.array([[env.q_ave(6) + 0.004]]), np.array([0.005]), 0.01, walker=w)
+    ntot = sum(s.n_local for s in systems)
+    f = np.zeros((ntot, 3))
+    fix.post_force(1, f)
+    o = 0
+    for w, (s, env) in enumerate(zip(systems, envs)):
+        n = s.n_local
+        q = env.q_ave(6)
+        assert abs(fix.cv("Q6.AVE", w) - q) <= CV_TOL * q, w
+        F_ref = -env.dQ_dx(6, fix.cv_weight("Q6.AVE", w))
+        assert rel_force_err(f[o:o + n], F_ref) <= F_TOL, w
+        o += n

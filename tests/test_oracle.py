@@ -216,6 +216,27 @@ def test_gradient_with_binding_cap(mdt, orc):
             assert abs(fd - g[k, a]) <= 1e-6 * np.abs(g).max() + 1e-9
 
 
+def test_group_gradient_fd(mdt, orc):
+    """CAL on a group: Q6.AVE averages over the group's centers only; the gradient of that
+    average reaches group and non-group atoms alike and matches central differences."""
+    s = fcc_system(mdt, 4, sigma=0.2, seed=3)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    grp = (x[:n, 0] < x[:n, 0].mean()).astype(np.uint8)
+    env = orc.Env(x, off, idx, own, n, 3.9, 18, group=grp)
+    q, ql = env.steinhardt(6)
+    assert abs(env.q_ave(6) - ql.sum() / grp.sum()) < 1e-15
+    g = env.dQ_dx(6)
+    h = 1e-6
+    outside = np.nonzero(grp == 0)[0]
+    for k in (int(np.nonzero(grp)[0][3]), int(outside[5]), 100):
+        for a in range(3):
+            p = orc.Env(_move_all(x, n, own, k, a, h), off, idx, own, n, 3.9, 18, group=grp)
+            m = orc.Env(_move_all(x, n, own, k, a, -h), off, idx, own, n, 3.9, 18, group=grp)
+            fd = (p.q_ave(6) - m.q_ave(6)) / (2 * h)
+            assert abs(fd - g[k, a]) <= 1e-6 * np.abs(g).max() + 1e-9
+
+
 # ------------------------------------------------------------------ grid
 def test_grid_single_hill_closed_form(orc):
     g = orc.Grid([0.0], [1.0], [1000])
