@@ -238,12 +238,43 @@ __global__ __launch_bounds__(256) void k_topology(Geo g, int64_t n, int32_t* __r
     }
 }
 
+// Packed FP32 position: xyz rounded to FP32, and in the fourth word the FP64
+// rounding residual x - fl32(x) of each coordinate as a signed 10-bit multiple of
+// ulp(fl32(x))/1024.  One 16-byte load then gives a pair vector to ~1e-7
+// relative, the precision of the FP32 arithmetic that consumes it.
+__device__ __forceinline__ uint32_t pack_res1(double x, float xf) {
+    const uint32_t E = (__float_as_uint(xf) >> 23) & 0xFFu;
+    if (E < 34u) return 0u;
+    const double sc = (double)__uint_as_float((E - 33u) << 23);   // ulp(xf) / 1024, a power of two
+    double q = rint((x - (double)xf) / sc);
+    q = fmin(fmax(q, -511.0), 511.0);
+    return (uint32_t)(int)q & 0x3FFu;
+}
+__device__ __forceinline__ float unpack_res1(float xf, uint32_t w, int c) {
+    const int q = ((int)(w << (22 - 10 * c))) >> 22;
+    const uint32_t E = (__float_as_uint(xf) >> 23) & 0xFFu;
+    return (float)q * (E >= 34u ? __uint_as_float((E - 33u) << 23) : 0.0f);
+}
+struct PPos { float x, y, z, rx, ry, rz; };
+__device__ __forceinline__ PPos unpack_pos(const float4& p) {
+    const uint32_t w = __float_as_uint(p.w);
+    return PPos{p.x, p.y, p.z, unpack_res1(p.x, w, 0), unpack_res1(p.y, w, 1), unpack_res1(p.z, w, 2)};
+}
+// x_j - x_i from packed positions
+__device__ __forceinline__ void pdiff(const PPos& pj, const PPos& pi, float& d0, float& d1, float& d2) {
+    d0 = (pj.x - pi.x) + (pj.rx - pi.rx);
+    d1 = (pj.y - pi.y) + (pj.ry - pi.ry);
+    d2 = (pj.z - pi.z) + (pj.rz - pi.rz);
+}
+
 __global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4, float4* __restrict__ x4f,
                           int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double a = x[3 * i], b = x[3 * i + 1], c = x[3 * i + 2];
         x4[i] = make_double4(a, b, c, 0.0);
-        x4f[i] = make_float4((float)a, (float)b, (float)c, 0.0f);
+        const float af = (float)a, bf = (float)b, cf = (float)c;
+        const uint32_t w = pack_res1(a, af) | (pack_res1(b, bf) << 10) | (pack_res1(c, cf) << 20);
+        x4f[i] = make_float4(af, bf, cf, __uint_as_float(w));
     }
 }
 
@@ -365,24 +396,36 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     M128 own{0, 0}, rcm{0, 0};
     int cnt = 0;
     // Scan in batches of kScan entries per lane (all list loads of a batch, then all
-    // position loads).  The FP32 positions decide every candidate whose FP32 r^2 is
-    // farther from the cutoffs than its rounding error can reach; the rest (and
-    // near-coincident pairs) are decided with the exact FP64 r^2, so the masks are
-    // the ones the FP64 scan gives.  Error bound for |coordinates| <= X and
-    // |d| <= rc: each FP32 difference is within del = 2uX + u|d| of the exact one,
-    // so |r2_32 - r2| <= 2 sqrt(3) rc del + 3 del^2 + 3u rc^2 (u = 2^-24); the
-    // margin is twice that.
+    // packed-position loads).  The FP32 pair vector from the packed positions is
+    // within |dd| <= 2u|d| + uX/128 of the exact one (u = 2^-24, |coordinates| <= X),
+    // so its r^2 is within 7u r^2 + 0.03 u X rc of the FP64 r^2; every candidate
+    // farther than twice that from the cutoffs is decided in FP32, the rest (and
+    // near-coincident pairs) with the exact FP64 r^2, so the masks are the FP64
+    // scan's.  Entries within rc add their Y_lm at once: when the cap does not bind
+    // they are exactly the selected set.
     constexpr int kScan = 4;
-    const float4 xif = g.x4f[i];
+    const PPos pi = unpack_pos(g.x4f[i]);
     float margin;
     {
         constexpr float u = 5.9604645e-8f;
         const float rcf = (float)__builtin_sqrt(rc2s);
-        const float X = fmaxf(fmaxf(fabsf(xif.x), fabsf(xif.y)), fabsf(xif.z)) + rcf + 1.0f;
-        const float del = 2.0f * u * X + u * rcf * 1.01f;
-        margin = 2.0f * (3.4641016f * rcf * del + 3.0f * del * del + 3.0f * u * (float)rc2s) + 1e-6f;
+        const float X = fmaxf(fmaxf(fabsf(pi.x), fabsf(pi.y)), fabsf(pi.z)) + rcf + 1.0f;
+        margin = 2.0f * (7.0f * u * (float)rc2s + 0.03f * u * X * rcf) + 1e-6f;
     }
     const float r2lo = (float)rc2 - margin, r2hi = (float)rc2s + margin;
+    float q[QF];
+    #pragma unroll
+    for (int k = 0; k < QF; ++k) q[k] = 0.0f;
+    auto add_ylm = [&](float d0, float d1, float d2, float r2f) {
+        const float ir = __builtin_amdgcn_rsqf(r2f);
+        const float nx = d0 * ir, ny = d1 * ir, uu = d2 * ir;
+        const float u2 = uu * uu;
+        float zr[LMAX + 1], zi[LMAX + 1];
+        zpow<LMAX>(nx, ny, zr, zi);
+        if constexpr (LMASK & kL3) acc_ylm<3>(uu, u2, zr, zi, q + rec_off(LMASK, 3));
+        if constexpr (LMASK & kL4) acc_ylm<4>(uu, u2, zr, zi, q + rec_off(LMASK, 4));
+        if constexpr (LMASK & kL6) acc_ylm<6>(uu, u2, zr, zi, q + rec_off(LMASK, 6));
+    };
     for (int p0 = lane; p0 < c; p0 += kScan * kLanes) {
         int jj[kScan];
         #pragma unroll
@@ -397,15 +440,22 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         for (int u = 0; u < kScan; ++u) {
             const int p = p0 + u * kLanes;
             if (p < c) {
-                const float e0 = xx[u].x - xif.x, e1 = xx[u].y - xif.y, e2 = xx[u].z - xif.z;
+                float e0, e1, e2;
+                pdiff(unpack_pos(xx[u]), pi, e0, e1, e2);
                 const float r2f = __builtin_fmaf(e2, e2, __builtin_fmaf(e1, e1, e0 * e0));
+                bool in = false;
                 if (r2f < r2lo && r2f > 1e-2f) {
-                    mset(rcm, p); ++cnt; mset(own, p);
+                    mset(rcm, p); in = true;
                 } else if (!(r2f > r2hi)) {
                     double d0, d1, d2;
                     const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
                     if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
-                    if (!((r2 > rc2) || (r2 < 1e-12))) { ++cnt; mset(own, p); }
+                    in = !((r2 > rc2) || (r2 < 1e-12));
+                }
+                if (in) {
+                    ++cnt;
+                    mset(own, p);
+                    add_ylm(e0, e1, e2, r2f);
                 }
             }
         }
@@ -424,38 +474,18 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
                 if (s < N && sp[s] >= 0) mset(own, sp[s]);
         }
         own = grp_bcast0(own);
+        // the cap binds: the Y_lm of the kept set only
+        #pragma unroll
+        for (int k = 0; k < QF; ++k) q[k] = 0.0f;
+        M128 it = own;
+        mdrop(it, lane);
+        for (int p = mnext(it); p >= 0; p = mnext(it)) {
+            float e0, e1, e2;
+            pdiff(unpack_pos(g.x4f[g.idx[beg + p]]), pi, e0, e1, e2);
+            add_ylm(e0, e1, e2, __builtin_fmaf(e2, e2, __builtin_fmaf(e1, e1, e0 * e0)));
+        }
     }
     const int n = cnt < N ? cnt : N;
-    float q[QF];
-    #pragma unroll
-    for (int k = 0; k < QF; ++k) q[k] = 0.0f;
-    M128 it = own;
-    mdrop(it, lane);
-    // two-stage prefetch: the next pair's position is in flight while this one is evaluated
-    int p = mnext(it);
-    double4 xc = g.x4[p >= 0 ? g.idx[beg + p] : i];
-    int pn = mnext(it);
-    int jn = pn >= 0 ? g.idx[beg + pn] : i;
-    while (p >= 0) {
-        const double4 xn = g.x4[jn];
-        const int pnn = mnext(it);
-        const int jnn = pnn >= 0 ? g.idx[beg + pnn] : i;
-        double d0, d1, d2, r2;
-        {
-#pragma clang fp contract(off)
-            d0 = xc.x - xi.x; d1 = xc.y - xi.y; d2 = xc.z - xi.z;
-            r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        }
-        const float ir = __builtin_amdgcn_rsqf((float)r2);
-        const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
-        const float u2 = u * u;
-        float zr[LMAX + 1], zi[LMAX + 1];
-        zpow<LMAX>(nx, ny, zr, zi);
-        if constexpr (LMASK & kL3) acc_ylm<3>(u, u2, zr, zi, q + rec_off(LMASK, 3));
-        if constexpr (LMASK & kL4) acc_ylm<4>(u, u2, zr, zi, q + rec_off(LMASK, 4));
-        if constexpr (LMASK & kL6) acc_ylm<6>(u, u2, zr, zi, q + rec_off(LMASK, 6));
-        p = pn; xc = xn; pn = pnn; jn = jnn;
-    }
     const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
     #pragma unroll
     for (int k = kRecMask; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
@@ -613,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
     constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
     const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
     const float beta = lq > 0.0f ? (float)(P * e.inv_ng[tl.walker]) / (lq * (float)(n + 1)) : 0.0f;
-    if (valid && mine) {
+    if (valid && mine && d.wslot >= 0) {      // gradient coefficients only when LQ.AVE is biased
         float o4[4];
         #pragma unroll
         for (int k = 0; k < 4; ++k) o4[k] = 4 * sl + k < W ? beta * qb[k] : 0.0f;
@@ -743,10 +773,9 @@ __device__ __forceinline__ int quad_bcast(int v) {
 
 // LDS staging for the cooperative candidate loads: per lane group one 64-byte
 // quad of each of the four candidate records (padded by one float4 so records and
-// groups fall on distinct banks) and the four positions.
+// groups fall on distinct banks).
 struct ForceStage {
     float4 rec[kBlock / kLanes][kLanes][kLanes + 1];
-    double4 pos[kBlock / kLanes][kLanes];
 };
 
 template <int LMASK, bool EXTRA>
@@ -775,15 +804,15 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     const int beg = g.off[i];
     float Ki[QF];
     load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
-    const double4 xi = g.x4[i];
+    const PPos pi = unpack_pos(g.x4f[i]);
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
     // Rounds over the slack-widened rc candidates, lane l taking the set bits of
-    // rank l, l+4, ...  In each round the group's four candidate records and
-    // positions are fetched cooperatively (every load instruction of the group
-    // touches one line), staged in LDS, and each lane evaluates its own pair.
+    // rank l, l+4, ...  In each round the group's four candidate records are
+    // fetched cooperatively (every load instruction of the group touches one line)
+    // and staged in LDS; each lane loads its own candidate's packed position and
+    // evaluates its pair.
     const int nrounds = (mpop_count(rcm) + kLanes - 1) / kLanes;   // uniform within the group
     float4 (*srec)[kLanes + 1] = stg.rec[a];
-    double4* spos = stg.pos[a];
     M128 it = rcm;
     mdrop(it, lane);
     for (int rd = 0; rd < nrounds; ++rd) {
@@ -792,11 +821,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
         const int o = p >= 0 ? g.eown[beg + p] : i;
         const int j = p >= 0 ? g.idx[beg + p] : i;
         const int o0 = quad_bcast<0>(o), o1 = quad_bcast<1>(o), o2 = quad_bcast<2>(o), o3 = quad_bcast<3>(o);
-        // positions: lanes 0,1 fetch the halves of candidate 2h, lanes 2,3 of candidate 2h+1
-        const int jh0 = __builtin_amdgcn_mov_dpp(j, 0 | (0 << 2) | (1 << 4) | (1 << 6), 0xF, 0xF, false);
-        const int jh1 = __builtin_amdgcn_mov_dpp(j, 2 | (2 << 2) | (3 << 4) | (3 << 6), 0xF, 0xF, false);
-        const float4 w0 = reinterpret_cast<const float4*>(g.x4 + jh0)[lane & 1];
-        const float4 w1 = reinterpret_cast<const float4*>(g.x4 + jh1)[lane & 1];
+        const float4 pj = g.x4f[j];                  // this lane's candidate, packed position
         // records one quad at a time: lane l fetches float4 l of the quad of each
         // of the four records, then reads back its own candidate's quad
         float rr[NV * 4];
@@ -818,11 +843,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
                 }
             });
         });
-        wave_lds_sync();
-        reinterpret_cast<float4*>(spos + (lane >> 1))[lane & 1] = w0;
-        reinterpret_cast<float4*>(spos + 2 + (lane >> 1))[lane & 1] = w1;
-        wave_lds_sync();
-        const double4 xj = spos[lane];
         if (p >= 0) {
             const uint64_t mlo = (uint64_t)__float_as_uint(rr[0]) | ((uint64_t)__float_as_uint(rr[1]) << 32);
             const uint64_t mhi = (uint64_t)__float_as_uint(rr[2]) | ((uint64_t)__float_as_uint(rr[3]) << 32);
@@ -841,14 +861,10 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
                         C[rec_off(LMASK, 3) + k] = (is_own ? Ki[rec_off(LMASK, 3) + k] : 0.0f) -
                                                    (is_rev ? Ko[rec_off(LMASK, 3) + k] : 0.0f);
                 }
-                double d0, d1, d2, r2;
-                {
-#pragma clang fp contract(off)
-                    d0 = xj.x - xi.x; d1 = xj.y - xi.y; d2 = xj.z - xi.z;
-                    r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                }
-                const float ir = __builtin_amdgcn_rsqf((float)r2);
-                const float nx = (float)d0 * ir, ny = (float)d1 * ir, u = (float)d2 * ir;
+                float d0, d1, d2;
+                pdiff(unpack_pos(pj), pi, d0, d1, d2);
+                const float ir = __builtin_amdgcn_rsqf(__builtin_fmaf(d2, d2, __builtin_fmaf(d1, d1, d0 * d0)));
+                const float nx = d0 * ir, ny = d1 * ir, u = d2 * ir;
                 const float u2 = u * u;
                 float zr[LMAX + 1], zi[LMAX + 1];
                 zpow<LMAX>(nx, ny, zr, zi);
