@@ -163,10 +163,27 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 }
 inline int xcd_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
+// FP64 lane move through DPP (both 32-bit halves); CTRL is the DPP control word.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)__double_as_longlong(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(__double_as_longlong(v) >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)__double_as_longlong(v), l);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(__double_as_longlong(v) >> 32), l);
+    return __longlong_as_double(((long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+// Sum over the 64 lanes of a wave (all lanes active) in a fixed order: pairs in
+// quads, quads in rows (mirrors), then the four row totals in order; every lane
+// gets the same value.
 __device__ __forceinline__ double wave_sum(double v) {
-    #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    return ((readlane_f64(v, 15) + readlane_f64(v, 31)) + readlane_f64(v, 47)) + readlane_f64(v, 63);
 }
 
 // Block-wide FP64 sum of NV values per thread in a fixed order; result on thread 0.
@@ -1155,24 +1172,35 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     }
     // 1. CV values: one deterministic FP64 tree over tiles per slot (each thread
     //    sums its tiles in a fixed order, then waves, then the wave totals in
-    //    order).  Slots go one after another so only U loads per thread are live:
-    //    a 1024-thread block has 128 VGPRs per lane.
-    constexpr int U = 8;
+    //    order).  Up to four slots at a time have all their loads in flight
+    //    together; a 1024-thread block has 128 VGPRs per lane.
+    constexpr int U = 4;
+    constexpr int SG = 4;
     constexpr int NW = kRedThreads / 64;
     const int lane64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int c = 0; c < r.ncv; ++c) {
-        const int src = r.cv_src[c];
-        if (src < 0) continue;
-        const double* pp = r.partial + (int64_t)src * r.ntiles;
-        double acc = 0.0;
+    for (int c0 = 0; c0 < r.ncv; c0 += SG) {
+        double acc[SG];
+        #pragma unroll
+        for (int g = 0; g < SG; ++g) acc[g] = 0.0;
         for (int t = t0 + threadIdx.x; t < t1; t += U * kRedThreads) {
-            double x[U];
+            double x[SG][U];
             #pragma unroll
-            for (int u = 0; u < U; ++u) x[u] = (t + u * kRedThreads < t1) ? pp[t + u * kRedThreads] : 0.0;
-            acc += ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+            for (int g = 0; g < SG; ++g) {
+                const int c = c0 + g;
+                const int src = c < r.ncv ? r.cv_src[c] : -1;
+                const double* pp = r.partial + (int64_t)(src >= 0 ? src : 0) * r.ntiles;
+                #pragma unroll
+                for (int u = 0; u < U; ++u)
+                    x[g][u] = (src >= 0 && t + u * kRedThreads < t1) ? pp[t + u * kRedThreads] : 0.0;
+            }
+            #pragma unroll
+            for (int g = 0; g < SG; ++g) acc[g] += (x[g][0] + x[g][1]) + (x[g][2] + x[g][3]);
         }
-        acc = wave_sum(acc);
-        if (lane64 == 0) sh[c * NW + wv] = acc;
+        #pragma unroll
+        for (int g = 0; g < SG; ++g) {
+            const double v = wave_sum(acc[g]);
+            if (lane64 == 0 && c0 + g < r.ncv) sh[(c0 + g) * NW + wv] = v;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
