@@ -90,6 +90,11 @@ struct EnvDev {
     const double* inv_ng;     // per walker 1/N_group
     const double* wcv;        // per walker [kMaxSlotCV] weights dV/dv (from the reduce kernel)
     int32_t wslot[3];         // wcv slot of Q_l.AVE per present degree (-1 = not biased)
+    // an unbiased Lechner-Dellago CV on this (biased) environment, accumulated by
+    // the force kernel from the neighbour records it loads anyway (0 = none)
+    int32_t fld_l;
+    int32_t fld_slot;         // its partial-sum slot
+    float* fld_lq;            // its per-atom lq
 };
 
 // Lechner-Dellago local Q_l on an environment.
@@ -163,7 +168,7 @@ struct ReduceDev {
     const double* inv_ng;     // per walker
     const double* direct;     // per walker [kMaxSlotCV] values written by other kernels
     int32_t ncv;              // value slots
-    int32_t cv_src[kMaxSlotCV];  // >= 0: partial slot; -1: direct
+    int32_t cv_src[kMaxSlotCV];  // >= 0: partial slot; -1: direct; -2 - s: partial slot s summed after the force pass
     int32_t nsym;
     int32_t sym_slot[kMaxSym];
     int32_t D;
@@ -215,6 +220,8 @@ hipError_t launch_ld(const Geo& g, const EnvDev& e, const LdDev& d, hipStream_t 
 hipError_t launch_ld_coeff(const Geo& g, const EnvDev& e, const LdDev& d, hipStream_t st);
 hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st);
 hipError_t launch_reduce(const ReduceDev& r, hipStream_t st);
+// values of CVs whose partial sums are produced after the reduce (unbiased, fused into the force pass)
+hipError_t launch_cv_tail(const ReduceDev& r, hipStream_t st);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
 hipError_t launch_virial(const double4* x4, const double* f, const int32_t* image, int first, int n,
                          const double L[3], double* out, hipStream_t st);

@@ -368,6 +368,7 @@ struct CvH {
     int env = -1;
     int slot = -1;          // value slot
     int psum = -1;          // partial slot (Q, LD)
+    bool fused = false;     // LD: unbiased, accumulated by the force pass of its environment
     bool biased = false;
     // LD
     DBuf<float> lq, bq;
@@ -854,12 +855,26 @@ void fix_build(mdt_fix* fx) {
             d.extra = nullptr;
         }
     }
+    // an unbiased local-Q CV on a biased environment without a biased local-Q rides
+    // on that environment's force pass (one per environment)
+    for (auto* e : fx->envs) { e->dev.fld_l = 0; e->dev.fld_slot = -1; e->dev.fld_lq = nullptr; }
+    for (auto* c : fx->cvs) {
+        c->fused = false;
+        if (c->kind != CV_LD || c->biased) continue;
+        EnvH* e = fx->envs[c->env];
+        if (e->biased && e->dev.extra == nullptr && e->dev.fld_l == 0) { c->fused = true; e->dev.fld_l = c->l; }
+    }
     for (auto* c : fx->cvs) {
         if (c->kind == CV_LD) {
             c->psum = nsum++;
             const int W = 2 * c->l + 1, BS = (W + 3) / 4 * 4;
             c->lq.alloc(NT);
             c->bq.alloc((size_t)NT * BS);
+            if (c->fused) {
+                EnvH* e = fx->envs[c->env];
+                e->dev.fld_slot = c->psum;
+                e->dev.fld_lq = c->lq.p;
+            }
         } else if (c->kind == CV_SK) {
             const int nk = (int)(c->kvec.size() / 3);
             std::vector<double> krev(c->kvec.size());
@@ -947,7 +962,8 @@ void fix_build(mdt_fix* fx) {
     r.direct = fx->direct.p;
     r.ncv = (int)fx->cvs.size();
     // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
-    for (size_t c = 0; c < fx->cvs.size(); ++c) r.cv_src[c] = fx->cvs[c]->kind == CV_SK ? -1 : fx->cvs[c]->psum;
+    for (size_t c = 0; c < fx->cvs.size(); ++c)
+        r.cv_src[c] = fx->cvs[c]->kind == CV_SK ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
     r.inv_ng = fx->envs.empty() ? nullptr : fx->envs[0]->inv_ng.p;
     for (auto* e : fx->envs)
         if (e->group != fx->envs[0]->group) throw Err(MDT_ENOTSUP, "all STEINH CALs must use the same group");
@@ -1217,7 +1233,7 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         T.run("k_env", st, [&] { HIPCHK(launch_env(ge, e->dev, st)); });
     }
     for (auto* c : fx->cvs) {
-        if (c->kind == CV_LD) {
+        if (c->kind == CV_LD && !c->fused) {
             EnvH* e = fx->envs[c->env];
             Geo ge = g;
             ge.group = e->group == "all" ? nullptr : ctx->group.p;
@@ -1255,6 +1271,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(g, s, f, st)); });
         }
     }
+    bool tail = false;
+    for (auto* c : fx->cvs) tail |= c->fused;
+    if (tail) T.run("k_cv_tail", st, [&] { HIPCHK(launch_cv_tail(fx->red, st)); });
 }
 
 void use_device(mdt_ctx* ctx) {

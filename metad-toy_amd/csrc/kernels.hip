@@ -795,8 +795,16 @@ struct ForceStage {
     float4 rec[kBlock / kLanes][kLanes][kLanes + 1];
 };
 
-template <int LMASK, bool EXTRA>
+template <int LMASK, bool EXTRA, int LDL>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __restrict__ f) {
+    static_assert(LDL == 0 || (LDL == 3 && (LMASK & kL3)) || (LDL == 4 && (LMASK & kL4)) || (LDL == 6 && (LMASK & kL6)),
+                  "the fused local-Q degree must be on the environment");
+    constexpr int LW = LDL ? 2 * LDL + 1 : 1;           // fused local-Q accumulators
+    constexpr int LO = LDL ? rec_off(LMASK, LDL) : 0;
+    __shared__ double shl[4];
+    float qb[LW];
+    #pragma unroll
+    for (int k = 0; k < LW; ++k) qb[k] = 0.0f;
     constexpr int QF = rec_used(LMASK);
     constexpr int NL = rec_nl(LMASK);
     constexpr int LMAX = lmax_of(LMASK);
@@ -865,18 +873,52 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
             const uint64_t mhi = (uint64_t)__float_as_uint(rr[2]) | ((uint64_t)__float_as_uint(rr[3]) << 32);
             const bool is_own = mbit(own, p);
             const bool is_rev = rp != 0xFF && (rp < 64 ? ((mlo >> rp) & 1ull) : ((mhi >> (rp - 64)) & 1ull));
-            if (is_own || is_rev) {
-                float Ko[QF];
-                coeff_from_record<LMASK, EXTRA>(e, o, wl, rr, Ko);
-                float C[QF];
-                #pragma unroll
-                for (int k = kRecMask; k < QF; ++k) C[k] = (is_own ? Ki[k] : 0.0f) + (is_rev ? Ko[k] : 0.0f);
-                // reverse pair enters with (-1)^l
-                if constexpr (LMASK & kL3) {
+            if constexpr (LDL != 0) {
+                if (is_own) {
                     #pragma unroll
-                    for (int k = 0; k < 7; ++k)
-                        C[rec_off(LMASK, 3) + k] = (is_own ? Ki[rec_off(LMASK, 3) + k] : 0.0f) -
-                                                   (is_rev ? Ko[rec_off(LMASK, 3) + k] : 0.0f);
+                    for (int k = 0; k < LW; ++k) qb[k] += rr[LO + k];
+                }
+            }
+            if (is_own || is_rev) {
+                // C = [own] K_i + [rev] (-1)^l K_o with K_o = w_l a_o q_o (+ extra_o): the
+                // reverse flag and the sign fold into one scale per degree
+                float C[QF];
+                auto comb = [&](auto ll) {
+                    constexpr int L = decltype(ll)::value;
+                    if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
+                        constexpr int ob = rec_off(LMASK, L);
+                        constexpr float sg = (L & 1) ? -1.0f : 1.0f;
+                        const float sc = is_rev ? sg * wl[rec_li(LMASK, L)] * rr[rec_a(LMASK, L)] : 0.0f;
+                        #pragma unroll
+                        for (int k = 0; k < 2 * L + 1; ++k)
+                            C[ob + k] = __builtin_fmaf(sc, rr[ob + k], is_own ? Ki[ob + k] : 0.0f);
+                    }
+                };
+                comb(std::integral_constant<int, 3>{});
+                comb(std::integral_constant<int, 4>{});
+                comb(std::integral_constant<int, 6>{});
+                if constexpr (EXTRA) {
+                    if (is_rev) {
+                        const float4* xp = reinterpret_cast<const float4*>(e.extra + (int64_t)o * RS);
+                        float x[NV * 4];
+                        #pragma unroll
+                        for (int k = 1; k < NV; ++k) {
+                            const float4 v = xp[k];
+                            x[4 * k] = v.x; x[4 * k + 1] = v.y; x[4 * k + 2] = v.z; x[4 * k + 3] = v.w;
+                        }
+                        auto addx = [&](auto ll) {
+                            constexpr int L = decltype(ll)::value;
+                            if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
+                                constexpr int ob = rec_off(LMASK, L);
+                                constexpr float sg = (L & 1) ? -1.0f : 1.0f;
+                                #pragma unroll
+                                for (int k = 0; k < 2 * L + 1; ++k) C[ob + k] = __builtin_fmaf(sg, x[ob + k], C[ob + k]);
+                            }
+                        };
+                        addx(std::integral_constant<int, 3>{});
+                        addx(std::integral_constant<int, 4>{});
+                        addx(std::integral_constant<int, 6>{});
+                    }
                 }
                 float d0, d1, d2;
                 pdiff(unpack_pos(pj), pi, d0, d1, d2);
@@ -901,6 +943,29 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
         f[3 * (int64_t)i] += (double)Fx;
         f[3 * (int64_t)i + 1] += (double)Fy;
         f[3 * (int64_t)i + 2] += (double)Fz;
+    }
+    if constexpr (LDL != 0) {
+        // Lechner-Dellago lq(i) = |qbar_i|, qbar_i = (q_i + sum_{j in S(i)} q_j) / (n_i + 1),
+        // from the neighbour records gathered above
+        const bool center = valid && (g.group == nullptr || g.group[i]);
+        const float* qi = e.rec + (int64_t)i * RS + LO;
+        const int n = e.nsel[i];
+        const float inv = 1.0f / (float)(n + 1);
+        float s2 = 0.0f;
+        #pragma unroll
+        for (int k = 0; k < LW; ++k) {
+            const float v = (grp_sum(qb[k]) + qi[k]) * inv;
+            s2 += (k == 0 ? 1.0f : 2.0f) * v * v;
+        }
+        constexpr float P = (float)(4.0 * kPi / (2 * LDL + 1));
+        const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
+        double part[1] = {0.0};
+        if (valid && lane == 0) {
+            e.fld_lq[i] = lq;
+            part[0] = (double)lq;
+        }
+        block_sum<1>(part, shl);
+        if (threadIdx.x == 0) e.partial[(int64_t)e.fld_slot * g.ntiles + t] = part[0];
     }
 }
 
@@ -1211,7 +1276,9 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
                 for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
                 cv[c] = a * r.inv_ng[w];
             } else {
-                cv[c] = r.direct[(int64_t)w * kMaxSlotCV + c];
+                // direct values; CVs summed after the force pass (cv_src <= -2) are
+                // unbiased and written by k_cv_tail
+                cv[c] = r.cv_src[c] == -1 ? r.direct[(int64_t)w * kMaxSlotCV + c] : 0.0;
             }
         }
         // 2. dimensions
@@ -1273,6 +1340,23 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
             wc[r.sym_slot[q]] += acc;
         }
         r.step[w] = step + 1;
+    }
+}
+
+// Values of the CVs whose partial sums come from the force pass (unbiased local-Q
+// fused into k_force): one workgroup per walker, the same fixed-order tile sum.
+__global__ __launch_bounds__(kBlock) void k_cv_tail(ReduceDev r) {
+    __shared__ double sh[kBlock / 64];
+    const int w = blockIdx.x;
+    const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
+    for (int c = 0; c < r.ncv; ++c) {
+        if (r.cv_src[c] > -2) continue;
+        const double* pp = r.partial + (int64_t)(-2 - r.cv_src[c]) * r.ntiles;
+        double v[1] = {0.0};
+        for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v[0] += pp[t];
+        block_sum<1>(v, sh);
+        if (threadIdx.x == 0) r.state[(int64_t)w * kStStride + kStCv + c] = v[0] * r.inv_ng[w];
+        __syncthreads();
     }
 }
 
@@ -1559,23 +1643,34 @@ hipError_t launch_ld_coeff(const Geo& g, const EnvDev& e, const LdDev& d, hipStr
     return hipErrorInvalidValue;
 }
 
+template <int LM>
+static void force_dispatch(const Geo& g, const EnvDev& e, double* f, dim3 grid, hipStream_t st) {
+    if (e.extra != nullptr) { hipLaunchKernelGGL((k_force<LM, true, 0>), grid, dim3(kBlock), 0, st, g, e, f); return; }
+    if constexpr ((LM & kL3) != 0) if (e.fld_l == 3) { hipLaunchKernelGGL((k_force<LM, false, 3>), grid, dim3(kBlock), 0, st, g, e, f); return; }
+    if constexpr ((LM & kL4) != 0) if (e.fld_l == 4) { hipLaunchKernelGGL((k_force<LM, false, 4>), grid, dim3(kBlock), 0, st, g, e, f); return; }
+    if constexpr ((LM & kL6) != 0) if (e.fld_l == 6) { hipLaunchKernelGGL((k_force<LM, false, 6>), grid, dim3(kBlock), 0, st, g, e, f); return; }
+    hipLaunchKernelGGL((k_force<LM, false, 0>), grid, dim3(kBlock), 0, st, g, e, f);
+}
+
 hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st) {
     if (g.ntiles == 0) return hipSuccess;
+    if (e.fld_l != 0 && e.extra != nullptr) return hipErrorInvalidValue;
     dim3 grid(xcd_grid(g.ntiles));
-    const bool ex = e.extra != nullptr;
-#define MDT_F(LM) do { if (ex) hipLaunchKernelGGL((k_force<LM, true>), grid, dim3(kBlock), 0, st, g, e, f); \
-                       else hipLaunchKernelGGL((k_force<LM, false>), grid, dim3(kBlock), 0, st, g, e, f); } while (0)
     switch (e.lmask) {
-        case kL6: MDT_F(kL6); break;
-        case kL4: MDT_F(kL4); break;
-        case kL4 | kL6: MDT_F(kL4 | kL6); break;
-        case kL3: MDT_F(kL3); break;
-        case kL3 | kL4: MDT_F(kL3 | kL4); break;
-        case kL3 | kL6: MDT_F(kL3 | kL6); break;
-        case kL3 | kL4 | kL6: MDT_F(kL3 | kL4 | kL6); break;
+        case kL6: force_dispatch<kL6>(g, e, f, grid, st); break;
+        case kL4: force_dispatch<kL4>(g, e, f, grid, st); break;
+        case kL4 | kL6: force_dispatch<kL4 | kL6>(g, e, f, grid, st); break;
+        case kL3: force_dispatch<kL3>(g, e, f, grid, st); break;
+        case kL3 | kL4: force_dispatch<kL3 | kL4>(g, e, f, grid, st); break;
+        case kL3 | kL6: force_dispatch<kL3 | kL6>(g, e, f, grid, st); break;
+        case kL3 | kL4 | kL6: force_dispatch<kL3 | kL4 | kL6>(g, e, f, grid, st); break;
         default: return hipErrorInvalidValue;
     }
-#undef MDT_F
+    return hipGetLastError();
+}
+
+hipError_t launch_cv_tail(const ReduceDev& r, hipStream_t st) {
+    hipLaunchKernelGGL(k_cv_tail, dim3(r.n_walkers), dim3(kBlock), 0, st, r);
     return hipGetLastError();
 }
 

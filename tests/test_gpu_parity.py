@@ -147,6 +147,40 @@ def test_ld_local_q(mdt, orc):
     assert rel_force_err(f, F_ref) <= F_TOL
 
 
+def test_ld_unbiased_alongside_forces(mdt, orc):
+    """An unbiased local Q6 on the biased Q6/Q4 environment (the headline workload's
+    arrangement): its value and per-atom lq come out of the force pass, every step,
+    through post_force and through the captured graph."""
+    s = fcc_system(mdt, 8, sigma=0.3, sigma_above=0.1, seed=9)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    qb, lq = env.ld(6)
+    q6, q4 = env.q_ave(6), env.q_ave(4)
+    args = ("GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE "
+            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME LQ6 STEINH LD 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE SYMBOL v3 LQ6.AVE DIM 1 0 2 2000 v1+v2 WT 0")
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, args)
+    fix.init()
+    fix.add_hills(np.array([[q6 + q4 + 0.004]]), np.array([0.005]), 0.01)
+    for step in (1, 2):
+        f = np.zeros((n, 3))
+        fix.post_force(step, f)
+        assert abs(fix.cv("LQ6.AVE") - lq.mean()) <= CV_TOL * lq.mean(), step
+        assert np.abs(fix.extract("LQ6.local_q", n) - lq).max() <= CV_TOL * lq.max()
+        assert abs(fix.cv("Q6.AVE") - q6) <= CV_TOL * q6
+        F_ref = -(env.dQ_dx(6, fix.cv_weight("Q6.AVE")) + env.dQ_dx(4, fix.cv_weight("Q4.AVE")))
+        assert rel_force_err(f, F_ref) <= F_TOL
+        assert fix.cv_weight("LQ6.AVE") == 0.0
+    fix.zero_forces()
+    fix.run(3, 2)
+    fix.sync()
+    assert abs(fix.cv("LQ6.AVE") - lq.mean()) <= CV_TOL * lq.mean()
+
+
 def test_lockstep_grid_c1(mdt, orc):
     """Lock-step replay: moving atoms, hills every 10 steps (PACE 10), grid, V, dV/ds, forces each step.
 
