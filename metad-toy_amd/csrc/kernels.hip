@@ -1344,17 +1344,24 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
 }
 
 // Values of the CVs whose partial sums come from the force pass (unbiased local-Q
-// fused into k_force): one workgroup per walker, the same fixed-order tile sum.
-__global__ __launch_bounds__(kBlock) void k_cv_tail(ReduceDev r) {
-    __shared__ double sh[kBlock / 64];
+// fused into k_force): one 1024-thread workgroup per walker, four independent loads
+// in flight per thread, the same fixed-order tree as the reduce kernel.
+__global__ __launch_bounds__(1024) void k_cv_tail(ReduceDev r) {
+    constexpr int NT = 1024, U = 4;
+    __shared__ double sh[NT / 64];
     const int w = blockIdx.x;
     const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
     for (int c = 0; c < r.ncv; ++c) {
         if (r.cv_src[c] > -2) continue;
         const double* pp = r.partial + (int64_t)(-2 - r.cv_src[c]) * r.ntiles;
         double v[1] = {0.0};
-        for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v[0] += pp[t];
-        block_sum<1>(v, sh);
+        for (int t = t0 + threadIdx.x; t < t1; t += U * NT) {
+            double x[U];
+            #pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = t + u * NT < t1 ? pp[t + u * NT] : 0.0;
+            v[0] += (x[0] + x[1]) + (x[2] + x[3]);
+        }
+        block_sum<1, NT / 64>(v, sh);
         if (threadIdx.x == 0) r.state[(int64_t)w * kStStride + kStCv + c] = v[0] * r.inv_ng[w];
         __syncthreads();
     }
@@ -1670,7 +1677,7 @@ hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st
 }
 
 hipError_t launch_cv_tail(const ReduceDev& r, hipStream_t st) {
-    hipLaunchKernelGGL(k_cv_tail, dim3(r.n_walkers), dim3(kBlock), 0, st, r);
+    hipLaunchKernelGGL(k_cv_tail, dim3(r.n_walkers), dim3(1024), 0, st, r);
     return hipGetLastError();
 }
 
