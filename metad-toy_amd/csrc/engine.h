@@ -148,7 +148,6 @@ struct MlpDev {
     MlpNet* stage;            // per walker, being fitted
     float* adam_m;
     float* adam_v;
-    int64_t* adam_t;
     int32_t* ok;              // per walker: 1 = live network trusted
     double* err;              // per walker: last held-out max |dV|
     int64_t* fits;
@@ -236,7 +235,7 @@ hipError_t mlp_data(const MlpDev& m, int w, int64_t n, uint64_t seed, float4* s,
 hipError_t mlp_box(const MlpDev& m, int w, hipStream_t st);
 hipError_t mlp_stats(const MlpDev& m, int w, int64_t n, const double* v, hipStream_t st);
 hipError_t mlp_train_step(const MlpDev& m, int w, int64_t first, int64_t count, int64_t n_total, const float4* s,
-                          const double* v, float* gpart, float lr, float* gsum, hipStream_t st);
+                          const double* v, float* gpart, float lr, float* gsum, int64_t step, hipStream_t st);
 hipError_t mlp_forward(const MlpNet* net, int64_t n, const float4* s, const double* vref, double* vout,
                        double* blk_err, hipStream_t st);
 hipError_t mlp_commit(const MlpDev& m, int w, int nblk, const double* blk_err, double tol, hipStream_t st);

@@ -402,6 +402,7 @@ struct mdt_fix {
     int64_t mlp_refit = 500, mlp_points = 1000000, mlp_holdout = 65536, mlp_batch = 4096;
     int mlp_epochs = 8, mlp_epochs_first = 40;
     double mlp_lr = 1e-2, mlp_tol = 0.1, mlp_spread = 2.0;
+    double mlp_lr_refit = -1.0;   // learning rate of warm-started refits (< 0: MLP_LR / 10)
     uint64_t mlp_seed = 20241203;
     int64_t mlp_round = 0;
     std::vector<double> lo, hi;
@@ -414,6 +415,7 @@ struct mdt_fix {
     // device state
     bool inited = false;
     uint64_t built_gen = 0;
+    uint64_t builds = 0;           // fix_build calls (device buffers may move)
     int nslot_sum = 0;
     DBuf<double> partial, direct, grid, hills, state, wcv;
     DBuf<int64_t> nhills, step;
@@ -421,7 +423,7 @@ struct mdt_fix {
     DBuf<ExprProg> progs;
     DBuf<MlpNet> mlp_live, mlp_stage;
     DBuf<float> mlp_m, mlp_v, mlp_gpart;
-    DBuf<int64_t> mlp_t, mlp_fits, mlp_fallbacks;
+    DBuf<int64_t> mlp_fits, mlp_fallbacks;
     DBuf<double> mlp_err, mlp_box_lo, mlp_box_hi, mlp_vdata, mlp_blkerr;
     DBuf<float4> mlp_sdata;
     std::vector<int> mlp_has_fit;
@@ -434,6 +436,9 @@ struct mdt_fix {
     // graph
     hipGraphExec_t gexec = nullptr;
     uint64_t graph_gen = 0;
+    // captured training loops of the learned bias, per (walker, first fit)
+    std::map<std::pair<int, int>, hipGraphExec_t> mlp_graphs;
+    uint64_t mlp_graph_gen = 0;
     // host staging
     double* pin = nullptr;        // pinned scratch for small transfers
     size_t pin_n = 0;
@@ -442,6 +447,7 @@ struct mdt_fix {
 
     ~mdt_fix() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& kv : mlp_graphs) (void)hipGraphExecDestroy(kv.second);
         for (FILE* fp : hfiles) if (fp) fclose(fp);
         for (auto* e : envs) delete e;
         for (auto* c : cvs) delete c;
@@ -514,6 +520,10 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
             else if (kw == "MLP_EPOCHS") fx->mlp_epochs = (int)parse_int(v, kw.c_str());
             else if (kw == "MLP_EPOCHS_FIRST") fx->mlp_epochs_first = (int)parse_int(v, kw.c_str());
             else if (kw == "MLP_LR") fx->mlp_lr = parse_num(v, kw.c_str());
+            else if (kw == "MLP_LR_REFIT") {
+                fx->mlp_lr_refit = parse_num(v, kw.c_str());
+                if (!(fx->mlp_lr_refit >= 0)) throw Err(MDT_EINVAL, fmt("%s: value out of range", kw.c_str()));
+            }
             else if (kw == "MLP_TOL") fx->mlp_tol = parse_num(v, kw.c_str());
             else if (kw == "MLP_SPREAD") fx->mlp_spread = parse_num(v, kw.c_str());
             else if (kw == "MLP_SEED") fx->mlp_seed = (uint64_t)parse_int(v, kw.c_str());
@@ -937,9 +947,9 @@ void fix_build(mdt_fix* fx) {
         if (fx->hill_type == 3) {
             fx->mlp_live.alloc(nw); fx->mlp_stage.alloc(nw);
             fx->mlp_m.alloc((size_t)nw * kMlpP); fx->mlp_v.alloc((size_t)nw * kMlpP);
-            fx->mlp_t.alloc(nw); fx->mlp_fits.alloc(nw); fx->mlp_fallbacks.alloc(nw);
+            fx->mlp_fits.alloc(nw); fx->mlp_fallbacks.alloc(nw);
             fx->mlp_err.alloc(nw); fx->mlp_box_lo.alloc((size_t)3 * nw); fx->mlp_box_hi.alloc((size_t)3 * nw);
-            fx->mlp_t.upload(z.data(), nw, st); fx->mlp_fits.upload(z.data(), nw, st);
+            fx->mlp_fits.upload(z.data(), nw, st);
             fx->mlp_fallbacks.upload(z.data(), nw, st);
             HIPCHK(launch_fill(fx->mlp_err.p, nw, 0.0, st));
             const int64_t ntot = fx->mlp_points + fx->mlp_holdout;
@@ -994,7 +1004,7 @@ void fix_build(mdt_fix* fx) {
         for (int d = 0; d < 3; ++d) m.sigma[d] = fx->sigmas[d];
         m.spread = fx->mlp_spread;
         m.live = fx->mlp_live.p; m.stage = fx->mlp_stage.p;
-        m.adam_m = fx->mlp_m.p; m.adam_v = fx->mlp_v.p; m.adam_t = fx->mlp_t.p;
+        m.adam_m = fx->mlp_m.p; m.adam_v = fx->mlp_v.p;
         m.ok = fx->mlp_ok.p; m.err = fx->mlp_err.p; m.fits = fx->mlp_fits.p; m.fallbacks = fx->mlp_fallbacks.p;
         m.box_lo_out = fx->mlp_box_lo.p; m.box_hi_out = fx->mlp_box_hi.p;
     }
@@ -1004,6 +1014,7 @@ void fix_build(mdt_fix* fx) {
         HIPCHK(hipHostMalloc((void**)&fx->pin, fx->pin_n * sizeof(double), hipHostMallocDefault));
     }
     fx->built_gen = ctx->generation;
+    fx->builds++;
     if (!fx->inited) {
         fx->inited = true;
         open_hills(fx);
@@ -1150,8 +1161,9 @@ void mlp_refit(mdt_fix* fx, int w) {
     const MlpDev& m = fx->mdev;
     MlpNet* stage = fx->mlp_stage.p + w;
     const bool first = !fx->mlp_has_fit[w];
+    // a refit continues from the last fitted network, kept in the staging slot
+    // whether or not the held-out gate adopted it
     if (first) HIPCHK(mlp_init(stage, fx->mlp_seed + 977 * w, st));
-    else HIPCHK(mlp_copy_net(fx->mlp_live.p + w, stage, st));
     HIPCHK(launch_fill_f(fx->mlp_m.p + (size_t)w * kMlpP, kMlpP, 0.0f, st));
     HIPCHK(launch_fill_f(fx->mlp_v.p + (size_t)w * kMlpP, kMlpP, 0.0f, st));
     HIPCHK(mlp_box(m, w, st));
@@ -1162,15 +1174,42 @@ void mlp_refit(mdt_fix* fx, int w) {
     HIPCHK(mlp_stats(m, w, M, fx->mlp_vdata.p, st));
     const int epochs = first ? fx->mlp_epochs_first : fx->mlp_epochs;
     const int64_t B = fx->mlp_batch;
-    int64_t it = 0;
-    const int64_t nit = epochs * ((M + B - 1) / B);
-    for (int e = 0; e < epochs; ++e)
-        for (int64_t b0 = 0; b0 < M; b0 += B, ++it) {
-            // cosine decay of the learning rate over the fit
-            const double lr = fx->mlp_lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.141592653589793 * (double)it / (double)std::max<int64_t>(nit, 1))));
-            HIPCHK(mlp_train_step(m, w, b0, std::min(B, M - b0), M, fx->mlp_sdata.p, fx->mlp_vdata.p,
-                                  fx->mlp_gpart.p, (float)lr, nullptr, st));
+    // The training loop is the same sequence of launches for every fit of a kind
+    // (first or later): it is captured once as a graph and replayed.
+    if (fx->mlp_graph_gen != fx->builds) {
+        for (auto& kv : fx->mlp_graphs) (void)hipGraphExecDestroy(kv.second);
+        fx->mlp_graphs.clear();
+        fx->mlp_graph_gen = fx->builds;
+    }
+    const std::pair<int, int> key(w, first ? 1 : 0);
+    auto git = fx->mlp_graphs.find(key);
+    if (git == fx->mlp_graphs.end()) {
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipSuccess;
+        int64_t it = 0;
+        const int64_t nit = epochs * ((M + B - 1) / B);
+        for (int e = 0; e < epochs && ce == hipSuccess; ++e)
+            for (int64_t b0 = 0; b0 < M && ce == hipSuccess; b0 += B, ++it) {
+                // cosine decay of the learning rate over the fit
+                // a refit starts from the live network: fine-tune at a lower rate
+                const double lr0 = first ? fx->mlp_lr : (fx->mlp_lr_refit >= 0.0 ? fx->mlp_lr_refit : 0.1 * fx->mlp_lr);
+                const double lr = lr0 * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.141592653589793 * (double)it / (double)std::max<int64_t>(nit, 1))));
+                ce = mlp_train_step(m, w, b0, std::min(B, M - b0), M, fx->mlp_sdata.p, fx->mlp_vdata.p,
+                                    fx->mlp_gpart.p, (float)lr, nullptr, it + 1, st);
+            }
+        hipError_t ee = hipStreamEndCapture(st, &gr);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            throw Err(MDT_EHIP, fmt("capturing the training loop: %s", hipGetErrorString(ce != hipSuccess ? ce : ee)));
         }
+        hipGraphExec_t ex = nullptr;
+        hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ie != hipSuccess) throw Err(MDT_EHIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(ie)));
+        git = fx->mlp_graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(git->second, st));
     const int nblk = (int)((Hn + 63) / 64);
     HIPCHK(mlp_forward(stage, Hn, fx->mlp_sdata.p + M, fx->mlp_vdata.p + M, nullptr, fx->mlp_blkerr.p, st));
     HIPCHK(mlp_commit(m, w, nblk, fx->mlp_blkerr.p, fx->mlp_tol * fx->temp * fx->boltz, st));
@@ -1835,13 +1874,9 @@ int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int
     DBuf<float> gpart, gsum, am, av;
     gpart.alloc((size_t)std::max(nb, 1) * kMlpP);
     gsum.alloc(kMlpP); am.alloc(kMlpP); av.alloc(kMlpP);
-    DBuf<int64_t> at;
-    at.alloc(1);
-    int64_t zero = 0;
-    at.upload(&zero, 1, ctx->stream);
     MlpDev m{};
-    m.D = 3; m.stage = net.p; m.live = net.p; m.adam_m = am.p; m.adam_v = av.p; m.adam_t = at.p;
-    HIPCHK(mlp_train_step(m, 0, 0, np, np, ds.p, dv.p, gpart.p, 0.0f, gsum.p, ctx->stream));
+    m.D = 3; m.stage = net.p; m.live = net.p; m.adam_m = am.p; m.adam_v = av.p;
+    HIPCHK(mlp_train_step(m, 0, 0, np, np, ds.p, dv.p, gpart.p, 0.0f, gsum.p, 1, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(grad, gsum.p, sizeof(float) * kMlpP, hipMemcpyDeviceToHost));
     API_END

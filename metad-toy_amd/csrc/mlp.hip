@@ -14,6 +14,7 @@
 //                  3-wide and 1-wide layers on FP32 VALU; deterministic block
 //                  gradient partials
 //   k_mlp_adam     gradient reduction over blocks + Adam update, BF16 copy of W2
+//                  (the training loop of a fit is replayed as one captured graph)
 //   k_mlp_forward  batched MFMA inference (held-out check, bias evaluation)
 //   k_mlp_commit   held-out gate: adopt the fitted network or fall back to the
 //                  exact hills sum
@@ -395,17 +396,36 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
     for (int k = threadIdx.x; k < kMlpP; k += blockDim.x) out[k] = grad[k];
 }
 
-// sum of block partials (fixed order) + Adam
+// sum of block partials (fixed order) + Adam.  A workgroup covers 64 parameters:
+// thread t sums the partials b = q, q+4, ... (q = t / 64) of parameter t % 64 with
+// independent loads, and the four group sums are added in order.
+// step: 1-based Adam step within the current fit (the moments restart with each fit)
 __global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, int w, int nblocks, const float* __restrict__ gpart,
-                                                   float lr, float* __restrict__ gsum_out) {
+                                                   float lr, float* __restrict__ gsum_out, int64_t step) {
+    __shared__ float part[4][64];
     MlpNet* net = m.stage + w;
     float* mo = m.adam_m + (int64_t)w * kMlpP;
     float* ve = m.adam_v + (int64_t)w * kMlpP;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t step = m.adam_t[w];
+    const int pl = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + pl;
+    float acc = 0.0f;
     if (k < kMlpP) {
-        float gs = 0.0f;
-        for (int b = 0; b < nblocks; ++b) gs += gpart[(int64_t)b * kMlpP + k];
+        constexpr int U = 8;
+        for (int b0 = q; b0 < nblocks; b0 += 4 * U) {
+            float x[U];
+            #pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b = b0 + 4 * u;
+                x[u] = b < nblocks ? gpart[(int64_t)b * kMlpP + k] : 0.0f;
+            }
+            #pragma unroll
+            for (int u = 0; u < U; ++u) acc += x[u];
+        }
+    }
+    part[q][pl] = acc;
+    __syncthreads();
+    if (q == 0 && k < kMlpP) {
+        const float gs = ((part[0][pl] + part[1][pl]) + part[2][pl]) + part[3][pl];
         if (gsum_out) gsum_out[k] = gs;
         if (lr > 0.0f) {
             const float b1 = 0.9f, b2 = 0.999f;
@@ -425,8 +445,6 @@ __global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, int w, int nblocks, 
         }
     }
 }
-
-__global__ void k_mlp_adam_step(MlpDev m, int w) { if (threadIdx.x == 0 && blockIdx.x == 0) m.adam_t[w] += 1; }
 
 // batched MFMA inference of a network on n raw points; optional error vs exact labels
 __global__ __launch_bounds__(256) void k_mlp_forward(const MlpNet* __restrict__ net, int64_t n,
@@ -548,13 +566,12 @@ hipError_t mlp_stats(const MlpDev& m, int w, int64_t n, const double* v, hipStre
     return hipGetLastError();
 }
 hipError_t mlp_train_step(const MlpDev& m, int w, int64_t first, int64_t count, int64_t n_total, const float4* s,
-                          const double* v, float* gpart, float lr, float* gsum, hipStream_t st) {
+                          const double* v, float* gpart, float lr, float* gsum, int64_t step, hipStream_t st) {
     const int nb = (int)((count + 63) / 64);
     hipLaunchKernelGGL(k_mlp_train, dim3(nb), dim3(256), 0, st, m, w, first, count, n_total, s, v, gpart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (lr > 0.0f) hipLaunchKernelGGL(k_mlp_adam_step, dim3(1), dim3(64), 0, st, m, w);
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpP + 255) / 256), dim3(256), 0, st, m, w, nb, gpart, lr, gsum);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpP + 63) / 64), dim3(256), 0, st, m, w, nb, gpart, lr, gsum, step);
     return hipGetLastError();
 }
 hipError_t mlp_forward(const MlpNet* net, int64_t n, const float4* s, const double* vref, double* vout,
