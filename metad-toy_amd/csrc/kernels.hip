@@ -1440,10 +1440,12 @@ __global__ void k_fill_f(float* p, int64_t n, float v) {
 constexpr int kSkTile = 8;    // k vectors per workgroup in the amplitude pass
 constexpr int kSkChunk = 16384;   // atoms per chunk (host side uses the same value)
 
+// sin and cos of 2 pi ph (ph in revolutions): FP64 reduction to [-0.5, 0.5], then the
+// hardware sine/cosine, which take their argument in revolutions
 __device__ __forceinline__ void sincos_rev(double ph, float& s, float& c) {
-    const double fr = ph - rint(ph);                 // in [-0.5, 0.5]
-    const float a = (float)fr * 6.28318530717958647692f;
-    __sincosf(a, &s, &c);
+    const float fr = (float)(ph - rint(ph));
+    s = __builtin_amdgcn_sinf(fr);
+    c = __builtin_amdgcn_cosf(fr);
 }
 
 // A(k) partial sums: workgroup (k tile, atom chunk); k vectors are uniform per
@@ -1459,21 +1461,34 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
         const int qq = kb + q < sk.nk ? kb + q : sk.nk - 1;
         kx[q] = sk.k[3 * qq]; ky[q] = sk.k[3 * qq + 1]; kz[q] = sk.k[3 * qq + 2];
     }
+    // FP32 sums over runs of kSkRun atoms of a thread, FP64 across runs and threads:
+    // a run's rounding error (<= kSkRun^2 * 2^-24 in units of |e^{ik.r}| = 1) stays
+    // below 1e-6 of |A| even at a Bragg peak
+    constexpr int kSkRun = 8;
     double acc[kSkTile * 2];
     #pragma unroll
     for (int q = 0; q < kSkTile * 2; ++q) acc[q] = 0.0;
-    for (int a = threadIdx.x; a < count; a += blockDim.x) {
-        const int i = start + a;
-        if (g.group && !g.group[i]) continue;
-        const double4 xv = g.x4[i];
+    for (int a0 = threadIdx.x; a0 < count; a0 += kSkRun * blockDim.x) {
+        float run[kSkTile * 2];
         #pragma unroll
-        for (int q = 0; q < kSkTile; ++q) {
-            const double ph = xv.x * kx[q] + xv.y * ky[q] + xv.z * kz[q];
-            float sn, cs;
-            sincos_rev(ph, sn, cs);
-            acc[2 * q] += (double)cs;
-            acc[2 * q + 1] += (double)sn;
+        for (int q = 0; q < kSkTile * 2; ++q) run[q] = 0.0f;
+        for (int r = 0; r < kSkRun; ++r) {
+            const int a = a0 + r * blockDim.x;
+            if (a >= count) break;
+            const int i = start + a;
+            if (g.group && !g.group[i]) continue;
+            const double4 xv = g.x4[i];
+            #pragma unroll
+            for (int q = 0; q < kSkTile; ++q) {
+                const double ph = xv.x * kx[q] + xv.y * ky[q] + xv.z * kz[q];
+                float sn, cs;
+                sincos_rev(ph, sn, cs);
+                run[2 * q] += cs;
+                run[2 * q + 1] += sn;
+            }
         }
+        #pragma unroll
+        for (int q = 0; q < kSkTile * 2; ++q) acc[q] += (double)run[q];
     }
     block_sum<kSkTile * 2>(acc, sh);
     if (threadIdx.x == 0) {
