@@ -128,6 +128,27 @@ def test_fix_semantic_errors(mdt):
         assert frag in str(e.value), (tail, str(e.value))
 
 
+def test_learned_bias_options(mdt):
+    """Gaussian_Hill_type 3 with its MLP_* options parses; out-of-range values and a
+    learned bias on fewer than three dimensions are refused."""
+    ctx = mdt.Context(-1, 1)
+    base = ("GAUSSIAN 0.01 0.005 10 SIGMAS 0.01 0.01 0.5 "
+            "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME S SK all FCC111 5.26 "
+            "CV_dim 3 SYMBOL v1 Q4.AVE SYMBOL v2 Q6.AVE SYMBOL v3 S.AVE "
+            "DIM 1 0 1 100 v1 DIM 2 0 1 100 v2 DIM 3 0 100 100 v3 Gaussian_Hill_type 3 ")
+    mdt.Fix(ctx, base + "MLP_REFIT 250 MLP_POINTS 65536 MLP_HOLDOUT 4096 MLP_BATCH 1024 MLP_EPOCHS 4 "
+                        "MLP_EPOCHS_FIRST 20 MLP_LR 0.005 MLP_LR_REFIT 0.0005 MLP_TOL 0.1 MLP_SPREAD 2 MLP_SEED 7").close()
+    for bad in ("MLP_REFIT 0", "MLP_BATCH 8", "MLP_LR -1", "MLP_LR_REFIT -0.1", "MLP_TOL -1", "MLP_FOO 1"):
+        with pytest.raises(mdt.MdtError) as e:
+            mdt.Fix(ctx, base + bad)
+        assert e.value.code == -22, bad
+    with pytest.raises(mdt.MdtError):
+        mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                     "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 Gaussian_Hill_type 3")
+
+
 def test_parse_only_context_refuses_steps(mdt):
     ctx = mdt.Context(-1, 1)
     fix = mdt.Fix(ctx, GOOD)
