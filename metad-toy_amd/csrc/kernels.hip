@@ -350,6 +350,16 @@ __device__ __forceinline__ M128 grp_bcast0(M128 m) {
     return m;
 }
 
+// Orders one wave's LDS writes and reads when lanes exchange data through LDS:
+// the hardware completes a wave's LDS accesses in issue order, and the fences
+// keep the compiler from moving a lane's accesses across the exchange point
+// (it cannot see that other lanes read what this lane writes).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // ---------------------------------------------------------------------------
 // k_env: selection + q_lm for every degree in LMASK, kLanes lanes per atom.
 // Phase A scans the list with FP64 r^2; when the number within rc does not
@@ -383,6 +393,29 @@ __device__ __forceinline__ void literal_insert(const Geo& g, int beg, int c, dou
     }
 }
 
+// The same insertion with the sorted arrays in LDS (one lane per atom), so that the
+// rare cap-binding path does not set the register budget of the whole kernel.
+template <int NMAX>
+__device__ __forceinline__ void literal_insert_lds(const Geo& g, int beg, int c, double xi0, double xi1,
+                                                   double xi2, double rc2, int N, double* sr2, int* sp) {
+    for (int s = 0; s < N; ++s) { sr2[s] = __builtin_inf(); sp[s] = -1; }
+    for (int p = 0; p < c; ++p) {
+        double d0, d1, d2;
+        double r2 = pair_r2(g.x4, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+        if ((r2 > rc2) || (r2 < 1e-12)) continue;
+        double r = r2;
+        int q = p;
+        for (int s = 0; s < N; ++s) {
+            const double tr = sr2[s];
+            if (tr > r) {
+                const int tq = sp[s];
+                sr2[s] = r; sp[s] = q;
+                r = tr; q = tq;
+            }
+        }
+    }
+}
+
 // candidates with r^2 <= rc^2 (1 + kRcSlack) in this atom's frame: a superset of
 // every pair that either side of a pair can select (frames differ by rounding)
 constexpr double kRcSlack = 1e-9;
@@ -410,31 +443,88 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     const double4 xi = g.x4[i];
     const double rc2 = e.rc2, rc2s = e.rc2 * (1.0 + kRcSlack);
     const int N = e.N;
-    M128 own{0, 0}, rcm{0, 0};
-    int cnt = 0;
-    // Scan in batches of kScan entries per lane (all list loads of a batch, then all
-    // packed-position loads).  The FP32 pair vector from the packed positions is
-    // within |dd| <= 2u|d| + uX/128 of the exact one (u = 2^-24, |coordinates| <= X),
-    // so its r^2 is within 7u r^2 + 0.03 u X rc of the FP64 r^2; every candidate
-    // farther than twice that from the cutoffs is decided in FP32, the rest (and
+    // Phase A: masks.  Each pass of the scan takes 16 consecutive entries of the
+    // list, entry base + 4u + lane in slot u of the lane, with all list loads, then
+    // all position loads, issued before any test.  The FP32 pair vector of the
+    // rounded positions has |dd_k| <= u(2X + |d_k|) (u = 2^-24, |coordinates| <= X),
+    // so its r^2 is within 7u X r + 5u r^2 of the FP64 r^2; every candidate farther
+    // than twice that from the cutoffs is decided in FP32, the rest (and
     // near-coincident pairs) with the exact FP64 r^2, so the masks are the FP64
-    // scan's.  Entries within rc add their Y_lm at once: when the cap does not bind
-    // they are exactly the selected set.
-    constexpr int kScan = 4;
-    const PPos pi = unpack_pos(g.x4f[i]);
+    // scan's.  A wave ballot per slot gives each group the 4 flags of its 4 lanes,
+    // so the masks and counts are group-uniform without any exchange, and every
+    // entry within rc is appended, packed position and all, to the atom's LDS list
+    // in list order: when the cap does not bind that list is the selected set.
+    constexpr int KC = NMAX < 32 ? NMAX : 32;       // LDS list length per atom
+    constexpr int KB = (KC + kLanes - 1) / kLanes;
+    // per atom: the kept list, or the sorted arrays of the literal insertion
+    constexpr int KW = KC > (12 * NMAX + 15) / 16 ? KC : (12 * NMAX + 15) / 16;
+    __shared__ float4 kept[kTile][KW];
+    const int gsh = (threadIdx.x & 63) & ~(kLanes - 1);   // bit offset of the group in a ballot
+    const float4 pif = g.x4f[i];
     float margin;
     {
         constexpr float u = 5.9604645e-8f;
         const float rcf = (float)__builtin_sqrt(rc2s);
-        const float X = fmaxf(fmaxf(fabsf(pi.x), fabsf(pi.y)), fabsf(pi.z)) + rcf + 1.0f;
-        margin = 2.0f * (7.0f * u * (float)rc2s + 0.03f * u * X * rcf) + 1e-6f;
+        const float X = fmaxf(fmaxf(fabsf(pif.x), fabsf(pif.y)), fabsf(pif.z)) + rcf + 1.0f;
+        margin = 2.0f * (5.0f * u * (float)rc2s + 7.0f * u * X * rcf) + 1e-6f;
     }
     const float r2lo = (float)rc2 - margin, r2hi = (float)rc2s + margin;
+    M128 own{0, 0}, rcm{0, 0};
+    int cnt = 0;                                     // group-uniform
+    for (int base = 0; base < c; base += 4 * kLanes) {   // trip count uniform within the group
+        int jj[4];
+        #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = base + u * kLanes + lane;
+            jj[u] = g.idx[beg + (p < c ? p : c - 1)];
+        }
+        float4 xx[4];
+        #pragma unroll
+        for (int u = 0; u < 4; ++u) xx[u] = g.x4f[jj[u]];
+        uint32_t cin = 0, crc = 0;
+        #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = base + u * kLanes + lane;
+            bool in = false, inrc = false;
+            if (p < c) {
+                const float e0 = xx[u].x - pif.x, e1 = xx[u].y - pif.y, e2 = xx[u].z - pif.z;
+                const float r2f = __builtin_fmaf(e2, e2, __builtin_fmaf(e1, e1, e0 * e0));
+                if (r2f < r2lo && r2f > 1e-2f) {
+                    in = inrc = true;
+                } else if (!(r2f > r2hi)) {
+                    double d0, d1, d2;
+                    const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
+                    inrc = r2 >= 1e-12 && r2 <= rc2s;
+                    in = !((r2 > rc2) || (r2 < 1e-12));
+                }
+            }
+            const uint32_t ni = (uint32_t)(__ballot(in) >> gsh) & 0xFu;
+            const uint32_t nr = (uint32_t)(__ballot(inrc) >> gsh) & 0xFu;
+            if (in) {
+                const int rk = cnt + __popc(ni & ((1u << lane) - 1u));
+                if (rk < KC) kept[a][rk] = xx[u];
+            }
+            cnt += __popc(ni);
+            cin |= ni << (4 * u);
+            crc |= nr << (4 * u);
+        }
+        if (base < 64) {
+            own.lo |= (uint64_t)cin << base;
+            rcm.lo |= (uint64_t)crc << base;
+        } else {
+            own.hi |= (uint64_t)cin << (base - 64);
+            rcm.hi |= (uint64_t)crc << (base - 64);
+        }
+    }
+    // Phase B: Y_lm of the kept set, round-robin over the group's lanes.
+    const PPos pi = unpack_pos(pif);
     float q[QF];
     #pragma unroll
     for (int k = 0; k < QF; ++k) q[k] = 0.0f;
-    auto add_ylm = [&](float d0, float d1, float d2, float r2f) {
-        const float ir = __builtin_amdgcn_rsqf(r2f);
+    auto add_ylm = [&](const float4& pj) {
+        float d0, d1, d2;
+        pdiff(unpack_pos(pj), pi, d0, d1, d2);
+        const float ir = __builtin_amdgcn_rsqf(__builtin_fmaf(d2, d2, __builtin_fmaf(d1, d1, d0 * d0)));
         const float nx = d0 * ir, ny = d1 * ir, uu = d2 * ir;
         const float u2 = uu * uu;
         float zr[LMAX + 1], zi[LMAX + 1];
@@ -443,64 +533,30 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         if constexpr (LMASK & kL4) acc_ylm<4>(uu, u2, zr, zi, q + rec_off(LMASK, 4));
         if constexpr (LMASK & kL6) acc_ylm<6>(uu, u2, zr, zi, q + rec_off(LMASK, 6));
     };
-    for (int p0 = lane; p0 < c; p0 += kScan * kLanes) {
-        int jj[kScan];
+    wave_lds_sync();
+    if (cnt <= N && cnt <= KC) {                     // uniform within the group
+        float4 pk[KB];
         #pragma unroll
-        for (int u = 0; u < kScan; ++u) {
-            const int p = p0 + u * kLanes;
-            jj[u] = g.idx[beg + (p < c ? p : p0)];
-        }
-        float4 xx[kScan];
+        for (int s = 0; s < KB; ++s)
+            if (lane + s * kLanes < cnt) pk[s] = kept[a][lane + s * kLanes];
         #pragma unroll
-        for (int u = 0; u < kScan; ++u) xx[u] = g.x4f[jj[u]];
-        #pragma unroll
-        for (int u = 0; u < kScan; ++u) {
-            const int p = p0 + u * kLanes;
-            if (p < c) {
-                float e0, e1, e2;
-                pdiff(unpack_pos(xx[u]), pi, e0, e1, e2);
-                const float r2f = __builtin_fmaf(e2, e2, __builtin_fmaf(e1, e1, e0 * e0));
-                bool in = false;
-                if (r2f < r2lo && r2f > 1e-2f) {
-                    mset(rcm, p); in = true;
-                } else if (!(r2f > r2hi)) {
-                    double d0, d1, d2;
-                    const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
-                    if (r2 >= 1e-12 && r2 <= rc2s) mset(rcm, p);
-                    in = !((r2 > rc2) || (r2 < 1e-12));
-                }
-                if (in) {
-                    ++cnt;
-                    mset(own, p);
-                    add_ylm(e0, e1, e2, r2f);
-                }
+        for (int s = 0; s < KB; ++s)
+            if (lane + s * kLanes < cnt) add_ylm(pk[s]);
+    } else {
+        if (cnt > N) {                               // the cap binds: the literal insertion
+            if (lane == 0) {
+                double* sr2 = reinterpret_cast<double*>(&kept[a][0]);
+                int* sp = reinterpret_cast<int*>(sr2 + NMAX);
+                literal_insert_lds<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
+                own = M128{0, 0};
+                for (int s = 0; s < N; ++s)
+                    if (sp[s] >= 0) mset(own, sp[s]);
             }
+            own = grp_bcast0(own);
         }
-    }
-    cnt = grp_sum_i(cnt);
-    own = grp_or(own);
-    rcm = grp_or(rcm);
-    if (cnt > N) {                                   // uniform within the group
-        if (lane == 0) {
-            double sr2[NMAX];
-            int sp[NMAX];
-            literal_insert<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
-            own = M128{0, 0};
-            #pragma unroll
-            for (int s = 0; s < NMAX; ++s)
-                if (s < N && sp[s] >= 0) mset(own, sp[s]);
-        }
-        own = grp_bcast0(own);
-        // the cap binds: the Y_lm of the kept set only
-        #pragma unroll
-        for (int k = 0; k < QF; ++k) q[k] = 0.0f;
         M128 it = own;
         mdrop(it, lane);
-        for (int p = mnext(it); p >= 0; p = mnext(it)) {
-            float e0, e1, e2;
-            pdiff(unpack_pos(g.x4f[g.idx[beg + p]]), pi, e0, e1, e2);
-            add_ylm(e0, e1, e2, __builtin_fmaf(e2, e2, __builtin_fmaf(e1, e1, e0 * e0)));
-        }
+        for (int p = mnext(it); p >= 0; p = mnext(it)) add_ylm(g.x4f[g.idx[beg + p]]);
     }
     const int n = cnt < N ? cnt : N;
     const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
@@ -770,16 +826,6 @@ __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* 
     #pragma unroll
     for (int k = 1; k < NV; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
     coeff_from_record<LMASK, EXTRA>(e, a, wl, r, K);
-}
-
-// Orders one wave's LDS writes and reads when lanes exchange data through LDS:
-// the hardware completes a wave's LDS accesses in issue order, and the fences
-// keep the compiler from moving a lane's accesses across the exchange point
-// (it cannot see that other lanes read what this lane writes).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Quad broadcast: every lane of an aligned group of four gets lane r's value.
