@@ -141,14 +141,21 @@ __device__ __forceinline__ void acc_grad(float u, float u2, const float* zr, con
     sfor<L>([&](auto mm) {
         constexpr int M = decltype(mm)::value + 1;
         const float kr = K[2 * M - 1], ki = K[2 * M];
-        // conj(K) z^(M-1)
-        float tr = __builtin_fmaf(kr, zr[M - 1], ki * zi[M - 1]);
-        float ti = __builtin_fmaf(kr, zi[M - 1], -ki * zr[M - 1]);
+        // conj(K) z^(M-1)  (z^0 = 1)
+        float tr, ti;
+        if constexpr (M == 1) {
+            tr = kr; ti = -ki;
+        } else {
+            tr = __builtin_fmaf(kr, zr[M - 1], ki * zi[M - 1]);
+            ti = __builtin_fmaf(kr, zi[M - 1], -ki * zr[M - 1]);
+        }
         float a = ypoly<L, M, 1>(u, u2);          // mu m c_lm
         gx = __builtin_fmaf(a, tr, gx);
         gy = __builtin_fmaf(-a, ti, gy);
-        float b = ypoly<L, M, 2>(u, u2);          // mu c'_lm
-        gz = __builtin_fmaf(b, __builtin_fmaf(kr, zr[M], ki * zi[M]), gz);
+        if constexpr (M < L) {                    // c'_lm = 0 for m = l
+            float b = ypoly<L, M, 2>(u, u2);      // mu c'_lm
+            gz = __builtin_fmaf(b, __builtin_fmaf(kr, zr[M], ki * zi[M]), gz);
+        }
     });
 }
 
@@ -267,21 +274,24 @@ __device__ __forceinline__ uint32_t pack_res1(double x, float xf) {
     q = fmin(fmax(q, -511.0), 511.0);
     return (uint32_t)(int)q & 0x3FFu;
 }
-__device__ __forceinline__ float unpack_res1(float xf, uint32_t w, int c) {
+// The residual times 2^34: q ulp(xf)/1024 2^34 = q 2^(e) with e the frexp exponent
+// of xf (q = 0 whenever the packing stored none).
+__device__ __forceinline__ float unpack_res34(float xf, uint32_t w, int c) {
     const int q = ((int)(w << (22 - 10 * c))) >> 22;
-    const uint32_t E = (__float_as_uint(xf) >> 23) & 0xFFu;
-    return (float)q * (E >= 34u ? __uint_as_float((E - 33u) << 23) : 0.0f);
+    return __builtin_amdgcn_ldexpf((float)q, __builtin_amdgcn_frexp_expf(xf));
 }
-struct PPos { float x, y, z, rx, ry, rz; };
+struct PPos { float x, y, z, rx, ry, rz; };   // r* = residual * 2^34
 __device__ __forceinline__ PPos unpack_pos(const float4& p) {
     const uint32_t w = __float_as_uint(p.w);
-    return PPos{p.x, p.y, p.z, unpack_res1(p.x, w, 0), unpack_res1(p.y, w, 1), unpack_res1(p.z, w, 2)};
+    return PPos{p.x, p.y, p.z, unpack_res34(p.x, w, 0), unpack_res34(p.y, w, 1), unpack_res34(p.z, w, 2)};
 }
-// x_j - x_i from packed positions
+// x_j - x_i from packed positions: (xf_j - xf_i) + (r_j - r_i), the residual
+// difference rescaled exactly by the power of two
 __device__ __forceinline__ void pdiff(const PPos& pj, const PPos& pi, float& d0, float& d1, float& d2) {
-    d0 = (pj.x - pi.x) + (pj.rx - pi.rx);
-    d1 = (pj.y - pi.y) + (pj.ry - pi.ry);
-    d2 = (pj.z - pi.z) + (pj.rz - pi.rz);
+    constexpr float s = 0x1p-34f;
+    d0 = __builtin_fmaf(s, pj.rx - pi.rx, pj.x - pi.x);
+    d1 = __builtin_fmaf(s, pj.ry - pi.ry, pj.y - pi.y);
+    d2 = __builtin_fmaf(s, pj.rz - pi.rz, pj.z - pi.z);
 }
 
 __global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4, float4* __restrict__ x4f,
@@ -884,10 +894,21 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     // evaluates its pair.
     const int nrounds = (mpop_count(rcm) + kLanes - 1) / kLanes;   // uniform within the group
     float4 (*srec)[kLanes + 1] = stg.rec[a];
+    // lists of at most 64 entries (the usual case, uniform over the wave): the
+    // lane's share of the low word is walked with plain 64-bit clears
+    const bool wide = __ballot(rcm.hi != 0ull) != 0ull;
     M128 it = rcm;
     mdrop(it, lane);
+    uint64_t it64 = it.lo;
     for (int rd = 0; rd < nrounds; ++rd) {
-        const int p = mnext(it);
+        int p;
+        if (!wide) {
+            p = (int)__ffsll((unsigned long long)it64) - 1;
+            #pragma unroll
+            for (int k = 0; k < kLanes; ++k) it64 &= it64 - 1ull;
+        } else {
+            p = mnext(it);
+        }
         const int rp = p >= 0 ? g.revpos[beg + p] : 0xFF;
         const int o = p >= 0 ? g.eown[beg + p] : i;
         const int j = p >= 0 ? g.idx[beg + p] : i;
@@ -914,11 +935,17 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
                 }
             });
         });
+        bool is_own = false, is_rev = false;
         if (p >= 0) {
             const uint64_t mlo = (uint64_t)__float_as_uint(rr[0]) | ((uint64_t)__float_as_uint(rr[1]) << 32);
             const uint64_t mhi = (uint64_t)__float_as_uint(rr[2]) | ((uint64_t)__float_as_uint(rr[3]) << 32);
-            const bool is_own = mbit(own, p);
-            const bool is_rev = rp != 0xFF && (rp < 64 ? ((mlo >> rp) & 1ull) : ((mhi >> (rp - 64)) & 1ull));
+            is_own = mbit(own, p);
+            is_rev = rp != 0xFF && (rp < 64 ? ((mlo >> rp) & 1ull) : ((mhi >> (rp - 64)) & 1ull));
+        }
+        // every active pair of the wave selected from both sides (the usual case when
+        // the cap does not bind): the combination needs no per-lane selects
+        const bool mutual = __ballot(is_own != is_rev) == 0ull;
+        {
             if constexpr (LDL != 0) {
                 if (is_own) {
                     #pragma unroll
@@ -934,10 +961,16 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
                     if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
                         constexpr int ob = rec_off(LMASK, L);
                         constexpr float sg = (L & 1) ? -1.0f : 1.0f;
-                        const float sc = is_rev ? sg * wl[rec_li(LMASK, L)] * rr[rec_a(LMASK, L)] : 0.0f;
-                        #pragma unroll
-                        for (int k = 0; k < 2 * L + 1; ++k)
-                            C[ob + k] = __builtin_fmaf(sc, rr[ob + k], is_own ? Ki[ob + k] : 0.0f);
+                        const float s0 = sg * wl[rec_li(LMASK, L)] * rr[rec_a(LMASK, L)];
+                        if (mutual) {
+                            #pragma unroll
+                            for (int k = 0; k < 2 * L + 1; ++k) C[ob + k] = __builtin_fmaf(s0, rr[ob + k], Ki[ob + k]);
+                        } else {
+                            const float sc = is_rev ? s0 : 0.0f;
+                            #pragma unroll
+                            for (int k = 0; k < 2 * L + 1; ++k)
+                                C[ob + k] = __builtin_fmaf(sc, rr[ob + k], is_own ? Ki[ob + k] : 0.0f);
+                        }
                     }
                 };
                 comb(std::integral_constant<int, 3>{});
