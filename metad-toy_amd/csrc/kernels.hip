@@ -335,22 +335,23 @@ __device__ __forceinline__ int mnext(M128& it) {
     mdrop(it, kLanes - 1);
     return p;
 }
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int o) {
-    return (uint64_t)__shfl_xor((long long)v, o, 64);
+// Quad butterflies through DPP (quad_perm [1,0,3,2] then [2,3,0,1]); callers keep
+// every lane of the wave active.
+static_assert(kLanes == 4, "the group helpers are quad permutations");
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
 }
-__device__ __forceinline__ M128 grp_or(M128 m) {
-    #pragma unroll
-    for (int o = 1; o < kLanes; o <<= 1) { m.lo |= shfl_xor_u64(m.lo, o); m.hi |= shfl_xor_u64(m.hi, o); }
-    return m;
+__device__ __forceinline__ uint64_t grp_or_u64(uint64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo |= dpp_u32<0xB1>(lo); hi |= dpp_u32<0xB1>(hi);
+    lo |= dpp_u32<0x4E>(lo); hi |= dpp_u32<0x4E>(hi);
+    return ((uint64_t)hi << 32) | lo;
 }
+__device__ __forceinline__ M128 grp_or(M128 m) { return M128{grp_or_u64(m.lo), grp_or_u64(m.hi)}; }
 __device__ __forceinline__ float grp_sum(float v) {
-    #pragma unroll
-    for (int o = 1; o < kLanes; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ int grp_sum_i(int v) {
-    #pragma unroll
-    for (int o = 1; o < kLanes; o <<= 1) v += __shfl_xor(v, o, 64);
+    v += __uint_as_float(dpp_u32<0xB1>(__float_as_uint(v)));
+    v += __uint_as_float(dpp_u32<0x4E>(__float_as_uint(v)));
     return v;
 }
 __device__ __forceinline__ M128 grp_bcast0(M128 m) {
