@@ -78,8 +78,9 @@ def oracle_steps(fn, budget_s):
             return k, dt
 
 
-def steinhardt_roofline(fix, system, kms):
-    """Roofline of the dominant per-atom kernel from the algorithmic work per launch."""
+def steinhardt_roofline(fix, system, kms, fused_ld):
+    """Roofline of the dominant per-atom kernel from the algorithmic work per launch.
+    fused_ld: an unbiased local-Q CV is accumulated by the force pass (no k_ld launch)."""
     n = system.n_local
     nsel = fix.extract("Q6.nsel", n)
     pairs = float(nsel.sum())
@@ -88,20 +89,17 @@ def steinhardt_roofline(fix, system, kms):
     n_ghost = x.shape[0] - n
     bytes_sel = float(np.sum(24 + 4 * (c + 1) + 4 * nsel + 4) + 24 * n_ghost)
     work = {"k_env": FLOP_FWD_PAIR * pairs, "k_force": FLOP_GRAD_PAIR * pairs, "k_ld": FLOP_LD_FWD_PAIR * pairs}
-    fused_ld = "k_ld" not in kms and "k_cv_tail" in kms
+    fused_ld = fused_ld and "k_ld" not in kms
     if fused_ld:
         # the unbiased local Q6 is accumulated by the force pass: its work moves there
         work["k_force"] += FLOP_LD_FWD_PAIR * pairs
-    # the gradient phase is two launches: the pair pass and the gather of the partners' shares
     tms = {k: kms[k] for k in work if k in kms}
-    if "k_force" in tms:
-        tms["k_force"] += kms.get("k_force_gather", 0.0)
     dom = max(tms, key=lambda k: tms[k])
     t = tms[dom] * 1e-3
     rate = work[dom] / t / 1e12
     per = {"k_env": FLOP_FWD_PAIR, "k_force": FLOP_GRAD_PAIR + (FLOP_LD_FWD_PAIR if fused_ld else 0.0),
            "k_ld": FLOP_LD_FWD_PAIR}[dom]
-    name = dom + ("+k_force_gather" if dom == "k_force" and "k_force_gather" in kms else "")
+    name = dom
     roof = {"kernel": name, "bound": "fp32", "achieved": rate, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": rate / FP32_PEAK_TFLOPS, "traffic": None,
             "algorithmic": f"{per:.0f} FLOP x {pairs:.0f} selected pairs per step", "mean_ms": tms[dom],
@@ -135,7 +133,7 @@ def run_c2(mdt, args):
     log(f"c2: {dt * 1e6 / args.steps:.1f} us/step, {value:.3e} atom-steps/s")
     kms = fix.profile(args.warmup + args.steps, args.profile_steps)
     log("kernel mean ms: " + json.dumps(kms))
-    roof, pairs = steinhardt_roofline(fix, system, kms)
+    roof, pairs = steinhardt_roofline(fix, system, kms, fused_ld=True)
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc

@@ -1310,9 +1310,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(g, s, f, st)); });
         }
     }
-    bool tail = false;
-    for (auto* c : fx->cvs) tail |= c->fused;
-    if (tail) T.run("k_cv_tail", st, [&] { HIPCHK(launch_cv_tail(fx->red, st)); });
+    // The values of unbiased local-Q CVs fused into the force pass (their tile sums
+    // of the last step) are reduced when the state is read, in fetch_state: nothing
+    // in the step consumes them.
 }
 
 void use_device(mdt_ctx* ctx) {
@@ -1335,6 +1335,12 @@ void set_steps(mdt_fix* fx, int64_t step) {
 void fetch_state(mdt_fix* fx) {
     const int nw = fx->nw();
     fx->h_state.resize((size_t)nw * kStStride);
+    bool tail = false;
+    for (auto* c : fx->cvs) tail |= c->fused;
+    if (tail && fx->built_gen == fx->ctx->generation) {
+        use_device(fx->ctx);
+        HIPCHK(launch_cv_tail(fx->red, fx->ctx->stream));
+    }
     HIPCHK(hipMemcpyAsync(fx->h_state.data(), fx->state.p, sizeof(double) * fx->h_state.size(), hipMemcpyDeviceToHost, fx->ctx->stream));
     HIPCHK(hipStreamSynchronize(fx->ctx->stream));
     fx->state_valid = true;
