@@ -173,8 +173,8 @@ inline int xcd_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 // FP64 lane move through DPP (both 32-bit halves); CTRL is the DPP control word.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)__double_as_longlong(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(__double_as_longlong(v) >> 32), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)__double_as_longlong(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(__double_as_longlong(v) >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 __device__ __forceinline__ double readlane_f64(double v, int l) {
@@ -340,7 +340,7 @@ __device__ __forceinline__ int mnext(M128& it) {
 static_assert(kLanes == 4, "the group helpers are quad permutations");
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 __device__ __forceinline__ uint64_t grp_or_u64(uint64_t v) {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         for (int p = mnext(it); p >= 0; p = mnext(it)) add_ylm(g.x4f[g.idx[beg + p]]);
     }
     const int n = cnt < N ? cnt : N;
-    const float inv_n = n > 0 ? 1.0f / (float)n : 0.0f;
+    const float inv_n = n > 0 ? __builtin_amdgcn_rcpf((float)n) : 0.0f;
     #pragma unroll
     for (int k = kRecMask; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
     float out[RS];
@@ -590,8 +590,8 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
             #pragma unroll
             for (int m = 1; m <= L; ++m) s2 += 2.0f * (q[o + 2 * m - 1] * q[o + 2 * m - 1] + q[o + 2 * m] * q[o + 2 * m]);
             constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
-            const float ql = __builtin_sqrtf(s2 * P);
-            out[rec_a(LMASK, L)] = (ql > 0.0f && n > 0) ? (float)(P * inv_ng) / (ql * (float)n) : 0.0f;
+            const float ql = __builtin_amdgcn_sqrtf(s2 * P);
+            out[rec_a(LMASK, L)] = (ql > 0.0f && n > 0) ? (float)(P * inv_ng) * __builtin_amdgcn_rcpf(ql * (float)n) : 0.0f;
             if (valid && lane == 0) e.ql[(int64_t)li * g.n_local_total + i] = ql;
             part[li] = (center && lane == 0) ? (double)ql : 0.0;
         }
@@ -842,7 +842,7 @@ __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* 
 // Quad broadcast: every lane of an aligned group of four gets lane r's value.
 template <int R>
 __device__ __forceinline__ int quad_bcast(int v) {
-    return __builtin_amdgcn_mov_dpp(v, R | (R << 2) | (R << 4) | (R << 6), 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, R | (R << 2) | (R << 4) | (R << 6), 0xF, 0xF, true);
 }
 
 // LDS staging for the cooperative candidate loads: per lane group one 64-byte
@@ -1030,7 +1030,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
         const bool center = valid && (g.group == nullptr || g.group[i]);
         const float* qi = e.rec + (int64_t)i * RS + LO;
         const int n = e.nsel[i];
-        const float inv = 1.0f / (float)(n + 1);
+        const float inv = __builtin_amdgcn_rcpf((float)(n + 1));
         float s2 = 0.0f;
         #pragma unroll
         for (int k = 0; k < LW; ++k) {
@@ -1038,7 +1038,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
             s2 += (k == 0 ? 1.0f : 2.0f) * v * v;
         }
         constexpr float P = (float)(4.0 * kPi / (2 * LDL + 1));
-        const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
+        const float lq = center ? __builtin_amdgcn_sqrtf(P * s2) : 0.0f;
         double part[1] = {0.0};
         if (valid && lane == 0) {
             e.fld_lq[i] = lq;
