@@ -451,7 +451,6 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     const bool center = valid && (g.group == nullptr || g.group[i]);
     const int beg = g.off[i];
     const int c = center ? g.off[i + 1] - beg : 0;
-    const double4 xi = g.x4[i];
     const double rc2 = e.rc2, rc2s = e.rc2 * (1.0 + kRcSlack);
     const int N = e.N;
     // Phase A: masks.  Each pass of the scan takes 16 consecutive entries of the
@@ -504,6 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
                     in = inrc = true;
                 } else if (!(r2f > r2hi)) {
                     double d0, d1, d2;
+                    const double4 xi = g.x4[i];     // the FP64 position only on this rare path
                     const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
                     inrc = r2 >= 1e-12 && r2 <= rc2s;
                     in = !((r2 > rc2) || (r2 < 1e-12));
@@ -558,6 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
             if (lane == 0) {
                 double* sr2 = reinterpret_cast<double*>(&kept[a][0]);
                 int* sp = reinterpret_cast<int*>(sr2 + NMAX);
+                const double4 xi = g.x4[i];
                 literal_insert_lds<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
                 own = M128{0, 0};
                 for (int s = 0; s < N; ++s)
@@ -844,6 +845,12 @@ template <int R>
 __device__ __forceinline__ int quad_bcast(int v) {
     return __builtin_amdgcn_mov_dpp(v, R | (R << 2) | (R << 4) | (R << 6), 0xF, 0xF, true);
 }
+template <int R>
+__device__ __forceinline__ float quad_bcastf(float v) { return __int_as_float(quad_bcast<R>(__float_as_int(v))); }
+template <int R>
+__device__ __forceinline__ uint64_t quad_bcast64(uint64_t v) {
+    return ((uint64_t)(uint32_t)quad_bcast<R>((int)(v >> 32)) << 32) | (uint32_t)quad_bcast<R>((int)(uint32_t)v);
+}
 
 // LDS staging for the cooperative candidate loads: per lane group one 64-byte
 // quad of each of the four candidate records (padded by one float4 so records and
@@ -879,14 +886,41 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     #pragma unroll
     for (int k = 0; k < NL; ++k)
         wl[k] = e.wslot[k] >= 0 ? (float)e.wcv[(int64_t)tl.walker * kMaxSlotCV + e.wslot[k]] : 0.0f;
-    const ulonglong2 mo = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i];
-    const ulonglong2 mr = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i + 1];
-    const M128 own{valid ? mo.x : 0ull, valid ? mo.y : 0ull};
-    const M128 rcm{valid ? mr.x : 0ull, valid ? mr.y : 0ull};
+    // The atom's own data is loaded once per group, each lane one share, and
+    // broadcast through DPP: the four mask words (own lo/hi, rc lo/hi), the
+    // record, the packed position.
+    const uint64_t mw = valid ? reinterpret_cast<const uint64_t*>(e.selmask)[4 * (int64_t)i + lane] : 0ull;
+    const M128 own{quad_bcast64<0>(mw), quad_bcast64<1>(mw)};
+    const M128 rcm{quad_bcast64<2>(mw), quad_bcast64<3>(mw)};
     const int beg = g.off[i];
     float Ki[QF];
-    load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
-    const PPos pi = unpack_pos(g.x4f[i]);
+    if constexpr (EXTRA) {
+        load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
+    } else {
+        constexpr int NCH = (NV + kLanes - 1) / kLanes;
+        const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)i * RS);
+        float4 ch[NCH];
+        #pragma unroll
+        for (int h = 0; h < NCH; ++h)
+            ch[h] = h * kLanes + lane < NV ? rp[h * kLanes + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float r[NV * 4];
+        #pragma unroll
+        for (int k = 0; k < kRecMask; ++k) r[k] = 0.0f;
+        sfor<NV>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            if constexpr (k >= 1) {
+                const float4 v = ch[k / kLanes];
+                r[4 * k] = quad_bcastf<k % kLanes>(v.x); r[4 * k + 1] = quad_bcastf<k % kLanes>(v.y);
+                r[4 * k + 2] = quad_bcastf<k % kLanes>(v.z); r[4 * k + 3] = quad_bcastf<k % kLanes>(v.w);
+            }
+        });
+        coeff_from_record<LMASK, EXTRA>(e, i, wl, r, Ki);
+    }
+    PPos pi;
+    {
+        const float pw = reinterpret_cast<const float*>(g.x4f + i)[lane];
+        pi = unpack_pos(make_float4(quad_bcastf<0>(pw), quad_bcastf<1>(pw), quad_bcastf<2>(pw), quad_bcastf<3>(pw)));
+    }
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
     // Rounds over the slack-widened rc candidates, lane l taking the set bits of
     // rank l, l+4, ...  In each round the group's four candidate records are
@@ -1027,20 +1061,22 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     if constexpr (LDL != 0) {
         // Lechner-Dellago lq(i) = |qbar_i|, qbar_i = (q_i + sum_{j in S(i)} q_j) / (n_i + 1),
         // from the neighbour records gathered above
-        const bool center = valid && (g.group == nullptr || g.group[i]);
-        const float* qi = e.rec + (int64_t)i * RS + LO;
-        const int n = e.nsel[i];
-        const float inv = __builtin_amdgcn_rcpf((float)(n + 1));
-        float s2 = 0.0f;
+        // group sums on every lane (DPP), then lane 0 alone loads q_i and finishes
         #pragma unroll
-        for (int k = 0; k < LW; ++k) {
-            const float v = (grp_sum(qb[k]) + qi[k]) * inv;
-            s2 += (k == 0 ? 1.0f : 2.0f) * v * v;
-        }
-        constexpr float P = (float)(4.0 * kPi / (2 * LDL + 1));
-        const float lq = center ? __builtin_amdgcn_sqrtf(P * s2) : 0.0f;
+        for (int k = 0; k < LW; ++k) qb[k] = grp_sum(qb[k]);
         double part[1] = {0.0};
         if (valid && lane == 0) {
+            const bool center = g.group == nullptr || g.group[i];
+            const float* qi = e.rec + (int64_t)i * RS + LO;
+            const float inv = __builtin_amdgcn_rcpf((float)(e.nsel[i] + 1));
+            float s2 = 0.0f;
+            #pragma unroll
+            for (int k = 0; k < LW; ++k) {
+                const float v = (qb[k] + qi[k]) * inv;
+                s2 += (k == 0 ? 1.0f : 2.0f) * v * v;
+            }
+            constexpr float P = (float)(4.0 * kPi / (2 * LDL + 1));
+            const float lq = center ? __builtin_amdgcn_sqrtf(P * s2) : 0.0f;
             e.fld_lq[i] = lq;
             part[0] = (double)lq;
         }
