@@ -20,25 +20,28 @@ constexpr int kStack = 16;
 // Bit set of Steinhardt degrees an environment computes.
 constexpr int kL3 = 1, kL4 = 2, kL6 = 4;
 
-// Per-atom record: the atom's own selection mask (2 x u64 in the first four
-// floats), then for each l present (3, 4, 6 in that order) a 16-byte aligned block
-// of 2l+1 floats (Re q_l0, then Re/Im q_lm for m = 1..l) followed by the scale
-// a_l = 4pi/(2l+1) / (N_group q_l n); blocks are padded to 4 floats and the record
-// to 64 bytes, so a neighbour's record is whole 64-byte quads that four lanes
-// fetch together, one float4 each.
-constexpr int kRecMask = 4;
+// Per-atom record: a header of four floats, the atom's packed position (the
+// x4f layout: xyz in FP32 and three 10-bit residuals in bits 0-29 of the fourth
+// word) with two flags in bits 30-31 of that word; then for each l present (3, 4,
+// 6 in that order) a 16-byte aligned block of 2l+1 floats (Re q_l0, then Re/Im
+// q_lm for m = 1..l) followed by the scale a_l = 4pi/(2l+1) / (N_group q_l n).
+// Blocks are padded to 4 floats and the record to 64 bytes, so a neighbour's
+// record is whole 64-byte quads that four lanes fetch together, one float4 each.
+constexpr int kRecHdr = 4;
+constexpr uint32_t kHdrCap = 1u << 30;    // the cap binds: the selection is not the whole in-rc set
+constexpr uint32_t kHdrBand = 1u << 31;   // some list entry needed the exact FP64 distance test
 __host__ __device__ constexpr int rec_nl(int lmask) {
     return ((lmask & kL3) ? 1 : 0) + ((lmask & kL4) ? 1 : 0) + ((lmask & kL6) ? 1 : 0);
 }
 __host__ __device__ constexpr int rec_blk(int l) { return (2 * l + 2 + 3) / 4 * 4; }
 __host__ __device__ constexpr int rec_used(int lmask) {
-    return kRecMask + ((lmask & kL3) ? rec_blk(3) : 0) + ((lmask & kL4) ? rec_blk(4) : 0) +
+    return kRecHdr + ((lmask & kL3) ? rec_blk(3) : 0) + ((lmask & kL4) ? rec_blk(4) : 0) +
            ((lmask & kL6) ? rec_blk(6) : 0);
 }
 __host__ __device__ constexpr int rec_stride(int lmask) { return (rec_used(lmask) + 15) / 16 * 16; }
 // offset of degree l's block inside the record
 __host__ __device__ constexpr int rec_off(int lmask, int l) {
-    return kRecMask + (l == 3 ? 0
+    return kRecHdr + (l == 3 ? 0
                      : l == 4 ? ((lmask & kL3) ? rec_blk(3) : 0)
                               : ((lmask & kL3) ? rec_blk(3) : 0) + ((lmask & kL4) ? rec_blk(4) : 0));
 }

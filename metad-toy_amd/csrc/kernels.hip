@@ -481,6 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     const float r2lo = (float)rc2 - margin, r2hi = (float)rc2s + margin;
     M128 own{0, 0}, rcm{0, 0};
     int cnt = 0;                                     // group-uniform
+    bool fbl = false;                                // this lane took the exact FP64 test
     for (int base = 0; base < c; base += 4 * kLanes) {   // trip count uniform within the group
         int jj[4];
         #pragma unroll
@@ -502,6 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
                 if (r2f < r2lo && r2f > 1e-2f) {
                     in = inrc = true;
                 } else if (!(r2f > r2hi)) {
+                    fbl = true;
                     double d0, d1, d2;
                     const double4 xi = g.x4[i];     // the FP64 position only on this rare path
                     const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
@@ -527,6 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
             rcm.hi |= (uint64_t)crc << (base - 64);
         }
     }
+    const bool band = ((uint32_t)(__ballot(fbl) >> gsh) & 0xFu) != 0u;
     // Phase B: Y_lm of the kept set, round-robin over the group's lanes.
     const PPos pi = unpack_pos(pif);
     float q[QF];
@@ -573,14 +576,16 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     const int n = cnt < N ? cnt : N;
     const float inv_n = n > 0 ? __builtin_amdgcn_rcpf((float)n) : 0.0f;
     #pragma unroll
-    for (int k = kRecMask; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
+    for (int k = kRecHdr; k < QF; ++k) q[k] = grp_sum(q[k]) * inv_n;
     float out[RS];
     #pragma unroll
     for (int k = 0; k < RS; ++k) out[k] = k < QF ? q[k] : 0.0f;
-    out[0] = __uint_as_float((uint32_t)own.lo);
-    out[1] = __uint_as_float((uint32_t)(own.lo >> 32));
-    out[2] = __uint_as_float((uint32_t)own.hi);
-    out[3] = __uint_as_float((uint32_t)(own.hi >> 32));
+    // header: the packed position and the two flags the force pass reads
+    out[0] = pif.x;
+    out[1] = pif.y;
+    out[2] = pif.z;
+    out[3] = __uint_as_float((__float_as_uint(pif.w) & ~(kHdrCap | kHdrBand)) | (cnt > N ? kHdrCap : 0u) |
+                             (band ? kHdrBand : 0u));
     const double inv_ng = e.inv_ng[tl.walker];
     auto finish = [&](auto ll) {
         constexpr int L = decltype(ll)::value;
@@ -802,7 +807,7 @@ __device__ __forceinline__ void coeff_from_record(const EnvDev& e, int a, const 
     if constexpr (EXTRA) {
         const float4* xp = reinterpret_cast<const float4*>(e.extra + (int64_t)a * RS);
         #pragma unroll
-        for (int k = 0; k < kRecMask; ++k) x[k] = 0.0f;
+        for (int k = 0; k < kRecHdr; ++k) x[k] = 0.0f;
         #pragma unroll
         for (int k = 1; k < NV; ++k) { float4 v = xp[k]; x[4*k] = v.x; x[4*k+1] = v.y; x[4*k+2] = v.z; x[4*k+3] = v.w; }
     }
@@ -834,7 +839,7 @@ __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* 
     const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)a * RS);
     float r[NV * 4];
     #pragma unroll
-    for (int k = 0; k < kRecMask; ++k) r[k] = 0.0f;
+    for (int k = 0; k < kRecHdr; ++k) r[k] = 0.0f;
     #pragma unroll
     for (int k = 1; k < NV; ++k) { float4 v = rp[k]; r[4*k] = v.x; r[4*k+1] = v.y; r[4*k+2] = v.z; r[4*k+3] = v.w; }
     coeff_from_record<LMASK, EXTRA>(e, a, wl, r, K);
@@ -894,8 +899,10 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     const M128 rcm{quad_bcast64<2>(mw), quad_bcast64<3>(mw)};
     const int beg = g.off[i];
     float Ki[QF];
+    uint32_t hwi;                                     // the atom's header flags word
     if constexpr (EXTRA) {
         load_coeff<LMASK, EXTRA>(e, i, wl, Ki);
+        hwi = __float_as_uint(e.rec[(int64_t)i * RS + 3]);
     } else {
         constexpr int NCH = (NV + kLanes - 1) / kLanes;
         const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)i * RS);
@@ -905,7 +912,8 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
             ch[h] = h * kLanes + lane < NV ? rp[h * kLanes + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float r[NV * 4];
         #pragma unroll
-        for (int k = 0; k < kRecMask; ++k) r[k] = 0.0f;
+        for (int k = 0; k < kRecHdr; ++k) r[k] = 0.0f;
+        hwi = __float_as_uint(quad_bcastf<0>(ch[0].w));
         sfor<NV>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
             if constexpr (k >= 1) {
@@ -948,7 +956,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
         const int o = p >= 0 ? g.eown[beg + p] : i;
         const int j = p >= 0 ? g.idx[beg + p] : i;
         const int o0 = quad_bcast<0>(o), o1 = quad_bcast<1>(o), o2 = quad_bcast<2>(o), o3 = quad_bcast<3>(o);
-        const float4 pj = g.x4f[j];                  // this lane's candidate, packed position
         // records one quad at a time: lane l fetches float4 l of the quad of each
         // of the four records, then reads back its own candidate's quad
         float rr[NV * 4];
@@ -970,12 +977,25 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
                 }
             });
         });
+        // the candidate's packed position: a local neighbour's is its record header,
+        // a ghost image's is loaded
+        float4 pj = make_float4(rr[0], rr[1], rr[2], rr[3]);
+        if (j >= g.n_local_total) pj = g.x4f[j];
+        // Reverse selection: the owner selects i.  When the owner's cap does not bind
+        // and no entry of i lies near rc (where the two frames' FP64 distances might
+        // decide differently), every candidate inside rc is selected by its owner;
+        // otherwise the owner's selection mask decides.
         bool is_own = false, is_rev = false;
         if (p >= 0) {
-            const uint64_t mlo = (uint64_t)__float_as_uint(rr[0]) | ((uint64_t)__float_as_uint(rr[1]) << 32);
-            const uint64_t mhi = (uint64_t)__float_as_uint(rr[2]) | ((uint64_t)__float_as_uint(rr[3]) << 32);
             is_own = mbit(own, p);
-            is_rev = rp != 0xFF && (rp < 64 ? ((mlo >> rp) & 1ull) : ((mhi >> (rp - 64)) & 1ull));
+            if (rp == 0xFF) {
+                is_rev = false;
+            } else if (((__float_as_uint(rr[3]) & kHdrCap) | (hwi & kHdrBand)) == 0u) {
+                is_rev = true;
+            } else {
+                const uint64_t w = reinterpret_cast<const uint64_t*>(e.selmask)[4 * (int64_t)o + (rp >> 6)];
+                is_rev = (w >> (rp & 63)) & 1ull;
+            }
         }
         // every active pair of the wave selected from both sides (the usual case when
         // the cap does not bind): the combination needs no per-lane selects
