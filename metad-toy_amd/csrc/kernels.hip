@@ -943,18 +943,31 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     M128 it = rcm;
     mdrop(it, lane);
     uint64_t it64 = it.lo;
-    for (int rd = 0; rd < nrounds; ++rd) {
-        int p;
+    auto next_p = [&]() {
+        int q;
         if (!wide) {
-            p = (int)__ffsll((unsigned long long)it64) - 1;
+            q = (int)__ffsll((unsigned long long)it64) - 1;
             #pragma unroll
             for (int k = 0; k < kLanes; ++k) it64 &= it64 - 1ull;
         } else {
-            p = mnext(it);
+            q = mnext(it);
         }
-        const int rp = p >= 0 ? g.revpos[beg + p] : 0xFF;
-        const int o = p >= 0 ? g.eown[beg + p] : i;
-        const int j = p >= 0 ? g.idx[beg + p] : i;
+        return q;
+    };
+    // the list words (reverse position, owner, index) of the next round are loaded
+    // while the current round runs
+    int pn = next_p();
+    int rpn = pn >= 0 ? g.revpos[beg + pn] : 0xFF;
+    int on = pn >= 0 ? g.eown[beg + pn] : i;
+    int jn = pn >= 0 ? g.idx[beg + pn] : i;
+    for (int rd = 0; rd < nrounds; ++rd) {
+        const int p = pn, rp = rpn, o = on, j = jn;
+        if (rd + 1 < nrounds) {                      // uniform within the group
+            pn = next_p();
+            rpn = pn >= 0 ? g.revpos[beg + pn] : 0xFF;
+            on = pn >= 0 ? g.eown[beg + pn] : i;
+            jn = pn >= 0 ? g.idx[beg + pn] : i;
+        }
         const int o0 = quad_bcast<0>(o), o1 = quad_bcast<1>(o), o2 = quad_bcast<2>(o), o3 = quad_bcast<3>(o);
         // records one quad at a time: lane l fetches float4 l of the quad of each
         // of the four records, then reads back its own candidate's quad
