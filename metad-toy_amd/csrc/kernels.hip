@@ -1377,6 +1377,9 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     __shared__ Dual stack[kStack];
     __shared__ double symv[kMaxSym];
     const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
+    // the step counter and hill count load while the partial sums do
+    const int64_t step = r.step[w];
+    const int64_t nhill0 = r.nhills[w];
     // expression programs to LDS (read serially by thread 0 below)
     {
         const int nwords = r.D * (int)(sizeof(ExprProg) / 4);
@@ -1444,7 +1447,6 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     }
     __syncthreads();
     // 3. deposit
-    const int64_t step = r.step[w];
     const bool dep = r.pace > 0 && (step % r.pace) == 0;
     double dV[kMaxDim] = {0, 0, 0};
     if (dep) {
@@ -1455,7 +1457,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         const double hgt = bc[0];
         if (r.kind == BIAS_GRID) grid_deposit_block(r, r.grid + (int64_t)w * r.grid_size, s, hgt, r.sigma[0]);
         if (threadIdx.x == 0) {
-            int64_t k = r.nhills[w];
+            int64_t k = nhill0;
             if (k < r.hills_cap) {
                 double* rec = r.hills + ((int64_t)w * r.hills_cap + k) * (r.D + 3);
                 rec[0] = (double)step;
