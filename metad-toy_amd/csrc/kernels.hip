@@ -870,7 +870,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
                   "the fused local-Q degree must be on the environment");
     constexpr int LW = LDL ? 2 * LDL + 1 : 1;           // fused local-Q accumulators
     constexpr int LO = LDL ? rec_off(LMASK, LDL) : 0;
-    __shared__ double shl[4];
     float qb[LW];
     #pragma unroll
     for (int k = 0; k < LW; ++k) qb[k] = 0.0f;
@@ -880,7 +879,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     constexpr int RS = rec_stride(LMASK);
     constexpr int NQ = RS / 16;                // 64-byte quads per record
     constexpr int NV = QF / 4;                 // float4 of the record that are used
-    __shared__ ForceStage stg;
+    // per wave: for each quad and candidate slot, the 64 lanes' float4 of one global
+    // to LDS load (32 KB per workgroup for two-quad records)
+    __shared__ float4 dst[kBlock / 64][NQ][kLanes][64];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
@@ -936,7 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     // and staged in LDS; each lane loads its own candidate's packed position and
     // evaluates its pair.
     const int nrounds = (mpop_count(rcm) + kLanes - 1) / kLanes;   // uniform within the group
-    float4 (*srec)[kLanes + 1] = stg.rec[a];
+    const int wv = threadIdx.x >> 6, gq = (threadIdx.x & 63) >> 2;   // wave, lane group within it
     // lists of at most 64 entries (the usual case, uniform over the wave): the
     // lane's share of the low word is walked with plain 64-bit clears
     const bool wide = __ballot(rcm.hi != 0ull) != 0ull;
@@ -962,34 +963,42 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     int jn = pn >= 0 ? g.idx[beg + pn] : i;
     for (int rd = 0; rd < nrounds; ++rd) {
         const int p = pn, rp = rpn, o = on, j = jn;
+        // The group's four candidate records go straight from global memory into
+        // LDS, one quad of one candidate per load instruction, every instruction
+        // touching one line per group.  In the load for candidate slot c, lane l of
+        // a group fetches float4 (l + c) mod 4 of the quad, so that each lane later
+        // reads its own candidate's quad from four different banks.
+        wave_lds_sync();                             // last round's reads are complete
+        {
+            const int oc[kLanes] = {quad_bcast<0>(o), quad_bcast<1>(o), quad_bcast<2>(o), quad_bcast<3>(o)};
+            #pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                #pragma unroll
+                for (int c = 0; c < kLanes; ++c) {
+                    const float4* src = reinterpret_cast<const float4*>(e.rec + (int64_t)oc[c] * RS) + 4 * q + ((lane + c) & 3);
+                    __builtin_amdgcn_global_load_lds((const void*)src,
+                                                     (__attribute__((address_space(3))) void*)&dst[wv][q][c][0], 16, 0, 0);
+                }
+            }
+        }
         if (rd + 1 < nrounds) {                      // uniform within the group
             pn = next_p();
             rpn = pn >= 0 ? g.revpos[beg + pn] : 0xFF;
             on = pn >= 0 ? g.eown[beg + pn] : i;
             jn = pn >= 0 ? g.idx[beg + pn] : i;
         }
-        const int o0 = quad_bcast<0>(o), o1 = quad_bcast<1>(o), o2 = quad_bcast<2>(o), o3 = quad_bcast<3>(o);
-        // records one quad at a time: lane l fetches float4 l of the quad of each
-        // of the four records, then reads back its own candidate's quad
         float rr[NV * 4];
-        sfor<NQ>([&](auto qq) {
-            constexpr int q = decltype(qq)::value;
-            const float4 v0 = reinterpret_cast<const float4*>(e.rec + (int64_t)o0 * RS)[4 * q + lane];
-            const float4 v1 = reinterpret_cast<const float4*>(e.rec + (int64_t)o1 * RS)[4 * q + lane];
-            const float4 v2 = reinterpret_cast<const float4*>(e.rec + (int64_t)o2 * RS)[4 * q + lane];
-            const float4 v3 = reinterpret_cast<const float4*>(e.rec + (int64_t)o3 * RS)[4 * q + lane];
-            wave_lds_sync();
-            srec[0][lane] = v0; srec[1][lane] = v1; srec[2][lane] = v2; srec[3][lane] = v3;
-            wave_lds_sync();
-            sfor<4>([&](auto kk) {
-                constexpr int k = decltype(kk)::value;
-                if constexpr (4 * q + k < NV) {
-                    const float4 u4 = srec[lane][k];
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            #pragma unroll
+            for (int k = 0; k < kLanes; ++k) {
+                if (4 * q + k < NV) {
+                    const float4 u4 = dst[wv][q][lane][4 * gq + ((k - lane) & 3)];
                     rr[16 * q + 4 * k] = u4.x; rr[16 * q + 4 * k + 1] = u4.y;
                     rr[16 * q + 4 * k + 2] = u4.z; rr[16 * q + 4 * k + 3] = u4.w;
                 }
-            });
-        });
+            }
+        }
         // the candidate's packed position: a local neighbour's is its record header,
         // a ghost image's is loaded
         float4 pj = make_float4(rr[0], rr[1], rr[2], rr[3]);
@@ -1113,7 +1122,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
             e.fld_lq[i] = lq;
             part[0] = (double)lq;
         }
-        block_sum<1>(part, shl);
+        // the staging buffer is free again: its first bytes hold the block-sum scratch
+        __syncthreads();
+        block_sum<1>(part, reinterpret_cast<double*>(&dst[0][0][0][0]));
         if (threadIdx.x == 0) e.partial[(int64_t)e.fld_slot * g.ntiles + t] = part[0];
     }
 }
