@@ -59,10 +59,12 @@ def c2_system(mdt):
     return s.build(RG)
 
 
-def timed_run(fix, step0, steps):
+def timed_run(fix, step0, steps, per_launch=None):
+    """Wall time of `steps` steps on the device (graphs of `per_launch` captured steps
+    per launch, the library's batch by default), ended by a state read."""
     fix.sync()
     t0 = time.perf_counter()
-    fix.run(step0, steps)
+    fix.run(step0, steps, per_launch)
     fix.sync()
     return time.perf_counter() - t0
 
@@ -131,7 +133,11 @@ def run_c2(mdt, args):
     dt = timed_run(fix, args.warmup, args.steps)
     value = n * args.steps / dt
     log(f"c2: {dt * 1e6 / args.steps:.1f} us/step, {value:.3e} atom-steps/s")
-    kms = fix.profile(args.warmup + args.steps, args.profile_steps)
+    # the same steps with one graph launch per step (what a caller that runs an
+    # integrator between steps pays)
+    dt1 = timed_run(fix, args.warmup + args.steps, args.steps, per_launch=1)
+    log(f"c2, one launch per step: {dt1 * 1e6 / args.steps:.1f} us/step, {n * args.steps / dt1:.3e} atom-steps/s")
+    kms = fix.profile(args.warmup + 2 * args.steps, args.profile_steps)
     log("kernel mean ms: " + json.dumps(kms))
     roof, pairs = steinhardt_roofline(fix, system, kms, fused_ld=True)
     cpu = None
@@ -148,7 +154,9 @@ def run_c2(mdt, args):
                "sample": f"{k} full step(s) of the same {n}-atom workload (selection, Q6, Q4, LQ6, exact "
                          f"Q6+Q4 forces) by the FP64 C++ oracle on 1 core, {t:.1f} s"}
     cfg = {"workload": "config 2: 256,000-atom FCC Ar half melted, Q6.AVE+Q4.AVE+LD-LQ6 (N=18, rc=3.9), "
-                       "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms}
+                       "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms,
+           "steps_per_graph_launch": 8,
+           "one_launch_per_step": {"value": n * args.steps / dt1, "ms_per_step": dt1 * 1e3 / args.steps}}
     return line(value, args.steps, args.warmup, dt * 1e3 / args.steps, "f32 pair math, f64 sums/grid", cfg, roof, cpu)
 
 

@@ -120,9 +120,14 @@ int mdt_fix_init(mdt_fix* fix);
 int mdt_fix_post_force(mdt_fix* fix, int64_t step, double* f, int on_device);
 
 /* `nsteps` consecutive steps step0..step0+nsteps-1 into the context's device
- * force buffer with one graph launch per step and no host synchronisation
- * (standalone/batched driver).  Results are complete after mdt_fix_sync. */
+ * force buffer with no host synchronisation (standalone/batched driver); every
+ * step is the whole path of mdt_fix_post_force.  Results are complete after
+ * mdt_fix_sync.  mdt_fix_run_steps replays graphs of `steps_per_launch` (1..64)
+ * captured steps; mdt_fix_run uses 8.  With a learned bias the launches stop at
+ * every refit step.  No reference counterpart: the reference has no driver
+ * outside the LAMMPS step loop. */
 int mdt_fix_run(mdt_fix* fix, int64_t step0, int64_t nsteps);
+int mdt_fix_run_steps(mdt_fix* fix, int64_t step0, int64_t nsteps, int steps_per_launch);
 int mdt_fix_sync(mdt_fix* fix);
 
 /* Run `nsteps` steps with plain launches and a HIP event pair around every

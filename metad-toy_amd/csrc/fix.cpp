@@ -433,8 +433,9 @@ struct mdt_fix {
     std::vector<int64_t> hills_written;   // per walker, hills already in the HILLS file
     std::vector<int64_t> hills_bound;     // per walker upper bound of deposited hills
     std::vector<FILE*> hfiles;
-    // graph
-    hipGraphExec_t gexec = nullptr;
+    // captured runs of k consecutive steps, by k (the per-step state, the step
+    // counter included, lives on the device, so steps replay back to back)
+    std::map<int, hipGraphExec_t> run_graphs;
     uint64_t graph_gen = 0;
     // captured training loops of the learned bias, per (walker, first fit)
     std::map<std::pair<int, int>, hipGraphExec_t> mlp_graphs;
@@ -446,7 +447,7 @@ struct mdt_fix {
     bool state_valid = false;
 
     ~mdt_fix() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& kv : run_graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto& kv : mlp_graphs) (void)hipGraphExecDestroy(kv.second);
         for (FILE* fp : hfiles) if (fp) fclose(fp);
         for (auto* e : envs) delete e;
@@ -1566,8 +1567,41 @@ int mdt_fix_post_force(mdt_fix* fx, int64_t step, double* f, int on_device) {
     API_END
 }
 
-int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
+constexpr int kRunBatch = 8;   // steps per graph launch in mdt_fix_run
+
+// The graph of k captured steps, built on first use and after a rebuild.
+static hipGraphExec_t run_graph(mdt_fix* fx, int k) {
+    mdt_ctx* ctx = fx->ctx;
+    if (fx->graph_gen != ctx->generation + 1) {
+        for (auto& kv : fx->run_graphs) (void)hipGraphExecDestroy(kv.second);
+        fx->run_graphs.clear();
+        fx->graph_gen = ctx->generation + 1;
+    }
+    auto it = fx->run_graphs.find(k);
+    if (it != fx->run_graphs.end()) return it->second;
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int s = 0; s < k; ++s) enqueue_step(fx, ctx->f.p);
+    } catch (...) {
+        hipGraph_t dummy = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) throw Err(MDT_EHIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e)));
+    fx->run_graphs[k] = ex;
+    return ex;
+}
+
+int mdt_fix_run_steps(mdt_fix* fx, int64_t step0, int64_t nsteps, int steps_per_launch) {
     FIX_BEGIN(fx)
+    if (steps_per_launch < 1 || steps_per_launch > 64) throw Err(MDT_EINVAL, "steps_per_launch must be in 1..64");
+    if (nsteps < 0) throw Err(MDT_EINVAL, "nsteps must be >= 0");
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
     ensure_built(fx);
@@ -1575,30 +1609,25 @@ int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
     ensure_hills(fx, ndep);
     for (auto& b : fx->hills_bound) b += ndep;
     set_steps(fx, step0);
-    if (!fx->gexec || fx->graph_gen != ctx->generation + 1) {
-        if (fx->gexec) { (void)hipGraphExecDestroy(fx->gexec); fx->gexec = nullptr; }
-        hipGraph_t gr = nullptr;
-        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueue_step(fx, ctx->f.p);
-        } catch (...) {
-            hipGraph_t dummy = nullptr;
-            (void)hipStreamEndCapture(ctx->stream, &dummy);
-            if (dummy) (void)hipGraphDestroy(dummy);
-            throw;
-        }
-        HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
-        hipError_t e = hipGraphInstantiate(&fx->gexec, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e != hipSuccess) throw Err(MDT_EHIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e)));
-        fx->graph_gen = ctx->generation + 1;
-    }
-    for (int64_t s = 0; s < nsteps; ++s) {
-        HIPCHK(hipGraphLaunch(fx->gexec, ctx->stream));
-        if (fx->hill_type == 3 && (step0 + s + 1) % fx->mlp_refit == 0) mlp_refit_all(fx);
+    const int K = steps_per_launch;
+    hipGraphExec_t g1 = run_graph(fx, 1);
+    hipGraphExec_t gk = K > 1 ? run_graph(fx, K) : g1;
+    // whole graphs of K steps, then single steps; with a learned bias each run of
+    // launches ends at a refit step
+    int64_t s = 0;
+    while (s < nsteps) {
+        int64_t seg = nsteps - s;
+        if (fx->hill_type == 3) seg = std::min<int64_t>(seg, fx->mlp_refit - (step0 + s) % fx->mlp_refit);
+        for (; seg >= K; seg -= K, s += K) HIPCHK(hipGraphLaunch(gk, ctx->stream));
+        for (; seg > 0; --seg, ++s) HIPCHK(hipGraphLaunch(g1, ctx->stream));
+        if (fx->hill_type == 3 && (step0 + s) % fx->mlp_refit == 0) mlp_refit_all(fx);
     }
     fx->state_valid = false;
     API_END
+}
+
+int mdt_fix_run(mdt_fix* fx, int64_t step0, int64_t nsteps) {
+    return mdt_fix_run_steps(fx, step0, nsteps, kRunBatch);
 }
 
 int mdt_fix_profile(mdt_fix* fx, int64_t step0, int64_t nsteps, char* names, int names_cap,

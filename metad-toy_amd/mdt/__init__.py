@@ -55,6 +55,7 @@ def lib():
         "mdt_fix_init": (I, [P]),
         "mdt_fix_post_force": (I, [P, I64, P, I]),
         "mdt_fix_run": (I, [P, I64, I64]),
+        "mdt_fix_run_steps": (I, [P, I64, I64, I]),
         "mdt_fix_sync": (I, [P]),
         "mdt_fix_profile": (I, [P, I64, I64, C.c_char_p, I, dp, I, C.POINTER(I)]),
         "mdt_fix_scalar": (I, [P, I, dp]),
@@ -279,8 +280,13 @@ class Fix:
             assert f.dtype == np.float64 and f.flags.c_contiguous
             self._chk(lib().mdt_fix_post_force(self._h, step, f.ctypes.data, 0))
 
-    def run(self, step0, nsteps):
-        self._chk(lib().mdt_fix_run(self._h, step0, nsteps))
+    def run(self, step0, nsteps, per_launch=None):
+        """nsteps steps on the device; per_launch = captured steps per graph launch
+        (default: the library's batch)."""
+        if per_launch is None:
+            self._chk(lib().mdt_fix_run(self._h, step0, nsteps))
+        else:
+            self._chk(lib().mdt_fix_run_steps(self._h, step0, nsteps, per_launch))
 
     def sync(self):
         self._chk(lib().mdt_fix_sync(self._h))

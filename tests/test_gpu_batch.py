@@ -132,3 +132,31 @@ def test_walkers_graph_equals_single(mdt, orc):
         assert f1.scalar() == fix.scalar(w)
         f1.close()
         c1.close()
+
+
+def test_run_batches_equal_single_launches(mdt):
+    """Steps replayed as graphs of 8 captured steps give the same bits as one launch per
+    step: forces, CVs, the bias state and the deposited hills (well-tempered, PACE 3,
+    23 steps: batches, a remainder, and deposits inside and across batches)."""
+    s = fcc_system(mdt, 6, sigma=0.15, seed=52)
+    n = s.n_local
+    args = ("GAUSSIAN 0.01 0.005 10 PACE 3 TEMP 300 HILLS_FILE NONE "
+            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE DIM 1 0 2 2000 v1+v2 WT 1")
+    out = []
+    for per_launch in (1, 8, None):
+        ctx = mdt.Context(0, 1)
+        ctx.bind_system(0, s)
+        fix = mdt.Fix(ctx, args)
+        fix.init()
+        fix.zero_forces()
+        fix.run(0, 23, per_launch)
+        fix.sync()
+        out.append((fix.forces(n), fix.cv("Q6.AVE"), fix.cv("Q4.AVE"), fix.scalar(), fix.hills(1)))
+        print(f"[run batches] per_launch={per_launch} V={out[-1][3]:.6g} hills={len(out[-1][4])}", flush=True)
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0])
+        assert o[1] == out[0][1] and o[2] == out[0][2] and o[3] == out[0][3]
+        assert np.array_equal(o[4], out[0][4])
+    assert len(out[0][4]) == 8          # steps 0, 3, ..., 21
