@@ -798,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_ld_coeff(Geo g, EnvDev e, LdDev d) {
 // the two coefficient sets added before the contraction.  kLanes lanes per
 // atom share the active pairs round-robin.
 // ---------------------------------------------------------------------------
-// K from a record already in registers (mask quad ignored)
+// K from a record already in registers (header quad ignored)
 template <int LMASK, bool EXTRA>
 __device__ __forceinline__ void coeff_from_record(const EnvDev& e, int a, const float* wl, const float* r, float* K) {
     constexpr int RS = rec_stride(LMASK);
@@ -835,7 +835,7 @@ __device__ __forceinline__ void coeff_from_record(const EnvDev& e, int a, const 
 template <int LMASK, bool EXTRA>
 __device__ __forceinline__ void load_coeff(const EnvDev& e, int a, const float* wl, float* K) {
     constexpr int RS = rec_stride(LMASK);
-    constexpr int NV = rec_used(LMASK) / 4;     // float4s actually used (the first holds the mask)
+    constexpr int NV = rec_used(LMASK) / 4;     // float4s actually used (the first is the header)
     const float4* rp = reinterpret_cast<const float4*>(e.rec + (int64_t)a * RS);
     float r[NV * 4];
     #pragma unroll
