@@ -1387,10 +1387,14 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     __shared__ double bc[8];
     __shared__ Dual stack[kStack];
     __shared__ double symv[kMaxSym];
+    __shared__ double wcs[kMaxSlotCV];
     const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
-    // the step counter and hill count load while the partial sums do
+    // per-walker scalars load while the partial sums do
     const int64_t step = r.step[w];
     const int64_t nhill0 = r.nhills[w];
+    const double inv_ng = r.inv_ng != nullptr ? r.inv_ng[w] : 0.0;   // NULL without a Steinhardt environment
+    const double mlp_err = r.kind == BIAS_MLP ? r.mlp_err[w] : 0.0;
+    const int mlp_ok = r.kind == BIAS_MLP ? r.mlp_ok[w] : 0;
     // expression programs to LDS (read serially by thread 0 below)
     {
         const int nwords = r.D * (int)(sizeof(ExprProg) / 4);
@@ -1437,7 +1441,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
                 double a = 0.0;
                 #pragma unroll
                 for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
-                cv[c] = a * r.inv_ng[w];
+                cv[c] = a * inv_ng;
             } else {
                 // direct values; CVs summed after the force pass (cv_src <= -2) are
                 // unbiased and written by k_cv_tail
@@ -1490,17 +1494,20 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         st[kStV] = V;
         st[kStHill] = dep ? bc[0] : 0.0;
         st[kStStep] = (double)step;
-        st[kStMlpErr] = r.kind == BIAS_MLP ? r.mlp_err[w] : 0.0;
-        st[kStMlpOk] = r.kind == BIAS_MLP ? (double)r.mlp_ok[w] : 0.0;
-        // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv
-        double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
-        for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = 0.0;
+        st[kStMlpErr] = mlp_err;
+        st[kStMlpOk] = (double)mlp_ok;
+        // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv, summed
+        // in LDS and stored once
+        for (int c = 0; c < kMaxSlotCV; ++c) wcs[c] = 0.0;
         for (int q = 0; q < r.nsym; ++q) {
             double acc = 0.0;
             #pragma unroll
             for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc += dV[d] * dsdv[d * kMaxSym + q];
-            wc[r.sym_slot[q]] += acc;
+            wcs[r.sym_slot[q]] += acc;
         }
+        double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
+        #pragma unroll
+        for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = wcs[c];
         r.step[w] = step + 1;
     }
 }
