@@ -153,8 +153,15 @@ def run_c2(mdt, args):
         cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
                "sample": f"{k} full step(s) of the same {n}-atom workload (selection, Q6, Q4, LQ6, exact "
                          f"Q6+Q4 forces) by the FP64 C++ oracle on 1 core, {t:.1f} s"}
+    # whole-step roofline: sum over phases of max(bytes / HBM peak, FLOP / FP32 peak):
+    # selection (HBM), forward q_lm, gradient, local-Q6 forward (FP32)
+    t_roof = (roof["selection_bytes"] / (HBM_PEAK_GBS * 1e9) +
+              (FLOP_FWD_PAIR + FLOP_GRAD_PAIR + FLOP_LD_FWD_PAIR) * pairs / (FP32_PEAK_TFLOPS * 1e12))
+    step_roof = {"t_roof_us": t_roof * 1e6, "t_step_us": dt * 1e6 / args.steps, "frac": t_roof / (dt / args.steps),
+                 "phases": "selection bytes / 8 TB/s + (261 + 903 + 28) FLOP x selected pairs / 157.3 TFLOP/s"}
     cfg = {"workload": "config 2: 256,000-atom FCC Ar half melted, Q6.AVE+Q4.AVE+LD-LQ6 (N=18, rc=3.9), "
                        "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms,
+           "step_roofline": step_roof,
            "steps_per_graph_launch": 8,
            "one_launch_per_step": {"value": n * args.steps / dt1, "ms_per_step": dt1 * 1e3 / args.steps}}
     return line(value, args.steps, args.warmup, dt * 1e3 / args.steps, "f32 pair math, f64 sums/grid", cfg, roof, cpu)
