@@ -857,13 +857,6 @@ __device__ __forceinline__ uint64_t quad_bcast64(uint64_t v) {
     return ((uint64_t)(uint32_t)quad_bcast<R>((int)(v >> 32)) << 32) | (uint32_t)quad_bcast<R>((int)(uint32_t)v);
 }
 
-// LDS staging for the cooperative candidate loads: per lane group one 64-byte
-// quad of each of the four candidate records (padded by one float4 so records and
-// groups fall on distinct banks).
-struct ForceStage {
-    float4 rec[kBlock / kLanes][kLanes][kLanes + 1];
-};
-
 template <int LMASK, bool EXTRA, int LDL>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __restrict__ f) {
     static_assert(LDL == 0 || (LDL == 3 && (LMASK & kL3)) || (LDL == 4 && (LMASK & kL4)) || (LDL == 6 && (LMASK & kL6)),
@@ -934,8 +927,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     // Rounds over the slack-widened rc candidates, lane l taking the set bits of
     // rank l, l+4, ...  In each round the group's four candidate records are
     // fetched cooperatively (every load instruction of the group touches one line)
-    // and staged in LDS; each lane loads its own candidate's packed position and
-    // evaluates its pair.
+    // straight into LDS; each lane reads back its own candidate's record, whose
+    // header carries the candidate's packed position (ghost images load theirs),
+    // and evaluates its pair.
     const int nrounds = (mpop_count(rcm) + kLanes - 1) / kLanes;   // uniform within the group
     const int wv = threadIdx.x >> 6, gq = (threadIdx.x & 63) >> 2;   // wave, lane group within it
     // lists of at most 64 entries (the usual case, uniform over the wave): the
