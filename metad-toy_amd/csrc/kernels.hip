@@ -1429,8 +1429,9 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < r.ncv; ++c) {
+    if (threadIdx.x < 64) {                          // wave 0: one CV slot per lane
+        const int c = threadIdx.x;
+        if (c < r.ncv) {
             if (r.cv_src[c] >= 0) {
                 double a = 0.0;
                 #pragma unroll
@@ -1438,25 +1439,63 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
                 cv[c] = a * inv_ng;
             } else {
                 // direct values; CVs summed after the force pass (cv_src <= -2) are
-                // unbiased and written by k_cv_tail
+                // unbiased and reduced when the state is read
                 cv[c] = r.cv_src[c] == -1 ? r.direct[(int64_t)w * kMaxSlotCV + c] : 0.0;
             }
         }
-        // 2. dimensions
-        for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
-        for (int d = 0; d < r.D; ++d) {
-            if (progs[d].linear) {
-                double v = progs[d].lin0;
-                for (int q = 0; q < r.nsym; ++q) { v += progs[d].lin[q] * symv[q]; dsdv[d * kMaxSym + q] = progs[d].lin[q]; }
-                s[d] = v;
-            } else {
-                expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
+        wave_lds_sync();
+        if (threadIdx.x == 0) {
+            // 2. dimensions
+            for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
+            for (int d = 0; d < r.D; ++d) {
+                if (progs[d].linear) {
+                    double v = progs[d].lin0;
+                    for (int q = 0; q < r.nsym; ++q) { v += progs[d].lin[q] * symv[q]; dsdv[d * kMaxSym + q] = progs[d].lin[q]; }
+                    s[d] = v;
+                } else {
+                    expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
+                }
             }
         }
     }
+    const bool dep = r.pace > 0 && (step % r.pace) == 0;
+    // thread 0: the walker's state, the chain-rule weights and the step counter
+    auto finish = [&](double V, const double* dVs) {
+        double* st = r.state + (int64_t)w * kStStride;
+        for (int c = 0; c < r.ncv; ++c) st[kStCv + c] = cv[c];
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { st[kStS + d] = s[d]; st[kStDV + d] = dVs[d]; }
+        st[kStV] = V;
+        st[kStHill] = dep ? bc[0] : 0.0;
+        st[kStStep] = (double)step;
+        st[kStMlpErr] = mlp_err;
+        st[kStMlpOk] = (double)mlp_ok;
+        // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv, summed
+        // in LDS and stored once
+        for (int c = 0; c < kMaxSlotCV; ++c) wcs[c] = 0.0;
+        for (int q = 0; q < r.nsym; ++q) {
+            double acc = 0.0;
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc += dVs[d] * dsdv[d * kMaxSym + q];
+            wcs[r.sym_slot[q]] += acc;
+        }
+        double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
+        #pragma unroll
+        for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = wcs[c];
+        r.step[w] = step + 1;
+    };
+    if (r.kind == BIAS_GRID && !dep) {
+        // a step without a deposit on a grid: thread 0 alone evaluates and finishes,
+        // with no further workgroup barrier (uniform over the block)
+        if (threadIdx.x == 0) {
+            double dVg[kMaxDim] = {0, 0, 0};
+            const double V = grid_eval(r, r.grid + (int64_t)w * r.grid_size, s, dVg);
+            finish(V, dVg);
+        }
+        return;
+    }
     __syncthreads();
     // 3. deposit
-    const bool dep = r.pace > 0 && (step % r.pace) == 0;
     double dV[kMaxDim] = {0, 0, 0};
     if (dep) {
         double Vb = 0.0;
@@ -1480,30 +1519,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     }
     // 4. bias value and gradient after the deposit
     const double V = bias_eval_block<kRedThreads / 64>(r, w, s, dV, sh);
-    if (threadIdx.x == 0) {
-        double* st = r.state + (int64_t)w * kStStride;
-        for (int c = 0; c < r.ncv; ++c) st[kStCv + c] = cv[c];
-        #pragma unroll
-        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { st[kStS + d] = s[d]; st[kStDV + d] = dV[d]; }
-        st[kStV] = V;
-        st[kStHill] = dep ? bc[0] : 0.0;
-        st[kStStep] = (double)step;
-        st[kStMlpErr] = mlp_err;
-        st[kStMlpOk] = (double)mlp_ok;
-        // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv, summed
-        // in LDS and stored once
-        for (int c = 0; c < kMaxSlotCV; ++c) wcs[c] = 0.0;
-        for (int q = 0; q < r.nsym; ++q) {
-            double acc = 0.0;
-            #pragma unroll
-            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc += dV[d] * dsdv[d * kMaxSym + q];
-            wcs[r.sym_slot[q]] += acc;
-        }
-        double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
-        #pragma unroll
-        for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = wcs[c];
-        r.step[w] = step + 1;
-    }
+    if (threadIdx.x == 0) finish(V, dV);
 }
 
 // Values of the CVs whose partial sums come from the force pass (unbiased local-Q
