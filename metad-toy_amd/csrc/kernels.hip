@@ -1707,33 +1707,52 @@ __global__ __launch_bounds__(kBlock) void k_sk_sum(SkDev sk) {
 
 // F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j}); one atom per thread, the k loop
 // index is uniform so the packed k data load through the scalar path
+// kSkAtoms atoms per thread (strided by the workgroup size): each wave-uniform k
+// record, loaded through the scalar cache, feeds kSkAtoms terms per lane, which
+// hides the load of the next record behind the arithmetic.
+constexpr int kSkAtoms = 4;
 __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f) {
     const int w = blockIdx.y;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a0 = blockIdx.x * blockDim.x * kSkAtoms + threadIdx.x;
     const int n = sk.n_atoms[w];
-    if ((int)(blockIdx.x * blockDim.x) >= n) return;
+    if ((int)(blockIdx.x * blockDim.x * kSkAtoms) >= n) return;
     const double wt = sk.wcv[(int64_t)w * kMaxSlotCV + sk.wslot];
     const float scale = (float)(wt * 2.0 / ((double)n * sk.nk));
-    const int i = sk.atom_begin[w] + (a < n ? a : 0);
-    const bool act = a < n && (g.group == nullptr || g.group[i]);
-    const double4 xv = g.x4[i];
+    int ii[kSkAtoms];
+    double px[kSkAtoms], py[kSkAtoms], pz[kSkAtoms];
+    float gx[kSkAtoms], gy[kSkAtoms], gz[kSkAtoms];
+    #pragma unroll
+    for (int u = 0; u < kSkAtoms; ++u) {
+        const int a = a0 + u * (int)blockDim.x;
+        ii[u] = sk.atom_begin[w] + (a < n ? a : 0);
+        const double4 xv = g.x4[ii[u]];
+        px[u] = xv.x; py[u] = xv.y; pz[u] = xv.z;
+        gx[u] = gy[u] = gz[u] = 0.0f;
+    }
     const SkK* kp = sk.kpack + (int64_t)w * sk.nk;
-    float gx = 0, gy = 0, gz = 0;
-    #pragma unroll 4
+    #pragma unroll 2
     for (int q = 0; q < sk.nk; ++q) {
         const SkK k = kp[q];
-        const double ph = xv.x * k.kr[0] + xv.y * k.kr[1] + xv.z * k.kr[2];
-        float sn, cs;
-        sincos_rev(ph, sn, cs);
-        const float imv = __builtin_fmaf(k.ar, sn, -k.ai * cs);
-        gx = __builtin_fmaf(k.kv[0], imv, gx);
-        gy = __builtin_fmaf(k.kv[1], imv, gy);
-        gz = __builtin_fmaf(k.kv[2], imv, gz);
+        #pragma unroll
+        for (int u = 0; u < kSkAtoms; ++u) {
+            const double ph = px[u] * k.kr[0] + py[u] * k.kr[1] + pz[u] * k.kr[2];
+            float sn, cs;
+            sincos_rev(ph, sn, cs);
+            const float imv = __builtin_fmaf(k.ar, sn, -k.ai * cs);
+            gx[u] = __builtin_fmaf(k.kv[0], imv, gx[u]);
+            gy[u] = __builtin_fmaf(k.kv[1], imv, gy[u]);
+            gz[u] = __builtin_fmaf(k.kv[2], imv, gz[u]);
+        }
     }
-    if (act) {
-        f[3 * (int64_t)i] += (double)(scale * gx);
-        f[3 * (int64_t)i + 1] += (double)(scale * gy);
-        f[3 * (int64_t)i + 2] += (double)(scale * gz);
+    #pragma unroll
+    for (int u = 0; u < kSkAtoms; ++u) {
+        const int a = a0 + u * (int)blockDim.x;
+        const int i = ii[u];
+        if (a < n && (g.group == nullptr || g.group[i])) {
+            f[3 * (int64_t)i] += (double)(scale * gx[u]);
+            f[3 * (int64_t)i + 1] += (double)(scale * gy[u]);
+            f[3 * (int64_t)i + 2] += (double)(scale * gz[u]);
+        }
     }
 }
 
@@ -1928,7 +1947,7 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
 
 hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t st) {
     // grid.y = walkers, grid.x covers the largest walker
-    dim3 grid((s.max_atoms + kBlock - 1) / kBlock, s.n_walkers);
+    dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
     hipLaunchKernelGGL(k_sk_force, grid, dim3(kBlock), 0, st, g, s, f);
     return hipGetLastError();
 }
