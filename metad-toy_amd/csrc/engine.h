@@ -16,6 +16,7 @@ constexpr int kMaxSym = 8;         // symbols usable in DIM expressions
 constexpr int kMaxDim = 3;
 constexpr int kMaxProg = 96;       // expression bytecode length per dimension
 constexpr int kStack = 16;
+constexpr int kRedArgWalkers = 64; // walkers whose tile ranges travel in the reduce kernel's arguments
 
 // Bit set of Steinhardt degrees an environment computes.
 constexpr int kL3 = 1, kL4 = 2, kL6 = 4;
@@ -175,6 +176,14 @@ struct ReduceDev {
     int32_t sym_slot[kMaxSym];
     int32_t D;
     const ExprProg* prog;     // D programs
+    // every program affine: s_d = aff0[d] + sum_c aff[d][c] v_c over value slots c
+    int32_t affine;
+    double aff0[kMaxDim];
+    const double* aff;        // [kMaxDim][kMaxSlotCV]
+    // per-walker tile ranges in the arguments (n_walkers <= kRedArgWalkers), so the
+    // partial sums load without waiting on tile_begin
+    int32_t tile_args;
+    int32_t tile_range[kRedArgWalkers + 1];
     // bias
     int32_t kind;
     int32_t pace;

@@ -421,6 +421,7 @@ struct mdt_fix {
     DBuf<int64_t> nhills, step;
     DBuf<int32_t> tile_begin, mlp_ok;
     DBuf<ExprProg> progs;
+    DBuf<double> aff;              // [kMaxDim][kMaxSlotCV] affine DIM coefficients per value slot
     DBuf<MlpNet> mlp_live, mlp_stage;
     DBuf<float> mlp_m, mlp_v, mlp_gpart;
     DBuf<int64_t> mlp_fits, mlp_fallbacks;
@@ -927,6 +928,16 @@ void fix_build(mdt_fix* fx) {
     HIPCHK(launch_fill(fx->wcv.p, (int64_t)nw * kMaxSlotCV, 0.0, st));
     fx->tile_begin.alloc(nw + 1); fx->tile_begin.upload(ctx->tile_begin.data(), nw + 1, st);
     fx->progs.alloc(progs.size()); fx->progs.upload(progs.data(), progs.size(), st);
+    // affine DIM programs as coefficients per value slot: s_d = lin0_d + sum_c A[d][c] v_c,
+    // A[d][c] = sum of lin_d[q] over the symbols q bound to slot c
+    std::vector<double> h_aff((size_t)kMaxDim * kMaxSlotCV, 0.0);
+    bool all_affine = !progs.empty();
+    for (size_t d = 0; d < progs.size(); ++d) {
+        if (!progs[d].linear) { all_affine = false; continue; }
+        for (size_t q = 0; q < fx->syms.size(); ++q)
+            h_aff[d * kMaxSlotCV + fx->cvs[fx->syms[q].cv]->slot] += progs[d].lin[q];
+    }
+    fx->aff.alloc(h_aff.size()); fx->aff.upload(h_aff.data(), h_aff.size(), st);
     if (!fx->inited) {
         // bias storage (first run only, like init_set_mode)
         const int D = fx->D();
@@ -982,6 +993,11 @@ void fix_build(mdt_fix* fx) {
     for (size_t q = 0; q < fx->syms.size(); ++q) r.sym_slot[q] = fx->cvs[fx->syms[q].cv]->slot;
     r.D = fx->D();
     r.prog = fx->progs.p;
+    r.affine = all_affine ? 1 : 0;
+    for (int d = 0; d < fx->D(); ++d) r.aff0[d] = all_affine ? progs[d].lin0 : 0.0;
+    r.aff = fx->aff.p;
+    r.tile_args = nw <= kRedArgWalkers ? 1 : 0;
+    if (r.tile_args) for (int k = 0; k <= nw; ++k) r.tile_range[k] = ctx->tile_begin[k];
     r.kind = fx->hill_type == 0 ? BIAS_GRID : fx->hill_type == 2 ? BIAS_HILLS : BIAS_MLP;
     r.pace = fx->pace;
     for (int d = 0; d < fx->D(); ++d) {

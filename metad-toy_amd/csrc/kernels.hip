@@ -1364,13 +1364,52 @@ __device__ __noinline__ void expr_eval(const ExprProg& p, int nsym, const double
     for (int q = 0; q < nsym; ++q) dval[q] = st[0].d[q];
 }
 
+// Wave 0 of the reduce kernel writes the walker's state and chain-rule weights:
+// lane c stores CV slot c's value and its weight w_c = sum_d dV/ds_d ds_d/dv_c
+// (ac = lane c's ds_d/dv_c); V, dV/ds and s are taken from lane 0.
+__device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, double cvv, const double* ac,
+                                              const double* sv, double V, const double* dV, double hill,
+                                              int64_t step, double mlp_err, int mlp_ok) {
+    V = readlane_f64(V, 0);
+    double dv[kMaxDim];
+    #pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) dv[d] = readlane_f64(dV[d], 0);
+    double* st = r.state + (int64_t)w * kStStride;
+    if (c < r.ncv) st[kStCv + c] = cvv;
+    if (c < kMaxSlotCV) {
+        double wc = 0.0;
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) wc += dv[d] * ac[d];
+        r.wcv[(int64_t)w * kMaxSlotCV + c] = wc;
+    }
+    if (c == 0) {
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { st[kStS + d] = sv[d]; st[kStDV + d] = dv[d]; }
+        st[kStV] = V;
+        st[kStHill] = hill;
+        st[kStStep] = (double)step;
+        st[kStMlpErr] = mlp_err;
+        st[kStMlpOk] = (double)mlp_ok;
+        r.step[w] = step + 1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_reduce: one workgroup per walker.  Post-force order of
 // FixMetadynamics::post_force (src/core/fix_crystallize.cu:409-450):
 // CVs -> expression -> deposit on step % pace == 0 (WT height from the bias
 // before the deposit) -> V and dV/ds after the deposit -> chain rule.
+//
+// The kernel is one chain of dependent memory round trips, so it is laid out
+// to have few: the partial sums load in the first one (tile ranges come with
+// the arguments), together with the per-walker scalars, the affine expression
+// coefficients and a small grid (to LDS).  After the CV sums, wave 0 carries
+// one CV slot per lane: an affine DIM value is a wave sum and the chain-rule
+// weight of slot c is lane c's own product, so no single-thread chains of
+// LDS reads remain on the usual path.
 // ---------------------------------------------------------------------------
 constexpr int kRedThreads = 1024;
+constexpr int kRedGridLds = 4096;     // grids up to this many bins are read from an LDS copy
 __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     const int w = blockIdx.x;
     __shared__ double sh[kMaxSlotCV * (kRedThreads / 64)];
@@ -1381,30 +1420,48 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     __shared__ double bc[8];
     __shared__ Dual stack[kStack];
     __shared__ double symv[kMaxSym];
-    __shared__ double wcs[kMaxSlotCV];
-    const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
-    // per-walker scalars load while the partial sums do
+    __shared__ double gl[kRedGridLds];
+    constexpr int U = 4;
+    constexpr int SG = 4;
+    constexpr int NW = kRedThreads / 64;
+    const int lane64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t0 = r.tile_args ? r.tile_range[w] : r.tile_begin[w];
+    const int t1 = r.tile_args ? r.tile_range[w + 1] : r.tile_begin[w + 1];
     const int64_t step = r.step[w];
     const int64_t nhill0 = r.nhills[w];
     const double inv_ng = r.inv_ng != nullptr ? r.inv_ng[w] : 0.0;   // NULL without a Steinhardt environment
     const double mlp_err = r.kind == BIAS_MLP ? r.mlp_err[w] : 0.0;
     const int mlp_ok = r.kind == BIAS_MLP ? r.mlp_ok[w] : 0;
-    // expression programs to LDS (read serially by thread 0 below)
-    {
+    // wave 0, lane c: CV slot c's source, direct value and affine coefficients
+    const bool slot_lane = wv == 0 && lane64 < kMaxSlotCV;
+    int src_c = -1;
+    #pragma unroll
+    for (int c = 0; c < kMaxSlotCV; ++c) if (lane64 == c) src_c = r.cv_src[c];
+    const double direct_c = slot_lane && lane64 < r.ncv && src_c == -1 ? r.direct[(int64_t)w * kMaxSlotCV + lane64] : 0.0;
+    double acoef[kMaxDim] = {0, 0, 0};
+    if (r.affine && slot_lane) {
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acoef[d] = r.aff[d * kMaxSlotCV + lane64];
+    }
+    if (!r.affine) {
+        // expression programs to LDS for the interpreter (thread 0 below)
         const int nwords = r.D * (int)(sizeof(ExprProg) / 4);
         const int* src = reinterpret_cast<const int*>(r.prog);
         int* dst = reinterpret_cast<int*>(progs);
         for (int k = threadIdx.x; k < nwords; k += blockDim.x) dst[k] = src[k];
     }
+    const bool grid_in_lds = r.kind == BIAS_GRID && r.grid_size <= kRedGridLds;
+    if (grid_in_lds) {
+        const double* src = r.grid + (int64_t)w * r.grid_size;
+        for (int k = threadIdx.x; k < (int)r.grid_size; k += blockDim.x) gl[k] = src[k];
+    }
     // 1. CV values: one deterministic FP64 tree over tiles per slot (each thread
     //    sums its tiles in a fixed order, then waves, then the wave totals in
     //    order).  Up to four slots at a time have all their loads in flight
     //    together; a 1024-thread block has 128 VGPRs per lane.
-    constexpr int U = 4;
-    constexpr int SG = 4;
-    constexpr int NW = kRedThreads / 64;
-    const int lane64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int c0 = 0; c0 < r.ncv; c0 += SG) {
+    #pragma unroll
+    for (int c0 = 0; c0 < kMaxSlotCV; c0 += SG) {
+        if (c0 >= r.ncv) break;
         double acc[SG];
         #pragma unroll
         for (int g = 0; g < SG; ++g) acc[g] = 0.0;
@@ -1429,70 +1486,69 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         }
     }
     __syncthreads();
-    if (threadIdx.x < 64) {                          // wave 0: one CV slot per lane
-        const int c = threadIdx.x;
+    const bool dep = r.pace > 0 && (step % r.pace) == 0;
+    if (wv == 0) {
+        // wave 0: lane c holds CV slot c
+        const int c = lane64;
+        double cvv = 0.0;
         if (c < r.ncv) {
-            if (r.cv_src[c] >= 0) {
+            if (src_c >= 0) {
                 double a = 0.0;
                 #pragma unroll
                 for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
-                cv[c] = a * inv_ng;
+                cvv = a * inv_ng;
             } else {
                 // direct values; CVs summed after the force pass (cv_src <= -2) are
                 // unbiased and reduced when the state is read
-                cv[c] = r.cv_src[c] == -1 ? r.direct[(int64_t)w * kMaxSlotCV + c] : 0.0;
+                cvv = src_c == -1 ? direct_c : 0.0;
             }
+            cv[c] = cvv;
         }
-        wave_lds_sync();
-        if (threadIdx.x == 0) {
-            // 2. dimensions
-            for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
-            for (int d = 0; d < r.D; ++d) {
-                if (progs[d].linear) {
-                    double v = progs[d].lin0;
-                    for (int q = 0; q < r.nsym; ++q) { v += progs[d].lin[q] * symv[q]; dsdv[d * kMaxSym + q] = progs[d].lin[q]; }
-                    s[d] = v;
-                } else {
-                    expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
-                }
-            }
-        }
-    }
-    const bool dep = r.pace > 0 && (step % r.pace) == 0;
-    // thread 0: the walker's state, the chain-rule weights and the step counter
-    auto finish = [&](double V, const double* dVs) {
-        double* st = r.state + (int64_t)w * kStStride;
-        for (int c = 0; c < r.ncv; ++c) st[kStCv + c] = cv[c];
-        #pragma unroll
-        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { st[kStS + d] = s[d]; st[kStDV + d] = dVs[d]; }
-        st[kStV] = V;
-        st[kStHill] = dep ? bc[0] : 0.0;
-        st[kStStep] = (double)step;
-        st[kStMlpErr] = mlp_err;
-        st[kStMlpOk] = (double)mlp_ok;
-        // 5. chain rule: w_c = sum_{symbols bound to c} sum_d dV/ds_d ds_d/dv, summed
-        // in LDS and stored once
-        for (int c = 0; c < kMaxSlotCV; ++c) wcs[c] = 0.0;
-        for (int q = 0; q < r.nsym; ++q) {
-            double acc = 0.0;
+        // 2. dimensions
+        double sv[kMaxDim] = {0, 0, 0};
+        if (r.affine) {
             #pragma unroll
-            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc += dVs[d] * dsdv[d * kMaxSym + q];
-            wcs[r.sym_slot[q]] += acc;
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) sv[d] = r.aff0[d] + wave_sum(acoef[d] * cvv);
+            if (c == 0) {
+                #pragma unroll
+                for (int d = 0; d < kMaxDim; ++d) if (d < r.D) s[d] = sv[d];
+            }
+        } else {
+            wave_lds_sync();
+            if (c == 0) {
+                for (int q = 0; q < r.nsym; ++q) symv[q] = cv[r.sym_slot[q]];
+                for (int d = 0; d < r.D; ++d) expr_eval(progs[d], r.nsym, symv, &s[d], &dsdv[d * kMaxSym], stack);
+            }
+            wave_lds_sync();
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) {
+                if (d >= r.D) continue;
+                sv[d] = s[d];
+                // d s_d / d v_c: the derivatives of every symbol bound to slot c
+                double a = 0.0;
+                #pragma unroll
+                for (int q = 0; q < kMaxSym; ++q)
+                    if (q < r.nsym && r.sym_slot[q] == c) a += dsdv[d * kMaxSym + q];
+                acoef[d] = a;
+            }
         }
-        double* wc = r.wcv + (int64_t)w * kMaxSlotCV;
-        #pragma unroll
-        for (int c = 0; c < kMaxSlotCV; ++c) wc[c] = wcs[c];
-        r.step[w] = step + 1;
-    };
-    if (r.kind == BIAS_GRID && !dep) {
-        // a step without a deposit on a grid: thread 0 alone evaluates and finishes,
-        // with no further workgroup barrier (uniform over the block)
-        if (threadIdx.x == 0) {
-            double dVg[kMaxDim] = {0, 0, 0};
-            const double V = grid_eval(r, r.grid + (int64_t)w * r.grid_size, s, dVg);
-            finish(V, dVg);
+        if (r.kind == BIAS_GRID && !dep) {
+            // a step without a deposit on a grid: lane 0 evaluates, wave 0 finishes;
+            // no further workgroup barrier (uniform over the block)
+            double V = 0.0, dVg[kMaxDim] = {0, 0, 0};
+            if (c == 0) V = grid_eval(r, grid_in_lds ? gl : r.grid + (int64_t)w * r.grid_size, sv, dVg);
+            reduce_finish(r, w, c, cvv, acoef, sv, V, dVg, 0.0, step, mlp_err, mlp_ok);
+            return;
         }
+    } else if (r.kind == BIAS_GRID && !dep) {
         return;
+    }
+    // keep the slot values and the chain-rule coefficients of wave 0 across the
+    // block-cooperative bias evaluation below
+    __shared__ double acoef_s[kMaxDim][kMaxSlotCV];
+    if (slot_lane) {
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) acoef_s[d][lane64] = acoef[d];
     }
     __syncthreads();
     // 3. deposit
@@ -1517,9 +1573,16 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         }
         __syncthreads();
     }
-    // 4. bias value and gradient after the deposit
+    // 4. bias value and gradient after the deposit (valid on thread 0)
     const double V = bias_eval_block<kRedThreads / 64>(r, w, s, dV, sh);
-    if (threadIdx.x == 0) finish(V, dV);
+    if (wv == 0) {
+        const int c = lane64;
+        const double cvv = c < r.ncv ? cv[c] : 0.0;
+        double ac[kMaxDim] = {0, 0, 0}, sv[kMaxDim] = {0, 0, 0};
+        #pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { ac[d] = c < kMaxSlotCV ? acoef_s[d][c] : 0.0; sv[d] = s[d]; }
+        reduce_finish(r, w, c, cvv, ac, sv, V, dV, dep ? bc[0] : 0.0, step, mlp_err, mlp_ok);
+    }
 }
 
 // Values of the CVs whose partial sums come from the force pass (unbiased local-Q
