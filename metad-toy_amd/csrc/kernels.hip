@@ -1450,10 +1450,18 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         int* dst = reinterpret_cast<int*>(progs);
         for (int k = threadIdx.x; k < nwords; k += blockDim.x) dst[k] = src[k];
     }
+    // a small grid is loaded to registers here and stored to LDS after the partial
+    // sums, so both sets of loads are in flight together
+    constexpr int GU = kRedGridLds / kRedThreads;
     const bool grid_in_lds = r.kind == BIAS_GRID && r.grid_size <= kRedGridLds;
+    double gr[GU];
     if (grid_in_lds) {
         const double* src = r.grid + (int64_t)w * r.grid_size;
-        for (int k = threadIdx.x; k < (int)r.grid_size; k += blockDim.x) gl[k] = src[k];
+        #pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            const int k = threadIdx.x + u * kRedThreads;
+            gr[u] = k < r.grid_size ? src[k] : 0.0;
+        }
     }
     // 1. CV values: one deterministic FP64 tree over tiles per slot (each thread
     //    sums its tiles in a fixed order, then waves, then the wave totals in
@@ -1483,6 +1491,13 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         for (int g = 0; g < SG; ++g) {
             const double v = wave_sum(acc[g]);
             if (lane64 == 0 && c0 + g < r.ncv) sh[(c0 + g) * NW + wv] = v;
+        }
+    }
+    if (grid_in_lds) {
+        #pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            const int k = threadIdx.x + u * kRedThreads;
+            if (k < r.grid_size) gl[k] = gr[u];
         }
     }
     __syncthreads();
