@@ -17,6 +17,7 @@ constexpr int kMaxDim = 3;
 constexpr int kMaxProg = 96;       // expression bytecode length per dimension
 constexpr int kStack = 16;
 constexpr int kRedArgWalkers = 64; // walkers whose tile ranges travel in the reduce kernel's arguments
+constexpr int64_t kMaxGridBins = 1ll << 28;   // bins per walker grid (2 GiB of FP64)
 
 // Bit set of Steinhardt degrees an environment computes.
 constexpr int kL3 = 1, kL4 = 2, kL6 = 4;

@@ -456,6 +456,8 @@ struct mdt_fix {
         if (pin) (void)hipHostFree(pin);
     }
     int D() const { return cv_dim; }
+    // the bias is a grid: the uniform grid (type 0) or the sparse-hash grid held dense (type 1)
+    bool grid_bias() const { return hill_type == 0 || hill_type == 1; }
     int nw() const { return (int)ctx->w.size(); }
 };
 
@@ -704,19 +706,30 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
         fx->nb.push_back((int)it->second.second[2]);
         if (!(fx->hi[d] > fx->lo[d])) throw Err(MDT_EINVAL, fmt("DIM %d: upper bound must exceed lower", d + 1));
     }
-    if (fx->hill_type == 1) throw Err(MDT_ENOTSUP, "Gaussian_Hill_type 1 (sparse hash grid) is not provided; use 0 (grid), 2 (exact hills sum) or 3 (learned bias)");
-    if (fx->hill_type != 0 && fx->hill_type != 2 && fx->hill_type != 3)
+    if (fx->hill_type < 0 || fx->hill_type > 3)
         throw Err(MDT_EINVAL, fmt("Gaussian_Hill_type %d unknown", fx->hill_type));
-    if (fx->hill_type == 0) {
-        if (fx->cv_dim > 2) throw Err(MDT_ENOTSUP, "the uniform grid supports 1 or 2 dimensions; use Gaussian_Hill_type 2 or 3 for 3-D");
-        for (int d = 0; d < fx->cv_dim; ++d)
+    if (fx->grid_bias()) {
+        // type 0: the uniform grid (GH_t0_uniformGrid, 1-D and 2-D); type 1: the
+        // sparse-hash grid (GH_t1_sparseHash, up to 3-D) held as a dense device grid
+        if (fx->hill_type == 0 && fx->cv_dim > 2)
+            throw Err(MDT_ENOTSUP, "the uniform grid supports 1 or 2 dimensions; use Gaussian_Hill_type 1, 2 or 3 for 3-D");
+        double bins = 1.0;
+        for (int d = 0; d < fx->cv_dim; ++d) {
             if (fx->nb[d] < 4) throw Err(MDT_EINVAL, "DIM bins must be >= 4 for Catmull-Rom interpolation");
+            bins *= fx->nb[d];
+        }
+        if (bins > (double)kMaxGridBins) throw Err(MDT_ENOTSUP, fmt("a bias grid of %.0f bins exceeds %lld per walker", bins, (long long)kMaxGridBins));
     }
     if (fx->hill_type == 3 && fx->cv_dim != 3) throw Err(MDT_ENOTSUP, "the learned bias is built for CV_dim 3");
     if (fx->sigmas.empty()) fx->sigmas.assign(fx->cv_dim, fx->sigma);
     if ((int)fx->sigmas.size() != fx->cv_dim)
         throw Err(MDT_EINVAL, fmt("SIGMAS has %zu values but CV_dim is %d", fx->sigmas.size(), fx->cv_dim));
     for (double v : fx->sigmas) if (!(v > 0)) throw Err(MDT_EINVAL, "SIGMAS must be positive");
+    if (fx->grid_bias())
+        for (double v : fx->sigmas)
+            if (v != fx->sigma)
+                throw Err(MDT_ENOTSUP, "SIGMAS (per-dimension widths) applies to Gaussian_Hill_type 2 and 3; "
+                                       "the grids deposit isotropic hills of the GAUSSIAN sigma");
     if (fx->syms.empty()) throw Err(MDT_EINVAL, "fix metad needs at least one SYMBOL");
     if ((int)fx->cvs.size() > kMaxSlotCV) throw Err(MDT_ENOTSUP, fmt("at most %d CALs", kMaxSlotCV));
     // compile the DIM expressions against the symbol table (DimSet.cu:54-66)
@@ -942,7 +955,7 @@ void fix_build(mdt_fix* fx) {
         // bias storage (first run only, like init_set_mode)
         const int D = fx->D();
         int64_t gsz = 1;
-        if (fx->hill_type == 0) for (int d = 0; d < D; ++d) gsz *= fx->nb[d];
+        if (fx->grid_bias()) for (int d = 0; d < D; ++d) gsz *= fx->nb[d];
         else gsz = 1;
         fx->grid.alloc((size_t)nw * gsz);
         HIPCHK(launch_fill(fx->grid.p, (int64_t)nw * gsz, 0.0, st));
@@ -998,7 +1011,7 @@ void fix_build(mdt_fix* fx) {
     r.aff = fx->aff.p;
     r.tile_args = nw <= kRedArgWalkers ? 1 : 0;
     if (r.tile_args) for (int k = 0; k <= nw; ++k) r.tile_range[k] = ctx->tile_begin[k];
-    r.kind = fx->hill_type == 0 ? BIAS_GRID : fx->hill_type == 2 ? BIAS_HILLS : BIAS_MLP;
+    r.kind = fx->grid_bias() ? BIAS_GRID : fx->hill_type == 2 ? BIAS_HILLS : BIAS_MLP;
     r.pace = fx->pace;
     for (int d = 0; d < fx->D(); ++d) {
         r.lo[d] = fx->lo[d]; r.hi[d] = fx->hi[d]; r.nb[d] = fx->nb[d];
@@ -1156,7 +1169,7 @@ void flush_hills(mdt_fix* fx) {
                 const double* rec = &buf[(size_t)k * (D + 3)];
                 fprintf(fp, "%ld", (long)rec[0]);
                 for (int d = 0; d < D; ++d) fprintf(fp, " %.16g", rec[1 + d]);
-                if (fx->hill_type == 0 || D == 1)
+                if (fx->grid_bias() || D == 1)
                     fprintf(fp, " %.16g %.16g\n", rec[1 + D], rec[2 + D]);
                 else {
                     fprintf(fp, " %.16g", rec[1 + D]);
@@ -1987,7 +2000,7 @@ int mdt_fix_bias_add_hills(mdt_fix* fx, int walker, int64_t nh, const double* s,
     use_device(ctx);
     ensure_built(fx);
     ctx->walker(walker);
-    if (fx->hill_type != 0) { ensure_hills(fx, nh); fx->hills_bound[walker] += nh; }
+    if (!fx->grid_bias()) { ensure_hills(fx, nh); fx->hills_bound[walker] += nh; }
     const int D = fx->D();
     DBuf<double> ds, dh;
     ds.alloc((size_t)std::max<int64_t>(nh, 1) * D); dh.alloc(std::max<int64_t>(nh, 1));
@@ -1995,7 +2008,7 @@ int mdt_fix_bias_add_hills(mdt_fix* fx, int walker, int64_t nh, const double* s,
     dh.upload(h, (size_t)nh, ctx->stream);
     HIPCHK(launch_bias_add(fx->red, walker, nh, ds.p, dh.p, sigma, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (fx->hill_type != 0) {
+    if (!fx->grid_bias()) {
         int64_t n = 0;
         HIPCHK(hipMemcpy(&n, fx->nhills.p + walker, sizeof(int64_t), hipMemcpyDeviceToHost));
         fx->hills_written[walker] = n;   // externally supplied hills are not echoed to HILLS
@@ -2006,7 +2019,7 @@ int mdt_fix_bias_add_hills(mdt_fix* fx, int walker, int64_t nh, const double* s,
 int mdt_fix_grid(mdt_fix* fx, int walker, double* out, int64_t n) {
     FIX_BEGIN(fx)
     ensure_built(fx);
-    if (fx->hill_type != 0) throw Err(MDT_EINVAL, "this bias has no grid");
+    if (!fx->grid_bias()) throw Err(MDT_EINVAL, "this bias has no grid");
     if (n != fx->red.grid_size) throw Err(MDT_EINVAL, fmt("grid has %lld bins", (long long)fx->red.grid_size));
     HIPCHK(hipStreamSynchronize(fx->ctx->stream));
     HIPCHK(hipMemcpy(out, fx->grid.p + (size_t)walker * fx->red.grid_size, sizeof(double) * n, hipMemcpyDeviceToHost));

@@ -1142,9 +1142,47 @@ __device__ __forceinline__ void center_frac_clamp(double cv, double lo, double h
     if (i > nb - 3) { i = nb - 3; f = 1.0; }
 }
 
+// 3-D grid (the sparse-hash bias of GH_t1_sparseHash.cu:283-407 as a dense grid):
+// tensor Catmull-Rom taken from the innermost dimension outwards, as the
+// reference's recursion does, each derivative divided by its bin width where it
+// is taken; NaN -> 0 per component.
+__device__ double grid_eval3(const ReduceDev& r, const double* grid, const double* s, double* dV) {
+#pragma clang fp contract(off)
+    double h[3], f[3];
+    int b[3];
+    #pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        h[d] = (r.hi[d] - r.lo[d]) / r.nb[d];
+        center_frac_clamp(s[d], r.lo[d], h[d], r.nb[d], b[d], f[d]);
+    }
+    const int64_t n1 = r.nb[1], n2 = r.nb[2];
+    // along dimension 2 for each of the 4x4 (dim 0, dim 1) rows: value and derivative
+    double v01[4], d1_0[4], d2_0[4];   // per dim-0 index: CR over dim 1 of (value, value, deriv/h2)
+    #pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double cv2[4], cd2[4];
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double* p = grid + ((int64_t)(b[0] - 1 + a) * n1 + (b[1] - 1 + c)) * n2 + (b[2] - 1);
+            cv2[c] = cr_value(p[0], p[1], p[2], p[3], f[2]);
+            cd2[c] = cr_deriv(p[0], p[1], p[2], p[3], f[2]) / h[2];
+        }
+        v01[a] = cr_value(cv2[0], cv2[1], cv2[2], cv2[3], f[1]);
+        d1_0[a] = cr_deriv(cv2[0], cv2[1], cv2[2], cv2[3], f[1]) / h[1];
+        d2_0[a] = cr_value(cd2[0], cd2[1], cd2[2], cd2[3], f[1]);
+    }
+    dV[0] = cr_deriv(v01[0], v01[1], v01[2], v01[3], f[0]) / h[0];
+    dV[1] = cr_value(d1_0[0], d1_0[1], d1_0[2], d1_0[3], f[0]);
+    dV[2] = cr_value(d2_0[0], d2_0[1], d2_0[2], d2_0[3], f[0]);
+    #pragma unroll
+    for (int d = 0; d < 3; ++d) if (isnan(dV[d])) dV[d] = 0.0;
+    return cr_value(v01[0], v01[1], v01[2], v01[3], f[0]);
+}
+
 // grid value and gradient at s (one thread)
 __device__ double grid_eval(const ReduceDev& r, const double* grid, const double* s, double* dV) {
 #pragma clang fp contract(off)
+    if (r.D == 3) return grid_eval3(r, grid, s, dV);
     if (r.D == 1) {
         double h = (r.hi[0] - r.lo[0]) / r.nb[0];
         int i; double x;
@@ -1177,7 +1215,7 @@ __device__ double grid_eval(const ReduceDev& r, const double* grid, const double
 // Deposit one hill into a grid with the block's threads (add_to_grid).
 __device__ void grid_deposit_block(const ReduceDev& r, double* grid, const double* s, double w, double sig) {
 #pragma clang fp contract(off)
-    int lower[2] = {0, 0}, upper[2] = {0, 0};
+    int lower[kMaxDim] = {0, 0, 0}, upper[kMaxDim] = {0, 0, 0};
     for (int k = 0; k < r.D; ++k) {
         double dcv = (r.hi[k] - r.lo[k]) / r.nb[k];
         int R = (int)ceil(4.0 * sig / dcv) + 1;
@@ -1190,6 +1228,21 @@ __device__ void grid_deposit_block(const ReduceDev& r, double* grid, const doubl
             double ctr = r.lo[0] + (g + 0.5) * (r.hi[0] - r.lo[0]) / r.nb[0];
             double dx = s[0] - ctr;
             grid[g] += w * exp(-0.5 * (dx * dx) / (sig * sig));
+        }
+    } else if (r.D == 3) {
+        // recursive_add2grid / gauss_calc<3> (GH_t1_sparseHash.cu:242-262,425-431):
+        // bin centre lo + (i + 1/2) dcv, exp(-0.5 sum_d dx_d^2 / sigma^2)
+        const int n0 = upper[0] - lower[0] + 1, n1 = upper[1] - lower[1] + 1, n2 = upper[2] - lower[2] + 1;
+        const double h0 = (r.hi[0] - r.lo[0]) / r.nb[0], h1 = (r.hi[1] - r.lo[1]) / r.nb[1];
+        const double h2 = (r.hi[2] - r.lo[2]) / r.nb[2];
+        for (int q = threadIdx.x; q < n0 * n1 * n2; q += blockDim.x) {
+            if (n0 <= 0 || n1 <= 0 || n2 <= 0) break;
+            const int i = lower[0] + q / (n1 * n2), j = lower[1] + (q / n2) % n1, k = lower[2] + q % n2;
+            const double d0 = s[0] - (r.lo[0] + (i + 0.5) * h0);
+            const double d1 = s[1] - (r.lo[1] + (j + 0.5) * h1);
+            const double d2 = s[2] - (r.lo[2] + (k + 0.5) * h2);
+            const double paw = d0 * d0 + d1 * d1 + d2 * d2;
+            grid[((int64_t)i * r.nb[1] + j) * r.nb[2] + k] += w * exp(-0.5 * (paw) / (sig * sig));
         }
     } else {
         const int n0 = upper[0] - lower[0] + 1, n1 = upper[1] - lower[1] + 1;

@@ -512,6 +512,81 @@ double orc_grid_eval(int D, const double* lo, const double* hi, const int* nb, c
     return cr_value(col[0], col[1], col[2], col[3], dy);
 }
 
+// ---------------------------------------------------------------------------
+// Sparse-hash bias (Gaussian_Hill_type 1), restated from GH_t1_sparseHash.cu as a
+// dense grid of D <= 3 dimensions (a bin the hash never touched reads 0, as a
+// missing key does in get_total_bias :385-393).  Deposit: add_to_grid :228-240
+// (window as the uniform grid), recursive_add2grid :242-262 (bin centre
+// lo + (i + 1/2) dcv), gauss_calc :413-431.  Evaluation: get_bias_energy
+// :395-407 and get_dVdcv :283-314 through mixed_recursive_logic :316-366, a
+// recursion from dimension 0 to the innermost dimension: Catmull-Rom values
+// along every dimension except the derivative one, whose Catmull-Rom
+// derivative is divided by that dimension's bin width where it is taken.
+// ---------------------------------------------------------------------------
+static long long gt1_index(int D, const int* nb, const int* c) {
+    long long k = 0;
+    for (int d = 0; d < D; ++d) k = k * nb[d] + c[d];
+    return k;
+}
+
+void orc_gridt1_deposit(int D, const double* lo, const double* hi, const int* nb, double* grid,
+                        const double* s, double w, double sig) {
+    int lower[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
+    double dcv[3] = {1, 1, 1};
+    for (int k = 0; k < D; ++k) {
+        dcv[k] = (hi[k] - lo[k]) / nb[k];
+        int R = static_cast<int>(std::ceil(4.0 * sig / dcv[k])) + 1;
+        int c = static_cast<int>((s[k] - lo[k]) / dcv[k]);
+        lower[k] = std::max(0, c - R);
+        upper[k] = std::min(nb[k] - 1, c + R);
+    }
+    int c[3] = {0, 0, 0};
+    for (c[0] = lower[0]; c[0] <= upper[0]; ++c[0])
+        for (c[1] = lower[1]; c[1] <= (D > 1 ? upper[1] : 0); ++c[1])
+            for (c[2] = lower[2]; c[2] <= (D > 2 ? upper[2] : 0); ++c[2]) {
+                double paw = 0.0;
+                for (int d = 0; d < D; ++d) {
+                    double dx = s[d] - (lo[d] + (c[d] + 0.5) * dcv[d]);
+                    paw += dx * dx;
+                }
+                grid[gt1_index(D, nb, c)] += w * std::exp(-0.5 * (paw) / (sig * sig));
+            }
+}
+
+static double gt1_rec(int D, const int* nb, const double* dcv, const double* grid, int cur, int target,
+                      int* coord, const double* frac) {
+    if (cur == D) return grid[gt1_index(D, nb, coord)];
+    double p[4];
+    const int origin = coord[cur];
+    for (int i = -1; i <= 2; ++i) {
+        int t = origin + i;
+        if (t < 0) t = 0;
+        if (t >= nb[cur]) t = nb[cur] - 1;
+        coord[cur] = t;
+        p[i + 1] = gt1_rec(D, nb, dcv, grid, cur + 1, target, coord, frac);
+    }
+    coord[cur] = origin;
+    if (cur == target) return cr_deriv(p[0], p[1], p[2], p[3], frac[cur]) / dcv[cur];
+    return cr_value(p[0], p[1], p[2], p[3], frac[cur]);
+}
+
+double orc_gridt1_eval(int D, const double* lo, const double* hi, const int* nb, const double* grid,
+                       const double* s, double* dVds) {
+    int base[3];
+    double frac[3], dcv[3];
+    for (int d = 0; d < D; ++d) {
+        dcv[d] = (hi[d] - lo[d]) / nb[d];
+        center_frac_clamp(s[d], lo[d], dcv[d], nb[d], base[d], frac[d]);
+    }
+    if (dVds) {
+        for (int k = 0; k < D; ++k) {
+            dVds[k] = gt1_rec(D, nb, dcv, grid, 0, k, base, frac);
+            if (std::isnan(dVds[k])) dVds[k] = 0.0;
+        }
+    }
+    return gt1_rec(D, nb, dcv, grid, 0, -1, base, frac);
+}
+
 // Well-tempered height (add_hill :194-208): w = h0 exp(-V(s)/(T kB (gamma-1))), V = 0 if !WT.
 double orc_wt_height(int D, const double* lo, const double* hi, const int* nb, const double* grid,
                      const double* s, double h0, double biasf, double T, double kB, int wt) {

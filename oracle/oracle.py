@@ -45,6 +45,8 @@ def lib():
             ("orc_grid_deposit", None, [I, P, P, P, P, P, D, D]),
             ("orc_grid_eval", D, [I, P, P, P, P, P, P]),
             ("orc_wt_height", D, [I, P, P, P, P, P, D, D, D, D, I]),
+            ("orc_gridt1_deposit", None, [I, P, P, P, P, P, D, D]),
+            ("orc_gridt1_eval", D, [I, P, P, P, P, P, P]),
             ("orc_hills_sum", None, [I, I, P, P, P, I, P, P, P]),
             ("orc_mlp_forward", None, [I, I, P, P, P, P, P, D, I, P, P, P]),
             ("orc_mlp_loss_grad", D, [I, I, P, P, P, P, P, D, I, P, P, P, P, P, P, P, P]),
@@ -221,6 +223,25 @@ class Grid:
                                    h0, biasf, T, kB, int(wt))
 
 
+class GridT1(Grid):
+    """Sparse-hash bias (Gaussian_Hill_type 1) as a dense grid of D <= 3 dimensions,
+    restated from GH_t1_sparseHash.cu (see orc_gridt1_* in oracle.cpp)."""
+
+    def deposit(self, s, w, sigma):
+        s = _f64(s)
+        lib().orc_gridt1_deposit(self.D, _p(self.lo), _p(self.hi), _p(self.nb), _p(self.g), _p(s), w, sigma)
+
+    def eval(self, s):
+        s = _f64(s)
+        dV = np.zeros(self.D)
+        V = lib().orc_gridt1_eval(self.D, _p(self.lo), _p(self.hi), _p(self.nb), _p(self.g), _p(s), _p(dV))
+        return V, dV
+
+    def wt_height(self, s, h0, biasf, T, kB, wt):
+        V = self.eval(s)[0] if wt else 0.0
+        return h0 * np.exp(-V / (T * kB * (biasf - 1.0)))
+
+
 def hills_sum(centers, heights, sigma, s):
     c = _f64(centers)
     D = c.shape[1]
@@ -238,7 +259,7 @@ def mlp_forward(params, s):
     H, Din = W1.shape
     V = np.zeros(s.shape[0])
     dV = np.zeros_like(s)
-    lib().orc_mlp_forward(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(b3), s.shape[0], _p(s),
+    lib().orc_mlp_forward(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(np.asarray(b3).reshape(-1)[0]), s.shape[0], _p(s),
                           _p(V), _p(dV))
     return V, dV
 
@@ -249,6 +270,6 @@ def mlp_loss_grad(params, s, y):
     H, Din = W1.shape
     g = [np.zeros_like(W1), np.zeros_like(b1), np.zeros_like(W2), np.zeros_like(b2), np.zeros_like(W3)]
     gb3 = np.zeros(1)
-    loss = lib().orc_mlp_loss_grad(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(b3), s.shape[0],
+    loss = lib().orc_mlp_loss_grad(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(np.asarray(b3).reshape(-1)[0]), s.shape[0],
                                    _p(s), _p(y), _p(g[0]), _p(g[1]), _p(g[2]), _p(g[3]), _p(g[4]), _p(gb3))
     return loss, g + [gb3[0]]

@@ -120,12 +120,30 @@ def test_fix_semantic_errors(mdt):
         ("CV_dim 2 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1", "DIM 2 is missing"),
         ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 2 0 1 100 v1", "out of range"),
         ("CV_dim 3 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 DIM 2 0 1 100 v1 DIM 3 0 1 100 v1", "grid supports"),
-        ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 Gaussian_Hill_type 1", "sparse hash"),
+        ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 3 v1 Gaussian_Hill_type 1", "bins must be >= 4"),
+        ("CV_dim 3 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 DIM 2 0 1 1000 v1 DIM 3 0 1 1000 v1 Gaussian_Hill_type 1",
+         "exceeds"),
+        ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 Gaussian_Hill_type 5", "unknown"),
+        ("CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1 SIGMAS 0.02", "isotropic"),
         ("CAL NAME Q6 STEINH Q 4 all CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 100 v1", "duplicated"),
     ]:
         with pytest.raises(mdt.MdtError) as e:
             mdt.Fix(ctx, base + tail)
         assert frag in str(e.value), (tail, str(e.value))
+
+
+def test_sparse_hash_grid_parses_in_3d(mdt):
+    """Gaussian_Hill_type 1 (the reference's sparse-hash grid, GH_t1_sparseHash.cu) is
+    accepted up to three dimensions; the uniform grid (type 0) stays at two."""
+    ctx = mdt.Context(-1, 1)
+    tail = ("CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 CAL NAME S SK all FCC111 5.26 "
+            "CV_dim 3 SYMBOL v1 Q4.AVE SYMBOL v2 Q6.AVE SYMBOL v3 S.AVE "
+            "DIM 1 0 1 100 v1 DIM 2 0 1 100 v2 DIM 3 0 100 100 v3 ")
+    mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 " + tail + "Gaussian_Hill_type 1").close()
+    with pytest.raises(mdt.MdtError) as e:
+        mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 " + tail + "Gaussian_Hill_type 0")
+    assert "grid supports" in str(e.value)
 
 
 def test_learned_bias_options(mdt):

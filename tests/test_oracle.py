@@ -303,6 +303,87 @@ def test_wt_height(orc):
     assert g.wt_height(s, 0.005, 10.0, 300.0, 8.617343e-5, 0) == 0.005
 
 
+def test_sparse_hash_grid_3d_single_hill_closed_form(orc):
+    """Sparse-hash deposit (GH_t1_sparseHash.cu:228-262): window of R = ceil(4 sigma/dx) + 1
+    bins around c = int((s - lo)/dx) per dimension, exp(-sum dx^2 / (2 sigma^2)) inside, 0 outside."""
+    lo, hi, nb = np.array([0.0, 0.0, 10.0]), np.array([1.0, 0.5, 30.0]), np.array([50, 40, 30])
+    g = orc.GridT1(lo, hi, nb)
+    s, w, sig = np.array([0.4321, 0.2101, 18.77]), 0.005, 0.04
+    g.deposit(s, w, sig)
+    dx = (hi - lo) / nb
+    ctr = [lo[d] + (np.arange(nb[d]) + 0.5) * dx[d] for d in range(3)]
+    C = np.meshgrid(*ctr, indexing="ij")
+    full = w * np.exp(-0.5 * sum((s[d] - C[d]) ** 2 for d in range(3)) / sig ** 2)
+    win = np.ones(tuple(nb), bool)
+    for d in range(3):
+        R = int(np.ceil(4 * sig / dx[d])) + 1
+        c = int((s[d] - lo[d]) / dx[d])
+        m = np.zeros(nb[d], bool)
+        m[max(0, c - R):min(nb[d] - 1, c + R) + 1] = True
+        win &= m.reshape([-1 if k == d else 1 for k in range(3)])
+    G = g.g.reshape(tuple(nb))
+    assert np.abs(G[win] - full[win]).max() < 1e-15
+    assert np.all(G[~win] == 0)
+
+
+def test_sparse_hash_grid_catmull_rom_exact_on_triquadratics(orc):
+    """The tensor Catmull-Rom of the recursion (GH_t1_sparseHash.cu:316-366) reproduces
+    polynomials of degree <= 2 in each variable: V and every dV/ds_d exactly."""
+    lo, hi, nb = [0.0, -1.0, 2.0], [2.0, 1.0, 5.0], [30, 24, 20]
+    g = orc.GridT1(lo, hi, nb)
+    ctr = [lo[d] + (np.arange(nb[d]) + 0.5) * (hi[d] - lo[d]) / nb[d] for d in range(3)]
+    X, Y, Z = np.meshgrid(*ctr, indexing="ij")
+
+    def P(x, y, z):
+        return 0.3 + 1.7 * x - 0.9 * x * x + 0.4 * y * z - 0.2 * z * z + 0.5 * x * y * z + 0.1 * x * x * y * y * z * z
+
+    def dP(x, y, z):
+        return np.array([1.7 - 1.8 * x + 0.5 * y * z + 0.2 * x * y * y * z * z,
+                         0.4 * z + 0.5 * x * z + 0.2 * x * x * y * z * z,
+                         0.4 * y - 0.4 * z + 0.5 * x * y + 0.2 * x * x * y * y * z])
+    g.g[:] = P(X, Y, Z).ravel()
+    rng = np.random.default_rng(11)
+    for _ in range(30):
+        sp = np.array([rng.uniform(0.2, 1.8), rng.uniform(-0.8, 0.8), rng.uniform(2.3, 4.6)])
+        V, dV = g.eval(sp)
+        assert abs(V - P(*sp)) < 1e-11
+        assert np.abs(dV - dP(*sp)).max() < 1e-9
+
+
+def test_sparse_hash_grid_agrees_with_uniform_grid(orc):
+    """In one and two dimensions the sparse-hash grid and the uniform grid deposit the same
+    Gaussians at the same bins and interpolate the same spline (the evaluation order differs,
+    so they agree to rounding)."""
+    rng = np.random.default_rng(3)
+    for D in (1, 2):
+        lo, hi, nb = [0.0] * D, [1.0] * D, [120] * D
+        a, b = orc.Grid(lo, hi, nb), orc.GridT1(lo, hi, nb)
+        for _ in range(40):
+            s = rng.uniform(0.1, 0.9, size=D)
+            a.deposit(s, 0.005, 0.03)
+            b.deposit(s, 0.005, 0.03)
+        assert np.abs(a.g - b.g).max() <= 1e-15 * max(1.0, np.abs(a.g).max()) * 40
+        for _ in range(20):
+            s = rng.uniform(0.0, 1.0, size=D)
+            Va, dVa = a.eval(s)
+            Vb, dVb = b.eval(s)
+            assert abs(Va - Vb) <= 1e-12 * max(1e-300, abs(Va)) + 1e-18
+            assert np.abs(dVa - dVb).max() <= 1e-10 * max(1e-300, np.abs(dVa).max()) + 1e-16
+
+
+def test_sparse_hash_grid_clamp_and_constant(orc):
+    g = orc.GridT1([0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [10, 12, 14])
+    g.g[:] = 0.7
+    for s in ([-1.0, 0.5, 0.5], [0.5, 2.0, 0.5], [0.0, 0.0, 1.0], [0.3, 0.6, 0.9]):
+        V, dV = g.eval(s)
+        assert abs(V - 0.7) < 1e-15 and np.abs(dV).max() < 1e-12
+    g.g[:] = np.arange(10 * 12 * 14, dtype=float)
+    # every coordinate below lo + 1.5 dx clamps to bin 1 with f = 0
+    assert g.eval([0.0, 0.0, 0.0])[0] == (1 * 12 + 1) * 14 + 1
+    # above hi - 2.5 dx to nb - 3 with f = 1: the value of bin nb - 2
+    assert g.eval([1.0, 1.0, 1.0])[0] == (8 * 12 + 10) * 14 + 12
+
+
 def test_reference_hills_fixture(orc):
     """The reference's own HILLS file (238 1-D hills, h 0.005, sigma 0.01) on the config-1 grid."""
     import os
