@@ -217,6 +217,25 @@ int mdt_mlp_forward(mdt_ctx* ctx, const float* params, const double* norm, int64
 int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s,
                       const double* v, float* grad);
 
+/* FES reconstruction from a HILLS list on a context's device.  Replaces the
+ * hill sum of tools/plot_hills.py:158-223 (the reference's post-processing
+ * script, numpy on the host).  hills: nh rows of (step, s_1..s_D, height,
+ * sigma), the HILLS line layout (src/grids/GH_t0_uniformGrid.cu:210-220).
+ * D = 1: out[n0] = sum over every hill of h exp(-(x - c)^2 / (2 sigma^2)) at
+ * the points x[n0] (:158-172).  D = 2: out[n0 * n1] (row-major, x fastest
+ * varying along the second index) sums each hill over the bins of its window
+ * int((c -+ 4 sigma - lo) / d), clamped to the grid (:174-223); lo[2] and d[2]
+ * are the grid's lower bounds and bin widths.  Hills are summed in list order
+ * in FP64, as the script's numpy reduction does. */
+int mdt_fes_sum(mdt_ctx* ctx, int D, int64_t nh, const double* hills, const double* x, int n0,
+                const double* y, int n1, const double* lo, const double* d, double* out);
+/* Marginal free energy of a 2-D FES F[n0 * n1] (tools/plot_hills.py:236-262):
+ * out = -kT ln(sum_k exp(-F/kT) * bin) along axis (1: over the second CV,
+ * giving n0 values F(CV1); 0: over the first, n1 values F(CV2)), shifted so
+ * that its minimum is 0. */
+int mdt_fes_marginal(mdt_ctx* ctx, const double* F, int n0, int n1, int axis, double kT, double bin,
+                     double* out);
+
 /* ------------------------------------------------------------------------
  * Synthetic systems (host; the standalone driver's stand-in for LAMMPS
  * create_atoms + neighbor build)

@@ -240,6 +240,10 @@ hipError_t launch_virial(const double4* x4, const double* f, const int32_t* imag
 hipError_t launch_fill_f(float* p, int64_t n, float v, hipStream_t st);
 hipError_t launch_bias_eval(const ReduceDev& r, int walker, int64_t np, const double* s,
                             double* V, double* dV, hipStream_t st);
+hipError_t launch_fes_sum(int D, int64_t nh, const double* hills, const double* x, int n0, const double* y,
+                          int n1, double lo0, double lo1, double d0, double d1, double* out, hipStream_t st);
+hipError_t launch_fes_marginal(const double* F, int n0, int n1, int axis, double kT, double bin, double* out,
+                               hipStream_t st);
 hipError_t launch_bias_add(const ReduceDev& r, int walker, int64_t nh, const double* s,
                            const double* h, double sigma, hipStream_t st);
 

@@ -1908,6 +1908,45 @@ static void upload_points(mdt_ctx* ctx, int64_t np, const double* s, DBuf<float4
     ds.upload(h.data(), h.size(), ctx->stream);
 }
 
+int mdt_fes_sum(mdt_ctx* ctx, int D, int64_t nh, const double* hills, const double* x, int n0,
+                const double* y, int n1, const double* lo, const double* d, double* out) {
+    API_BEGIN(ctx)
+    if (!ctx) return MDT_EINVAL;
+    if (D != 1 && D != 2) throw Err(MDT_ENOTSUP, "FES reconstruction covers 1 and 2 dimensions, as the reference script does");
+    if (nh < 0 || n0 < 1 || (D == 2 && n1 < 1)) throw Err(MDT_EINVAL, "empty FES grid or negative hill count");
+    if ((nh > 0 && !hills) || !x || (D == 2 && (!y || !lo || !d)) || !out) throw Err(MDT_EINVAL, "null array");
+    use_device(ctx);
+    const int64_t nout = D == 2 ? (int64_t)n0 * n1 : n0;
+    DBuf<double> dh, dx, dy, dout;
+    dh.alloc((size_t)std::max<int64_t>(nh, 1) * (D + 3));
+    dh.upload(hills, (size_t)nh * (D + 3), ctx->stream);
+    dx.alloc(n0); dx.upload(x, n0, ctx->stream);
+    dy.alloc(D == 2 ? n1 : 1);
+    if (D == 2) dy.upload(y, n1, ctx->stream);
+    dout.alloc((size_t)nout);
+    HIPCHK(launch_fes_sum(D, nh, dh.p, dx.p, n0, dy.p, D == 2 ? n1 : 1, D == 2 ? lo[0] : 0.0, D == 2 ? lo[1] : 0.0,
+                          D == 2 ? d[0] : 1.0, D == 2 ? d[1] : 1.0, dout.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_fes_marginal(mdt_ctx* ctx, const double* F, int n0, int n1, int axis, double kT, double bin, double* out) {
+    API_BEGIN(ctx)
+    if (!ctx) return MDT_EINVAL;
+    if (n0 < 1 || n1 < 1 || (axis != 0 && axis != 1) || !F || !out) throw Err(MDT_EINVAL, "bad marginal arguments");
+    if (!(kT > 0) || !(bin > 0)) throw Err(MDT_EINVAL, "kT and the bin width must be positive");
+    use_device(ctx);
+    DBuf<double> dF, dout;
+    dF.alloc((size_t)n0 * n1); dF.upload(F, (size_t)n0 * n1, ctx->stream);
+    const int nout = axis == 1 ? n0 : n1;
+    dout.alloc(nout);
+    HIPCHK(launch_fes_marginal(dF.p, n0, n1, axis, kT, bin, dout.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost));
+    API_END
+}
+
 int mdt_mlp_forward(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s, double* V) {
     API_BEGIN(ctx)
     use_device(ctx);

@@ -1716,6 +1716,85 @@ __global__ void k_bias_add(ReduceDev r, int w, int64_t nh, const double* s, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// FES reconstruction from a hills list (tools/plot_hills.py:158-235): one thread
+// per grid point sums the hills in list order (the order of the tool's numpy
+// reduction over hills).  1-D: every hill at every point, exp(-(dx^2) / (2 s^2));
+// 2-D: a hill covers the bins of its window int((c -+ 4 s - lo) / d), truncated
+// toward zero and clamped to the grid, exp(-(dx^2 + dy^2) / (2 s^2)).  hills rows:
+// step, s..., height, sigma.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int fes_trunc(double v) {
+    return (int)fmin(fmax(v, -2.0e9), 2.0e9);      // int() of the tool, range-limited
+}
+__global__ __launch_bounds__(kBlock) void k_fes_sum(int D, int64_t nh, const double* __restrict__ hills,
+                                                    const double* __restrict__ x, int n0,
+                                                    const double* __restrict__ y, int n1, double lo0,
+                                                    double lo1, double d0, double d1, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nout = D == 2 ? (int64_t)n0 * n1 : n0;
+    if (b >= nout) return;
+    const int stride = D + 3;
+    double F = 0.0;
+    if (D == 1) {
+        const double X = x[b];
+        for (int64_t k = 0; k < nh; ++k) {
+            const double* r = hills + k * stride;
+            const double t = X - r[1];
+            F += r[2] * exp(-(t * t) / (2.0 * (r[3] * r[3])));
+        }
+    } else {
+        const int i = (int)(b / n1), j = (int)(b % n1);
+        const double X = x[i], Y = y[j];
+        for (int64_t k = 0; k < nh; ++k) {
+            const double* r = hills + k * stride;
+            const double c1 = r[1], c2 = r[2], h = r[3], sg = r[4];
+            const double cut = 4.0 * sg;
+            const int rmin = max(0, fes_trunc((c1 - cut - lo0) / d0));
+            const int rmax = min(n0 - 1, fes_trunc((c1 + cut - lo0) / d0));
+            const int cmin = max(0, fes_trunc((c2 - cut - lo1) / d1));
+            const int cmax = min(n1 - 1, fes_trunc((c2 + cut - lo1) / d1));
+            if (i < rmin || i > rmax || j < cmin || j > cmax) continue;
+            const double a = X - c1, e = Y - c2;
+            F += h * exp(-(a * a + e * e) / (2.0 * (sg * sg)));
+        }
+    }
+    out[b] = F;
+}
+
+// Marginal of a 2-D FES (tools/plot_hills.py:236-262): -kT ln(sum exp(-F/kT) * bin)
+// along one axis (axis 1 sums over the second CV), one thread per output entry.
+__global__ __launch_bounds__(kBlock) void k_fes_marginal(const double* __restrict__ F, int n0, int n1, int axis,
+                                                         double kT, double bin, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nout = axis == 1 ? n0 : n1, nsum = axis == 1 ? n1 : n0;
+    if (m >= nout) return;
+    double acc = 0.0;
+    for (int k = 0; k < nsum; ++k) {
+        const double f = axis == 1 ? F[(int64_t)m * n1 + k] : F[(int64_t)k * n1 + m];
+        acc += exp(-f / kT);
+    }
+    out[m] = -kT * log(acc * bin);
+}
+
+// v -= min(v) over n values, one workgroup (the minimum is exact, so the result
+// does not depend on the reduction order)
+__global__ __launch_bounds__(kBlock) void k_sub_min(double* v, int n) {
+    __shared__ double sm[kBlock];
+    double m = __builtin_inf();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmin(m, v[k]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double mn = sm[0];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) v[k] -= mn;
+}
+
 // Virial partials of a force buffer: sum_i F_i (x) unwrap(x_i), components
 // xx yy zz xy xz yz (the tally of FixMetadynamics::post_force).
 __global__ __launch_bounds__(kBlock) void k_virial(const double4* __restrict__ x4, const double* __restrict__ f,
@@ -2080,6 +2159,25 @@ hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t 
     // grid.y = walkers, grid.x covers the largest walker
     dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
     hipLaunchKernelGGL(k_sk_force, grid, dim3(kBlock), 0, st, g, s, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_fes_sum(int D, int64_t nh, const double* hills, const double* x, int n0, const double* y,
+                          int n1, double lo0, double lo1, double d0, double d1, double* out, hipStream_t st) {
+    const int64_t nout = D == 2 ? (int64_t)n0 * n1 : n0;
+    if (nout <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fes_sum, dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, D, nh, hills,
+                       x, n0, y, n1, lo0, lo1, d0, d1, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fes_marginal(const double* F, int n0, int n1, int axis, double kT, double bin, double* out,
+                               hipStream_t st) {
+    const int nout = axis == 1 ? n0 : n1;
+    if (nout <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fes_marginal, dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, F, n0, n1,
+                       axis, kT, bin, out);
+    hipLaunchKernelGGL(k_sub_min, dim3(1), dim3(kBlock), 0, st, out, nout);
     return hipGetLastError();
 }
 

@@ -76,6 +76,8 @@ def lib():
         "mdt_fix_hills": (I, [P, I, C.POINTER(I64), dp, I64]),
         "mdt_mlp_forward": (I, [P, C.POINTER(C.c_float), dp, I64, dp, dp]),
         "mdt_mlp_loss_grad": (I, [P, C.POINTER(C.c_float), dp, I64, dp, dp, C.POINTER(C.c_float)]),
+        "mdt_fes_sum": (I, [P, I, I64, dp, dp, I, dp, I, dp, dp, dp]),
+        "mdt_fes_marginal": (I, [P, dp, I, I, I, D, D, dp]),
         "mdt_system_create": (I, [C.POINTER(P)]),
         "mdt_system_destroy": (None, [P]),
         "mdt_system_error": (C.c_char_p, [P]),
@@ -223,6 +225,35 @@ class Context:
 
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def fes_sum(ctx, hills, x, y=None, lo=None, d=None):
+    """Sum of a HILLS list's Gaussians on a grid (mdt_fes_sum): hills rows (step, s..., h, sigma);
+    x (and y, lo[2], d[2] for two dimensions) as tools/plot_hills.py builds them.  Returns F of
+    shape (len(x),) or (len(x), len(y))."""
+    hills = np.ascontiguousarray(np.atleast_2d(hills), np.float64)
+    D = hills.shape[1] - 3
+    x = np.ascontiguousarray(x, np.float64)
+    if D == 2:
+        y = np.ascontiguousarray(y, np.float64)
+        lo, d = np.ascontiguousarray(lo, np.float64), np.ascontiguousarray(d, np.float64)
+        out = np.zeros((x.size, y.size))
+        ctx._chk(lib().mdt_fes_sum(ctx._h, 2, hills.shape[0], _dp(hills), _dp(x), x.size, _dp(y), y.size,
+                                   _dp(lo), _dp(d), _dp(out)))
+    else:
+        out = np.zeros(x.size)
+        ctx._chk(lib().mdt_fes_sum(ctx._h, D, hills.shape[0], _dp(hills), _dp(x), x.size, None, 0, None, None,
+                                   _dp(out)))
+    return out
+
+
+def fes_marginal(ctx, F, axis, kT, bin_size):
+    """Marginal FES of a 2-D F along axis (1: F(CV1), 0: F(CV2)), minimum shifted to 0."""
+    F = np.ascontiguousarray(F, np.float64)
+    n0, n1 = F.shape
+    out = np.zeros(n0 if axis == 1 else n1)
+    ctx._chk(lib().mdt_fes_marginal(ctx._h, _dp(F), n0, n1, axis, kT, bin_size, _dp(out)))
+    return out
 
 
 def mlp_forward(ctx, params, norm, s):

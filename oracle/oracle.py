@@ -2,7 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
 the cpu_baseline leg of bench.py, never by the shipped library.  Every function
-here calls the C restatement; numpy is used only to marshal arrays.
+here calls the C restatement (numpy only marshals arrays), except the FES
+reconstruction at the end, restated in numpy because the reference's script
+(tools/plot_hills.py) is numpy.
 """
 import ctypes as C
 import os
@@ -273,3 +275,35 @@ def mlp_loss_grad(params, s, y):
     loss = lib().orc_mlp_loss_grad(Din, H, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), float(np.asarray(b3).reshape(-1)[0]), s.shape[0],
                                    _p(s), _p(y), _p(g[0]), _p(g[1]), _p(g[2]), _p(g[3]), _p(g[4]), _p(gb3))
     return loss, g + [gb3[0]]
+
+
+# ------------------------------------------------------------------ FES reconstruction
+# Restated from the reference's post-processing script tools/plot_hills.py in its own
+# language (numpy).  Hills rows: step, s..., height, sigma.
+def fes_sum_1d(hills, x):
+    """:158-172: every hill at every point, summed over hills in file order."""
+    hills = np.atleast_2d(np.asarray(hills, np.float64))
+    c, h, s = hills[:, 1, None], hills[:, 2, None], hills[:, 3, None]
+    return np.sum(h * np.exp(-((np.asarray(x)[None, :] - c) ** 2) / (2 * s ** 2)), axis=0)
+
+
+def fes_sum_2d(hills, x, y, lo, d):
+    """:174-223: each hill over the bins of its window int((c -+ 4 sigma - lo)/d), clamped."""
+    hills = np.atleast_2d(np.asarray(hills, np.float64))
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    F = np.zeros_like(X)
+    n0, n1 = X.shape
+    for _, c1, c2, h, s in hills:
+        cut, t2 = 4.0 * s, 2.0 * s ** 2
+        r0, r1 = max(0, int((c1 - cut - lo[0]) / d[0])), min(n0 - 1, int((c1 + cut - lo[0]) / d[0]))
+        q0, q1 = max(0, int((c2 - cut - lo[1]) / d[1])), min(n1 - 1, int((c2 + cut - lo[1]) / d[1]))
+        if r0 <= r1 and q0 <= q1:
+            Xl, Yl = X[r0:r1 + 1, q0:q1 + 1], Y[r0:r1 + 1, q0:q1 + 1]
+            F[r0:r1 + 1, q0:q1 + 1] += h * np.exp(-((Xl - c1) ** 2 + (Yl - c2) ** 2) / t2)
+    return F
+
+
+def fes_marginal(F, kT, axis, bin_size):
+    """:236-262: -kT ln(sum exp(-F/kT) * bin) along axis, minimum shifted to 0."""
+    m = -kT * np.log(np.sum(np.exp(-np.asarray(F) / kT), axis=axis) * bin_size)
+    return m - np.min(m)

@@ -183,3 +183,29 @@ def test_expression_errors(mdt, expr, frag):
     with pytest.raises(mdt.MdtError) as e:
         mdt.Fix(ctx, GOOD.replace('"v1+v2"', f'"{expr}"'))
     assert frag in str(e.value)
+
+
+def test_fes_input_parser(mdt):
+    """The FES tool reads its grid from the fix metad line as tools/plot_hills.py:41-119 does:
+    CAL blocks skipped, DIM bounds per index, defaults, continuation lines."""
+    import os
+    import sys
+    import tempfile
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "metad-toy_amd"))
+    from mdt import fes
+    deck = ("units metal\n"
+            "fix 1 all metad GAUSSIAN 0.02 0.004 8 PACE 5 &\n"
+            "   CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 # a comment\n"
+            "   CAL NAME Q4 STEINH Q 4 all CV_dim 2 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE\n"
+            "   DIM 1 0 1 200 v1 DIM 2 -0.5 0.5 150 v2 WT 1 METAD_RESTART 1\n")
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "run.in")
+        open(path, "w").write(deck)
+        p = fes.parse_metad_params(path)
+        assert fes.parse_metad_params(os.path.join(tmp, "missing.in")) is None
+    assert p["sigma"] == 0.02 and p["height0"] == 0.004 and p["biasf"] == 8.0 and p["pace"] == 5
+    assert p["cv_dim"] == 2 and p["continue"] is True
+    assert p["bounds"] == {1: [0.0, 1.0, 200], 2: [-0.5, 0.5, 150]}
+    axes, lo, d = fes.grid_axes(p)
+    assert np.array_equal(axes[0], np.linspace(0.0025, 0.9975, 200))
+    assert np.allclose(d, [0.005, 1.0 / 150]) and np.array_equal(lo, [0.0, -0.5])

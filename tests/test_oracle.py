@@ -484,3 +484,43 @@ def test_mlp_matches_torch_autograd(orc):
     assert abs(L.item() - loss) < 1e-12
     for a, b in zip(tg, grads):
         assert np.abs(a.numpy() - np.asarray(b)).max() < 1e-12
+
+
+# ------------------------------------------------------------------ FES reconstruction
+def test_fes_1d_agrees_with_the_windowed_grid_on_the_reference_hills(orc):
+    """tools/plot_hills.py's 1-D sum over the reference's own HILLS fixture agrees with the
+    (independently pinned) uniform-grid deposit up to the grid's window truncation."""
+    import os
+    rows = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "HILLS_reference_test_cvset"), comments="#")
+    x = np.linspace(0.0005, 0.9995, 1000)
+    F = orc.fes_sum_1d(rows, x)
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "grid_reference_hills.npy"))
+    assert np.abs(F - gold).max() <= 238 * 0.005 * np.exp(-8.0) + 1e-15
+
+
+def test_fes_2d_window_truncation_bound(orc):
+    """The 2-D windows drop only tails beyond ~4 sigma - d: bounded by h exp(-(4s - d)^2 / 2s^2) per hill."""
+    rng = np.random.default_rng(21)
+    n, lo, hi, nb = 60, np.array([0.0, -1.0]), np.array([1.0, 1.0]), np.array([80, 90])
+    d = (hi - lo) / nb
+    rows = np.column_stack([np.arange(n), rng.uniform(0.0, 1.0, n), rng.uniform(-1, 1, n),
+                            rng.uniform(0.001, 0.005, n), np.full(n, 0.04)])
+    x = np.linspace(lo[0] + d[0] / 2, hi[0] - d[0] / 2, nb[0])
+    y = np.linspace(lo[1] + d[1] / 2, hi[1] - d[1] / 2, nb[1])
+    F = orc.fes_sum_2d(rows, x, y, lo, d)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    full = sum(h * np.exp(-((X - a) ** 2 + (Y - b) ** 2) / (2 * s * s)) for _, a, b, h, s in rows)
+    bound = np.sum(rows[:, 3]) * np.exp(-((4 * 0.04 - d.max()) ** 2) / (2 * 0.04 ** 2))
+    assert np.abs(F - full).max() <= bound
+    assert F.max() > 10 * bound
+
+
+def test_fes_marginal_closed_forms(orc):
+    """Constant F has a zero marginal; a separable F = a_i + b_j has F(CV1) = a - min(a)."""
+    kT = 2.494
+    assert np.abs(orc.fes_marginal(np.full((7, 9), 3.0), kT, 1, 0.1)).max() == 0.0
+    rng = np.random.default_rng(5)
+    a, b = rng.uniform(0, 10, 30), rng.uniform(0, 10, 40)
+    F = a[:, None] + b[None, :]
+    assert np.abs(orc.fes_marginal(F, kT, 1, 0.05) - (a - a.min())).max() <= 1e-12
+    assert np.abs(orc.fes_marginal(F, kT, 0, 0.05) - (b - b.min())).max() <= 1e-12
