@@ -217,6 +217,24 @@ int mdt_mlp_forward(mdt_ctx* ctx, const float* params, const double* norm, int64
 int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int64_t np, const double* s,
                       const double* v, float* grad);
 
+/* Ghost images and full neighbour lists built on a context's device, for n
+ * local atoms x_local[n*3] (host, or device memory when on_device) in the
+ * periodic box [lo, hi) with list cutoff rg.  Replaces the lists LAMMPS hands
+ * the reference through NeighHub::ensure/list (src/core/neigh_hub.cu:191-231)
+ * with the same data the host generator mdt_system_build produces, in the same
+ * order: ghosts shift-major (sz, sy, sx over -1..1, sz outermost, zero shift
+ * skipped), local atoms ascending within a shift; each list visits the 27-bin
+ * stencil (dz outer, dx inner) and a bin's atoms in ascending index, keeping
+ * FP64 r^2 <= rg^2.  The result stays on the device (n_all atoms, nnz list
+ * entries): mdt_neighbor_fetch copies any of x_all[n_all*3],
+ * offsets[n+1], idx[nnz], ghost_owner[n_all-n] to the host, and
+ * mdt_neighbor_bind binds box, positions and lists to a walker as
+ * mdt_set_box + mdt_bind_positions + mdt_bind_neighbors would. */
+int mdt_neighbor_build(mdt_ctx* ctx, const double* x_local, int64_t n, const double lo[3], const double hi[3],
+                       double rg, int on_device, int64_t* n_all, int64_t* nnz);
+int mdt_neighbor_fetch(mdt_ctx* ctx, double* x_all, int32_t* offsets, int32_t* idx, int32_t* ghost_owner);
+int mdt_neighbor_bind(mdt_ctx* ctx, int walker);
+
 /* FES reconstruction from a HILLS list on a context's device.  Replaces the
  * hill sum of tools/plot_hills.py:158-223 (the reference's post-processing
  * script, numpy on the host).  hills: nh rows of (step, s_1..s_D, height,

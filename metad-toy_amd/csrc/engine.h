@@ -220,6 +220,28 @@ constexpr int kStMlpErr = kStStep + 1;         // learned bias: held-out max |dV
 constexpr int kStMlpOk = kStMlpErr + 1;        // learned bias: 1 if the network is in use
 constexpr int kStStride = 32;
 
+// Periodic box and bins of a device neighbour build (nlist.hip).
+struct NlBox {
+    double lo[3], hi[3], L[3];
+    double rg;                // list cutoff
+    double elo[3], bw[3];     // extended box lower corner (lo - rg) and bin widths
+    int32_t nb[3];            // bins per dimension
+};
+hipError_t nl_ghost_flags(const double* x, int64_t n, const NlBox& b, int32_t* flag, hipStream_t st);
+hipError_t nl_ghost_scatter(const double* x, int64_t n, const NlBox& b, const int32_t* flag, const int32_t* pos,
+                            double* xall, int32_t* owner, hipStream_t st);
+hipError_t nl_bin(const double* xall, int64_t na, const NlBox& b, int32_t* bin_of, int32_t* val,
+                  int32_t* bin_count, hipStream_t st);
+hipError_t nl_list_count(const double* xall, int64_t n, const NlBox& b, const int32_t* bin_of,
+                         const int32_t* bin_start, const int32_t* bin_atoms, int64_t* count, hipStream_t st);
+hipError_t nl_list_fill(const double* xall, int64_t n, const NlBox& b, const int32_t* bin_of,
+                        const int32_t* bin_start, const int32_t* bin_atoms, const int64_t* off, int32_t* idx,
+                        int32_t* off32, hipStream_t st);
+hipError_t nl_scan_i32(void* tmp, size_t& bytes, const int32_t* in, int32_t* out, int64_t n, hipStream_t st);
+hipError_t nl_scan_i64(void* tmp, size_t& bytes, const int64_t* in, int64_t* out, int64_t n, hipStream_t st);
+hipError_t nl_sort_pairs(void* tmp, size_t& bytes, const int32_t* kin, int32_t* kout, const int32_t* vin,
+                         int32_t* vout, int64_t n, int end_bit, hipStream_t st);
+
 // ---- launchers (kernels.hip) ----
 hipError_t launch_topology(const Geo& g, int64_t n_local_total, int32_t* eown, uint8_t* revpos,
                            hipStream_t st);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <functional>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -247,6 +248,12 @@ struct mdt_ctx {
     std::vector<Tile> h_tiles;
     std::vector<int32_t> tile_begin;
     std::vector<mdt_fix*> fixes;   // fixes created on this context (detached on destroy)
+    // the last device neighbour build (mdt_neighbor_build)
+    DBuf<double> nl_xall;
+    DBuf<int32_t> nl_off, nl_idx, nl_owner;
+    int64_t nl_n = 0, nl_n_all = 0, nl_nnz = 0;
+    double nl_lo[3] = {0, 0, 0}, nl_hi[3] = {0, 0, 0};
+    bool nl_ok = false;
 
     Geo geo() const {
         Geo g{};
@@ -1381,6 +1388,14 @@ int fail(mdt_ctx* ctx, const Err& e) {
     return e.code;
 }
 
+// temporary storage of a hipcub call: query, allocate, run
+static void cub_call(DBuf<uint8_t>& tmp, hipStream_t, const std::function<hipError_t(void*, size_t&)>& f) {
+    size_t bytes = 0;
+    HIPCHK(f(nullptr, bytes));
+    tmp.alloc(std::max<size_t>(bytes, 1));
+    HIPCHK(f((void*)tmp.p, bytes));
+}
+
 #define API_BEGIN(ctxp) mdt_ctx* ctx__ = (ctxp); try {
 #define FIX_BEGIN(fxp) if (!(fxp) || !(fxp)->ctx) return MDT_ESTATE; API_BEGIN((fxp)->ctx)
 #define API_END                                              \
@@ -1906,6 +1921,115 @@ static void upload_points(mdt_ctx* ctx, int64_t np, const double* s, DBuf<float4
     for (int64_t p = 0; p < np; ++p) h[p] = make_float4((float)s[3 * p], (float)s[3 * p + 1], (float)s[3 * p + 2], 0.0f);
     ds.alloc(h.size());
     ds.upload(h.data(), h.size(), ctx->stream);
+}
+
+int mdt_neighbor_build(mdt_ctx* ctx, const double* x_local, int64_t n, const double lo[3], const double hi[3],
+                       double rg, int on_device, int64_t* n_all, int64_t* nnz) {
+    API_BEGIN(ctx)
+    if (!ctx) return MDT_EINVAL;
+    if (n < 0 || (n > 0 && !x_local) || !lo || !hi) throw Err(MDT_EINVAL, "null positions or box");
+    if (!(rg > 0)) throw Err(MDT_EINVAL, "the list cutoff must be positive");
+    if (26 * n > (int64_t)INT32_MAX) throw Err(MDT_ENOTSUP, "more than 2^31 / 26 local atoms");
+    use_device(ctx);
+    hipStream_t st = ctx->stream;
+    NlBox b{};
+    int64_t nbins = 1;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = lo[d]; b.hi[d] = hi[d]; b.L[d] = hi[d] - lo[d];
+        if (!(b.L[d] > 2.0 * rg)) throw Err(MDT_EINVAL, "box edge must exceed twice the list cutoff");
+        b.elo[d] = lo[d] - rg;
+        const double ext = b.L[d] + 2 * rg;
+        b.nb[d] = std::max(1, (int)std::floor(ext / rg));
+        b.bw[d] = ext / b.nb[d];
+        nbins *= b.nb[d];
+    }
+    if (nbins + 1 > (int64_t)INT32_MAX) throw Err(MDT_ENOTSUP, "too many bins");
+    b.rg = rg;
+    ctx->nl_ok = false;
+    DBuf<double> dx;
+    dx.alloc((size_t)std::max<int64_t>(3 * n, 1));
+    if (n > 0) HIPCHK(hipMemcpyAsync(dx.p, x_local, sizeof(double) * 3 * n,
+                                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    DBuf<uint8_t> tmp;
+    // ghosts: flags per (shift, atom), compacted in that order
+    const int64_t nq = 26 * n;
+    DBuf<int32_t> flag, pos;
+    flag.alloc((size_t)std::max<int64_t>(nq, 1)); pos.alloc((size_t)std::max<int64_t>(nq, 1));
+    int64_t ng = 0;
+    if (n > 0) {
+        HIPCHK(nl_ghost_flags(dx.p, n, b, flag.p, st));
+        cub_call(tmp, st, [&](void* t, size_t& by) { return nl_scan_i32(t, by, flag.p, pos.p, nq, st); });
+        int32_t last[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&last[0], pos.p + nq - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&last[1], flag.p + nq - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        ng = (int64_t)last[0] + last[1];
+    }
+    const int64_t na = n + ng;
+    ctx->nl_xall.alloc((size_t)std::max<int64_t>(3 * na, 1));
+    ctx->nl_owner.alloc((size_t)std::max<int64_t>(ng, 1));
+    HIPCHK(nl_ghost_scatter(dx.p, n, b, flag.p, pos.p, ctx->nl_xall.p, ctx->nl_owner.p, st));
+    // bins: (bin, index) pairs sorted stably by bin, so a bin lists its atoms in ascending index
+    DBuf<int32_t> bin_of, val, bin_count, bin_start, key_sorted, bin_atoms;
+    bin_of.alloc((size_t)std::max<int64_t>(na, 1)); val.alloc((size_t)std::max<int64_t>(na, 1));
+    key_sorted.alloc((size_t)std::max<int64_t>(na, 1)); bin_atoms.alloc((size_t)std::max<int64_t>(na, 1));
+    bin_count.alloc((size_t)nbins + 1); bin_start.alloc((size_t)nbins + 1);
+    HIPCHK(hipMemsetAsync(bin_count.p, 0, sizeof(int32_t) * (nbins + 1), st));
+    HIPCHK(nl_bin(ctx->nl_xall.p, na, b, bin_of.p, val.p, bin_count.p, st));
+    cub_call(tmp, st, [&](void* t, size_t& by) { return nl_scan_i32(t, by, bin_count.p, bin_start.p, nbins + 1, st); });
+    int end_bit = 1;
+    while (end_bit < 31 && ((int64_t)1 << end_bit) < nbins) ++end_bit;
+    if (na > 0)
+        cub_call(tmp, st, [&](void* t, size_t& by) {
+            return nl_sort_pairs(t, by, bin_of.p, key_sorted.p, val.p, bin_atoms.p, na, end_bit, st);
+        });
+    // lists: count, offsets, fill
+    DBuf<int64_t> cnt, off64;
+    cnt.alloc((size_t)n + 1); off64.alloc((size_t)n + 1);
+    HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(int64_t) * (n + 1), st));
+    HIPCHK(nl_list_count(ctx->nl_xall.p, n, b, bin_of.p, bin_start.p, bin_atoms.p, cnt.p, st));
+    cub_call(tmp, st, [&](void* t, size_t& by) { return nl_scan_i64(t, by, cnt.p, off64.p, n + 1, st); });
+    int64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, off64.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (total > (int64_t)INT32_MAX) throw Err(MDT_ENOTSUP, "neighbour list exceeds 2^31 entries");
+    ctx->nl_idx.alloc((size_t)std::max<int64_t>(total, 1));
+    ctx->nl_off.alloc((size_t)n + 1);
+    HIPCHK(nl_list_fill(ctx->nl_xall.p, n, b, bin_of.p, bin_start.p, bin_atoms.p, off64.p, ctx->nl_idx.p,
+                        ctx->nl_off.p, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->nl_n = n; ctx->nl_n_all = na; ctx->nl_nnz = total;
+    for (int d = 0; d < 3; ++d) { ctx->nl_lo[d] = lo[d]; ctx->nl_hi[d] = hi[d]; }
+    ctx->nl_ok = true;
+    if (n_all) *n_all = na;
+    if (nnz) *nnz = total;
+    API_END
+}
+
+int mdt_neighbor_fetch(mdt_ctx* ctx, double* x_all, int32_t* offsets, int32_t* idx, int32_t* ghost_owner) {
+    API_BEGIN(ctx)
+    if (!ctx) return MDT_EINVAL;
+    if (!ctx->nl_ok) throw Err(MDT_ESTATE, "no device neighbour build on this context");
+    use_device(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (x_all) HIPCHK(hipMemcpy(x_all, ctx->nl_xall.p, sizeof(double) * 3 * ctx->nl_n_all, hipMemcpyDeviceToHost));
+    if (offsets) HIPCHK(hipMemcpy(offsets, ctx->nl_off.p, sizeof(int32_t) * (ctx->nl_n + 1), hipMemcpyDeviceToHost));
+    if (idx && ctx->nl_nnz) HIPCHK(hipMemcpy(idx, ctx->nl_idx.p, sizeof(int32_t) * ctx->nl_nnz, hipMemcpyDeviceToHost));
+    if (ghost_owner && ctx->nl_n_all > ctx->nl_n)
+        HIPCHK(hipMemcpy(ghost_owner, ctx->nl_owner.p, sizeof(int32_t) * (ctx->nl_n_all - ctx->nl_n),
+                         hipMemcpyDeviceToHost));
+    API_END
+}
+
+int mdt_neighbor_bind(mdt_ctx* ctx, int walker) {
+    if (!ctx) return MDT_EINVAL;
+    if (!ctx->nl_ok) { ctx->err = "no device neighbour build on this context"; return MDT_ESTATE; }
+    const int periodic[3] = {1, 1, 1};
+    int rc = mdt_set_box(ctx, walker, ctx->nl_lo, ctx->nl_hi, periodic);
+    if (rc == MDT_OK) rc = mdt_bind_positions(ctx, walker, ctx->nl_xall.p, ctx->nl_n, ctx->nl_n_all, 1);
+    if (rc == MDT_OK)
+        rc = mdt_bind_neighbors(ctx, walker, ctx->nl_off.p, ctx->nl_idx.p, ctx->nl_n, ctx->nl_owner.p, 1);
+    return rc;
 }
 
 int mdt_fes_sum(mdt_ctx* ctx, int D, int64_t nh, const double* hills, const double* x, int n0,

@@ -77,6 +77,9 @@ def lib():
         "mdt_mlp_forward": (I, [P, C.POINTER(C.c_float), dp, I64, dp, dp]),
         "mdt_mlp_loss_grad": (I, [P, C.POINTER(C.c_float), dp, I64, dp, dp, C.POINTER(C.c_float)]),
         "mdt_fes_sum": (I, [P, I, I64, dp, dp, I, dp, I, dp, dp, dp]),
+        "mdt_neighbor_build": (I, [P, P, I64, dp, dp, D, I, C.POINTER(I64), C.POINTER(I64)]),
+        "mdt_neighbor_fetch": (I, [P, dp, ip, ip, ip]),
+        "mdt_neighbor_bind": (I, [P, I]),
         "mdt_fes_marginal": (I, [P, dp, I, I, I, D, D, dp]),
         "mdt_system_create": (I, [C.POINTER(P)]),
         "mdt_system_destroy": (None, [P]),
@@ -217,6 +220,29 @@ class Context:
 
     def bind_system(self, walker, system):
         self._chk(lib().mdt_bind_system(self._h, walker, system._h))
+
+    def build_neighbors(self, x_local, lo, hi, list_cutoff):
+        """Ghost images and full lists built on this context's device (mdt_neighbor_build);
+        returns (n_all, nnz).  fetch_neighbors copies them out, bind_built_neighbors binds them."""
+        x = np.ascontiguousarray(x_local, np.float64).reshape(-1, 3)
+        na, nz = C.c_int64(), C.c_int64()
+        self._chk(lib().mdt_neighbor_build(self._h, x.ctypes.data, x.shape[0], _dp(np.asarray(lo, float)),
+                                           _dp(np.asarray(hi, float)), list_cutoff, 0, C.byref(na), C.byref(nz)))
+        self._nl = (x.shape[0], na.value, nz.value)
+        return na.value, nz.value
+
+    def fetch_neighbors(self):
+        """x_all (n_all,3), offsets (n+1), idx (nnz), ghost_owner (n_all-n) of the last device build."""
+        n, na, nz = self._nl
+        x = np.zeros((na, 3))
+        off = np.zeros(n + 1, np.int32)
+        idx = np.zeros(max(nz, 1), np.int32)
+        own = np.zeros(max(na - n, 1), np.int32)
+        self._chk(lib().mdt_neighbor_fetch(self._h, _dp(x), _ip(off), _ip(idx), _ip(own)))
+        return x, off, idx[:nz], own[: na - n]
+
+    def bind_built_neighbors(self, walker):
+        self._chk(lib().mdt_neighbor_bind(self._h, walker))
 
     def update_positions(self, walker, x_all):
         x = np.ascontiguousarray(x_all, np.float64)
