@@ -220,6 +220,20 @@ constexpr int kStMlpErr = kStStep + 1;         // learned bias: held-out max |dV
 constexpr int kStMlpOk = kStMlpErr + 1;        // learned bias: 1 if the network is in use
 constexpr int kStStride = 32;
 
+// DISTANCE CV between two atoms of each walker (one thread per walker).
+struct DistDev {
+    int32_t n_walkers;
+    int32_t a1, a2;           // local indices within a walker
+    const int64_t* atom_off;  // per walker first atom in the concatenated arrays
+    const double* box;        // per walker box lengths (3)
+    double* dvec;             // per walker (dx, dy, dz, r) of the last value pass
+    double* direct;           // per walker [kMaxSlotCV] CV values
+    const double* wcv;        // per walker [kMaxSlotCV] chain-rule weights
+    int32_t slot;
+};
+hipError_t launch_dist_value(const double4* x4, const DistDev& d, hipStream_t st);
+hipError_t launch_dist_force(const DistDev& d, double* f, hipStream_t st);
+
 // Periodic box and bins of a device neighbour build (nlist.hip).
 struct NlBox {
     double lo[3], hi[3], L[3];

@@ -353,7 +353,7 @@ void mdt_ctx::commit() {
 // ---------------------------------------------------------------------------
 namespace {
 
-enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2 };
+enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2, CV_DIST = 3 };
 
 struct EnvH {
     double rc = 0;
@@ -386,6 +386,10 @@ struct CvH {
     DBuf<float> ampf;
     DBuf<int32_t> chunk, chunk_begin, natoms, abegin;
     int nchunk = 0, max_atoms = 0;
+    // DISTANCE: local indices of the two atoms, per-walker offsets, boxes and (d, r)
+    int a1 = -1, a2 = -1;
+    DBuf<int64_t> doff;
+    DBuf<double> dbox, dvec;
 };
 
 struct SymH {
@@ -658,7 +662,17 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 fx->cvs.push_back(c);
             } else if (type == "SW_FUNC") {
                 throw Err(MDT_ENOTSUP, "CAL ... SW_FUNC: switching functions are not provided");
-            } else if (type == "STRU_FACTOR" || type == "WEIGHT_CHEM" || type == "DISTANCE") {
+            } else if (type == "DISTANCE") {
+                // DISTANCE id1 id2 (CVs_DISTANCE.cu:17-30): 1-based atom IDs, the same in every walker
+                if (i + 1 >= argc) throw Err(MDT_EINVAL, "Error: DISTANCE command requires 2 atom IDs.");
+                const long long id1 = parse_int(argv[i], "DISTANCE"), id2 = parse_int(argv[i + 1], "DISTANCE");
+                if (id1 < 1 || id2 < 1) throw Err(MDT_EINVAL, "DISTANCE atom IDs start at 1");
+                if (id1 == id2) throw Err(MDT_EINVAL, "DISTANCE needs two different atoms");
+                i += 2;
+                CvH* c = new CvH();
+                c->name = name; c->kind = CV_DIST; c->a1 = (int)(id1 - 1); c->a2 = (int)(id2 - 1);
+                fx->cvs.push_back(c);
+            } else if (type == "STRU_FACTOR" || type == "WEIGHT_CHEM") {
                 throw Err(MDT_ENOTSUP, fmt("CV type %s is outside this library's scope", type.c_str()));
             } else {
                 throw Err(MDT_EINVAL, fmt("Metad-toy Error: Unknown or unregistered object type '%s' specified in CAL '%s'.",
@@ -674,7 +688,9 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
             s.func = dot == std::string::npos ? "value" : target.substr(dot + 1);
             for (size_t k = 0; k < fx->cvs.size(); ++k) if (fx->cvs[k]->name == cname) s.cv = (int)k;
             if (s.cv < 0) throw Err(MDT_EINVAL, fmt("SYMBOL %s: CAL '%s' is not defined", s.name.c_str(), cname.c_str()));
-            if (s.func != "AVE") throw Err(MDT_EINVAL, fmt("We can't find the func %s.", s.func.c_str()));
+            // a DISTANCE CV has one value whatever the function is called (CVs_DISTANCE.cu:125-128)
+            if (s.func != "AVE" && fx->cvs[s.cv]->kind != CV_DIST)
+                throw Err(MDT_EINVAL, fmt("We can't find the func %s.", s.func.c_str()));
             for (auto& o : fx->syms) if (o.name == s.name) throw Err(MDT_EINVAL, fmt("SYMBOL %s defined twice", s.name.c_str()));
             if ((int)fx->syms.size() >= kMaxSym) throw Err(MDT_ENOTSUP, fmt("at most %d symbols", kMaxSym));
             fx->syms.push_back(s);
@@ -935,6 +951,20 @@ void fix_build(mdt_fix* fx) {
             c->apart.alloc((size_t)c->nchunk * nk * 2);
             c->amp.alloc((size_t)nw * nk * 2);
             c->kpack.alloc((size_t)nw * nk);
+        } else if (c->kind == CV_DIST) {
+            std::vector<int64_t> off(nw);
+            std::vector<double> box((size_t)nw * 3);
+            for (int k = 0; k < nw; ++k) {
+                WalkerIn& a = ctx->w[k];
+                if (c->a1 >= a.n_local || c->a2 >= a.n_local)
+                    throw Err(MDT_EINVAL, fmt("DISTANCE %s: atom ID beyond walker %d's %lld atoms", c->name.c_str(), k,
+                                              (long long)a.n_local));
+                off[k] = a.atom_off;
+                for (int d = 0; d < 3; ++d) box[3 * k + d] = a.hi[d] - a.lo[d];
+            }
+            c->doff.alloc(nw); c->doff.upload(off.data(), nw, st);
+            c->dbox.alloc((size_t)nw * 3); c->dbox.upload(box.data(), (size_t)nw * 3, st);
+            c->dvec.alloc((size_t)nw * 4);
         }
     }
     fx->nslot_sum = nsum;
@@ -1005,7 +1035,7 @@ void fix_build(mdt_fix* fx) {
     r.ncv = (int)fx->cvs.size();
     // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
     for (size_t c = 0; c < fx->cvs.size(); ++c)
-        r.cv_src[c] = fx->cvs[c]->kind == CV_SK ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
+        r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
     r.inv_ng = fx->envs.empty() ? nullptr : fx->envs[0]->inv_ng.p;
     for (auto* e : fx->envs)
         if (e->group != fx->envs[0]->group) throw Err(MDT_ENOTSUP, "all STEINH CALs must use the same group");
@@ -1297,6 +1327,14 @@ struct Timer {
     }
 };
 
+DistDev dist_dev(mdt_fix* fx, CvH* c) {
+    DistDev d{};
+    d.n_walkers = fx->nw(); d.a1 = c->a1; d.a2 = c->a2;
+    d.atom_off = c->doff.p; d.box = c->dbox.p; d.dvec = c->dvec.p;
+    d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot;
+    return d;
+}
+
 void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
     static Timer off_timer;
     Timer& T = tm ? *tm : off_timer;
@@ -1324,6 +1362,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
             s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             T.run("k_sk_amp", st, [&] { HIPCHK(launch_sk_amp(g, s, st)); });
+        } else if (c->kind == CV_DIST) {
+            const DistDev d = dist_dev(fx, c);
+            T.run("k_dist_value", st, [&] { HIPCHK(launch_dist_value(g.x4, d, st)); });
         }
     }
     T.run("k_reduce", st, [&] { HIPCHK(launch_reduce(fx->red, st)); });
@@ -1345,6 +1386,10 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             s.wslot = c->slot; s.wcv = fx->wcv.p; s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p;
             s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(g, s, f, st)); });
+        }
+        if (c->kind == CV_DIST && c->biased) {
+            const DistDev d = dist_dev(fx, c);
+            T.run("k_dist_force", st, [&] { HIPCHK(launch_dist_force(d, f, st)); });
         }
     }
     // The values of unbiased local-Q CVs fused into the force pass (their tile sums
@@ -1738,8 +1783,12 @@ static int find_cv(mdt_fix* fx, const char* cal_func) {
     size_t dot = t.find('.');
     std::string name = t.substr(0, dot);
     std::string func = dot == std::string::npos ? "AVE" : t.substr(dot + 1);
-    if (func != "AVE") throw Err(MDT_EINVAL, fmt("We can't find the func %s.", func.c_str()));
-    for (size_t k = 0; k < fx->cvs.size(); ++k) if (fx->cvs[k]->name == name) return (int)k;
+    for (size_t k = 0; k < fx->cvs.size(); ++k)
+        if (fx->cvs[k]->name == name) {
+            if (func != "AVE" && fx->cvs[k]->kind != CV_DIST)
+                throw Err(MDT_EINVAL, fmt("We can't find the func %s.", func.c_str()));
+            return (int)k;
+        }
     throw Err(MDT_EINVAL, fmt("no CAL named '%s'", name.c_str()));
 }
 
@@ -1774,7 +1823,7 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
     if (!c) throw Err(MDT_EINVAL, fmt("no CAL named '%s'", name.c_str()));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int64_t n = a.n_local, o = a.atom_off;
-    if (c->kind == CV_SK) throw Err(MDT_EINVAL, "SK has no per-atom properties");
+    if (c->kind == CV_SK || c->kind == CV_DIST) throw Err(MDT_EINVAL, "SK and DISTANCE have no per-atom properties");
     EnvH* e = fx->envs[c->env];
     if (prop == "stein_q" || (prop == "local_q" && c->kind == CV_LD)) {
         std::vector<float> v((size_t)n);
@@ -1809,7 +1858,7 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
     WalkerIn& a = ctx->walker(walker);
     CvH* c = nullptr;
     for (auto* q : fx->cvs) if (q->name == (cal_name ? cal_name : "")) c = q;
-    if (!c || c->kind == CV_SK) throw Err(MDT_EINVAL, "selection export needs a STEINH CAL name");
+    if (!c || c->kind == CV_SK || c->kind == CV_DIST) throw Err(MDT_EINVAL, "selection export needs a STEINH CAL name");
     EnvH* e = fx->envs[c->env];
     const int N = e->N;
     const int64_t n = a.n_local;

@@ -1795,6 +1795,46 @@ __global__ __launch_bounds__(kBlock) void k_sub_min(double* v, int n) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) v[k] -= mn;
 }
 
+// ---------------------------------------------------------------------------
+// DISTANCE CV (src/cvs/CVs_DISTANCE.cu:55-123): r = |x_a2 - x_a1| with each
+// component folded to the minimum image by the half-box rule of delta_x
+// (:55-85); the bias force is -dV/dr on a1 along -d/r and on a2 along +d/r
+// (bias_force :98-112 with get_dcvdx :114-121).  One thread per walker.
+// ---------------------------------------------------------------------------
+__global__ void k_dist_value(const double4* __restrict__ x4, DistDev d) {
+#pragma clang fp contract(off)
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.n_walkers) return;
+    const double4 p1 = x4[d.atom_off[w] + d.a1], p2 = x4[d.atom_off[w] + d.a2];
+    double v[3] = {p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double L = d.box[3 * w + k];
+        if (v[k] > L / 2) v[k] -= L;
+        else if (v[k] < -L / 2) v[k] += L;
+    }
+    const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double* o = d.dvec + 4 * w;
+    o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = r;
+    d.direct[(int64_t)w * kMaxSlotCV + d.slot] = r;
+}
+
+__global__ void k_dist_force(DistDev d, double* __restrict__ f) {
+#pragma clang fp contract(off)
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.n_walkers) return;
+    const double* o = d.dvec + 4 * w;
+    const double dV = d.wcv[(int64_t)w * kMaxSlotCV + d.slot];
+    double* f1 = f + 3 * (d.atom_off[w] + d.a1);
+    double* f2 = f + 3 * (d.atom_off[w] + d.a2);
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double g = -o[k] / o[3];
+        f1[k] -= dV * g;
+        f2[k] += dV * g;
+    }
+}
+
 // Virial partials of a force buffer: sum_i F_i (x) unwrap(x_i), components
 // xx yy zz xy xz yz (the tally of FixMetadynamics::post_force).
 __global__ __launch_bounds__(kBlock) void k_virial(const double4* __restrict__ x4, const double* __restrict__ f,
@@ -2178,6 +2218,16 @@ hipError_t launch_fes_marginal(const double* F, int n0, int n1, int axis, double
     hipLaunchKernelGGL(k_fes_marginal, dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, F, n0, n1,
                        axis, kT, bin, out);
     hipLaunchKernelGGL(k_sub_min, dim3(1), dim3(kBlock), 0, st, out, nout);
+    return hipGetLastError();
+}
+
+hipError_t launch_dist_value(const double4* x4, const DistDev& d, hipStream_t st) {
+    hipLaunchKernelGGL(k_dist_value, dim3((d.n_walkers + 63) / 64), dim3(64), 0, st, x4, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_dist_force(const DistDev& d, double* f, hipStream_t st) {
+    hipLaunchKernelGGL(k_dist_force, dim3((d.n_walkers + 63) / 64), dim3(64), 0, st, d, f);
     return hipGetLastError();
 }
 

@@ -307,3 +307,17 @@ def fes_marginal(F, kT, axis, bin_size):
     """:236-262: -kT ln(sum exp(-F/kT) * bin) along axis, minimum shifted to 0."""
     m = -kT * np.log(np.sum(np.exp(-np.asarray(F) / kT), axis=axis) * bin_size)
     return m - np.min(m)
+
+
+# ------------------------------------------------------------------ DISTANCE
+def distance_cv(x, L, a1, a2):
+    """src/cvs/CVs_DISTANCE.cu:55-121: r = |x_a2 - x_a1| with each component folded by the
+    half-box rule of delta_x (> L/2: -L; < -L/2: +L); returns r and the folded vector d, so
+    dr/dx_a1 = -d/r and dr/dx_a2 = d/r (get_dcvdx)."""
+    d = np.asarray(x[a2], np.float64) - np.asarray(x[a1], np.float64)
+    for k in range(3):
+        if d[k] > L[k] / 2:
+            d[k] -= L[k]
+        elif d[k] < -L[k] / 2:
+            d[k] += L[k]
+    return float(np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])), d

@@ -100,7 +100,10 @@ def test_fix_parser_accepts_reference_syntax(mdt):
     ("CAL NAME Q6 STEINH Q 8 all", -95),
     ("CAL NAME Q6 STEINH L 6 all", -95),
     ("CAL NAME Q6 STEINH Q 6 all SW_FUNC_r sw", -95),
-    ("CAL NAME Q6 DISTANCE 1 2", -95),
+    ("CAL NAME Q6 WEIGHT_CHEM 1 2", -95),
+    ("CAL NAME D DISTANCE 3 3", -22),
+    ("CAL NAME D DISTANCE 0 2", -22),
+    ("CAL NAME D DISTANCE 4", -22),
     ("CAL NAME Q6 NOPE 1 2", -22),
 ])
 def test_fix_parser_errors(mdt, bad, code):

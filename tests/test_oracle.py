@@ -524,3 +524,24 @@ def test_fes_marginal_closed_forms(orc):
     F = a[:, None] + b[None, :]
     assert np.abs(orc.fes_marginal(F, kT, 1, 0.05) - (a - a.min())).max() <= 1e-12
     assert np.abs(orc.fes_marginal(F, kT, 0, 0.05) - (b - b.min())).max() <= 1e-12
+
+
+# ------------------------------------------------------------------ DISTANCE
+def test_distance_minimum_image_and_gradient(orc):
+    """Across a periodic face the folded distance is the short one; the gradient is the FD
+    of the distance (a move that keeps the pair away from the half-box fold)."""
+    L = np.array([20.0, 15.0, 30.0])
+    x = np.array([[0.5, 1.0, 2.0], [19.0, 14.0, 28.5]])
+    r, d = orc.distance_cv(x, L, 0, 1)
+    assert np.allclose(d, [-1.5, -2.0, -3.5]) and abs(r - np.sqrt(1.5 ** 2 + 2 ** 2 + 3.5 ** 2)) < 1e-15
+    rng = np.random.default_rng(9)
+    y = rng.uniform(0, 1, size=(2, 3)) * L * 0.3
+    r0, d0 = orc.distance_cv(y, L, 0, 1)
+    h = 1e-6
+    for a, sgn in ((0, -1.0), (1, 1.0)):
+        for k in range(3):
+            yp, ym = y.copy(), y.copy()
+            yp[a, k] += h
+            ym[a, k] -= h
+            fd = (orc.distance_cv(yp, L, 0, 1)[0] - orc.distance_cv(ym, L, 0, 1)[0]) / (2 * h)
+            assert abs(fd - sgn * d0[k] / r0) < 1e-8
