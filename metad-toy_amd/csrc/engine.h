@@ -220,6 +220,21 @@ constexpr int kStMlpErr = kStStep + 1;         // learned bias: held-out max |dV
 constexpr int kStMlpOk = kStMlpErr + 1;        // learned bias: 1 if the network is in use
 constexpr int kStStride = 32;
 
+// STRU_FACTOR (Debye) CV: per-atom S_i over each atom's list within rc.
+struct SfDev {
+    double rc2, q;
+    double* S;                // per local atom S_i
+    double* part;             // per tile FP64 sum of S_i over group atoms
+    const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)
+    const double* inv_ng;     // per walker 1 / N_group
+    double* direct;           // per walker [kMaxSlotCV] CV values
+    const double* wcv;        // per walker [kMaxSlotCV] chain-rule weights
+    int32_t slot;
+    int32_t n_walkers;
+};
+hipError_t launch_sf_value(const Geo& g, const SfDev& d, hipStream_t st);
+hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t st);
+
 // DISTANCE CV between two atoms of each walker (one thread per walker).
 struct DistDev {
     int32_t n_walkers;

@@ -353,7 +353,7 @@ void mdt_ctx::commit() {
 // ---------------------------------------------------------------------------
 namespace {
 
-enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2, CV_DIST = 3 };
+enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2, CV_DIST = 3, CV_SF = 4 };
 
 struct EnvH {
     double rc = 0;
@@ -390,6 +390,11 @@ struct CvH {
     int a1 = -1, a2 = -1;
     DBuf<int64_t> doff;
     DBuf<double> dbox, dvec;
+    // STRU_FACTOR: cutoff, q, group; per-atom S_i, per-tile sums, per-walker 1/N_group
+    double sf_rc = 8.0, sf_q = 0.0;
+    std::string sf_group;
+    DBuf<double> sf_S, sf_part, sf_inv;
+    DBuf<int32_t> sf_tb;
 };
 
 struct SymH {
@@ -672,7 +677,39 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 CvH* c = new CvH();
                 c->name = name; c->kind = CV_DIST; c->a1 = (int)(id1 - 1); c->a2 = (int)(id2 - 1);
                 fx->cvs.push_back(c);
-            } else if (type == "STRU_FACTOR" || type == "WEIGHT_CHEM") {
+            } else if (type == "STRU_FACTOR") {
+                // STRU_FACTOR <group> [cutoff_r r] [q_factor q] [d_block_size b] (CVs_Strufactor.cu:34-150);
+                // the switching-function and chemical variants are not provided
+                if (i >= argc) throw Err(MDT_EINVAL, "Error: STRU_FACTOR command requires at least \"STRU_FACTOR <group> \".");
+                CvH* c = new CvH();
+                c->name = name; c->kind = CV_SF; c->sf_group = argv[i++];
+                bool have_q = false;
+                try {
+                    while (i < argc) {
+                        std::string k2 = argv[i];
+                        if (k2 == "cutoff_r") {
+                            need(1, "Error: 'cutoff_r' keyword requires a value");
+                            c->sf_rc = parse_num(argv[i + 1], "cutoff_r"); i += 2;
+                        } else if (k2 == "q_factor") {
+                            need(1, "Error: 'q_factor' keyword requires an integer");
+                            c->sf_q = parse_num(argv[i + 1], "q_factor"); have_q = true; i += 2;
+                            if (!(c->sf_q > 0)) throw Err(MDT_EINVAL, "Error: 'q_factor' must be > 0");
+                        } else if (k2 == "d_block_size") {
+                            need(1, "Error: 'd_block_size' keyword requires an integer");
+                            if (parse_int(argv[i + 1], "d_block_size") <= 0) throw Err(MDT_EINVAL, "Error: 'd_block_size' must be > 0");
+                            i += 2;
+                        } else if (k2 == "SW_func" || k2 == "Chemical" || k2 == "Chem_ctarget" || k2 == "Chem_sigma" ||
+                                   k2 == "Chem_weight") {
+                            throw Err(MDT_ENOTSUP, fmt("STRU_FACTOR %s: the switching-function and chemical variants are not provided", k2.c_str()));
+                        } else {
+                            break;
+                        }
+                    }
+                    if (!have_q) throw Err(MDT_EINVAL, "STRU_FACTOR needs q_factor (the reference leaves it unset)");
+                    if (!(c->sf_rc > 0)) throw Err(MDT_EINVAL, "STRU_FACTOR cutoff_r must be positive");
+                } catch (...) { delete c; throw; }
+                fx->cvs.push_back(c);
+            } else if (type == "WEIGHT_CHEM") {
                 throw Err(MDT_ENOTSUP, fmt("CV type %s is outside this library's scope", type.c_str()));
             } else {
                 throw Err(MDT_EINVAL, fmt("Metad-toy Error: Unknown or unregistered object type '%s' specified in CAL '%s'.",
@@ -951,6 +988,19 @@ void fix_build(mdt_fix* fx) {
             c->apart.alloc((size_t)c->nchunk * nk * 2);
             c->amp.alloc((size_t)nw * nk * 2);
             c->kpack.alloc((size_t)nw * nk);
+        } else if (c->kind == CV_SF) {
+            const bool all = c->sf_group == "all";
+            if (!all && !ctx->any_group)
+                throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", c->sf_group.c_str()));
+            std::vector<double> inv(nw);
+            for (int k = 0; k < nw; ++k) {
+                const double ng = all ? ng_all[k] : ng_grp[k];
+                inv[k] = ng > 0 ? 1.0 / ng : 0.0;
+            }
+            c->sf_inv.alloc(nw); c->sf_inv.upload(inv.data(), nw, st);
+            c->sf_S.alloc((size_t)std::max<int64_t>(NT, 1));
+            c->sf_part.alloc((size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
+            c->sf_tb.alloc(nw + 1); c->sf_tb.upload(ctx->tile_begin.data(), nw + 1, st);
         } else if (c->kind == CV_DIST) {
             std::vector<int64_t> off(nw);
             std::vector<double> box((size_t)nw * 3);
@@ -1035,7 +1085,7 @@ void fix_build(mdt_fix* fx) {
     r.ncv = (int)fx->cvs.size();
     // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
     for (size_t c = 0; c < fx->cvs.size(); ++c)
-        r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
+        r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST || fx->cvs[c]->kind == CV_SF) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
     r.inv_ng = fx->envs.empty() ? nullptr : fx->envs[0]->inv_ng.p;
     for (auto* e : fx->envs)
         if (e->group != fx->envs[0]->group) throw Err(MDT_ENOTSUP, "all STEINH CALs must use the same group");
@@ -1335,6 +1385,14 @@ DistDev dist_dev(mdt_fix* fx, CvH* c) {
     return d;
 }
 
+SfDev sf_dev(mdt_fix* fx, CvH* c) {
+    SfDev d{};
+    d.rc2 = c->sf_rc * c->sf_rc; d.q = c->sf_q;
+    d.S = c->sf_S.p; d.part = c->sf_part.p; d.tile_begin = c->sf_tb.p; d.inv_ng = c->sf_inv.p;
+    d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot; d.n_walkers = fx->nw();
+    return d;
+}
+
 void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
     static Timer off_timer;
     Timer& T = tm ? *tm : off_timer;
@@ -1365,6 +1423,11 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         } else if (c->kind == CV_DIST) {
             const DistDev d = dist_dev(fx, c);
             T.run("k_dist_value", st, [&] { HIPCHK(launch_dist_value(g.x4, d, st)); });
+        } else if (c->kind == CV_SF) {
+            const SfDev d = sf_dev(fx, c);
+            Geo gs = g;
+            gs.group = c->sf_group == "all" ? nullptr : ctx->group.p;
+            T.run("k_sf_value", st, [&] { HIPCHK(launch_sf_value(gs, d, st)); });
         }
     }
     T.run("k_reduce", st, [&] { HIPCHK(launch_reduce(fx->red, st)); });
@@ -1390,6 +1453,12 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         if (c->kind == CV_DIST && c->biased) {
             const DistDev d = dist_dev(fx, c);
             T.run("k_dist_force", st, [&] { HIPCHK(launch_dist_force(d, f, st)); });
+        }
+        if (c->kind == CV_SF && c->biased) {
+            const SfDev d = sf_dev(fx, c);
+            Geo gs = g;
+            gs.group = c->sf_group == "all" ? nullptr : ctx->group.p;
+            T.run("k_sf_force", st, [&] { HIPCHK(launch_sf_force(gs, d, f, st)); });
         }
     }
     // The values of unbiased local-Q CVs fused into the force pass (their tile sums
@@ -1823,7 +1892,7 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
     if (!c) throw Err(MDT_EINVAL, fmt("no CAL named '%s'", name.c_str()));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int64_t n = a.n_local, o = a.atom_off;
-    if (c->kind == CV_SK || c->kind == CV_DIST) throw Err(MDT_EINVAL, "SK and DISTANCE have no per-atom properties");
+    if (c->kind == CV_SK || c->kind == CV_DIST || c->kind == CV_SF) throw Err(MDT_EINVAL, "SK, DISTANCE and STRU_FACTOR have no per-atom export");
     EnvH* e = fx->envs[c->env];
     if (prop == "stein_q" || (prop == "local_q" && c->kind == CV_LD)) {
         std::vector<float> v((size_t)n);
@@ -1858,7 +1927,7 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
     WalkerIn& a = ctx->walker(walker);
     CvH* c = nullptr;
     for (auto* q : fx->cvs) if (q->name == (cal_name ? cal_name : "")) c = q;
-    if (!c || c->kind == CV_SK || c->kind == CV_DIST) throw Err(MDT_EINVAL, "selection export needs a STEINH CAL name");
+    if (!c || c->kind == CV_SK || c->kind == CV_DIST || c->kind == CV_SF) throw Err(MDT_EINVAL, "selection export needs a STEINH CAL name");
     EnvH* e = fx->envs[c->env];
     const int N = e->N;
     const int64_t n = a.n_local;

@@ -1796,6 +1796,100 @@ __global__ __launch_bounds__(kBlock) void k_sub_min(double* v, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// STRU_FACTOR (Debye form, CV function AVE; src/cvs/CVs_Strufactor.cu):
+// S_i = 1 + sum_j sin(q r)/(q r) s(r^2) over the list entries with
+// 1e-12 <= r^2 <= rc^2 (get_environment_Strufactor :1014-1080), summed in list
+// order before the 1 is added (structure_factor_cv_kernel :1082-1120), with the
+// switch s = 1 - ((r^2 - r_on)/(rc^2 - r_on))^3 above r_on = 0.8 rc^2;
+// CV = sum over group atoms of S_i / N_group (compute_cv_AVE :642-656).  FP64.
+// The gradient is the exact one: a pair (k, j) enters S_k when k is in the
+// group and S_j when j is, so atom k's force carries the weight
+// [k in G] + [j in G] (the reference's dcv kernel, :1136-1200, weights a pair
+// 1 + [j in G] and leaves atoms outside the group without the force).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sf_switch(double r2, double rc2, double r, double* ds) {
+    const double r_on = 0.8 * rc2;
+    if (r2 > r_on) {
+        const double u = (r2 - r_on) / (rc2 - r_on);
+        *ds = -3 * (u * u) * (2.0 * r) / (rc2 - r_on);
+        return 1.0 - u * u * u;
+    }
+    *ds = 0.0;
+    return 1.0;
+}
+
+__global__ __launch_bounds__(64) void k_sf_value(Geo g, SfDev d) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x;
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    const int a = threadIdx.x;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
+    double S = 0.0;
+    if (valid) {
+        const double4 p = g.x4[i];
+        double acc = 0.0;
+        for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
+            const double4 q = g.x4[g.idx[e]];
+            const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if ((r2 > d.rc2) || (r2 < 1e-12)) continue;
+            const double r = sqrt(r2), th = d.q * r;
+            double ds;
+            const double sw = sf_switch(r2, d.rc2, r, &ds);
+            acc += sin(th) / th * sw;
+        }
+        S = acc + 1.0;
+        d.S[i] = S;
+    }
+    const bool in_g = valid && (g.group == nullptr || g.group[i]);
+    const double part = wave_sum(in_g ? S : 0.0);
+    if (threadIdx.x == 0) d.part[t] = part;
+}
+
+// per walker: the tile sums in tile order -> CV value (one 64-lane wave)
+__global__ __launch_bounds__(64) void k_sf_sum(SfDev d) {
+    const int w = blockIdx.x;
+    const int t0 = d.tile_begin[w], t1 = d.tile_begin[w + 1];
+    double acc = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += 64) acc += d.part[t];
+    const double v = wave_sum(acc);
+    if (threadIdx.x == 0) d.direct[(int64_t)w * kMaxSlotCV + d.slot] = v * d.inv_ng[w];
+}
+
+__global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restrict__ f) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x;
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    const int a = threadIdx.x;
+    if (a >= tl.count) return;
+    const int k = tl.start + a;
+    const double W = d.wcv[(int64_t)tl.walker * kMaxSlotCV + d.slot] * d.inv_ng[tl.walker];
+    const double gk = (g.group == nullptr || g.group[k]) ? 1.0 : 0.0;
+    const double4 p = g.x4[k];
+    double F[3] = {0.0, 0.0, 0.0};
+    for (int e = g.off[k]; e < g.off[k + 1]; ++e) {
+        const int j = g.idx[e];
+        const double4 q = g.x4[j];
+        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if ((r2 > d.rc2) || (r2 < 1e-12)) continue;
+        const double r = sqrt(r2), th = d.q * r;
+        const double sn = sin(th), cs = cos(th);
+        double ds;
+        const double sw = sf_switch(r2, d.rc2, r, &ds);
+        const double fp = (cs / th - sn / (th * th)) * sw * d.q + ds * sn / th;   // d/dr of sin(q r)/(q r) s
+        const double gj = (g.group == nullptr || g.group[owner_of(g, j)]) ? 1.0 : 0.0;
+        const double c = (gk + gj) * fp / r;
+        F[0] += c * dx; F[1] += c * dy; F[2] += c * dz;
+    }
+    #pragma unroll
+    for (int q = 0; q < 3; ++q) f[3 * (int64_t)k + q] += W * F[q];
+}
+
+// ---------------------------------------------------------------------------
 // DISTANCE CV (src/cvs/CVs_DISTANCE.cu:55-123): r = |x_a2 - x_a1| with each
 // component folded to the minimum image by the half-box rule of delta_x
 // (:55-85); the bias force is -dV/dr on a1 along -d/r and on a2 along +d/r
@@ -2228,6 +2322,17 @@ hipError_t launch_dist_value(const double4* x4, const DistDev& d, hipStream_t st
 
 hipError_t launch_dist_force(const DistDev& d, double* f, hipStream_t st) {
     hipLaunchKernelGGL(k_dist_force, dim3((d.n_walkers + 63) / 64), dim3(64), 0, st, d, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_sf_value(const Geo& g, const SfDev& d, hipStream_t st) {
+    if (g.ntiles > 0) hipLaunchKernelGGL(k_sf_value, dim3(g.ntiles), dim3(64), 0, st, g, d);
+    hipLaunchKernelGGL(k_sf_sum, dim3(d.n_walkers), dim3(64), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t st) {
+    if (g.ntiles > 0) hipLaunchKernelGGL(k_sf_force, dim3(g.ntiles), dim3(64), 0, st, g, d, f);
     return hipGetLastError();
 }
 

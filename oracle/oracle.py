@@ -321,3 +321,66 @@ def distance_cv(x, L, a1, a2):
         elif d[k] < -L[k] / 2:
             d[k] += L[k]
     return float(np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])), d
+
+
+# ------------------------------------------------------------------ STRU_FACTOR (Debye, AVE)
+def _sf_switch(r2, rc2, r):
+    """s and ds/dr of src/cvs/CVs_Strufactor.cu:1112-1114,1178-1184 (r_on = 0.8 rc^2)."""
+    r_on = 0.8 * rc2
+    if r2 > r_on:
+        u = (r2 - r_on) / (rc2 - r_on)
+        return 1.0 - u * u * u, -3 * (u * u) * (2.0 * r) / (rc2 - r_on)
+    return 1.0, 0.0
+
+
+def sf_peratom(x_all, off, idx, n, rc, q):
+    """S_i = 1 + sum_j sin(q r)/(q r) s(r^2) over list entries with 1e-12 <= r^2 <= rc^2
+    (get_environment_Strufactor :1014-1080, structure_factor_cv_kernel :1082-1120)."""
+    x = np.asarray(x_all, np.float64)
+    rc2 = rc * rc
+    S = np.zeros(n)
+    for i in range(n):
+        acc = 0.0
+        for e in range(off[i], off[i + 1]):
+            j = idx[e]
+            dx, dy, dz = x[j] - x[i]
+            r2 = dx * dx + dy * dy + dz * dz
+            if r2 > rc2 or r2 < 1e-12:
+                continue
+            r = np.sqrt(r2)
+            th = q * r
+            s, _ = _sf_switch(r2, rc2, r)
+            acc += np.sin(th) / th * s
+        S[i] = acc + 1.0
+    return S
+
+
+def sf_cv(x_all, off, idx, n, rc, q, group=None):
+    """compute_cv_AVE (:642-656): sum of S_i over the group / N_group."""
+    S = sf_peratom(x_all, off, idx, n, rc, q)
+    g = np.ones(n, bool) if group is None else np.asarray(group, bool)
+    return float(np.sum(S[g]) / g.sum())
+
+
+def sf_grad(x_all, off, idx, n, own, rc, q, group=None):
+    """Exact dCV/dx of sf_cv for the n local atoms: pair (k, j) enters S_k for k in the group
+    and S_j for j in it (a ghost j counts as its owner)."""
+    x = np.asarray(x_all, np.float64)
+    g = np.ones(n, bool) if group is None else np.asarray(group, bool)
+    rc2 = rc * rc
+    out = np.zeros((n, 3))
+    for k in range(n):
+        for e in range(off[k], off[k + 1]):
+            j = idx[e]
+            d = x[j] - x[k]
+            r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+            if r2 > rc2 or r2 < 1e-12:
+                continue
+            r = np.sqrt(r2)
+            th = q * r
+            s, ds = _sf_switch(r2, rc2, r)
+            fp = (np.cos(th) / th - np.sin(th) / th ** 2) * s * q + ds * np.sin(th) / th
+            jo = j if j < n else own[j - n]
+            wt = float(g[k]) + float(g[jo])
+            out[k] -= wt * fp * d / r
+    return out / g.sum()

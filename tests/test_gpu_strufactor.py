@@ -1,0 +1,47 @@
+"""STRU_FACTOR (Debye form, src/cvs/CVs_Strufactor.cu) on the GPU against the oracle
+(run with -m gpu).
+
+Bars: the CV within 1e-12 of the oracle relative (FP64 throughout; sin/cos may differ in the
+last bit); forces within 1e-9 relative to their maximum (max |F - F_ref| / max |F_ref|).
+"""
+import numpy as np
+import pytest
+
+from helpers import fcc_system, rel_force_err
+
+pytestmark = pytest.mark.gpu
+
+RC, Q = 5.5, 2.0
+
+
+@pytest.mark.parametrize("grouped", [False, True])
+def test_stru_factor_value_and_forces(mdt, orc, grouped):
+    s = fcc_system(mdt, 5, sigma=0.15, seed=91)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    grp = (x[:n, 2] < x[:n, 2].mean()).astype(np.uint8) if grouped else None
+    cv = orc.sf_cv(x, off, idx, n, RC, Q, grp)
+    gname = "lower" if grouped else "all"
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    if grouped:
+        ctx.bind_group(0, grp)
+    fix = mdt.Fix(ctx, f"GAUSSIAN 0.05 0.005 10 PACE 0 HILLS_FILE NONE "
+                       f"CAL NAME S STRU_FACTOR {gname} cutoff_r {RC} q_factor {Q} "
+                       "CV_dim 1 SYMBOL v1 S.AVE DIM 1 0 5 500 v1 WT 0")
+    fix.init()
+    fix.add_hills(np.array([[cv + 0.02]]), np.array([0.005]), 0.05)
+    f = np.zeros((n, 3))
+    print(f"[strufactor] grouped={grouped} cv={cv:.8f}", flush=True)
+    fix.post_force(0, f)
+    assert abs(fix.cv("S.AVE") - cv) <= 1e-12 * abs(cv)
+    w = fix.cv_weight("S.AVE")
+    assert w != 0.0
+    F_ref = -w * orc.sf_grad(x, off, idx, n, own, RC, Q, grp)
+    assert rel_force_err(f, F_ref) <= 1e-9
+    if grouped:
+        # atoms outside the group that neighbour it carry force too (the exact gradient)
+        outside = ~grp.astype(bool)
+        assert np.abs(f[outside]).max() > 0.0
+    fix.close()
+    ctx.close()

@@ -545,3 +545,36 @@ def test_distance_minimum_image_and_gradient(orc):
             ym[a, k] -= h
             fd = (orc.distance_cv(yp, L, 0, 1)[0] - orc.distance_cv(ym, L, 0, 1)[0]) / (2 * h)
             assert abs(fd - sgn * d0[k] / r0) < 1e-8
+
+
+# ------------------------------------------------------------------ STRU_FACTOR (Debye)
+def test_stru_factor_dimer_closed_form(orc):
+    """Two atoms at r: S = 1 + sin(q r)/(q r) s(r^2) for each, with the cubic switch above 0.8 rc^2."""
+    q, rc = 2.0, 5.0
+    for r in (3.0, 4.8):
+        x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        off, idx = np.array([0, 1, 2]), np.array([1, 0])
+        s = 1.0 if r * r <= 0.8 * rc * rc else 1.0 - ((r * r - 0.8 * rc * rc) / (0.2 * rc * rc)) ** 3
+        S = orc.sf_peratom(x, off, idx, 2, rc, q)
+        assert np.allclose(S, 1.0 + np.sin(q * r) / (q * r) * s, rtol=0, atol=1e-15)
+
+
+def test_stru_factor_gradient_fd(orc):
+    """The exact gradient equals the FD of the CV, at group and non-group atoms (a small
+    cluster, full brute-force lists, no periodic images)."""
+    rng = np.random.default_rng(17)
+    n = 14
+    x = rng.uniform(0, 6.0, size=(n, 3))
+    off = np.arange(n + 1) * (n - 1)
+    idx = np.array([j for i in range(n) for j in range(n) if j != i])
+    grp = (np.arange(n) % 3 != 0)
+    q, rc = 1.7, 5.0
+    G = orc.sf_grad(x, off, idx, n, np.zeros(0, int), rc, q, grp)
+    h = 1e-6
+    for k in (0, 1, 5, 9):
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[k, a] += h
+            xm[k, a] -= h
+            fd = (orc.sf_cv(xp, off, idx, n, rc, q, grp) - orc.sf_cv(xm, off, idx, n, rc, q, grp)) / (2 * h)
+            assert abs(fd - G[k, a]) <= 1e-7 * max(1.0, np.abs(G).max())
