@@ -220,8 +220,17 @@ constexpr int kStMlpErr = kStStep + 1;         // learned bias: held-out max |dV
 constexpr int kStMlpOk = kStMlpErr + 1;        // learned bias: 1 if the network is in use
 constexpr int kStStride = 32;
 
+// Switching function of the reference (LINE, STEP, FERMI, TANH, RATIONAL).
+struct SwFn {
+    int32_t type;             // 0 LINE, 1 STEP, 2 FERMI, 3 TANH, 4 RATIONAL
+    int32_t n, m;
+    double r0, d0, alpha;
+};
+
 // STRU_FACTOR (Debye) CV: per-atom S_i over each atom's list within rc.
 struct SfDev {
+    int32_t count;            // 0: AVE (sum S_i / N_group); 1: COUNT (sum sw(S_i))
+    SwFn sw;
     double rc2, q;
     double* S;                // per local atom S_i
     double* part;             // per tile FP64 sum of S_i over group atoms

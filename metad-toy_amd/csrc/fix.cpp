@@ -393,6 +393,9 @@ struct CvH {
     // STRU_FACTOR: cutoff, q, group; per-atom S_i, per-tile sums, per-walker 1/N_group
     double sf_rc = 8.0, sf_q = 0.0;
     std::string sf_group;
+    bool sf_has_sw = false;
+    SwFn sf_sw{};
+    std::string func;       // the SYMBOL function bound to this CAL (AVE unless a SYMBOL says otherwise)
     DBuf<double> sf_S, sf_part, sf_inv;
     DBuf<int32_t> sf_tb;
 };
@@ -698,9 +701,32 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                             need(1, "Error: 'd_block_size' keyword requires an integer");
                             if (parse_int(argv[i + 1], "d_block_size") <= 0) throw Err(MDT_EINVAL, "Error: 'd_block_size' must be > 0");
                             i += 2;
-                        } else if (k2 == "SW_func" || k2 == "Chemical" || k2 == "Chem_ctarget" || k2 == "Chem_sigma" ||
-                                   k2 == "Chem_weight") {
-                            throw Err(MDT_ENOTSUP, fmt("STRU_FACTOR %s: the switching-function and chemical variants are not provided", k2.c_str()));
+                        } else if (k2 == "SW_func") {
+                            // SW_func <type> [r_0 v] [d_0 v] [alpha v] [n i] [m i] with the reference's
+                            // defaults per type (CVs_Strufactor.cu:73-124, zqc_switch_function.h:85-151)
+                            need(1, "Error: 'SW_func' keyword requires a type (FERMI, TANH, RATIONAL)");
+                            std::string t = argv[i + 1];
+                            SwFn w{};
+                            if (t == "STEP") { w.type = 1; w.r0 = 1.0; }
+                            else if (t == "FERMI") { w.type = 2; w.r0 = 1.0; w.alpha = 20.0; }
+                            else if (t == "TANH") { w.type = 3; w.r0 = 1.0; w.alpha = 20.0; }
+                            else if (t == "RATIONAL") { w.type = 4; w.r0 = 1.25; w.d0 = 0.0; w.n = 6; w.m = 12; }
+                            else throw Err(MDT_EINVAL, "Error: Unknown SW_func type. Choose from FERMI, TANH, RATIONAL.");
+                            i += 2;
+                            while (i < argc) {
+                                std::string p2 = argv[i];
+                                if (p2 == "r_0") { need(1, "Error: 'r_0' requires a numeric value"); w.r0 = parse_num(argv[i + 1], "r_0"); i += 2; }
+                                else if (p2 == "d_0") { need(1, "Error: 'd_0' requires a numeric value"); w.d0 = parse_num(argv[i + 1], "d_0"); i += 2; }
+                                else if (p2 == "alpha") { need(1, "Error: 'alpha' requires a numeric value"); w.alpha = parse_num(argv[i + 1], "alpha"); i += 2; }
+                                else if (p2 == "n") { need(1, "Error: 'n' requires an integer value"); w.n = (int)parse_int(argv[i + 1], "n"); i += 2; }
+                                else if (p2 == "m") { need(1, "Error: 'm' requires an integer value"); w.m = (int)parse_int(argv[i + 1], "m"); i += 2; }
+                                else break;
+                            }
+                            if (w.type == 4 && !(w.r0 != 0.0)) throw Err(MDT_EINVAL, "RATIONAL r_0 must be nonzero");
+                            c->sf_sw = w;
+                            c->sf_has_sw = true;
+                        } else if (k2 == "Chemical" || k2 == "Chem_ctarget" || k2 == "Chem_sigma" || k2 == "Chem_weight") {
+                            throw Err(MDT_ENOTSUP, fmt("STRU_FACTOR %s: the chemical variants are not provided", k2.c_str()));
                         } else {
                             break;
                         }
@@ -725,9 +751,21 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
             s.func = dot == std::string::npos ? "value" : target.substr(dot + 1);
             for (size_t k = 0; k < fx->cvs.size(); ++k) if (fx->cvs[k]->name == cname) s.cv = (int)k;
             if (s.cv < 0) throw Err(MDT_EINVAL, fmt("SYMBOL %s: CAL '%s' is not defined", s.name.c_str(), cname.c_str()));
-            // a DISTANCE CV has one value whatever the function is called (CVs_DISTANCE.cu:125-128)
-            if (s.func != "AVE" && fx->cvs[s.cv]->kind != CV_DIST)
-                throw Err(MDT_EINVAL, fmt("We can't find the func %s.", s.func.c_str()));
+            // a DISTANCE CV has one value whatever the function is called (CVs_DISTANCE.cu:125-128);
+            // STRU_FACTOR has AVE and COUNT (CVs_Strufactor.cu:616-636)
+            {
+                CvH* cc = fx->cvs[s.cv];
+                const bool ok = s.func == "AVE" || cc->kind == CV_DIST || (cc->kind == CV_SF && s.func == "COUNT");
+                if (!ok) throw Err(MDT_EINVAL, fmt("We can't find the func %s.", s.func.c_str()));
+                if (cc->kind == CV_SF) {
+                    if (!cc->func.empty() && cc->func != s.func)
+                        throw Err(MDT_ENOTSUP, fmt("CAL %s is bound as %s and %s; define one CAL per function", cc->name.c_str(),
+                                                   cc->func.c_str(), s.func.c_str()));
+                    if (s.func == "COUNT" && !cc->sf_has_sw)
+                        throw Err(MDT_EINVAL, fmt("%s.COUNT needs SW_func on the CAL", cc->name.c_str()));
+                    cc->func = s.func;
+                }
+            }
             for (auto& o : fx->syms) if (o.name == s.name) throw Err(MDT_EINVAL, fmt("SYMBOL %s defined twice", s.name.c_str()));
             if ((int)fx->syms.size() >= kMaxSym) throw Err(MDT_ENOTSUP, fmt("at most %d symbols", kMaxSym));
             fx->syms.push_back(s);
@@ -1387,6 +1425,8 @@ DistDev dist_dev(mdt_fix* fx, CvH* c) {
 
 SfDev sf_dev(mdt_fix* fx, CvH* c) {
     SfDev d{};
+    d.count = c->func == "COUNT" ? 1 : 0;
+    d.sw = c->sf_sw;
     d.rc2 = c->sf_rc * c->sf_rc; d.q = c->sf_q;
     d.S = c->sf_S.p; d.part = c->sf_part.p; d.tile_begin = c->sf_tb.p; d.inv_ng = c->sf_inv.p;
     d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot; d.n_walkers = fx->nw();
@@ -1854,7 +1894,7 @@ static int find_cv(mdt_fix* fx, const char* cal_func) {
     std::string func = dot == std::string::npos ? "AVE" : t.substr(dot + 1);
     for (size_t k = 0; k < fx->cvs.size(); ++k)
         if (fx->cvs[k]->name == name) {
-            if (func != "AVE" && fx->cvs[k]->kind != CV_DIST)
+            if (func != "AVE" && fx->cvs[k]->kind != CV_DIST && !(fx->cvs[k]->kind == CV_SF && func == fx->cvs[k]->func))
                 throw Err(MDT_EINVAL, fmt("We can't find the func %s.", func.c_str()));
             return (int)k;
         }

@@ -1807,6 +1807,62 @@ __global__ __launch_bounds__(kBlock) void k_sub_min(double* v, int n) {
 // [k in G] + [j in G] (the reference's dcv kernel, :1136-1200, weights a pair
 // 1 + [j in G] and leaves atoms outside the group without the force).
 // ---------------------------------------------------------------------------
+// The reference's switching functions (include/utils/zqc_switch_function.h:216-337),
+// f and df/dx, including its RATIONAL branches (1/(1+x^6) for n = 6, m = 12; 1/(1+x^4)
+// for 4, 8; 1/(1+x^n) for m = 2n; 1 + x^m for n = 2m; else (1-x^n)/(1-x^m)).  One
+// departure: the derivative of the general branch at x = 1 is its limit
+// n (n - m) / (2 m r_0), where the reference returns the negative of it (:325-327).
+__device__ double sw_f(const SwFn& p, double v) {
+    switch (p.type) {
+        case 1: return v < p.r0 ? 1.0 : 0.0;
+        case 2: return 1.0 / (1.0 + exp(-p.alpha * (v - p.r0)));
+        case 3: return 0.5 * (1.0 + tanh(p.alpha * (v - p.r0)));
+        case 4: {
+            const double x = (v - p.d0) / p.r0;
+            if (p.n == 6 && p.m == 12) { const double x2 = x * x, x6 = x2 * x2 * x2; return 1.0 / (1.0 + x6); }
+            if (p.n == 4 && p.m == 8) { const double x2 = x * x, x4 = x2 * x2; return 1.0 / (1.0 + x4); }
+            if (2 * p.n == p.m) return 1.0 / (1.0 + pow(x, p.n));
+            if (p.n == 2 * p.m) return 1.0 + pow(x, p.m);
+            return (1 - pow(x, p.n)) / (1 - pow(x, p.m));
+        }
+        default: return 1.0;
+    }
+}
+__device__ double sw_df(const SwFn& p, double v) {
+    switch (p.type) {
+        case 2: {
+            const double e = exp(-p.alpha * (v - p.r0)), dn = 1.0 + e;
+            return (p.alpha * e) / (dn * dn);
+        }
+        case 3: {
+            const double t = tanh(p.alpha * (v - p.r0));
+            return 0.5 * p.alpha * (1.0 - t * t);
+        }
+        case 4: {
+            const double x = (v - p.d0) / p.r0;
+            if (p.n == 6 && p.m == 12) {
+                const double x2 = x * x, x5 = x2 * x2 * x, x6 = x2 * x2 * x2;
+                return -p.n * x5 / ((1.0 + x6) * (1.0 + x6)) / p.r0;
+            }
+            if (p.n == 4 && p.m == 8) {
+                const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+                return -(double)p.n * x3 / ((1.0 + x4) * (1.0 + x4)) / p.r0;
+            }
+            if (2 * p.n == p.m) {
+                const double a = pow(x, p.n - 1), b = pow(x, p.n);
+                return -(double)p.n * a / ((1.0 + b) * (1.0 + b)) / p.r0;
+            }
+            if (p.n == 2 * p.m) return (double)p.m * pow(x, p.m - 1) / p.r0;
+            const double xm1 = pow(x, p.m - 1), xm = xm1 * x;
+            // at x = 1 the limit n (n - m) / (2 m r_0); the reference's limit has the opposite sign
+            if (fabs(1.0 - xm) < 1e-12) return 0.5 * (double)(p.n * (p.n - p.m)) / (double)p.m / p.r0;
+            const double xn1 = pow(x, p.n - 1);
+            return ((double)(p.n - p.m) * xn1 * xm + p.m * xm1 - p.n * xn1) / ((1 - xm) * (1 - xm)) / p.r0;
+        }
+        default: return 0.0;
+    }
+}
+
 __device__ __forceinline__ double sf_switch(double r2, double rc2, double r, double* ds) {
     const double r_on = 0.8 * rc2;
     if (r2 > r_on) {
@@ -1844,7 +1900,8 @@ __global__ __launch_bounds__(64) void k_sf_value(Geo g, SfDev d) {
         d.S[i] = S;
     }
     const bool in_g = valid && (g.group == nullptr || g.group[i]);
-    const double part = wave_sum(in_g ? S : 0.0);
+    const double term = d.count ? sw_f(d.sw, S) : S;             // COUNT: compute_cv_COUNT :658-674
+    const double part = wave_sum(in_g ? term : 0.0);
     if (threadIdx.x == 0) d.part[t] = part;
 }
 
@@ -1855,7 +1912,7 @@ __global__ __launch_bounds__(64) void k_sf_sum(SfDev d) {
     double acc = 0.0;
     for (int t = t0 + threadIdx.x; t < t1; t += 64) acc += d.part[t];
     const double v = wave_sum(acc);
-    if (threadIdx.x == 0) d.direct[(int64_t)w * kMaxSlotCV + d.slot] = v * d.inv_ng[w];
+    if (threadIdx.x == 0) d.direct[(int64_t)w * kMaxSlotCV + d.slot] = d.count ? v : v * d.inv_ng[w];
 }
 
 __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restrict__ f) {
@@ -1866,8 +1923,11 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
     const int a = threadIdx.x;
     if (a >= tl.count) return;
     const int k = tl.start + a;
-    const double W = d.wcv[(int64_t)tl.walker * kMaxSlotCV + d.slot] * d.inv_ng[tl.walker];
-    const double gk = (g.group == nullptr || g.group[k]) ? 1.0 : 0.0;
+    // AVE: pair weight ([k in G] + [j in G]) / N_group; COUNT: [k in G] sw'(S_k) + [j in G] sw'(S_j)
+    // (structure_factor_dcv_COUNT_kernel :1203-1271 for k in G)
+    const double wc = d.wcv[(int64_t)tl.walker * kMaxSlotCV + d.slot];
+    const double W = d.count ? wc : wc * d.inv_ng[tl.walker];
+    const double gk = (g.group == nullptr || g.group[k]) ? (d.count ? sw_df(d.sw, d.S[k]) : 1.0) : 0.0;
     const double4 p = g.x4[k];
     double F[3] = {0.0, 0.0, 0.0};
     for (int e = g.off[k]; e < g.off[k + 1]; ++e) {
@@ -1881,7 +1941,8 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
         double ds;
         const double sw = sf_switch(r2, d.rc2, r, &ds);
         const double fp = (cs / th - sn / (th * th)) * sw * d.q + ds * sn / th;   // d/dr of sin(q r)/(q r) s
-        const double gj = (g.group == nullptr || g.group[owner_of(g, j)]) ? 1.0 : 0.0;
+        const int jo = owner_of(g, j);
+        const double gj = (g.group == nullptr || g.group[jo]) ? (d.count ? sw_df(d.sw, d.S[jo]) : 1.0) : 0.0;
         const double c = (gk + gj) * fp / r;
         F[0] += c * dx; F[1] += c * dy; F[2] += c * dz;
     }

@@ -384,3 +384,80 @@ def sf_grad(x_all, off, idx, n, own, rc, q, group=None):
             wt = float(g[k]) + float(g[jo])
             out[k] -= wt * fp * d / r
     return out / g.sum()
+
+
+# ------------------------------------------------------------------ switching functions
+def sw_f(p, v):
+    """include/utils/zqc_switch_function.h:216-265 (f); p = dict(type, r0, d0, alpha, n, m),
+    type one of LINE STEP FERMI TANH RATIONAL."""
+    t = p["type"]
+    if t == "STEP":
+        return 1.0 if v < p["r0"] else 0.0
+    if t == "FERMI":
+        return 1.0 / (1.0 + np.exp(-p["alpha"] * (v - p["r0"])))
+    if t == "TANH":
+        return 0.5 * (1.0 + np.tanh(p["alpha"] * (v - p["r0"])))
+    if t == "RATIONAL":
+        n, m = p["n"], p["m"]
+        x = (v - p["d0"]) / p["r0"]
+        if 2 * n == m:
+            return 1.0 / (1.0 + x ** n)
+        if n == 2 * m:
+            return 1.0 + x ** m
+        return (1 - x ** n) / (1 - x ** m)
+    return 1.0
+
+
+def sw_df(p, v):
+    """:230-337 (df/dx): the reference's closed forms per type, except the x = 1 limit of the
+    general RATIONAL branch, taken with the sign the derivative has."""
+    t = p["type"]
+    if t == "FERMI":
+        e = np.exp(-p["alpha"] * (v - p["r0"]))
+        return p["alpha"] * e / (1.0 + e) ** 2
+    if t == "TANH":
+        return 0.5 * p["alpha"] * (1.0 - np.tanh(p["alpha"] * (v - p["r0"])) ** 2)
+    if t == "RATIONAL":
+        n, m, r0 = p["n"], p["m"], p["r0"]
+        x = (v - p["d0"]) / r0
+        if 2 * n == m:
+            return -n * x ** (n - 1) / (1.0 + x ** n) ** 2 / r0
+        if n == 2 * m:
+            return m * x ** (m - 1) / r0
+        xm = x ** m
+        if abs(1.0 - xm) < 1e-12:
+            # the limit of the derivative at x = 1 (the reference's :325-327 has the sign flipped)
+            return 0.5 * (n * (n - m)) / m / r0
+        return ((n - m) * x ** (n - 1) * xm + m * x ** (m - 1) - n * x ** (n - 1)) / (1 - xm) ** 2 / r0
+    return 0.0
+
+
+def sf_count(x_all, off, idx, n, rc, q, sw, group=None):
+    """compute_cv_COUNT (CVs_Strufactor.cu:658-674): sum over the group of sw(S_i)."""
+    S = sf_peratom(x_all, off, idx, n, rc, q)
+    g = np.ones(n, bool) if group is None else np.asarray(group, bool)
+    return float(sum(sw_f(sw, S[i]) for i in range(n) if g[i]))
+
+
+def sf_count_grad(x_all, off, idx, n, own, rc, q, sw, group=None):
+    """Exact dCOUNT/dx: pair (k, j) carries [k in G] sw'(S_k) + [j in G] sw'(S_j)."""
+    x = np.asarray(x_all, np.float64)
+    S = sf_peratom(x_all, off, idx, n, rc, q)
+    g = np.ones(n, bool) if group is None else np.asarray(group, bool)
+    rc2 = rc * rc
+    out = np.zeros((n, 3))
+    for k in range(n):
+        for e in range(off[k], off[k + 1]):
+            j = idx[e]
+            d = x[j] - x[k]
+            r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+            if r2 > rc2 or r2 < 1e-12:
+                continue
+            r = np.sqrt(r2)
+            th = q * r
+            s, ds = _sf_switch(r2, rc2, r)
+            fp = (np.cos(th) / th - np.sin(th) / th ** 2) * s * q + ds * np.sin(th) / th
+            jo = j if j < n else own[j - n]
+            wt = (sw_df(sw, S[k]) if g[k] else 0.0) + (sw_df(sw, S[jo]) if g[jo] else 0.0)
+            out[k] -= wt * fp * d / r
+    return out

@@ -45,3 +45,34 @@ def test_stru_factor_value_and_forces(mdt, orc, grouped):
         assert np.abs(f[outside]).max() > 0.0
     fix.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("swname", ["FERMI", "RATIONAL", "TANH"])
+def test_stru_factor_count_with_switching_function(mdt, orc, swname):
+    """S.COUNT = sum over the group of sw(S_i) (compute_cv_COUNT), with its exact forces."""
+    spec, sw = {"FERMI": ("FERMI r_0 2.0 alpha 4", dict(type="FERMI", r0=2.0, alpha=4.0)),
+                "RATIONAL": ("RATIONAL r_0 2.0 d_0 0.5 n 6 m 12", dict(type="RATIONAL", r0=2.0, d0=0.5, n=6, m=12)),
+                "TANH": ("TANH r_0 2.1 alpha 3", dict(type="TANH", r0=2.1, alpha=3.0))}[swname]
+    s = fcc_system(mdt, 5, sigma=0.15, seed=92)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    grp = (x[:n, 0] < x[:n, 0].mean()).astype(np.uint8)
+    cv = orc.sf_count(x, off, idx, n, RC, Q, sw, grp)
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    ctx.bind_group(0, grp)
+    fix = mdt.Fix(ctx, f"GAUSSIAN 2.0 0.005 10 PACE 0 HILLS_FILE NONE "
+                       f"CAL NAME S STRU_FACTOR left cutoff_r {RC} q_factor {Q} SW_func {spec} "
+                       "CV_dim 1 SYMBOL v1 S.COUNT DIM 1 0 600 600 v1 WT 0")
+    fix.init()
+    fix.add_hills(np.array([[cv + 1.0]]), np.array([0.005]), 2.0)
+    f = np.zeros((n, 3))
+    print(f"[strufactor count] {swname} cv={cv:.8f}", flush=True)
+    fix.post_force(0, f)
+    assert abs(fix.cv("S.COUNT") - cv) <= 1e-12 * abs(cv)
+    w = fix.cv_weight("S.COUNT")
+    assert w != 0.0
+    F_ref = -w * orc.sf_count_grad(x, off, idx, n, own, RC, Q, sw, grp)
+    assert rel_force_err(f, F_ref) <= 1e-9
+    fix.close()
+    ctx.close()

@@ -102,7 +102,8 @@ def test_fix_parser_accepts_reference_syntax(mdt):
     ("CAL NAME Q6 STEINH Q 6 all SW_FUNC_r sw", -95),
     ("CAL NAME Q6 WEIGHT_CHEM 1 2", -95),
     ("CAL NAME S STRU_FACTOR all cutoff_r 5", -22),
-    ("CAL NAME S STRU_FACTOR all q_factor 2 SW_func FERMI", -95),
+    ("CAL NAME S STRU_FACTOR all q_factor 2 Chemical", -95),
+    ("CAL NAME S STRU_FACTOR all q_factor 2 SW_func NOPE", -22),
     ("CAL NAME S STRU_FACTOR all q_factor 0", -22),
     ("CAL NAME D DISTANCE 3 3", -22),
     ("CAL NAME D DISTANCE 0 2", -22),

@@ -578,3 +578,48 @@ def test_stru_factor_gradient_fd(orc):
             xm[k, a] -= h
             fd = (orc.sf_cv(xp, off, idx, n, rc, q, grp) - orc.sf_cv(xm, off, idx, n, rc, q, grp)) / (2 * h)
             assert abs(fd - G[k, a]) <= 1e-7 * max(1.0, np.abs(G).max())
+
+
+def test_switching_functions_derivatives_and_values(orc):
+    """Each df is the FD of its f (every RATIONAL branch, the n = 2m one included, which is
+    1 + x^m in the reference); FERMI and TANH are 1/2 at r_0, RATIONAL 1 at d_0 (m = 2n)."""
+    cases = [dict(type="FERMI", r0=1.2, alpha=7.0), dict(type="TANH", r0=0.8, alpha=5.0),
+             dict(type="RATIONAL", r0=1.25, d0=0.1, n=6, m=12), dict(type="RATIONAL", r0=1.1, d0=0.0, n=4, m=8),
+             dict(type="RATIONAL", r0=1.3, d0=0.0, n=3, m=6), dict(type="RATIONAL", r0=0.9, d0=0.0, n=8, m=4),
+             dict(type="RATIONAL", r0=1.4, d0=0.2, n=5, m=9)]
+    for p in cases:
+        for v in (0.3, 0.75, 1.6, 2.2):
+            h = 1e-6
+            fd = (orc.sw_f(p, v + h) - orc.sw_f(p, v - h)) / (2 * h)
+            if p["type"] == "RATIONAL" and abs((v - p["d0"]) / p["r0"] - 1.0) < 1e-9 and 2 * p["n"] != p["m"]:
+                # x = 1 in the general branch (0/0): the limit n (n - m) / (2 m r_0), which the
+                # FD reaches only to its cancellation error
+                n_, m_ = p["n"], p["m"]
+                assert orc.sw_df(p, v) == 0.5 * (n_ * (n_ - m_)) / m_ / p["r0"]
+                assert abs(fd - orc.sw_df(p, v)) <= 1e-4 * max(1.0, abs(fd)), (p, v)
+                continue
+            assert abs(fd - orc.sw_df(p, v)) <= 1e-6 * max(1.0, abs(fd)), (p, v)
+    assert orc.sw_f(cases[0], 1.2) == 0.5 and orc.sw_f(cases[1], 0.8) == 0.5
+    assert orc.sw_f(cases[2], 0.1) == 1.0
+    assert orc.sw_f(cases[5], 0.9) == 2.0          # n = 2m: 1 + x^m at x = 1
+    assert orc.sw_f(dict(type="STEP", r0=1.0), 0.99) == 1.0 and orc.sw_f(dict(type="STEP", r0=1.0), 1.0) == 0.0
+
+
+def test_stru_factor_count_gradient_fd(orc):
+    rng = np.random.default_rng(18)
+    n = 12
+    x = rng.uniform(0, 5.5, size=(n, 3))
+    off = np.arange(n + 1) * (n - 1)
+    idx = np.array([j for i in range(n) for j in range(n) if j != i])
+    grp = (np.arange(n) % 4 != 0)
+    sw = dict(type="FERMI", r0=1.5, alpha=4.0)
+    q, rc = 1.7, 5.0
+    G = orc.sf_count_grad(x, off, idx, n, np.zeros(0, int), rc, q, sw, grp)
+    h = 1e-6
+    for k in (0, 2, 7):
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[k, a] += h
+            xm[k, a] -= h
+            fd = (orc.sf_count(xp, off, idx, n, rc, q, sw, grp) - orc.sf_count(xm, off, idx, n, rc, q, sw, grp)) / (2 * h)
+            assert abs(fd - G[k, a]) <= 1e-7 * max(1.0, np.abs(G).max())
