@@ -1595,11 +1595,13 @@ void mdt_destroy(mdt_ctx* ctx) {
     if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (mdt_fix* f : ctx->fixes) {
-        if (ctx->stream) { try { flush_hills(f); } catch (...) {} }
+        // a fix whose init failed has no device hill log to flush
+        if (ctx->stream && f->inited) { try { flush_hills(f); } catch (...) {} }
         f->ctx = nullptr;
     }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    (void)hipGetLastError();   // best-effort teardown leaves no error for the next launch to report
 }
 
 const char* mdt_last_error(const mdt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -1732,6 +1734,7 @@ void mdt_fix_destroy(mdt_fix* fx) {
     }
     if (fx->device >= 0) (void)hipSetDevice(fx->device);
     delete fx;
+    (void)hipGetLastError();
 }
 
 int mdt_fix_init(mdt_fix* fx) {

@@ -85,3 +85,16 @@ def test_distance_in_walkers_and_graph_runs(mdt, orc):
         assert np.abs(F[2] + F[39]).max() <= 1e-15 * max(1.0, np.abs(F).max())
     fix.close()
     ctx.close()
+
+
+def test_distance_atom_beyond_a_walker_is_refused(mdt):
+    systems = [fcc_system(mdt, 4, sigma=0.1, seed=85), fcc_system(mdt, 3, sigma=0.1, seed=86)]   # 256, 108 atoms
+    ctx = mdt.Context(0, 2)
+    for w, sm in enumerate(systems):
+        ctx.bind_system(w, sm)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.05 0.005 10 PACE 1 HILLS_FILE NONE CAL NAME D DISTANCE 3 200 "
+                       "CV_dim 1 SYMBOL v1 D.AVE DIM 1 0 20 400 v1 WT 0")
+    with pytest.raises(mdt.MdtError) as e:
+        fix.init()
+    assert "beyond walker 1" in str(e.value)
+    ctx.close()

@@ -80,3 +80,10 @@ def test_fes_command_line(mdt, orc, tmp_path, monkeypatch):
     assert np.abs(m1[:, 0] - x).max() <= 1e-8 * np.abs(x).max()
     assert np.abs(m1[:, 1] - orc.fes_marginal(ref, KBT, 1, d[1])).max() <= 1e-7 * KBT
     assert os.path.exists(tmp_path / "Marginal_FES_CV2.dat")
+
+
+def test_fes_without_hills(mdt):
+    ctx = mdt.Context(0, 1)
+    F = mdt.fes_sum(ctx, np.zeros((0, 4)), np.linspace(0.0, 1.0, 11))
+    assert F.shape == (11,) and not F.any()
+    ctx.close()

@@ -94,3 +94,15 @@ def test_device_build_at_256k_atoms(mdt):
     assert na == s.n_all and nz == host[2].size
     assert np.array_equal(dev[1], host[1]) and np.array_equal(dev[2], host[2]) and np.array_equal(dev[3], host[3])
     ctx.close()
+
+
+def test_device_build_of_an_empty_box(mdt):
+    """No local atoms: no ghosts, offsets [0], no entries."""
+    ctx = mdt.Context(0, 1)
+    na, nz = ctx.build_neighbors(np.zeros((0, 3)), [0, 0, 0], [20.0, 20.0, 20.0], 5.9)
+    assert (na, nz) == (0, 0)
+    x, off, idx, own = ctx.fetch_neighbors()
+    assert x.shape == (0, 3) and off.tolist() == [0] and idx.size == 0 and own.size == 0
+    with pytest.raises(mdt.MdtError):
+        ctx.build_neighbors(np.zeros((4, 3)), [0, 0, 0], [10.0, 20.0, 20.0], 5.9)   # edge < 2 rg
+    ctx.close()
