@@ -163,8 +163,24 @@ def run_c2(mdt, args):
                        "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms,
            "step_roofline": step_roof,
            "steps_per_graph_launch": 8,
-           "one_launch_per_step": {"value": n * args.steps / dt1, "ms_per_step": dt1 * 1e3 / args.steps}}
+           "one_launch_per_step": {"value": n * args.steps / dt1, "ms_per_step": dt1 * 1e3 / args.steps},
+           "neighbor_build": neighbor_build_ms(mdt, system)}
     return line(value, args.steps, args.warmup, dt * 1e3 / args.steps, "f32 pair math, f64 sums/grid", cfg, roof, cpu)
+
+
+def neighbor_build_ms(mdt, system, reps=5):
+    """Ghost images and full lists of the same system rebuilt on the device (mdt_neighbor_build,
+    outside the timed step: an MD loop rebuilds lists every few steps), mean of reps calls."""
+    x = system.arrays()[0][: system.n_local]
+    lo, hi = system.box()
+    ctx = mdt.Context(0, 1)
+    na, nz = ctx.build_neighbors(x, lo, hi, RG)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.build_neighbors(x, lo, hi, RG)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    ctx.close()
+    return {"ms": ms, "atoms_with_ghosts": na, "list_entries": nz, "list_cutoff": RG}
 
 
 # ------------------------------------------------------------------ config 1

@@ -336,6 +336,7 @@ struct SkDev {
     double* amp;              // per walker [nk][2]
     SkK* kpack;               // per walker [nk]
     float* ampf;              // per walker [nk][2] scaled for the gradient
+    double* sterm;            // per walker [nk] |A(k)|^2 / N
     const int32_t* chunk_begin;  // per walker first chunk (n_walkers + 1)
     double* direct;           // per walker [kMaxSlotCV]
     int32_t slot;             // direct slot of SK.AVE

@@ -381,7 +381,7 @@ struct CvH {
     DBuf<float> lq, bq;
     // SK
     std::vector<double> kvec;   // nk*3
-    DBuf<double> kd, krev, apart, amp;
+    DBuf<double> kd, krev, apart, amp, sterm;
     DBuf<SkK> kpack;
     DBuf<float> ampf;
     DBuf<int32_t> chunk, chunk_begin, natoms, abegin;
@@ -1025,6 +1025,7 @@ void fix_build(mdt_fix* fx) {
             c->abegin.alloc(nw); c->abegin.upload(ab.data(), nw, st);
             c->apart.alloc((size_t)c->nchunk * nk * 2);
             c->amp.alloc((size_t)nw * nk * 2);
+            c->sterm.alloc((size_t)nw * nk);
             c->kpack.alloc((size_t)nw * nk);
         } else if (c->kind == CV_SF) {
             const bool all = c->sf_group == "all";
@@ -1457,6 +1458,7 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             s.nk = (int)(c->kvec.size() / 3);
             s.k = c->krev.p; s.kvec = c->kd.p; s.nchunk = c->nchunk; s.chunk = c->chunk.p;
             s.apart = c->apart.p; s.amp = c->amp.p; s.kpack = c->kpack.p; s.chunk_begin = c->chunk_begin.p;
+            s.sterm = c->sterm.p;
             s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
             s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             T.run("k_sk_amp", st, [&] { HIPCHK(launch_sk_amp(g, s, st)); });

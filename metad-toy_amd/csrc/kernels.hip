@@ -2085,27 +2085,45 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
 }
 
 // per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S, packed k data
-__global__ __launch_bounds__(kBlock) void k_sk_sum(SkDev sk) {
-    __shared__ double sh[4];
-    const int w = blockIdx.x;
+// Amplitudes of walker w = blockIdx.y: one thread per k sums that k's chunk
+// partials in chunk order (eight loads in flight), writes A(k), the packed k
+// record of the gradient pass and the term |A(k)|^2 / N.
+__global__ __launch_bounds__(64) void k_sk_fold(SkDev sk) {
+    const int w = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= sk.nk) return;
     const int c0 = sk.chunk_begin[w], c1 = sk.chunk_begin[w + 1];
     const double N = (double)sk.n_atoms[w];
-    double v[1] = {0.0};
-    for (int q = threadIdx.x; q < sk.nk; q += blockDim.x) {
-        double re = 0, im = 0;
-        for (int c = c0; c < c1; ++c) {
-            re += sk.apart[((int64_t)c * sk.nk + q) * 2];
-            im += sk.apart[((int64_t)c * sk.nk + q) * 2 + 1];
-        }
-        sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
-        sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
-        SkK kp;
-        for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
-        kp.ar = (float)re; kp.ai = (float)im;
-        kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
-        sk.kpack[(int64_t)w * sk.nk + q] = kp;
-        v[0] += (re * re + im * im) / N;
+    double re = 0, im = 0;
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {
+        double2 v[8];
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = reinterpret_cast<const double2*>(sk.apart)[(int64_t)(c + u) * sk.nk + q];
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) { re += v[u].x; im += v[u].y; }
     }
+    for (; c < c1; ++c) {
+        const double2 v = reinterpret_cast<const double2*>(sk.apart)[(int64_t)c * sk.nk + q];
+        re += v.x; im += v.y;
+    }
+    sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
+    sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
+    SkK kp;
+    for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
+    kp.ar = (float)re; kp.ai = (float)im;
+    kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
+    sk.kpack[(int64_t)w * sk.nk + q] = kp;
+    sk.sterm[(int64_t)w * sk.nk + q] = (re * re + im * im) / N;
+}
+
+// SK.AVE of walker w = blockIdx.x: the mean over k of |A(k)|^2 / N, a fixed-order
+// FP64 sum (thread-strided, then the block tree)
+__global__ __launch_bounds__(kBlock) void k_sk_cv(SkDev sk) {
+    __shared__ double sh[4];
+    const int w = blockIdx.x;
+    double v[1] = {0.0};
+    for (int q = threadIdx.x; q < sk.nk; q += blockDim.x) v[0] += sk.sterm[(int64_t)w * sk.nk + q];
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) sk.direct[(int64_t)w * kMaxSlotCV + sk.slot] = v[0] / sk.nk;
 }
@@ -2346,7 +2364,10 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
     hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sk_sum, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
+    hipLaunchKernelGGL(k_sk_fold, dim3((s.nk + 63) / 64, s.n_walkers), dim3(64), 0, st, s);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sk_cv, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
     return hipGetLastError();
 }
 
