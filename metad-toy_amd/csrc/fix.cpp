@@ -1005,7 +1005,13 @@ void fix_build(mdt_fix* fx) {
             c->kd.alloc(c->kvec.size()); c->kd.upload(c->kvec.data(), c->kvec.size(), st);
             c->krev.alloc(krev.size()); c->krev.upload(krev.data(), krev.size(), st);
             std::vector<int32_t> ch, cb(nw + 1, 0), na(nw), ab(nw);
-            const int CH = 16384;   // atoms per chunk of the amplitude pass
+            // atoms per chunk of the amplitude pass: about 2,048 workgroups (k tiles x chunks)
+            // whatever the k count, between 256 and 16,384 atoms
+            int64_t tot_atoms = 0;
+            for (int k = 0; k < nw; ++k) tot_atoms += ctx->w[k].n_local;
+            const int64_t ktiles = (nk + 7) / 8;
+            int CH = 256;
+            while (CH < 16384 && (tot_atoms + CH - 1) / CH * ktiles > 2048) CH *= 2;
             c->max_atoms = 0;
             for (int k = 0; k < nw; ++k) {
                 WalkerIn& a = ctx->w[k];

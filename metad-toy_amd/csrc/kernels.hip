@@ -1256,6 +1256,13 @@ __device__ void grid_deposit_block(const ReduceDev& r, double* grid, const doubl
     }
 }
 
+// A wave-uniform FP64 value moved to scalar registers (both halves from the first lane).
+__device__ __forceinline__ double uniform_f64(double v) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)__double_as_longlong(v));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)(__double_as_longlong(v) >> 32));
+    return __longlong_as_double(((long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 // Exact FP64 hills sum at s, block-cooperative; V and dV valid on thread 0.
 template <int NW>
 __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh,
@@ -1263,18 +1270,39 @@ __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, d
     const double* H = r.hills + (int64_t)w * r.hills_cap * (r.D + 3);
     const int64_t nh = r.nhills[w];
     double acc[4] = {0, 0, 0, 0};
-    for (int64_t h = h_first + threadIdx.x; h < nh; h += blockDim.x) {
-        const double* rec = H + h * (r.D + 3);
-        double ex = 0, dd[kMaxDim] = {0, 0, 0};
+    // a thread's hills h, h + B, h + 2B, ... in that order, four records loaded ahead
+    // of their arithmetic (the loop is latency-bound one record at a time)
+    constexpr int HU = 4;
+    const int64_t B = blockDim.x;
+    // 1/(2 sigma^2) per dimension, once (FP64 divisions cost more than the exp); twice
+    // it is 1/sigma^2 exactly.  The values are uniform: held in scalar registers.
+    double i2s2[kMaxDim] = {0, 0, 0};
+    #pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) if (d < r.D) i2s2[d] = uniform_f64(0.5 / (r.sigma[d] * r.sigma[d]));
+    for (int64_t h = h_first + threadIdx.x; h < nh; h += HU * B) {
+        double c[HU][kMaxDim], hgt[HU];
         #pragma unroll
-        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) {
-            dd[d] = s[d] - rec[1 + d];
-            ex += dd[d] * dd[d] / (2.0 * r.sigma[d] * r.sigma[d]);
+        for (int u = 0; u < HU; ++u) {
+            const int64_t hu = h + u * B < nh ? h + u * B : h;
+            const double* rec = H + hu * (r.D + 3);
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) c[u][d] = d < r.D ? rec[1 + d] : 0.0;
+            hgt[u] = rec[1 + r.D];
         }
-        double t = rec[1 + r.D] * exp(-ex);
-        acc[0] += t;
         #pragma unroll
-        for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc[1 + d] -= t * dd[d] / (r.sigma[d] * r.sigma[d]);
+        for (int u = 0; u < HU; ++u) {
+            if (h + u * B >= nh) break;
+            double ex = 0, dd[kMaxDim] = {0, 0, 0};
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) {
+                dd[d] = s[d] - c[u][d];
+                ex += dd[d] * dd[d] * i2s2[d];
+            }
+            double t = hgt[u] * exp(-ex);
+            acc[0] += t;
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc[1 + d] -= t * dd[d] * (2.0 * i2s2[d]);
+        }
     }
     block_sum<4, NW>(acc, sh);
     if (threadIdx.x == 0) {
@@ -2022,7 +2050,6 @@ __global__ void k_fill_f(float* p, int64_t n, float v) {
 // FP64 phase reduces to its fractional part before the FP32 sin/cos.
 // ---------------------------------------------------------------------------
 constexpr int kSkTile = 8;    // k vectors per workgroup in the amplitude pass
-constexpr int kSkChunk = 16384;   // atoms per chunk (host side uses the same value)
 
 // sin and cos of 2 pi ph (ph in revolutions): FP64 reduction to [-0.5, 0.5], then the
 // hardware sine/cosine, which take their argument in revolutions
@@ -2085,28 +2112,23 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
 }
 
 // per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S, packed k data
-// Amplitudes of walker w = blockIdx.y: one thread per k sums that k's chunk
-// partials in chunk order (eight loads in flight), writes A(k), the packed k
-// record of the gradient pass and the term |A(k)|^2 / N.
+// Amplitudes of walker w = blockIdx.y, k = blockIdx.x: one wave sums that k's
+// chunk partials, lane l taking chunks l, l+64, ... in order, then the fixed-order
+// wave sum; lane 0 writes A(k), the packed k record of the gradient pass and the
+// term |A(k)|^2 / N.
 __global__ __launch_bounds__(64) void k_sk_fold(SkDev sk) {
     const int w = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= sk.nk) return;
+    const int q = blockIdx.x;
     const int c0 = sk.chunk_begin[w], c1 = sk.chunk_begin[w + 1];
-    const double N = (double)sk.n_atoms[w];
     double re = 0, im = 0;
-    int c = c0;
-    for (; c + 8 <= c1; c += 8) {
-        double2 v[8];
-        #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = reinterpret_cast<const double2*>(sk.apart)[(int64_t)(c + u) * sk.nk + q];
-        #pragma unroll
-        for (int u = 0; u < 8; ++u) { re += v[u].x; im += v[u].y; }
-    }
-    for (; c < c1; ++c) {
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += 64) {
         const double2 v = reinterpret_cast<const double2*>(sk.apart)[(int64_t)c * sk.nk + q];
         re += v.x; im += v.y;
     }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (threadIdx.x != 0) return;
+    const double N = (double)sk.n_atoms[w];
     sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
     sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
     SkK kp;
@@ -2364,7 +2386,7 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
     hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sk_fold, dim3((s.nk + 63) / 64, s.n_walkers), dim3(64), 0, st, s);
+    hipLaunchKernelGGL(k_sk_fold, dim3(s.nk, s.n_walkers), dim3(64), 0, st, s);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sk_cv, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
