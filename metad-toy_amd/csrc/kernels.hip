@@ -2112,22 +2112,23 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
 }
 
 // per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S, packed k data
-// Amplitudes of walker w = blockIdx.y, k = blockIdx.x: one wave sums that k's
-// chunk partials, lane l taking chunks l, l+64, ... in order, then the fixed-order
-// wave sum; lane 0 writes A(k), the packed k record of the gradient pass and the
-// term |A(k)|^2 / N.
-__global__ __launch_bounds__(64) void k_sk_fold(SkDev sk) {
+// Amplitudes of walker w = blockIdx.y, k = blockIdx.x: a workgroup sums that k's
+// chunk partials, thread t taking chunks t, t+256, ... in order, then the
+// fixed-order block sum; thread 0 writes A(k), the packed k record of the gradient
+// pass and the term |A(k)|^2 / N.
+__global__ __launch_bounds__(kBlock) void k_sk_fold(SkDev sk) {
+    __shared__ double sh[2 * (kBlock / 64)];
     const int w = blockIdx.y;
     const int q = blockIdx.x;
     const int c0 = sk.chunk_begin[w], c1 = sk.chunk_begin[w + 1];
-    double re = 0, im = 0;
-    for (int c = c0 + (int)threadIdx.x; c < c1; c += 64) {
+    double v2[2] = {0.0, 0.0};
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += kBlock) {
         const double2 v = reinterpret_cast<const double2*>(sk.apart)[(int64_t)c * sk.nk + q];
-        re += v.x; im += v.y;
+        v2[0] += v.x; v2[1] += v.y;
     }
-    re = wave_sum(re);
-    im = wave_sum(im);
+    block_sum<2>(v2, sh);
     if (threadIdx.x != 0) return;
+    const double re = v2[0], im = v2[1];
     const double N = (double)sk.n_atoms[w];
     sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
     sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
@@ -2386,7 +2387,7 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
     hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sk_fold, dim3(s.nk, s.n_walkers), dim3(64), 0, st, s);
+    hipLaunchKernelGGL(k_sk_fold, dim3(s.nk, s.n_walkers), dim3(kBlock), 0, st, s);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sk_cv, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
