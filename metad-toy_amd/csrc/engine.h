@@ -244,6 +244,31 @@ struct SfDev {
 hipError_t launch_sf_value(const Geo& g, const SfDev& d, hipStream_t st);
 hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t st);
 
+// STEINH L (switching-weighted local Q of degree l): per-atom state, per-tile sums
+// and per-entry pair gradients.  Arrays of 2(l+1) doubles hold Re/Im for m = 0..l.
+struct LqDev {
+    int32_t l;
+    int32_t mode;             // 0 MEAN_SOLID, 1 NSOLID, 2 FRAC
+    SwFn swr, swcv;           // sigma(r) on list entries; f on LQ_l
+    double eps;               // entries with sigma < eps do not count
+    double* S;                // per local atom sum of sigma (0: no entry counts)
+    double* q;                // per local atom q_lm
+    double* LQ;               // per local atom LQ_lm (0 outside the group)
+    double* ql;               // per local atom LQ_l
+    double* K;                // per local atom W dCV/dLQ_lm / (S + 1)
+    double* g;                // per CSR entry: d(W CV)/d(pair vector), 3 doubles
+    double* part;             // per tile (sum q f, sum f)
+    double* sums;             // per walker (sum f, CV)
+    const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)
+    const double* ng;         // per walker N_group
+    double* direct;           // per walker [kMaxSlotCV] CV values
+    const double* wcv;        // per walker [kMaxSlotCV] chain-rule weights
+    int32_t slot;
+    int32_t n_walkers;
+};
+hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st);
+hipError_t launch_lq_force(const Geo& g, const LqDev& d, double* f, hipStream_t st);
+
 // DISTANCE CV between two atoms of each walker (one thread per walker).
 struct DistDev {
     int32_t n_walkers;

@@ -353,7 +353,7 @@ void mdt_ctx::commit() {
 // ---------------------------------------------------------------------------
 namespace {
 
-enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2, CV_DIST = 3, CV_SF = 4 };
+enum CvKind { CV_Q = 0, CV_LD = 1, CV_SK = 2, CV_DIST = 3, CV_SF = 4, CV_LQ = 5 };
 
 struct EnvH {
     double rc = 0;
@@ -398,7 +398,22 @@ struct CvH {
     std::string func;       // the SYMBOL function bound to this CAL (AVE unless a SYMBOL says otherwise)
     DBuf<double> sf_S, sf_part, sf_inv;
     DBuf<int32_t> sf_tb;
+    // STEINH L: sigma(r), f(LQ_l), eps, group; per-atom state, per-entry pair gradients
+    SwFn lq_swr{}, lq_swcv{};
+    double lq_eps = 1e-6;
+    std::string lq_group;
+    DBuf<double> lq_S, lq_q, lq_LQ, lq_ql, lq_K, lq_g, lq_part, lq_sums, lq_ng;
+    DBuf<int32_t> lq_tb;
 };
+
+// STEINH L functions (CVs_Steinhardt_Local.cu:631-690): MEAN_SOLID (alias AVE), NSOLID
+// (alias SW_FUNC), FRAC; returns the mode (0, 1, 2) or -1
+int lq_mode(const std::string& f) {
+    if (f == "MEAN_SOLID" || f == "AVE") return 0;
+    if (f == "NSOLID" || f == "SW_FUNC") return 1;
+    if (f == "FRAC") return 2;
+    return -1;
+}
 
 struct SymH {
     std::string name;
@@ -431,6 +446,7 @@ struct mdt_fix {
     std::vector<EnvH*> envs;
     std::vector<CvH*> cvs;
     std::vector<SymH> syms;
+    std::map<std::string, SwFn> swreg;          // CAL NAME <name> SW_FUNC ... (sw_registry)
     // device state
     bool inited = false;
     uint64_t built_gen = 0;
@@ -573,9 +589,42 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
             need(3, "CAL requires NAME <name> <type> ...");
             if (strcmp(argv[i + 1], "NAME")) throw Err(MDT_EINVAL, "Error: CAL requires NAME keyword.");
             std::string name = argv[i + 2], type = argv[i + 3];
-            for (auto* c : fx->cvs)
-                if (c->name == name) throw Err(MDT_EINVAL, fmt("Error: CAL name '%s' is duplicated.", name.c_str()));
+            if (type == "SW_FUNC") {
+                if (fx->swreg.count(name)) throw Err(MDT_EINVAL, fmt("Error: CAL name '%s' is duplicated.", name.c_str()));
+            } else {
+                for (auto* c : fx->cvs)
+                    if (c->name == name) throw Err(MDT_EINVAL, fmt("Error: CAL name '%s' is duplicated.", name.c_str()));
+            }
             i += 4;
+            // <type> [r_0 v] [d_0 v] [alpha v] [n i] [m i] with the reference's defaults per
+            // type (include/utils/zqc_switch_function.h:85-151); i at the type on entry
+            auto parse_swspec = [&]() {
+                if (i >= argc) throw Err(MDT_EINVAL, "Error: SW_FUNC command requires a type (LINE, STEP, FERMI, TANH, RATIONAL).");
+                std::string t = argv[i];
+                SwFn w{};
+                if (t == "STEP") { w.type = 1; w.r0 = 1.0; }
+                else if (t == "FERMI") { w.type = 2; w.r0 = 1.0; w.alpha = 20.0; }
+                else if (t == "TANH") { w.type = 3; w.r0 = 1.0; w.alpha = 20.0; }
+                else if (t == "RATIONAL") { w.type = 4; w.r0 = 1.25; w.d0 = 0.0; w.n = 6; w.m = 12; }
+                else throw Err(MDT_EINVAL, "Error: Unknown SW_func type. Choose from FERMI, TANH, RATIONAL.");
+                i += 1;
+                while (i < argc) {
+                    std::string p2 = argv[i];
+                    if (p2 == "r_0") { need(1, "Error: 'r_0' requires a numeric value"); w.r0 = parse_num(argv[i + 1], "r_0"); i += 2; }
+                    else if (p2 == "d_0") { need(1, "Error: 'd_0' requires a numeric value"); w.d0 = parse_num(argv[i + 1], "d_0"); i += 2; }
+                    else if (p2 == "alpha") { need(1, "Error: 'alpha' requires a numeric value"); w.alpha = parse_num(argv[i + 1], "alpha"); i += 2; }
+                    else if (p2 == "n") { need(1, "Error: 'n' requires an integer value"); w.n = (int)parse_int(argv[i + 1], "n"); i += 2; }
+                    else if (p2 == "m") { need(1, "Error: 'm' requires an integer value"); w.m = (int)parse_int(argv[i + 1], "m"); i += 2; }
+                    else break;
+                }
+                if (w.type == 4 && !(w.r0 != 0.0)) throw Err(MDT_EINVAL, "RATIONAL r_0 must be nonzero");
+                return w;
+            };
+            auto find_sw = [&](const char* nm) {
+                auto it = fx->swreg.find(nm);
+                if (it == fx->swreg.end()) throw Err(MDT_EINVAL, fmt("Error: SwitchFunction named %s not found in registry!", nm));
+                return it->second;
+            };
             if (type == "STEINH") {
                 if (i + 2 >= argc)
                     throw Err(MDT_EINVAL, "Error: STEINH command requires \"STEINH <Q/L/LD> <3/4/6> <group> \".");
@@ -588,6 +637,7 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 if (l == 8 || l == 12) throw Err(MDT_ENOTSUP, fmt("STEINH order %d is accepted by the reference parser but has no implementation there either", l));
                 double rc = 4.0, eps = 1e-6;
                 int N = 12;
+                SwFn swr{}, swcv{};          // LINE unless named (get_default_step is LINE)
                 while (i < argc) {
                     std::string k2 = argv[i];
                     if (k2 == "cutoff_r") { need(1, "'cutoff_r' keyword requires a value"); rc = parse_num(argv[i + 1], "cutoff_r"); i += 2; }
@@ -596,12 +646,20 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                     else if (k2 == "d_block_size") { need(1, "'d_block_size' keyword requires an integer");
                         if (parse_int(argv[i + 1], "d_block_size") <= 0) throw Err(MDT_EINVAL, "Error: 'd_block_size' must be > 0");
                         i += 2; }
-                    else if (k2 == "SW_FUNC_r" || k2 == "SW_FUNC_cv")
-                        throw Err(MDT_ENOTSUP, fmt("%s: switching-function STEINH variants are not provided", k2.c_str()));
+                    else if (k2 == "SW_FUNC_r") { need(1, "Error: 'SW_FUNC_r' keyword requires a value"); swr = find_sw(argv[i + 1]); i += 2; }
+                    else if (k2 == "SW_FUNC_cv") { need(1, "Error: 'SW_FUNC_cv' keyword requires a value"); swcv = find_sw(argv[i + 1]); i += 2; }
                     else break;
                 }
-                (void)eps;
-                if (qt == "L") throw Err(MDT_ENOTSUP, "STEINH L (switching-weighted local Q) is not provided; use STEINH LD (Lechner-Dellago over the N nearest)");
+                if (qt == "L") {
+                    // sigma applies to every list entry; cutoff_r and cutoff_Natoms only size the
+                    // reference's own neighbour request (CVs_Steinhardt.cu:125-148)
+                    if (!(eps >= 0)) throw Err(MDT_EINVAL, "cutoff_eps must be >= 0");
+                    CvH* c = new CvH();
+                    c->name = name; c->kind = CV_LQ; c->l = l; c->lq_group = group;
+                    c->lq_swr = swr; c->lq_swcv = swcv; c->lq_eps = eps;
+                    fx->cvs.push_back(c);
+                    continue;
+                }
                 if (qt != "Q" && qt != "LD") throw Err(MDT_EINVAL, "Error: Steinhardt type must be 'Q', 'L' or 'LD'.");
                 if (N < 1 || N > kMaxSlots) throw Err(MDT_ENOTSUP, fmt("cutoff_Natoms %d outside [1,%d]", N, kMaxSlots));
                 if (!(rc > 0)) throw Err(MDT_EINVAL, "cutoff_r must be positive");
@@ -669,7 +727,8 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 if (c->kvec.empty()) { delete c; throw Err(MDT_EINVAL, "SK needs at least one k vector"); }
                 fx->cvs.push_back(c);
             } else if (type == "SW_FUNC") {
-                throw Err(MDT_ENOTSUP, "CAL ... SW_FUNC: switching functions are not provided");
+                // CAL NAME <name> SW_FUNC <type> ... (fix_crystallize.cu:144-151)
+                fx->swreg[name] = parse_swspec();
             } else if (type == "DISTANCE") {
                 // DISTANCE id1 id2 (CVs_DISTANCE.cu:17-30): 1-based atom IDs, the same in every walker
                 if (i + 1 >= argc) throw Err(MDT_EINVAL, "Error: DISTANCE command requires 2 atom IDs.");
@@ -702,28 +761,10 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                             if (parse_int(argv[i + 1], "d_block_size") <= 0) throw Err(MDT_EINVAL, "Error: 'd_block_size' must be > 0");
                             i += 2;
                         } else if (k2 == "SW_func") {
-                            // SW_func <type> [r_0 v] [d_0 v] [alpha v] [n i] [m i] with the reference's
-                            // defaults per type (CVs_Strufactor.cu:73-124, zqc_switch_function.h:85-151)
+                            // SW_func <type> [r_0 v] [d_0 v] [alpha v] [n i] [m i] (CVs_Strufactor.cu:73-124)
                             need(1, "Error: 'SW_func' keyword requires a type (FERMI, TANH, RATIONAL)");
-                            std::string t = argv[i + 1];
-                            SwFn w{};
-                            if (t == "STEP") { w.type = 1; w.r0 = 1.0; }
-                            else if (t == "FERMI") { w.type = 2; w.r0 = 1.0; w.alpha = 20.0; }
-                            else if (t == "TANH") { w.type = 3; w.r0 = 1.0; w.alpha = 20.0; }
-                            else if (t == "RATIONAL") { w.type = 4; w.r0 = 1.25; w.d0 = 0.0; w.n = 6; w.m = 12; }
-                            else throw Err(MDT_EINVAL, "Error: Unknown SW_func type. Choose from FERMI, TANH, RATIONAL.");
-                            i += 2;
-                            while (i < argc) {
-                                std::string p2 = argv[i];
-                                if (p2 == "r_0") { need(1, "Error: 'r_0' requires a numeric value"); w.r0 = parse_num(argv[i + 1], "r_0"); i += 2; }
-                                else if (p2 == "d_0") { need(1, "Error: 'd_0' requires a numeric value"); w.d0 = parse_num(argv[i + 1], "d_0"); i += 2; }
-                                else if (p2 == "alpha") { need(1, "Error: 'alpha' requires a numeric value"); w.alpha = parse_num(argv[i + 1], "alpha"); i += 2; }
-                                else if (p2 == "n") { need(1, "Error: 'n' requires an integer value"); w.n = (int)parse_int(argv[i + 1], "n"); i += 2; }
-                                else if (p2 == "m") { need(1, "Error: 'm' requires an integer value"); w.m = (int)parse_int(argv[i + 1], "m"); i += 2; }
-                                else break;
-                            }
-                            if (w.type == 4 && !(w.r0 != 0.0)) throw Err(MDT_EINVAL, "RATIONAL r_0 must be nonzero");
-                            c->sf_sw = w;
+                            i += 1;
+                            c->sf_sw = parse_swspec();
                             c->sf_has_sw = true;
                         } else if (k2 == "Chemical" || k2 == "Chem_ctarget" || k2 == "Chem_sigma" || k2 == "Chem_weight") {
                             throw Err(MDT_ENOTSUP, fmt("STRU_FACTOR %s: the chemical variants are not provided", k2.c_str()));
@@ -755,7 +796,15 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
             // STRU_FACTOR has AVE and COUNT (CVs_Strufactor.cu:616-636)
             {
                 CvH* cc = fx->cvs[s.cv];
-                const bool ok = s.func == "AVE" || cc->kind == CV_DIST || (cc->kind == CV_SF && s.func == "COUNT");
+                if (cc->kind == CV_LQ) {
+                    if (lq_mode(s.func) < 0)
+                        throw Err(MDT_EINVAL, fmt("Unknown STEIN_LocalQL func %s. Use MEAN_SOLID (alias AVE), NSOLID (alias SW_FUNC), or FRAC.", s.func.c_str()));
+                    if (!cc->func.empty() && lq_mode(cc->func) != lq_mode(s.func))
+                        throw Err(MDT_ENOTSUP, fmt("CAL %s is bound as %s and %s; define one CAL per function", cc->name.c_str(),
+                                                   cc->func.c_str(), s.func.c_str()));
+                    cc->func = s.func;
+                }
+                const bool ok = s.func == "AVE" || cc->kind == CV_DIST || (cc->kind == CV_SF && s.func == "COUNT") || cc->kind == CV_LQ;
                 if (!ok) throw Err(MDT_EINVAL, fmt("We can't find the func %s.", s.func.c_str()));
                 if (cc->kind == CV_SF) {
                     if (!cc->func.empty() && cc->func != s.func)
@@ -1046,6 +1095,20 @@ void fix_build(mdt_fix* fx) {
             c->sf_S.alloc((size_t)std::max<int64_t>(NT, 1));
             c->sf_part.alloc((size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
             c->sf_tb.alloc(nw + 1); c->sf_tb.upload(ctx->tile_begin.data(), nw + 1, st);
+        } else if (c->kind == CV_LQ) {
+            const bool all = c->lq_group == "all";
+            if (!all && !ctx->any_group)
+                throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", c->lq_group.c_str()));
+            std::vector<double> ng(nw);
+            for (int k = 0; k < nw; ++k) ng[k] = all ? ng_all[k] : ng_grp[k];
+            const size_t LS = 2 * (size_t)(c->l + 1), n1 = (size_t)std::max<int64_t>(NT, 1);
+            c->lq_ng.alloc(nw); c->lq_ng.upload(ng.data(), nw, st);
+            c->lq_S.alloc(n1); c->lq_ql.alloc(n1);
+            c->lq_q.alloc(n1 * LS); c->lq_LQ.alloc(n1 * LS); c->lq_K.alloc(n1 * LS);
+            c->lq_g.alloc((size_t)std::max<int64_t>(ctx->total_nnz, 1) * 3);
+            c->lq_part.alloc(2 * (size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
+            c->lq_sums.alloc(2 * (size_t)nw);
+            c->lq_tb.alloc(nw + 1); c->lq_tb.upload(ctx->tile_begin.data(), nw + 1, st);
         } else if (c->kind == CV_DIST) {
             std::vector<int64_t> off(nw);
             std::vector<double> box((size_t)nw * 3);
@@ -1130,7 +1193,7 @@ void fix_build(mdt_fix* fx) {
     r.ncv = (int)fx->cvs.size();
     // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
     for (size_t c = 0; c < fx->cvs.size(); ++c)
-        r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST || fx->cvs[c]->kind == CV_SF) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
+        r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST || fx->cvs[c]->kind == CV_SF || fx->cvs[c]->kind == CV_LQ) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
     r.inv_ng = fx->envs.empty() ? nullptr : fx->envs[0]->inv_ng.p;
     for (auto* e : fx->envs)
         if (e->group != fx->envs[0]->group) throw Err(MDT_ENOTSUP, "all STEINH CALs must use the same group");
@@ -1440,6 +1503,16 @@ SfDev sf_dev(mdt_fix* fx, CvH* c) {
     return d;
 }
 
+LqDev lq_dev(mdt_fix* fx, CvH* c) {
+    LqDev d{};
+    d.l = c->l; d.mode = lq_mode(c->func.empty() ? "MEAN_SOLID" : c->func);
+    d.swr = c->lq_swr; d.swcv = c->lq_swcv; d.eps = c->lq_eps;
+    d.S = c->lq_S.p; d.q = c->lq_q.p; d.LQ = c->lq_LQ.p; d.ql = c->lq_ql.p; d.K = c->lq_K.p; d.g = c->lq_g.p;
+    d.part = c->lq_part.p; d.sums = c->lq_sums.p; d.tile_begin = c->lq_tb.p; d.ng = c->lq_ng.p;
+    d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot; d.n_walkers = fx->nw();
+    return d;
+}
+
 void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
     static Timer off_timer;
     Timer& T = tm ? *tm : off_timer;
@@ -1476,6 +1549,11 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             Geo gs = g;
             gs.group = c->sf_group == "all" ? nullptr : ctx->group.p;
             T.run("k_sf_value", st, [&] { HIPCHK(launch_sf_value(gs, d, st)); });
+        } else if (c->kind == CV_LQ) {
+            const LqDev d = lq_dev(fx, c);
+            Geo gs = g;
+            gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
+            T.run("k_lq_value", st, [&] { HIPCHK(launch_lq_value(gs, d, st)); });
         }
     }
     T.run("k_reduce", st, [&] { HIPCHK(launch_reduce(fx->red, st)); });
@@ -1507,6 +1585,12 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             Geo gs = g;
             gs.group = c->sf_group == "all" ? nullptr : ctx->group.p;
             T.run("k_sf_force", st, [&] { HIPCHK(launch_sf_force(gs, d, f, st)); });
+        }
+        if (c->kind == CV_LQ && c->biased) {
+            const LqDev d = lq_dev(fx, c);
+            Geo gs = g;
+            gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
+            T.run("k_lq_force", st, [&] { HIPCHK(launch_lq_force(gs, d, f, st)); });
         }
     }
     // The values of unbiased local-Q CVs fused into the force pass (their tile sums
@@ -1905,7 +1989,13 @@ static int find_cv(mdt_fix* fx, const char* cal_func) {
     std::string func = dot == std::string::npos ? "AVE" : t.substr(dot + 1);
     for (size_t k = 0; k < fx->cvs.size(); ++k)
         if (fx->cvs[k]->name == name) {
-            if (func != "AVE" && fx->cvs[k]->kind != CV_DIST && !(fx->cvs[k]->kind == CV_SF && func == fx->cvs[k]->func))
+            const CvH* c = fx->cvs[k];
+            if (c->kind == CV_LQ) {
+                if (dot != std::string::npos && lq_mode(func) != lq_mode(c->func.empty() ? "MEAN_SOLID" : c->func))
+                    throw Err(MDT_EINVAL, fmt("We can't find the func %s.", func.c_str()));
+                return (int)k;
+            }
+            if (func != "AVE" && c->kind != CV_DIST && !(c->kind == CV_SF && func == c->func))
                 throw Err(MDT_EINVAL, fmt("We can't find the func %s.", func.c_str()));
             return (int)k;
         }
@@ -1943,7 +2033,8 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
     if (!c) throw Err(MDT_EINVAL, fmt("no CAL named '%s'", name.c_str()));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int64_t n = a.n_local, o = a.atom_off;
-    if (c->kind == CV_SK || c->kind == CV_DIST || c->kind == CV_SF) throw Err(MDT_EINVAL, "SK, DISTANCE and STRU_FACTOR have no per-atom export");
+    if (c->kind == CV_SK || c->kind == CV_DIST || c->kind == CV_SF || c->kind == CV_LQ)
+        throw Err(MDT_EINVAL, "SK, DISTANCE, STRU_FACTOR and STEINH L have no per-atom export");
     EnvH* e = fx->envs[c->env];
     if (prop == "stein_q" || (prop == "local_q" && c->kind == CV_LD)) {
         std::vector<float> v((size_t)n);
@@ -1978,7 +2069,8 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
     WalkerIn& a = ctx->walker(walker);
     CvH* c = nullptr;
     for (auto* q : fx->cvs) if (q->name == (cal_name ? cal_name : "")) c = q;
-    if (!c || c->kind == CV_SK || c->kind == CV_DIST || c->kind == CV_SF) throw Err(MDT_EINVAL, "selection export needs a STEINH CAL name");
+    if (!c || c->kind == CV_SK || c->kind == CV_DIST || c->kind == CV_SF || c->kind == CV_LQ)
+        throw Err(MDT_EINVAL, "selection export needs a STEINH Q or LD CAL name");
     EnvH* e = fx->envs[c->env];
     const int N = e->N;
     const int64_t n = a.n_local;

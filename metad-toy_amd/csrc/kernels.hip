@@ -1979,6 +1979,289 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
 }
 
 // ---------------------------------------------------------------------------
+// STEINH L: switching-weighted local Q (src/cvs/CVs_Steinhardt_Local.cu).
+// Every list entry of atom a with sigma(r) >= eps counts (get_environment_
+// Steinhardt_LocalQ :1210-1285): S_a = sum sigma, q_lm(a) = sum sigma Y_lm / S_a
+// (:1310-1452).  A group atom i has LQ_lm(i) = (q_lm(i) + sum sigma q_lm(j)) /
+// (S_i + 1) and LQ_l(i) (:1490-1579); the CV is sum q f(q) / sum f(q)
+// (MEAN_SOLID), sum f(q) (NSOLID) or sum f(q) / N_group (FRAC) (:694-745).
+// FP64 throughout, Y_lm from the same orthonormal polynomial table as the FP32
+// path.  The gradient is exact and all atoms carry it: per entry e = (a -> j)
+// the kernels form g_e = d(W CV)/d(x_j - x_a); the force on a is the sum of g
+// over its own entries minus g of the reverse entries (the images of a in its
+// neighbours' lists), so no atomics are needed.
+// ---------------------------------------------------------------------------
+template <int L, int M, bool DER, int K>
+struct YC64 {
+    static constexpr double v = DER ? (K + 1 <= L ? (K + 1) * kY.c[L][M][K + 1] : 0.0) : kY.c[L][M][K];
+};
+template <int L, int M, bool DER, int K, int TOP>
+struct H64 {
+    static __device__ __forceinline__ double run(double v) {
+        if constexpr (K >= TOP) return YC64<L, M, DER, TOP>::v;
+        else return fma(v, H64<L, M, DER, K + 2, TOP>::run(v), YC64<L, M, DER, K>::v);
+    }
+};
+// c_lm(u) (DER = false) or c'_lm(u) (DER = true) in FP64
+template <int L, int M, bool DER>
+__device__ __forceinline__ double ypoly64(double u, double u2) {
+    constexpr int DEG = DER ? (L - M - 1) : (L - M);
+    if constexpr (DEG < 0) {
+        return 0.0;
+    } else {
+        const double h = H64<L, M, DER, DEG % 2, DEG>::run(u2);
+        if constexpr (DEG % 2) return h * u; else return h;
+    }
+}
+
+// acc[2m], acc[2m+1] += s Re/Im Y_lm(n), m = 0..L
+template <int L>
+__device__ __forceinline__ void ylm64_acc(double nx, double ny, double u, double s, double* acc) {
+    const double u2 = u * u;
+    double zr = 1.0, zi = 0.0;
+    sfor<L + 1>([&](auto mm) {
+        constexpr int M = decltype(mm)::value;
+        if constexpr (M > 0) {
+            const double t = zr * nx - zi * ny;
+            zi = zr * ny + zi * nx;
+            zr = t;
+        }
+        const double c = s * ypoly64<L, M, false>(u, u2);
+        acc[2 * M] += c * zr;
+        acc[2 * M + 1] += c * zi;
+    });
+}
+
+// <H, Y(d)> and its gradient in d: grad_n of sum_m c_lm(u) (H_re Re z^m + H_im Im z^m),
+// z = n_x + i n_y, projected: grad_d = (grad_n - (n . grad_n) n) / r
+template <int L>
+__device__ __forceinline__ double ylm64_dot_grad(double nx, double ny, double u, double r, const double* H,
+                                                 double& gx, double& gy, double& gz) {
+    const double u2 = u * u;
+    double zr = 1.0, zi = 0.0, pr = 0.0, pi = 0.0;    // z^m and z^(m-1)
+    double val = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+    sfor<L + 1>([&](auto mm) {
+        constexpr int M = decltype(mm)::value;
+        if constexpr (M > 0) {
+            pr = zr; pi = zi;
+            const double t = zr * nx - zi * ny;
+            zi = zr * ny + zi * nx;
+            zr = t;
+        }
+        const double hr = H[2 * M], hi = H[2 * M + 1];
+        const double c = ypoly64<L, M, false>(u, u2), cd = ypoly64<L, M, true>(u, u2);
+        const double proj = hr * zr + hi * zi;
+        val += c * proj;
+        az += cd * proj;
+        if constexpr (M > 0) {
+            ax += c * M * (hr * pr + hi * pi);
+            ay += c * M * (hi * pr - hr * pi);
+        }
+    });
+    const double dn = nx * ax + ny * ay + u * az, ir = 1.0 / r;
+    gx = (ax - dn * nx) * ir;
+    gy = (ay - dn * ny) * ir;
+    gz = (az - dn * u) * ir;
+    return val;
+}
+
+template <int L>
+__global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
+    constexpr int LS = 2 * (L + 1);
+    const Tile tl = g.tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.count) return;
+    const int i = tl.start + threadIdx.x;
+    const double4 p = g.x4[i];
+    double acc[LS];
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) acc[k] = 0.0;
+    double S = 0.0;
+    int cnt = 0;
+    for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
+        const double4 q = g.x4[g.idx[e]];
+        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < 1e-12) continue;
+        const double r = sqrt(r2), s = sw_f(d.swr, r);
+        if (s < d.eps) continue;
+        S += s;
+        ++cnt;
+        const double ir = 1.0 / r;
+        ylm64_acc<L>(dx * ir, dy * ir, dz * ir, s, acc);
+    }
+    const bool live = cnt > 0 && S >= 1e-12;
+    const double inv = live ? 1.0 / S : 0.0;
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) d.q[(int64_t)i * LS + k] = acc[k] * inv;
+    d.S[i] = live ? S : 0.0;
+}
+
+template <int L>
+__global__ __launch_bounds__(64) void k_lq_lq(Geo g, LqDev d) {
+    constexpr int LS = 2 * (L + 1);
+    const int t = blockIdx.x;
+    const Tile tl = g.tiles[t];
+    const bool valid = (int)threadIdx.x < tl.count;
+    const int i = tl.start + (valid ? (int)threadIdx.x : 0);
+    const bool in_g = valid && (g.group == nullptr || g.group[i]);
+    double v = 0.0;
+    if (in_g && d.S[i] > 0.0) {
+        const double4 p = g.x4[i];
+        double acc[LS];
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) acc[k] = d.q[(int64_t)i * LS + k];
+        for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
+            const int j = g.idx[e];
+            const double4 q = g.x4[j];
+            const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < 1e-12) continue;
+            const double s = sw_f(d.swr, sqrt(r2));
+            if (s < d.eps) continue;
+            const double* qo = d.q + (int64_t)owner_of(g, j) * LS;
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) acc[k] += s * qo[k];
+        }
+        const double inv = 1.0 / (d.S[i] + 1.0);
+        double sq = 0.0;
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) {
+            acc[k] *= inv;
+            d.LQ[(int64_t)i * LS + k] = acc[k];
+            sq += (k < 2 ? 1.0 : 2.0) * acc[k] * acc[k];
+        }
+        v = sqrt(sq * (4.0 * kPi) / (double)(2 * L + 1));
+    } else if (valid) {
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) d.LQ[(int64_t)i * LS + k] = 0.0;
+    }
+    if (valid) d.ql[i] = v;
+    const double f = in_g ? sw_f(d.swcv, v) : 0.0;
+    const double sq = wave_sum(in_g ? v * f : 0.0);
+    const double sf = wave_sum(f);
+    if (threadIdx.x == 0) { d.part[2 * t] = sq; d.part[2 * t + 1] = sf; }
+}
+
+// per walker: sum q f and sum f over the tiles in tile order -> CV (one wave)
+__global__ __launch_bounds__(64) void k_lq_sum(LqDev d) {
+    const int w = blockIdx.x;
+    const int t0 = d.tile_begin[w], t1 = d.tile_begin[w + 1];
+    double a = 0.0, b = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += 64) { a += d.part[2 * t]; b += d.part[2 * t + 1]; }
+    const double sq = wave_sum(a), sf = wave_sum(b);
+    if (threadIdx.x != 0) return;
+    const double N = d.ng[w];
+    const double cv = d.mode == 0 ? (sf != 0.0 ? sq / sf : 0.0) : d.mode == 1 ? sf : (N > 0.0 ? sf / N : 0.0);
+    d.direct[(int64_t)w * kMaxSlotCV + d.slot] = cv;
+    d.sums[2 * w] = sf;
+    d.sums[2 * w + 1] = cv;
+}
+
+// K_i = W dCV/dLQ_lm(i) / (S_i + 1): w_i = dCV/dLQ_l(i) per mode, then
+// dLQ_l/dLQ_lm = 4 pi/(2l+1) c_m LQ_lm / LQ_l (c_0 = 1, c_m = 2 for m >= 1)
+template <int L>
+__global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
+    constexpr int LS = 2 * (L + 1);
+    const Tile tl = g.tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.count) return;
+    const int i = tl.start + threadIdx.x;
+    const int w = tl.walker;
+    const double ql = d.ql[i];
+    double c = 0.0;
+    if ((g.group == nullptr || g.group[i]) && d.S[i] > 0.0 && ql > 0.0) {
+        const double W = d.wcv[(int64_t)w * kMaxSlotCV + d.slot];
+        const double sf = d.sums[2 * w], cv = d.sums[2 * w + 1], N = d.ng[w];
+        const double f = sw_f(d.swcv, ql), fp = sw_df(d.swcv, ql);
+        const double wi = d.mode == 0 ? (sf != 0.0 ? (f + fp * (ql - cv)) / sf : 0.0)
+                        : d.mode == 1 ? fp : (N > 0.0 ? fp / N : 0.0);
+        c = W * wi * (4.0 * kPi / (double)(2 * L + 1)) / ql / (d.S[i] + 1.0);
+    }
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) d.K[(int64_t)i * LS + k] = (k < 2 ? c : 2.0 * c) * d.LQ[(int64_t)i * LS + k];
+}
+
+// per entry of a: g_e = (1/S_a) [s' <H_a, Y - q_a> n + s grad_d <H_a, Y>] + s' <K_a, q_j - LQ_a> n
+// with H_a = K_a + sum over counted entries of s K_owner(j)
+template <int L>
+__global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
+    constexpr int LS = 2 * (L + 1);
+    const Tile tl = g.tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.count) return;
+    const int i = tl.start + threadIdx.x;
+    const int beg = g.off[i], end = g.off[i + 1];
+    const double Si = d.S[i];
+    if (!(Si > 0.0)) {
+        for (int e = beg; e < end; ++e) { d.g[3 * (int64_t)e] = 0.0; d.g[3 * (int64_t)e + 1] = 0.0; d.g[3 * (int64_t)e + 2] = 0.0; }
+        return;
+    }
+    const double4 p = g.x4[i];
+    double H[LS], Ki[LS];
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) { Ki[k] = d.K[(int64_t)i * LS + k]; H[k] = Ki[k]; }
+    for (int e = beg; e < end; ++e) {
+        const int j = g.idx[e];
+        const double4 q = g.x4[j];
+        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < 1e-12) continue;
+        const double s = sw_f(d.swr, sqrt(r2));
+        if (s < d.eps) continue;
+        const double* Ko = d.K + (int64_t)owner_of(g, j) * LS;
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) H[k] += s * Ko[k];
+    }
+    double hq = 0.0, kl = 0.0;
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) {
+        hq += H[k] * d.q[(int64_t)i * LS + k];
+        kl += Ki[k] * d.LQ[(int64_t)i * LS + k];
+    }
+    const double iS = 1.0 / Si;
+    for (int e = beg; e < end; ++e) {
+        const int j = g.idx[e];
+        const double4 q = g.x4[j];
+        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double G0 = 0.0, G1 = 0.0, G2 = 0.0;
+        if (r2 >= 1e-12) {
+            const double r = sqrt(r2);
+            const double s = sw_f(d.swr, r);
+            if (s >= d.eps) {
+                const double ds = sw_df(d.swr, r), ir = 1.0 / r;
+                const double nx = dx * ir, ny = dy * ir, nz = dz * ir;
+                double gx, gy, gz;
+                const double hy = ylm64_dot_grad<L>(nx, ny, nz, r, H, gx, gy, gz) - hq;
+                const double* qo = d.q + (int64_t)owner_of(g, j) * LS;
+                double kq = -kl;
+                #pragma unroll
+                for (int k = 0; k < LS; ++k) kq += Ki[k] * qo[k];
+                const double cr = ds * hy * iS + ds * kq, ct = s * iS;
+                G0 = cr * nx + ct * gx;
+                G1 = cr * ny + ct * gy;
+                G2 = cr * nz + ct * gz;
+            }
+        }
+        d.g[3 * (int64_t)e] = G0; d.g[3 * (int64_t)e + 1] = G1; d.g[3 * (int64_t)e + 2] = G2;
+    }
+}
+
+// F_a += sum over a's entries of g_e - sum of g at their reverse entries
+__global__ __launch_bounds__(64) void k_lq_force(Geo g, LqDev d, double* __restrict__ f) {
+    const Tile tl = g.tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.count) return;
+    const int i = tl.start + threadIdx.x;
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
+        F0 += d.g[3 * (int64_t)e]; F1 += d.g[3 * (int64_t)e + 1]; F2 += d.g[3 * (int64_t)e + 2];
+        const int rp = g.revpos[e];
+        if (rp == 0xFF) continue;
+        const int64_t er = (int64_t)g.off[g.eown[e]] + rp;
+        F0 -= d.g[3 * er]; F1 -= d.g[3 * er + 1]; F2 -= d.g[3 * er + 2];
+    }
+    f[3 * (int64_t)i] += F0; f[3 * (int64_t)i + 1] += F1; f[3 * (int64_t)i + 2] += F2;
+}
+
+// ---------------------------------------------------------------------------
 // DISTANCE CV (src/cvs/CVs_DISTANCE.cu:55-123): r = |x_a2 - x_a1| with each
 // component folded to the minimum image by the half-box rule of delta_x
 // (:55-85); the bias force is -dV/dr on a1 along -d/r and on a2 along +d/r
@@ -2438,6 +2721,29 @@ hipError_t launch_sf_value(const Geo& g, const SfDev& d, hipStream_t st) {
 
 hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t st) {
     if (g.ntiles > 0) hipLaunchKernelGGL(k_sf_force, dim3(g.ntiles), dim3(64), 0, st, g, d, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st) {
+    if (d.l != 3 && d.l != 4 && d.l != 6) return hipErrorInvalidValue;
+    if (g.ntiles > 0) {
+        const dim3 gr(g.ntiles), bl(64);
+        if (d.l == 3) { hipLaunchKernelGGL(k_lq_q<3>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<3>, gr, bl, 0, st, g, d); }
+        if (d.l == 4) { hipLaunchKernelGGL(k_lq_q<4>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<4>, gr, bl, 0, st, g, d); }
+        if (d.l == 6) { hipLaunchKernelGGL(k_lq_q<6>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<6>, gr, bl, 0, st, g, d); }
+    }
+    hipLaunchKernelGGL(k_lq_sum, dim3(d.n_walkers), dim3(64), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_lq_force(const Geo& g, const LqDev& d, double* f, hipStream_t st) {
+    if (d.l != 3 && d.l != 4 && d.l != 6) return hipErrorInvalidValue;
+    if (g.ntiles == 0) return hipGetLastError();
+    const dim3 gr(g.ntiles), bl(64);
+    if (d.l == 3) { hipLaunchKernelGGL(k_lq_k<3>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<3>, gr, bl, 0, st, g, d); }
+    if (d.l == 4) { hipLaunchKernelGGL(k_lq_k<4>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<4>, gr, bl, 0, st, g, d); }
+    if (d.l == 6) { hipLaunchKernelGGL(k_lq_k<6>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<6>, gr, bl, 0, st, g, d); }
+    hipLaunchKernelGGL(k_lq_force, gr, bl, 0, st, g, d, f);
     return hipGetLastError();
 }
 

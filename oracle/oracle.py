@@ -2,9 +2,11 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
 the cpu_baseline leg of bench.py, never by the shipped library.  Every function
-here calls the C restatement (numpy only marshals arrays), except the FES
-reconstruction at the end, restated in numpy because the reference's script
-(tools/plot_hills.py) is numpy.
+here calls the C restatement (numpy only marshals arrays), except the parts at the
+end: the FES reconstruction (numpy, as the reference's tools/plot_hills.py is), and
+DISTANCE, STRU_FACTOR, the switching functions and STEINH L, restated as plain Python
+loops over atoms and list entries for the small systems the tests use (Y_lm and its
+gradient still come from the C restatement).
 """
 import ctypes as C
 import os
@@ -460,4 +462,126 @@ def sf_count_grad(x_all, off, idx, n, own, rc, q, sw, group=None):
             jo = j if j < n else own[j - n]
             wt = (sw_df(sw, S[k]) if g[k] else 0.0) + (sw_df(sw, S[jo]) if g[jo] else 0.0)
             out[k] -= wt * fp * d / r
+    return out
+
+
+# ------------------------------------------------------------------ STEINH L (switching-weighted local Q)
+LQ_MODES = ("MEAN_SOLID", "NSOLID", "FRAC")
+
+
+def _lq_state(x_all, off, idx, n, own, l, swr, eps, group):
+    """Per atom a over its list entries with sigma(r) >= eps (get_environment_Steinhardt_LocalQ,
+    src/cvs/CVs_Steinhardt_Local.cu:1210-1285): S_a = sum sigma, q_lm(a) = sum sigma Y_lm / S_a
+    (steinhardt_Local_cv_get_qlm :1310-1452; zero when no entry counts or S_a < 1e-12).  For a
+    group atom i: LQ_lm(i) = (q_lm(i) + sum sigma_ij q_lm(j)) / (S_i + 1) with q_lm of a ghost j
+    its owner's, and LQ_l(i) = sqrt(4 pi/(2l+1) sum_m |LQ_lm|^2) (steinhardt_Local_cv_get_LQl
+    :1490-1579)."""
+    x = np.asarray(x_all, np.float64)
+    g = np.ones(n, bool) if group is None else np.asarray(group, bool)
+    S = np.zeros(n)
+    q = np.zeros((n, l + 1), complex)
+    live = np.zeros(n, bool)
+    nb = [[] for _ in range(n)]
+    for a in range(n):
+        acc = np.zeros(l + 1, complex)
+        for e in range(off[a], off[a + 1]):
+            j = idx[e]
+            d = x[j] - x[a]
+            r2 = float(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+            if r2 < 1e-12:
+                continue
+            r = float(np.sqrt(r2))
+            s = sw_f(swr, r)
+            if s < eps:
+                continue
+            o = j if j < n else int(own[j - n])
+            nb[a].append((o, s, sw_df(swr, r), d, r))
+            S[a] += s
+            acc += s * ylm(l, d)
+        live[a] = len(nb[a]) > 0 and S[a] >= 1e-12
+        q[a] = acc * (1.0 / S[a]) if live[a] else 0.0
+    LQ = np.zeros((n, l + 1), complex)
+    ql = np.zeros(n)
+    cm = np.where(np.arange(l + 1) > 0, 2.0, 1.0)
+    for i in range(n):
+        if not (g[i] and live[i]):
+            continue
+        acc = q[i].copy()
+        for (o, s, _, _, _) in nb[i]:
+            acc += s * q[o]
+        LQ[i] = acc * (1.0 / (S[i] + 1.0))
+        ql[i] = float(np.sqrt(np.sum(cm * np.abs(LQ[i]) ** 2) * 4.0 * np.pi / (2 * l + 1)))
+    return dict(g=g, S=S, q=q, live=live, nb=nb, LQ=LQ, ql=ql, cm=cm)
+
+
+def _lq_reduce(st, swcv, mode):
+    """reduce_lq_sw_sums / compute_cv_{AVE,NSOLID,FRAC} (:694-745): over local group atoms,
+    Sq = sum q f(q), Sf = sum f(q); MEAN_SOLID = Sq/Sf (0 if Sf = 0), NSOLID = Sf,
+    FRAC = Sf / N_group."""
+    g, ql = st["g"], st["ql"]
+    f = np.array([sw_f(swcv, v) for v in ql[g]])
+    Sq, Sf, N = float(np.sum(ql[g] * f)), float(np.sum(f)), int(g.sum())
+    if mode == "MEAN_SOLID":
+        cv = Sq / Sf if Sf != 0.0 else 0.0
+    elif mode == "NSOLID":
+        cv = Sf
+    else:
+        cv = Sf / N if N > 0 else 0.0
+    return cv, Sf, N
+
+
+def lq_cv(x_all, off, idx, n, own, l, swr, swcv, eps=1e-6, mode="MEAN_SOLID", group=None):
+    """The STEINH L CV value; swr, swcv as for sw_f (dict(type="LINE") for the default)."""
+    st = _lq_state(x_all, off, idx, n, own, l, swr, eps, group)
+    return _lq_reduce(st, swcv, mode)[0]
+
+
+def lq_peratom(x_all, off, idx, n, own, l, swr, eps=1e-6, group=None):
+    """Per-atom LQ_l (0 outside the group and where no entry counts)."""
+    return _lq_state(x_all, off, idx, n, own, l, swr, eps, group)["ql"]
+
+
+def lq_grad(x_all, off, idx, n, own, l, swr, swcv, eps=1e-6, mode="MEAN_SOLID", group=None):
+    """Exact dCV/dx of lq_cv for the n local atoms (a ghost entry counts for its owner).
+    Adjoint chain: G_i = dCV/dLQ_lm(i) = w_i 4pi/(2l+1) c_m LQ_lm / LQ_l with w_i = dCV/dLQ_l(i)
+    (MEAN_SOLID (f + f'(q - CV)) / Sf, NSOLID f', FRAC f' / N); K_i = G_i / (S_i + 1);
+    H_a = dCV/dq_lm(a) = K_a + sum over centres i that count a of sigma_ia K_i (scattered
+    here, exactly); then each counted pair vector d = x_j - x_a carries
+    (1/S_a) [sigma' <H_a, Y(d) - q_a> d/r + sigma grad_d <H_a, Y(d)>] + sigma' <K_a, q_j - LQ_a> d/r.
+    The reference's own force (steinhardt_Local_dcv_AVE_ij/jk_kernel :1623-2130) is not a
+    pin: it moves group atoms only."""
+    st = _lq_state(x_all, off, idx, n, own, l, swr, eps, group)
+    cv, Sf, N = _lq_reduce(st, swcv, mode)
+    g, S, q, live, nb, LQ, ql, cm = (st[k] for k in ("g", "S", "q", "live", "nb", "LQ", "ql", "cm"))
+    K = np.zeros((n, l + 1), complex)
+    for i in range(n):
+        if not (g[i] and live[i] and ql[i] > 0.0):
+            continue
+        f, fp = sw_f(swcv, ql[i]), sw_df(swcv, ql[i])
+        if mode == "MEAN_SOLID":
+            w = (f + fp * (ql[i] - cv)) / Sf if Sf != 0.0 else 0.0
+        elif mode == "NSOLID":
+            w = fp
+        else:
+            w = fp / N if N > 0 else 0.0
+        K[i] = w * 4.0 * np.pi / (2 * l + 1) * cm * LQ[i] / ql[i] / (S[i] + 1.0)
+    H = K.copy()
+    for i in range(n):
+        for (o, s, _, _, _) in nb[i]:
+            H[o] += s * K[i]
+
+    def dot(A, B):
+        return float(np.sum(A.real * B.real + A.imag * B.imag))
+
+    out = np.zeros((n, 3))
+    for a in range(n):
+        if not live[a]:
+            continue
+        for (o, s, ds, d, r) in nb[a]:
+            Y = ylm(l, d)
+            dY = ylm_grad(l, d)                      # (l+1, 3) complex
+            gvec = (ds * dot(H[a], Y - q[a]) * d / r + s * (H[a].real @ dY.real + H[a].imag @ dY.imag)) / S[a]
+            gvec = gvec + ds * dot(K[a], q[o] - LQ[a]) * d / r
+            out[a] -= gvec
+            out[o] += gvec
     return out

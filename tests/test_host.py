@@ -98,8 +98,11 @@ def test_fix_parser_accepts_reference_syntax(mdt):
     ("FOO 1", -22),
     ("CAL NAME Q6 STEINH Q 5 all", -22),
     ("CAL NAME Q6 STEINH Q 8 all", -95),
-    ("CAL NAME Q6 STEINH L 6 all", -95),
-    ("CAL NAME Q6 STEINH Q 6 all SW_FUNC_r sw", -95),
+    ("CAL NAME Q6 STEINH L 8 all", -95),
+    ("CAL NAME Q6 STEINH Q 6 all SW_FUNC_r sw", -22),
+    ("CAL NAME sw SW_FUNC NOPE", -22),
+    ("CAL NAME sw SW_FUNC RATIONAL r_0 0", -22),
+    ("CAL NAME sw SW_FUNC STEP CAL NAME sw SW_FUNC FERMI", -22),
     ("CAL NAME Q6 WEIGHT_CHEM 1 2", -95),
     ("CAL NAME S STRU_FACTOR all cutoff_r 5", -22),
     ("CAL NAME S STRU_FACTOR all q_factor 2 Chemical", -95),
@@ -216,3 +219,28 @@ def test_fes_input_parser(mdt):
     axes, lo, d = fes.grid_axes(p)
     assert np.array_equal(axes[0], np.linspace(0.0025, 0.9975, 200))
     assert np.allclose(d, [0.005, 1.0 / 150]) and np.array_equal(lo, [0.0, -0.5])
+
+
+def test_steinh_l_parsing(mdt):
+    """STEINH L with named switching functions: the functions of the CAL (MEAN_SOLID/AVE,
+    NSOLID/SW_FUNC, FRAC), one function per CAL, unknown names refused; a SW_FUNC name may
+    equal a CV name (separate registries, fix_crystallize.cu:144-160)."""
+    ctx = mdt.Context(-1, 1)
+    head = ("GAUSSIAN 0.01 0.005 10 CAL NAME sr SW_FUNC RATIONAL r_0 3.0 CAL NAME sc SW_FUNC FERMI r_0 0.4 alpha 10 "
+            "CAL NAME L6 STEINH L 6 all cutoff_eps 1e-5 SW_FUNC_r sr SW_FUNC_cv sc ")
+    for fn in ("MEAN_SOLID", "AVE", "NSOLID", "SW_FUNC", "FRAC"):
+        mdt.Fix(ctx, head + f"CV_dim 1 SYMBOL v1 L6.{fn} DIM 1 0 1 100 v1").close()
+    mdt.Fix(ctx, head + "CV_dim 1 SYMBOL v1 L6.AVE SYMBOL v2 L6.MEAN_SOLID DIM 1 0 1 100 v1+v2").close()
+    mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 CAL NAME L6 SW_FUNC STEP r_0 3.9 CAL NAME L6 STEINH L 6 all SW_FUNC_r L6 "
+                 "CV_dim 1 SYMBOL v1 L6.FRAC DIM 1 0 1 100 v1").close()
+    for tail, frag in [
+        ("CV_dim 1 SYMBOL v1 L6.COUNT DIM 1 0 1 100 v1", "Use MEAN_SOLID"),
+        ("CV_dim 1 SYMBOL v1 L6.NSOLID SYMBOL v2 L6.FRAC DIM 1 0 1 100 v1+v2", "one CAL per function"),
+    ]:
+        with pytest.raises(mdt.MdtError) as e:
+            mdt.Fix(ctx, head + tail)
+        assert frag in str(e.value), (tail, str(e.value))
+    with pytest.raises(mdt.MdtError) as e:
+        mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 CAL NAME L6 STEINH L 6 all SW_FUNC_cv nope CV_dim 1 SYMBOL v1 L6.AVE DIM 1 0 1 100 v1")
+    assert "not found in registry" in str(e.value)
+    ctx.close()

@@ -623,3 +623,82 @@ def test_stru_factor_count_gradient_fd(orc):
             xm[k, a] -= h
             fd = (orc.sf_count(xp, off, idx, n, rc, q, sw, grp) - orc.sf_count(xm, off, idx, n, rc, q, sw, grp)) / (2 * h)
             assert abs(fd - G[k, a]) <= 1e-7 * max(1.0, np.abs(G).max())
+
+
+# ---------------------------------------------------------------- STEINH L
+def _cluster(n, seed, L=6.0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, L, size=(n, 3))
+    off = np.arange(n + 1) * (n - 1)
+    idx = np.array([j for i in range(n) for j in range(n) if j != i])
+    return x, off, idx
+
+
+@pytest.mark.parametrize("l,q_ref", [(6, 0.574524), (4, 0.190941)])
+def test_lq_perfect_fcc_known_answer(mdt, orc, l, q_ref):
+    """Perfect FCC with STEP r_0 = 3.9: every atom weighs its 12 nearest by 1, all q_lm are
+    equal, so LQ_l = q_l of the 12-neighbour shell (0.574524 for l = 6, 0.190941 for l = 4).
+    With the default LINE f: MEAN_SOLID = LQ_l, NSOLID = N, FRAC = 1."""
+    s = fcc_system(mdt, 4, sigma=0.0)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    step, line = dict(type="STEP", r0=3.9), dict(type="LINE")
+    ql = orc.lq_peratom(x, off, idx, n, own, l, step)
+    assert np.abs(ql - q_ref).max() <= 1e-6
+    assert np.ptp(ql) <= 1e-12
+    assert abs(orc.lq_cv(x, off, idx, n, own, l, step, line, mode="MEAN_SOLID") - ql.mean()) <= 1e-14
+    assert orc.lq_cv(x, off, idx, n, own, l, step, line, mode="NSOLID") == n
+    assert orc.lq_cv(x, off, idx, n, own, l, step, line, mode="FRAC") == 1.0
+    fermi = dict(type="FERMI", r0=0.4, alpha=10.0)
+    assert abs(orc.lq_cv(x, off, idx, n, own, l, step, fermi, mode="NSOLID")
+               - n * orc.sw_f(fermi, ql[0])) <= 1e-9 * n
+
+
+@pytest.mark.parametrize("mode", ["MEAN_SOLID", "NSOLID", "FRAC"])
+@pytest.mark.parametrize("l", [4, 6])
+def test_lq_gradient_fd_cluster(orc, mode, l):
+    """The exact gradient equals the central FD of the CV at group and non-group atoms
+    (a 13-atom cluster, all pairs listed, RATIONAL weights in r, FERMI f on LQ_l)."""
+    x, off, idx = _cluster(13, 40 + l)
+    n = 13
+    grp = np.arange(n) % 4 != 1
+    swr = dict(type="RATIONAL", r0=2.6, d0=0.0, n=6, m=12)
+    swcv = dict(type="FERMI", r0=0.35, alpha=8.0)
+    own = np.zeros(0, int)
+    G = orc.lq_grad(x, off, idx, n, own, l, swr, swcv, 1e-6, mode, grp)
+    h = 1e-6
+    scale = max(1e-3, np.abs(G).max())
+    for k in (0, 1, 5, 10):
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[k, a] += h
+            xm[k, a] -= h
+            fd = (orc.lq_cv(xp, off, idx, n, own, l, swr, swcv, 1e-6, mode, grp)
+                  - orc.lq_cv(xm, off, idx, n, own, l, swr, swcv, 1e-6, mode, grp)) / (2 * h)
+            assert abs(fd - G[k, a]) <= 2e-7 * scale, (mode, k, a, fd, G[k, a])
+    # translation invariance (no periodic images: the forces sum to zero)
+    assert np.abs(G.sum(axis=0)).max() <= 1e-12 * max(1.0, np.abs(G).max() * n)
+
+
+def test_lq_gradient_fd_periodic_with_ghosts(mdt, orc):
+    """On a periodic system the ghost entries fold onto their owners: FD with ghost-consistent
+    moves (Mover) equals the gradient, and the forces sum to zero."""
+    from helpers import Mover
+    s = fcc_system(mdt, 3, sigma=0.25, seed=44)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    grp = (x[:n, 2] < x[:n, 2].mean())
+    swr = dict(type="RATIONAL", r0=3.0, d0=0.0, n=6, m=12)
+    swcv = dict(type="TANH", r0=0.3, alpha=6.0)
+    G = orc.lq_grad(x, off, idx, n, own, 6, swr, swcv, 1e-6, "MEAN_SOLID", grp)
+    mv = Mover(x, n, own)
+    h = 1e-6
+    for k in (0, 17, 60):
+        for a in range(3):
+            xl = x[:n].copy()
+            xl[k, a] += h
+            cp = orc.lq_cv(mv.set_local(xl), off, idx, n, own, 6, swr, swcv, 1e-6, "MEAN_SOLID", grp)
+            xl[k, a] -= 2 * h
+            cm = orc.lq_cv(mv.set_local(xl), off, idx, n, own, 6, swr, swcv, 1e-6, "MEAN_SOLID", grp)
+            assert abs((cp - cm) / (2 * h) - G[k, a]) <= 2e-7 * np.abs(G).max(), (k, a)
+    assert np.abs(G.sum(axis=0)).max() <= 1e-12 * np.abs(G).max() * n
