@@ -256,7 +256,11 @@ struct LqDev {
     double* LQ;               // per local atom LQ_lm (0 outside the group)
     double* ql;               // per local atom LQ_l
     double* K;                // per local atom W dCV/dLQ_lm / (S + 1)
-    double* g;                // per CSR entry: d(W CV)/d(pair vector), 3 doubles
+    const int64_t* apos;      // per local atom: ELL position of its first entry (tile base + lane);
+                              // entry k at apos + 64 k
+    double* g;                // per ELL entry: d(W CV)/d(pair vector), 3 doubles
+    double2* es;              // per ELL entry: (sigma, dsigma/dr), sigma = 0 where the entry does not count
+    const int64_t* erev;      // per ELL entry: ELL index of the reverse entry, -1 = none
     double* part;             // per tile (sum q f, sum f)
     double* sums;             // per walker (sum f, CV)
     const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)
@@ -268,6 +272,7 @@ struct LqDev {
 };
 hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st);
 hipError_t launch_lq_force(const Geo& g, const LqDev& d, double* f, hipStream_t st);
+hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st);
 
 // DISTANCE CV between two atoms of each walker (one thread per walker).
 struct DistDev {

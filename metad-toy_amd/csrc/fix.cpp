@@ -403,7 +403,9 @@ struct CvH {
     double lq_eps = 1e-6;
     std::string lq_group;
     DBuf<double> lq_S, lq_q, lq_LQ, lq_ql, lq_K, lq_g, lq_part, lq_sums, lq_ng;
+    DBuf<double2> lq_es;
     DBuf<int32_t> lq_tb;
+    DBuf<int64_t> lq_apos, lq_erev;
 };
 
 // STEINH L functions (CVs_Steinhardt_Local.cu:631-690): MEAN_SOLID (alias AVE), NSOLID
@@ -1105,7 +1107,26 @@ void fix_build(mdt_fix* fx) {
             c->lq_ng.alloc(nw); c->lq_ng.upload(ng.data(), nw, st);
             c->lq_S.alloc(n1); c->lq_ql.alloc(n1);
             c->lq_q.alloc(n1 * LS); c->lq_LQ.alloc(n1 * LS); c->lq_K.alloc(n1 * LS);
-            c->lq_g.alloc((size_t)std::max<int64_t>(ctx->total_nnz, 1) * 3);
+            // per-tile ELL layout of the per-entry words: tile t holds 64 x (its longest list)
+            std::vector<int32_t> hoff((size_t)NT + 1, 0);
+            HIPCHK(hipStreamSynchronize(st));
+            if (NT > 0) HIPCHK(hipMemcpy(hoff.data(), ctx->off.p, sizeof(int32_t) * (NT + 1), hipMemcpyDeviceToHost));
+            std::vector<int64_t> apos((size_t)std::max<int64_t>(NT, 1), 0);
+            int64_t ell = 0;
+            for (const Tile& t : ctx->h_tiles) {
+                int mx = 0;
+                for (int a = 0; a < t.count; ++a) {
+                    mx = std::max(mx, hoff[t.start + a + 1] - hoff[t.start + a]);
+                    apos[t.start + a] = ell + a;
+                }
+                ell += 64 * (int64_t)mx;
+            }
+            const size_t ell1 = (size_t)std::max<int64_t>(ell, 1);
+            c->lq_apos.alloc(apos.size()); c->lq_apos.upload(apos.data(), apos.size(), st);
+            c->lq_g.alloc(ell1 * 3);
+            c->lq_es.alloc(ell1);
+            c->lq_erev.alloc(ell1);
+            HIPCHK(launch_lq_erev(ctx->geo(), NT, c->lq_apos.p, c->lq_erev.p, st));
             c->lq_part.alloc(2 * (size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
             c->lq_sums.alloc(2 * (size_t)nw);
             c->lq_tb.alloc(nw + 1); c->lq_tb.upload(ctx->tile_begin.data(), nw + 1, st);
@@ -1508,6 +1529,7 @@ LqDev lq_dev(mdt_fix* fx, CvH* c) {
     d.l = c->l; d.mode = lq_mode(c->func.empty() ? "MEAN_SOLID" : c->func);
     d.swr = c->lq_swr; d.swcv = c->lq_swcv; d.eps = c->lq_eps;
     d.S = c->lq_S.p; d.q = c->lq_q.p; d.LQ = c->lq_LQ.p; d.ql = c->lq_ql.p; d.K = c->lq_K.p; d.g = c->lq_g.p;
+    d.es = c->lq_es.p; d.erev = c->lq_erev.p; d.apos = c->lq_apos.p;
     d.part = c->lq_part.p; d.sums = c->lq_sums.p; d.tile_begin = c->lq_tb.p; d.ng = c->lq_ng.p;
     d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot; d.n_walkers = fx->nw();
     return d;

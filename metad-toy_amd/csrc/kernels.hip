@@ -1989,7 +1989,9 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
 // path.  The gradient is exact and all atoms carry it: per entry e = (a -> j)
 // the kernels form g_e = d(W CV)/d(x_j - x_a); the force on a is the sum of g
 // over its own entries minus g of the reverse entries (the images of a in its
-// neighbours' lists), so no atomics are needed.
+// neighbours' lists), so no atomics are needed.  Per-entry words live in a per-tile
+// ELL layout: entry k of atom a sits at apos[a] + 64 k, apos[a] = tile base + lane,
+// so the 64 lanes of a tile touch 64 consecutive words on every round.
 // ---------------------------------------------------------------------------
 template <int L, int M, bool DER, int K>
 struct YC64 {
@@ -2065,63 +2067,105 @@ __device__ __forceinline__ double ylm64_dot_grad(double nx, double ny, double u,
     return val;
 }
 
+// acc += sum over entries [beg, end) of sigma_e rec[owner_e]; the per-entry words come
+// first and two entries' records are loaded before either is used
+template <int LS>
+__device__ __forceinline__ void gather_weighted(const Geo& g, const double2* __restrict__ es,
+                                                const double* __restrict__ rec, int beg, int end, int64_t ap,
+                                                double* acc) {
+    int e = beg;
+    for (; e + 1 < end; e += 2) {
+        const double s0 = es[ap + 64 * (int64_t)(e - beg)].x, s1 = es[ap + 64 * (int64_t)(e + 1 - beg)].x;
+        const double* r0 = rec + (int64_t)g.eown[e] * LS;
+        const double* r1 = rec + (int64_t)g.eown[e + 1] * LS;
+        double v0[LS], v1[LS];
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) { v0[k] = r0[k]; v1[k] = r1[k]; }
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) acc[k] += s0 * v0[k] + s1 * v1[k];
+    }
+    if (e < end) {
+        const double s0 = es[ap + 64 * (int64_t)(e - beg)].x;
+        const double* r0 = rec + (int64_t)g.eown[e] * LS;
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) acc[k] += s0 * r0[k];
+    }
+}
+
+// Store each lane's LS-double row of a tile's consecutive atoms: the rows pass through
+// LDS so the wave writes the tile's contiguous block in order.  Every lane of the
+// one-wave workgroup calls it (count <= 64 rows are written).
+template <int LS>
+__device__ __forceinline__ void tile_store_rows(double* __restrict__ dst, const double* v, int count, double* sh) {
+    const int lane = threadIdx.x;
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) sh[lane * LS + k] = v[k];
+    __syncthreads();
+    for (int t = lane; t < count * LS; t += 64) dst[t] = sh[t];
+    __syncthreads();
+}
+
 template <int L>
 __global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
     constexpr int LS = 2 * (L + 1);
-    const Tile tl = g.tiles[blockIdx.x];
-    if ((int)threadIdx.x >= tl.count) return;
-    const int i = tl.start + threadIdx.x;
+    __shared__ double sh[64 * LS];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    const bool valid = (int)threadIdx.x < tl.count;
+    const int i = tl.start + (valid ? (int)threadIdx.x : 0);
     const double4 p = g.x4[i];
     double acc[LS];
     #pragma unroll
     for (int k = 0; k < LS; ++k) acc[k] = 0.0;
     double S = 0.0;
     int cnt = 0;
-    for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
+    const int beg = g.off[i];
+    const int64_t ap = d.apos[i];
+    for (int e = beg; e < (valid ? g.off[i + 1] : beg); ++e) {
         const double4 q = g.x4[g.idx[e]];
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < 1e-12) continue;
-        const double r = sqrt(r2), s = sw_f(d.swr, r);
-        if (s < d.eps) continue;
-        S += s;
-        ++cnt;
-        const double ir = 1.0 / r;
-        ylm64_acc<L>(dx * ir, dy * ir, dz * ir, s, acc);
+        double s = 0.0, ds = 0.0;
+        if (r2 >= 1e-12) {
+            const double r = sqrt(r2);
+            s = sw_f(d.swr, r);
+            if (s >= d.eps) {
+                ds = sw_df(d.swr, r);
+                S += s;
+                ++cnt;
+                const double ir = 1.0 / r;
+                ylm64_acc<L>(dx * ir, dy * ir, dz * ir, s, acc);
+            } else {
+                s = 0.0;
+            }
+        }
+        d.es[ap + 64 * (int64_t)(e - beg)] = make_double2(s, ds);
     }
     const bool live = cnt > 0 && S >= 1e-12;
     const double inv = live ? 1.0 / S : 0.0;
     #pragma unroll
-    for (int k = 0; k < LS; ++k) d.q[(int64_t)i * LS + k] = acc[k] * inv;
-    d.S[i] = live ? S : 0.0;
+    for (int k = 0; k < LS; ++k) acc[k] *= inv;
+    tile_store_rows<LS>(d.q + (int64_t)tl.start * LS, acc, tl.count, sh);
+    if (valid) d.S[i] = live ? S : 0.0;
 }
 
 template <int L>
 __global__ __launch_bounds__(64) void k_lq_lq(Geo g, LqDev d) {
     constexpr int LS = 2 * (L + 1);
-    const int t = blockIdx.x;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
     const bool valid = (int)threadIdx.x < tl.count;
     const int i = tl.start + (valid ? (int)threadIdx.x : 0);
     const bool in_g = valid && (g.group == nullptr || g.group[i]);
     double v = 0.0;
     if (in_g && d.S[i] > 0.0) {
-        const double4 p = g.x4[i];
+        const double* __restrict__ qa = d.q;
         double acc[LS];
         #pragma unroll
-        for (int k = 0; k < LS; ++k) acc[k] = d.q[(int64_t)i * LS + k];
-        for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
-            const int j = g.idx[e];
-            const double4 q = g.x4[j];
-            const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < 1e-12) continue;
-            const double s = sw_f(d.swr, sqrt(r2));
-            if (s < d.eps) continue;
-            const double* qo = d.q + (int64_t)owner_of(g, j) * LS;
-            #pragma unroll
-            for (int k = 0; k < LS; ++k) acc[k] += s * qo[k];
-        }
+        for (int k = 0; k < LS; ++k) acc[k] = qa[(int64_t)i * LS + k];
+        gather_weighted<LS>(g, d.es, qa, g.off[i], g.off[i + 1], d.apos[i], acc);
         const double inv = 1.0 / (d.S[i] + 1.0);
         double sq = 0.0;
         #pragma unroll
@@ -2162,13 +2206,16 @@ __global__ __launch_bounds__(64) void k_lq_sum(LqDev d) {
 template <int L>
 __global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
     constexpr int LS = 2 * (L + 1);
-    const Tile tl = g.tiles[blockIdx.x];
-    if ((int)threadIdx.x >= tl.count) return;
-    const int i = tl.start + threadIdx.x;
+    __shared__ double sh[64 * LS];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    const bool valid = (int)threadIdx.x < tl.count;
+    const int i = tl.start + (valid ? (int)threadIdx.x : 0);
     const int w = tl.walker;
     const double ql = d.ql[i];
     double c = 0.0;
-    if ((g.group == nullptr || g.group[i]) && d.S[i] > 0.0 && ql > 0.0) {
+    if (valid && (g.group == nullptr || g.group[i]) && d.S[i] > 0.0 && ql > 0.0) {
         const double W = d.wcv[(int64_t)w * kMaxSlotCV + d.slot];
         const double sf = d.sums[2 * w], cv = d.sums[2 * w + 1], N = d.ng[w];
         const double f = sw_f(d.swcv, ql), fp = sw_df(d.swcv, ql);
@@ -2176,8 +2223,10 @@ __global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
                         : d.mode == 1 ? fp : (N > 0.0 ? fp / N : 0.0);
         c = W * wi * (4.0 * kPi / (double)(2 * L + 1)) / ql / (d.S[i] + 1.0);
     }
+    double kv[LS];
     #pragma unroll
-    for (int k = 0; k < LS; ++k) d.K[(int64_t)i * LS + k] = (k < 2 ? c : 2.0 * c) * d.LQ[(int64_t)i * LS + k];
+    for (int k = 0; k < LS; ++k) kv[k] = c != 0.0 ? (k < 2 ? c : 2.0 * c) * d.LQ[(int64_t)i * LS + k] : 0.0;
+    tile_store_rows<LS>(d.K + (int64_t)tl.start * LS, kv, tl.count, sh);
 }
 
 // per entry of a: g_e = (1/S_a) [s' <H_a, Y - q_a> n + s grad_d <H_a, Y>] + s' <K_a, q_j - LQ_a> n
@@ -2185,31 +2234,28 @@ __global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
 template <int L>
 __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
     constexpr int LS = 2 * (L + 1);
-    const Tile tl = g.tiles[blockIdx.x];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
     if ((int)threadIdx.x >= tl.count) return;
     const int i = tl.start + threadIdx.x;
     const int beg = g.off[i], end = g.off[i + 1];
+    const int64_t ap = d.apos[i];
     const double Si = d.S[i];
     if (!(Si > 0.0)) {
-        for (int e = beg; e < end; ++e) { d.g[3 * (int64_t)e] = 0.0; d.g[3 * (int64_t)e + 1] = 0.0; d.g[3 * (int64_t)e + 2] = 0.0; }
+        for (int e = beg; e < end; ++e) {
+            const int64_t o = 3 * (ap + 64 * (int64_t)(e - beg));
+            d.g[o] = 0.0; d.g[o + 1] = 0.0; d.g[o + 2] = 0.0;
+        }
         return;
     }
     const double4 p = g.x4[i];
+    const double* __restrict__ Ka = d.K;
+    const double* __restrict__ qa = d.q;
     double H[LS], Ki[LS];
     #pragma unroll
-    for (int k = 0; k < LS; ++k) { Ki[k] = d.K[(int64_t)i * LS + k]; H[k] = Ki[k]; }
-    for (int e = beg; e < end; ++e) {
-        const int j = g.idx[e];
-        const double4 q = g.x4[j];
-        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < 1e-12) continue;
-        const double s = sw_f(d.swr, sqrt(r2));
-        if (s < d.eps) continue;
-        const double* Ko = d.K + (int64_t)owner_of(g, j) * LS;
-        #pragma unroll
-        for (int k = 0; k < LS; ++k) H[k] += s * Ko[k];
-    }
+    for (int k = 0; k < LS; ++k) { Ki[k] = Ka[(int64_t)i * LS + k]; H[k] = Ki[k]; }
+    gather_weighted<LS>(g, d.es, Ka, beg, end, ap, H);
     double hq = 0.0, kl = 0.0;
     #pragma unroll
     for (int k = 0; k < LS; ++k) {
@@ -2217,46 +2263,59 @@ __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
         kl += Ki[k] * d.LQ[(int64_t)i * LS + k];
     }
     const double iS = 1.0 / Si;
+    double* __restrict__ go = d.g;
     for (int e = beg; e < end; ++e) {
-        const int j = g.idx[e];
-        const double4 q = g.x4[j];
-        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
+        const int64_t el = ap + 64 * (int64_t)(e - beg);
+        const double2 sd = d.es[el];
         double G0 = 0.0, G1 = 0.0, G2 = 0.0;
-        if (r2 >= 1e-12) {
-            const double r = sqrt(r2);
-            const double s = sw_f(d.swr, r);
-            if (s >= d.eps) {
-                const double ds = sw_df(d.swr, r), ir = 1.0 / r;
-                const double nx = dx * ir, ny = dy * ir, nz = dz * ir;
-                double gx, gy, gz;
-                const double hy = ylm64_dot_grad<L>(nx, ny, nz, r, H, gx, gy, gz) - hq;
-                const double* qo = d.q + (int64_t)owner_of(g, j) * LS;
-                double kq = -kl;
-                #pragma unroll
-                for (int k = 0; k < LS; ++k) kq += Ki[k] * qo[k];
-                const double cr = ds * hy * iS + ds * kq, ct = s * iS;
-                G0 = cr * nx + ct * gx;
-                G1 = cr * ny + ct * gy;
-                G2 = cr * nz + ct * gz;
-            }
+        if (sd.x != 0.0) {
+            const double4 q = g.x4[g.idx[e]];
+            const double* qo = qa + (int64_t)g.eown[e] * LS;
+            double kq = -kl;
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) kq += Ki[k] * qo[k];
+            const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz), ir = 1.0 / r;
+            const double nx = dx * ir, ny = dy * ir, nz = dz * ir;
+            double gx, gy, gz;
+            const double hy = ylm64_dot_grad<L>(nx, ny, nz, r, H, gx, gy, gz) - hq;
+            const double cr = sd.y * hy * iS + sd.y * kq, ct = sd.x * iS;
+            G0 = cr * nx + ct * gx;
+            G1 = cr * ny + ct * gy;
+            G2 = cr * nz + ct * gz;
         }
-        d.g[3 * (int64_t)e] = G0; d.g[3 * (int64_t)e + 1] = G1; d.g[3 * (int64_t)e + 2] = G2;
+        go[3 * el] = G0; go[3 * el + 1] = G1; go[3 * el + 2] = G2;
+    }
+}
+
+// per entry: the ELL index of its reverse entry (-1 = none), from the list topology
+__global__ void k_lq_erev(Geo g, int64_t n, const int64_t* __restrict__ apos, int64_t* __restrict__ erev) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int beg = g.off[a];
+    for (int e = beg; e < g.off[a + 1]; ++e) {
+        const int rp = g.revpos[e];
+        erev[apos[a] + 64 * (int64_t)(e - beg)] = rp == 0xFF ? -1 : apos[g.eown[e]] + 64 * (int64_t)rp;
     }
 }
 
 // F_a += sum over a's entries of g_e - sum of g at their reverse entries
 __global__ __launch_bounds__(64) void k_lq_force(Geo g, LqDev d, double* __restrict__ f) {
-    const Tile tl = g.tiles[blockIdx.x];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
     if ((int)threadIdx.x >= tl.count) return;
     const int i = tl.start + threadIdx.x;
+    const double* __restrict__ gg = d.g;
+    const int len = g.off[i + 1] - g.off[i];
+    const int64_t ap = d.apos[i];
     double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-    for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
-        F0 += d.g[3 * (int64_t)e]; F1 += d.g[3 * (int64_t)e + 1]; F2 += d.g[3 * (int64_t)e + 2];
-        const int rp = g.revpos[e];
-        if (rp == 0xFF) continue;
-        const int64_t er = (int64_t)g.off[g.eown[e]] + rp;
-        F0 -= d.g[3 * er]; F1 -= d.g[3 * er + 1]; F2 -= d.g[3 * er + 2];
+    for (int k = 0; k < len; ++k) {
+        const int64_t el = ap + 64 * (int64_t)k;
+        const int64_t er = d.erev[el];
+        F0 += gg[3 * el]; F1 += gg[3 * el + 1]; F2 += gg[3 * el + 2];
+        if (er < 0) continue;
+        F0 -= gg[3 * er]; F1 -= gg[3 * er + 1]; F2 -= gg[3 * er + 2];
     }
     f[3 * (int64_t)i] += F0; f[3 * (int64_t)i + 1] += F1; f[3 * (int64_t)i + 2] += F2;
 }
@@ -2727,7 +2786,7 @@ hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t 
 hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st) {
     if (d.l != 3 && d.l != 4 && d.l != 6) return hipErrorInvalidValue;
     if (g.ntiles > 0) {
-        const dim3 gr(g.ntiles), bl(64);
+        const dim3 gr(xcd_grid(g.ntiles)), bl(64);
         if (d.l == 3) { hipLaunchKernelGGL(k_lq_q<3>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<3>, gr, bl, 0, st, g, d); }
         if (d.l == 4) { hipLaunchKernelGGL(k_lq_q<4>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<4>, gr, bl, 0, st, g, d); }
         if (d.l == 6) { hipLaunchKernelGGL(k_lq_q<6>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<6>, gr, bl, 0, st, g, d); }
@@ -2736,10 +2795,15 @@ hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_lq_erev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, apos, erev);
+    return hipGetLastError();
+}
+
 hipError_t launch_lq_force(const Geo& g, const LqDev& d, double* f, hipStream_t st) {
     if (d.l != 3 && d.l != 4 && d.l != 6) return hipErrorInvalidValue;
     if (g.ntiles == 0) return hipGetLastError();
-    const dim3 gr(g.ntiles), bl(64);
+    const dim3 gr(xcd_grid(g.ntiles)), bl(64);
     if (d.l == 3) { hipLaunchKernelGGL(k_lq_k<3>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<3>, gr, bl, 0, st, g, d); }
     if (d.l == 4) { hipLaunchKernelGGL(k_lq_k<4>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<4>, gr, bl, 0, st, g, d); }
     if (d.l == 6) { hipLaunchKernelGGL(k_lq_k<6>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<6>, gr, bl, 0, st, g, d); }
