@@ -35,6 +35,11 @@ FLOP_FWD_PAIR = 261.0        # q_lm of {Q4, Q6} per selected pair
 FLOP_GRAD_PAIR = 903.0       # gradient of {Q4, Q6} per selected pair
 FLOP_LD_FWD_PAIR = 28.0      # local Q6 average per selected pair
 FLOP_SK = 24.0               # structure factor per (atom, k): 10 forward + 14 gradient
+FP64_PEAK_TFLOPS = 78.6
+# STEINH L 6 per counted list entry, FP64: sigma, sigma', Y_6m and its accumulation (k_lq_q,
+# 140); the weighted q average (k_lq_lq, 28); the weighted K gather (28) and the pair gradient
+# <H, Y>, grad <H, Y>, <K, q_j> (310) of k_lq_pair
+FLOP_LQ_Q, FLOP_LQ_LQ, FLOP_LQ_PAIR = 140.0, 28.0, 338.0
 KB = 8.617343e-5
 A, RC, RG = 5.26, 3.9, 5.9
 METRIC = "atom-steps/s, Q6+Q4 CV+bias forces @256k atoms; NN bias vs hills @<=0.1 kT"
@@ -393,6 +398,59 @@ def run_c5(mdt, args):
                 metric="atom-steps/s, config 5 (learned bias; exact hills-sum result in config.results)")
 
 
+LQ_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE CAL NAME sr SW_FUNC RATIONAL r_0 3.0 "
+           "CAL NAME sc SW_FUNC FERMI r_0 0.35 alpha 10 "
+           "CAL NAME LQ STEINH L 6 all SW_FUNC_r sr SW_FUNC_cv sc "
+           "CV_dim 1 SYMBOL v1 LQ.MEAN_SOLID DIM 1 0 2 2000 v1 WT 1")
+
+
+def run_lq(mdt, args):
+    """STEINH L 6 MEAN_SOLID (RATIONAL weights in r, FERMI f on LQ_6) on the 256k-atom system."""
+    system = c2_system(mdt)
+    n = system.n_local
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, system)
+    fix = mdt.Fix(ctx, LQ_ARGS)
+    fix.init()
+    fix.run(0, 10)
+    steps = max(20, args.steps // 20)
+    dt = timed_run(fix, 10, steps)
+    value = n * steps / dt
+    log(f"lq: {dt * 1e6 / steps:.1f} us/step, {value:.3e} atom-steps/s")
+    kms = fix.profile(10 + steps, 10)
+    log("kernel mean ms: " + json.dumps(kms))
+    x, off, idx, own = system.arrays()
+    entries = float(off[-1])          # RATIONAL r_0 = 3 is above cutoff_eps on the whole 5.9 A list
+    fl = FLOP_LQ_PAIR * entries
+    t = kms["k_lq_pair"] * 1e-3
+    roof = {"kernel": "k_lq_pair", "bound": "fp64", "achieved": fl / t / 1e12, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": fl / t / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+            "algorithmic": f"{FLOP_LQ_PAIR:.0f} FLOP x {entries:.0f} list entries per step", "mean_ms": kms["k_lq_pair"]}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        sm = mdt.System().add_fcc(10, 10, 10, A, 0.6, 0.1, 10 * A / 2.0, seed=2024).build(RG)
+        xs, offs, idxs, owns = sm.arrays()
+        swr = dict(type="RATIONAL", r0=3.0, d0=0.0, n=6, m=12)
+        swc = dict(type="FERMI", r0=0.35, alpha=10.0)
+
+        def one():
+            orc.lq_cv(xs, offs, idxs, sm.n_local, owns, 6, swr, swc, 1e-6, "MEAN_SOLID")
+            orc.lq_grad(xs, offs, idxs, sm.n_local, owns, 6, swr, swc, 1e-6, "MEAN_SOLID")
+        k, tc = oracle_steps(one, min(args.cpu_budget, 20.0))
+        cpu = {"value": sm.n_local * k / tc, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{k} step(s) (CV and exact gradient) of a {sm.n_local}-atom system built the same way "
+                         f"(10^3 FCC cells, lower half jittered 0.6 A) by the oracle (Python loops over list "
+                         f"entries, Y_lm in C) on 1 core, {tc:.1f} s"}
+    cfg = {"workload": "STEINH L 6 MEAN_SOLID on the 256,000-atom half-melted FCC system (RATIONAL r_0 3.0 "
+                       "weights on the 5.9 A list, FERMI r_0 0.35 alpha 10 on LQ_6), DIM [0,2]x2000 WT",
+           "atoms": n, "list_entries": entries, "cv_value": fix.cv("LQ.MEAN_SOLID"), "kernels_ms": kms,
+           "flop_per_entry": {"k_lq_q": FLOP_LQ_Q, "k_lq_lq": FLOP_LQ_LQ, "k_lq_pair": FLOP_LQ_PAIR},
+           "path_fp64_tflops": (FLOP_LQ_Q + FLOP_LQ_LQ + FLOP_LQ_PAIR) * entries / (dt / steps) / 1e12}
+    return line(value, steps, 10, dt * 1e3 / steps, "f64", cfg, roof, cpu,
+                metric="atom-steps/s, STEINH L 6 MEAN_SOLID CV+bias forces @256k atoms")
+
+
 def run_copy(mdt, args):
     """Plain device copy of a known size: the calibration for HBM byte counters and the
     achievable copy bandwidth of the chip (reads 1 GiB, writes 1 GiB per launch)."""
@@ -424,7 +482,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--profile-steps", type=int, default=50)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lq", "copy"])
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -436,7 +494,7 @@ def main():
     if not os.path.exists(mdt.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
-    out = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5,
+    out = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "lq": run_lq,
            "copy": run_copy}[args.config](mdt, args)
     print(json.dumps(out), flush=True)
 

@@ -270,8 +270,8 @@ struct LqDev {
     int32_t slot;
     int32_t n_walkers;
 };
-hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st);
-hipError_t launch_lq_force(const Geo& g, const LqDev& d, double* f, hipStream_t st);
+// stage 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values); 3 k_lq_k, 4 k_lq_pair, 5 k_lq_force (forces)
+hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st);
 hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st);
 
 // DISTANCE CV between two atoms of each walker (one thread per walker).

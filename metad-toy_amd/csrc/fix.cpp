@@ -1575,7 +1575,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             const LqDev d = lq_dev(fx, c);
             Geo gs = g;
             gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
-            T.run("k_lq_value", st, [&] { HIPCHK(launch_lq_value(gs, d, st)); });
+            T.run("k_lq_q", st, [&] { HIPCHK(launch_lq(0, gs, d, nullptr, st)); });
+            T.run("k_lq_lq", st, [&] { HIPCHK(launch_lq(1, gs, d, nullptr, st)); });
+            T.run("k_lq_sum", st, [&] { HIPCHK(launch_lq(2, gs, d, nullptr, st)); });
         }
     }
     T.run("k_reduce", st, [&] { HIPCHK(launch_reduce(fx->red, st)); });
@@ -1612,7 +1614,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             const LqDev d = lq_dev(fx, c);
             Geo gs = g;
             gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
-            T.run("k_lq_force", st, [&] { HIPCHK(launch_lq_force(gs, d, f, st)); });
+            T.run("k_lq_k", st, [&] { HIPCHK(launch_lq(3, gs, d, f, st)); });
+            T.run("k_lq_pair", st, [&] { HIPCHK(launch_lq(4, gs, d, f, st)); });
+            T.run("k_lq_force", st, [&] { HIPCHK(launch_lq(5, gs, d, f, st)); });
         }
     }
     // The values of unbiased local-Q CVs fused into the force pass (their tile sums

@@ -2186,13 +2186,15 @@ __global__ __launch_bounds__(64) void k_lq_lq(Geo g, LqDev d) {
     if (threadIdx.x == 0) { d.part[2 * t] = sq; d.part[2 * t + 1] = sf; }
 }
 
-// per walker: sum q f and sum f over the tiles in tile order -> CV (one wave)
-__global__ __launch_bounds__(64) void k_lq_sum(LqDev d) {
+// per walker: sum q f and sum f over the tiles in a fixed order -> CV (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_lq_sum(LqDev d) {
+    __shared__ double sh[2 * (kBlock / 64)];
     const int w = blockIdx.x;
     const int t0 = d.tile_begin[w], t1 = d.tile_begin[w + 1];
-    double a = 0.0, b = 0.0;
-    for (int t = t0 + threadIdx.x; t < t1; t += 64) { a += d.part[2 * t]; b += d.part[2 * t + 1]; }
-    const double sq = wave_sum(a), sf = wave_sum(b);
+    double v[2] = {0.0, 0.0};
+    for (int t = t0 + threadIdx.x; t < t1; t += kBlock) { v[0] += d.part[2 * t]; v[1] += d.part[2 * t + 1]; }
+    block_sum<2>(v, sh);
+    const double sq = v[0], sf = v[1];
     if (threadIdx.x != 0) return;
     const double N = d.ng[w];
     const double cv = d.mode == 0 ? (sf != 0.0 ? sq / sf : 0.0) : d.mode == 1 ? sf : (N > 0.0 ? sf / N : 0.0);
@@ -2783,16 +2785,31 @@ hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_lq_value(const Geo& g, const LqDev& d, hipStream_t st) {
-    if (d.l != 3 && d.l != 4 && d.l != 6) return hipErrorInvalidValue;
-    if (g.ntiles > 0) {
-        const dim3 gr(xcd_grid(g.ntiles)), bl(64);
-        if (d.l == 3) { hipLaunchKernelGGL(k_lq_q<3>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<3>, gr, bl, 0, st, g, d); }
-        if (d.l == 4) { hipLaunchKernelGGL(k_lq_q<4>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<4>, gr, bl, 0, st, g, d); }
-        if (d.l == 6) { hipLaunchKernelGGL(k_lq_q<6>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_lq<6>, gr, bl, 0, st, g, d); }
+// one STEINH L kernel: 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values, before the reduce);
+// 3 k_lq_k, 4 k_lq_pair, 5 k_lq_force (forces, after it)
+template <int L>
+hipError_t launch_lq_l(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st) {
+    const dim3 gr(xcd_grid(g.ntiles)), bl(64);
+    if (stage == 2) { hipLaunchKernelGGL(k_lq_sum, dim3(d.n_walkers), dim3(kBlock), 0, st, d); return hipGetLastError(); }
+    if (g.ntiles == 0) return hipGetLastError();
+    switch (stage) {
+        case 0: hipLaunchKernelGGL(k_lq_q<L>, gr, bl, 0, st, g, d); break;
+        case 1: hipLaunchKernelGGL(k_lq_lq<L>, gr, bl, 0, st, g, d); break;
+        case 3: hipLaunchKernelGGL(k_lq_k<L>, gr, bl, 0, st, g, d); break;
+        case 4: hipLaunchKernelGGL(k_lq_pair<L>, gr, bl, 0, st, g, d); break;
+        case 5: hipLaunchKernelGGL(k_lq_force, gr, bl, 0, st, g, d, f); break;
+        default: return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_lq_sum, dim3(d.n_walkers), dim3(64), 0, st, d);
     return hipGetLastError();
+}
+
+hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st) {
+    switch (d.l) {
+        case 3: return launch_lq_l<3>(stage, g, d, f, st);
+        case 4: return launch_lq_l<4>(stage, g, d, f, st);
+        case 6: return launch_lq_l<6>(stage, g, d, f, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st) {
@@ -2800,15 +2817,5 @@ hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t*
     return hipGetLastError();
 }
 
-hipError_t launch_lq_force(const Geo& g, const LqDev& d, double* f, hipStream_t st) {
-    if (d.l != 3 && d.l != 4 && d.l != 6) return hipErrorInvalidValue;
-    if (g.ntiles == 0) return hipGetLastError();
-    const dim3 gr(xcd_grid(g.ntiles)), bl(64);
-    if (d.l == 3) { hipLaunchKernelGGL(k_lq_k<3>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<3>, gr, bl, 0, st, g, d); }
-    if (d.l == 4) { hipLaunchKernelGGL(k_lq_k<4>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<4>, gr, bl, 0, st, g, d); }
-    if (d.l == 6) { hipLaunchKernelGGL(k_lq_k<6>, gr, bl, 0, st, g, d); hipLaunchKernelGGL(k_lq_pair<6>, gr, bl, 0, st, g, d); }
-    hipLaunchKernelGGL(k_lq_force, gr, bl, 0, st, g, d, f);
-    return hipGetLastError();
-}
 
 }  // namespace mdt
