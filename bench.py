@@ -37,9 +37,9 @@ FLOP_LD_FWD_PAIR = 28.0      # local Q6 average per selected pair
 FLOP_SK = 24.0               # structure factor per (atom, k): 10 forward + 14 gradient
 FP64_PEAK_TFLOPS = 78.6
 # STEINH L 6 per counted list entry, FP64: sigma, sigma', Y_6m and its accumulation (k_lq_q,
-# 140); the weighted q average (k_lq_lq, 28); the weighted K gather (28) and the pair gradient
-# <H, Y>, grad <H, Y>, <K, q_j> (310) of k_lq_pair
-FLOP_LQ_Q, FLOP_LQ_LQ, FLOP_LQ_PAIR = 140.0, 28.0, 338.0
+# 140); the weighted q average (k_lq_lq, 28); the weighted K sum and <K_j, q_a> (k_lq_h, 56);
+# the pair gradient <H, Y>, grad <H, Y> and its assembly (k_lq_pair, 282)
+FLOP_LQ_Q, FLOP_LQ_LQ, FLOP_LQ_H, FLOP_LQ_PAIR = 140.0, 28.0, 56.0, 282.0
 KB = 8.617343e-5
 A, RC, RG = 5.26, 3.9, 5.9
 METRIC = "atom-steps/s, Q6+Q4 CV+bias forces @256k atoms; NN bias vs hills @<=0.1 kT"
@@ -445,8 +445,8 @@ def run_lq(mdt, args):
     cfg = {"workload": "STEINH L 6 MEAN_SOLID on the 256,000-atom half-melted FCC system (RATIONAL r_0 3.0 "
                        "weights on the 5.9 A list, FERMI r_0 0.35 alpha 10 on LQ_6), DIM [0,2]x2000 WT",
            "atoms": n, "list_entries": entries, "cv_value": fix.cv("LQ.MEAN_SOLID"), "kernels_ms": kms,
-           "flop_per_entry": {"k_lq_q": FLOP_LQ_Q, "k_lq_lq": FLOP_LQ_LQ, "k_lq_pair": FLOP_LQ_PAIR},
-           "path_fp64_tflops": (FLOP_LQ_Q + FLOP_LQ_LQ + FLOP_LQ_PAIR) * entries / (dt / steps) / 1e12}
+           "flop_per_entry": {"k_lq_q": FLOP_LQ_Q, "k_lq_lq": FLOP_LQ_LQ, "k_lq_h": FLOP_LQ_H, "k_lq_pair": FLOP_LQ_PAIR},
+           "path_fp64_tflops": (FLOP_LQ_Q + FLOP_LQ_LQ + FLOP_LQ_H + FLOP_LQ_PAIR) * entries / (dt / steps) / 1e12}
     return line(value, steps, 10, dt * 1e3 / steps, "f64", cfg, roof, cpu,
                 metric="atom-steps/s, STEINH L 6 MEAN_SOLID CV+bias forces @256k atoms")
 

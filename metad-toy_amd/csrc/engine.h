@@ -256,6 +256,9 @@ struct LqDev {
     double* LQ;               // per local atom LQ_lm (0 outside the group)
     double* ql;               // per local atom LQ_l
     double* K;                // per local atom W dCV/dLQ_lm / (S + 1)
+    double* H;                // per local atom dCV/dq_lm (K plus the weighted K of its entries)
+    double2* hk;              // per local atom (<H, q>, <K, LQ>)
+    double* u;                // per ELL entry a -> j: <K_owner(j), q_a>
     const int64_t* apos;      // per local atom: ELL position of its first entry (tile base + lane);
                               // entry k at apos + 64 k
     double* g;                // per ELL entry: d(W CV)/d(pair vector), 3 doubles
@@ -270,7 +273,7 @@ struct LqDev {
     int32_t slot;
     int32_t n_walkers;
 };
-// stage 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values); 3 k_lq_k, 4 k_lq_pair, 5 k_lq_force (forces)
+// stage 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values); 3 k_lq_k, 4 k_lq_h, 5 k_lq_pair, 6 k_lq_force (forces)
 hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st);
 hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st);
 

@@ -403,7 +403,8 @@ struct CvH {
     double lq_eps = 1e-6;
     std::string lq_group;
     DBuf<double> lq_S, lq_q, lq_LQ, lq_ql, lq_K, lq_g, lq_part, lq_sums, lq_ng;
-    DBuf<double2> lq_es;
+    DBuf<double2> lq_es, lq_hk;
+    DBuf<double> lq_H, lq_u;
     DBuf<int32_t> lq_tb;
     DBuf<int64_t> lq_apos, lq_erev;
 };
@@ -1106,7 +1107,8 @@ void fix_build(mdt_fix* fx) {
             const size_t LS = 2 * (size_t)(c->l + 1), n1 = (size_t)std::max<int64_t>(NT, 1);
             c->lq_ng.alloc(nw); c->lq_ng.upload(ng.data(), nw, st);
             c->lq_S.alloc(n1); c->lq_ql.alloc(n1);
-            c->lq_q.alloc(n1 * LS); c->lq_LQ.alloc(n1 * LS); c->lq_K.alloc(n1 * LS);
+            c->lq_q.alloc(n1 * LS); c->lq_LQ.alloc(n1 * LS); c->lq_K.alloc(n1 * LS); c->lq_H.alloc(n1 * LS);
+            c->lq_hk.alloc(n1);
             // per-tile ELL layout of the per-entry words: tile t holds 64 x (its longest list)
             std::vector<int32_t> hoff((size_t)NT + 1, 0);
             HIPCHK(hipStreamSynchronize(st));
@@ -1125,6 +1127,7 @@ void fix_build(mdt_fix* fx) {
             c->lq_apos.alloc(apos.size()); c->lq_apos.upload(apos.data(), apos.size(), st);
             c->lq_g.alloc(ell1 * 3);
             c->lq_es.alloc(ell1);
+            c->lq_u.alloc(ell1);
             c->lq_erev.alloc(ell1);
             HIPCHK(launch_lq_erev(ctx->geo(), NT, c->lq_apos.p, c->lq_erev.p, st));
             c->lq_part.alloc(2 * (size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
@@ -1530,6 +1533,7 @@ LqDev lq_dev(mdt_fix* fx, CvH* c) {
     d.swr = c->lq_swr; d.swcv = c->lq_swcv; d.eps = c->lq_eps;
     d.S = c->lq_S.p; d.q = c->lq_q.p; d.LQ = c->lq_LQ.p; d.ql = c->lq_ql.p; d.K = c->lq_K.p; d.g = c->lq_g.p;
     d.es = c->lq_es.p; d.erev = c->lq_erev.p; d.apos = c->lq_apos.p;
+    d.H = c->lq_H.p; d.hk = c->lq_hk.p; d.u = c->lq_u.p;
     d.part = c->lq_part.p; d.sums = c->lq_sums.p; d.tile_begin = c->lq_tb.p; d.ng = c->lq_ng.p;
     d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot; d.n_walkers = fx->nw();
     return d;
@@ -1615,8 +1619,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             Geo gs = g;
             gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
             T.run("k_lq_k", st, [&] { HIPCHK(launch_lq(3, gs, d, f, st)); });
-            T.run("k_lq_pair", st, [&] { HIPCHK(launch_lq(4, gs, d, f, st)); });
-            T.run("k_lq_force", st, [&] { HIPCHK(launch_lq(5, gs, d, f, st)); });
+            T.run("k_lq_h", st, [&] { HIPCHK(launch_lq(4, gs, d, f, st)); });
+            T.run("k_lq_pair", st, [&] { HIPCHK(launch_lq(5, gs, d, f, st)); });
+            T.run("k_lq_force", st, [&] { HIPCHK(launch_lq(6, gs, d, f, st)); });
         }
     }
     // The values of unbiased local-Q CVs fused into the force pass (their tile sums

@@ -2231,8 +2231,54 @@ __global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
     tile_store_rows<LS>(d.K + (int64_t)tl.start * LS, kv, tl.count, sh);
 }
 
-// per entry of a: g_e = (1/S_a) [s' <H_a, Y - q_a> n + s grad_d <H_a, Y>] + s' <K_a, q_j - LQ_a> n
-// with H_a = K_a + sum over counted entries of s K_owner(j)
+// H_a = K_a + sum over counted entries of s K_owner(j), with <H_a, q_a> and <K_a, LQ_a>; for
+// each counted entry e = (a -> j) also u_e = <K_owner(j), q_a>, which is the <K_i, q_j> the
+// reverse entry of e needs (i = owner(j), j = a): the pair pass reads 8 bytes there instead
+// of gathering a neighbour's q row
+template <int L>
+__global__ __launch_bounds__(64) void k_lq_h(Geo g, LqDev d) {
+    constexpr int LS = 2 * (L + 1);
+    __shared__ double sh[64 * LS];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= g.ntiles) return;
+    const Tile tl = g.tiles[t];
+    const bool valid = (int)threadIdx.x < tl.count;
+    const int i = tl.start + (valid ? (int)threadIdx.x : 0);
+    const double* __restrict__ Ka = d.K;
+    const double* __restrict__ qa = d.q;
+    double H[LS], qi[LS];
+    double hq = 0.0, kl = 0.0;
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) { H[k] = Ka[(int64_t)i * LS + k]; qi[k] = qa[(int64_t)i * LS + k]; }
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) kl += H[k] * d.LQ[(int64_t)i * LS + k];
+    if (valid) {
+        // u on every entry, counted here or not: the reverse entry may count where this one
+        // falls below cutoff_eps by rounding, and u does not depend on sigma
+        const bool live = d.S[i] > 0.0;
+        const int beg = g.off[i], end = g.off[i + 1];
+        const int64_t ap = d.apos[i];
+        for (int e = beg; e < end; ++e) {
+            const int64_t el = ap + 64 * (int64_t)(e - beg);
+            const double s = live ? d.es[el].x : 0.0;
+            const double* ko = Ka + (int64_t)g.eown[e] * LS;
+            double v[LS];
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) v[k] = ko[k];
+            double u = 0.0;
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) { H[k] += s * v[k]; u += v[k] * qi[k]; }
+            d.u[el] = u;
+        }
+        #pragma unroll
+        for (int k = 0; k < LS; ++k) hq += H[k] * qi[k];
+    }
+    if (valid) d.hk[i] = make_double2(hq, kl);
+    tile_store_rows<LS>(d.H + (int64_t)tl.start * LS, H, tl.count, sh);
+}
+
+// per entry of a: g_e = (1/S_a) [s' <H_a, Y - q_a> n + s grad_d <H_a, Y>] + s' (<K_a, q_j> - <K_a, LQ_a>) n,
+// <K_a, q_j> read at the reverse entry (u, formed by k_lq_h)
 template <int L>
 __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
     constexpr int LS = 2 * (L + 1);
@@ -2244,43 +2290,33 @@ __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
     const int beg = g.off[i], end = g.off[i + 1];
     const int64_t ap = d.apos[i];
     const double Si = d.S[i];
+    double* __restrict__ go = d.g;
     if (!(Si > 0.0)) {
         for (int e = beg; e < end; ++e) {
             const int64_t o = 3 * (ap + 64 * (int64_t)(e - beg));
-            d.g[o] = 0.0; d.g[o + 1] = 0.0; d.g[o + 2] = 0.0;
+            go[o] = 0.0; go[o + 1] = 0.0; go[o + 2] = 0.0;
         }
         return;
     }
     const double4 p = g.x4[i];
-    const double* __restrict__ Ka = d.K;
-    const double* __restrict__ qa = d.q;
-    double H[LS], Ki[LS];
+    double H[LS];
     #pragma unroll
-    for (int k = 0; k < LS; ++k) { Ki[k] = Ka[(int64_t)i * LS + k]; H[k] = Ki[k]; }
-    gather_weighted<LS>(g, d.es, Ka, beg, end, ap, H);
-    double hq = 0.0, kl = 0.0;
-    #pragma unroll
-    for (int k = 0; k < LS; ++k) {
-        hq += H[k] * d.q[(int64_t)i * LS + k];
-        kl += Ki[k] * d.LQ[(int64_t)i * LS + k];
-    }
+    for (int k = 0; k < LS; ++k) H[k] = d.H[(int64_t)i * LS + k];
+    const double2 hk = d.hk[i];
     const double iS = 1.0 / Si;
-    double* __restrict__ go = d.g;
     for (int e = beg; e < end; ++e) {
         const int64_t el = ap + 64 * (int64_t)(e - beg);
         const double2 sd = d.es[el];
         double G0 = 0.0, G1 = 0.0, G2 = 0.0;
         if (sd.x != 0.0) {
             const double4 q = g.x4[g.idx[e]];
-            const double* qo = qa + (int64_t)g.eown[e] * LS;
-            double kq = -kl;
-            #pragma unroll
-            for (int k = 0; k < LS; ++k) kq += Ki[k] * qo[k];
+            const int64_t er = d.erev[el];
+            const double kq = (er >= 0 ? d.u[er] : 0.0) - hk.y;
             const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
             const double r = sqrt(dx * dx + dy * dy + dz * dz), ir = 1.0 / r;
             const double nx = dx * ir, ny = dy * ir, nz = dz * ir;
             double gx, gy, gz;
-            const double hy = ylm64_dot_grad<L>(nx, ny, nz, r, H, gx, gy, gz) - hq;
+            const double hy = ylm64_dot_grad<L>(nx, ny, nz, r, H, gx, gy, gz) - hk.x;
             const double cr = sd.y * hy * iS + sd.y * kq, ct = sd.x * iS;
             G0 = cr * nx + ct * gx;
             G1 = cr * ny + ct * gy;
@@ -2786,7 +2822,7 @@ hipError_t launch_sf_force(const Geo& g, const SfDev& d, double* f, hipStream_t 
 }
 
 // one STEINH L kernel: 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values, before the reduce);
-// 3 k_lq_k, 4 k_lq_pair, 5 k_lq_force (forces, after it)
+// 3 k_lq_k, 4 k_lq_h, 5 k_lq_pair, 6 k_lq_force (forces, after it)
 template <int L>
 hipError_t launch_lq_l(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st) {
     const dim3 gr(xcd_grid(g.ntiles)), bl(64);
@@ -2796,8 +2832,9 @@ hipError_t launch_lq_l(int stage, const Geo& g, const LqDev& d, double* f, hipSt
         case 0: hipLaunchKernelGGL(k_lq_q<L>, gr, bl, 0, st, g, d); break;
         case 1: hipLaunchKernelGGL(k_lq_lq<L>, gr, bl, 0, st, g, d); break;
         case 3: hipLaunchKernelGGL(k_lq_k<L>, gr, bl, 0, st, g, d); break;
-        case 4: hipLaunchKernelGGL(k_lq_pair<L>, gr, bl, 0, st, g, d); break;
-        case 5: hipLaunchKernelGGL(k_lq_force, gr, bl, 0, st, g, d, f); break;
+        case 4: hipLaunchKernelGGL(k_lq_h<L>, gr, bl, 0, st, g, d); break;
+        case 5: hipLaunchKernelGGL(k_lq_pair<L>, gr, bl, 0, st, g, d); break;
+        case 6: hipLaunchKernelGGL(k_lq_force, gr, bl, 0, st, g, d, f); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
