@@ -264,6 +264,8 @@ struct LqDev {
     double* g;                // per ELL entry: d(W CV)/d(pair vector), 3 doubles
     double2* es;              // per ELL entry: (sigma, dsigma/dr), sigma = 0 where the entry does not count
     const int64_t* erev;      // per ELL entry: ELL index of the reverse entry, -1 = none
+    const int32_t* eidx;      // per ELL entry: the list index (all-space atom)
+    const int32_t* eown;      // per ELL entry: its local owner
     double* part;             // per tile (sum q f, sum f)
     double* sums;             // per walker (sum f, CV)
     const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)
@@ -275,7 +277,8 @@ struct LqDev {
 };
 // stage 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values); 3 k_lq_k, 4 k_lq_h, 5 k_lq_pair, 6 k_lq_force (forces)
 hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st);
-hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st);
+hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, int32_t* eidx, int32_t* eown,
+                          hipStream_t st);
 
 // DISTANCE CV between two atoms of each walker (one thread per walker).
 struct DistDev {

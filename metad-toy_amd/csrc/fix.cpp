@@ -405,7 +405,7 @@ struct CvH {
     DBuf<double> lq_S, lq_q, lq_LQ, lq_ql, lq_K, lq_g, lq_part, lq_sums, lq_ng;
     DBuf<double2> lq_es, lq_hk;
     DBuf<double> lq_H, lq_u;
-    DBuf<int32_t> lq_tb;
+    DBuf<int32_t> lq_tb, lq_eidx, lq_eown;
     DBuf<int64_t> lq_apos, lq_erev;
 };
 
@@ -1129,7 +1129,9 @@ void fix_build(mdt_fix* fx) {
             c->lq_es.alloc(ell1);
             c->lq_u.alloc(ell1);
             c->lq_erev.alloc(ell1);
-            HIPCHK(launch_lq_erev(ctx->geo(), NT, c->lq_apos.p, c->lq_erev.p, st));
+            c->lq_eidx.alloc(ell1);
+            c->lq_eown.alloc(ell1);
+            HIPCHK(launch_lq_erev(ctx->geo(), NT, c->lq_apos.p, c->lq_erev.p, c->lq_eidx.p, c->lq_eown.p, st));
             c->lq_part.alloc(2 * (size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
             c->lq_sums.alloc(2 * (size_t)nw);
             c->lq_tb.alloc(nw + 1); c->lq_tb.upload(ctx->tile_begin.data(), nw + 1, st);
@@ -1534,6 +1536,7 @@ LqDev lq_dev(mdt_fix* fx, CvH* c) {
     d.S = c->lq_S.p; d.q = c->lq_q.p; d.LQ = c->lq_LQ.p; d.ql = c->lq_ql.p; d.K = c->lq_K.p; d.g = c->lq_g.p;
     d.es = c->lq_es.p; d.erev = c->lq_erev.p; d.apos = c->lq_apos.p;
     d.H = c->lq_H.p; d.hk = c->lq_hk.p; d.u = c->lq_u.p;
+    d.eidx = c->lq_eidx.p; d.eown = c->lq_eown.p;
     d.part = c->lq_part.p; d.sums = c->lq_sums.p; d.tile_begin = c->lq_tb.p; d.ng = c->lq_ng.p;
     d.direct = fx->direct.p; d.wcv = fx->wcv.p; d.slot = c->slot; d.n_walkers = fx->nw();
     return d;

@@ -2070,14 +2070,14 @@ __device__ __forceinline__ double ylm64_dot_grad(double nx, double ny, double u,
 // acc += sum over entries [beg, end) of sigma_e rec[owner_e]; the per-entry words come
 // first and two entries' records are loaded before either is used
 template <int LS>
-__device__ __forceinline__ void gather_weighted(const Geo& g, const double2* __restrict__ es,
+__device__ __forceinline__ void gather_weighted(const int32_t* __restrict__ eo, const double2* __restrict__ es,
                                                 const double* __restrict__ rec, int beg, int end, int64_t ap,
                                                 double* acc) {
     int e = beg;
     for (; e + 1 < end; e += 2) {
         const double s0 = es[ap + 64 * (int64_t)(e - beg)].x, s1 = es[ap + 64 * (int64_t)(e + 1 - beg)].x;
-        const double* r0 = rec + (int64_t)g.eown[e] * LS;
-        const double* r1 = rec + (int64_t)g.eown[e + 1] * LS;
+        const double* r0 = rec + (int64_t)eo[ap + 64 * (int64_t)(e - beg)] * LS;
+        const double* r1 = rec + (int64_t)eo[ap + 64 * (int64_t)(e + 1 - beg)] * LS;
         double v0[LS], v1[LS];
         #pragma unroll
         for (int k = 0; k < LS; ++k) { v0[k] = r0[k]; v1[k] = r1[k]; }
@@ -2086,7 +2086,7 @@ __device__ __forceinline__ void gather_weighted(const Geo& g, const double2* __r
     }
     if (e < end) {
         const double s0 = es[ap + 64 * (int64_t)(e - beg)].x;
-        const double* r0 = rec + (int64_t)g.eown[e] * LS;
+        const double* r0 = rec + (int64_t)eo[ap + 64 * (int64_t)(e - beg)] * LS;
         #pragma unroll
         for (int k = 0; k < LS; ++k) acc[k] += s0 * r0[k];
     }
@@ -2123,7 +2123,7 @@ __global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
     const int beg = g.off[i];
     const int64_t ap = d.apos[i];
     for (int e = beg; e < (valid ? g.off[i + 1] : beg); ++e) {
-        const double4 q = g.x4[g.idx[e]];
+        const double4 q = g.x4[d.eidx[ap + 64 * (int64_t)(e - beg)]];
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         double s = 0.0, ds = 0.0;
@@ -2165,7 +2165,7 @@ __global__ __launch_bounds__(64) void k_lq_lq(Geo g, LqDev d) {
         double acc[LS];
         #pragma unroll
         for (int k = 0; k < LS; ++k) acc[k] = qa[(int64_t)i * LS + k];
-        gather_weighted<LS>(g, d.es, qa, g.off[i], g.off[i + 1], d.apos[i], acc);
+        gather_weighted<LS>(d.eown, d.es, qa, g.off[i], g.off[i + 1], d.apos[i], acc);
         const double inv = 1.0 / (d.S[i] + 1.0);
         double sq = 0.0;
         #pragma unroll
@@ -2261,7 +2261,7 @@ __global__ __launch_bounds__(64) void k_lq_h(Geo g, LqDev d) {
         for (int e = beg; e < end; ++e) {
             const int64_t el = ap + 64 * (int64_t)(e - beg);
             const double s = live ? d.es[el].x : 0.0;
-            const double* ko = Ka + (int64_t)g.eown[e] * LS;
+            const double* ko = Ka + (int64_t)d.eown[el] * LS;
             double v[LS];
             #pragma unroll
             for (int k = 0; k < LS; ++k) v[k] = ko[k];
@@ -2309,7 +2309,7 @@ __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
         const double2 sd = d.es[el];
         double G0 = 0.0, G1 = 0.0, G2 = 0.0;
         if (sd.x != 0.0) {
-            const double4 q = g.x4[g.idx[e]];
+            const double4 q = g.x4[d.eidx[el]];
             const int64_t er = d.erev[el];
             const double kq = (er >= 0 ? d.u[er] : 0.0) - hk.y;
             const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
@@ -2327,13 +2327,17 @@ __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
 }
 
 // per entry: the ELL index of its reverse entry (-1 = none), from the list topology
-__global__ void k_lq_erev(Geo g, int64_t n, const int64_t* __restrict__ apos, int64_t* __restrict__ erev) {
+__global__ void k_lq_erev(Geo g, int64_t n, const int64_t* __restrict__ apos, int64_t* __restrict__ erev,
+                          int32_t* __restrict__ eidx, int32_t* __restrict__ eown) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     const int beg = g.off[a];
     for (int e = beg; e < g.off[a + 1]; ++e) {
         const int rp = g.revpos[e];
-        erev[apos[a] + 64 * (int64_t)(e - beg)] = rp == 0xFF ? -1 : apos[g.eown[e]] + 64 * (int64_t)rp;
+        const int64_t el = apos[a] + 64 * (int64_t)(e - beg);
+        erev[el] = rp == 0xFF ? -1 : apos[g.eown[e]] + 64 * (int64_t)rp;
+        eidx[el] = g.idx[e];
+        eown[el] = g.eown[e];
     }
 }
 
@@ -2849,8 +2853,9 @@ hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStre
     }
 }
 
-hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_lq_erev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, apos, erev);
+hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, int32_t* eidx, int32_t* eown,
+                          hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_lq_erev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, apos, erev, eidx, eown);
     return hipGetLastError();
 }
 
