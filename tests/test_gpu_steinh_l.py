@@ -149,3 +149,31 @@ def test_steinh_l_256k_perfect_fcc_known_answer(mdt):
     assert np.abs(F.sum(axis=0)).max() <= 1e-9 * np.abs(F).max() * s.n_local
     fix.close()
     ctx.close()
+
+
+def test_steinh_l_on_device_built_lists(mdt):
+    """Lists from the device neighbour build give the same bits as the host's (same order,
+    so the same per-tile ELL layout and sums)."""
+    s = fcc_system(mdt, 5, sigma=0.3, sigma_above=0.1, seed=131)
+    n = s.n_local
+    out = []
+    for use_device in (False, True):
+        ctx = mdt.Context(0, 1)
+        if use_device:
+            lo, hi = s.box()
+            ctx.build_neighbors(s.arrays()[0][:n], lo, hi, 5.9)
+            ctx.bind_built_neighbors(0)
+        else:
+            ctx.bind_system(0, s)
+        fix = mdt.Fix(ctx, _deck(6, "NSOLID", "all").replace("PACE 0", "PACE 1"))
+        fix.init()
+        f = np.zeros((n, 3))
+        for step in range(3):
+            fix.post_force(step, f)
+        out.append((f.copy(), fix.cv("LQ.NSOLID")))
+        fix.close()
+        ctx.close()
+    print(f"[steinh L device lists] cv={out[1][1]:.12f} max|F|={np.abs(out[1][0]).max():.3e}", flush=True)
+    assert np.abs(out[0][0]).max() > 0.0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
