@@ -2123,7 +2123,7 @@ __global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
     const int beg = g.off[i];
     const int64_t ap = d.apos[i];
     for (int e = beg; e < (valid ? g.off[i + 1] : beg); ++e) {
-        const double4 q = g.x4[d.eidx[ap + 64 * (int64_t)(e - beg)]];
+        const double4 q = g.x4[g.idx[e]];
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         double s = 0.0, ds = 0.0;
