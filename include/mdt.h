@@ -105,6 +105,16 @@ int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_devic
 int mdt_fix_create(mdt_ctx* ctx, int argc, const char* const* argv, mdt_fix** fix);
 void mdt_fix_destroy(mdt_fix* fix);
 
+/* Parsed value of a fix keyword, for tools that read a `fix metad` line without
+ * running it (e.g. on a parse-only context, device -1): key "GAUSSIAN" gives
+ * (sigma, height, biasf); "PACE", "CV_dim", "TEMP", "BOLTZ", "WT",
+ * "METAD_RESTART", "Gaussian_Hill_type" one value; "SIGMAS" one per dimension;
+ * "DIM" (lower, upper, bins) per dimension.  *n receives the count; at most cap
+ * values are written to out.  Replaces the keyword scan of the post-processing
+ * script tools/plot_hills.py:41-119 with the fix's own parser
+ * (FixMetadynamics::FixMetadynamics, src/core/fix_crystallize.cu:32-302). */
+int mdt_fix_param(mdt_fix* fix, const char* key, double* out, int cap, int* n);
+
 /* First-run set-up: allocate the bias, open/replay HILLS (METAD_RESTART 1).
  * Replaces FixMetadynamics::init (:343-378), GH_t0_uniformGrid::init_set_mode
  * and init_hills (src/grids/GH_t0_uniformGrid.cu:73-192).  Called implicitly

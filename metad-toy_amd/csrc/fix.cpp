@@ -953,6 +953,7 @@ bool prog_uses(const ExprProg& p, int sym) {
 }
 
 void open_hills(mdt_fix* fx);
+void ensure_hills(mdt_fix* fx, int64_t extra);
 
 void fix_build(mdt_fix* fx) {
     mdt_ctx* ctx = fx->ctx;
@@ -1298,6 +1299,8 @@ void replay_hills(mdt_fix* fx, int w, const std::string& path) {
     fclose(fp);
     if (h.empty()) return;
     hipStream_t st = fx->ctx->stream;
+    // a hills-list bias keeps every replayed hill: grow its device log first
+    if (!fx->grid_bias()) ensure_hills(fx, (int64_t)h.size());
     DBuf<double> ds, dh;
     // hills with different sigma are deposited in runs of equal sigma
     size_t a = 0;
@@ -1861,6 +1864,27 @@ void mdt_fix_destroy(mdt_fix* fx) {
     if (fx->device >= 0) (void)hipSetDevice(fx->device);
     delete fx;
     (void)hipGetLastError();
+}
+
+int mdt_fix_param(mdt_fix* fx, const char* key, double* out, int cap, int* n) {
+    FIX_BEGIN(fx)
+    const std::string k = key ? key : "";
+    std::vector<double> v;
+    if (k == "GAUSSIAN") v = {fx->sigma, fx->height0, fx->biasf};
+    else if (k == "PACE") v = {(double)fx->pace};
+    else if (k == "CV_dim") v = {(double)fx->cv_dim};
+    else if (k == "TEMP") v = {fx->temp};
+    else if (k == "BOLTZ") v = {fx->boltz};
+    else if (k == "WT") v = {fx->wt ? 1.0 : 0.0};
+    else if (k == "METAD_RESTART") v = {fx->restart ? 1.0 : 0.0};
+    else if (k == "Gaussian_Hill_type") v = {(double)fx->hill_type};
+    else if (k == "SIGMAS") v = fx->sigmas;
+    else if (k == "DIM")
+        for (int d = 0; d < fx->cv_dim; ++d) { v.push_back(fx->lo[d]); v.push_back(fx->hi[d]); v.push_back((double)fx->nb[d]); }
+    else throw Err(MDT_EINVAL, fmt("unknown fix parameter '%s'", k.c_str()));
+    if (n) *n = (int)v.size();
+    if (out) for (int q = 0; q < cap && q < (int)v.size(); ++q) out[q] = v[q];
+    API_END
 }
 
 int mdt_fix_init(mdt_fix* fx) {

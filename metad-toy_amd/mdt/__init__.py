@@ -53,6 +53,7 @@ def lib():
         "mdt_fix_create": (I, [P, I, C.POINTER(C.c_char_p), C.POINTER(P)]),
         "mdt_fix_destroy": (None, [P]),
         "mdt_fix_init": (I, [P]),
+        "mdt_fix_param": (I, [P, C.c_char_p, dp, I, C.POINTER(I)]),
         "mdt_fix_post_force": (I, [P, I64, P, I]),
         "mdt_fix_run": (I, [P, I64, I64]),
         "mdt_fix_run_steps": (I, [P, I64, I64, I]),
@@ -328,6 +329,15 @@ class Fix:
 
     def init(self):
         self._chk(lib().mdt_fix_init(self._h))
+
+    def param(self, key):
+        """Parsed values of a fix keyword (mdt_fix_param): GAUSSIAN, PACE, CV_dim, TEMP, BOLTZ,
+        WT, METAD_RESTART, Gaussian_Hill_type, SIGMAS, DIM (lower, upper, bins per dimension)."""
+        n = C.c_int()
+        self._chk(lib().mdt_fix_param(self._h, key.encode(), None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        self._chk(lib().mdt_fix_param(self._h, key.encode(), _dp(out), n.value, C.byref(n)))
+        return out[: n.value]
 
     def post_force(self, step, f=None):
         """One MD step; f (host float64 array, n_local_total x 3) receives += bias forces."""

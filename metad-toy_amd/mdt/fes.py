@@ -2,7 +2,8 @@
 
 Mirrors the reference's post-processing script tools/plot_hills.py (usage
 ``python -m mdt.fes <LAMMPS_Input_File> <HILLS_File>``): the grid comes from
-the ``fix metad`` line of the LAMMPS input (parse_metad_params, :41-119), the
+the ``fix metad`` command of the LAMMPS input, read by the library's own
+fix parser (the script scans the keywords itself, :41-119), the
 hills are summed on it (:158-223) and written to FES.dat (:226); for two CVs
 the marginal free energies along each CV are written to
 Marginal_FES_CV1.dat and Marginal_FES_CV2.dat (:236-285).
@@ -13,74 +14,65 @@ HILLS file named on the command line is the one read (the script reads
 ./HILLS whatever it is given, :8), and no plots are drawn.
 """
 import os
+import shlex
 import sys
 
 import numpy as np
 
-from . import Context
+from . import Context, Fix
 
 T = 300.0
 R = 8.3144621
 KBT = R * T / 1000.0        # the script's k_B T (kJ/mol), used by the marginals
 
 
-def parse_metad_params(input_file_path):
-    """The script's keyword scan of the `fix ... metad` line (tools/plot_hills.py:41-119):
-    the tokens after `fix ID group metad`, and every token of the lines after it; CAL
-    blocks are skipped up to the next top-level keyword.  Returns None without a fix."""
+def fix_words(input_file_path):
+    """Words of the first ``fix <ID> <group> metad ...`` command of a LAMMPS input file,
+    read by LAMMPS's own input rules: a line whose last non-blank character is ``&``
+    continues on the next line, ``#`` outside quotes starts a comment, quoted words keep
+    their blanks.  Returns the words after ``metad``, or None (no file, no such fix)."""
     try:
         with open(input_file_path) as f:
-            content = f.read()
+            raw = f.read().splitlines()
     except FileNotFoundError:
         return None
-    content = content.replace("&\n", " ").replace("&", " ")
-    tokens, found = [], False
-    for line in content.split("\n"):
-        line = line.split("#")[0].strip()
-        if not line:
+    command = ""
+    for line in raw:
+        body = line.rstrip()
+        if body.endswith("&"):
+            command += body[:-1] + " "
             continue
-        t = line.split()
-        if t[0] == "fix" and "metad" in t:
-            tokens.extend(t[4:])
-            found = True
-        elif found:
-            tokens.extend(t)
-    if not found:
+        command += body
+        words = shlex.split(command, comments=True)
+        command = ""
+        if len(words) >= 4 and words[0] == "fix" and words[3] == "metad":
+            return words[4:]
+    return None
+
+
+def parse_metad_params(input_file_path):
+    """Grid and hill parameters of the input's ``fix metad``, read by the library's own
+    keyword parser on a parse-only context (mdt_fix_param): the values the running fix
+    would use.  Returns None when the file or the fix is missing; a keyword list the fix
+    itself would refuse raises MdtError with the parser's message."""
+    words = fix_words(input_file_path)
+    if words is None:
         return None
-    p = {"sigma": 0.05, "height0": 0.1, "biasf": 10.0, "pace": 100, "cv_dim": 1, "nbin_num": 100,
-         "continue": False, "bounds": {}}
-    stop = ("GAUSSIAN", "PACE", "CV_dim", "DIM", "METAD_RESTART", "WT", "CAL")
-    i = 0
-    while i < len(tokens):
-        a = tokens[i]
-        if a == "GAUSSIAN":
-            p["sigma"], p["height0"], p["biasf"] = float(tokens[i + 1]), float(tokens[i + 2]), float(tokens[i + 3])
-            i += 4
-        elif a == "PACE":
-            p["pace"] = int(tokens[i + 1])
-            i += 2
-        elif a == "CV_dim":
-            p["cv_dim"] = int(tokens[i + 1])
-            i += 2
-        elif a == "CAL":
-            i += 1
-            while i < len(tokens) and tokens[i] not in stop:
-                i += 1
-        elif a == "DIM":
-            try:
-                k = int(tokens[i + 1])
-                p["bounds"][k] = [float(tokens[i + 2]), float(tokens[i + 3]), int(tokens[i + 4])]
-                p["nbin_num"] = int(tokens[i + 4])
-            except (ValueError, IndexError):
-                pass
-            i += 6
-        elif a == "METAD_RESTART":
-            p["continue"] = int(tokens[i + 1]) != 0
-            i += 2
-        elif a == "WT":
-            i += 2
-        else:
-            i += 1
+    ctx = Context(-1, 1)
+    try:
+        fx = Fix(ctx, words)
+        try:
+            sigma, height0, biasf = (float(v) for v in fx.param("GAUSSIAN"))
+            D = int(fx.param("CV_dim")[0])
+            dims = fx.param("DIM").reshape(D, 3)
+            p = {"sigma": sigma, "height0": height0, "biasf": biasf, "pace": int(fx.param("PACE")[0]),
+                 "cv_dim": D, "continue": bool(fx.param("METAD_RESTART")[0]),
+                 "bounds": {k + 1: [float(a), float(b), int(n)] for k, (a, b, n) in enumerate(dims)}}
+            p["nbin_num"] = p["bounds"][D][2]
+        finally:
+            fx.close()
+    finally:
+        ctx.close()
     return p
 
 
@@ -89,7 +81,7 @@ def grid_axes(params):
     lo + d/2 to hi - d/2, :125-156), with the bounds and widths per CV."""
     axes, lo, d = [], [], []
     for k in range(1, params["cv_dim"] + 1):
-        a, b, n = params["bounds"].get(k, [0.0, 1.0, 100])
+        a, b, n = params["bounds"][k]
         w = (b - a) / n
         axes.append(np.linspace(a + w / 2, b - w / 2, n))
         lo.append(a)

@@ -196,23 +196,34 @@ def test_expression_errors(mdt, expr, frag):
 
 
 def test_fes_input_parser(mdt):
-    """The FES tool reads its grid from the fix metad line as tools/plot_hills.py:41-119 does:
-    CAL blocks skipped, DIM bounds per index, defaults, continuation lines."""
+    """The FES tool takes its grid from the input's fix metad command, joined and stripped by
+    LAMMPS's input rules (trailing '&' continues, '#' comments) and read by the library's own
+    fix parser on a parse-only context: DIM bounds per index, GAUSSIAN, PACE, METAD_RESTART.
+    A keyword list the fix refuses is refused with the parser's message."""
     import os
-    import sys
     import tempfile
-    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "metad-toy_amd"))
     from mdt import fes
     deck = ("units metal\n"
+            "# fix 2 all metad GAUSSIAN 9 9 9 (a comment, not a command)\n"
             "fix 1 all metad GAUSSIAN 0.02 0.004 8 PACE 5 &\n"
-            "   CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 # a comment\n"
-            "   CAL NAME Q4 STEINH Q 4 all CV_dim 2 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE\n"
-            "   DIM 1 0 1 200 v1 DIM 2 -0.5 0.5 150 v2 WT 1 METAD_RESTART 1\n")
+            "   CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 &\n"
+            "   CAL NAME Q4 STEINH Q 4 all CV_dim 2 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE &\n"
+            "   DIM 1 0 1 200 v1 DIM 2 -0.5 0.5 150 v2 WT 1 METAD_RESTART 1   # trailing comment\n"
+            "run 100\n")
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "run.in")
         open(path, "w").write(deck)
+        assert fes.fix_words(path)[:6] == ["GAUSSIAN", "0.02", "0.004", "8", "PACE", "5"]
+        assert fes.fix_words(path)[-1] == "1"
         p = fes.parse_metad_params(path)
         assert fes.parse_metad_params(os.path.join(tmp, "missing.in")) is None
+        open(path, "w").write("units metal\nrun 10\n")
+        assert fes.parse_metad_params(path) is None
+        open(path, "w").write("fix 1 all metad GAUSSIAN 0.02 0.004 8 CAL NAME Q6 STEINH Q 6 all &\n"
+                              "  CV_dim 1 SYMBOL v1 Q6.AVE\n")
+        with pytest.raises(mdt.MdtError) as e:
+            fes.parse_metad_params(path)
+        assert "DIM 1 is missing" in str(e.value)
     assert p["sigma"] == 0.02 and p["height0"] == 0.004 and p["biasf"] == 8.0 and p["pace"] == 5
     assert p["cv_dim"] == 2 and p["continue"] is True
     assert p["bounds"] == {1: [0.0, 1.0, 200], 2: [-0.5, 0.5, 150]}
