@@ -28,6 +28,9 @@ PKG = os.path.join(ROOT, "metad-toy_amd")
 sys.path.insert(0, PKG)
 sys.path.insert(0, ROOT)
 
+from mdt.workloads import (A, C1_ARGS, C2_ARGS, C3_ARGS, C4_ARGS, C5_BASE, KB, LQ_ARGS, RC, RG,  # noqa: E402
+                           c1_system, c2_system, c3_system, c4_systems, wt_hills)
+
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 # algorithmic work (FMA = 2 FLOP, transcendental = 1)
@@ -40,28 +43,11 @@ FP64_PEAK_TFLOPS = 78.6
 # 140); the weighted q average (k_lq_lq, 28); the weighted K sum and <K_j, q_a> (k_lq_h, 56);
 # the pair gradient <H, Y>, grad <H, Y> and its assembly (k_lq_pair, 282)
 FLOP_LQ_Q, FLOP_LQ_LQ, FLOP_LQ_H, FLOP_LQ_PAIR = 140.0, 28.0, 56.0, 282.0
-KB = 8.617343e-5
-A, RC, RG = 5.26, 3.9, 5.9
 METRIC = "atom-steps/s, Q6+Q4 CV+bias forces @256k atoms; NN bias vs hills @<=0.1 kT"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def steinh(name, l, kind="Q", N=18):
-    return f"CAL NAME {name} STEINH {kind} {l} all cutoff_r {RC} cutoff_Natoms {N} "
-
-
-C2_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE " + steinh("Q6", 6) + steinh("Q4", 4) +
-           steinh("LQ6", 6, "LD") + "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE SYMBOL v3 LQ6.AVE "
-           "DIM 1 0 2 2000 v1+v2 WT 1")
-
-
-def c2_system(mdt):
-    n = 40
-    s = mdt.System().add_fcc(n, n, n, A, 0.6, 0.1, n * A / 2.0, seed=2024)
-    return s.build(RG)
 
 
 def timed_run(fix, step0, steps, per_launch=None):
@@ -190,12 +176,11 @@ def neighbor_build_ms(mdt, system, reps=5):
 
 # ------------------------------------------------------------------ config 1
 def run_c1(mdt, args):
-    s = mdt.System().add_fcc(10, 10, 10, A, 0.1, 0.1, 0.0, seed=1).build(RG)
+    s = c1_system(mdt)
     n = s.n_local
     ctx = mdt.Context(0, 1)
     ctx.bind_system(0, s)
-    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE TEMP 300 " + steinh("Q6", 6) +
-                  "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0")
+    fix = mdt.Fix(ctx, C1_ARGS)
     fix.init()
     fix.run(0, args.warmup)
     steps = 10000
@@ -221,23 +206,12 @@ def run_c1(mdt, args):
 
 
 # ------------------------------------------------------------------ config 3
-def sk_kvecs(nrand=2048, seed=31):
-    k0 = 2 * np.pi / A
-    ks = [(sx * k0, sy * k0, sz * k0) for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
-    return ks, f"FCC111 {A} NRAND {nrand} KMIN 1.5 KMAX 3.5 SEED {seed}"
-
-
 def run_c3(mdt, args):
-    L = 52 * A
-    s = mdt.System().set_box([0, 0, 0], [L, L, L]).add_fcc(52, 52, 26, A, 0.0, 0.0, 0.0, seed=3)
-    n_fcc = 4 * 52 * 52 * 26
-    s.add_uniform(500000 - n_fcc, L / 2, L, seed=4).build(RG)
+    s = c3_system(mdt)
     n = s.n_local
-    _, kspec = sk_kvecs()
     ctx = mdt.Context(0, 1)
     ctx.bind_system(0, s)
-    fix = mdt.Fix(ctx, f"GAUSSIAN 0.5 0.005 10 PACE 10 HILLS_FILE NONE CAL NAME SK SK all {kspec} "
-                       "CV_dim 1 SYMBOL v1 SK.AVE DIM 1 0 50 1000 v1 WT 0")
+    fix = mdt.Fix(ctx, C3_ARGS)
     fix.init()
     fix.run(0, 2)
     steps = max(5, args.steps // 100)
@@ -281,25 +255,6 @@ def run_c3(mdt, args):
 
 
 # ------------------------------------------------------------------ config 4
-def c4_systems(mdt, nw=64, seed=44):
-    rng = np.random.default_rng(seed)
-    out = []
-    for w in range(nw):
-        nt = int(rng.integers(2000, 64001))
-        if rng.random() < 0.5:
-            m = max(4, int(round((nt / 4) ** (1 / 3))))
-            s = mdt.System().add_fcc(m, m, m, A, 0.1, 0.1, 0.0, seed=1000 + w)
-        else:
-            L = (nt / 0.021) ** (1 / 3)
-            s = mdt.System().set_box([0, 0, 0], [L, L, L]).add_uniform(nt, 0.0, L, seed=2000 + w)
-        out.append(s.build(RG))
-    return out
-
-
-C4_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE " + steinh("Q4", 4) + steinh("Q6", 6) +
-           "CV_dim 2 SYMBOL v1 Q4.AVE SYMBOL v2 Q6.AVE DIM 1 0 1 200 v1 DIM 2 0 1 200 v2 WT 1")
-
-
 def run_c4(mdt, args):
     systems = c4_systems(mdt)
     ntot = sum(s.n_local for s in systems)
@@ -335,27 +290,11 @@ def run_c4(mdt, args):
 
 
 # ------------------------------------------------------------------ config 5
-def wt_hills(s0, nh, kt, biasf=10.0, h0=0.005, seed=5):
-    """Hills of a well-tempered run diffusing around s0: centres spread 2 sigma, heights
-    h0 exp(-V(c)/(kT (biasf-1))) with V the sum of the earlier hills (FP64)."""
-    sig = np.array([0.01, 0.01, 0.5])
-    rng = np.random.default_rng(seed)
-    c = s0 + rng.normal(size=(nh, 3)) * 2.0 * sig
-    h = np.zeros(nh)
-    for k in range(nh):
-        V = float(np.sum(h[:k] * np.exp(-0.5 * np.sum(((c[:k] - c[k]) / sig) ** 2, axis=1)))) if k else 0.0
-        h[k] = h0 * np.exp(-V / (kt * (biasf - 1.0)))
-    return c, h
-
-
 def run_c5(mdt, args):
     system = c2_system(mdt)
     n = system.n_local
-    kspec = f"FCC111 {A}"
     kt = 300.0 * KB
-    base = ("GAUSSIAN 0.01 0.005 10 PACE 10 TEMP 300 HILLS_FILE NONE SIGMAS 0.01 0.01 0.5 " + steinh("Q4", 4) +
-            steinh("Q6", 6) + f"CAL NAME S111 SK all {kspec} CV_dim 3 SYMBOL v1 Q4.AVE SYMBOL v2 Q6.AVE "
-            "SYMBOL v3 S111.AVE DIM 1 0 1 100 v1 DIM 2 0 1 100 v2 DIM 3 0 200000 100 v3 WT 1 ")
+    base = C5_BASE
     res = {}
     steps = 2000
     nh0 = 19800
@@ -396,12 +335,6 @@ def run_c5(mdt, args):
     return line(res["mlp"]["atom_steps_per_s"], steps, 20, res["mlp"]["us_per_step"] * 1e-3,
                 "bf16 MFMA network (f32 accum), f64 labels", cfg, None, None,
                 metric="atom-steps/s, config 5 (learned bias; exact hills-sum result in config.results)")
-
-
-LQ_ARGS = ("GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE CAL NAME sr SW_FUNC RATIONAL r_0 3.0 "
-           "CAL NAME sc SW_FUNC FERMI r_0 0.35 alpha 10 "
-           "CAL NAME LQ STEINH L 6 all SW_FUNC_r sr SW_FUNC_cv sc "
-           "CV_dim 1 SYMBOL v1 LQ.MEAN_SOLID DIM 1 0 2 2000 v1 WT 1")
 
 
 def run_lq(mdt, args):

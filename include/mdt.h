@@ -109,7 +109,8 @@ void mdt_fix_destroy(mdt_fix* fix);
  * running it (e.g. on a parse-only context, device -1): key "GAUSSIAN" gives
  * (sigma, height, biasf); "PACE", "CV_dim", "TEMP", "BOLTZ", "WT",
  * "METAD_RESTART", "Gaussian_Hill_type" one value; "SIGMAS" one per dimension;
- * "DIM" (lower, upper, bins) per dimension.  *n receives the count; at most cap
+ * "DIM" (lower, upper, bins) per dimension; "<CAL>.KVEC" the k vectors of an
+ * SK CAL, (kx, ky, kz) each.  *n receives the count; at most cap
  * values are written to out.  Replaces the keyword scan of the post-processing
  * script tools/plot_hills.py:41-119 with the fix's own parser
  * (FixMetadynamics::FixMetadynamics, src/core/fix_crystallize.cu:32-302). */
@@ -160,13 +161,17 @@ int mdt_fix_cv_weight(mdt_fix* fix, int walker, const char* cal_func, double* va
 
 /* Per-atom export "CAL.prop" for n_local atoms of a walker (Fix::extract
  * "colvar_peratom:<CAL>.<prop>", :592-613; props: stein_q, local_q, nsel,
- * count_rc). */
+ * count_rc; qlm (2l+1 per atom); kept: the neighbours the last step selected,
+ * N per atom in list order, -1 after the last, decoded from the step's own
+ * selection mask). */
 int mdt_fix_extract_peratom(mdt_fix* fix, int walker, const char* key, double* out);
 
-/* Literal N-nearest selection of a CAL's environment at the current
- * positions, in the reference slot order (get_environment_Steinhardt_Q,
+/* N-nearest selection of a CAL's environment at the current positions, in the
+ * reference slot order (get_environment_Steinhardt_Q,
  * src/cvs/CVs_Steinhardt.cu:1324-1406): sel_idx[n_local*N] (-1 = empty),
- * count_rc[n_local], n_sel[n_local].  Any pointer may be NULL. */
+ * count_rc[n_local], n_sel[n_local].  Computed by an instance of the step's
+ * own selection kernel that runs the insertion for every atom and writes its
+ * slots.  Any pointer may be NULL. */
 int mdt_fix_selection(mdt_fix* fix, int walker, const char* cal_name, int32_t* sel_idx,
                       int32_t* count_rc, int32_t* n_sel);
 

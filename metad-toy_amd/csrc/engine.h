@@ -100,6 +100,10 @@ struct EnvDev {
     int32_t fld_l;
     int32_t fld_slot;         // its partial-sum slot
     float* fld_lq;            // its per-atom lq
+    // selection export (the k_env export instance only): slot-ordered neighbour
+    // indices [n_local_total * N] and count within rc [n_local_total]
+    int32_t* sel_out;
+    int32_t* sel_cnt;
 };
 
 // Lechner-Dellago local Q_l on an environment.
@@ -322,8 +326,8 @@ hipError_t launch_topology(const Geo& g, int64_t n_local_total, int32_t* eown, u
 hipError_t launch_pack_x4(const double* x, double4* x4, float4* x4f, int64_t n, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, hipStream_t st);
 hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st);
-hipError_t launch_select_literal(const Geo& g, int32_t first, int32_t count, double rc2, int N,
-                                 int32_t* sel_idx, int32_t* cnt_rc, int32_t* nsel, hipStream_t st);
+// the selection export instance of k_env (e.sel_out, e.sel_cnt set)
+hipError_t launch_select_export(const Geo& g, const EnvDev& e, hipStream_t st);
 hipError_t launch_ld(const Geo& g, const EnvDev& e, const LdDev& d, hipStream_t st);
 hipError_t launch_ld_coeff(const Geo& g, const EnvDev& e, const LdDev& d, hipStream_t st);
 hipError_t launch_force(const Geo& g, const EnvDev& e, double* f, hipStream_t st);

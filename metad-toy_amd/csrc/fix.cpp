@@ -1881,6 +1881,14 @@ int mdt_fix_param(mdt_fix* fx, const char* key, double* out, int cap, int* n) {
     else if (k == "SIGMAS") v = fx->sigmas;
     else if (k == "DIM")
         for (int d = 0; d < fx->cv_dim; ++d) { v.push_back(fx->lo[d]); v.push_back(fx->hi[d]); v.push_back((double)fx->nb[d]); }
+    else if (k.size() > 5 && k.compare(k.size() - 5, 5, ".KVEC") == 0) {
+        // the k vectors of an SK CAL (explicit, (111) and seeded random), kx ky kz each
+        const std::string name = k.substr(0, k.size() - 5);
+        const CvH* c = nullptr;
+        for (auto* q : fx->cvs) if (q->name == name) c = q;
+        if (!c || c->kind != CV_SK) throw Err(MDT_EINVAL, fmt("no SK CAL named '%s'", name.c_str()));
+        v = c->kvec;
+    }
     else throw Err(MDT_EINVAL, fmt("unknown fix parameter '%s'", k.c_str()));
     if (n) *n = (int)v.size();
     if (out) for (int q = 0; q < cap && q < (int)v.size(); ++q) out[q] = v[q];
@@ -2107,6 +2115,22 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
         std::vector<int32_t> v((size_t)n);
         HIPCHK(hipMemcpy(v.data(), (prop == "nsel" ? e->nsel.p : e->cnt.p) + o, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) out[i] = v[i];
+    } else if (prop == "kept") {
+        // the set the last step's k_env kept (its selection mask), as the neighbour
+        // indices of each atom in list order, N per atom, -1 after the last
+        const int N = e->N;
+        std::vector<uint64_t> m((size_t)n * 4);
+        HIPCHK(hipMemcpy(m.data(), e->selmask.p + (size_t)o * 4, sizeof(uint64_t) * m.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) {
+            int k = 0;
+            const int64_t c0 = a.off[i], c = (int64_t)a.off[i + 1] - a.off[i];
+            for (int64_t p = 0; p < c && p < kMaxList; ++p)
+                if ((m[(size_t)i * 4 + (p >> 6)] >> (p & 63)) & 1ull) {
+                    if (k >= N) throw Err(MDT_EINVAL, "selection mask holds more than N entries");
+                    out[i * N + k++] = a.idx[c0 + p];
+                }
+            for (; k < N; ++k) out[i * N + k] = -1;
+        }
     } else if (prop == "qlm") {
         // q_lm of degree l: 2l+1 floats per atom (Re q_l0, Re/Im q_lm m>=1)
         const int RS = rec_stride(e->lmask), O = rec_off(e->lmask, c->l), W = 2 * c->l + 1;  // q block of l
@@ -2131,16 +2155,21 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
         throw Err(MDT_EINVAL, "selection export needs a STEINH Q or LD CAL name");
     EnvH* e = fx->envs[c->env];
     const int N = e->N;
-    const int64_t n = a.n_local;
-    DBuf<int32_t> si, cr, ns;
-    si.alloc((size_t)std::max<int64_t>(n, 1) * N); cr.alloc(std::max<int64_t>(n, 1)); ns.alloc(std::max<int64_t>(n, 1));
+    const int64_t n = a.n_local, NT = ctx->total_local;
+    // the scan and insertion of the step's k_env, run for every atom of every walker
+    DBuf<int32_t> si, cr;
+    si.alloc((size_t)std::max<int64_t>(NT, 1) * N); cr.alloc(std::max<int64_t>(NT, 1));
     Geo g = ctx->geo();
     g.group = e->group == "all" ? nullptr : ctx->group.p;
-    HIPCHK(launch_select_literal(g, (int32_t)a.atom_off, (int32_t)n, e->rc * e->rc, N, si.p, cr.p, ns.p, ctx->stream));
+    EnvDev d = e->dev;
+    d.sel_out = si.p; d.sel_cnt = cr.p;
+    HIPCHK(launch_select_export(g, d, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> cnt((size_t)std::max<int64_t>(n, 1));
+    HIPCHK(hipMemcpy(cnt.data(), cr.p + a.atom_off, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     if (sel_idx) {
         std::vector<int32_t> v((size_t)n * N);
-        HIPCHK(hipMemcpy(v.data(), si.p, sizeof(int32_t) * v.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v.data(), si.p + (size_t)a.atom_off * N, sizeof(int32_t) * v.size(), hipMemcpyDeviceToHost));
         for (auto& j : v) {
             if (j < 0) continue;
             if (j < ctx->total_local) j = (int32_t)(j - a.atom_off);
@@ -2148,8 +2177,8 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
         }
         std::memcpy(sel_idx, v.data(), sizeof(int32_t) * v.size());
     }
-    if (count_rc) HIPCHK(hipMemcpy(count_rc, cr.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    if (n_sel) HIPCHK(hipMemcpy(n_sel, ns.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (count_rc) std::memcpy(count_rc, cnt.data(), sizeof(int32_t) * n);
+    if (n_sel) for (int64_t i = 0; i < n; ++i) n_sel[i] = std::min(cnt[i], N);
     API_END
 }
 

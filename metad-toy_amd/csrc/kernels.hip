@@ -378,34 +378,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // then), otherwise lane 0 replays the reference insertion to find the kept
 // set.  Phase B accumulates Y_lm in FP32 over the kept set.
 // ---------------------------------------------------------------------------
-template <int NMAX>
-__device__ __forceinline__ void literal_insert(const Geo& g, int beg, int c, double xi0, double xi1,
-                                               double xi2, double rc2, int N, double* sr2, int* sp) {
-    #pragma unroll
-    for (int s = 0; s < NMAX; ++s) { sr2[s] = __builtin_inf(); sp[s] = -1; }
-    for (int p = 0; p < c; ++p) {
-        double d0, d1, d2;
-        double r2 = pair_r2(g.x4, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
-        if ((r2 > rc2) || (r2 < 1e-12)) continue;
-        double r = r2;
-        int q = p;
-        #pragma unroll
-        for (int s = 0; s < NMAX; ++s) {
-            if (s < N) {
-                const bool sw = sr2[s] > r;
-                const double tr = sr2[s];
-                const int tq = sp[s];
-                sr2[s] = sw ? r : tr;
-                sp[s] = sw ? q : tq;
-                r = sw ? tr : r;
-                q = sw ? tq : q;
-            }
-        }
-    }
-}
-
-// The same insertion with the sorted arrays in LDS (one lane per atom), so that the
-// rare cap-binding path does not set the register budget of the whole kernel.
+// The reference insertion (get_environment_Steinhardt_Q) over one atom's list, with
+// the sorted arrays in LDS (one lane per atom), so that the rare cap-binding path
+// does not set the register budget of the whole kernel: skip r^2 > rc^2 or
+// r^2 < 1e-12, then walk the N slots with strict '>' carrying the displaced entry.
 template <int NMAX>
 __device__ __forceinline__ void literal_insert_lds(const Geo& g, int beg, int c, double xi0, double xi1,
                                                    double xi2, double rc2, int N, double* sr2, int* sp) {
@@ -431,7 +407,11 @@ __device__ __forceinline__ void literal_insert_lds(const Geo& g, int beg, int c,
 // every pair that either side of a pair can select (frames differ by rounding)
 constexpr double kRcSlack = 1e-9;
 
-template <int NMAX, int LMASK>
+// EXPORT instance: phase A, then the insertion for every atom, whatever its count,
+// writing the slots in the reference order (sel_out) and the count within rc; no
+// q_lm, record or partial-sum writes.  Its kept set equals the step's by construction
+// (the same scan and the same insertion).
+template <int NMAX, int LMASK, bool EXPORT = false>
 __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     constexpr int NL = rec_nl(LMASK);
     constexpr int QF = rec_used(LMASK);
@@ -548,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         if constexpr (LMASK & kL6) acc_ylm<6>(uu, u2, zr, zi, q + rec_off(LMASK, 6));
     };
     wave_lds_sync();
-    if (cnt <= N && cnt <= KC) {                     // uniform within the group
+    if (!EXPORT && cnt <= N && cnt <= KC) {          // uniform within the group
         float4 pk[KB];
         #pragma unroll
         for (int s = 0; s < KB; ++s)
@@ -557,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         for (int s = 0; s < KB; ++s)
             if (lane + s * kLanes < cnt) add_ylm(pk[s]);
     } else {
-        if (cnt > N) {                               // the cap binds: the literal insertion
+        if (EXPORT || cnt > N) {                     // the cap binds: the literal insertion
             if (lane == 0) {
                 double* sr2 = reinterpret_cast<double*>(&kept[a][0]);
                 int* sp = reinterpret_cast<int*>(sr2 + NMAX);
@@ -566,9 +546,16 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
                 own = M128{0, 0};
                 for (int s = 0; s < N; ++s)
                     if (sp[s] >= 0) mset(own, sp[s]);
+                if constexpr (EXPORT) {
+                    if (valid) {
+                        for (int s = 0; s < N; ++s) e.sel_out[(int64_t)i * N + s] = sp[s] >= 0 ? g.idx[beg + sp[s]] : -1;
+                        e.sel_cnt[i] = cnt;
+                    }
+                }
             }
             own = grp_bcast0(own);
         }
+        if constexpr (EXPORT) return;
         M128 it = own;
         mdrop(it, lane);
         for (int p = mnext(it); p >= 0; p = mnext(it)) add_ylm(g.x4f[g.idx[beg + p]]);
@@ -622,34 +609,6 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         #pragma unroll
         for (int k = 0; k < NL; ++k) e.partial[(int64_t)e.slot_ql[k] * g.ntiles + t] = part[k];
     }
-}
-
-// Literal selection export in the reference slot order (tests, per-atom export).
-template <int NMAX>
-__global__ __launch_bounds__(kBlock) void k_select_literal(Geo g, int first, int count, double rc2, int N,
-                                                           int32_t* sel_idx, int32_t* cnt_rc, int32_t* nsel) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= count) return;
-    const int i = first + a;
-    const bool center = g.group == nullptr || g.group[i];
-    const int beg = g.off[i];
-    const int c = center ? g.off[i + 1] - beg : 0;
-    const double4 xi = g.x4[i];
-    double sr2[NMAX];
-    int sp[NMAX];
-    literal_insert<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
-    int cnt = 0;
-    for (int p = 0; p < c; ++p) {
-        double d0, d1, d2;
-        double r2 = pair_r2(g.x4, g.idx[beg + p], xi.x, xi.y, xi.z, d0, d1, d2);
-        if ((r2 > rc2) || (r2 < 1e-12)) continue;
-        ++cnt;
-    }
-    #pragma unroll
-    for (int s = 0; s < NMAX; ++s)
-        if (s < N) sel_idx[(int64_t)a * N + s] = sp[s] >= 0 ? g.idx[beg + sp[s]] : -1;
-    cnt_rc[a] = cnt;
-    nsel[a] = cnt < N ? cnt : N;
 }
 
 // Reverse-selected candidates of atom i: the candidate's owner selects i.
@@ -2641,17 +2600,17 @@ hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st) {
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_select_literal(const Geo& g, int32_t first, int32_t count, double rc2, int N,
-                                 int32_t* sel_idx, int32_t* cnt_rc, int32_t* nsel, hipStream_t st) {
-    if (count == 0) return hipSuccess;
-    dim3 grid((count + kBlock - 1) / kBlock);
-#define MDT_SEL(NM) hipLaunchKernelGGL(k_select_literal<NM>, grid, dim3(kBlock), 0, st, g, first, count, rc2, N, sel_idx, cnt_rc, nsel)
-    if (N <= 12) MDT_SEL(12);
-    else if (N <= 16) MDT_SEL(16);
-    else if (N <= 18) MDT_SEL(18);
-    else if (N <= 24) MDT_SEL(24);
-    else if (N <= 32) MDT_SEL(32);
-    else if (N <= 64) MDT_SEL(64);
+hipError_t launch_select_export(const Geo& g, const EnvDev& e, hipStream_t st) {
+    if (g.ntiles == 0) return hipSuccess;
+    if (!e.sel_out || !e.sel_cnt) return hipErrorInvalidValue;
+    dim3 grid(xcd_grid(g.ntiles));
+#define MDT_SEL(NM) hipLaunchKernelGGL((k_env<NM, kL6, true>), grid, dim3(kBlock), 0, st, g, e)
+    if (e.N <= 12) MDT_SEL(12);
+    else if (e.N <= 16) MDT_SEL(16);
+    else if (e.N <= 18) MDT_SEL(18);
+    else if (e.N <= 24) MDT_SEL(24);
+    else if (e.N <= 32) MDT_SEL(32);
+    else if (e.N <= 64) MDT_SEL(64);
     else return hipErrorInvalidValue;
 #undef MDT_SEL
     return hipGetLastError();

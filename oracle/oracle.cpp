@@ -33,6 +33,9 @@
 #include <cstdio>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <algorithm>
 
 namespace {
@@ -216,6 +219,8 @@ void orc_ylm_coef(double* out /*7*7*7*/) { std::memcpy(out, ptab().coef, sizeof(
 void orc_select(int n_center, const int* centers, const int* offsets, const int* idx,
                 const double* x, double rc2, int N, double box_sum2,
                 int* sel_idx, int* sel_pos, double* sel_d, int* count_rc, int* n_sel) {
+    // centers are independent: one thread per run of centers, each center's scan serial
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int c = 0; c < n_center; ++c) {
         int i = centers[c];
         double cx = x[3 * i], cy = x[3 * i + 1], cz = x[3 * i + 2];
@@ -255,15 +260,16 @@ void orc_select(int n_center, const int* centers, const int* offsets, const int*
 void orc_steinhardt(int l, int n_center, int N, const double* sel_d, const int* n_sel,
                     double* qlm, double* ql) {
     const int W = 2 * (l + 1);
-    std::vector<double> Y(W);
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int c = 0; c < n_center; ++c) {
+        double Y[2 * 13];
         double* q = qlm + (size_t)c * W;
         for (int k = 0; k < W; ++k) q[k] = 0.0;
         int n = n_sel[c];
         if (n == 0) { ql[c] = 0.0; continue; }
         for (int s = 0; s < n; ++s) {
             const double* d = sel_d + ((size_t)c * N + s) * 4;
-            ylm_reference(l, d[0], d[1], d[2], Y.data());
+            ylm_reference(l, d[0], d[1], d[2], Y);
             for (int k = 0; k < W; ++k) q[k] += Y[k];
         }
         double inv = 1.0 / (double)n;
@@ -299,6 +305,7 @@ void orc_ld_local(int l, int n_center, int N, int n_local, const int* ghost_owne
                   const int* sel_idx, const int* n_sel, const double* qlm,
                   double* qbar, double* lq) {
     const int W = 2 * (l + 1);
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < n_center; ++i) {
         double* qb = qbar + (size_t)i * W;
         for (int k = 0; k < W; ++k) qb[k] = qlm[(size_t)i * W + k];
@@ -330,27 +337,50 @@ void orc_grad_from_coeff(int l, int n_center, int N, int n_local, const int* gho
                          const int* sel_idx, const double* sel_d, const int* n_sel,
                          const double* K, double* dVdx, const int* centers) {
     const int W = 2 * (l + 1);
-    std::vector<double> dY(6 * (l + 1));
-    for (int c = 0; c < n_center; ++c) {
-        int n = n_sel[c];
-        if (n == 0) continue;
-        const double* k = K + (size_t)c * W;
-        for (int s = 0; s < n; ++s) {
-            const double* d = sel_d + ((size_t)c * N + s) * 4;
-            ylm_grad(l, d[0], d[1], d[2], dY.data());
-            double G[3] = {0, 0, 0};
-            for (int m = 0; m <= l; ++m) {
-                double mu = m ? 2.0 : 1.0;
-                for (int a = 0; a < 3; ++a)
-                    G[a] += mu * (k[2 * m] * dY[m * 6 + 2 * a] + k[2 * m + 1] * dY[m * 6 + 2 * a + 1]);
+    // The pair terms scatter to two atoms: each thread accumulates a static block of
+    // centers into its own array, and the arrays are added in thread order.
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<double> acc((size_t)nt * n_local * 3, 0.0);
+    #pragma omp parallel num_threads(nt)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double* out = acc.data() + (size_t)t * n_local * 3;
+        std::vector<double> dY(6 * (l + 1));
+        #pragma omp for schedule(static)
+        for (int c = 0; c < n_center; ++c) {
+            int n = n_sel[c];
+            if (n == 0) continue;
+            const double* k = K + (size_t)c * W;
+            for (int s = 0; s < n; ++s) {
+                const double* d = sel_d + ((size_t)c * N + s) * 4;
+                ylm_grad(l, d[0], d[1], d[2], dY.data());
+                double G[3] = {0, 0, 0};
+                for (int m = 0; m <= l; ++m) {
+                    double mu = m ? 2.0 : 1.0;
+                    for (int a = 0; a < 3; ++a)
+                        G[a] += mu * (k[2 * m] * dY[m * 6 + 2 * a] + k[2 * m + 1] * dY[m * 6 + 2 * a + 1]);
+                }
+                int o = owner_of(sel_idx[(size_t)c * N + s], n_local, ghost_owner);
+                const size_t ci = centers ? (size_t)centers[c] : (size_t)c;
+                for (int a = 0; a < 3; ++a) {
+                    G[a] /= (double)n;
+                    out[3 * (size_t)o + a] += G[a];
+                    out[3 * ci + a] -= G[a];
+                }
             }
-            int o = owner_of(sel_idx[(size_t)c * N + s], n_local, ghost_owner);
-            const size_t ci = centers ? (size_t)centers[c] : (size_t)c;
-            for (int a = 0; a < 3; ++a) {
-                G[a] /= (double)n;
-                dVdx[3 * (size_t)o + a] += G[a];
-                dVdx[3 * ci + a] -= G[a];
-            }
+        }
+        #pragma omp barrier
+        #pragma omp for schedule(static)
+        for (int64_t q = 0; q < (int64_t)n_local * 3; ++q) {
+            double v = 0.0;
+            for (int u = 0; u < nt; ++u) v += acc[(size_t)u * n_local * 3 + q];
+            dVdx[q] += v;
         }
     }
 }

@@ -130,17 +130,22 @@ class Env:
         lib().orc_select(nc, _p(centers), _p(offs), _p(cidx), _p(self.x), rc * rc, N, box_sum2,
                          _p(self.sel_idx), _p(self.sel_pos), _p(self.sel_d), _p(self.cnt), _p(self.nsel))
         self._full = group is None
+        self._st = {}
 
     def owner_of(self, j):
         j = np.asarray(j)
         return np.where(j < self.n_local, j, self.owner[np.clip(j - self.n_local, 0, None)])
 
     def steinhardt(self, l):
-        nc = self.centers.size
-        q = np.zeros(nc * 2 * (l + 1))
-        ql = np.zeros(nc)
-        lib().orc_steinhardt(l, nc, self.N, _p(self.sel_d), _p(self.nsel), _p(q), _p(ql))
-        return q.reshape(nc, 2 * (l + 1)), ql
+        """(q_lm, q_l) of every center; computed once per degree (the selection is fixed)."""
+        if l not in self._st:
+            nc = self.centers.size
+            q = np.zeros(nc * 2 * (l + 1))
+            ql = np.zeros(nc)
+            lib().orc_steinhardt(l, nc, self.N, _p(self.sel_d), _p(self.nsel), _p(q), _p(ql))
+            self._st[l] = (q.reshape(nc, 2 * (l + 1)), ql)
+        q, ql = self._st[l]
+        return q.copy(), ql.copy()
 
     def q_ave(self, l):
         _, ql = self.steinhardt(l)
@@ -193,9 +198,11 @@ def sk(x, k):
     return v, A[0::2] + 1j * A[1::2], S
 
 
-def sk_grad(x, k, w=1.0):
+def sk_grad(x, k, w=1.0, A=None):
+    """w dCV/dr_j (n, 3); A = the amplitudes from sk() when already computed."""
     x, k = _f64(x).reshape(-1, 3), _f64(k).reshape(-1, 3)
-    _, A, _ = sk(x, k)
+    if A is None:
+        _, A, _ = sk(x, k)
     Ai = np.zeros(2 * k.shape[0])
     Ai[0::2], Ai[1::2] = A.real, A.imag
     out = np.zeros_like(x)

@@ -1,7 +1,7 @@
 """GPU parity of the HIP path against the FP64 oracle (run on an MI355X with -m gpu).
 
 Bars (written in each test):
-  selection (indices and slot order)  bit-exact
+  selection (kept set of the step, and slot order)  bit-exact
   per-atom q_l and CV values          1e-5 relative
   forces                              1e-4 relative (max |F - F_ref| / max |F_ref|)
 """
@@ -52,14 +52,22 @@ def test_selection_bit_exact(mdt, orc, case):
     ctx = _ctx(mdt, s)
     fix = _fix_q(mdt, ctx, N=N, rc=rc)
     fix.post_force(0)
-    sel, cnt, ns = fix.selection("Q6", n, N)
     env = orc.Env(x, off, idx, own, n, rc, N)
+    # the step's own selection: the set its k_env kept (decoded from the step's mask),
+    # in list order, against the oracle's literal insertion
+    kept = fix.extract("Q6.kept", n, N).astype(np.int64)
+    want = np.full((n, N), -1, np.int64)
+    for i in range(n):
+        pos = np.sort(env.sel_pos[i][env.sel_pos[i] >= 0])
+        want[i, : pos.size] = idx[off[i] + pos]
+    assert np.array_equal(kept, want), f"kept set differs at {np.nonzero((kept != want).any(axis=1))[0][:5]}"
+    assert np.array_equal(fix.extract("Q6.nsel", n).astype(np.int32), env.nsel)
+    assert np.array_equal(fix.extract("Q6.count_rc", n).astype(np.int32), env.cnt)
+    # slot order: the export instance of the same kernel runs the insertion for every atom
+    sel, cnt, ns = fix.selection("Q6", n, N)
     assert np.array_equal(cnt, env.cnt)
     assert np.array_equal(ns, env.nsel)
     assert np.array_equal(sel, env.sel_idx), "slot order differs"
-    # the fused hot path keeps the same set
-    assert np.array_equal(fix.extract("Q6.nsel", n).astype(np.int32), env.nsel)
-    assert np.array_equal(fix.extract("Q6.count_rc", n).astype(np.int32), env.cnt)
     if case == "liquid_capped":
         assert (env.cnt > N).sum() > 100
     if case == "fcc_at_cutoff":

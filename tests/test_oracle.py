@@ -166,6 +166,26 @@ def test_selection_bruteforce_no_ties(orc):
         assert pos[:ns] == order
 
 
+def test_tie_case_separates_insertion_from_stable_sort(mdt, orc):
+    """The GPU selection test's tie case (perfect FCC, N = 8 of 12 equidistant neighbours)
+    tells the reference insertion apart from the nearest-N rule with list-order ties: a
+    kernel that kept the stable N smallest, the likeliest slip, keeps a different set for
+    hundreds of atoms there, so it fails the set comparison of the GPU test."""
+    s = fcc_system(mdt, 6, sigma=0.0)
+    x, off, idx, own = s.arrays()
+    n, N, rc = s.n_local, 8, 3.9
+    env = orc.Env(x, off, idx, own, n, rc, N)
+    differ = 0
+    for i in range(n):
+        seg = idx[off[i]:off[i + 1]]
+        d = x[seg] - x[i]
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        ok = np.nonzero((r2 <= rc * rc) & (r2 >= 1e-12))[0]
+        stable = sorted(ok, key=lambda p: (r2[p], p))[:N]
+        differ += set(seg[stable]) != set(env.sel_idx[i][env.sel_idx[i] >= 0])
+    assert differ > 100
+
+
 # ------------------------------------------------------------------ gradients
 def _move_all(x_all, n, owner, k, a, h):
     y = x_all.copy()
