@@ -183,6 +183,15 @@ int mdt_fix_get_forces(mdt_fix* fix, int walker, double* f_host);
  * (src/core/fix_crystallize.cu:453-469).  image: LAMMPS image flags, 3 ints
  * per local atom, NULL = all zero. */
 int mdt_fix_virial(mdt_fix* fix, int walker, const int32_t* image, double v[6]);
+/* The virial of mdt_fix_virial with nothing allocated: no reference counterpart;
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  image: device pointer to
+ * 3 ints per local atom, or NULL; v: device pointer to 6 doubles, written on the
+ * context's stream (complete after mdt_fix_sync or any later synchronising call),
+ * the same bits as mdt_fix_virial.  Block partial sums go to a fixed per-device
+ * scratch array, so a walker may hold at most 8192 * 256 = 2,097,152 local atoms
+ * (larger: MDT_EINVAL), and calls that overlap on one device race. */
+int mdt_fix_virial_noalloc(mdt_fix* fix, int walker, const int32_t* image, double* v);
 /* Zero the context's device force buffer. */
 int mdt_fix_zero_forces(mdt_fix* fix);
 

@@ -335,6 +335,13 @@ hipError_t launch_reduce(const ReduceDev& r, hipStream_t st);
 // values of CVs whose partial sums are produced after the reduce (unbiased, fused into the force pass)
 hipError_t launch_cv_tail(const ReduceDev& r, hipStream_t st);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// allocation-free virial: the per-device scratch of block partials (looked up once per
+// device by scratch_setup, which must run with no graph capture active) and the
+// fixed-order sum of a walker's partials into v[6]
+constexpr int kScratchBlocks = 8192;
+hipError_t scratch_setup(int device);
+double* scratch_ptr(int device);
+hipError_t launch_virial_sum(const double* part, int nb, double* v, hipStream_t st);
 hipError_t launch_virial(const double4* x4, const double* f, const int32_t* image, int first, int n,
                          const double L[3], double* out, hipStream_t st);
 hipError_t launch_fill_f(float* p, int64_t n, float v, hipStream_t st);

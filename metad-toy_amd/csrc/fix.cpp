@@ -1711,7 +1711,9 @@ int mdt_create(mdt_ctx** out, int device, int n_walkers) {
     mdt_ctx* c = new mdt_ctx();
     c->device = device;
     c->w.resize(n_walkers);
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        scratch_setup(device) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return MDT_EHIP;
     }
@@ -2476,6 +2478,26 @@ int mdt_fix_virial(mdt_fix* fx, int walker, const int32_t* image, double v[6]) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < 6; ++k) v[k] = 0.0;
     for (int b = 0; b < nb; ++b) for (int k = 0; k < 6; ++k) v[k] += h[(size_t)b * 6 + k];
+    API_END
+}
+
+int mdt_fix_virial_noalloc(mdt_fix* fx, int walker, const int32_t* image, double* v) {
+    FIX_BEGIN(fx)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    WalkerIn& a = ctx->walker(walker);
+    if (!ctx->committed) throw Err(MDT_ESTATE, "no forces yet");
+    if (!v) throw Err(MDT_EINVAL, "v must be a device pointer to 6 doubles");
+    const int64_t n = a.n_local;
+    if (n > (int64_t)kScratchBlocks * kBlock)
+        throw Err(MDT_EINVAL, fmt("walker %d has %lld local atoms; the allocation-free virial takes at most %lld",
+                                  walker, (long long)n, (long long)kScratchBlocks * kBlock));
+    double* part = scratch_ptr(ctx->device);
+    if (!part) throw Err(MDT_ESTATE, "device scratch was not set up (mdt_create)");
+    const int nb = (int)((n + kBlock - 1) / kBlock);
+    const double L[3] = {a.hi[0] - a.lo[0], a.hi[1] - a.lo[1], a.hi[2] - a.lo[2]};
+    HIPCHK(launch_virial(ctx->x4.p, ctx->f.p, image, (int)a.atom_off, (int)n, L, part, ctx->stream));
+    HIPCHK(launch_virial_sum(part, nb, v, ctx->stream));
     API_END
 }
 

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <utility>
 
 #include "engine.h"
@@ -2815,6 +2816,60 @@ hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStre
 hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, int32_t* eidx, int32_t* eown,
                           hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_lq_erev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, apos, erev, eidx, eown);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free virial (mdt_fix_virial_noalloc): the per-block
+// partial sums of k_virial, 6 doubles per 256-atom block, for walkers of up to
+// kScratchBlocks * kBlock local atoms.  One copy per device, kept until the process
+// ends; its device address is looked up once per device (scratch_setup) and only
+// that address is handed to kernels.  Every call on a device shares it, so calls that
+// overlap on one device race; every block's partials are written before they are
+// read, so nothing in it has to start at zero.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[6 * kScratchBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the six sums of a walker's block partials, each in block order (the order of the
+// host sum of mdt_fix_virial, so both give the same bits)
+__global__ void k_virial_sum(const double* __restrict__ part, int nb, double* __restrict__ v) {
+    const int k = threadIdx.x;
+    if (k >= 6) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[6 * b + k];
+    v[k] = s;
+}
+}  // namespace
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return hipSuccess;
+}
+
+double* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+hipError_t launch_virial_sum(const double* part, int nb, double* v, hipStream_t st) {
+    hipLaunchKernelGGL(k_virial_sum, dim3(1), dim3(64), 0, st, part, nb, v);
     return hipGetLastError();
 }
 

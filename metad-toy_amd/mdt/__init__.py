@@ -68,6 +68,7 @@ def lib():
         "mdt_fix_get_forces": (I, [P, I, dp]),
         "mdt_fix_zero_forces": (I, [P]),
         "mdt_fix_virial": (I, [P, I, ip, dp]),
+        "mdt_fix_virial_noalloc": (I, [P, I, P, P]),
         "mdt_fix_mlp_refit": (I, [P, I]),
         "mdt_fix_mlp_status": (I, [P, I, dp, ip, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)]),
         "mdt_fix_mlp_params": (I, [P, I, I, C.POINTER(C.c_float), dp]),
@@ -409,6 +410,11 @@ class Fix:
         img = None if image is None else np.ascontiguousarray(image, np.int32)
         self._chk(lib().mdt_fix_virial(self._h, walker, _ip(img), _dp(v)))
         return v
+
+    def virial_noalloc(self, walker, image_dev, v_dev):
+        """mdt_fix_virial_noalloc: image_dev (device int32, 3 per local atom, or None) and
+        v_dev (device float64, 6) are raw device addresses; nothing is allocated."""
+        self._chk(lib().mdt_fix_virial_noalloc(self._h, walker, image_dev, v_dev))
 
     def mlp_refit(self, walker=0):
         self._chk(lib().mdt_fix_mlp_refit(self._h, walker))
