@@ -111,6 +111,39 @@ def line(value, steps, warmup, ms, dtype, config, roof, cpu, metric=METRIC, unit
             "config": config, "roofline": roof, "cpu_baseline": cpu}
 
 
+class EngineBuffer:
+    """Device copy of an array made with the HIP runtime the library links: the MD
+    engine's own position buffer in a real run."""
+
+    def __init__(self, a):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        a = np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        if self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(a.nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if self.hip.hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+    def free(self):
+        self.hip.hipFree(self.ptr)
+
+
+def windows(fix, step0, steps, nwin, per_launch):
+    """Median over nwin windows of `steps` timed steps (each window bracketed by stream
+    synchronisations and a state read); returns (median seconds, all window seconds)."""
+    ts = []
+    for k in range(nwin):
+        ts.append(timed_run(fix, step0 + k * steps, steps, per_launch))
+    return float(np.median(ts)), ts
+
+
+def cpu_info(orc):
+    model, ncpu = orc.host_cpu()
+    return {"cores": orc.threads(), "host_logical_cpus": ncpu, "cpu_model": model}
+
+
 # ------------------------------------------------------------------ config 2 (headline)
 def run_c2(mdt, args):
     log("config 2: building 256k-atom system")
@@ -118,17 +151,24 @@ def run_c2(mdt, args):
     n = system.n_local
     ctx = mdt.Context(0, 1)
     ctx.bind_system(0, system)
+    # every step reads the positions from the engine's device buffer (packed at the
+    # start of the step, inside its graph): the per-step position ingestion of an MD step
+    xbuf = EngineBuffer(system.arrays()[0])
+    ctx.attach_positions(0, xbuf.ptr.value)
     fix = mdt.Fix(ctx, C2_ARGS)
     fix.init()
-    fix.run(0, args.warmup)
-    dt = timed_run(fix, args.warmup, args.steps)
+    fix.run(0, args.warmup, per_launch=1)
+    step = args.warmup
+    # headline: one graph launch per step, what a caller that integrates between steps pays
+    dt, ts1 = windows(fix, step, args.steps, args.windows, 1)
+    step += args.windows * args.steps
     value = n * args.steps / dt
-    log(f"c2: {dt * 1e6 / args.steps:.1f} us/step, {value:.3e} atom-steps/s")
-    # the same steps with one graph launch per step (what a caller that runs an
-    # integrator between steps pays)
-    dt1 = timed_run(fix, args.warmup + args.steps, args.steps, per_launch=1)
-    log(f"c2, one launch per step: {dt1 * 1e6 / args.steps:.1f} us/step, {n * args.steps / dt1:.3e} atom-steps/s")
-    kms = fix.profile(args.warmup + 2 * args.steps, args.profile_steps)
+    log(f"c2, one launch per step: {dt * 1e6 / args.steps:.1f} us/step (median of {args.windows}), "
+        f"{value:.3e} atom-steps/s")
+    dt8, ts8 = windows(fix, step, args.steps, args.windows, 8)
+    step += args.windows * args.steps
+    log(f"c2, 8 steps per graph launch: {dt8 * 1e6 / args.steps:.1f} us/step, {n * args.steps / dt8:.3e} atom-steps/s")
+    kms = fix.profile(step, args.profile_steps)
     log("kernel mean ms: " + json.dumps(kms))
     roof, pairs = steinhardt_roofline(fix, system, kms, fused_ld=True)
     cpu = None
@@ -141,9 +181,12 @@ def run_c2(mdt, args):
             env.q_ave(6), env.q_ave(4), env.ld(6)
             -(env.dQ_dx(6, 1.0) + env.dQ_dx(4, 1.0))
         k, t = oracle_steps(one, args.cpu_budget)
-        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+        ci = cpu_info(orc)
+        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": ci["cores"], "kind": "port",
                "sample": f"{k} full step(s) of the same {n}-atom workload (selection, Q6, Q4, LQ6, exact "
-                         f"Q6+Q4 forces) by the FP64 C++ oracle on 1 core, {t:.1f} s"}
+                         f"Q6+Q4 forces) by the FP64 C++ oracle, OpenMP over atoms on {ci['cores']} threads "
+                         f"({ci['cpu_model']}, {ci['host_logical_cpus']} logical CPUs), {t:.1f} s",
+               "cpu_model": ci["cpu_model"], "host_logical_cpus": ci["host_logical_cpus"]}
     # whole-step roofline: sum over phases of max(bytes / HBM peak, FLOP / FP32 peak):
     # selection (HBM), forward q_lm, gradient, local-Q6 forward (FP32)
     t_roof = (roof["selection_bytes"] / (HBM_PEAK_GBS * 1e9) +
@@ -152,10 +195,13 @@ def run_c2(mdt, args):
                  "phases": "selection bytes / 8 TB/s + (261 + 903 + 28) FLOP x selected pairs / 157.3 TFLOP/s"}
     cfg = {"workload": "config 2: 256,000-atom FCC Ar half melted, Q6.AVE+Q4.AVE+LD-LQ6 (N=18, rc=3.9), "
                        "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms,
-           "step_roofline": step_roof,
-           "steps_per_graph_launch": 8,
-           "one_launch_per_step": {"value": n * args.steps / dt1, "ms_per_step": dt1 * 1e3 / args.steps},
+           "step": "positions read from the engine's device buffer, selection, Q6, Q4, LQ6.AVE, FP64 sums, "
+                   "hill deposit and bias, chain rule, exact forces: one graph launch per step",
+           "windows_s": ts1, "step_roofline": step_roof,
+           "eight_steps_per_graph_launch": {"value": n * args.steps / dt8, "ms_per_step": dt8 * 1e3 / args.steps,
+                                            "windows_s": ts8},
            "neighbor_build": neighbor_build_ms(mdt, system)}
+    xbuf.free()
     return line(value, args.steps, args.warmup, dt * 1e3 / args.steps, "f32 pair math, f64 sums/grid", cfg, roof, cpu)
 
 
@@ -415,13 +461,14 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--windows", type=int, default=5, help="timed windows of --steps steps; the median is reported")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lq", "copy"])
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the multi-GPU decomposition is a design, not built)", file=sys.stderr)
         sys.exit(2)
     import mdt
     if not os.path.exists(mdt.LIB_PATH):

@@ -88,8 +88,17 @@ int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group);
 /* Convenience: box, positions and lists of a host-built mdt_system. */
 int mdt_bind_system(mdt_ctx* ctx, int walker, const mdt_system* sys);
 
-/* New positions for an already bound walker (same n_all, same lists). */
+/* New positions for an already bound walker (same n_all, same lists).  A device
+ * buffer is packed straight into the context's layout (one kernel, no copy). */
 int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_device);
+
+/* Read a walker's positions from the MD engine's device buffer x (n_all*3
+ * doubles, locals then ghosts, the layout bound last) at the start of every step,
+ * inside the step's launches and graphs: the per-step upload of
+ * Steinhardt_env::get_env (src/cvs/CVs_Steinhardt.cu:552-566) without a copy or a
+ * host call.  The buffer must stay valid and keep its layout until the next
+ * attach; x == NULL detaches. */
+int mdt_attach_positions(mdt_ctx* ctx, int walker, const double* x);
 
 /* ------------------------------------------------------------------------
  * fix metad

@@ -324,6 +324,8 @@ hipError_t nl_sort_pairs(void* tmp, size_t& bytes, const int32_t* kin, int32_t* 
 hipError_t launch_topology(const Geo& g, int64_t n_local_total, int32_t* eown, uint8_t* revpos,
                            hipStream_t st);
 hipError_t launch_pack_x4(const double* x, double4* x4, float4* x4f, int64_t n, hipStream_t st);
+hipError_t launch_pack_walker(const double* x, int64_t n_local, int64_t n_all, double4* x4_loc, float4* x4f_loc,
+                              double4* x4_gh, float4* x4f_gh, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, hipStream_t st);
 hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st);
 // the selection export instance of k_env (e.sel_out, e.sel_cnt set)

@@ -222,6 +222,7 @@ struct WalkerIn {
     std::vector<double> x;
     std::vector<int32_t> off, idx, owner;
     std::vector<uint8_t> group;
+    const double* x_src = nullptr;   // attached engine buffer read at the start of every step
     // after commit
     int64_t atom_off = 0, ghost_off = 0, nnz_off = 0;
     int32_t tile_off = 0, ntiles = 0;
@@ -236,6 +237,7 @@ struct mdt_ctx {
     std::string err;
     bool committed = false;
     uint64_t generation = 0;   // bumps on every commit (invalidates graphs)
+    uint64_t generation_attach = 0;   // bumps on every attach (invalidates graphs)
     int64_t total_local = 0, total_ghost = 0, total_nnz = 0;
     bool any_group = false;
     DBuf<double> x;
@@ -268,6 +270,12 @@ struct mdt_ctx {
         return w[k];
     }
     void commit();
+    // pack walker k's positions x (n_all*3 device doubles, locals then ghosts)
+    void pack(int k, const double* xd) {
+        WalkerIn& a = w[k];
+        HIPCHK(launch_pack_walker(xd, a.n_local, a.n_all, x4.p + a.atom_off, x4f.p + a.atom_off,
+                                  x4.p + total_local + a.ghost_off, x4f.p + total_local + a.ghost_off, stream));
+    }
 };
 
 void mdt_ctx::commit() {
@@ -476,6 +484,7 @@ struct mdt_fix {
     // counter included, lives on the device, so steps replay back to back)
     std::map<int, hipGraphExec_t> run_graphs;
     uint64_t graph_gen = 0;
+    uint64_t graph_attach = 0;
     // captured training loops of the learned bias, per (walker, first fit)
     std::map<std::pair<int, int>, hipGraphExec_t> mlp_graphs;
     uint64_t mlp_graph_gen = 0;
@@ -1551,6 +1560,9 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
     mdt_ctx* ctx = fx->ctx;
     hipStream_t st = ctx->stream;
     Geo g = ctx->geo();
+    // positions of walkers attached to an engine buffer are read at the start of the step
+    for (size_t k = 0; k < ctx->w.size(); ++k)
+        if (ctx->w[k].x_src) T.run("k_pack_walker", st, [&] { ctx->pack((int)k, ctx->w[k].x_src); });
     for (auto* e : fx->envs) {
         Geo ge = g;
         ge.group = e->group == "all" ? nullptr : ctx->group.p;
@@ -1630,9 +1642,11 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             T.run("k_lq_force", st, [&] { HIPCHK(launch_lq(6, gs, d, f, st)); });
         }
     }
-    // The values of unbiased local-Q CVs fused into the force pass (their tile sums
-    // of the last step) are reduced when the state is read, in fetch_state: nothing
-    // in the step consumes them.
+    // The values of unbiased local-Q CVs fused into the force pass: the fixed-order
+    // sum of the tile partials the force pass wrote this step.
+    bool tail = false;
+    for (auto* c : fx->cvs) tail |= c->fused;
+    if (tail) T.run("k_cv_tail", st, [&] { HIPCHK(launch_cv_tail(fx->red, st)); });
 }
 
 void use_device(mdt_ctx* ctx) {
@@ -1655,12 +1669,6 @@ void set_steps(mdt_fix* fx, int64_t step) {
 void fetch_state(mdt_fix* fx) {
     const int nw = fx->nw();
     fx->h_state.resize((size_t)nw * kStStride);
-    bool tail = false;
-    for (auto* c : fx->cvs) tail |= c->fused;
-    if (tail && fx->built_gen == fx->ctx->generation) {
-        use_device(fx->ctx);
-        HIPCHK(launch_cv_tail(fx->red, fx->ctx->stream));
-    }
     HIPCHK(hipMemcpyAsync(fx->h_state.data(), fx->state.p, sizeof(double) * fx->h_state.size(), hipMemcpyDeviceToHost, fx->ctx->stream));
     HIPCHK(hipStreamSynchronize(fx->ctx->stream));
     fx->state_valid = true;
@@ -1766,6 +1774,7 @@ int mdt_bind_positions(mdt_ctx* ctx, int walker, const double* x, int64_t n_loca
         return mdt_update_positions(ctx, walker, x, on_device);
     }
     a.n_local = n_local; a.n_all = n_all;
+    a.x_src = nullptr;                          // a new layout detaches an engine buffer
     a.x.resize((size_t)n_all * 3);
     copy_in(a.x.data(), x, sizeof(double) * 3 * n_all, on_device);
     a.has_x = true;
@@ -1832,14 +1841,24 @@ int mdt_update_positions(mdt_ctx* ctx, int walker, const double* x, int on_devic
         copy_in(a.x.data(), x, sizeof(double) * 3 * a.n_all, on_device);
         return MDT_OK;
     }
-    hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * a.atom_off, x, sizeof(double) * 3 * a.n_local, k, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), x + 3 * a.n_local,
-                          sizeof(double) * 3 * (a.n_all - a.n_local), k, ctx->stream));
-    HIPCHK(launch_pack_x4(ctx->x.p + 3 * a.atom_off, ctx->x4.p + a.atom_off, ctx->x4f.p + a.atom_off, a.n_local, ctx->stream));
-    HIPCHK(launch_pack_x4(ctx->x.p + 3 * (ctx->total_local + a.ghost_off), ctx->x4.p + ctx->total_local + a.ghost_off,
-                          ctx->x4f.p + ctx->total_local + a.ghost_off, a.n_all - a.n_local, ctx->stream));
-    if (!on_device) HIPCHK(hipStreamSynchronize(ctx->stream));   // pageable source must stay valid
+    if (on_device) {
+        ctx->pack(walker, x);                   // straight from the caller's device buffer
+    } else {
+        // staged through the context's FP64 buffer (the walker's slice of it)
+        double* st = ctx->x.p + 3 * a.atom_off + 3 * a.ghost_off;
+        HIPCHK(hipMemcpyAsync(st, x, sizeof(double) * 3 * a.n_all, hipMemcpyHostToDevice, ctx->stream));
+        ctx->pack(walker, st);
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // pageable source must stay valid
+    }
+    API_END
+}
+
+int mdt_attach_positions(mdt_ctx* ctx, int walker, const double* x) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "bind positions (and lists) before attaching a position buffer");
+    a.x_src = x;
+    ctx->generation_attach++;
     API_END
 }
 
@@ -1933,7 +1952,8 @@ constexpr int kRunBatch = 8;   // steps per graph launch in mdt_fix_run
 // The graph of k captured steps, built on first use and after a rebuild.
 static hipGraphExec_t run_graph(mdt_fix* fx, int k) {
     mdt_ctx* ctx = fx->ctx;
-    if (fx->graph_gen != ctx->generation + 1) {
+    if (fx->graph_gen != ctx->generation + 1 || fx->graph_attach != ctx->generation_attach) {
+        fx->graph_attach = ctx->generation_attach;
         for (auto& kv : fx->run_graphs) (void)hipGraphExecDestroy(kv.second);
         fx->run_graphs.clear();
         fx->graph_gen = ctx->generation + 1;

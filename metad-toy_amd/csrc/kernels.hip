@@ -306,6 +306,23 @@ __global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4
     }
 }
 
+// One walker's positions as the MD engine holds them (locals then ghost images,
+// 3 FP64 each) packed into the context's concatenated layout: locals at x4[loc..],
+// ghosts at x4[gh..].  Runs at the start of a step when the walker's positions are
+// read from an attached engine buffer, and in mdt_update_positions.
+__global__ void k_pack_walker(const double* __restrict__ x, int64_t n_local, int64_t n_all, double4* __restrict__ x4_loc,
+                              float4* __restrict__ x4f_loc, double4* __restrict__ x4_gh, float4* __restrict__ x4f_gh) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_all; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = x[3 * i], b = x[3 * i + 1], c = x[3 * i + 2];
+        const float af = (float)a, bf = (float)b, cf = (float)c;
+        const uint32_t w = pack_res1(a, af) | (pack_res1(b, bf) << 10) | (pack_res1(c, cf) << 20);
+        const bool loc = i < n_local;
+        const int64_t k = loc ? i : i - n_local;
+        (loc ? x4_loc : x4_gh)[k] = make_double4(a, b, c, 0.0);
+        (loc ? x4f_loc : x4f_gh)[k] = make_float4(af, bf, cf, __uint_as_float(w));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 128-bit candidate masks and lane-group helpers.  The per-atom kernels run
 // kLanes adjacent lanes per atom; lane l takes the set bits of rank l, l+kLanes,
@@ -1239,8 +1256,12 @@ __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, d
     double i2s2[kMaxDim] = {0, 0, 0};
     #pragma unroll
     for (int d = 0; d < kMaxDim; ++d) if (d < r.D) i2s2[d] = uniform_f64(0.5 / (r.sigma[d] * r.sigma[d]));
+    // A hill's widths are the bias widths scaled by its own sigma over the first bias
+    // width (a hill replayed from a HILLS file keeps the file's sigma); deposits carry
+    // the first width, so their scale is exactly 1 and costs no division.
+    const double sig0 = uniform_f64(r.sigma[0]);
     for (int64_t h = h_first + threadIdx.x; h < nh; h += HU * B) {
-        double c[HU][kMaxDim], hgt[HU];
+        double c[HU][kMaxDim], hgt[HU], hs[HU];
         #pragma unroll
         for (int u = 0; u < HU; ++u) {
             const int64_t hu = h + u * B < nh ? h + u * B : h;
@@ -1248,20 +1269,22 @@ __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, d
             #pragma unroll
             for (int d = 0; d < kMaxDim; ++d) c[u][d] = d < r.D ? rec[1 + d] : 0.0;
             hgt[u] = rec[1 + r.D];
+            hs[u] = rec[2 + r.D];
         }
         #pragma unroll
         for (int u = 0; u < HU; ++u) {
             if (h + u * B >= nh) break;
+            const double sc = hs[u] == sig0 ? 1.0 : (sig0 * sig0) / (hs[u] * hs[u]);
             double ex = 0, dd[kMaxDim] = {0, 0, 0};
             #pragma unroll
             for (int d = 0; d < kMaxDim; ++d) if (d < r.D) {
                 dd[d] = s[d] - c[u][d];
                 ex += dd[d] * dd[d] * i2s2[d];
             }
-            double t = hgt[u] * exp(-ex);
+            double t = hgt[u] * exp(-ex * sc);
             acc[0] += t;
             #pragma unroll
-            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc[1 + d] -= t * dd[d] * (2.0 * i2s2[d]);
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc[1 + d] -= t * dd[d] * (2.0 * i2s2[d] * sc);
         }
     }
     block_sum<4, NW>(acc, sh);
@@ -2576,6 +2599,14 @@ hipError_t launch_pack_x4(const double* x, double4* x4, float4* x4f, int64_t n, 
     if (n == 0) return hipSuccess;
     int64_t b = (n + 255) / 256; if (b > 8192) b = 8192;
     hipLaunchKernelGGL(k_pack_x4, dim3((unsigned)b), dim3(256), 0, st, x, x4, x4f, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_walker(const double* x, int64_t n_local, int64_t n_all, double4* x4_loc, float4* x4f_loc,
+                              double4* x4_gh, float4* x4f_gh, hipStream_t st) {
+    if (n_all == 0) return hipSuccess;
+    int64_t b = (n_all + 255) / 256; if (b > 8192) b = 8192;
+    hipLaunchKernelGGL(k_pack_walker, dim3((unsigned)b), dim3(256), 0, st, x, n_local, n_all, x4_loc, x4f_loc, x4_gh, x4f_gh);
     return hipGetLastError();
 }
 

@@ -68,6 +68,7 @@ constexpr int kHillChunk = 256;
 __global__ __launch_bounds__(256) void k_mlp_data(MlpDev m, int w, int64_t n, uint64_t seed, float4* __restrict__ s_out,
                                                    double* __restrict__ v_out) {
     __shared__ double hc[kHillChunk * 4];
+    __shared__ double hsc[kHillChunk];
     const int64_t nh_all = m.nhills[w];
     const int64_t nfit = nh_all;
     const double* Hs = m.hills + (int64_t)w * m.hills_cap * (m.D + 3);
@@ -96,17 +97,21 @@ __global__ __launch_bounds__(256) void k_mlp_data(MlpDev m, int w, int64_t n, ui
         for (int k = threadIdx.x; k < kHillChunk; k += blockDim.x) {
             const int64_t h = h0 + k;
             if (h < nfit) {
+                // a replayed hill keeps its own sigma: its widths are the bias widths times
+                // sigma / sigma_0; the scale rides on the centre coordinates' units
                 const double* rec = Hs + h * (m.D + 3);
                 hc[4 * k] = rec[1]; hc[4 * k + 1] = rec[2]; hc[4 * k + 2] = rec[3]; hc[4 * k + 3] = rec[4];
+                hsc[k] = rec[5] == m.sigma[0] ? 1.0 : (m.sigma[0] * m.sigma[0]) / (rec[5] * rec[5]);
             } else {
                 hc[4 * k + 3] = 0.0; hc[4 * k] = hc[4 * k + 1] = hc[4 * k + 2] = 1e300;
+                hsc[k] = 1.0;
             }
         }
         __syncthreads();
         const int lim = nfit - h0 < kHillChunk ? (int)(nfit - h0) : kHillChunk;
         for (int k = 0; k < lim; ++k) {
             const double a = s[0] - hc[4 * k], b = s[1] - hc[4 * k + 1], c = s[2] - hc[4 * k + 2];
-            const double e = a * a * i2s0 + b * b * i2s1 + c * c * i2s2;
+            const double e = (a * a * i2s0 + b * b * i2s1 + c * c * i2s2) * hsc[k];
             if (e < 40.0) v += hc[4 * k + 3] * exp(-e);
         }
     }

@@ -50,6 +50,7 @@ def lib():
         "mdt_bind_group": (I, [P, I, u8p]),
         "mdt_bind_system": (I, [P, I, P]),
         "mdt_update_positions": (I, [P, I, P, I]),
+        "mdt_attach_positions": (I, [P, I, P]),
         "mdt_fix_create": (I, [P, I, C.POINTER(C.c_char_p), C.POINTER(P)]),
         "mdt_fix_destroy": (None, [P]),
         "mdt_fix_init": (I, [P]),
@@ -249,6 +250,15 @@ class Context:
     def update_positions(self, walker, x_all):
         x = np.ascontiguousarray(x_all, np.float64)
         self._chk(lib().mdt_update_positions(self._h, walker, x.ctypes.data, 0))
+
+    def update_positions_device(self, walker, x_dev):
+        """mdt_update_positions from a device address (n_all*3 doubles)."""
+        self._chk(lib().mdt_update_positions(self._h, walker, x_dev, 1))
+
+    def attach_positions(self, walker, x_dev):
+        """mdt_attach_positions: every step reads the walker's positions from the device
+        address x_dev (None detaches)."""
+        self._chk(lib().mdt_attach_positions(self._h, walker, x_dev))
 
 
 def _fp(a):

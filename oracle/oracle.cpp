@@ -187,6 +187,14 @@ extern "C" {
 
 // Oracle build tag: lets a test confirm it loaded this file's build.
 int orc_version() { return 1; }
+// threads an OpenMP region of the oracle runs on (1 without OpenMP)
+int orc_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 void orc_ylm(int l, double dx, double dy, double dz, double* Y) { ylm_reference(l, dx, dy, dz, Y); }
 void orc_ylm_grad(int l, double dx, double dy, double dz, double* dY) { ylm_grad(l, dx, dy, dz, dY); }

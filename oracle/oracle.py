@@ -32,6 +32,7 @@ def lib():
         P = C.c_void_p
         I, D = C.c_int, C.c_double
         for name, res, args in [
+            ("orc_num_threads", I, []),
             ("orc_ylm", None, [I, D, D, D, P]),
             ("orc_ylm_poly", None, [I, D, D, D, P]),
             ("orc_ylm_grad", None, [I, D, D, D, P]),
@@ -59,6 +60,25 @@ def lib():
             f.restype = res
             f.argtypes = args
     return _L
+
+
+def threads():
+    """Threads the oracle's OpenMP loops run on."""
+    return lib().orc_num_threads()
+
+
+def host_cpu():
+    """Model name of the host CPU and its logical core count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
 
 def _p(a):
@@ -253,13 +273,27 @@ class GridT1(Grid):
         return h0 * np.exp(-V / (T * kB * (biasf - 1.0)))
 
 
-def hills_sum(centers, heights, sigma, s):
+def hills_sum(centers, heights, sigma, s, hill_sigma=None):
+    """Exact sum of Gaussians at points s (np, D): V (np,) and dV/ds (np, D).  sigma: widths
+    per dimension; hill_sigma (per hill, optional): hill h has widths sigma * hill_sigma[h] /
+    sigma[0] (a hill replayed from a HILLS file with its own sigma)."""
     c = _f64(centers)
     D = c.shape[1]
     h, sg, s = _f64(heights), _f64(sigma), _f64(np.atleast_2d(s))
     V = np.zeros(s.shape[0])
     dV = np.zeros_like(s)
-    lib().orc_hills_sum(D, h.size, _p(c), _p(h), _p(sg), s.shape[0], _p(s), _p(V), _p(dV))
+    if hill_sigma is None:
+        lib().orc_hills_sum(D, h.size, _p(c), _p(h), _p(sg), s.shape[0], _p(s), _p(V), _p(dV))
+        return V, dV
+    hs = _f64(hill_sigma)
+    for v in np.unique(hs):
+        k = np.nonzero(hs == v)[0]
+        Vk, dVk = np.zeros_like(V), np.zeros_like(dV)
+        sk_ = _f64(sg * (v / sg[0]))
+        ck, hk = _f64(c[k]), _f64(h[k])
+        lib().orc_hills_sum(D, hk.size, _p(ck), _p(hk), _p(sk_), s.shape[0], _p(s), _p(Vk), _p(dVk))
+        V += Vk
+        dV += dVk
     return V, dV
 
 
