@@ -85,6 +85,14 @@ int mdt_bind_neighbors(mdt_ctx* ctx, int walker, const int32_t* offsets, const i
  * Replaces the mask/groupbit compaction of refresh_lmpbox (:391-425). */
 int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group);
 
+/* Membership of a named group (the group word of a CAL: STEINH ... <group>,
+ * STRU_FACTOR <group>), 1 = in group, for local atoms; in_group == NULL unbinds
+ * it.  CALs may name different groups, each averaged over its own atoms, as
+ * the reference pools environments per group (Steinhardt_env::get_or_create,
+ * src/cvs/CVs_Steinhardt.cu:191-220, over LAMMPS atom->mask group bits).  A
+ * group name without a binding of its own uses the mdt_bind_group membership. */
+int mdt_bind_group_named(mdt_ctx* ctx, int walker, const char* name, const uint8_t* in_group);
+
 /* Convenience: box, positions and lists of a host-built mdt_system. */
 int mdt_bind_system(mdt_ctx* ctx, int walker, const mdt_system* sys);
 

@@ -12,6 +12,10 @@ constexpr int kTile = kBlock / kLanes;   // atoms per tile (one workgroup)
 constexpr int kMaxList = 128;      // max neighbour-list length per atom (2 x 64-bit masks)
 constexpr int kMaxSlots = 64;      // max cutoff_Natoms
 constexpr int kMaxSlotCV = 16;     // CV value slots per walker
+// per-walker chain-rule weights: [0, kMaxSlotCV) dV/dv of each CV slot; from kMaxSlotCV
+// on, per representative slot, the sum over the slots aliased to it (CALs with identical
+// STEINH Q definitions share one environment degree and its force pass)
+constexpr int kWcvStride = 2 * kMaxSlotCV;
 constexpr int kMaxSym = 8;         // symbols usable in DIM expressions
 constexpr int kMaxDim = 3;
 constexpr int kMaxProg = 96;       // expression bytecode length per dimension
@@ -93,7 +97,7 @@ struct EnvDev {
     double* partial;          // [slot][tile]
     int32_t slot_ql[3];       // partial slot per present degree (order 3,4,6)
     const double* inv_ng;     // per walker 1/N_group
-    const double* wcv;        // per walker [kMaxSlotCV] weights dV/dv (from the reduce kernel)
+    const double* wcv;        // per walker [kWcvStride] weights dV/dv (from the reduce kernel)
     int32_t wslot[3];         // wcv slot of Q_l.AVE per present degree (-1 = not biased)
     // an unbiased Lechner-Dellago CV on this (biased) environment, accumulated by
     // the force kernel from the neighbour records it loads anyway (0 = none)
@@ -173,9 +177,10 @@ struct ReduceDev {
     const Tile* tiles;
     const int32_t* tile_begin;  // per walker first tile, n_walkers + 1
     const double* partial;    // [slot][tile]
-    const double* inv_ng;     // per walker
+    const double* cv_inv;     // per walker [kMaxSlotCV]: 1 / group count of each partial-sum CV
     const double* direct;     // per walker [kMaxSlotCV] values written by other kernels
     int32_t ncv;              // value slots
+    int32_t walias[kMaxSlotCV];  // representative slot of each slot's weight sum (-1: none)
     int32_t cv_src[kMaxSlotCV];  // >= 0: partial slot; -1: direct; -2 - s: partial slot s summed after the force pass
     int32_t nsym;
     int32_t sym_slot[kMaxSym];
@@ -206,7 +211,7 @@ struct ReduceDev {
     // per-walker step counters and outputs
     int64_t* step;            // per walker
     double* state;            // per walker [kStateStride]
-    double* wcv;              // per walker [kMaxSlotCV]
+    double* wcv;              // per walker [kWcvStride]
     // learned bias
     const MlpNet* mlp;        // per walker live network, NULL unless kind == BIAS_MLP
     const int32_t* mlp_ok;    // per walker: 1 = network trusted, 0 = use exact sum
@@ -241,7 +246,7 @@ struct SfDev {
     const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)
     const double* inv_ng;     // per walker 1 / N_group
     double* direct;           // per walker [kMaxSlotCV] CV values
-    const double* wcv;        // per walker [kMaxSlotCV] chain-rule weights
+    const double* wcv;        // per walker [kWcvStride] chain-rule weights
     int32_t slot;
     int32_t n_walkers;
 };
@@ -275,7 +280,7 @@ struct LqDev {
     const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)
     const double* ng;         // per walker N_group
     double* direct;           // per walker [kMaxSlotCV] CV values
-    const double* wcv;        // per walker [kMaxSlotCV] chain-rule weights
+    const double* wcv;        // per walker [kWcvStride] chain-rule weights
     int32_t slot;
     int32_t n_walkers;
 };
@@ -292,7 +297,7 @@ struct DistDev {
     const double* box;        // per walker box lengths (3)
     double* dvec;             // per walker (dx, dy, dz, r) of the last value pass
     double* direct;           // per walker [kMaxSlotCV] CV values
-    const double* wcv;        // per walker [kMaxSlotCV] chain-rule weights
+    const double* wcv;        // per walker [kWcvStride] chain-rule weights
     int32_t slot;
 };
 hipError_t launch_dist_value(const double4* x4, const DistDev& d, hipStream_t st);

@@ -221,7 +221,9 @@ struct WalkerIn {
     int64_t n_local = 0, n_all = 0, nnz = 0;
     std::vector<double> x;
     std::vector<int32_t> off, idx, owner;
-    std::vector<uint8_t> group;
+    // group memberships of local atoms by group name ("" = the default, bound by
+    // mdt_bind_group, used by any group name without a binding of its own)
+    std::map<std::string, std::vector<uint8_t>> groups;
     const double* x_src = nullptr;   // attached engine buffer read at the start of every step
     // after commit
     int64_t atom_off = 0, ghost_off = 0, nnz_off = 0;
@@ -244,7 +246,11 @@ struct mdt_ctx {
     DBuf<double4> x4;
     DBuf<float4> x4f;
     DBuf<int32_t> off, idx, owner, eown;
-    DBuf<uint8_t> revpos, group;
+    DBuf<uint8_t> revpos;
+    // device memberships by group name, concatenated over walkers; per-walker counts
+    std::map<std::string, DBuf<uint8_t>> dgroups;
+    std::map<std::string, std::vector<double>> gcount;
+    std::map<std::string, std::string> gmissing;   // name -> the first walker without a membership
     DBuf<Tile> tiles;
     DBuf<double> f;
     std::vector<Tile> h_tiles;
@@ -264,6 +270,27 @@ struct mdt_ctx {
         g.group = nullptr;
         g.tiles = tiles.p; g.ntiles = (int32_t)h_tiles.size(); g.n_local_total = (int32_t)total_local;
         return g;
+    }
+    // device membership of a group name (nullptr for "all"); a name without a binding
+    // of its own falls back to the default group
+    const std::string& group_key(const std::string& name) const {
+        static const std::string dflt;
+        if (dgroups.count(name)) return name;
+        if (dgroups.count(dflt)) return dflt;
+        throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", name.c_str()));
+    }
+    const uint8_t* group_ptr(const std::string& name) const {
+        if (name == "all") return nullptr;
+        const std::string& k = group_key(name);
+        auto m = gmissing.find(k);
+        if (m != gmissing.end())
+            throw Err(MDT_EINVAL, fmt("group '%s': %s", name.c_str(), m->second.c_str()));
+        return dgroups.at(k).p;
+    }
+    double group_count(const std::string& name, int walker) const {
+        if (name == "all") return (double)w[walker].n_local;
+        group_ptr(name);
+        return gcount.at(group_key(name))[walker];
     }
     WalkerIn& walker(int k) {
         if (k < 0 || k >= (int)w.size()) throw Err(MDT_EINVAL, fmt("walker %d out of range [0,%zu)", k, w.size()));
@@ -300,10 +327,12 @@ void mdt_ctx::commit() {
     total_local = tl; total_ghost = tg; total_nnz = tn;
     std::vector<double> hx((size_t)(tl + tg) * 3);
     std::vector<int32_t> hoff((size_t)tl + 1), hidx((size_t)tn), hown((size_t)std::max<int64_t>(tg, 1));
-    std::vector<uint8_t> hgroup;
-    any_group = false;
-    for (auto& a : w) any_group |= !a.group.empty();
-    if (any_group) hgroup.assign((size_t)tl, 1);
+    std::map<std::string, std::vector<uint8_t>> hgroups;
+    for (auto& a : w)
+        for (auto& kv : a.groups) hgroups[kv.first].assign((size_t)tl, 1);
+    any_group = !hgroups.empty();
+    gcount.clear();
+    gmissing.clear();
     h_tiles.clear();
     tile_begin.assign(w.size() + 1, 0);
     for (size_t k = 0; k < w.size(); ++k) {
@@ -322,7 +351,22 @@ void mdt_ctx::commit() {
             if (o < 0 || o >= a.n_local) throw Err(MDT_EINVAL, fmt("walker %zu: ghost %lld has owner %d outside [0,n_local)", k, (long long)g, o));
             hown[a.ghost_off + g] = (int32_t)(a.atom_off + o);
         }
-        if (!a.group.empty()) std::memcpy(&hgroup[a.atom_off], a.group.data(), a.n_local);
+        for (auto& kv : hgroups) {
+            auto it = a.groups.find(kv.first);
+            if (it == a.groups.end()) it = a.groups.find(std::string());
+            std::vector<double>& cnt = gcount[kv.first];
+            cnt.resize(w.size(), 0.0);
+            if (it != a.groups.end()) {
+                std::memcpy(&kv.second[a.atom_off], it->second.data(), a.n_local);
+                double c = 0;
+                for (auto v : it->second) c += v ? 1 : 0;
+                cnt[k] = c;
+            } else {
+                cnt[k] = (double)a.n_local;
+                if (!gmissing.count(kv.first))
+                    gmissing[kv.first] = fmt("walker %zu has no membership bound for it (nor a default group)", k);
+            }
+        }
         a.tile_off = (int32_t)h_tiles.size();
         for (int64_t s = 0; s < a.n_local; s += kTile) {
             Tile t{};
@@ -345,8 +389,12 @@ void mdt_ctx::commit() {
     HIPCHK(launch_pack_x4(x.p, x4.p, x4f.p, tl + tg, stream));
     eown.alloc(std::max<size_t>(hidx.size(), 1));
     revpos.alloc(std::max<size_t>(hidx.size(), 1));
-    if (any_group) { group.alloc(hgroup.size()); group.upload(hgroup.data(), hgroup.size(), stream); }
-    else group.release();
+    dgroups.clear();
+    for (auto& kv : hgroups) {
+        DBuf<uint8_t>& d = dgroups[kv.first];
+        d.alloc(kv.second.size());
+        d.upload(kv.second.data(), kv.second.size(), stream);
+    }
     tiles.alloc(std::max<size_t>(h_tiles.size(), 1)); tiles.upload(h_tiles.data(), h_tiles.size(), stream);
     f.alloc((size_t)std::max<int64_t>(tl, 1) * 3);
     HIPCHK(launch_fill(f.p, tl * 3, 0.0, stream));
@@ -463,7 +511,7 @@ struct mdt_fix {
     uint64_t built_gen = 0;
     uint64_t builds = 0;           // fix_build calls (device buffers may move)
     int nslot_sum = 0;
-    DBuf<double> partial, direct, grid, hills, state, wcv;
+    DBuf<double> partial, direct, grid, hills, state, wcv, cv_inv;
     DBuf<int64_t> nhills, step;
     DBuf<int32_t> tile_begin, mlp_ok;
     DBuf<ExprProg> progs;
@@ -973,6 +1021,7 @@ void fix_build(mdt_fix* fx) {
     const std::vector<ExprProg>& progs = fx->h_progs;
     // slots
     int nsum = 0;
+    std::vector<int> walias(kMaxSlotCV, -1);
     for (size_t c = 0; c < fx->cvs.size(); ++c) fx->cvs[c]->slot = (int)c;
     for (auto* c : fx->cvs) c->biased = false;
     for (size_t q = 0; q < fx->syms.size(); ++q) {
@@ -981,14 +1030,11 @@ void fix_build(mdt_fix* fx) {
         if (used) fx->cvs[fx->syms[q].cv]->biased = true;
     }
     // group counts per walker
-    std::vector<double> ng_all(nw), ng_grp(nw);
-    for (int k = 0; k < nw; ++k) {
-        WalkerIn& a = ctx->w[k];
-        ng_all[k] = (double)a.n_local;
-        double g = 0;
-        if (!a.group.empty()) for (auto v : a.group) g += v ? 1 : 0; else g = (double)a.n_local;
-        ng_grp[k] = g;
-    }
+    auto group_counts = [&](const std::string& name) {
+        std::vector<double> v(nw);
+        for (int k = 0; k < nw; ++k) v[k] = ctx->group_count(name, k);
+        return v;
+    };
     // environments
     for (auto* e : fx->envs) {
         const int lm = e->lmask;
@@ -998,12 +1044,8 @@ void fix_build(mdt_fix* fx) {
         e->nsel.alloc(NT); e->cnt.alloc(NT); e->selmask.alloc((size_t)NT * 4);
         e->inv_ng.alloc(nw);
         std::vector<double> inv(nw);
-        const bool all = e->group == "all";
-        if (!all && !ctx->any_group) throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", e->group.c_str()));
-        for (int k = 0; k < nw; ++k) {
-            double ng = all ? ng_all[k] : ng_grp[k];
-            inv[k] = ng > 0 ? 1.0 / ng : 0.0;
-        }
+        const std::vector<double> ngs = group_counts(e->group);
+        for (int k = 0; k < nw; ++k) inv[k] = ngs[k] > 0 ? 1.0 / ngs[k] : 0.0;
         e->inv_ng.upload(inv.data(), nw, st);
         EnvDev& d = e->dev;
         d = EnvDev{};
@@ -1016,12 +1058,16 @@ void fix_build(mdt_fix* fx) {
             if (!(lm & bit)) continue;
             d.slot_ql[li] = nsum++;
             d.wslot[li] = -1;
+            // CALs with identical definitions share this degree: its force pass takes the
+            // sum of their weights, kept at the first one's combined slot
+            int rep_slot = -1;
             for (auto* c : fx->cvs)
                 if (c->kind == CV_Q && fx->envs[c->env] == e && c->l == l) {
                     c->psum = d.slot_ql[li];
                     if (c->biased) {
-                        if (d.wslot[li] >= 0) throw Err(MDT_ENOTSUP, "two biased CALs with identical STEINH Q definitions");
-                        d.wslot[li] = c->slot;
+                        if (rep_slot < 0) rep_slot = c->slot;
+                        walias[c->slot] = rep_slot;
+                        d.wslot[li] = kMaxSlotCV + rep_slot;
                     }
                 }
             li++;
@@ -1096,24 +1142,15 @@ void fix_build(mdt_fix* fx) {
             c->sterm.alloc((size_t)nw * nk);
             c->kpack.alloc((size_t)nw * nk);
         } else if (c->kind == CV_SF) {
-            const bool all = c->sf_group == "all";
-            if (!all && !ctx->any_group)
-                throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", c->sf_group.c_str()));
+            const std::vector<double> ngs = group_counts(c->sf_group);
             std::vector<double> inv(nw);
-            for (int k = 0; k < nw; ++k) {
-                const double ng = all ? ng_all[k] : ng_grp[k];
-                inv[k] = ng > 0 ? 1.0 / ng : 0.0;
-            }
+            for (int k = 0; k < nw; ++k) inv[k] = ngs[k] > 0 ? 1.0 / ngs[k] : 0.0;
             c->sf_inv.alloc(nw); c->sf_inv.upload(inv.data(), nw, st);
             c->sf_S.alloc((size_t)std::max<int64_t>(NT, 1));
             c->sf_part.alloc((size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
             c->sf_tb.alloc(nw + 1); c->sf_tb.upload(ctx->tile_begin.data(), nw + 1, st);
         } else if (c->kind == CV_LQ) {
-            const bool all = c->lq_group == "all";
-            if (!all && !ctx->any_group)
-                throw Err(MDT_EINVAL, fmt("group '%s' used but no group membership was bound (mdt_bind_group)", c->lq_group.c_str()));
-            std::vector<double> ng(nw);
-            for (int k = 0; k < nw; ++k) ng[k] = all ? ng_all[k] : ng_grp[k];
+            const std::vector<double> ng = group_counts(c->lq_group);
             const size_t LS = 2 * (size_t)(c->l + 1), n1 = (size_t)std::max<int64_t>(NT, 1);
             c->lq_ng.alloc(nw); c->lq_ng.upload(ng.data(), nw, st);
             c->lq_S.alloc(n1); c->lq_ql.alloc(n1);
@@ -1168,8 +1205,19 @@ void fix_build(mdt_fix* fx) {
     HIPCHK(launch_fill(fx->direct.p, (int64_t)nw * kMaxSlotCV, 0.0, st));
     fx->state.alloc((size_t)nw * kStStride);
     HIPCHK(launch_fill(fx->state.p, (int64_t)nw * kStStride, 0.0, st));
-    fx->wcv.alloc((size_t)nw * kMaxSlotCV);
-    HIPCHK(launch_fill(fx->wcv.p, (int64_t)nw * kMaxSlotCV, 0.0, st));
+    fx->wcv.alloc((size_t)nw * kWcvStride);
+    HIPCHK(launch_fill(fx->wcv.p, (int64_t)nw * kWcvStride, 0.0, st));
+    // 1 / group count per walker of every CV whose value is a sum over its group
+    {
+        std::vector<double> h((size_t)nw * kMaxSlotCV, 0.0);
+        for (auto* c : fx->cvs) {
+            if (c->kind != CV_Q && c->kind != CV_LD) continue;
+            const std::vector<double> ng = group_counts(fx->envs[c->env]->group);
+            for (int k = 0; k < nw; ++k) h[(size_t)k * kMaxSlotCV + c->slot] = ng[k] > 0 ? 1.0 / ng[k] : 0.0;
+        }
+        fx->cv_inv.alloc(h.size());
+        fx->cv_inv.upload(h.data(), h.size(), st);
+    }
     fx->tile_begin.alloc(nw + 1); fx->tile_begin.upload(ctx->tile_begin.data(), nw + 1, st);
     fx->progs.alloc(progs.size()); fx->progs.upload(progs.data(), progs.size(), st);
     // affine DIM programs as coefficients per value slot: s_d = lin0_d + sum_c A[d][c] v_c,
@@ -1230,9 +1278,8 @@ void fix_build(mdt_fix* fx) {
     // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
     for (size_t c = 0; c < fx->cvs.size(); ++c)
         r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST || fx->cvs[c]->kind == CV_SF || fx->cvs[c]->kind == CV_LQ) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
-    r.inv_ng = fx->envs.empty() ? nullptr : fx->envs[0]->inv_ng.p;
-    for (auto* e : fx->envs)
-        if (e->group != fx->envs[0]->group) throw Err(MDT_ENOTSUP, "all STEINH CALs must use the same group");
+    r.cv_inv = fx->cv_inv.p;
+    for (int c = 0; c < kMaxSlotCV; ++c) r.walias[c] = c < (int)fx->cvs.size() ? (walias[c] >= 0 ? walias[c] : c) : -1;
     r.nsym = (int)fx->syms.size();
     for (size_t q = 0; q < fx->syms.size(); ++q) r.sym_slot[q] = fx->cvs[fx->syms[q].cv]->slot;
     r.D = fx->D();
@@ -1565,14 +1612,14 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         if (ctx->w[k].x_src) T.run("k_pack_walker", st, [&] { ctx->pack((int)k, ctx->w[k].x_src); });
     for (auto* e : fx->envs) {
         Geo ge = g;
-        ge.group = e->group == "all" ? nullptr : ctx->group.p;
+        ge.group = ctx->group_ptr(e->group);
         T.run("k_env", st, [&] { HIPCHK(launch_env(ge, e->dev, st)); });
     }
     for (auto* c : fx->cvs) {
         if (c->kind == CV_LD && !c->fused) {
             EnvH* e = fx->envs[c->env];
             Geo ge = g;
-            ge.group = e->group == "all" ? nullptr : ctx->group.p;
+            ge.group = ctx->group_ptr(e->group);
             LdDev d{};
             d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = c->biased ? c->slot : -1;
             T.run("k_ld", st, [&] { HIPCHK(launch_ld(ge, e->dev, d, st)); });
@@ -1591,12 +1638,12 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         } else if (c->kind == CV_SF) {
             const SfDev d = sf_dev(fx, c);
             Geo gs = g;
-            gs.group = c->sf_group == "all" ? nullptr : ctx->group.p;
+            gs.group = ctx->group_ptr(c->sf_group);
             T.run("k_sf_value", st, [&] { HIPCHK(launch_sf_value(gs, d, st)); });
         } else if (c->kind == CV_LQ) {
             const LqDev d = lq_dev(fx, c);
             Geo gs = g;
-            gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
+            gs.group = ctx->group_ptr(c->lq_group);
             T.run("k_lq_q", st, [&] { HIPCHK(launch_lq(0, gs, d, nullptr, st)); });
             T.run("k_lq_lq", st, [&] { HIPCHK(launch_lq(1, gs, d, nullptr, st)); });
             T.run("k_lq_sum", st, [&] { HIPCHK(launch_lq(2, gs, d, nullptr, st)); });
@@ -1629,13 +1676,13 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         if (c->kind == CV_SF && c->biased) {
             const SfDev d = sf_dev(fx, c);
             Geo gs = g;
-            gs.group = c->sf_group == "all" ? nullptr : ctx->group.p;
+            gs.group = ctx->group_ptr(c->sf_group);
             T.run("k_sf_force", st, [&] { HIPCHK(launch_sf_force(gs, d, f, st)); });
         }
         if (c->kind == CV_LQ && c->biased) {
             const LqDev d = lq_dev(fx, c);
             Geo gs = g;
-            gs.group = c->lq_group == "all" ? nullptr : ctx->group.p;
+            gs.group = ctx->group_ptr(c->lq_group);
             T.run("k_lq_k", st, [&] { HIPCHK(launch_lq(3, gs, d, f, st)); });
             T.run("k_lq_h", st, [&] { HIPCHK(launch_lq(4, gs, d, f, st)); });
             T.run("k_lq_pair", st, [&] { HIPCHK(launch_lq(5, gs, d, f, st)); });
@@ -1807,8 +1854,20 @@ int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group) {
     API_BEGIN(ctx)
     WalkerIn& a = ctx->walker(walker);
     if (!a.has_x) throw Err(MDT_ESTATE, "bind positions before the group");
-    if (in_group) a.group.assign(in_group, in_group + a.n_local);
-    else a.group.clear();
+    if (in_group) a.groups[std::string()].assign(in_group, in_group + a.n_local);
+    else a.groups.erase(std::string());
+    ctx->committed = false;
+    API_END
+}
+
+int mdt_bind_group_named(mdt_ctx* ctx, int walker, const char* name, const uint8_t* in_group) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "bind positions before the group");
+    const std::string nm = name ? name : "";
+    if (nm == "all") throw Err(MDT_EINVAL, "group 'all' is every local atom and cannot be bound");
+    if (in_group) a.groups[nm].assign(in_group, in_group + a.n_local);
+    else a.groups.erase(nm);
     ctx->committed = false;
     API_END
 }
@@ -2102,7 +2161,7 @@ int mdt_fix_cv_weight(mdt_fix* fx, int walker, const char* cal_func, double* val
     int c = find_cv(fx, cal_func);
     state_of(fx, walker);
     double w = 0;
-    HIPCHK(hipMemcpy(&w, fx->wcv.p + (size_t)walker * kMaxSlotCV + fx->cvs[c]->slot, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&w, fx->wcv.p + (size_t)walker * kWcvStride + fx->cvs[c]->slot, sizeof(double), hipMemcpyDeviceToHost));
     *value = w;
     API_END
 }
@@ -2182,7 +2241,7 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
     DBuf<int32_t> si, cr;
     si.alloc((size_t)std::max<int64_t>(NT, 1) * N); cr.alloc(std::max<int64_t>(NT, 1));
     Geo g = ctx->geo();
-    g.group = e->group == "all" ? nullptr : ctx->group.p;
+    g.group = ctx->group_ptr(e->group);
     EnvDev d = e->dev;
     d.sel_out = si.p; d.sel_cnt = cr.p;
     HIPCHK(launch_select_export(g, d, ctx->stream));

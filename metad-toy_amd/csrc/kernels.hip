@@ -737,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_ld_coeff(Geo g, EnvDev e, LdDev d) {
     const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
     const bool valid = a < tl.count;
     const int i = tl.start + (valid ? a : 0);
-    const double w = e.wcv[(int64_t)tl.walker * kMaxSlotCV + d.wslot];
+    const double w = e.wcv[(int64_t)tl.walker * kWcvStride + d.wslot];
     const ulonglong2 mr = reinterpret_cast<const ulonglong2*>(e.selmask)[2 * i + 1];
     const int beg = g.off[i];
     M128 rcm{valid ? mr.x : 0ull, valid ? mr.y : 0ull};
@@ -861,7 +861,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     float wl[NL];
     #pragma unroll
     for (int k = 0; k < NL; ++k)
-        wl[k] = e.wslot[k] >= 0 ? (float)e.wcv[(int64_t)tl.walker * kMaxSlotCV + e.wslot[k]] : 0.0f;
+        wl[k] = e.wslot[k] >= 0 ? (float)e.wcv[(int64_t)tl.walker * kWcvStride + e.wslot[k]] : 0.0f;
     // The atom's own data is loaded once per group, each lane one share, and
     // broadcast through DPP: the four mask words (own lo/hi, rc lo/hi), the
     // record, the packed position.
@@ -1440,12 +1440,21 @@ __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, 
     for (int d = 0; d < kMaxDim; ++d) dv[d] = readlane_f64(dV[d], 0);
     double* st = r.state + (int64_t)w * kStStride;
     if (c < r.ncv) st[kStCv + c] = cvv;
+    double wc = 0.0;
     if (c < kMaxSlotCV) {
-        double wc = 0.0;
         #pragma unroll
         for (int d = 0; d < kMaxDim; ++d) if (d < r.D) wc += dv[d] * ac[d];
-        r.wcv[(int64_t)w * kMaxSlotCV + c] = wc;
+        r.wcv[(int64_t)w * kWcvStride + c] = wc;
     }
+    // the weight sum of each representative slot over the slots aliased to it, in slot
+    // order (a slot that nothing aliases gets its own weight)
+    double comb = 0.0;
+    #pragma unroll
+    for (int q = 0; q < kMaxSlotCV; ++q) {
+        const double v = __shfl(wc, q, 64);
+        if (r.walias[q] == c) comb += v;
+    }
+    if (c < kMaxSlotCV) r.wcv[(int64_t)w * kWcvStride + kMaxSlotCV + c] = comb;
     if (c == 0) {
         #pragma unroll
         for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { st[kStS + d] = sv[d]; st[kStDV + d] = dv[d]; }
@@ -1493,7 +1502,6 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     const int t1 = r.tile_args ? r.tile_range[w + 1] : r.tile_begin[w + 1];
     const int64_t step = r.step[w];
     const int64_t nhill0 = r.nhills[w];
-    const double inv_ng = r.inv_ng != nullptr ? r.inv_ng[w] : 0.0;   // NULL without a Steinhardt environment
     const double mlp_err = r.kind == BIAS_MLP ? r.mlp_err[w] : 0.0;
     const int mlp_ok = r.kind == BIAS_MLP ? r.mlp_ok[w] : 0;
     // wave 0, lane c: CV slot c's source, direct value and affine coefficients
@@ -1575,7 +1583,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
                 double a = 0.0;
                 #pragma unroll
                 for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
-                cvv = a * inv_ng;
+                cvv = a * r.cv_inv[(int64_t)w * kMaxSlotCV + c];
             } else {
                 // direct values; CVs summed after the force pass (cv_src <= -2) are
                 // unbiased and reduced when the state is read
@@ -1683,7 +1691,7 @@ __global__ __launch_bounds__(1024) void k_cv_tail(ReduceDev r) {
             v[0] += (x[0] + x[1]) + (x[2] + x[3]);
         }
         block_sum<1, NT / 64>(v, sh);
-        if (threadIdx.x == 0) r.state[(int64_t)w * kStStride + kStCv + c] = v[0] * r.inv_ng[w];
+        if (threadIdx.x == 0) r.state[(int64_t)w * kStStride + kStCv + c] = v[0] * r.cv_inv[(int64_t)w * kMaxSlotCV + c];
         __syncthreads();
     }
 }
@@ -1936,7 +1944,7 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
     const int k = tl.start + a;
     // AVE: pair weight ([k in G] + [j in G]) / N_group; COUNT: [k in G] sw'(S_k) + [j in G] sw'(S_j)
     // (structure_factor_dcv_COUNT_kernel :1203-1271 for k in G)
-    const double wc = d.wcv[(int64_t)tl.walker * kMaxSlotCV + d.slot];
+    const double wc = d.wcv[(int64_t)tl.walker * kWcvStride + d.slot];
     const double W = d.count ? wc : wc * d.inv_ng[tl.walker];
     const double gk = (g.group == nullptr || g.group[k]) ? (d.count ? sw_df(d.sw, d.S[k]) : 1.0) : 0.0;
     const double4 p = g.x4[k];
@@ -2201,7 +2209,7 @@ __global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
     const double ql = d.ql[i];
     double c = 0.0;
     if (valid && (g.group == nullptr || g.group[i]) && d.S[i] > 0.0 && ql > 0.0) {
-        const double W = d.wcv[(int64_t)w * kMaxSlotCV + d.slot];
+        const double W = d.wcv[(int64_t)w * kWcvStride + d.slot];
         const double sf = d.sums[2 * w], cv = d.sums[2 * w + 1], N = d.ng[w];
         const double f = sw_f(d.swcv, ql), fp = sw_df(d.swcv, ql);
         const double wi = d.mode == 0 ? (sf != 0.0 ? (f + fp * (ql - cv)) / sf : 0.0)
@@ -2374,7 +2382,7 @@ __global__ void k_dist_force(DistDev d, double* __restrict__ f) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= d.n_walkers) return;
     const double* o = d.dvec + 4 * w;
-    const double dV = d.wcv[(int64_t)w * kMaxSlotCV + d.slot];
+    const double dV = d.wcv[(int64_t)w * kWcvStride + d.slot];
     double* f1 = f + 3 * (d.atom_off[w] + d.a1);
     double* f2 = f + 3 * (d.atom_off[w] + d.a2);
     #pragma unroll
@@ -2529,7 +2537,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
     const int a0 = blockIdx.x * blockDim.x * kSkAtoms + threadIdx.x;
     const int n = sk.n_atoms[w];
     if ((int)(blockIdx.x * blockDim.x * kSkAtoms) >= n) return;
-    const double wt = sk.wcv[(int64_t)w * kMaxSlotCV + sk.wslot];
+    const double wt = sk.wcv[(int64_t)w * kWcvStride + sk.wslot];
     const float scale = (float)(wt * 2.0 / ((double)n * sk.nk));
     int ii[kSkAtoms];
     double px[kSkAtoms], py[kSkAtoms], pz[kSkAtoms];

@@ -48,6 +48,7 @@ def lib():
         "mdt_bind_positions": (I, [P, I, P, I64, I64, I]),
         "mdt_bind_neighbors": (I, [P, I, P, P, I64, P, I]),
         "mdt_bind_group": (I, [P, I, u8p]),
+        "mdt_bind_group_named": (I, [P, I, C.c_char_p, u8p]),
         "mdt_bind_system": (I, [P, I, P]),
         "mdt_update_positions": (I, [P, I, P, I]),
         "mdt_attach_positions": (I, [P, I, P]),
@@ -220,6 +221,11 @@ class Context:
         m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
         self._chk(lib().mdt_bind_group(self._h, walker,
                                        None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def bind_group_named(self, walker, name, mask):
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        self._chk(lib().mdt_bind_group_named(self._h, walker, name.encode(),
+                                             None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8))))
 
     def bind_system(self, walker, system):
         self._chk(lib().mdt_bind_system(self._h, walker, system._h))

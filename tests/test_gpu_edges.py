@@ -158,3 +158,62 @@ def test_walkers_from_tiny_to_large(mdt, orc):
         F_ref = -env.dQ_dx(6, fix.cv_weight("Q6.AVE", w))
         assert rel_force_err(f[o:o + n], F_ref) <= F_TOL, w
         o += n
+
+
+@pytest.mark.parametrize("q4_group", ["upper", "all"])
+def test_cals_on_different_groups(mdt, orc, q4_group):
+    """Q6 on group 'lower' and Q4 on another group (its complement, or all), both biased
+    through v1+v2: each CV averages over its own group (its own count) and its force acts on
+    its own group's atoms, as the reference pools one environment per group
+    (CVs_Steinhardt.cu:191-220) and applies each CV's bias to its group (:841-868)."""
+    s = fcc_system(mdt, 8, sigma=0.15, seed=43)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    lower = (x[:n, 2] < x[:n, 2].mean()).astype(np.uint8)
+    upper = (1 - lower).astype(np.uint8)
+    g4 = upper if q4_group == "upper" else None
+    env6 = orc.Env(x, off, idx, own, n, 3.9, 18, group=lower)
+    env4 = orc.Env(x, off, idx, own, n, 3.9, 18, group=g4)
+    q6, q4 = env6.q_ave(6), env4.q_ave(4)
+    print(f"[groups] n={n} lower={int(lower.sum())} Q6(lower)={q6:.6f} Q4({q4_group})={q4:.6f}", flush=True)
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    ctx.bind_group_named(0, "lower", lower)
+    ctx.bind_group_named(0, "upper", upper)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                       "CAL NAME Q6 STEINH Q 6 lower cutoff_r 3.9 cutoff_Natoms 18 "
+                       f"CAL NAME Q4 STEINH Q 4 {q4_group} cutoff_r 3.9 cutoff_Natoms 18 "
+                       "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE DIM 1 0 2 2000 v1+v2 WT 0")
+    f = _biased_step(fix, q6 + q4 + 0.004, n)
+    assert abs(fix.cv("Q6.AVE") - q6) <= CV_TOL * q6
+    assert abs(fix.cv("Q4.AVE") - q4) <= CV_TOL * q4
+    w = fix.cv_weight("Q6.AVE")
+    assert w != 0.0 and w == fix.cv_weight("Q4.AVE")
+    m6 = lower.astype(bool)[:, None]
+    m4 = np.ones((n, 1), bool) if g4 is None else upper.astype(bool)[:, None]
+    F_ref = -(np.where(m6, env6.dQ_dx(6, w), 0.0) + np.where(m4, env4.dQ_dx(4, w), 0.0))
+    assert rel_force_err(f, F_ref) <= F_TOL
+    fix.close()
+    ctx.close()
+
+
+def test_two_biased_cals_with_one_definition(mdt, orc):
+    """Two CALs with identical STEINH Q 6 definitions, both biased (DIM v1+2*v2): one shared
+    environment degree whose force pass takes the sum of the two weights."""
+    s = fcc_system(mdt, 7, sigma=0.2, seed=44)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    env = orc.Env(x, off, idx, own, n, 3.9, 18)
+    q6 = env.q_ave(6)
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                            "CAL NAME A STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                            "CAL NAME B STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                            "CV_dim 1 SYMBOL v1 A.AVE SYMBOL v2 B.AVE DIM 1 0 3 3000 v1+2*v2 WT 0")
+    f = _biased_step(fix, 3 * q6 + 0.006, n)
+    assert fix.cv("A.AVE") == fix.cv("B.AVE")
+    assert abs(fix.cv("A.AVE") - q6) <= CV_TOL * q6
+    wa, wb = fix.cv_weight("A.AVE"), fix.cv_weight("B.AVE")
+    assert wa != 0.0 and abs(wb - 2.0 * wa) <= 1e-12 * abs(wb)
+    assert rel_force_err(f, -env.dQ_dx(6, wa + wb)) <= F_TOL
+    fix.close()
+    ctx.close()
