@@ -77,7 +77,10 @@ int mdt_bind_positions(mdt_ctx* ctx, int walker, const double* x, int64_t n_loca
 /* Full neighbour list of every local atom: offsets[n_center+1], idx[offsets[n]],
  * entries in [0, n_all); ghost_owner[n_all-n_local] maps each ghost image to
  * its local atom (LAMMPS: atom->map(tag[j])).  LAMMPS NEIGHMASK bits must
- * already be stripped.  Each list may hold at most 128 entries. */
+ * already be stripped.  Lists may be of any length: when one is longer than 128
+ * entries, the N-nearest selection walks near lists cut just above its largest
+ * cutoff (list order kept, rebuilt on the device when atoms move), and at most
+ * 128 entries may lie within that cutoff. */
 int mdt_bind_neighbors(mdt_ctx* ctx, int walker, const int32_t* offsets, const int32_t* idx,
                        int64_t n_center, const int32_t* ghost_owner, int on_device);
 

@@ -9,7 +9,10 @@ namespace mdt {
 constexpr int kBlock = 256;        // threads per workgroup for per-atom kernels
 constexpr int kLanes = 4;          // lanes per atom in the per-atom kernels
 constexpr int kTile = kBlock / kLanes;   // atoms per tile (one workgroup)
-constexpr int kMaxList = 128;      // max neighbour-list length per atom (2 x 64-bit masks)
+constexpr int kMaxList = 128;      // max near-list length per atom (2 x 64-bit masks)
+// candidates with r^2 <= rc^2 (1 + kRcSlack) in an atom's frame: a superset of every
+// pair that either side of a pair can select (the two frames differ by rounding)
+constexpr double kRcSlack = 1e-9;
 constexpr int kMaxSlots = 64;      // max cutoff_Natoms
 constexpr int kMaxSlotCV = 16;     // CV value slots per walker
 // per-walker chain-rule weights: [0, kMaxSlotCV) dV/dv of each CV slot; from kMaxSlotCV
@@ -74,13 +77,34 @@ struct Geo {
     const int32_t* off;       // n_local_total + 1 (global CSR offsets)
     const int32_t* idx;       // global all-space indices
     const int32_t* owner;     // ghost -> local index (indexed by j - n_local_total)
-    const int32_t* eown;      // per CSR entry: local owner of idx[e]
-    const uint8_t* revpos;    // per CSR entry: position of the reverse entry (the image of this
+    const int32_t* eown;      // per list entry: local owner of idx[e]
+    const uint8_t* revpos;    // per list entry: position of the reverse entry (the image of this
                               // center in the owner's list), 0xFF = none
+    const int32_t* cnt;       // per local atom: list length (the near list); NULL: off[i+1] - off[i]
     const uint8_t* group;     // per local atom, NULL = all
     const Tile* tiles;
     int32_t ntiles;
     int32_t n_local_total;
+};
+
+// The near list: every bound list cut to the entries within a radius Rp above the largest
+// Steinhardt cutoff, in list order (so the N-nearest insertion, which never keeps an entry
+// beyond rc, selects the same entries in the same order), stored in place at the bound
+// list's offsets with a length per atom.  It is rebuilt inside the step whenever some atom
+// has moved more than a threshold since the last build (k_pack_walker raises the flag of
+// the current epoch; the reduce kernel advances the epoch once per step).
+struct NearDev {
+    const int32_t* off;       // bound CSR offsets
+    const int32_t* idx;       // bound entries
+    int32_t* nidx;            // near entries at off[i] .. off[i] + ncnt[i]
+    int32_t* ncnt;            // per local atom
+    float4* xpr;              // positions at the last build (packed layout: locals, ghosts)
+    int32_t* flag;            // [2] rebuild flags, by epoch parity
+    int64_t* epoch;           // step epoch counter
+    int32_t* overflow;        // set when an atom has more than kMaxList near entries
+    double rp2;               // Rp^2
+    float move2;              // squared displacement that forces a rebuild
+    int32_t n_local, n_all;
 };
 
 // One N-nearest environment (cutoff, cap) and the Steinhardt degrees on it.
@@ -212,6 +236,7 @@ struct ReduceDev {
     int64_t* step;            // per walker
     double* state;            // per walker [kStateStride]
     double* wcv;              // per walker [kWcvStride]
+    int64_t* epoch;           // near-list epoch, advanced once per step (NULL: no near list)
     // learned bias
     const MlpNet* mlp;        // per walker live network, NULL unless kind == BIAS_MLP
     const int32_t* mlp_ok;    // per walker: 1 = network trusted, 0 = use exact sum
@@ -286,8 +311,8 @@ struct LqDev {
 };
 // stage 0 k_lq_q, 1 k_lq_lq, 2 k_lq_sum (values); 3 k_lq_k, 4 k_lq_h, 5 k_lq_pair, 6 k_lq_force (forces)
 hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStream_t st);
-hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, int32_t* eidx, int32_t* eown,
-                          hipStream_t st);
+hipError_t launch_lq_erev(const Geo& g, int64_t n, const int32_t* reown, const int32_t* rrev, const int64_t* apos,
+                          int64_t* erev, int32_t* eidx, int32_t* eown, hipStream_t st);
 
 // DISTANCE CV between two atoms of each walker (one thread per walker).
 struct DistDev {
@@ -326,11 +351,24 @@ hipError_t nl_sort_pairs(void* tmp, size_t& bytes, const int32_t* kin, int32_t* 
                          int32_t* vout, int64_t n, int end_bit, hipStream_t st);
 
 // ---- launchers (kernels.hip) ----
-hipError_t launch_topology(const Geo& g, int64_t n_local_total, int32_t* eown, uint8_t* revpos,
-                           hipStream_t st);
+hipError_t launch_prune(const Geo& g, const NearDev& nd, int force, hipStream_t st);
+// owners and reverse positions of the near lists (built when the epoch's flag is up, or forced)
+hipError_t launch_topology_near(const Geo& g, const NearDev& nd, int32_t* eown, uint8_t* revpos, int force,
+                                hipStream_t st);
+// owners and reverse positions of the bound lists (at most kMaxList entries each)
+hipError_t launch_topology(const Geo& g, int64_t n, int32_t* eown, uint8_t* revpos, hipStream_t st);
+// the topology of the bound lists with int32 reverse positions (-1 = none), for the
+// kernels that run on the bound lists (STEINH L)
+hipError_t launch_topology_raw(const Geo& g, int64_t n, int32_t* eown, int32_t* revpos, hipStream_t st);
+
 hipError_t launch_pack_x4(const double* x, double4* x4, float4* x4f, int64_t n, hipStream_t st);
 hipError_t launch_pack_walker(const double* x, int64_t n_local, int64_t n_all, double4* x4_loc, float4* x4f_loc,
                               double4* x4_gh, float4* x4f_gh, hipStream_t st);
+// the same, comparing with the positions of the last near-list build (xpr at the same
+// offsets as x4f) and raising the rebuild flag of the current epoch
+hipError_t launch_pack_walker_near(const double* x, int64_t n_local, int64_t n_all, double4* x4_loc, float4* x4f_loc,
+                                   double4* x4_gh, float4* x4f_gh, const float4* xpr_loc, const float4* xpr_gh,
+                                   const NearDev& nd, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, hipStream_t st);
 hipError_t launch_env(const Geo& g, const EnvDev& e, hipStream_t st);
 // the selection export instance of k_env (e.sel_out, e.sel_cnt set)

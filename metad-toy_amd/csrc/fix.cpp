@@ -245,8 +245,20 @@ struct mdt_ctx {
     DBuf<double> x;
     DBuf<double4> x4;
     DBuf<float4> x4f;
-    DBuf<int32_t> off, idx, owner, eown;
+    DBuf<int32_t> off, idx, owner;
+    // owners and reverse positions of the lists the Steinhardt kernels walk: the bound
+    // lists when none is longer than kMaxList, else the near lists
+    DBuf<int32_t> eown;
     DBuf<uint8_t> revpos;
+    int64_t max_list = 0;       // longest bound list
+    // the near lists (see NearDev): in place at the bound offsets, the build positions,
+    // the epoch flags; built when a bound list is longer than kMaxList
+    DBuf<int32_t> nidx, ncnt, nflag, noverflow;
+    DBuf<float4> xpr;
+    DBuf<int64_t> nepoch;
+    double near_rp2 = 0.0;      // 0: no near list
+    double near_rc = 0.0;       // the slack-widened cutoff the near lists serve
+    uint64_t near_gen = 0;      // generation it was built for
     // device memberships by group name, concatenated over walkers; per-walker counts
     std::map<std::string, DBuf<uint8_t>> dgroups;
     std::map<std::string, std::vector<double>> gcount;
@@ -263,13 +275,44 @@ struct mdt_ctx {
     double nl_lo[3] = {0, 0, 0}, nl_hi[3] = {0, 0, 0};
     bool nl_ok = false;
 
+    // the bound lists (no topology): the kernels that walk whole lists
     Geo geo() const {
         Geo g{};
         g.x4 = x4.p; g.x4f = x4f.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p;
-        g.eown = eown.p; g.revpos = revpos.p;
+        g.eown = nullptr; g.revpos = nullptr; g.cnt = nullptr;
         g.group = nullptr;
         g.tiles = tiles.p; g.ntiles = (int32_t)h_tiles.size(); g.n_local_total = (int32_t)total_local;
         return g;
+    }
+    bool near_on() const { return near_rp2 > 0.0 && near_gen == generation; }
+    // the lists of the N-nearest selection and its gradient, with their topology
+    Geo geo_near() const {
+        Geo g = geo();
+        g.eown = eown.p; g.revpos = revpos.p;
+        if (near_on()) { g.idx = nidx.p; g.cnt = ncnt.p; }
+        return g;
+    }
+    NearDev near_dev() {
+        NearDev d{};
+        d.off = off.p; d.idx = idx.p; d.nidx = nidx.p; d.ncnt = ncnt.p;
+        d.xpr = xpr.p; d.flag = nflag.p; d.epoch = nepoch.p; d.overflow = noverflow.p;
+        d.rp2 = near_rp2;
+        // a pair can cross from beyond Rp into rc only when its separation changed by more
+        // than Rp - rc, i.e. when an atom moved more than half of that
+        const double half = 0.5 * (std::sqrt(near_rp2) - near_rc);
+        d.move2 = (float)(0.95 * half * half);
+        d.n_local = (int32_t)total_local; d.n_all = (int32_t)(total_local + total_ghost);
+        return d;
+    }
+    // (re)build the near lists for radius^2 rp2 when they are missing, stale or too short
+    void ensure_near(double rc2);
+    // bring the near lists up to date inside the stream (a rebuild runs only when some
+    // atom moved past the threshold since the last one)
+    void refresh_near() {
+        if (!near_on()) return;
+        const NearDev nd = near_dev();
+        HIPCHK(launch_prune(geo(), nd, 0, stream));
+        HIPCHK(launch_topology_near(geo_near(), nd, eown.p, revpos.p, 0, stream));
     }
     // device membership of a group name (nullptr for "all"); a name without a binding
     // of its own falls back to the default group
@@ -300,13 +343,18 @@ struct mdt_ctx {
     // pack walker k's positions x (n_all*3 device doubles, locals then ghosts)
     void pack(int k, const double* xd) {
         WalkerIn& a = w[k];
-        HIPCHK(launch_pack_walker(xd, a.n_local, a.n_all, x4.p + a.atom_off, x4f.p + a.atom_off,
-                                  x4.p + total_local + a.ghost_off, x4f.p + total_local + a.ghost_off, stream));
+        if (near_on())
+            HIPCHK(launch_pack_walker_near(xd, a.n_local, a.n_all, x4.p + a.atom_off, x4f.p + a.atom_off,
+                                           x4.p + total_local + a.ghost_off, x4f.p + total_local + a.ghost_off,
+                                           xpr.p + a.atom_off, xpr.p + total_local + a.ghost_off, near_dev(), stream));
+        else
+            HIPCHK(launch_pack_walker(xd, a.n_local, a.n_all, x4.p + a.atom_off, x4f.p + a.atom_off,
+                                      x4.p + total_local + a.ghost_off, x4f.p + total_local + a.ghost_off, stream));
     }
 };
 
 void mdt_ctx::commit() {
-    int64_t tl = 0, tg = 0, tn = 0;
+    int64_t tl = 0, tg = 0, tn = 0, ml = 0;
     for (size_t k = 0; k < w.size(); ++k) {
         WalkerIn& a = w[k];
         if (!a.has_box || !a.has_x || !a.has_nl)
@@ -315,10 +363,8 @@ void mdt_ctx::commit() {
             throw Err(MDT_EINVAL, fmt("walker %zu: neighbour list has %zu centers, expected n_local=%lld",
                                       k, a.off.size() - 1, (long long)a.n_local));
         for (int64_t i = 0; i < a.n_local; ++i) {
-            int64_t c = (int64_t)a.off[i + 1] - a.off[i];
-            if (c < 0 || c > kMaxList)
-                throw Err(MDT_ENOTSUP, fmt("walker %zu atom %lld has %lld list entries; at most %d are supported",
-                                           k, (long long)i, (long long)c, kMaxList));
+            if (a.off[i + 1] < a.off[i]) throw Err(MDT_EINVAL, fmt("walker %zu: offsets decrease at atom %lld", k, (long long)i));
+            ml = std::max<int64_t>(ml, (int64_t)a.off[i + 1] - a.off[i]);
         }
         a.atom_off = tl; a.ghost_off = tg; a.nnz_off = tn;
         tl += a.n_local; tg += a.n_all - a.n_local; tn += a.nnz;
@@ -387,8 +433,6 @@ void mdt_ctx::commit() {
     x4.alloc((size_t)(tl + tg));
     x4f.alloc((size_t)(tl + tg));
     HIPCHK(launch_pack_x4(x.p, x4.p, x4f.p, tl + tg, stream));
-    eown.alloc(std::max<size_t>(hidx.size(), 1));
-    revpos.alloc(std::max<size_t>(hidx.size(), 1));
     dgroups.clear();
     for (auto& kv : hgroups) {
         DBuf<uint8_t>& d = dgroups[kv.first];
@@ -396,12 +440,51 @@ void mdt_ctx::commit() {
         d.upload(kv.second.data(), kv.second.size(), stream);
     }
     tiles.alloc(std::max<size_t>(h_tiles.size(), 1)); tiles.upload(h_tiles.data(), h_tiles.size(), stream);
+    max_list = ml;
+    eown.alloc(std::max<size_t>(hidx.size(), 1));
+    revpos.alloc(std::max<size_t>(hidx.size(), 1));
+    if (ml <= kMaxList) HIPCHK(launch_topology(geo(), tl, eown.p, revpos.p, stream));
     f.alloc((size_t)std::max<int64_t>(tl, 1) * 3);
     HIPCHK(launch_fill(f.p, tl * 3, 0.0, stream));
-    HIPCHK(launch_topology(geo(), tl, eown.p, revpos.p, stream));
     HIPCHK(hipStreamSynchronize(stream));
     committed = true;
     generation++;
+}
+
+void mdt_ctx::ensure_near(double rc2) {
+    const double rc = std::sqrt(rc2);
+    if (near_on() && near_rc >= rc) return;
+    const size_t nl = (size_t)std::max<int64_t>(total_local, 1);
+    const size_t na = (size_t)std::max<int64_t>(total_local + total_ghost, 1);
+    nidx.alloc((size_t)std::max<int64_t>(total_nnz, 1));
+    ncnt.alloc(nl); xpr.alloc(na);
+    nflag.alloc(2); noverflow.alloc(1); nepoch.alloc(1);
+    const int32_t z2[2] = {0, 0};
+    const int64_t z64 = 0;
+    nflag.upload(z2, 2, stream);
+    nepoch.upload(&z64, 1, stream);
+    near_rc = rc;
+    near_gen = generation;
+    // the widest margin above rc (rc/8, rc/16, ...) whose lists all fit kMaxList entries:
+    // a narrower margin only means earlier rebuilds
+    for (double f = 0.125; ; f *= 0.5) {
+        const bool last = f < 1.0 / 1024.0;
+        const double rp = last ? rc : rc * (1.0 + f);
+        near_rp2 = rp * rp;
+        noverflow.upload(z2, 1, stream);
+        HIPCHK(launch_prune(geo(), near_dev(), 1, stream));
+        int32_t of = 0;
+        HIPCHK(hipMemcpyAsync(&of, noverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (!of) break;
+        if (last) {
+            near_rp2 = 0.0;
+            throw Err(MDT_ENOTSUP, fmt("an atom has more than %d list entries within the Steinhardt cutoff %.6g; "
+                                       "at most %d are supported", kMaxList, rc, kMaxList));
+        }
+    }
+    HIPCHK(launch_topology_near(geo_near(), near_dev(), eown.p, revpos.p, 1, stream));
+    HIPCHK(hipStreamSynchronize(stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -1035,6 +1118,13 @@ void fix_build(mdt_fix* fx) {
         for (int k = 0; k < nw; ++k) v[k] = ctx->group_count(name, k);
         return v;
     };
+    // lists longer than kMaxList: the Steinhardt kernels walk near lists cut just above the
+    // largest slack-widened Steinhardt cutoff
+    if (!fx->envs.empty() && ctx->max_list > kMaxList) {
+        double rc2 = 0.0;
+        for (auto* e : fx->envs) rc2 = std::max(rc2, e->rc * e->rc * (1.0 + kRcSlack));
+        ctx->ensure_near(rc2);
+    }
     // environments
     for (auto* e : fx->envs) {
         const int lm = e->lmask;
@@ -1178,7 +1268,16 @@ void fix_build(mdt_fix* fx) {
             c->lq_erev.alloc(ell1);
             c->lq_eidx.alloc(ell1);
             c->lq_eown.alloc(ell1);
-            HIPCHK(launch_lq_erev(ctx->geo(), NT, c->lq_apos.p, c->lq_erev.p, c->lq_eidx.p, c->lq_eown.p, st));
+            {
+                // owners and reverse positions on the bound lists, for the per-entry layout
+                DBuf<int32_t> reown, rrev;
+                reown.alloc((size_t)std::max<int64_t>(ctx->total_nnz, 1));
+                rrev.alloc((size_t)std::max<int64_t>(ctx->total_nnz, 1));
+                HIPCHK(launch_topology_raw(ctx->geo(), NT, reown.p, rrev.p, st));
+                HIPCHK(launch_lq_erev(ctx->geo(), NT, reown.p, rrev.p, c->lq_apos.p, c->lq_erev.p, c->lq_eidx.p,
+                                      c->lq_eown.p, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
             c->lq_part.alloc(2 * (size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
             c->lq_sums.alloc(2 * (size_t)nw);
             c->lq_tb.alloc(nw + 1); c->lq_tb.upload(ctx->tile_begin.data(), nw + 1, st);
@@ -1304,6 +1403,7 @@ void fix_build(mdt_fix* fx) {
     r.h0 = fx->height0; r.biasf = fx->biasf; r.kT = fx->temp * fx->boltz; r.wt = fx->wt ? 1 : 0;
     r.hills = fx->hills.p; r.nhills = fx->nhills.p; r.hills_cap = fx->hills_cap;
     r.step = fx->step.p; r.state = fx->state.p; r.wcv = fx->wcv.p;
+    r.epoch = ctx->near_on() ? ctx->nepoch.p : nullptr;
     r.mlp = fx->mlp_live.p; r.mlp_ok = fx->mlp_ok.p; r.mlp_err = fx->mlp_err.p;
     if (fx->hill_type == 3) {
         MlpDev& m = fx->mdev;
@@ -1606,10 +1706,17 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
     Timer& T = tm ? *tm : off_timer;
     mdt_ctx* ctx = fx->ctx;
     hipStream_t st = ctx->stream;
-    Geo g = ctx->geo();
     // positions of walkers attached to an engine buffer are read at the start of the step
     for (size_t k = 0; k < ctx->w.size(); ++k)
         if (ctx->w[k].x_src) T.run("k_pack_walker", st, [&] { ctx->pack((int)k, ctx->w[k].x_src); });
+    if (!fx->envs.empty() && ctx->near_on()) {
+        const NearDev nd = ctx->near_dev();
+        T.run("k_prune", st, [&] { HIPCHK(launch_prune(ctx->geo(), nd, 0, st)); });
+        T.run("k_topology", st, [&] {
+            HIPCHK(launch_topology_near(ctx->geo_near(), nd, ctx->eown.p, ctx->revpos.p, 0, st)); });
+    }
+    // Steinhardt selection and gradient kernels walk the near lists, the rest the bound ones
+    const Geo g = ctx->geo_near(), graw = ctx->geo();
     for (auto* e : fx->envs) {
         Geo ge = g;
         ge.group = ctx->group_ptr(e->group);
@@ -1637,12 +1744,12 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             T.run("k_dist_value", st, [&] { HIPCHK(launch_dist_value(g.x4, d, st)); });
         } else if (c->kind == CV_SF) {
             const SfDev d = sf_dev(fx, c);
-            Geo gs = g;
+            Geo gs = graw;
             gs.group = ctx->group_ptr(c->sf_group);
             T.run("k_sf_value", st, [&] { HIPCHK(launch_sf_value(gs, d, st)); });
         } else if (c->kind == CV_LQ) {
             const LqDev d = lq_dev(fx, c);
-            Geo gs = g;
+            Geo gs = graw;
             gs.group = ctx->group_ptr(c->lq_group);
             T.run("k_lq_q", st, [&] { HIPCHK(launch_lq(0, gs, d, nullptr, st)); });
             T.run("k_lq_lq", st, [&] { HIPCHK(launch_lq(1, gs, d, nullptr, st)); });
@@ -1675,13 +1782,13 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
         }
         if (c->kind == CV_SF && c->biased) {
             const SfDev d = sf_dev(fx, c);
-            Geo gs = g;
+            Geo gs = graw;
             gs.group = ctx->group_ptr(c->sf_group);
             T.run("k_sf_force", st, [&] { HIPCHK(launch_sf_force(gs, d, f, st)); });
         }
         if (c->kind == CV_LQ && c->biased) {
             const LqDev d = lq_dev(fx, c);
-            Geo gs = g;
+            Geo gs = graw;
             gs.group = ctx->group_ptr(c->lq_group);
             T.run("k_lq_k", st, [&] { HIPCHK(launch_lq(3, gs, d, f, st)); });
             T.run("k_lq_h", st, [&] { HIPCHK(launch_lq(4, gs, d, f, st)); });
@@ -1717,7 +1824,14 @@ void fetch_state(mdt_fix* fx) {
     const int nw = fx->nw();
     fx->h_state.resize((size_t)nw * kStStride);
     HIPCHK(hipMemcpyAsync(fx->h_state.data(), fx->state.p, sizeof(double) * fx->h_state.size(), hipMemcpyDeviceToHost, fx->ctx->stream));
+    int32_t of = 0;
+    if (fx->ctx->near_on())
+        HIPCHK(hipMemcpyAsync(&of, fx->ctx->noverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost, fx->ctx->stream));
     HIPCHK(hipStreamSynchronize(fx->ctx->stream));
+    if (of)
+        throw Err(MDT_ENOTSUP, fmt("an atom came to have more than %d list entries within %.6g of it; at most %d are "
+                                   "supported (the results since are not valid)", kMaxList,
+                                   std::sqrt(fx->ctx->near_rp2), kMaxList));
     fx->state_valid = true;
 }
 
@@ -2199,16 +2313,29 @@ int mdt_fix_extract_peratom(mdt_fix* fx, int walker, const char* key, double* ou
     } else if (prop == "kept") {
         // the set the last step's k_env kept (its selection mask), as the neighbour
         // indices of each atom in list order, N per atom, -1 after the last
+        // mask positions index the list the step walked: the bound list, or the near list
+        // in place at the bound offsets
         const int N = e->N;
         std::vector<uint64_t> m((size_t)n * 4);
         HIPCHK(hipMemcpy(m.data(), e->selmask.p + (size_t)o * 4, sizeof(uint64_t) * m.size(), hipMemcpyDeviceToHost));
+        std::vector<int32_t> cnt((size_t)std::max<int64_t>(n, 1)), ni((size_t)std::max<int64_t>(a.nnz, 1));
+        if (ctx->near_on()) {
+            HIPCHK(hipMemcpy(cnt.data(), ctx->ncnt.p + o, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+            if (a.nnz) HIPCHK(hipMemcpy(ni.data(), ctx->nidx.p + a.nnz_off, sizeof(int32_t) * a.nnz, hipMemcpyDeviceToHost));
+        } else {
+            for (int64_t i = 0; i < n; ++i) cnt[i] = a.off[i + 1] - a.off[i];
+            if (a.nnz) HIPCHK(hipMemcpy(ni.data(), ctx->idx.p + a.nnz_off, sizeof(int32_t) * a.nnz, hipMemcpyDeviceToHost));
+        }
         for (int64_t i = 0; i < n; ++i) {
             int k = 0;
-            const int64_t c0 = a.off[i], c = (int64_t)a.off[i + 1] - a.off[i];
-            for (int64_t p = 0; p < c && p < kMaxList; ++p)
+            const int64_t c0 = a.off[i];
+            for (int64_t p = 0; p < cnt[i]; ++p)
                 if ((m[(size_t)i * 4 + (p >> 6)] >> (p & 63)) & 1ull) {
                     if (k >= N) throw Err(MDT_EINVAL, "selection mask holds more than N entries");
-                    out[i * N + k++] = a.idx[c0 + p];
+                    int32_t j = ni[(size_t)(c0 + p)];      // context index -> the walker's own
+                    j = j < ctx->total_local ? (int32_t)(j - a.atom_off)
+                                             : (int32_t)(a.n_local + (j - ctx->total_local - a.ghost_off));
+                    out[i * N + k++] = j;
                 }
             for (; k < N; ++k) out[i * N + k] = -1;
         }
@@ -2240,7 +2367,8 @@ int mdt_fix_selection(mdt_fix* fx, int walker, const char* cal_name, int32_t* se
     // the scan and insertion of the step's k_env, run for every atom of every walker
     DBuf<int32_t> si, cr;
     si.alloc((size_t)std::max<int64_t>(NT, 1) * N); cr.alloc(std::max<int64_t>(NT, 1));
-    Geo g = ctx->geo();
+    ctx->refresh_near();                 // positions may have moved since the last step
+    Geo g = ctx->geo_near();
     g.group = ctx->group_ptr(e->group);
     EnvDev d = e->dev;
     d.sel_out = si.p; d.sel_cnt = cr.p;

@@ -235,11 +235,22 @@ __device__ __forceinline__ double pair_r2(const double4* __restrict__ x4, int j,
 // inside the list cutoff when every box edge exceeds twice the cutoff, so the
 // reverse entry is unique.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int list_len(const Geo& g, int i) {
+    return g.cnt ? g.cnt[i] : g.off[i + 1] - g.off[i];
+}
+
+// Owner and reverse position of every list entry: for entry k -> j, the local owner i
+// of j and the position in i's list of the image of k that sits at -(x_j - x_k) from
+// i (NONE when i's list holds none).  With a near-list NearDev the build runs only when
+// the current epoch's rebuild flag is up (or when forced).
+template <class RT, int NONE>
 __global__ __launch_bounds__(256) void k_topology(Geo g, int64_t n, int32_t* __restrict__ eown,
-                                                   uint8_t* __restrict__ revpos) {
+                                                   RT* __restrict__ revpos, const int32_t* __restrict__ flag,
+                                                   const int64_t* __restrict__ epoch, int force) {
+    if (flag && !force && flag[*epoch & 1] == 0) return;
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const int beg = g.off[k], end = g.off[k + 1];
+    const int beg = g.off[k], end = beg + list_len(g, (int)k);
     const double4 xk = g.x4[k];
     for (int e = beg; e < end; ++e) {
         const int j = g.idx[e];
@@ -247,9 +258,9 @@ __global__ __launch_bounds__(256) void k_topology(Geo g, int64_t n, int32_t* __r
         const double4 xj = g.x4[j];
         const double d0 = xj.x - xk.x, d1 = xj.y - xk.y, d2 = xj.z - xk.z;
         const double4 xi = g.x4[i];
-        int best = 0xFF;
+        int best = NONE;
         double bd = 1e-8;
-        const int b2 = g.off[i], e2 = g.off[i + 1];
+        const int b2 = g.off[i], e2 = b2 + list_len(g, i);
         for (int q = b2; q < e2; ++q) {
             const int t = g.idx[q];
             if (owner_of(g, t) != (int)k) continue;
@@ -259,8 +270,33 @@ __global__ __launch_bounds__(256) void k_topology(Geo g, int64_t n, int32_t* __r
             if (dd < bd) { bd = dd; best = q - b2; }
         }
         eown[e] = i;
-        revpos[e] = (uint8_t)best;
+        revpos[e] = (RT)best;
     }
+}
+
+// Near-list build: the entries of each bound list within Rp (FP64 r^2 <= Rp^2, the
+// atom itself excluded), in list order, at the list's own offsets; the build positions
+// of every atom (locals and ghosts) are kept for the displacement test.
+__global__ __launch_bounds__(256) void k_prune(Geo g, NearDev nd, int force) {
+    if (!force && nd.flag[*nd.epoch & 1] == 0) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nd.n_all) return;
+    nd.xpr[i] = g.x4f[i];
+    if (i >= nd.n_local) return;
+    const int beg = nd.off[i], end = nd.off[i + 1];
+    const double4 xi = g.x4[i];
+    int k = 0;
+    for (int e = beg; e < end; ++e) {
+        const int j = nd.idx[e];
+        const double4 xj = g.x4[j];
+        const double d0 = xj.x - xi.x, d1 = xj.y - xi.y, d2 = xj.z - xi.z;
+        const double r2 = (d0 * d0 + d1 * d1) + d2 * d2;
+        if (r2 > nd.rp2 || r2 < 1e-12) continue;
+        if (k < kMaxList) nd.nidx[beg + k] = j;
+        ++k;
+    }
+    if (k > kMaxList) { *nd.overflow = 1; k = kMaxList; }
+    nd.ncnt[i] = k;
 }
 
 // Packed FP32 position: xyz rounded to FP32, and in the fourth word the FP64
@@ -321,6 +357,30 @@ __global__ void k_pack_walker(const double* __restrict__ x, int64_t n_local, int
         (loc ? x4_loc : x4_gh)[k] = make_double4(a, b, c, 0.0);
         (loc ? x4f_loc : x4f_gh)[k] = make_float4(af, bf, cf, __uint_as_float(w));
     }
+}
+
+// The same, with the displacement test of the near list: any atom farther than
+// sqrt(move2) from its position at the last build raises the current epoch's rebuild
+// flag; the next epoch's flag is cleared here (nothing reads it during this epoch).
+__global__ void k_pack_walker_near(const double* __restrict__ x, int64_t n_local, int64_t n_all, double4* __restrict__ x4_loc,
+                                   float4* __restrict__ x4f_loc, double4* __restrict__ x4_gh, float4* __restrict__ x4f_gh,
+                                   const float4* __restrict__ xpr_loc, const float4* __restrict__ xpr_gh, NearDev nd) {
+    const int64_t e = *nd.epoch;
+    if (blockIdx.x == 0 && threadIdx.x == 0) nd.flag[(e + 1) & 1] = 0;
+    bool moved = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_all; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = x[3 * i], b = x[3 * i + 1], c = x[3 * i + 2];
+        const float af = (float)a, bf = (float)b, cf = (float)c;
+        const uint32_t w = pack_res1(a, af) | (pack_res1(b, bf) << 10) | (pack_res1(c, cf) << 20);
+        const bool loc = i < n_local;
+        const int64_t k = loc ? i : i - n_local;
+        (loc ? x4_loc : x4_gh)[k] = make_double4(a, b, c, 0.0);
+        (loc ? x4f_loc : x4f_gh)[k] = make_float4(af, bf, cf, __uint_as_float(w));
+        const float4 p = (loc ? xpr_loc : xpr_gh)[k];
+        const float d0 = af - p.x, d1 = bf - p.y, d2 = cf - p.z;
+        moved |= d0 * d0 + d1 * d1 + d2 * d2 > nd.move2;
+    }
+    if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0) nd.flag[e & 1] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -421,9 +481,6 @@ __device__ __forceinline__ void literal_insert_lds(const Geo& g, int beg, int c,
     }
 }
 
-// candidates with r^2 <= rc^2 (1 + kRcSlack) in this atom's frame: a superset of
-// every pair that either side of a pair can select (frames differ by rounding)
-constexpr double kRcSlack = 1e-9;
 
 // EXPORT instance: phase A, then the insertion for every atom, whatever its count,
 // writing the slots in the reference order (sel_out) and the count within rc; no
@@ -448,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
 
     const bool center = valid && (g.group == nullptr || g.group[i]);
     const int beg = g.off[i];
-    const int c = center ? g.off[i + 1] - beg : 0;
+    const int c = center ? list_len(g, i) : 0;
     const double rc2 = e.rc2, rc2s = e.rc2 * (1.0 + kRcSlack);
     const int N = e.N;
     // Phase A: masks.  Each pass of the scan takes 16 consecutive entries of the
@@ -1498,6 +1555,9 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     constexpr int SG = 4;
     constexpr int NW = kRedThreads / 64;
     const int lane64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // one step per launch: the near-list epoch moves on (every kernel that reads it this
+    // step, the packs, the near-list build and the selection, has completed)
+    if (w == 0 && threadIdx.x == 0 && r.epoch) *r.epoch += 1;
     const int t0 = r.tile_args ? r.tile_range[w] : r.tile_begin[w];
     const int t1 = r.tile_args ? r.tile_range[w + 1] : r.tile_begin[w + 1];
     const int64_t step = r.step[w];
@@ -2318,17 +2378,18 @@ __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
 }
 
 // per entry: the ELL index of its reverse entry (-1 = none), from the list topology
-__global__ void k_lq_erev(Geo g, int64_t n, const int64_t* __restrict__ apos, int64_t* __restrict__ erev,
-                          int32_t* __restrict__ eidx, int32_t* __restrict__ eown) {
+__global__ void k_lq_erev(Geo g, int64_t n, const int32_t* __restrict__ reown, const int32_t* __restrict__ rrev,
+                          const int64_t* __restrict__ apos, int64_t* __restrict__ erev, int32_t* __restrict__ eidx,
+                          int32_t* __restrict__ eown) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     const int beg = g.off[a];
     for (int e = beg; e < g.off[a + 1]; ++e) {
-        const int rp = g.revpos[e];
+        const int rp = rrev[e];
         const int64_t el = apos[a] + 64 * (int64_t)(e - beg);
-        erev[el] = rp == 0xFF ? -1 : apos[g.eown[e]] + 64 * (int64_t)rp;
+        erev[el] = rp < 0 ? -1 : apos[reown[e]] + 64 * (int64_t)rp;
         eidx[el] = g.idx[e];
-        eown[el] = g.eown[e];
+        eown[el] = reown[e];
     }
 }
 
@@ -2597,9 +2658,32 @@ hipError_t env_dispatch_l(const Geo& g, const EnvDev& e, dim3 grid, hipStream_t 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+
+hipError_t launch_topology_near(const Geo& g, const NearDev& nd, int32_t* eown, uint8_t* revpos, int force,
+                                hipStream_t st) {
+    if (nd.n_local == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_topology<uint8_t, 0xFF>), dim3((unsigned)((nd.n_local + 255) / 256)), dim3(256), 0, st, g,
+                       (int64_t)nd.n_local, eown, revpos, (const int32_t*)nd.flag, (const int64_t*)nd.epoch, force);
+    return hipGetLastError();
+}
+
 hipError_t launch_topology(const Geo& g, int64_t n, int32_t* eown, uint8_t* revpos, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_topology, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, eown, revpos);
+    hipLaunchKernelGGL((k_topology<uint8_t, 0xFF>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, eown, revpos,
+                       (const int32_t*)nullptr, (const int64_t*)nullptr, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_topology_raw(const Geo& g, int64_t n, int32_t* eown, int32_t* revpos, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_topology<int32_t, -1>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, eown, revpos,
+                       (const int32_t*)nullptr, (const int64_t*)nullptr, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_prune(const Geo& g, const NearDev& nd, int force, hipStream_t st) {
+    if (nd.n_all == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prune, dim3((unsigned)((nd.n_all + 255) / 256)), dim3(256), 0, st, g, nd, force);
     return hipGetLastError();
 }
 
@@ -2615,6 +2699,15 @@ hipError_t launch_pack_walker(const double* x, int64_t n_local, int64_t n_all, d
     if (n_all == 0) return hipSuccess;
     int64_t b = (n_all + 255) / 256; if (b > 8192) b = 8192;
     hipLaunchKernelGGL(k_pack_walker, dim3((unsigned)b), dim3(256), 0, st, x, n_local, n_all, x4_loc, x4f_loc, x4_gh, x4f_gh);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_walker_near(const double* x, int64_t n_local, int64_t n_all, double4* x4_loc, float4* x4f_loc,
+                                   double4* x4_gh, float4* x4f_gh, const float4* xpr_loc, const float4* xpr_gh,
+                                   const NearDev& nd, hipStream_t st) {
+    int64_t b = (n_all + 255) / 256; if (b > 8192) b = 8192; if (b < 1) b = 1;
+    hipLaunchKernelGGL(k_pack_walker_near, dim3((unsigned)b), dim3(256), 0, st, x, n_local, n_all, x4_loc, x4f_loc,
+                       x4_gh, x4f_gh, xpr_loc, xpr_gh, nd);
     return hipGetLastError();
 }
 
@@ -2852,9 +2945,11 @@ hipError_t launch_lq(int stage, const Geo& g, const LqDev& d, double* f, hipStre
     }
 }
 
-hipError_t launch_lq_erev(const Geo& g, int64_t n, const int64_t* apos, int64_t* erev, int32_t* eidx, int32_t* eown,
+hipError_t launch_lq_erev(const Geo& g, int64_t n, const int32_t* reown, const int32_t* rrev, const int64_t* apos,
+                          int64_t* erev, int32_t* eidx, int32_t* eown,
                           hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_lq_erev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, apos, erev, eidx, eown);
+    if (n > 0) hipLaunchKernelGGL(k_lq_erev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, n, reown, rrev, apos,
+                                  erev, eidx, eown);
     return hipGetLastError();
 }
 

@@ -217,3 +217,50 @@ def test_two_biased_cals_with_one_definition(mdt, orc):
     assert rel_force_err(f, -env.dQ_dx(6, wa + wb)) <= F_TOL
     fix.close()
     ctx.close()
+
+
+def test_reference_q4_deck_long_lists_and_rebuilds(mdt, orc):
+    """The reference's own Q4 deck geometry (test/STEINH/Q4/in.test-steinhardt: FCC a = 3.26,
+    STEINH Q 4 cutoff_r 6.0 cutoff_Natoms 12, list cutoff 6.0 + 2.0 skin): ~240 list entries
+    per atom, beyond the 128 a selection mask holds, so the Steinhardt kernels walk near
+    lists cut just above rc (here rc (1 + 1/64): the 48-atom shell at 6.1 A sits right above
+    rc).  Selection (slot order), Q4.AVE and forces against the oracle on the bound lists,
+    while the atoms move far past the rebuild threshold (half the margin above rc), so the
+    near lists are rebuilt inside the steps."""
+    from helpers import Mover
+    s = fcc_system(mdt, 8, a=3.26, sigma=0.05, seed=48, rg=8.0)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    c = np.diff(off)
+    print(f"[q4 deck] n={n} list min={c.min()} mean={c.mean():.1f} max={c.max()}", flush=True)
+    assert c.min() > 200 and c.max() > 240
+    N, rc = 12, 6.0
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                            f"CAL NAME Q4 STEINH Q 4 all cutoff_r {rc} cutoff_Natoms {N} "
+                            "CV_dim 1 SYMBOL v1 Q4.AVE DIM 1 0 1 1000 v1 WT 0")
+    env = orc.Env(x, off, idx, own, n, rc, N)
+    fix.init()
+    fix.add_hills(np.array([[env.q_ave(4) + 0.004]]), np.array([0.005]), 0.01)
+    mv = Mover(x, n, own)
+    rng = np.random.default_rng(8)
+    xl = x[:n].copy()
+    moved = np.zeros(n)
+    for step in range(1, 9):
+        xa = mv.set_local(xl)
+        ctx.update_positions(0, xa)
+        f = np.zeros((n, 3))
+        fix.post_force(step, f)
+        env = orc.Env(xa, off, idx, own, n, rc, N)
+        q4 = env.q_ave(4)
+        assert abs(fix.cv("Q4.AVE") - q4) <= CV_TOL * q4, step
+        assert np.array_equal(fix.extract("Q4.count_rc", n).astype(np.int32), env.cnt), step
+        sel, cnt, ns = fix.selection("Q4", n, N)
+        assert np.array_equal(sel, env.sel_idx), step
+        w = fix.cv_weight("Q4.AVE")
+        assert rel_force_err(f, -env.dQ_dx(4, w)) <= F_TOL, step
+        d = rng.normal(scale=0.06, size=xl.shape)
+        xl = xl + d
+        moved = np.linalg.norm(xl - x[:n], axis=1)
+    # the run went far past the rebuild threshold (under 0.05 A here) and stayed inside the
+    # list skin (1 A)
+    assert moved.max() > 0.3 and moved.max() < 1.0, moved.max()
