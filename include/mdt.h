@@ -96,6 +96,11 @@ int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group);
  * group name without a binding of its own uses the mdt_bind_group membership. */
 int mdt_bind_group_named(mdt_ctx* ctx, int walker, const char* name, const uint8_t* in_group);
 
+/* This context's share of a walker's atoms (1 = mine): the atoms whose terms it adds
+ * to the CV sums, as one MPI rank adds its local atoms; every atom still takes part
+ * (neighbours, q_lm) and receives forces.  mine == NULL: every atom (the default). */
+int mdt_bind_share(mdt_ctx* ctx, int walker, const uint8_t* mine);
+
 /* Convenience: box, positions and lists of a host-built mdt_system. */
 int mdt_bind_system(mdt_ctx* ctx, int walker, const mdt_system* sys);
 
@@ -149,6 +154,24 @@ int mdt_fix_init(mdt_fix* fix);
  * f == NULL accumulates into the context's own device force buffer.
  * Replaces FixMetadynamics::post_force (src/core/fix_crystallize.cu:386-472). */
 int mdt_fix_post_force(mdt_fix* fix, int64_t step, double* f, int on_device);
+
+/* One step split at the global reduction, for callers that spread a system over
+ * several contexts (MPI ranks, each holding a share of the atoms: mdt_bind_share).
+ * mdt_fix_partials runs the step up to the CV sums and writes, per walker, the
+ * context's partial sums: per CAL in order, STEINH Q/LD (sum of q_l over the share's
+ * group atoms, their count) and SK (Re A(k), Im A(k) per k vector, atom count);
+ * *n = their total (out == NULL: only the count).  The caller sums these vectors
+ * element-wise over the contexts (MPI_Allreduce) and hands the sums to
+ * mdt_fix_resume, which finishes the step: CV values, the hill deposit, bias and
+ * chain rule, and f += the forces from this context's share of the CVs (f as in
+ * mdt_fix_post_force; over all contexts these add up to the forces of the whole
+ * system).  Replaces the per-CV MPI_Allreduce of STEIN_QL::compute_cv_AVE
+ * (src/cvs/CVs_Steinhardt.cu:824-839) and the bias broadcast of
+ * FixMetadynamics::post_force (src/core/fix_crystallize.cu:437-442).  A fix that
+ * calls mdt_fix_partials steps only this way from then on; DISTANCE, STRU_FACTOR
+ * and STEINH L CALs are refused in it. */
+int mdt_fix_partials(mdt_fix* fix, int64_t step, double* out, int64_t cap, int64_t* n);
+int mdt_fix_resume(mdt_fix* fix, int64_t step, const double* reduced, double* f, int on_device);
 
 /* `nsteps` consecutive steps step0..step0+nsteps-1 into the context's device
  * force buffer with no host synchronisation (standalone/batched driver); every

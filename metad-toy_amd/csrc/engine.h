@@ -82,6 +82,7 @@ struct Geo {
                               // center in the owner's list), 0xFF = none
     const int32_t* cnt;       // per local atom: list length (the near list); NULL: off[i+1] - off[i]
     const uint8_t* group;     // per local atom, NULL = all
+    const uint8_t* share;     // per local atom: adds to this context's CV sums (NULL = all)
     const Tile* tiles;
     int32_t ntiles;
     int32_t n_local_total;
@@ -120,7 +121,6 @@ struct EnvDev {
     float* extra;             // optional n_local_total * rec_stride: extra coefficients
     double* partial;          // [slot][tile]
     int32_t slot_ql[3];       // partial slot per present degree (order 3,4,6)
-    const double* inv_ng;     // per walker 1/N_group
     const double* wcv;        // per walker [kWcvStride] weights dV/dv (from the reduce kernel)
     int32_t wslot[3];         // wcv slot of Q_l.AVE per present degree (-1 = not biased)
     // an unbiased Lechner-Dellago CV on this (biased) environment, accumulated by
@@ -436,10 +436,16 @@ struct SkDev {
     const double* wcv;
     const int32_t* n_atoms;   // per walker
     const int32_t* atom_begin;// per walker
+    const double* n_norm;     // per walker N of S = |A|^2 / N (NULL: n_atoms)
     int32_t n_walkers;
     int32_t max_atoms;
 };
 hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st);
 hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t st);
+// the gradient records and S terms of amplitudes written into s.amp (reduced over contexts)
+hipError_t launch_sk_pack(const SkDev& s, hipStream_t st);
+// raw FP64 sums of the tile partials of every CV slot with psum[c] >= 0 (a partial slot),
+// per walker: out[w * kMaxSlotCV + c], in a fixed order
+hipError_t launch_cv_sums(const ReduceDev& r, const int32_t* psum, double* out, hipStream_t st);
 
 }  // namespace mdt

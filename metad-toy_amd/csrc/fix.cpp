@@ -224,6 +224,7 @@ struct WalkerIn {
     // group memberships of local atoms by group name ("" = the default, bound by
     // mdt_bind_group, used by any group name without a binding of its own)
     std::map<std::string, std::vector<uint8_t>> groups;
+    std::vector<uint8_t> share;       // this context's share of the CV sums (empty: every atom)
     const double* x_src = nullptr;   // attached engine buffer read at the start of every step
     // after commit
     int64_t atom_off = 0, ghost_off = 0, nnz_off = 0;
@@ -263,6 +264,8 @@ struct mdt_ctx {
     std::map<std::string, DBuf<uint8_t>> dgroups;
     std::map<std::string, std::vector<double>> gcount;
     std::map<std::string, std::string> gmissing;   // name -> the first walker without a membership
+    DBuf<uint8_t> dshare;                          // concatenated shares (when any walker has one)
+    bool any_share = false;
     DBuf<Tile> tiles;
     DBuf<double> f;
     std::vector<Tile> h_tiles;
@@ -281,6 +284,7 @@ struct mdt_ctx {
         g.x4 = x4.p; g.x4f = x4f.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p;
         g.eown = nullptr; g.revpos = nullptr; g.cnt = nullptr;
         g.group = nullptr;
+        g.share = any_share ? dshare.p : nullptr;
         g.tiles = tiles.p; g.ntiles = (int32_t)h_tiles.size(); g.n_local_total = (int32_t)total_local;
         return g;
     }
@@ -334,6 +338,21 @@ struct mdt_ctx {
         if (name == "all") return (double)w[walker].n_local;
         group_ptr(name);
         return gcount.at(group_key(name))[walker];
+    }
+    // atoms of a group in this context's share of a walker (the count its CV sums cover)
+    double share_count(const std::string& name, int walker) const {
+        const WalkerIn& a = w[walker];
+        if (a.share.empty()) return group_count(name, walker);
+        const std::vector<uint8_t>* m = nullptr;
+        if (name != "all") {
+            group_ptr(name);
+            auto it = a.groups.find(name);
+            if (it == a.groups.end()) it = a.groups.find(std::string());
+            if (it != a.groups.end()) m = &it->second;
+        }
+        double c = 0;
+        for (int64_t i = 0; i < a.n_local; ++i) c += (a.share[i] && (!m || (*m)[i])) ? 1 : 0;
+        return c;
     }
     WalkerIn& walker(int k) {
         if (k < 0 || k >= (int)w.size()) throw Err(MDT_EINVAL, fmt("walker %d out of range [0,%zu)", k, w.size()));
@@ -433,6 +452,17 @@ void mdt_ctx::commit() {
     x4.alloc((size_t)(tl + tg));
     x4f.alloc((size_t)(tl + tg));
     HIPCHK(launch_pack_x4(x.p, x4.p, x4f.p, tl + tg, stream));
+    any_share = false;
+    for (auto& a : w) any_share |= !a.share.empty();
+    if (any_share) {
+        std::vector<uint8_t> hs((size_t)tl, 1);
+        for (auto& a : w)
+            if (!a.share.empty()) std::memcpy(&hs[a.atom_off], a.share.data(), a.n_local);
+        dshare.alloc(hs.size());
+        dshare.upload(hs.data(), hs.size(), stream);
+    } else {
+        dshare.release();
+    }
     dgroups.clear();
     for (auto& kv : hgroups) {
         DBuf<uint8_t>& d = dgroups[kv.first];
@@ -504,7 +534,6 @@ struct EnvH {
     DBuf<float> rec, ql, extra;
     DBuf<int32_t> nsel, cnt;
     DBuf<uint64_t> selmask;
-    DBuf<double> inv_ng;
 };
 
 struct CvH {
@@ -520,7 +549,7 @@ struct CvH {
     DBuf<float> lq, bq;
     // SK
     std::vector<double> kvec;   // nk*3
-    DBuf<double> kd, krev, apart, amp, sterm;
+    DBuf<double> kd, krev, apart, amp, sterm, nnorm;
     DBuf<SkK> kpack;
     DBuf<float> ampf;
     DBuf<int32_t> chunk, chunk_begin, natoms, abegin;
@@ -605,6 +634,11 @@ struct mdt_fix {
     DBuf<double> mlp_err, mlp_box_lo, mlp_box_hi, mlp_vdata, mlp_blkerr;
     DBuf<float4> mlp_sdata;
     std::vector<int> mlp_has_fit;
+    // split reductions (mdt_fix_partials / mdt_fix_resume): the CV sums of this context's
+    // share of the atoms leave the step, sums over every share come back
+    bool split = false, built_split = false;
+    DBuf<double> psums;             // [walker][kMaxSlotCV] raw sums
+    DBuf<int32_t> psum_of;          // [kMaxSlotCV] partial slot of each CV (-1: none)
     MlpDev mdev{};
     ReduceDev red{};
     int64_t hills_cap = 0;
@@ -1132,16 +1166,10 @@ void fix_build(mdt_fix* fx) {
         e->rec.alloc((size_t)NT * RS);
         e->ql.alloc((size_t)NT * NL);
         e->nsel.alloc(NT); e->cnt.alloc(NT); e->selmask.alloc((size_t)NT * 4);
-        e->inv_ng.alloc(nw);
-        std::vector<double> inv(nw);
-        const std::vector<double> ngs = group_counts(e->group);
-        for (int k = 0; k < nw; ++k) inv[k] = ngs[k] > 0 ? 1.0 / ngs[k] : 0.0;
-        e->inv_ng.upload(inv.data(), nw, st);
         EnvDev& d = e->dev;
         d = EnvDev{};
         d.rc2 = e->rc * e->rc; d.N = e->N; d.lmask = lm;
         d.rec = e->rec.p; d.ql = e->ql.p; d.nsel = e->nsel.p; d.cnt_rc = e->cnt.p; d.selmask = e->selmask.p;
-        d.inv_ng = e->inv_ng.p;
         int li = 0;
         for (int l : {3, 4, 6}) {
             int bit = l == 3 ? kL3 : l == 4 ? kL4 : kL6;
@@ -1181,7 +1209,7 @@ void fix_build(mdt_fix* fx) {
     for (auto* e : fx->envs) { e->dev.fld_l = 0; e->dev.fld_slot = -1; e->dev.fld_lq = nullptr; }
     for (auto* c : fx->cvs) {
         c->fused = false;
-        if (c->kind != CV_LD || c->biased) continue;
+        if (c->kind != CV_LD || c->biased || fx->split) continue;
         EnvH* e = fx->envs[c->env];
         if (e->biased && e->dev.extra == nullptr && e->dev.fld_l == 0) { c->fused = true; e->dev.fld_l = c->l; }
     }
@@ -1231,6 +1259,7 @@ void fix_build(mdt_fix* fx) {
             c->amp.alloc((size_t)nw * nk * 2);
             c->sterm.alloc((size_t)nw * nk);
             c->kpack.alloc((size_t)nw * nk);
+            if (fx->split) c->nnorm.alloc(nw); else c->nnorm.release();
         } else if (c->kind == CV_SF) {
             const std::vector<double> ngs = group_counts(c->sf_group);
             std::vector<double> inv(nw);
@@ -1311,8 +1340,10 @@ void fix_build(mdt_fix* fx) {
         std::vector<double> h((size_t)nw * kMaxSlotCV, 0.0);
         for (auto* c : fx->cvs) {
             if (c->kind != CV_Q && c->kind != CV_LD) continue;
-            const std::vector<double> ng = group_counts(fx->envs[c->env]->group);
-            for (int k = 0; k < nw; ++k) h[(size_t)k * kMaxSlotCV + c->slot] = ng[k] > 0 ? 1.0 / ng[k] : 0.0;
+            for (int k = 0; k < nw; ++k) {
+                const double ng = ctx->share_count(fx->envs[c->env]->group, k);
+                h[(size_t)k * kMaxSlotCV + c->slot] = ng > 0 ? 1.0 / ng : 0.0;
+            }
         }
         fx->cv_inv.alloc(h.size());
         fx->cv_inv.upload(h.data(), h.size(), st);
@@ -1376,7 +1407,19 @@ void fix_build(mdt_fix* fx) {
     r.ncv = (int)fx->cvs.size();
     // every CV divides by its own group count: Q/LD partial sums use the env's inv_ng
     for (size_t c = 0; c < fx->cvs.size(); ++c)
-        r.cv_src[c] = (fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST || fx->cvs[c]->kind == CV_SF || fx->cvs[c]->kind == CV_LQ) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
+        r.cv_src[c] = (fx->split || fx->cvs[c]->kind == CV_SK || fx->cvs[c]->kind == CV_DIST || fx->cvs[c]->kind == CV_SF ||
+                       fx->cvs[c]->kind == CV_LQ) ? -1 : fx->cvs[c]->fused ? -2 - fx->cvs[c]->psum : fx->cvs[c]->psum;
+    if (fx->split) {
+        // the sums over every share arrive from the host: each CV's value is a direct one
+        for (auto* c : fx->cvs)
+            if (c->kind == CV_DIST || c->kind == CV_SF || c->kind == CV_LQ)
+                throw Err(MDT_ENOTSUP, fmt("CAL %s: split reductions cover STEINH Q/LD and SK", c->name.c_str()));
+        std::vector<int32_t> po(kMaxSlotCV, -1);
+        for (auto* c : fx->cvs) if (c->kind == CV_Q || c->kind == CV_LD) po[c->slot] = c->psum;
+        fx->psum_of.alloc(kMaxSlotCV); fx->psum_of.upload(po.data(), kMaxSlotCV, st);
+        fx->psums.alloc((size_t)nw * kMaxSlotCV);
+    }
+    fx->built_split = fx->split;
     r.cv_inv = fx->cv_inv.p;
     for (int c = 0; c < kMaxSlotCV; ++c) r.walias[c] = c < (int)fx->cvs.size() ? (walias[c] >= 0 ? walias[c] : c) : -1;
     r.nsym = (int)fx->syms.size();
@@ -1701,9 +1744,9 @@ LqDev lq_dev(mdt_fix* fx, CvH* c) {
     return d;
 }
 
-void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
-    static Timer off_timer;
-    Timer& T = tm ? *tm : off_timer;
+// The CV values of the step (through the per-atom sums and the direct values) on the
+// context stream: positions, near lists, selection and q_lm, and every value kernel.
+void enqueue_values(mdt_fix* fx, Timer& T) {
     mdt_ctx* ctx = fx->ctx;
     hipStream_t st = ctx->stream;
     // positions of walkers attached to an engine buffer are read at the start of the step
@@ -1728,7 +1771,7 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             Geo ge = g;
             ge.group = ctx->group_ptr(e->group);
             LdDev d{};
-            d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = c->biased ? c->slot : -1;
+            d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = c->biased ? kMaxSlotCV + c->slot : -1;
             T.run("k_ld", st, [&] { HIPCHK(launch_ld(ge, e->dev, d, st)); });
         } else if (c->kind == CV_SK) {
             SkDev s{};
@@ -1738,7 +1781,10 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             s.sterm = c->sterm.p;
             s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
             s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
-            T.run("k_sk_amp", st, [&] { HIPCHK(launch_sk_amp(g, s, st)); });
+            s.n_norm = fx->split ? c->nnorm.p : nullptr;
+            Geo gk = g;
+            gk.group = g.share;             // a share sums its own atoms' amplitudes
+            T.run("k_sk_amp", st, [&] { HIPCHK(launch_sk_amp(gk, s, st)); });
         } else if (c->kind == CV_DIST) {
             const DistDev d = dist_dev(fx, c);
             T.run("k_dist_value", st, [&] { HIPCHK(launch_dist_value(g.x4, d, st)); });
@@ -1756,12 +1802,19 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             T.run("k_lq_sum", st, [&] { HIPCHK(launch_lq(2, gs, d, nullptr, st)); });
         }
     }
+}
+
+// The rest of the step: deposit, bias and chain rule, forces into f.
+void enqueue_rest(mdt_fix* fx, double* f, Timer& T) {
+    mdt_ctx* ctx = fx->ctx;
+    hipStream_t st = ctx->stream;
+    const Geo g = ctx->geo_near(), graw = ctx->geo();
     T.run("k_reduce", st, [&] { HIPCHK(launch_reduce(fx->red, st)); });
     for (auto* c : fx->cvs) {
         if (c->kind == CV_LD && c->biased) {
             EnvH* e = fx->envs[c->env];
             LdDev d{};
-            d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = c->slot;
+            d.l = c->l; d.lq = c->lq.p; d.bq = c->bq.p; d.slot = c->psum; d.wslot = kMaxSlotCV + c->slot;
             T.run("k_ld_coeff", st, [&] { HIPCHK(launch_ld_coeff(g, e->dev, d, st)); });
         }
     }
@@ -1774,7 +1827,10 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
             s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p; s.kpack = c->kpack.p;
             s.wslot = c->slot; s.wcv = fx->wcv.p; s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p;
             s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
-            T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(g, s, f, st)); });
+            s.n_norm = fx->split ? c->nnorm.p : nullptr;
+            Geo gk = g;
+            gk.group = g.share;             // a share pushes its own atoms
+            T.run("k_sk_force", st, [&] { HIPCHK(launch_sk_force(gk, s, f, st)); });
         }
         if (c->kind == CV_DIST && c->biased) {
             const DistDev d = dist_dev(fx, c);
@@ -1803,6 +1859,14 @@ void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
     if (tail) T.run("k_cv_tail", st, [&] { HIPCHK(launch_cv_tail(fx->red, st)); });
 }
 
+void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
+    static Timer off_timer;
+    Timer& T = tm ? *tm : off_timer;
+    if (fx->split) throw Err(MDT_ESTATE, "a fix in split mode steps through mdt_fix_partials and mdt_fix_resume");
+    enqueue_values(fx, T);
+    enqueue_rest(fx, f, T);
+}
+
 void use_device(mdt_ctx* ctx) {
     if (ctx->device < 0) throw Err(MDT_ESTATE, "parse-only context (device -1) cannot run steps");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1810,7 +1874,8 @@ void use_device(mdt_ctx* ctx) {
 
 void ensure_built(mdt_fix* fx) {
     if (fx->ctx->device < 0) throw Err(MDT_ESTATE, "parse-only context (device -1) cannot run steps");
-    if (!fx->ctx->committed || !fx->inited || fx->built_gen != fx->ctx->generation) fix_build(fx);
+    if (!fx->ctx->committed || !fx->inited || fx->built_gen != fx->ctx->generation || fx->split != fx->built_split)
+        fix_build(fx);
 }
 
 void set_steps(mdt_fix* fx, int64_t step) {
@@ -1986,6 +2051,16 @@ int mdt_bind_group_named(mdt_ctx* ctx, int walker, const char* name, const uint8
     API_END
 }
 
+int mdt_bind_share(mdt_ctx* ctx, int walker, const uint8_t* mine) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "bind positions before the share");
+    if (mine) a.share.assign(mine, mine + a.n_local);
+    else a.share.clear();
+    ctx->committed = false;
+    API_END
+}
+
 int mdt_bind_system(mdt_ctx* ctx, int walker, const mdt_system* sys) {
     API_BEGIN(ctx)
     if (!sys) throw Err(MDT_EINVAL, "null system");
@@ -2115,6 +2190,131 @@ int mdt_fix_post_force(mdt_fix* fx, int64_t step, double* f, int on_device) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         for (size_t k = 0; k < tmp.size(); ++k) f[k] += tmp[k];
     }
+    fetch_state(fx);
+    flush_hills(fx);
+    API_END
+}
+
+// Layout of a walker's reduction vector: per CV slot in CAL order, STEINH Q/LD (sum over
+// the share's group atoms of q_l, atom count), SK (Re A, Im A per k, atom count).
+static int64_t split_len(const mdt_fix* fx) {
+    int64_t n = 0;
+    for (auto* c : fx->cvs) n += c->kind == CV_SK ? 2 * (int64_t)(c->kvec.size() / 3) + 1 : 2;
+    return n;
+}
+
+int mdt_fix_partials(mdt_fix* fx, int64_t step, double* out, int64_t cap, int64_t* n) {
+    FIX_BEGIN(fx)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    fx->split = true;
+    ensure_built(fx);
+    const int nw = fx->nw();
+    const int64_t L = split_len(fx);
+    if (n) *n = L * nw;
+    if (!out) return MDT_OK;
+    if (cap < L * nw) throw Err(MDT_EINVAL, fmt("the reduction vector has %lld values", (long long)(L * nw)));
+    set_steps(fx, step);
+    Timer T;
+    enqueue_values(fx, T);
+    HIPCHK(launch_cv_sums(fx->red, fx->psum_of.p, fx->psums.p, ctx->stream));
+    std::vector<double> ps((size_t)nw * kMaxSlotCV);
+    HIPCHK(hipMemcpyAsync(ps.data(), fx->psums.p, sizeof(double) * ps.size(), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<std::vector<double>> amps(fx->cvs.size());
+    for (size_t q = 0; q < fx->cvs.size(); ++q) {
+        const CvH* c = fx->cvs[q];
+        if (c->kind != CV_SK) continue;
+        amps[q].resize((size_t)nw * c->kvec.size() / 3 * 2);
+        HIPCHK(hipMemcpyAsync(amps[q].data(), c->amp.p, sizeof(double) * amps[q].size(), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nw; ++k) {
+        double* o = out + (size_t)k * L;
+        for (size_t q = 0; q < fx->cvs.size(); ++q) {
+            const CvH* c = fx->cvs[q];
+            if (c->kind == CV_SK) {
+                const size_t nk = c->kvec.size() / 3;
+                std::memcpy(o, amps[q].data() + (size_t)k * nk * 2, sizeof(double) * nk * 2);
+                o += nk * 2;
+                *o++ = ctx->share_count("all", k);
+            } else {
+                *o++ = ps[(size_t)k * kMaxSlotCV + c->slot];
+                *o++ = ctx->share_count(fx->envs[c->env]->group, k);
+            }
+        }
+    }
+    API_END
+}
+
+int mdt_fix_resume(mdt_fix* fx, int64_t step, const double* reduced, double* f, int on_device) {
+    FIX_BEGIN(fx)
+    mdt_ctx* ctx = fx->ctx;
+    use_device(ctx);
+    if (!fx->split || fx->built_split != fx->split || fx->built_gen != ctx->generation)
+        throw Err(MDT_ESTATE, "mdt_fix_resume follows mdt_fix_partials of the same step");
+    if (!reduced) throw Err(MDT_EINVAL, "null reduction vector");
+    const int nw = fx->nw();
+    const int64_t L = split_len(fx);
+    // the CV values, and 1 / N of the per-atom averages, from the sums over every share
+    std::vector<double> direct((size_t)nw * kMaxSlotCV, 0.0), inv((size_t)nw * kMaxSlotCV, 0.0);
+    hipStream_t st = ctx->stream;
+    for (size_t q = 0; q < fx->cvs.size(); ++q) {
+        CvH* c = fx->cvs[q];
+        if (c->kind != CV_SK) continue;
+        const size_t nk = c->kvec.size() / 3;
+        std::vector<double> a((size_t)nw * nk * 2), nn(nw);
+        for (int k = 0; k < nw; ++k) {
+            const double* o = reduced + (size_t)k * L;
+            for (size_t p = 0; p < q; ++p)
+                o += fx->cvs[p]->kind == CV_SK ? 2 * (fx->cvs[p]->kvec.size() / 3) + 1 : 2;
+            std::memcpy(&a[(size_t)k * nk * 2], o, sizeof(double) * nk * 2);
+            nn[k] = o[nk * 2];
+        }
+        c->amp.upload(a.data(), a.size(), st);
+        c->nnorm.upload(nn.data(), nn.size(), st);
+    }
+    for (int k = 0; k < nw; ++k) {
+        const double* o = reduced + (size_t)k * L;
+        for (auto* c : fx->cvs) {
+            if (c->kind == CV_SK) {
+                const size_t nk = c->kvec.size() / 3;
+                const double N = o[nk * 2];
+                double v = 0.0;
+                for (size_t p = 0; p < nk; ++p) v += (o[2 * p] * o[2 * p] + o[2 * p + 1] * o[2 * p + 1]) / N;
+                direct[(size_t)k * kMaxSlotCV + c->slot] = N > 0 ? v / (double)nk : 0.0;
+                o += nk * 2 + 1;
+            } else {
+                const double N = o[1];
+                direct[(size_t)k * kMaxSlotCV + c->slot] = N > 0 ? o[0] / N : 0.0;
+                inv[(size_t)k * kMaxSlotCV + c->slot] = N > 0 ? 1.0 / N : 0.0;
+                o += 2;
+            }
+        }
+    }
+    fx->direct.upload(direct.data(), direct.size(), st);
+    fx->cv_inv.upload(inv.data(), inv.size(), st);
+    for (auto* c : fx->cvs) {
+        if (c->kind != CV_SK) continue;
+        SkDev s{};
+        s.nk = (int)(c->kvec.size() / 3);
+        s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p; s.kpack = c->kpack.p; s.sterm = c->sterm.p;
+        s.n_atoms = c->natoms.p; s.n_norm = c->nnorm.p; s.n_walkers = nw;
+        HIPCHK(launch_sk_pack(s, st));
+    }
+    ensure_hills(fx, 1);
+    for (auto& b : fx->hills_bound) b += 1;
+    double* fd = ctx->f.p;
+    if (f && on_device) fd = f;
+    else if (f) HIPCHK(launch_fill(ctx->f.p, ctx->total_local * 3, 0.0, st));
+    Timer T;
+    enqueue_rest(fx, fd, T);
+    if (f && !on_device) {
+        std::vector<double> tmp((size_t)ctx->total_local * 3);
+        HIPCHK(hipMemcpyAsync(tmp.data(), ctx->f.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t k = 0; k < tmp.size(); ++k) f[k] += tmp[k];
+    }
+    (void)step;
     fetch_state(fx);
     flush_hills(fx);
     API_END

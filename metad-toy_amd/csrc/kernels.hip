@@ -648,7 +648,9 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     out[2] = pif.z;
     out[3] = __uint_as_float((__float_as_uint(pif.w) & ~(kHdrCap | kHdrBand)) | (cnt > N ? kHdrCap : 0u) |
                              (band ? kHdrBand : 0u));
-    const double inv_ng = e.inv_ng[tl.walker];
+    // the atom's share of the CV sums (an atom outside the context's share keeps its q_lm
+    // for its neighbours' use, but adds nothing to the sums nor to their gradient)
+    const bool inshare = g.share == nullptr || g.share[i];
     auto finish = [&](auto ll) {
         constexpr int L = decltype(ll)::value;
         if constexpr ((L == 3 && (LMASK & kL3)) || (L == 4 && (LMASK & kL4)) || (L == 6 && (LMASK & kL6))) {
@@ -659,9 +661,9 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
             for (int m = 1; m <= L; ++m) s2 += 2.0f * (q[o + 2 * m - 1] * q[o + 2 * m - 1] + q[o + 2 * m] * q[o + 2 * m]);
             constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
             const float ql = __builtin_amdgcn_sqrtf(s2 * P);
-            out[rec_a(LMASK, L)] = (ql > 0.0f && n > 0) ? (float)(P * inv_ng) * __builtin_amdgcn_rcpf(ql * (float)n) : 0.0f;
+            out[rec_a(LMASK, L)] = (ql > 0.0f && n > 0 && inshare) ? P * __builtin_amdgcn_rcpf(ql * (float)n) : 0.0f;
             if (valid && lane == 0) e.ql[(int64_t)li * g.n_local_total + i] = ql;
-            part[li] = (center && lane == 0) ? (double)ql : 0.0;
+            part[li] = (center && inshare && lane == 0) ? (double)ql : 0.0;
         }
     };
     finish(std::integral_constant<int, 3>{});
@@ -706,7 +708,8 @@ __device__ __forceinline__ M128 reverse_mask(const Geo& g, const EnvDev& e, int 
 
 // ---------------------------------------------------------------------------
 // Lechner-Dellago local q: qbar_i = (q_i + sum_{j in S(i)} q_owner(j)) / (n_i + 1).
-// Writes lq, bq = P inv_ng / (lq (n+1)) * qbar, and per-tile sums of lq.
+// Writes lq, bq = P / (lq (n+1)) * qbar, and per-tile sums of lq (the 1/N_group of the
+// average enters through the chain-rule weight).
 // ---------------------------------------------------------------------------
 template <int L, int LMASK>
 __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
@@ -766,7 +769,8 @@ __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
     s2 = grp_sum(s2);
     constexpr float P = (float)(4.0 * kPi / (2 * L + 1));
     const float lq = center ? __builtin_sqrtf(P * s2) : 0.0f;
-    const float beta = lq > 0.0f ? (float)(P * e.inv_ng[tl.walker]) / (lq * (float)(n + 1)) : 0.0f;
+    const bool inshare = g.share == nullptr || g.share[i];
+    const float beta = lq > 0.0f && inshare ? P / (lq * (float)(n + 1)) : 0.0f;
     if (valid && mine && d.wslot >= 0) {      // gradient coefficients only when LQ.AVE is biased
         float o4[4];
         #pragma unroll
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_ld(Geo g, EnvDev e, LdDev d) {
     }
     if (valid && lane == 0) {
         d.lq[i] = lq;
-        part[0] = (double)lq;
+        part[0] = inshare ? (double)lq : 0.0;
     }
     block_sum<1>(part, sh);
     if (threadIdx.x == 0) e.partial[(int64_t)d.slot * g.ntiles + t] = part[0];
@@ -1148,7 +1152,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
             constexpr float P = (float)(4.0 * kPi / (2 * LDL + 1));
             const float lq = center ? __builtin_amdgcn_sqrtf(P * s2) : 0.0f;
             e.fld_lq[i] = lq;
-            part[0] = (double)lq;
+            part[0] = (g.share == nullptr || g.share[i]) ? (double)lq : 0.0;
         }
         // the staging buffer is free again: its first bytes hold the block-sum scratch
         __syncthreads();
@@ -1504,11 +1508,13 @@ __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, 
         r.wcv[(int64_t)w * kWcvStride + c] = wc;
     }
     // the weight sum of each representative slot over the slots aliased to it, in slot
-    // order (a slot that nothing aliases gets its own weight)
+    // order, each weight times its CV's 1/N_group (a slot that nothing aliases gets its
+    // own): the force passes of per-group averages take it
+    const double wn = c < kMaxSlotCV ? wc * r.cv_inv[(int64_t)w * kMaxSlotCV + c] : 0.0;
     double comb = 0.0;
     #pragma unroll
     for (int q = 0; q < kMaxSlotCV; ++q) {
-        const double v = __shfl(wc, q, 64);
+        const double v = __shfl(wn, q, 64);
         if (r.walias[q] == c) comb += v;
     }
     if (c < kMaxSlotCV) r.wcv[(int64_t)w * kWcvStride + kMaxSlotCV + c] = comb;
@@ -1752,6 +1758,30 @@ __global__ __launch_bounds__(1024) void k_cv_tail(ReduceDev r) {
         }
         block_sum<1, NT / 64>(v, sh);
         if (threadIdx.x == 0) r.state[(int64_t)w * kStStride + kStCv + c] = v[0] * r.cv_inv[(int64_t)w * kMaxSlotCV + c];
+        __syncthreads();
+    }
+}
+
+// Raw sums of the tile partials, one 1024-thread workgroup per walker, in the tree
+// order of k_cv_tail: the context's share of every per-atom-sum CV, before the sum over
+// contexts and the division by the group count.
+__global__ __launch_bounds__(1024) void k_cv_sums(ReduceDev r, const int32_t* __restrict__ psum, double* __restrict__ out) {
+    constexpr int NT = 1024, U = 4;
+    __shared__ double sh[NT / 64];
+    const int w = blockIdx.x;
+    const int t0 = r.tile_begin[w], t1 = r.tile_begin[w + 1];
+    for (int c = 0; c < r.ncv; ++c) {
+        if (psum[c] < 0) continue;
+        const double* pp = r.partial + (int64_t)psum[c] * r.ntiles;
+        double v[1] = {0.0};
+        for (int t = t0 + threadIdx.x; t < t1; t += U * NT) {
+            double x[U];
+            #pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = t + u * NT < t1 ? pp[t + u * NT] : 0.0;
+            v[0] += (x[0] + x[1]) + (x[2] + x[3]);
+        }
+        block_sum<1, NT / 64>(v, sh);
+        if (threadIdx.x == 0) out[(int64_t)w * kMaxSlotCV + c] = v[0];
         __syncthreads();
     }
 }
@@ -2565,9 +2595,25 @@ __global__ __launch_bounds__(kBlock) void k_sk_fold(SkDev sk) {
     block_sum<2>(v2, sh);
     if (threadIdx.x != 0) return;
     const double re = v2[0], im = v2[1];
-    const double N = (double)sk.n_atoms[w];
+    const double N = sk.n_norm ? sk.n_norm[w] : (double)sk.n_atoms[w];
     sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
     sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
+    SkK kp;
+    for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
+    kp.ar = (float)re; kp.ai = (float)im;
+    kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
+    sk.kpack[(int64_t)w * sk.nk + q] = kp;
+    sk.sterm[(int64_t)w * sk.nk + q] = (re * re + im * im) / N;
+}
+
+// The gradient record and |A|^2 / N of amplitudes already in sk.amp (summed over the
+// contexts that each hold a share of the atoms): walker blockIdx.y, k = the thread.
+__global__ __launch_bounds__(kBlock) void k_sk_pack(SkDev sk) {
+    const int w = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= sk.nk) return;
+    const double re = sk.amp[((int64_t)w * sk.nk + q) * 2], im = sk.amp[((int64_t)w * sk.nk + q) * 2 + 1];
+    const double N = sk.n_norm ? sk.n_norm[w] : (double)sk.n_atoms[w];
     SkK kp;
     for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
     kp.ar = (float)re; kp.ai = (float)im;
@@ -2599,7 +2645,8 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
     const int n = sk.n_atoms[w];
     if ((int)(blockIdx.x * blockDim.x * kSkAtoms) >= n) return;
     const double wt = sk.wcv[(int64_t)w * kWcvStride + sk.wslot];
-    const float scale = (float)(wt * 2.0 / ((double)n * sk.nk));
+    const double nn = sk.n_norm ? sk.n_norm[w] : (double)n;
+    const float scale = (float)(wt * 2.0 / (nn * sk.nk));
     int ii[kSkAtoms];
     double px[kSkAtoms], py[kSkAtoms], pz[kSkAtoms];
     float gx[kSkAtoms], gy[kSkAtoms], gz[kSkAtoms];
@@ -2867,6 +2914,16 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sk_cv, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_sk_pack(const SkDev& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_sk_pack, dim3((s.nk + kBlock - 1) / kBlock, s.n_walkers), dim3(kBlock), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_cv_sums(const ReduceDev& r, const int32_t* psum, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cv_sums, dim3(r.n_walkers), dim3(1024), 0, st, r, psum, out);
     return hipGetLastError();
 }
 

@@ -49,6 +49,9 @@ def lib():
         "mdt_bind_neighbors": (I, [P, I, P, P, I64, P, I]),
         "mdt_bind_group": (I, [P, I, u8p]),
         "mdt_bind_group_named": (I, [P, I, C.c_char_p, u8p]),
+        "mdt_bind_share": (I, [P, I, u8p]),
+        "mdt_fix_partials": (I, [P, I64, dp, I64, C.POINTER(I64)]),
+        "mdt_fix_resume": (I, [P, I64, dp, P, I]),
         "mdt_bind_system": (I, [P, I, P]),
         "mdt_update_positions": (I, [P, I, P, I]),
         "mdt_attach_positions": (I, [P, I, P]),
@@ -227,6 +230,10 @@ class Context:
         self._chk(lib().mdt_bind_group_named(self._h, walker, name.encode(),
                                              None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8))))
 
+    def bind_share(self, walker, mine):
+        m = None if mine is None else np.ascontiguousarray(mine, np.uint8)
+        self._chk(lib().mdt_bind_share(self._h, walker, None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8))))
+
     def bind_system(self, walker, system):
         self._chk(lib().mdt_bind_system(self._h, walker, system._h))
 
@@ -363,6 +370,23 @@ class Fix:
         else:
             assert f.dtype == np.float64 and f.flags.c_contiguous
             self._chk(lib().mdt_fix_post_force(self._h, step, f.ctypes.data, 0))
+
+    def partials(self, step):
+        """mdt_fix_partials: this context's reduction vector of the step (all walkers)."""
+        n = C.c_int64()
+        self._chk(lib().mdt_fix_partials(self._h, step, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        self._chk(lib().mdt_fix_partials(self._h, step, _dp(out), n.value, C.byref(n)))
+        return out[: n.value]
+
+    def resume(self, step, reduced, f=None):
+        """mdt_fix_resume with the reduction vector summed over the contexts."""
+        r = np.ascontiguousarray(reduced, np.float64)
+        if f is None:
+            self._chk(lib().mdt_fix_resume(self._h, step, _dp(r), None, 0))
+        else:
+            assert f.dtype == np.float64 and f.flags.c_contiguous
+            self._chk(lib().mdt_fix_resume(self._h, step, _dp(r), f.ctypes.data, 0))
 
     def run(self, step0, nsteps, per_launch=None):
         """nsteps steps on the device; per_launch = captured steps per graph launch
