@@ -350,6 +350,8 @@ def run_c5(mdt, args):
         ctx.bind_system(0, system)
         extra = "Gaussian_Hill_type 2" if kind == "hills" else ("Gaussian_Hill_type 3 MLP_REFIT 500 "
                                                                "MLP_POINTS 1000000 MLP_HOLDOUT 65536")
+        if kind == "mlp" and args.mlp_plain:
+            extra += " MLP_GRAPH 0"
         fix = mdt.Fix(ctx, base + extra)
         fix.init()
         fix.post_force(0)
@@ -465,6 +467,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lq", "copy"])
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mlp-plain", action="store_true",
+                    help="c5: the learned bias trains with plain launches instead of a replayed captured epoch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

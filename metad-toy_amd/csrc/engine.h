@@ -191,6 +191,7 @@ struct MlpDev {
     int64_t* fallbacks;
     double* box_lo_out;       // per walker [3]: box of the hill centres
     double* box_hi_out;
+    int64_t* adam_t;          // per walker Adam step of the current fit (advanced by k_mlp_train)
 };
 
 // Everything the per-walker reduce/bias kernel needs (device pointers).
@@ -400,11 +401,14 @@ hipError_t launch_bias_add(const ReduceDev& r, int walker, int64_t nh, const dou
                            const double* h, double sigma, hipStream_t st);
 
 // learned bias (mlp.hip)
-hipError_t mlp_data(const MlpDev& m, int w, int64_t n, uint64_t seed, float4* s, double* v, hipStream_t st);
+// mode 0: fresh points and labels; 1: labels of the points in s raised by the hills since
+// the last fit; 2: fresh points around the hills since the last fit (k_mlp_data)
+hipError_t mlp_data(const MlpDev& m, int w, int64_t n, uint64_t seed, float4* s, double* v, int mode, hipStream_t st);
 hipError_t mlp_box(const MlpDev& m, int w, hipStream_t st);
 hipError_t mlp_stats(const MlpDev& m, int w, int64_t n, const double* v, hipStream_t st);
 hipError_t mlp_train_step(const MlpDev& m, int w, int64_t first, int64_t count, int64_t n_total, const float4* s,
-                          const double* v, float* gpart, float lr, float* gsum, int64_t step, hipStream_t st);
+                          const double* v, float* gpart, float lr, float* gsum, int64_t step, int64_t nit,
+                          hipStream_t st);
 hipError_t mlp_forward(const MlpNet* net, int64_t n, const float4* s, const double* vref, double* vout,
                        double* blk_err, hipStream_t st);
 hipError_t mlp_commit(const MlpDev& m, int w, int nblk, const double* blk_err, double tol, hipStream_t st);

@@ -608,6 +608,7 @@ struct mdt_fix {
     int mlp_epochs = 8, mlp_epochs_first = 40;
     double mlp_lr = 1e-2, mlp_tol = 0.1, mlp_spread = 2.0;
     double mlp_lr_refit = -1.0;   // learning rate of warm-started refits (< 0: MLP_LR / 10)
+    int mlp_graph = 1;            // 1: replay a captured training epoch; 0: plain launches
     uint64_t mlp_seed = 20241203;
     int64_t mlp_round = 0;
     std::vector<double> lo, hi;
@@ -630,7 +631,9 @@ struct mdt_fix {
     DBuf<double> aff;              // [kMaxDim][kMaxSlotCV] affine DIM coefficients per value slot
     DBuf<MlpNet> mlp_live, mlp_stage;
     DBuf<float> mlp_m, mlp_v, mlp_gpart;
-    DBuf<int64_t> mlp_fits, mlp_fallbacks;
+    DBuf<int64_t> mlp_fits, mlp_fallbacks, mlp_t;
+    std::vector<int64_t> mlp_bound_fit;   // per walker: hills bound at the last fit
+    std::vector<int64_t> mlp_roff;        // per walker: next training block to replace
     DBuf<double> mlp_err, mlp_box_lo, mlp_box_hi, mlp_vdata, mlp_blkerr;
     DBuf<float4> mlp_sdata;
     std::vector<int> mlp_has_fit;
@@ -743,6 +746,7 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
             else if (kw == "MLP_TOL") fx->mlp_tol = parse_num(v, kw.c_str());
             else if (kw == "MLP_SPREAD") fx->mlp_spread = parse_num(v, kw.c_str());
             else if (kw == "MLP_SEED") fx->mlp_seed = (uint64_t)parse_int(v, kw.c_str());
+            else if (kw == "MLP_GRAPH") fx->mlp_graph = parse_int(v, kw.c_str()) != 0 ? 1 : 0;
             else throw Err(MDT_EINVAL, fmt("Error: Unknown keyword in fix metadynamics command: %s", kw.c_str()));
             if (fx->mlp_refit < 1 || fx->mlp_points < 64 || fx->mlp_holdout < 64 || fx->mlp_batch < 64 ||
                 fx->mlp_epochs < 0 || fx->mlp_epochs_first < 0 || !(fx->mlp_lr >= 0) || !(fx->mlp_tol >= 0))
@@ -1381,7 +1385,8 @@ void fix_build(mdt_fix* fx) {
         if (fx->hill_type == 3) {
             fx->mlp_live.alloc(nw); fx->mlp_stage.alloc(nw);
             fx->mlp_m.alloc((size_t)nw * kMlpP); fx->mlp_v.alloc((size_t)nw * kMlpP);
-            fx->mlp_fits.alloc(nw); fx->mlp_fallbacks.alloc(nw);
+            fx->mlp_fits.alloc(nw); fx->mlp_fallbacks.alloc(nw); fx->mlp_t.alloc(nw);
+            fx->mlp_bound_fit.assign(nw, 0); fx->mlp_roff.assign(nw, 0);
             fx->mlp_err.alloc(nw); fx->mlp_box_lo.alloc((size_t)3 * nw); fx->mlp_box_hi.alloc((size_t)3 * nw);
             fx->mlp_fits.upload(z.data(), nw, st);
             fx->mlp_fallbacks.upload(z.data(), nw, st);
@@ -1458,6 +1463,7 @@ void fix_build(mdt_fix* fx) {
         m.adam_m = fx->mlp_m.p; m.adam_v = fx->mlp_v.p;
         m.ok = fx->mlp_ok.p; m.err = fx->mlp_err.p; m.fits = fx->mlp_fits.p; m.fallbacks = fx->mlp_fallbacks.p;
         m.box_lo_out = fx->mlp_box_lo.p; m.box_hi_out = fx->mlp_box_hi.p;
+        m.adam_t = fx->mlp_t.p;
     }
     for (auto* e : fx->envs) e->dev.partial = fx->partial.p, e->dev.wcv = fx->wcv.p;
     if (!fx->pin) {
@@ -1607,8 +1613,12 @@ void flush_hills(mdt_fix* fx) {
 }
 
 // Fit the learned bias of walker w to the hills deposited so far (asynchronous,
-// on the context stream): sample + exact FP64 labels, Adam epochs on the MFMA
-// kernels, held-out max |dV| gate against MLP_TOL kT.
+// on the context stream): training points with exact FP64 labels, Adam epochs on the
+// MFMA kernels, held-out max |dV| gate against MLP_TOL kT.  The first fit draws every
+// point; a refit keeps the points, adds the hills deposited since the last fit to their
+// labels and redraws a block around those hills (its share of all hills, 1/64 to 1/4 of
+// the set).  One epoch of training steps is captured as a graph and replayed once per
+// epoch: the Adam step and the learning-rate schedule come from a device counter.
 void mlp_refit(mdt_fix* fx, int w) {
     hipStream_t st = fx->ctx->stream;
     const MlpDev& m = fx->mdev;
@@ -1619,42 +1629,58 @@ void mlp_refit(mdt_fix* fx, int w) {
     if (first) HIPCHK(mlp_init(stage, fx->mlp_seed + 977 * w, st));
     HIPCHK(launch_fill_f(fx->mlp_m.p + (size_t)w * kMlpP, kMlpP, 0.0f, st));
     HIPCHK(launch_fill_f(fx->mlp_v.p + (size_t)w * kMlpP, kMlpP, 0.0f, st));
-    HIPCHK(mlp_box(m, w, st));
+    HIPCHK(hipMemsetAsync(fx->mlp_t.p + w, 0, sizeof(int64_t), st));
     const uint64_t seed = fx->mlp_seed * 0x9E3779B97F4A7C15ull + (uint64_t)(fx->mlp_round++) * 7919 + w;
     const int64_t M = fx->mlp_points, Hn = fx->mlp_holdout;
-    HIPCHK(mlp_data(m, w, M, seed, fx->mlp_sdata.p, fx->mlp_vdata.p, st));
-    HIPCHK(mlp_data(m, w, Hn, seed ^ 0xA5A5A5A5A5A5A5A5ull, fx->mlp_sdata.p + M, fx->mlp_vdata.p + M, st));
+    if (first) {
+        HIPCHK(mlp_box(m, w, st));
+        HIPCHK(mlp_data(m, w, M, seed, fx->mlp_sdata.p, fx->mlp_vdata.p, 0, st));
+    } else {
+        HIPCHK(mlp_data(m, w, M, seed, fx->mlp_sdata.p, fx->mlp_vdata.p, 1, st));
+        const int64_t now = std::max<int64_t>(fx->hills_bound[w], 1), prev = fx->mlp_bound_fit[w];
+        int64_t R = (int64_t)((double)M * (double)std::max<int64_t>(now - prev, 0) / (double)now);
+        R = std::min(std::max(R, M / 64), M / 4);
+        int64_t off = fx->mlp_roff[w] % M;
+        const int64_t r1 = std::min(R, M - off);
+        HIPCHK(mlp_data(m, w, r1, seed ^ 0x5bd1e995ull, fx->mlp_sdata.p + off, fx->mlp_vdata.p + off, 2, st));
+        if (R > r1) HIPCHK(mlp_data(m, w, R - r1, seed ^ 0x27d4eb2full, fx->mlp_sdata.p, fx->mlp_vdata.p, 2, st));
+        fx->mlp_roff[w] = (off + R) % M;
+        HIPCHK(mlp_box(m, w, st));
+    }
+    fx->mlp_bound_fit[w] = fx->hills_bound[w];
+    HIPCHK(mlp_data(m, w, Hn, seed ^ 0xA5A5A5A5A5A5A5A5ull, fx->mlp_sdata.p + M, fx->mlp_vdata.p + M, 0, st));
     HIPCHK(mlp_stats(m, w, M, fx->mlp_vdata.p, st));
     const int epochs = first ? fx->mlp_epochs_first : fx->mlp_epochs;
     const int64_t B = fx->mlp_batch;
-    // The training loop is the same sequence of launches for every fit of a kind
-    // (first or later): it is captured once as a graph and replayed.
+    const int64_t nb = (M + B - 1) / B;
     if (fx->mlp_graph_gen != fx->builds) {
         for (auto& kv : fx->mlp_graphs) (void)hipGraphExecDestroy(kv.second);
         fx->mlp_graphs.clear();
         fx->mlp_graph_gen = fx->builds;
     }
+    // one epoch: nb training steps over the batches in order; a refit starts from the
+    // live network and fine-tunes at a lower rate
+    const double lr0 = first ? fx->mlp_lr : (fx->mlp_lr_refit >= 0.0 ? fx->mlp_lr_refit : 0.1 * fx->mlp_lr);
+    const int64_t nit = std::max<int64_t>((int64_t)epochs * nb, 1);
+    if (!fx->mlp_graph) {
+        for (int e = 0; e < epochs; ++e)
+            for (int64_t b0 = 0; b0 < M; b0 += B)
+                HIPCHK(mlp_train_step(m, w, b0, std::min(B, M - b0), M, fx->mlp_sdata.p, fx->mlp_vdata.p,
+                                      fx->mlp_gpart.p, (float)lr0, nullptr, 0, nit, st));
+    }
     const std::pair<int, int> key(w, first ? 1 : 0);
     auto git = fx->mlp_graphs.find(key);
-    if (git == fx->mlp_graphs.end()) {
+    if (fx->mlp_graph && git == fx->mlp_graphs.end()) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipSuccess;
-        int64_t it = 0;
-        const int64_t nit = epochs * ((M + B - 1) / B);
-        for (int e = 0; e < epochs && ce == hipSuccess; ++e)
-            for (int64_t b0 = 0; b0 < M && ce == hipSuccess; b0 += B, ++it) {
-                // cosine decay of the learning rate over the fit
-                // a refit starts from the live network: fine-tune at a lower rate
-                const double lr0 = first ? fx->mlp_lr : (fx->mlp_lr_refit >= 0.0 ? fx->mlp_lr_refit : 0.1 * fx->mlp_lr);
-                const double lr = lr0 * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.141592653589793 * (double)it / (double)std::max<int64_t>(nit, 1))));
-                ce = mlp_train_step(m, w, b0, std::min(B, M - b0), M, fx->mlp_sdata.p, fx->mlp_vdata.p,
-                                    fx->mlp_gpart.p, (float)lr, nullptr, it + 1, st);
-            }
+        for (int64_t b0 = 0; b0 < M && ce == hipSuccess; b0 += B)
+            ce = mlp_train_step(m, w, b0, std::min(B, M - b0), M, fx->mlp_sdata.p, fx->mlp_vdata.p,
+                                fx->mlp_gpart.p, (float)lr0, nullptr, 0, nit, st);
         hipError_t ee = hipStreamEndCapture(st, &gr);
         if (ce != hipSuccess || ee != hipSuccess) {
             if (gr) (void)hipGraphDestroy(gr);
-            throw Err(MDT_EHIP, fmt("capturing the training loop: %s", hipGetErrorString(ce != hipSuccess ? ce : ee)));
+            throw Err(MDT_EHIP, fmt("capturing the training epoch: %s", hipGetErrorString(ce != hipSuccess ? ce : ee)));
         }
         hipGraphExec_t ex = nullptr;
         hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
@@ -1662,7 +1688,8 @@ void mlp_refit(mdt_fix* fx, int w) {
         if (ie != hipSuccess) throw Err(MDT_EHIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(ie)));
         git = fx->mlp_graphs.emplace(key, ex).first;
     }
-    HIPCHK(hipGraphLaunch(git->second, st));
+    if (fx->mlp_graph)
+        for (int e = 0; e < epochs; ++e) HIPCHK(hipGraphLaunch(git->second, st));
     const int nblk = (int)((Hn + 63) / 64);
     HIPCHK(mlp_forward(stage, Hn, fx->mlp_sdata.p + M, fx->mlp_vdata.p + M, nullptr, fx->mlp_blkerr.p, st));
     HIPCHK(mlp_commit(m, w, nblk, fx->mlp_blkerr.p, fx->mlp_tol * fx->temp * fx->boltz, st));
@@ -2860,7 +2887,7 @@ int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int
     gsum.alloc(kMlpP); am.alloc(kMlpP); av.alloc(kMlpP);
     MlpDev m{};
     m.D = 3; m.stage = net.p; m.live = net.p; m.adam_m = am.p; m.adam_v = av.p;
-    HIPCHK(mlp_train_step(m, 0, 0, np, np, ds.p, dv.p, gpart.p, 0.0f, gsum.p, 1, ctx->stream));
+    HIPCHK(mlp_train_step(m, 0, 0, np, np, ds.p, dv.p, gpart.p, 0.0f, gsum.p, 1, 0, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(grad, gsum.p, sizeof(float) * kMlpP, hipMemcpyDeviceToHost));
     API_END

@@ -62,24 +62,35 @@ __device__ __forceinline__ double gauss(uint64_t& s) {
 __device__ __forceinline__ float tanh_f(float z) { return tanhf(z); }
 
 // ---------------------------------------------------------------------------
-// data: n points; mode 0 = training stream, 1 = held-out stream
+// Training and held-out points with exact FP64 labels (the sum of every hill).
+//   mode 0: fresh points (90 % within spread sigma of a random hill, 10 % uniform over
+//           the hill box widened by 4 sigma), labelled by every hill;
+//   mode 1: the points already in s_out keep their place; the hills deposited since the
+//           last fit (stage n_fit .. nhills) are added to their labels;
+//   mode 2: fresh points around the hills deposited since the last fit, labelled by
+//           every hill.
 // ---------------------------------------------------------------------------
 constexpr int kHillChunk = 256;
 __global__ __launch_bounds__(256) void k_mlp_data(MlpDev m, int w, int64_t n, uint64_t seed, float4* __restrict__ s_out,
-                                                   double* __restrict__ v_out) {
+                                                   double* __restrict__ v_out, int mode) {
     __shared__ double hc[kHillChunk * 4];
     __shared__ double hsc[kHillChunk];
-    const int64_t nh_all = m.nhills[w];
-    const int64_t nfit = nh_all;
+    const int64_t nh = m.nhills[w];
+    const int64_t hprev = m.stage[w].n_fit < nh ? m.stage[w].n_fit : nh;
+    const int64_t lab0 = mode == 1 ? hprev : 0;                  // first hill of the label sum
+    const int64_t pick0 = mode == 2 && hprev < nh ? hprev : 0;   // first hill sampled around
     const double* Hs = m.hills + (int64_t)w * m.hills_cap * (m.D + 3);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s[3] = {0, 0, 0};
-    if (p < n && nfit > 0) {
+    if (p < n && mode == 1) {
+        const float4 q = s_out[p];
+        s[0] = q.x; s[1] = q.y; s[2] = q.z;
+    } else if (p < n && nh > 0) {
         uint64_t st = mix64(seed ^ (uint64_t)(p * 0x2545F4914F6CDD1Dull));
         const double pick = u01(st);
         if (pick < 0.9) {
-            int64_t h = (int64_t)(u01(st) * (double)nfit);
-            if (h >= nfit) h = nfit - 1;
+            int64_t h = pick0 + (int64_t)(u01(st) * (double)(nh - pick0));
+            if (h >= nh) h = nh - 1;
             for (int d = 0; d < 3; ++d) s[d] = Hs[h * (m.D + 3) + 1 + d] + m.spread * m.sigma[d] * gauss(st);
         } else {
             // uniform over the hills' box widened by 4 sigma
@@ -92,13 +103,13 @@ __global__ __launch_bounds__(256) void k_mlp_data(MlpDev m, int w, int64_t n, ui
     const double i2s0 = 0.5 / (m.sigma[0] * m.sigma[0]), i2s1 = 0.5 / (m.sigma[1] * m.sigma[1]),
                  i2s2 = 0.5 / (m.sigma[2] * m.sigma[2]);
     double v = 0.0;
-    for (int64_t h0 = 0; h0 < nfit; h0 += kHillChunk) {
+    for (int64_t h0 = lab0; h0 < nh; h0 += kHillChunk) {
         __syncthreads();
         for (int k = threadIdx.x; k < kHillChunk; k += blockDim.x) {
             const int64_t h = h0 + k;
-            if (h < nfit) {
+            if (h < nh) {
                 // a replayed hill keeps its own sigma: its widths are the bias widths times
-                // sigma / sigma_0; the scale rides on the centre coordinates' units
+                // sigma / sigma_0
                 const double* rec = Hs + h * (m.D + 3);
                 hc[4 * k] = rec[1]; hc[4 * k + 1] = rec[2]; hc[4 * k + 2] = rec[3]; hc[4 * k + 3] = rec[4];
                 hsc[k] = rec[5] == m.sigma[0] ? 1.0 : (m.sigma[0] * m.sigma[0]) / (rec[5] * rec[5]);
@@ -108,7 +119,7 @@ __global__ __launch_bounds__(256) void k_mlp_data(MlpDev m, int w, int64_t n, ui
             }
         }
         __syncthreads();
-        const int lim = nfit - h0 < kHillChunk ? (int)(nfit - h0) : kHillChunk;
+        const int lim = nh - h0 < kHillChunk ? (int)(nh - h0) : kHillChunk;
         for (int k = 0; k < lim; ++k) {
             const double a = s[0] - hc[4 * k], b = s[1] - hc[4 * k + 1], c = s[2] - hc[4 * k + 2];
             const double e = (a * a * i2s0 + b * b * i2s1 + c * c * i2s2) * hsc[k];
@@ -116,8 +127,12 @@ __global__ __launch_bounds__(256) void k_mlp_data(MlpDev m, int w, int64_t n, ui
         }
     }
     if (p < n) {
-        s_out[p] = make_float4((float)s[0], (float)s[1], (float)s[2], 0.0f);
-        v_out[p] = v;
+        if (mode == 1) {
+            v_out[p] += v;
+        } else {
+            s_out[p] = make_float4((float)s[0], (float)s[1], (float)s[2], 0.0f);
+            v_out[p] = v;
+        }
     }
 }
 
@@ -268,6 +283,9 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
     __shared__ float grad[kMlpP];
     __shared__ float xs[4][16][4];
     const MlpNet* net = m.stage + w;
+    // one training step per launch: its Adam update (the next kernel) reads the advanced
+    // step count, so a captured epoch replays with the right step and learning rate
+    if (blockIdx.x == 0 && threadIdx.x == 0 && m.adam_t) m.adam_t[w] += 1;
     load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
     for (int k = threadIdx.x; k < kMlpP; k += blockDim.x) grad[k] = 0.0f;
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
@@ -404,9 +422,15 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
 // sum of block partials (fixed order) + Adam.  A workgroup covers 64 parameters:
 // thread t sums the partials b = q, q+4, ... (q = t / 64) of parameter t % 64 with
 // independent loads, and the four group sums are added in order.
-// step: 1-based Adam step within the current fit (the moments restart with each fit)
+// step: 1-based Adam step within the current fit (the moments restart with each fit);
+// nit > 0: the step is the device counter m.adam_t[w] and the rate lr follows the cosine
+// schedule lr (0.01 + 0.99 (1 + cos(pi (step - 1) / nit)) / 2) over the nit steps of the fit
 __global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, int w, int nblocks, const float* __restrict__ gpart,
-                                                   float lr, float* __restrict__ gsum_out, int64_t step) {
+                                                   float lr, float* __restrict__ gsum_out, int64_t step, int64_t nit) {
+    if (nit > 0) {
+        step = m.adam_t[w];
+        lr = lr * (float)(0.01 + 0.99 * 0.5 * (1.0 + cos(3.141592653589793 * (double)(step - 1) / (double)nit)));
+    }
     __shared__ float part[4][64];
     MlpNet* net = m.stage + w;
     float* mo = m.adam_m + (int64_t)w * kMlpP;
@@ -557,9 +581,9 @@ __global__ void k_mlp_init(MlpNet* net, uint64_t seed) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t mlp_data(const MlpDev& m, int w, int64_t n, uint64_t seed, float4* s, double* v, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_data, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m, w, n, seed, s, v);
+hipError_t mlp_data(const MlpDev& m, int w, int64_t n, uint64_t seed, float4* s, double* v, int mode, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_data, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m, w, n, seed, s, v, mode);
     return hipGetLastError();
 }
 hipError_t mlp_box(const MlpDev& m, int w, hipStream_t st) {
@@ -571,12 +595,13 @@ hipError_t mlp_stats(const MlpDev& m, int w, int64_t n, const double* v, hipStre
     return hipGetLastError();
 }
 hipError_t mlp_train_step(const MlpDev& m, int w, int64_t first, int64_t count, int64_t n_total, const float4* s,
-                          const double* v, float* gpart, float lr, float* gsum, int64_t step, hipStream_t st) {
+                          const double* v, float* gpart, float lr, float* gsum, int64_t step, int64_t nit,
+                          hipStream_t st) {
     const int nb = (int)((count + 63) / 64);
     hipLaunchKernelGGL(k_mlp_train, dim3(nb), dim3(256), 0, st, m, w, first, count, n_total, s, v, gpart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpP + 63) / 64), dim3(256), 0, st, m, w, nb, gpart, lr, gsum, step);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpP + 63) / 64), dim3(256), 0, st, m, w, nb, gpart, lr, gsum, step, nit);
     return hipGetLastError();
 }
 hipError_t mlp_forward(const MlpNet* net, int64_t n, const float4* s, const double* vref, double* vout,

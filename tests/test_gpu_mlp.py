@@ -144,6 +144,16 @@ def test_mlp_fit_gate_and_recent_hills(mdt, orc):
     # the learned gradient is close to the exact one where the bias matters
     big = np.abs(dVx).max(axis=0)
     assert np.all(np.abs(dV - dVx).max(axis=0) <= 0.1 * big)
+    # a refit keeps its training points and adds the ten new hills to their labels (and
+    # redraws a block around them); its held-out points are labelled afresh by every hill,
+    # so stale labels would show as a held-out error of the new hills' size
+    fix.mlp_refit()
+    st2 = fix.mlp_status()
+    print(f"refit: {st2}", flush=True)
+    assert st2["fits"] == 2 and st2["n_fit"] == 160
+    assert st2["err"] <= 0.1 * KT, st2
+    V3, _ = fix.bias_eval(pts)
+    assert np.abs(V3 - Vx2).max() <= 0.15 * KT
 
 
 def test_mlp_fallback_is_the_exact_sum(mdt, orc):
