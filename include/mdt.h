@@ -96,6 +96,13 @@ int mdt_bind_group(mdt_ctx* ctx, int walker, const uint8_t* in_group);
  * group name without a binding of its own uses the mdt_bind_group membership. */
 int mdt_bind_group_named(mdt_ctx* ctx, int walker, const char* name, const uint8_t* in_group);
 
+/* Atom types of a walker's local atoms (LAMMPS atom->type, 1-based), for the
+ * chemically weighted CVs: STRU_FACTOR ... Chemical (Stru_fact_chem_env,
+ * src/cvs/CVs_Strufactor_chem.cu:29-66,95-123) and WEIGHT_CHEM
+ * (src/cvs/CV_Weighted_Chemical_Pair_Order.cu:32-248).  Ghost images take their
+ * owner's type.  type == NULL unbinds them. */
+int mdt_bind_types(mdt_ctx* ctx, int walker, const int32_t* type);
+
 /* This context's share of a walker's atoms (1 = mine): the atoms whose terms it adds
  * to the CV sums, as one MPI rank adds its local atoms; every atom still takes part
  * (neighbours, q_lm) and receives forces.  mine == NULL: every atom (the default). */

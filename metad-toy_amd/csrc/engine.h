@@ -265,8 +265,10 @@ struct SwFn {
 // STRU_FACTOR (Debye) CV: per-atom S_i over each atom's list within rc.
 struct SfDev {
     int32_t count;            // 0: AVE (sum S_i / N_group); 1: COUNT (sum sw(S_i))
+    int32_t pair;             // pair term g(r): 0 sin(q r)/(q r) s(r^2) (STRU_FACTOR); 1 s(r^2) (WEIGHT_CHEM)
     SwFn sw;
     double rc2, q;
+    const double* aw;         // per atom (locals then ghosts) chemical weight w_type; NULL = 1
     double* S;                // per local atom S_i
     double* part;             // per tile FP64 sum of S_i over group atoms
     const int32_t* tile_begin;  // per walker first tile (n_walkers + 1)

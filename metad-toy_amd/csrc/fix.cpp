@@ -225,6 +225,7 @@ struct WalkerIn {
     // mdt_bind_group, used by any group name without a binding of its own)
     std::map<std::string, std::vector<uint8_t>> groups;
     std::vector<uint8_t> share;       // this context's share of the CV sums (empty: every atom)
+    std::vector<int32_t> types;       // atom types of local atoms (LAMMPS atom->type), empty: unbound
     const double* x_src = nullptr;   // attached engine buffer read at the start of every step
     // after commit
     int64_t atom_off = 0, ghost_off = 0, nnz_off = 0;
@@ -563,6 +564,13 @@ struct CvH {
     std::string sf_group;
     bool sf_has_sw = false;
     SwFn sf_sw{};
+    // chemical weights w_type of STRU_FACTOR ... Chemical and of WEIGHT_CHEM: Chem_map values,
+    // else exp(-(type - Chem_ctarget)^2 / (2 Chem_sigma^2)) (CVs_Strufactor_chem.cu:44-66)
+    int sf_pair = 0;                // 0: sin(q r)/(q r) s(r^2); 1: s(r^2) (WEIGHT_CHEM)
+    bool sf_chem = false;
+    double chem_c = 0.0, chem_sigma = 1.0;
+    std::map<int, double> chem_map;
+    DBuf<double> sf_aw;
     std::string func;       // the SYMBOL function bound to this CAL (AVE unless a SYMBOL says otherwise)
     DBuf<double> sf_S, sf_part, sf_inv;
     DBuf<int32_t> sf_tb;
@@ -801,6 +809,43 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 if (w.type == 4 && !(w.r0 != 0.0)) throw Err(MDT_EINVAL, "RATIONAL r_0 must be nonzero");
                 return w;
             };
+            // the chemical options at argv[i]; false when argv[i] is none of them
+            auto parse_chem = [&](CvH* c) {
+                const std::string k2 = argv[i];
+                if (k2 == "Chemical") { c->sf_chem = true; i += 1; return true; }
+                if (k2 == "Chem_ctarget") {
+                    need(1, "Error: 'Chem_ctarget' keyword requires a number");
+                    c->chem_c = parse_num(argv[i + 1], "Chem_ctarget"); i += 2; return true;
+                }
+                if (k2 == "Chem_sigma") {
+                    need(1, "Error: 'sigma' keyword requires a number");
+                    c->chem_sigma = parse_num(argv[i + 1], "Chem_sigma"); i += 2; return true;
+                }
+                if (k2 != "Chem_map") return false;
+                i += 1;
+                // (type weight) pairs: "(t,w)" or "(t w)" as one word, or "(t" "w)" as two
+                while (i < argc && argv[i][0] == '(') {
+                    std::string a = argv[i];
+                    std::string body;
+                    if (a.size() > 1 && a.back() == ')') {
+                        body = a.substr(1, a.size() - 2);
+                        i += 1;
+                    } else {
+                        if (i + 1 >= argc) throw Err(MDT_EINVAL, fmt("Error: Invalid Chem_map format near %s", argv[i]));
+                        const std::string b = argv[i + 1];
+                        if (b.empty() || b.back() != ')')
+                            throw Err(MDT_EINVAL, "Error: Chem_map pairs must be formatted as (type weight)");
+                        body = a.substr(1) + " " + b.substr(0, b.size() - 1);
+                        i += 2;
+                    }
+                    const size_t sep = body.find_first_of(" ,");
+                    if (sep == std::string::npos)
+                        throw Err(MDT_EINVAL, "Error: Chem_map pairs must be formatted as (type weight)");
+                    const std::string ts = body.substr(0, sep), wsv = body.substr(sep + 1);
+                    c->chem_map[(int)parse_int(ts.c_str(), "Chem_map type")] = parse_num(wsv.c_str(), "Chem_map weight");
+                }
+                return true;
+            };
             auto find_sw = [&](const char* nm) {
                 auto it = fx->swreg.find(nm);
                 if (it == fx->swreg.end()) throw Err(MDT_EINVAL, fmt("Error: SwitchFunction named %s not found in registry!", nm));
@@ -921,8 +966,8 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 c->name = name; c->kind = CV_DIST; c->a1 = (int)(id1 - 1); c->a2 = (int)(id2 - 1);
                 fx->cvs.push_back(c);
             } else if (type == "STRU_FACTOR") {
-                // STRU_FACTOR <group> [cutoff_r r] [q_factor q] [d_block_size b] (CVs_Strufactor.cu:34-150);
-                // the switching-function and chemical variants are not provided
+                // STRU_FACTOR <group> [cutoff_r r] [q_factor q] [d_block_size b] [SW_func ...] [Chemical]
+                // [Chem_ctarget c] [Chem_sigma s] [Chem_map (type weight)...] (CVs_Strufactor.cu:34-190)
                 if (i >= argc) throw Err(MDT_EINVAL, "Error: STRU_FACTOR command requires at least \"STRU_FACTOR <group> \".");
                 CvH* c = new CvH();
                 c->name = name; c->kind = CV_SF; c->sf_group = argv[i++];
@@ -947,8 +992,8 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                             i += 1;
                             c->sf_sw = parse_swspec();
                             c->sf_has_sw = true;
-                        } else if (k2 == "Chemical" || k2 == "Chem_ctarget" || k2 == "Chem_sigma" || k2 == "Chem_weight") {
-                            throw Err(MDT_ENOTSUP, fmt("STRU_FACTOR %s: the chemical variants are not provided", k2.c_str()));
+                        } else if (parse_chem(c)) {
+                            // Chemical, Chem_ctarget, Chem_sigma, Chem_map
                         } else {
                             break;
                         }
@@ -958,7 +1003,37 @@ void parse_fix(mdt_fix* fx, int argc, const char* const* argv) {
                 } catch (...) { delete c; throw; }
                 fx->cvs.push_back(c);
             } else if (type == "WEIGHT_CHEM") {
-                throw Err(MDT_ENOTSUP, fmt("CV type %s is outside this library's scope", type.c_str()));
+                // WEIGHT_CHEM <group> [cutoff_r r] [d_block_size b] [SW_func ...] [Chem_ctarget c]
+                // [Chem_sigma s] [Chem_map (type weight)...] (CV_Weighted_Chemical_Pair_Order.cu:32-172):
+                // per atom 1 + w_i sum_j w_j s(r^2), always chemically weighted
+                if (i >= argc) throw Err(MDT_EINVAL, "Error: CHEM_PAIR command requires at least \"CHEM_PAIR <group> \".");
+                CvH* c = new CvH();
+                c->name = name; c->kind = CV_SF; c->sf_group = argv[i++];
+                c->sf_pair = 1; c->sf_chem = true;
+                try {
+                    while (i < argc) {
+                        std::string k2 = argv[i];
+                        if (k2 == "cutoff_r") {
+                            need(1, "Error: 'cutoff_r' keyword requires a value");
+                            c->sf_rc = parse_num(argv[i + 1], "cutoff_r"); i += 2;
+                        } else if (k2 == "d_block_size") {
+                            need(1, "Error: 'd_block_size' keyword requires an integer");
+                            if (parse_int(argv[i + 1], "d_block_size") <= 0) throw Err(MDT_EINVAL, "Error: 'd_block_size' must be > 0");
+                            i += 2;
+                        } else if (k2 == "SW_func") {
+                            need(1, "Error: 'SW_func' keyword requires a type (FERMI, TANH, RATIONAL)");
+                            i += 1;
+                            c->sf_sw = parse_swspec();
+                            c->sf_has_sw = true;
+                        } else if (k2 != "Chemical" && parse_chem(c)) {
+                            // Chem_ctarget, Chem_sigma, Chem_map
+                        } else {
+                            break;
+                        }
+                    }
+                    if (!(c->sf_rc > 0)) throw Err(MDT_EINVAL, "WEIGHT_CHEM cutoff_r must be positive");
+                } catch (...) { delete c; throw; }
+                fx->cvs.push_back(c);
             } else {
                 throw Err(MDT_EINVAL, fmt("Metad-toy Error: Unknown or unregistered object type '%s' specified in CAL '%s'.",
                                           type.c_str(), name.c_str()));
@@ -1269,6 +1344,28 @@ void fix_build(mdt_fix* fx) {
             std::vector<double> inv(nw);
             for (int k = 0; k < nw; ++k) inv[k] = ngs[k] > 0 ? 1.0 / ngs[k] : 0.0;
             c->sf_inv.alloc(nw); c->sf_inv.upload(inv.data(), nw, st);
+            if (c->sf_chem) {
+                // w of every atom, ghost images taking their owner's type
+                std::vector<double> aw((size_t)std::max<int64_t>(ctx->total_local + ctx->total_ghost, 1), 1.0);
+                auto wt = [&](int t) {
+                    auto it = c->chem_map.find(t);
+                    if (it != c->chem_map.end()) return it->second;
+                    const double dd = (double)t - c->chem_c;
+                    return std::exp(-(dd * dd) / (2.0 * c->chem_sigma * c->chem_sigma));
+                };
+                for (int k = 0; k < nw; ++k) {
+                    const WalkerIn& a = ctx->w[k];
+                    if ((int64_t)a.types.size() != a.n_local)
+                        throw Err(MDT_EINVAL, fmt("CAL %s is chemically weighted: bind the atom types (mdt_bind_types)",
+                                                  c->name.c_str()));
+                    for (int64_t q = 0; q < a.n_local; ++q) aw[(size_t)(a.atom_off + q)] = wt(a.types[q]);
+                    for (int64_t q = 0; q < a.n_all - a.n_local; ++q)
+                        aw[(size_t)(ctx->total_local + a.ghost_off + q)] = wt(a.types[a.owner[q]]);
+                }
+                c->sf_aw.alloc(aw.size()); c->sf_aw.upload(aw.data(), aw.size(), st);
+            } else {
+                c->sf_aw.release();
+            }
             c->sf_S.alloc((size_t)std::max<int64_t>(NT, 1));
             c->sf_part.alloc((size_t)std::max<size_t>(ctx->h_tiles.size(), 1));
             c->sf_tb.alloc(nw + 1); c->sf_tb.upload(ctx->tile_begin.data(), nw + 1, st);
@@ -1751,6 +1848,8 @@ DistDev dist_dev(mdt_fix* fx, CvH* c) {
 SfDev sf_dev(mdt_fix* fx, CvH* c) {
     SfDev d{};
     d.count = c->func == "COUNT" ? 1 : 0;
+    d.pair = c->sf_pair;
+    d.aw = c->sf_chem ? c->sf_aw.p : nullptr;
     d.sw = c->sf_sw;
     d.rc2 = c->sf_rc * c->sf_rc; d.q = c->sf_q;
     d.S = c->sf_S.p; d.part = c->sf_part.p; d.tile_begin = c->sf_tb.p; d.inv_ng = c->sf_inv.p;
@@ -2074,6 +2173,16 @@ int mdt_bind_group_named(mdt_ctx* ctx, int walker, const char* name, const uint8
     if (nm == "all") throw Err(MDT_EINVAL, "group 'all' is every local atom and cannot be bound");
     if (in_group) a.groups[nm].assign(in_group, in_group + a.n_local);
     else a.groups.erase(nm);
+    ctx->committed = false;
+    API_END
+}
+
+int mdt_bind_types(mdt_ctx* ctx, int walker, const int32_t* type) {
+    API_BEGIN(ctx)
+    WalkerIn& a = ctx->walker(walker);
+    if (!a.has_x) throw Err(MDT_ESTATE, "bind positions before the atom types");
+    if (type) a.types.assign(type, type + a.n_local);
+    else a.types.clear();
     ctx->committed = false;
     API_END
 }

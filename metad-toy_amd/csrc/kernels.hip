@@ -1983,6 +1983,22 @@ __device__ __forceinline__ double sf_switch(double r2, double rc2, double r, dou
     return 1.0;
 }
 
+// Per-atom pair sums of STRU_FACTOR (g = sin(q r)/(q r) s, CVs_Strufactor.cu:1082-1120, and
+// with the chemical weights cv_kernel_structure_factor_chem, CVs_Strufactor_chem.cu:151-191)
+// and WEIGHT_CHEM (g = s, Weighted_chem_pair_cv_kernel,
+// CV_Weighted_Chemical_Pair_Order.cu:647-694): S_i = 1 + w_i sum_j w_j g(r_ij) over list
+// entries with 1e-12 <= r^2 <= rc^2; g and dg/dr.
+__device__ __forceinline__ double sf_pair(const SfDev& d, double r2, double* dg) {
+    const double r = sqrt(r2);
+    double ds;
+    const double sw = sf_switch(r2, d.rc2, r, &ds);
+    if (d.pair) { *dg = ds; return sw; }
+    const double th = d.q * r;
+    const double sn = sin(th), cs = cos(th);
+    *dg = (cs / th - sn / (th * th)) * sw * d.q + ds * sn / th;   // d/dr of sin(q r)/(q r) s
+    return sn / th * sw;
+}
+
 __global__ __launch_bounds__(64) void k_sf_value(Geo g, SfDev d) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x;
@@ -1996,16 +2012,16 @@ __global__ __launch_bounds__(64) void k_sf_value(Geo g, SfDev d) {
         const double4 p = g.x4[i];
         double acc = 0.0;
         for (int e = g.off[i]; e < g.off[i + 1]; ++e) {
-            const double4 q = g.x4[g.idx[e]];
+            const int j = g.idx[e];
+            const double4 q = g.x4[j];
             const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
             if ((r2 > d.rc2) || (r2 < 1e-12)) continue;
-            const double r = sqrt(r2), th = d.q * r;
-            double ds;
-            const double sw = sf_switch(r2, d.rc2, r, &ds);
-            acc += sin(th) / th * sw;
+            double dg;
+            const double gv = sf_pair(d, r2, &dg);
+            acc += d.aw ? d.aw[j] * gv : gv;
         }
-        S = acc + 1.0;
+        S = (d.aw ? d.aw[i] * acc : acc) + 1.0;
         d.S[i] = S;
     }
     const bool in_g = valid && (g.group == nullptr || g.group[i]);
@@ -2037,6 +2053,7 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
     const double wc = d.wcv[(int64_t)tl.walker * kWcvStride + d.slot];
     const double W = d.count ? wc : wc * d.inv_ng[tl.walker];
     const double gk = (g.group == nullptr || g.group[k]) ? (d.count ? sw_df(d.sw, d.S[k]) : 1.0) : 0.0;
+    const double wk = d.aw ? d.aw[k] : 1.0;
     const double4 p = g.x4[k];
     double F[3] = {0.0, 0.0, 0.0};
     for (int e = g.off[k]; e < g.off[k + 1]; ++e) {
@@ -2045,11 +2062,10 @@ __global__ __launch_bounds__(64) void k_sf_force(Geo g, SfDev d, double* __restr
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         if ((r2 > d.rc2) || (r2 < 1e-12)) continue;
-        const double r = sqrt(r2), th = d.q * r;
-        const double sn = sin(th), cs = cos(th);
-        double ds;
-        const double sw = sf_switch(r2, d.rc2, r, &ds);
-        const double fp = (cs / th - sn / (th * th)) * sw * d.q + ds * sn / th;   // d/dr of sin(q r)/(q r) s
+        const double r = sqrt(r2);
+        double fp;
+        (void)sf_pair(d, r2, &fp);
+        if (d.aw) fp *= wk * d.aw[j];
         const int jo = owner_of(g, j);
         const double gj = (g.group == nullptr || g.group[jo]) ? (d.count ? sw_df(d.sw, d.S[jo]) : 1.0) : 0.0;
         const double c = (gk + gj) * fp / r;

@@ -50,6 +50,7 @@ def lib():
         "mdt_bind_group": (I, [P, I, u8p]),
         "mdt_bind_group_named": (I, [P, I, C.c_char_p, u8p]),
         "mdt_bind_share": (I, [P, I, u8p]),
+        "mdt_bind_types": (I, [P, I, ip]),
         "mdt_fix_partials": (I, [P, I64, dp, I64, C.POINTER(I64)]),
         "mdt_fix_resume": (I, [P, I64, dp, P, I]),
         "mdt_bind_system": (I, [P, I, P]),
@@ -229,6 +230,10 @@ class Context:
         m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
         self._chk(lib().mdt_bind_group_named(self._h, walker, name.encode(),
                                              None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def bind_types(self, walker, types):
+        t = None if types is None else np.ascontiguousarray(types, np.int32)
+        self._chk(lib().mdt_bind_types(self._h, walker, _ip(t)))
 
     def bind_share(self, walker, mine):
         m = None if mine is None else np.ascontiguousarray(mine, np.uint8)

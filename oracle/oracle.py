@@ -376,10 +376,37 @@ def _sf_switch(r2, rc2, r):
     return 1.0, 0.0
 
 
-def sf_peratom(x_all, off, idx, n, rc, q):
-    """S_i = 1 + sum_j sin(q r)/(q r) s(r^2) over list entries with 1e-12 <= r^2 <= rc^2
-    (get_environment_Strufactor :1014-1080, structure_factor_cv_kernel :1082-1120)."""
+def chem_weights(types, c_target=0.0, sigma=1.0, cmap=None):
+    """Per-atom chemical weight of Stru_fact_chem_env (CVs_Strufactor_chem.cu:44-66): the
+    Chem_map value of the atom's type, else exp(-(type - c_target)^2 / (2 sigma^2))."""
+    cmap = cmap or {}
+    return np.array([cmap[t] if t in cmap else np.exp(-((t - c_target) ** 2) / (2.0 * sigma * sigma))
+                     for t in np.asarray(types, np.int64)], np.float64)
+
+
+def _sf_g(r2, rc2, q, pair):
+    """Pair term g and dg/dr: pair 0, sin(q r)/(q r) s(r^2) (STRU_FACTOR,
+    structure_factor_cv_kernel :1082-1120); pair 1, s(r^2) (WEIGHT_CHEM,
+    Weighted_chem_pair_cv_kernel, CV_Weighted_Chemical_Pair_Order.cu:647-694)."""
+    r = np.sqrt(r2)
+    s, ds = _sf_switch(r2, rc2, r)
+    if pair:
+        return s, ds
+    th = q * r
+    return np.sin(th) / th * s, (np.cos(th) / th - np.sin(th) / th ** 2) * s * q + ds * np.sin(th) / th
+
+
+def _aw_of(aw, n_all):
+    return np.ones(n_all) if aw is None else np.asarray(aw, np.float64)
+
+
+def sf_peratom(x_all, off, idx, n, rc, q, aw=None, pair=0):
+    """S_i = 1 + w_i sum_j w_j g(r_ij) over list entries with 1e-12 <= r^2 <= rc^2
+    (get_environment_Strufactor :1014-1080; the chemical form multiplies the pair term by
+    the two atoms' weights, cv_kernel_structure_factor_chem, CVs_Strufactor_chem.cu:151-191).
+    aw: weights of every atom (locals then ghosts), None = 1."""
     x = np.asarray(x_all, np.float64)
+    w = _aw_of(aw, x.shape[0])
     rc2 = rc * rc
     S = np.zeros(n)
     for i in range(n):
@@ -390,25 +417,24 @@ def sf_peratom(x_all, off, idx, n, rc, q):
             r2 = dx * dx + dy * dy + dz * dz
             if r2 > rc2 or r2 < 1e-12:
                 continue
-            r = np.sqrt(r2)
-            th = q * r
-            s, _ = _sf_switch(r2, rc2, r)
-            acc += np.sin(th) / th * s
-        S[i] = acc + 1.0
+            g, _ = _sf_g(r2, rc2, q, pair)
+            acc += w[j] * g
+        S[i] = w[i] * acc + 1.0
     return S
 
 
-def sf_cv(x_all, off, idx, n, rc, q, group=None):
+def sf_cv(x_all, off, idx, n, rc, q, group=None, aw=None, pair=0):
     """compute_cv_AVE (:642-656): sum of S_i over the group / N_group."""
-    S = sf_peratom(x_all, off, idx, n, rc, q)
+    S = sf_peratom(x_all, off, idx, n, rc, q, aw, pair)
     g = np.ones(n, bool) if group is None else np.asarray(group, bool)
     return float(np.sum(S[g]) / g.sum())
 
 
-def sf_grad(x_all, off, idx, n, own, rc, q, group=None):
+def sf_grad(x_all, off, idx, n, own, rc, q, group=None, aw=None, pair=0):
     """Exact dCV/dx of sf_cv for the n local atoms: pair (k, j) enters S_k for k in the group
-    and S_j for j in it (a ghost j counts as its owner)."""
+    and S_j for j in it (a ghost j counts as its owner), with weight w_k w_j."""
     x = np.asarray(x_all, np.float64)
+    w = _aw_of(aw, x.shape[0])
     g = np.ones(n, bool) if group is None else np.asarray(group, bool)
     rc2 = rc * rc
     out = np.zeros((n, 3))
@@ -420,11 +446,9 @@ def sf_grad(x_all, off, idx, n, own, rc, q, group=None):
             if r2 > rc2 or r2 < 1e-12:
                 continue
             r = np.sqrt(r2)
-            th = q * r
-            s, ds = _sf_switch(r2, rc2, r)
-            fp = (np.cos(th) / th - np.sin(th) / th ** 2) * s * q + ds * np.sin(th) / th
+            _, fp = _sf_g(r2, rc2, q, pair)
             jo = j if j < n else own[j - n]
-            wt = float(g[k]) + float(g[jo])
+            wt = (float(g[k]) + float(g[jo])) * w[k] * w[j]
             out[k] -= wt * fp * d / r
     return out / g.sum()
 
@@ -475,17 +499,18 @@ def sw_df(p, v):
     return 0.0
 
 
-def sf_count(x_all, off, idx, n, rc, q, sw, group=None):
+def sf_count(x_all, off, idx, n, rc, q, sw, group=None, aw=None, pair=0):
     """compute_cv_COUNT (CVs_Strufactor.cu:658-674): sum over the group of sw(S_i)."""
-    S = sf_peratom(x_all, off, idx, n, rc, q)
+    S = sf_peratom(x_all, off, idx, n, rc, q, aw, pair)
     g = np.ones(n, bool) if group is None else np.asarray(group, bool)
     return float(sum(sw_f(sw, S[i]) for i in range(n) if g[i]))
 
 
-def sf_count_grad(x_all, off, idx, n, own, rc, q, sw, group=None):
-    """Exact dCOUNT/dx: pair (k, j) carries [k in G] sw'(S_k) + [j in G] sw'(S_j)."""
+def sf_count_grad(x_all, off, idx, n, own, rc, q, sw, group=None, aw=None, pair=0):
+    """Exact dCOUNT/dx: pair (k, j) carries ([k in G] sw'(S_k) + [j in G] sw'(S_j)) w_k w_j."""
     x = np.asarray(x_all, np.float64)
-    S = sf_peratom(x_all, off, idx, n, rc, q)
+    w = _aw_of(aw, x.shape[0])
+    S = sf_peratom(x_all, off, idx, n, rc, q, aw, pair)
     g = np.ones(n, bool) if group is None else np.asarray(group, bool)
     rc2 = rc * rc
     out = np.zeros((n, 3))
@@ -497,11 +522,9 @@ def sf_count_grad(x_all, off, idx, n, own, rc, q, sw, group=None):
             if r2 > rc2 or r2 < 1e-12:
                 continue
             r = np.sqrt(r2)
-            th = q * r
-            s, ds = _sf_switch(r2, rc2, r)
-            fp = (np.cos(th) / th - np.sin(th) / th ** 2) * s * q + ds * np.sin(th) / th
+            _, fp = _sf_g(r2, rc2, q, pair)
             jo = j if j < n else own[j - n]
-            wt = (sw_df(sw, S[k]) if g[k] else 0.0) + (sw_df(sw, S[jo]) if g[jo] else 0.0)
+            wt = ((sw_df(sw, S[k]) if g[k] else 0.0) + (sw_df(sw, S[jo]) if g[jo] else 0.0)) * w[k] * w[j]
             out[k] -= wt * fp * d / r
     return out
 

@@ -103,9 +103,11 @@ def test_fix_parser_accepts_reference_syntax(mdt):
     ("CAL NAME sw SW_FUNC NOPE", -22),
     ("CAL NAME sw SW_FUNC RATIONAL r_0 0", -22),
     ("CAL NAME sw SW_FUNC STEP CAL NAME sw SW_FUNC FERMI", -22),
-    ("CAL NAME Q6 WEIGHT_CHEM 1 2", -95),
+    ("CAL NAME Q6 WEIGHT_CHEM 1 2", -22),
+    ("CAL NAME W WEIGHT_CHEM all Chem_map (1", -22),
+    ("CAL NAME W WEIGHT_CHEM all Chem_map (1 2", -22),
+    ("CAL NAME W WEIGHT_CHEM all Chem_map (1)", -22),
     ("CAL NAME S STRU_FACTOR all cutoff_r 5", -22),
-    ("CAL NAME S STRU_FACTOR all q_factor 2 Chemical", -95),
     ("CAL NAME S STRU_FACTOR all q_factor 2 SW_func NOPE", -22),
     ("CAL NAME S STRU_FACTOR all q_factor 0", -22),
     ("CAL NAME D DISTANCE 3 3", -22),
@@ -118,6 +120,23 @@ def test_fix_parser_errors(mdt, bad, code):
     with pytest.raises(mdt.MdtError) as e:
         mdt.Fix(ctx, bad)
     assert e.value.code == code
+
+
+def test_chemical_options_parse(mdt):
+    """STRU_FACTOR ... Chemical and WEIGHT_CHEM take Chem_ctarget, Chem_sigma and Chem_map
+    pairs written "(t,w)", "(t w)" or as the two words "(t" "w)" (CVs_Strufactor.cu:125-183)."""
+    ctx = mdt.Context(-1, 1)
+    tail = " CV_dim 1 SYMBOL v1 S.AVE DIM 1 0 10 100 v1"
+    for spec in ("STRU_FACTOR all q_factor 2 Chemical Chem_ctarget 1 Chem_sigma 0.5",
+                 "STRU_FACTOR all q_factor 2 Chemical Chem_map (1,1.0) (2,-0.5)",
+                 "WEIGHT_CHEM all cutoff_r 5 Chem_map (1 1.0) (2 -0.5)",
+                 "WEIGHT_CHEM all Chem_map (1 1.0) (2 -0.5) SW_func FERMI r_0 3"):
+        import shlex
+        words = ["GAUSSIAN", "0.01", "0.005", "10", "CAL", "NAME", "S"] + shlex.split(spec.replace("(1 1.0)", '"(1 1.0)"')) + tail.split()
+        mdt.Fix(ctx, words).close()
+    mdt.Fix(ctx, ["GAUSSIAN", "0.01", "0.005", "10", "CAL", "NAME", "S", "WEIGHT_CHEM", "all", "Chem_map", "(1", "2.0)",
+                  "(3", "0.5)"] + tail.split()).close()
+    ctx.close()
 
 
 def test_fix_semantic_errors(mdt):

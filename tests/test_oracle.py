@@ -600,6 +600,61 @@ def test_stru_factor_gradient_fd(orc):
             assert abs(fd - G[k, a]) <= 1e-7 * max(1.0, np.abs(G).max())
 
 
+def test_chem_weights_gaussian_and_map(orc):
+    """Chem weights: exp(-(t - c)^2 / (2 sigma^2)) for unmapped types (1 at t = c, e^-1/2 one
+    sigma away), the Chem_map value for mapped ones."""
+    w = orc.chem_weights([0, 1, 2, 3], c_target=2.0, sigma=0.5, cmap={3: 0.25})
+    assert w[2] == 1.0 and w[3] == 0.25
+    assert abs(w[1] - np.exp(-2.0)) <= 1e-15 and abs(w[0] - np.exp(-8.0)) <= 1e-15
+    assert orc.chem_weights([1])[0] == np.exp(-0.5)
+
+
+@pytest.mark.parametrize("pair", [0, 1])
+def test_chem_dimer_closed_form_and_scaling(orc, pair):
+    """Two atoms with weights w_0, w_1 at r: S_i = 1 + w_0 w_1 g(r), g = sin(q r)/(q r) s(r^2)
+    (STRU_FACTOR Chemical) or s(r^2) (WEIGHT_CHEM); scaling every weight by c scales S - 1 by
+    c^2 in a cluster."""
+    q, rc = 2.0, 5.0
+    for r in (3.0, 4.7):
+        x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        off, idx = np.array([0, 1, 2]), np.array([1, 0])
+        s = 1.0 if r * r <= 0.8 * rc * rc else 1.0 - ((r * r - 0.8 * rc * rc) / (0.2 * rc * rc)) ** 3
+        g = s if pair else np.sin(q * r) / (q * r) * s
+        S = orc.sf_peratom(x, off, idx, 2, rc, q, aw=[0.3, 0.7], pair=pair)
+        assert np.allclose(S, 1.0 + 0.21 * g, rtol=0, atol=1e-15)
+    x, off, idx = _cluster(10, 23)
+    aw = np.random.default_rng(3).uniform(0.2, 1.0, 10)
+    S1 = orc.sf_peratom(x, off, idx, 10, rc, q, aw=aw, pair=pair)
+    S3 = orc.sf_peratom(x, off, idx, 10, rc, q, aw=3.0 * aw, pair=pair)
+    assert np.allclose(S3 - 1.0, 9.0 * (S1 - 1.0), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("pair", [0, 1])
+def test_chem_gradients_fd(orc, pair):
+    """AVE and COUNT gradients with unequal weights equal the central FD of the CV, at group
+    and non-group atoms."""
+    x, off, idx = _cluster(12, 29, L=5.5)
+    n = 12
+    aw = orc.chem_weights(np.arange(n) % 3 + 1, c_target=2.0, sigma=0.8, cmap={3: 0.4})
+    grp = (np.arange(n) % 4 != 1)
+    q, rc = 1.7, 5.0
+    sw = dict(type="FERMI", r0=1.3, alpha=5.0)
+    none = np.zeros(0, int)
+    Ga = orc.sf_grad(x, off, idx, n, none, rc, q, grp, aw=aw, pair=pair)
+    Gc = orc.sf_count_grad(x, off, idx, n, none, rc, q, sw, grp, aw=aw, pair=pair)
+    h = 1e-6
+    for k in (0, 1, 6, 11):
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[k, a] += h
+            xm[k, a] -= h
+            fa = (orc.sf_cv(xp, off, idx, n, rc, q, grp, aw, pair) - orc.sf_cv(xm, off, idx, n, rc, q, grp, aw, pair)) / (2 * h)
+            fc = (orc.sf_count(xp, off, idx, n, rc, q, sw, grp, aw, pair)
+                  - orc.sf_count(xm, off, idx, n, rc, q, sw, grp, aw, pair)) / (2 * h)
+            assert abs(fa - Ga[k, a]) <= 1e-7 * max(1.0, np.abs(Ga).max())
+            assert abs(fc - Gc[k, a]) <= 1e-7 * max(1.0, np.abs(Gc).max())
+
+
 def test_switching_functions_derivatives_and_values(orc):
     """Each df is the FD of its f (every RATIONAL branch, the n = 2m one included, which is
     1 + x^m in the reference); FERMI and TANH are 1/2 at r_0, RATIONAL 1 at d_0 (m = 2n)."""
