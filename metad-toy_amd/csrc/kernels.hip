@@ -346,10 +346,25 @@ __global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4
 // 3 FP64 each) packed into the context's concatenated layout: locals at x4[loc..],
 // ghosts at x4[gh..].  Runs at the start of a step when the walker's positions are
 // read from an attached engine buffer, and in mdt_update_positions.
-__global__ void k_pack_walker(const double* __restrict__ x, int64_t n_local, int64_t n_all, double4* __restrict__ x4_loc,
-                              float4* __restrict__ x4f_loc, double4* __restrict__ x4_gh, float4* __restrict__ x4f_gh) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_all; i += (int64_t)gridDim.x * blockDim.x) {
-        const double a = x[3 * i], b = x[3 * i + 1], c = x[3 * i + 2];
+// A workgroup takes 256 atoms at a time: their 768 coordinates are read with
+// unit-stride loads (each load instruction touches consecutive lines) into LDS,
+// then each thread packs its own atom.
+__global__ __launch_bounds__(256) void k_pack_walker(const double* __restrict__ x, int64_t n_local, int64_t n_all,
+                                                     double4* __restrict__ x4_loc, float4* __restrict__ x4f_loc,
+                                                     double4* __restrict__ x4_gh, float4* __restrict__ x4f_gh) {
+    __shared__ double xs[3 * 256];
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < n_all; base += (int64_t)gridDim.x * 256) {
+        const int64_t nd = 3 * (n_all - base < 256 ? n_all - base : 256);
+        __syncthreads();                               // the previous chunk's reads are complete
+        #pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int k = threadIdx.x + 256 * u;
+            if (k < nd) xs[k] = x[3 * base + k];
+        }
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        if (i >= n_all) continue;
+        const double a = xs[3 * threadIdx.x], b = xs[3 * threadIdx.x + 1], c = xs[3 * threadIdx.x + 2];
         const float af = (float)a, bf = (float)b, cf = (float)c;
         const uint32_t w = pack_res1(a, af) | (pack_res1(b, bf) << 10) | (pack_res1(c, cf) << 20);
         const bool loc = i < n_local;
@@ -1494,7 +1509,7 @@ __device__ __noinline__ void expr_eval(const ExprProg& p, int nsym, const double
 // (ac = lane c's ds_d/dv_c); V, dV/ds and s are taken from lane 0.
 __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, double cvv, const double* ac,
                                               const double* sv, double V, const double* dV, double hill,
-                                              int64_t step, double mlp_err, int mlp_ok) {
+                                              int64_t step, double mlp_err, int mlp_ok, double inv_c) {
     V = readlane_f64(V, 0);
     double dv[kMaxDim];
     #pragma unroll
@@ -1510,7 +1525,7 @@ __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, 
     // the weight sum of each representative slot over the slots aliased to it, in slot
     // order, each weight times its CV's 1/N_group (a slot that nothing aliases gets its
     // own): the force passes of per-group averages take it
-    const double wn = c < kMaxSlotCV ? wc * r.cv_inv[(int64_t)w * kMaxSlotCV + c] : 0.0;
+    const double wn = c < kMaxSlotCV ? wc * inv_c : 0.0;
     double comb = 0.0;
     #pragma unroll
     for (int q = 0; q < kMaxSlotCV; ++q) {
@@ -1576,6 +1591,8 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     #pragma unroll
     for (int c = 0; c < kMaxSlotCV; ++c) if (lane64 == c) src_c = r.cv_src[c];
     const double direct_c = slot_lane && lane64 < r.ncv && src_c == -1 ? r.direct[(int64_t)w * kMaxSlotCV + lane64] : 0.0;
+    // lane c's 1/N, loaded with the first round trip
+    const double inv_c = slot_lane ? r.cv_inv[(int64_t)w * kMaxSlotCV + lane64] : 0.0;
     double acoef[kMaxDim] = {0, 0, 0};
     if (r.affine && slot_lane) {
         #pragma unroll
@@ -1649,7 +1666,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
                 double a = 0.0;
                 #pragma unroll
                 for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
-                cvv = a * r.cv_inv[(int64_t)w * kMaxSlotCV + c];
+                cvv = a * inv_c;
             } else {
                 // direct values; CVs summed after the force pass (cv_src <= -2) are
                 // unbiased and reduced when the state is read
@@ -1690,7 +1707,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
             // no further workgroup barrier (uniform over the block)
             double V = 0.0, dVg[kMaxDim] = {0, 0, 0};
             if (c == 0) V = grid_eval(r, grid_in_lds ? gl : r.grid + (int64_t)w * r.grid_size, sv, dVg);
-            reduce_finish(r, w, c, cvv, acoef, sv, V, dVg, 0.0, step, mlp_err, mlp_ok);
+            reduce_finish(r, w, c, cvv, acoef, sv, V, dVg, 0.0, step, mlp_err, mlp_ok, inv_c);
             return;
         }
     } else if (r.kind == BIAS_GRID && !dep) {
@@ -1734,7 +1751,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         double ac[kMaxDim] = {0, 0, 0}, sv[kMaxDim] = {0, 0, 0};
         #pragma unroll
         for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { ac[d] = c < kMaxSlotCV ? acoef_s[d][c] : 0.0; sv[d] = s[d]; }
-        reduce_finish(r, w, c, cvv, ac, sv, V, dV, dep ? bc[0] : 0.0, step, mlp_err, mlp_ok);
+        reduce_finish(r, w, c, cvv, ac, sv, V, dV, dep ? bc[0] : 0.0, step, mlp_err, mlp_ok, inv_c);
     }
 }
 
