@@ -410,19 +410,18 @@ def run_lq(mdt, args):
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
-        sm = mdt.System().add_fcc(10, 10, 10, A, 0.6, 0.1, 10 * A / 2.0, seed=2024).build(RG)
-        xs, offs, idxs, owns = sm.arrays()
         swr = dict(type="RATIONAL", r0=3.0, d0=0.0, n=6, m=12)
         swc = dict(type="FERMI", r0=0.35, alpha=10.0)
 
         def one():
-            orc.lq_cv(xs, offs, idxs, sm.n_local, owns, 6, swr, swc, 1e-6, "MEAN_SOLID")
-            orc.lq_grad(xs, offs, idxs, sm.n_local, owns, 6, swr, swc, 1e-6, "MEAN_SOLID")
-        k, tc = oracle_steps(one, min(args.cpu_budget, 20.0))
-        cpu = {"value": sm.n_local * k / tc, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{k} step(s) (CV and exact gradient) of a {sm.n_local}-atom system built the same way "
-                         f"(10^3 FCC cells, lower half jittered 0.6 A) by the oracle (Python loops over list "
-                         f"entries, Y_lm in C) on 1 core, {tc:.1f} s"}
+            orc.lq_c(x, off, idx, n, own, 6, swr, swc, 1e-6, "MEAN_SOLID")
+        k, tc = oracle_steps(one, args.cpu_budget)
+        ci = cpu_info(orc)
+        cpu = {"value": n * k / tc, "unit": "atom-steps/s", "cores": ci["cores"], "kind": "port",
+               "sample": f"{k} full step(s) (CV and exact gradient) of the same {n}-atom workload by the FP64 C "
+                         f"oracle, OpenMP over atoms on {ci['cores']} threads ({ci['cpu_model']}, "
+                         f"{ci['host_logical_cpus']} logical CPUs), {tc:.1f} s",
+               "cpu_model": ci["cpu_model"], "host_logical_cpus": ci["host_logical_cpus"]}
     cfg = {"workload": "STEINH L 6 MEAN_SOLID on the 256,000-atom half-melted FCC system (RATIONAL r_0 3.0 "
                        "weights on the 5.9 A list, FERMI r_0 0.35 alpha 10 on LQ_6), DIM [0,2]x2000 WT",
            "atoms": n, "list_entries": entries, "cv_value": fix.cv("LQ.MEAN_SOLID"), "kernels_ms": kms,

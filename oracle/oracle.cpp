@@ -746,3 +746,182 @@ double orc_mlp_loss_grad(int Din, int H, const double* W1, const double* b1, con
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- STEINH L (OpenMP)
+// Switching function of a keyword: p = {type, r0, d0, alpha, n, m}, type 0 LINE,
+// 1 STEP, 2 FERMI, 3 TANH, 4 RATIONAL (include/utils/zqc_switch_function.h:216-337;
+// the same branches as oracle.py's sw_f / sw_df, which tests pin against each other).
+static double lq_sw_f(const double* p, double v) {
+    const int t = (int)p[0];
+    if (t == 1) return v < p[1] ? 1.0 : 0.0;
+    if (t == 2) return 1.0 / (1.0 + std::exp(-p[3] * (v - p[1])));
+    if (t == 3) return 0.5 * (1.0 + std::tanh(p[3] * (v - p[1])));
+    if (t == 4) {
+        const int n = (int)p[4], m = (int)p[5];
+        const double x = (v - p[2]) / p[1];
+        if (2 * n == m) return 1.0 / (1.0 + std::pow(x, n));
+        if (n == 2 * m) return 1.0 + std::pow(x, m);
+        return (1.0 - std::pow(x, n)) / (1.0 - std::pow(x, m));
+    }
+    return 1.0;
+}
+static double lq_sw_df(const double* p, double v) {
+    const int t = (int)p[0];
+    if (t == 2) { const double e = std::exp(-p[3] * (v - p[1])); return p[3] * e / ((1.0 + e) * (1.0 + e)); }
+    if (t == 3) { const double th = std::tanh(p[3] * (v - p[1])); return 0.5 * p[3] * (1.0 - th * th); }
+    if (t == 4) {
+        const int n = (int)p[4], m = (int)p[5];
+        const double r0 = p[1], x = (v - p[2]) / r0;
+        if (2 * n == m) { const double xn = std::pow(x, n); return -n * std::pow(x, n - 1) / ((1.0 + xn) * (1.0 + xn)) / r0; }
+        if (n == 2 * m) return m * std::pow(x, m - 1) / r0;
+        const double xm = std::pow(x, m);
+        if (std::fabs(1.0 - xm) < 1e-12) return 0.5 * (n * (n - m)) / m / r0;
+        return ((n - m) * std::pow(x, n - 1) * xm + m * std::pow(x, m - 1) - n * std::pow(x, n - 1)) /
+               ((1.0 - xm) * (1.0 - xm)) / r0;
+    }
+    return 0.0;
+}
+
+extern "C" {
+// The STEINH L CV and its exact gradient (oracle.py lq_cv / lq_grad restated in C with
+// OpenMP over atoms; src/cvs/CVs_Steinhardt_Local.cu:1210-1285 environment, :1310-1452
+// q_lm, :1490-1579 LQ_l, :694-745 the MEAN_SOLID / NSOLID / FRAC reductions).  mode 0
+// MEAN_SOLID, 1 NSOLID, 2 FRAC; group NULL = every local atom; grad (n x 3) may be NULL.
+// Sums over atoms are taken in atom order, so the result does not depend on the thread
+// count; the gradient's scatter goes through per-thread arrays added in thread order.
+double orc_lq(int l, int n, const double* x, const int* off, const int* idx, const int* own,
+              const unsigned char* group, const double* swr, const double* swcv, double eps, int mode,
+              double* grad) {
+    const int L1 = l + 1;
+    std::vector<double> S(n, 0.0), q(2 * (size_t)n * L1, 0.0), LQ(2 * (size_t)n * L1, 0.0), ql(n, 0.0);
+    std::vector<unsigned char> live(n, 0);
+    auto inG = [&](int i) { return group == nullptr || group[i] != 0; };
+    auto entry = [&](int a, int e, double* d, double& r) {
+        const int j = idx[e];
+        d[0] = x[3 * j] - x[3 * a]; d[1] = x[3 * j + 1] - x[3 * a + 1]; d[2] = x[3 * j + 2] - x[3 * a + 2];
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 < 1e-12) return -1.0;
+        r = std::sqrt(r2);
+        const double s = lq_sw_f(swr, r);
+        return s < eps ? -1.0 : s;
+    };
+    // per-atom S and q_lm
+#pragma omp parallel for schedule(static)
+    for (int a = 0; a < n; ++a) {
+        double Y[14], acc[14] = {0};
+        int cnt = 0;
+        for (int e = off[a]; e < off[a + 1]; ++e) {
+            double d[3], r;
+            const double s = entry(a, e, d, r);
+            if (s < 0.0) continue;
+            ++cnt;
+            S[a] += s;
+            ylm_reference(l, d[0], d[1], d[2], Y);
+            for (int k = 0; k < 2 * L1; ++k) acc[k] += s * Y[k];
+        }
+        live[a] = cnt > 0 && S[a] >= 1e-12;
+        for (int k = 0; k < 2 * L1; ++k) q[(size_t)a * 2 * L1 + k] = live[a] ? acc[k] * (1.0 / S[a]) : 0.0;
+    }
+    // LQ_lm and LQ_l of the group atoms
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        if (!(inG(i) && live[i])) continue;
+        double acc[14];
+        for (int k = 0; k < 2 * L1; ++k) acc[k] = q[(size_t)i * 2 * L1 + k];
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            double d[3], r;
+            const double s = entry(i, e, d, r);
+            if (s < 0.0) continue;
+            const int o = owner_of(idx[e], n, own);
+            for (int k = 0; k < 2 * L1; ++k) acc[k] += s * q[(size_t)o * 2 * L1 + k];
+        }
+        double s2 = 0.0;
+        for (int m = 0; m <= l; ++m) {
+            const double re = acc[2 * m] / (S[i] + 1.0), im = acc[2 * m + 1] / (S[i] + 1.0);
+            LQ[(size_t)i * 2 * L1 + 2 * m] = re;
+            LQ[(size_t)i * 2 * L1 + 2 * m + 1] = im;
+            s2 += (m > 0 ? 2.0 : 1.0) * (re * re + im * im);
+        }
+        ql[i] = std::sqrt(s2 * 4.0 * M_PI / (2 * l + 1));
+    }
+    double Sq = 0.0, Sf = 0.0;
+    int N = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!inG(i)) continue;
+        const double f = lq_sw_f(swcv, ql[i]);
+        Sq += ql[i] * f; Sf += f; ++N;
+    }
+    const double cv = mode == 0 ? (Sf != 0.0 ? Sq / Sf : 0.0) : mode == 1 ? Sf : (N > 0 ? Sf / N : 0.0);
+    if (grad == nullptr) return cv;
+    // K_i = w_i 4 pi/(2l+1) c_m LQ_lm / LQ_l / (S_i + 1), H = K + scattered sigma K
+    std::vector<double> K(2 * (size_t)n * L1, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        if (!(inG(i) && live[i] && ql[i] > 0.0)) continue;
+        const double f = lq_sw_f(swcv, ql[i]), fp = lq_sw_df(swcv, ql[i]);
+        const double w = mode == 0 ? (Sf != 0.0 ? (f + fp * (ql[i] - cv)) / Sf : 0.0) : mode == 1 ? fp : (N > 0 ? fp / N : 0.0);
+        for (int m = 0; m <= l; ++m)
+            for (int c = 0; c < 2; ++c)
+                K[(size_t)i * 2 * L1 + 2 * m + c] = w * 4.0 * M_PI / (2 * l + 1) * (m > 0 ? 2.0 : 1.0) *
+                                                     LQ[(size_t)i * 2 * L1 + 2 * m + c] / ql[i] / (S[i] + 1.0);
+    }
+    std::vector<double> H(K);
+    for (int i = 0; i < n; ++i) {
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            double d[3], r;
+            const double s = entry(i, e, d, r);
+            if (s < 0.0) continue;
+            const int o = owner_of(idx[e], n, own);
+            for (int k = 0; k < 2 * L1; ++k) H[(size_t)o * 2 * L1 + k] += s * K[(size_t)i * 2 * L1 + k];
+        }
+    }
+    const int nt = orc_num_threads();
+    std::vector<double> acc((size_t)nt * n * 3, 0.0);
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num();
+#else
+        const int tid = 0;
+#endif
+        double* out = acc.data() + (size_t)tid * n * 3;
+#pragma omp for schedule(static)
+        for (int a = 0; a < n; ++a) {
+            if (!live[a]) continue;
+            const double* Ha = &H[(size_t)a * 2 * L1];
+            const double* Ka = &K[(size_t)a * 2 * L1];
+            const double* qa = &q[(size_t)a * 2 * L1];
+            const double* LQa = &LQ[(size_t)a * 2 * L1];
+            for (int e = off[a]; e < off[a + 1]; ++e) {
+                double d[3], r;
+                const double s = entry(a, e, d, r);
+                if (s < 0.0) continue;
+                const double ds = lq_sw_df(swr, r);
+                const int o = owner_of(idx[e], n, own);
+                const double* qo = &q[(size_t)o * 2 * L1];
+                double Y[14], dY[42];
+                ylm_reference(l, d[0], d[1], d[2], Y);
+                ylm_grad(l, d[0], d[1], d[2], dY);
+                double hy = 0.0, kq = 0.0, hd[3] = {0, 0, 0};
+                for (int m = 0; m <= l; ++m) {
+                    hy += Ha[2 * m] * (Y[2 * m] - qa[2 * m]) + Ha[2 * m + 1] * (Y[2 * m + 1] - qa[2 * m + 1]);
+                    kq += Ka[2 * m] * (qo[2 * m] - LQa[2 * m]) + Ka[2 * m + 1] * (qo[2 * m + 1] - LQa[2 * m + 1]);
+                    for (int c = 0; c < 3; ++c) hd[c] += Ha[2 * m] * dY[m * 6 + 2 * c] + Ha[2 * m + 1] * dY[m * 6 + 2 * c + 1];
+                }
+                for (int c = 0; c < 3; ++c) {
+                    const double g = (ds * hy * d[c] / r + s * hd[c]) / S[a] + ds * kq * d[c] / r;
+                    out[3 * a + c] -= g;
+                    out[3 * o + c] += g;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 3 * n; ++k) {
+        double v = 0.0;
+        for (int t = 0; t < nt; ++t) v += acc[(size_t)t * n * 3 + k];
+        grad[k] = v;
+    }
+    return cv;
+}
+
+}  // extern "C"

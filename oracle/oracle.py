@@ -55,6 +55,7 @@ def lib():
             ("orc_hills_sum", None, [I, I, P, P, P, I, P, P, P]),
             ("orc_mlp_forward", None, [I, I, P, P, P, P, P, D, I, P, P, P]),
             ("orc_mlp_loss_grad", D, [I, I, P, P, P, P, P, D, I, P, P, P, P, P, P, P, P]),
+            ("orc_lq", D, [I, I, P, P, P, P, P, P, P, D, I, P]),
         ]:
             f = getattr(_L, name)
             f.restype = res
@@ -592,6 +593,26 @@ def _lq_reduce(st, swcv, mode):
     else:
         cv = Sf / N if N > 0 else 0.0
     return cv, Sf, N
+
+
+_SW_CODE = {"LINE": 0, "STEP": 1, "FERMI": 2, "TANH": 3, "RATIONAL": 4}
+
+
+def _sw_vec(p):
+    return np.array([_SW_CODE[p["type"]], p.get("r0", 0.0), p.get("d0", 0.0), p.get("alpha", 0.0),
+                     p.get("n", 0), p.get("m", 0)], np.float64)
+
+
+def lq_c(x_all, off, idx, n, own, l, swr, swcv, eps=1e-6, mode="MEAN_SOLID", group=None, grad=True):
+    """(CV, dCV/dx or None): lq_cv and lq_grad restated in C with OpenMP over atoms
+    (orc_lq), for systems the Python loops take too long on."""
+    x = _f64(x_all).reshape(-1, 3)
+    o, ix, ow = _i32(off), _i32(idx), _i32(own if len(own) else np.zeros(1, np.int32))
+    g = None if group is None else np.ascontiguousarray(group, np.uint8)
+    G = np.zeros((n, 3)) if grad else None
+    pr, pc = _sw_vec(swr), _sw_vec(swcv)          # held until the call returns
+    cv = lib().orc_lq(l, n, _p(x), _p(o), _p(ix), _p(ow), _p(g), _p(pr), _p(pc), eps, LQ_MODES.index(mode), _p(G))
+    return cv, G
 
 
 def lq_cv(x_all, off, idx, n, own, l, swr, swcv, eps=1e-6, mode="MEAN_SOLID", group=None):

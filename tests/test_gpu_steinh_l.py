@@ -177,3 +177,30 @@ def test_steinh_l_on_device_built_lists(mdt):
     assert np.abs(out[0][0]).max() > 0.0
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+def test_steinh_l_256k_half_melted_against_c_oracle(mdt, orc):
+    """The 256,000-atom half-melted FCC system (40^3 cells, lower half jittered 0.6 A) under
+    STEINH L 6 MEAN_SOLID, value and forces against the OpenMP C restatement of the oracle
+    (itself equal to the Python STEINH L oracle to 1e-12, test_oracle.py)."""
+    from mdt.workloads import c2_system
+    s = c2_system(mdt)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    t0 = time.perf_counter()
+    cv, G = orc.lq_c(x, off, idx, n, own, 6, SWR[1], SWC[1], 1e-6, "MEAN_SOLID")
+    print(f"[steinh L 256k] oracle {time.perf_counter() - t0:.1f} s, cv={cv:.10f}", flush=True)
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    fix = mdt.Fix(ctx, _deck(6, "MEAN_SOLID", "all"))
+    fix.init()
+    fix.add_hills(np.array([[cv + 0.02]]), np.array([0.005]), 0.05)
+    f = np.zeros((n, 3))
+    fix.post_force(0, f)
+    got = fix.cv("LQ.MEAN_SOLID")
+    assert abs(got - cv) <= 1e-12 * abs(cv), (got, cv)
+    w = fix.cv_weight("LQ.MEAN_SOLID")
+    assert w != 0.0
+    assert rel_force_err(f, -w * G) <= 1e-9
+    fix.close()
+    ctx.close()

@@ -777,3 +777,33 @@ def test_lq_gradient_fd_periodic_with_ghosts(mdt, orc):
             cm = orc.lq_cv(mv.set_local(xl), off, idx, n, own, 6, swr, swcv, 1e-6, "MEAN_SOLID", grp)
             assert abs((cp - cm) / (2 * h) - G[k, a]) <= 2e-7 * np.abs(G).max(), (k, a)
     assert np.abs(G.sum(axis=0)).max() <= 1e-12 * np.abs(G).max() * n
+
+
+@pytest.mark.parametrize("mode", ["MEAN_SOLID", "NSOLID", "FRAC"])
+def test_lq_c_restatement_matches_python(orc, mode):
+    """orc_lq (C, OpenMP) equals the Python STEINH L oracle, value and gradient, on a
+    cluster with a group and a RATIONAL r weight, and on the periodic lattice deck."""
+    x, off, idx = _cluster(16, 41, L=6.5)
+    n = 16
+    grp = (np.arange(n) % 5 != 2)
+    swr = dict(type="RATIONAL", r0=3.0, d0=0.0, n=6, m=12)
+    swc = dict(type="FERMI", r0=0.3, alpha=12.0)
+    none = np.zeros(0, int)
+    cv_p = orc.lq_cv(x, off, idx, n, none, 6, swr, swc, 1e-6, mode, grp)
+    G_p = orc.lq_grad(x, off, idx, n, none, 6, swr, swc, 1e-6, mode, grp)
+    cv_c, G_c = orc.lq_c(x, off, idx, n, none, 6, swr, swc, 1e-6, mode, grp)
+    assert abs(cv_c - cv_p) <= 1e-12 * max(1.0, abs(cv_p))
+    assert np.abs(G_c - G_p).max() <= 1e-11 * max(1.0, np.abs(G_p).max())
+
+
+def test_lq_c_periodic_matches_python(mdt, orc):
+    s = fcc_system(mdt, 3, sigma=0.2, seed=77)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    swr = dict(type="RATIONAL", r0=3.0, d0=0.0, n=6, m=12)
+    swc = dict(type="TANH", r0=0.35, alpha=9.0)
+    cv_p = orc.lq_cv(x, off, idx, n, own, 4, swr, swc, 1e-6, "MEAN_SOLID")
+    G_p = orc.lq_grad(x, off, idx, n, own, 4, swr, swc, 1e-6, "MEAN_SOLID")
+    cv_c, G_c = orc.lq_c(x, off, idx, n, own, 4, swr, swc, 1e-6, "MEAN_SOLID")
+    assert abs(cv_c - cv_p) <= 1e-12 * max(1.0, abs(cv_p))
+    assert np.abs(G_c - G_p).max() <= 1e-11 * max(1.0, np.abs(G_p).max())
