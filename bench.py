@@ -243,8 +243,10 @@ def run_c1(mdt, args):
             env.q_ave(6)
             env.dQ_dx(6, 1.0)
         k, t = oracle_steps(one, min(args.cpu_budget, 10.0))
-        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{k} steps of the 4,000-atom Q6 case (selection, Q6.AVE, exact forces), FP64 oracle, {t:.1f} s"}
+        ci = cpu_info(orc)
+        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": ci["cores"], "kind": "port",
+               "sample": f"{k} steps of the 4,000-atom Q6 case (selection, Q6.AVE, exact forces), FP64 oracle, "
+                         f"OpenMP over atoms on {ci['cores']} threads, {t:.1f} s"}
     cfg = {"workload": "config 1: 4,000-atom FCC Ar (sigma 0.1 A), Q6.AVE on [0,1]x1000, PACE 10, WT 0, "
                        "10,000 steps", "atoms": n, "kernels_ms": kms}
     return line(value, steps, args.warmup, dt * 1e3 / steps, "f32 pair math, f64 sums/grid", cfg, None, cpu,
@@ -288,7 +290,7 @@ def run_c3(mdt, args):
             t = time.perf_counter() - t0
             if t >= budget:
                 break
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cores = cpu_info(orc)["cores"]
         # a full step is 2,056 vectors: scale the measured vectors per second to a step
         cpu = {"value": n / (t * nk / done_k), "unit": "atom-steps/s", "cores": cores, "kind": "port",
                "sample": f"S(k) and its gradient for {done_k} k vectors of the 500,000-atom slab (FP64 oracle, "
@@ -325,9 +327,10 @@ def run_c4(mdt, args):
             env.q_ave(4), env.q_ave(6)
             -(env.dQ_dx(4, 1.0) + env.dQ_dx(6, 1.0))
         k, t = oracle_steps(one, min(args.cpu_budget, 10.0))
-        cpu = {"value": s0.n_local * k / t, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{k} steps of walker 0 ({s0.n_local} atoms, Q4+Q6 and forces), FP64 oracle, {t:.1f} s; "
-                         "the oracle runs walkers one after another"}
+        ci = cpu_info(orc)
+        cpu = {"value": s0.n_local * k / t, "unit": "atom-steps/s", "cores": ci["cores"], "kind": "port",
+               "sample": f"{k} steps of walker 0 ({s0.n_local} atoms, Q4+Q6 and forces), FP64 oracle, OpenMP over "
+                         f"atoms on {ci['cores']} threads, {t:.1f} s; the oracle runs walkers one after another"}
     cfg = {"workload": "config 4: 64 walkers, 2k-64k atoms each (FCC sigma 0.1 or liquid 0.021/A^3), "
                        "2-D grids 200x200 on (Q4.AVE, Q6.AVE), WT, one graph launch per step",
            "atoms_total": ntot, "walkers": len(systems), "kernels_ms": kms}
