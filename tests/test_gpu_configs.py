@@ -63,11 +63,12 @@ def test_c2_full_size(mdt, orc):
     g = orc.Grid([0.0], [2.0], [2000])
     for c, h in zip(hc, hh):
         g.deposit(c, h, 0.01)
-    worst = 0.0
+    worst = cvw = 0.0
     heights = []
     for step in range(11):
         f = np.zeros((n, 3))
         fix.post_force(step, f)
+        cvw = max(cvw, *(abs(fix.cv(k) - r) / r for k, r in (("Q6.AVE", q6), ("Q4.AVE", q4), ("LQ6.AVE", lq6))))
         assert abs(fix.cv("Q6.AVE") - q6) <= CV_TOL * q6, step
         assert abs(fix.cv("Q4.AVE") - q4) <= CV_TOL * q4, step
         assert abs(fix.cv("LQ6.AVE") - lq6) <= CV_TOL * lq6, step
@@ -82,7 +83,7 @@ def test_c2_full_size(mdt, orc):
         e = rel_force_err(f, -w * G)
         worst = max(worst, e)
         assert e <= F_TOL, (step, e)
-    print(f"[c2] worst force rel err {worst:.3e}", flush=True)
+    print(f"[c2] worst CV rel err {cvw:.3e}, worst force rel err {worst:.3e}", flush=True)
     hills = fix.hills(1)             # the deposits of the run (added hills go to the grid only)
     assert list(hills[:, 0]) == [0.0, 10.0]
     assert np.abs(hills[:, 1] - s_ref).max() <= CV_TOL * s_ref
@@ -117,6 +118,7 @@ def test_c3_full_size(mdt, orc):
     f = np.zeros((n, 3))
     fix.post_force(1, f)
     assert abs(fix.cv("SK.AVE") - v_ref) <= CV_TOL * abs(v_ref)
+    print(f"[c3] CV rel err {abs(fix.cv('SK.AVE') - v_ref) / abs(v_ref):.3e}", flush=True)
     w = fix.cv_weight("SK.AVE")
     assert w != 0.0
     F_ref = -orc.sk_grad(x, k, w, A)
@@ -148,9 +150,10 @@ def test_c4_full_size(mdt, orc):
     f = np.zeros((sum(ns), 3))
     fix.post_force(0, f)
     a = 0
-    worst = 0.0
+    worst = cvw = 0.0
     for w, (env, q4, q6, g) in enumerate(refs):
         n = ns[w]
+        cvw = max(cvw, abs(fix.cv("Q4.AVE", w) - q4) / q4, abs(fix.cv("Q6.AVE", w) - q6) / q6)
         assert abs(fix.cv("Q4.AVE", w) - q4) <= CV_TOL * q4, w
         assert abs(fix.cv("Q6.AVE", w) - q6) <= CV_TOL * q6, w
         s_ref = [q4, q6]
@@ -163,7 +166,7 @@ def test_c4_full_size(mdt, orc):
         worst = max(worst, e)
         assert e <= F_TOL, (w, e)
         a += n
-    print(f"[c4] worst force rel err {worst:.3e}", flush=True)
+    print(f"[c4] worst CV rel err {cvw:.3e}, worst force rel err {worst:.3e}", flush=True)
 
 
 def test_c5_full_size_exact_hills(mdt, orc):
@@ -203,5 +206,7 @@ def test_c5_full_size_exact_hills(mdt, orc):
         assert abs(wg - wr) <= 1e-9 * np.abs(hh).max() / sg, (wg, wr)
     F_ref = -(env.dQ_dx(4, wq4) + env.dQ_dx(6, wq6) + orc.sk_grad(x[:n], k, ws, A))
     e = rel_force_err(f, F_ref)
-    print(f"[c5] force rel err {e:.3e}", flush=True)
+    cvw = max(abs(fix.cv(k) - r) / r for k, r in (("Q4.AVE", q4), ("Q6.AVE", q6), ("S111.AVE", v_sk)))
+    print(f"[c5] CV rel err {cvw:.3e}, force rel err {e:.3e}, V rel err {abs(fix.scalar() - V[0]) / abs(V[0]):.3e}",
+          flush=True)
     assert e <= F_TOL

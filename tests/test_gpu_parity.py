@@ -233,6 +233,52 @@ def test_lockstep_grid_c1(mdt, orc):
     print(f"lock-step worst force rel err {worst:.3e}")
 
 
+def test_lockstep_c1_ten_thousand_steps(mdt, orc):
+    """The 4,000-atom case for its full 10,000 steps of moving atoms (PACE 10, 1,000 deposits):
+    Q6.AVE, V and dV/ds against the oracle every step, forces every 100th step, the grid at
+    the end."""
+    s = fcc_system(mdt, 10, sigma=0.1, seed=21)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    ctx = _ctx(mdt, s)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 10 HILLS_FILE NONE "
+                       "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+                       "CV_dim 1 SYMBOL v1 Q6.AVE DIM 1 0 1 1000 v1 WT 0")
+    mv = Mover(x, n, own)
+    rng = np.random.default_rng(3)
+    g = orc.Grid([0.0], [1.0], [1000])
+    xl = x[:n].copy()
+    worst = cvw = 0.0
+    f = np.zeros((n, 3))
+    for step in range(10000):
+        xa = mv.set_local(xl)
+        ctx.update_positions(0, xa)
+        f[:] = 0.0
+        fix.post_force(step, f)
+        env = orc.Env(xa, off, idx, own, n, 3.9, 18)
+        q6 = env.q_ave(6)
+        cvw = max(cvw, abs(fix.cv("Q6.AVE") - q6) / q6)
+        assert abs(fix.cv("Q6.AVE") - q6) <= CV_TOL * q6, step
+        if step % 10 == 0:
+            g.deposit([q6], g.wt_height([q6], 0.005, 10.0, 300.0, 8.617343e-5, 0), 0.01)
+        V, dV = g.eval([q6])
+        assert abs(fix.scalar() - V) <= 1e-4 * abs(V), step
+        w = fix.cv_weight("Q6.AVE")
+        assert abs(w - dV[0]) <= 1e-4 * 0.005 / 0.01, (step, w, dV[0])
+        if step % 100 == 0:
+            e = rel_force_err(f, -env.dQ_dx(6, w))
+            worst = max(worst, e)
+            assert e <= F_TOL, (step, e)
+        if step % 2000 == 0:
+            print(f"[c1 lock-step] step {step}: CV rel err so far {cvw:.3e}, force {worst:.3e}", flush=True)
+        xl = xl + rng.normal(scale=0.005, size=xl.shape)
+    assert fix.hills(1).shape[0] == 1000
+    grid = fix.grid(1000)
+    assert np.abs(grid - g.g).max() <= 1e-4 * np.abs(g.g).max()
+    print(f"[c1 lock-step] 10000 steps: worst CV rel err {cvw:.3e}, worst force rel err {worst:.3e}, "
+          f"grid max abs err {np.abs(grid - g.g).max():.3e} of {np.abs(g.g).max():.3e}", flush=True)
+
+
 def test_graph_run_matches_post_force(mdt, orc):
     s = fcc_system(mdt, 8, sigma=0.2, seed=2)
     n = s.n_local
