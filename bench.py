@@ -434,6 +434,60 @@ def run_lq(mdt, args):
                 metric="atom-steps/s, STEINH L 6 MEAN_SOLID CV+bias forces @256k atoms")
 
 
+# ------------------------------------------------------------------ the reference's Q4 deck
+REF_Q4_MS = 30.9375      # test/STEINH/Q4/log.lammps:72-84: fix metad, 9.2812 s / 300 steps, 32,000 atoms
+
+
+def run_q4ref(mdt, args):
+    """The reference's own STEINH Q4 deck (test/STEINH/Q4/in.test-steinhardt): 32,000 atoms of
+    FCC a = 3.26 A (20^3 cells, jittered 0.02 A for its 10 K), STEINH Q 4 cutoff_r 6.0
+    cutoff_Natoms 12 on lists of 6.0 + 2.0 skin (~250 entries: the near-list path, and the
+    cap binds for every atom), PACE 1, WT, DIM [0,40] x 400, sigma 0.2, height 0.01; positions
+    read from the engine buffer, one graph launch per step.  Its only on-path timing is the
+    reference's log: 30.9 ms per step on an unstated NVIDIA GPU (a debug build), with fix nvt."""
+    s = mdt.System().add_fcc(20, 20, 20, 3.26, 0.02, 0.02, 0.0, seed=10).build(8.0)
+    n = s.n_local
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    xbuf = EngineBuffer(s.arrays()[0])
+    ctx.attach_positions(0, xbuf.ptr.value)
+    deck = ("GAUSSIAN 0.2 0.01 10.0 PACE 1 TEMP 10 HILLS_FILE NONE "
+            "CAL NAME Q4 STEINH Q 4 all cutoff_Natoms 12 cutoff_r 6.0 "
+            "CV_dim 1 SYMBOL v1 Q4.AVE DIM 1 0 40 400 v1 WT 1")
+    fix = mdt.Fix(ctx, deck)
+    fix.init()
+    fix.run(0, args.warmup, per_launch=1)
+    steps = max(100, args.steps // 4)
+    dt, ts = windows(fix, args.warmup, steps, args.windows, 1)
+    value = n * steps / dt
+    log(f"q4ref: {dt * 1e6 / steps:.1f} us/step, {value:.3e} atom-steps/s "
+        f"(reference log: {REF_Q4_MS:.1f} ms/step)")
+    kms = fix.profile(args.warmup + args.windows * steps, args.profile_steps)
+    log("kernel mean ms: " + json.dumps(kms))
+    x, off, idx, own = s.arrays()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        def one():
+            env = orc.Env(x, off, idx, own, n, 6.0, 12)
+            env.q_ave(4)
+            env.dQ_dx(4, 1.0)
+        k, t = oracle_steps(one, args.cpu_budget)
+        ci = cpu_info(orc)
+        cpu = {"value": n * k / t, "unit": "atom-steps/s", "cores": ci["cores"], "kind": "port",
+               "sample": f"{k} steps (selection, Q4.AVE, exact forces) of the same 32,000-atom deck by the FP64 "
+                         f"oracle, OpenMP over atoms on {ci['cores']} threads, {t:.1f} s"}
+    cfg = {"workload": "the reference's STEINH Q4 deck: 32,000-atom FCC a=3.26 A, Q 4 N=12 rc=6.0, lists 8.0 A "
+                       "(mean %.0f entries), PACE 1, WT" % (off[-1] / n),
+           "atoms": n, "kernels_ms": kms, "windows_s": ts,
+           "reference_log_ms_per_step": REF_Q4_MS,
+           "reference_log_atom_steps_per_s": n / (REF_Q4_MS * 1e-3)}
+    xbuf.free()
+    return line(value, steps, args.warmup, dt * 1e3 / steps, "f32 pair math, f64 sums/grid", cfg, None, cpu,
+                metric="atom-steps/s, the reference's STEINH Q4 deck (32,000 atoms)")
+
+
 def run_copy(mdt, args):
     """Plain device copy of a known size: the calibration for HBM byte counters and the
     achievable copy bandwidth of the chip (reads 1 GiB, writes 1 GiB per launch)."""
@@ -466,7 +520,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--windows", type=int, default=5, help="timed windows of --steps steps; the median is reported")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lq", "copy"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lq", "q4ref", "copy"])
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mlp-plain", action="store_true",
@@ -481,7 +535,7 @@ def main():
         import __graft_entry__
         __graft_entry__.build()
     out = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "lq": run_lq,
-           "copy": run_copy}[args.config](mdt, args)
+           "q4ref": run_q4ref, "copy": run_copy}[args.config](mdt, args)
     print(json.dumps(out), flush=True)
 
 

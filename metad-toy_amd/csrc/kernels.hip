@@ -468,29 +468,49 @@ __device__ __forceinline__ void wave_lds_sync() {
 // k_env: selection + q_lm for every degree in LMASK, kLanes lanes per atom.
 // Phase A scans the list with FP64 r^2; when the number within rc does not
 // exceed N every such candidate is kept (what the reference insertion keeps
-// then), otherwise lane 0 replays the reference insertion to find the kept
-// set.  Phase B accumulates Y_lm in FP32 over the kept set.
+// then), otherwise the atom's lanes replay the reference insertion to find the
+// kept set.  Phase B accumulates Y_lm in FP32 over the kept set.
 // ---------------------------------------------------------------------------
 // The reference insertion (get_environment_Steinhardt_Q) over one atom's list, with
-// the sorted arrays in LDS (one lane per atom), so that the rare cap-binding path
-// does not set the register budget of the whole kernel: skip r^2 > rc^2 or
-// r^2 < 1e-12, then walk the N slots with strict '>' carrying the displaced entry.
+// the sorted arrays in LDS (written by lane 0 of the atom's group), so that the
+// cap-binding path does not set the register budget of the whole kernel: skip
+// r^2 > rc^2 or r^2 < 1e-12, then walk the N slots with strict '>' carrying the
+// displaced entry.  Entries outside rc never enter, so only the within-rc mask is
+// walked, four entries at a time in list order: lane l computes the FP64 r^2 of the
+// l-th (the four loads in flight together) and lane 0 inserts the four in order,
+// entry by entry as the reference does, so the slots are the reference's.
+// Every lane of the group calls it (the group's control flow is uniform).
 template <int NMAX>
-__device__ __forceinline__ void literal_insert_lds(const Geo& g, int beg, int c, double xi0, double xi1,
-                                                   double xi2, double rc2, int N, double* sr2, int* sp) {
-    for (int s = 0; s < N; ++s) { sr2[s] = __builtin_inf(); sp[s] = -1; }
-    for (int p = 0; p < c; ++p) {
-        double d0, d1, d2;
-        double r2 = pair_r2(g.x4, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
-        if ((r2 > rc2) || (r2 < 1e-12)) continue;
-        double r = r2;
-        int q = p;
-        for (int s = 0; s < N; ++s) {
-            const double tr = sr2[s];
-            if (tr > r) {
-                const int tq = sp[s];
-                sr2[s] = r; sp[s] = q;
-                r = tr; q = tq;
+__device__ __forceinline__ void literal_insert_group(const Geo& g, int beg, const M128& inrc, double xi0, double xi1,
+                                                     double xi2, double rc2, int N, double* sr2, int* sp, int lane) {
+    if (lane == 0)
+        for (int s = 0; s < N; ++s) { sr2[s] = __builtin_inf(); sp[s] = -1; }
+    M128 it = inrc;
+    mdrop(it, lane);
+    const int nr = (mpop_count(inrc) + kLanes - 1) / kLanes;
+    const int src0 = (int)(threadIdx.x & ~(unsigned)(kLanes - 1)) & 63;
+    for (int rd = 0; rd < nr; ++rd) {
+        const int p = mnext(it);
+        double r2 = __builtin_inf();
+        if (p >= 0) {
+            double d0, d1, d2;
+            r2 = pair_r2(g.x4, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
+        }
+        #pragma unroll
+        for (int k = 0; k < kLanes; ++k) {
+            const int q0 = __shfl(p, src0 + k, 64);
+            const double r0 = __shfl(r2, src0 + k, 64);
+            if (lane == 0 && q0 >= 0 && !((r0 > rc2) || (r0 < 1e-12))) {
+                double r = r0;
+                int q = q0;
+                for (int s = 0; s < N; ++s) {
+                    const double tr = sr2[s];
+                    if (tr > r) {
+                        const int tq = sp[s];
+                        sr2[s] = r; sp[s] = q;
+                        r = tr; q = tq;
+                    }
+                }
             }
         }
     }
@@ -628,11 +648,11 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
             if (lane + s * kLanes < cnt) add_ylm(pk[s]);
     } else {
         if (EXPORT || cnt > N) {                     // the cap binds: the literal insertion
+            double* sr2 = reinterpret_cast<double*>(&kept[a][0]);
+            int* sp = reinterpret_cast<int*>(sr2 + NMAX);
+            const double4 xi = g.x4[i];
+            literal_insert_group<NMAX>(g, beg, own, xi.x, xi.y, xi.z, rc2, N, sr2, sp, lane);
             if (lane == 0) {
-                double* sr2 = reinterpret_cast<double*>(&kept[a][0]);
-                int* sp = reinterpret_cast<int*>(sr2 + NMAX);
-                const double4 xi = g.x4[i];
-                literal_insert_lds<NMAX>(g, beg, c, xi.x, xi.y, xi.z, rc2, N, sr2, sp);
                 own = M128{0, 0};
                 for (int s = 0; s < N; ++s)
                     if (sp[s] >= 0) mset(own, sp[s]);

@@ -264,3 +264,36 @@ def test_reference_q4_deck_long_lists_and_rebuilds(mdt, orc):
     # the run went far past the rebuild threshold (under 0.05 A here) and stayed inside the
     # list skin (1 A)
     assert moved.max() > 0.3 and moved.max() < 1.0, moved.max()
+
+
+def test_reference_q4_deck_full_size(mdt, orc):
+    """The reference's Q4 deck at its own size: 32,000 atoms (20^3 FCC cells, a = 3.26 A,
+    jittered 0.02 A), STEINH Q 4 N = 12 rc = 6.0 on 8.0 A lists, PACE 1 with WT: two steps
+    (a deposit each) with Q4.AVE, V, dV/ds, the selection in slot order and the forces
+    against the oracle."""
+    s = mdt.System().add_fcc(20, 20, 20, 3.26, 0.02, 0.02, 0.0, seed=10).build(8.0)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    print(f"[q4 deck 32k] n={n} mean list {off[-1] / n:.1f}", flush=True)
+    env = orc.Env(x, off, idx, own, n, 6.0, 12)
+    q4 = env.q_ave(4)
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.2 0.01 10.0 PACE 1 TEMP 10 HILLS_FILE NONE "
+                            "CAL NAME Q4 STEINH Q 4 all cutoff_Natoms 12 cutoff_r 6.0 "
+                            "CV_dim 1 SYMBOL v1 Q4.AVE DIM 1 0 40 400 v1 WT 1")
+    fix.init()
+    g = orc.Grid([0.0], [40.0], [400])
+    for step in range(2):
+        f = np.zeros((n, 3))
+        fix.post_force(step, f)
+        assert abs(fix.cv("Q4.AVE") - q4) <= CV_TOL * q4, step
+        g.deposit([q4], g.wt_height([q4], 0.01, 10.0, 10.0, 8.617343e-5, 1), 0.2)
+        V, dV = g.eval([q4])
+        assert abs(fix.scalar() - V) <= 1e-4 * abs(V), step
+        w = fix.cv_weight("Q4.AVE")
+        assert abs(w - dV[0]) <= 1e-4 * 0.01 / 0.2, (step, w, dV[0])
+        assert rel_force_err(f, -env.dQ_dx(4, w)) <= F_TOL, step
+    sel, cnt, ns = fix.selection("Q4", n, 12)
+    assert np.array_equal(sel, env.sel_idx)
+    assert np.array_equal(cnt, env.cnt)
+    fix.close()
+    ctx.close()
