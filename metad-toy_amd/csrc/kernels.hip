@@ -500,7 +500,9 @@ __device__ __forceinline__ void literal_insert_group(const Geo& g, int beg, cons
         for (int k = 0; k < kLanes; ++k) {
             const int q0 = __shfl(p, src0 + k, 64);
             const double r0 = __shfl(r2, src0 + k, 64);
-            if (lane == 0 && q0 >= 0 && !((r0 > rc2) || (r0 < 1e-12))) {
+            // the slots stay sorted (non-decreasing), so an entry no smaller than the last
+            // slot changes nothing: it is passed over after one read
+            if (lane == 0 && q0 >= 0 && !((r0 > rc2) || (r0 < 1e-12)) && sr2[N - 1] > r0) {
                 double r = r0;
                 int q = q0;
                 for (int s = 0; s < N; ++s) {
