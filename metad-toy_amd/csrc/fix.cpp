@@ -1242,6 +1242,13 @@ void fix_build(mdt_fix* fx) {
     for (auto* e : fx->envs) {
         const int lm = e->lmask;
         const int RS = rec_stride(lm), NL = rec_nl(lm);
+        // the force pass addresses records and list words with 32-bit byte offsets
+        if ((uint64_t)NT * (uint64_t)RS * 4u >= (1ull << 31))
+            throw Err(MDT_ENOTSUP, fmt("%lld local atoms in one context: at most %lld are supported",
+                                       (long long)NT, (long long)((1ull << 31) / (4u * (uint64_t)RS)) - 1));
+        if ((uint64_t)ctx->total_nnz * 4u >= (1ull << 31))
+            throw Err(MDT_ENOTSUP, fmt("%lld list entries in one context: at most %lld are supported",
+                                       (long long)ctx->total_nnz, (long long)((1ull << 29) - 1)));
         e->rec.alloc((size_t)NT * RS);
         e->ql.alloc((size_t)NT * NL);
         e->nsel.alloc(NT); e->cnt.alloc(NT); e->selmask.alloc((size_t)NT * 4);

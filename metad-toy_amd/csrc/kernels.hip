@@ -956,6 +956,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
     const bool valid = a < tl.count;
     const int i = tl.start + (valid ? a : 0);
+    // the neighbour records as a raw buffer (32-bit offsets; loads past its end read 0)
+    const __amdgpu_buffer_rsrc_t rrec = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)e.rec, (short)0, (int)((uint32_t)g.n_local_total * (uint32_t)(RS * 4)), 0x00020000);
     float wl[NL];
     #pragma unroll
     for (int k = 0; k < NL; ++k)
@@ -1006,7 +1009,8 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     // header carries the candidate's packed position (ghost images load theirs),
     // and evaluates its pair.
     const int nrounds = (mpop_count(rcm) + kLanes - 1) / kLanes;   // uniform within the group
-    const int wv = threadIdx.x >> 6, gq = (threadIdx.x & 63) >> 2;   // wave, lane group within it
+    // wave (wave-uniform: the LDS destinations of the record loads are scalar), lane group within it
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), gq = (threadIdx.x & 63) >> 2;
     // lists of at most 64 entries (the usual case, uniform over the wave): the
     // lane's share of the low word is walked with plain 64-bit clears
     const bool wide = __ballot(rcm.hi != 0ull) != 0ull;
@@ -1027,9 +1031,18 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     // the list words (reverse position, owner, index) of the next round are loaded
     // while the current round runs
     int pn = next_p();
-    int rpn = pn >= 0 ? g.revpos[beg + pn] : 0xFF;
-    int on = pn >= 0 ? g.eown[beg + pn] : i;
-    int jn = pn >= 0 ? g.idx[beg + pn] : i;
+    // the list words as raw buffers addressed by 32-bit offsets
+    const __amdgpu_buffer_rsrc_t rrev = __builtin_amdgcn_make_buffer_rsrc((void*)g.revpos, (short)0, 0x7FFFFFFF, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rown = __builtin_amdgcn_make_buffer_rsrc((void*)g.eown, (short)0, 0x7FFFFFFF, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ridx = __builtin_amdgcn_make_buffer_rsrc((void*)g.idx, (short)0, 0x7FFFFFFF, 0x00020000);
+    auto words = [&](int p, int& rp, int& o, int& j) {
+        const uint32_t ent = (uint32_t)(beg + p);
+        rp = p >= 0 ? (int)__builtin_amdgcn_raw_buffer_load_b8(rrev, ent, 0, 0) : 0xFF;
+        o = p >= 0 ? (int)__builtin_amdgcn_raw_buffer_load_b32(rown, 4u * ent, 0, 0) : i;
+        j = p >= 0 ? (int)__builtin_amdgcn_raw_buffer_load_b32(ridx, 4u * ent, 0, 0) : i;
+    };
+    int rpn, on, jn;
+    words(pn, rpn, on, jn);
     for (int rd = 0; rd < nrounds; ++rd) {
         const int p = pn, rp = rpn, o = on, j = jn;
         // The group's four candidate records go straight from global memory into
@@ -1040,21 +1053,21 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
         wave_lds_sync();                             // last round's reads are complete
         {
             const int oc[kLanes] = {quad_bcast<0>(o), quad_bcast<1>(o), quad_bcast<2>(o), quad_bcast<3>(o)};
-            #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+            sfor<NQ>([&](auto qq) {
+                constexpr int q = decltype(qq)::value;
                 #pragma unroll
                 for (int c = 0; c < kLanes; ++c) {
-                    const float4* src = reinterpret_cast<const float4*>(e.rec + (int64_t)oc[c] * RS) + 4 * q + ((lane + c) & 3);
-                    __builtin_amdgcn_global_load_lds((const void*)src,
-                                                     (__attribute__((address_space(3))) void*)&dst[wv][q][c][0], 16, 0, 0);
+                    const uint32_t vo = (uint32_t)oc[c] * (uint32_t)(RS * 4) + (uint32_t)(((lane + c) & 3) * 16);
+                    // the quad's byte offset goes in the scalar offset, which moves the global
+                    // address only (the immediate offset would move the LDS destination too)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rrec, (__attribute__((address_space(3))) void*)&dst[wv][q][c][0],
+                                                             16, vo, 64 * q, 0, 0);
                 }
-            }
+            });
         }
         if (rd + 1 < nrounds) {                      // uniform within the group
             pn = next_p();
-            rpn = pn >= 0 ? g.revpos[beg + pn] : 0xFF;
-            on = pn >= 0 ? g.eown[beg + pn] : i;
-            jn = pn >= 0 ? g.idx[beg + pn] : i;
+            words(pn, rpn, on, jn);
         }
         float rr[NV * 4];
         #pragma unroll
