@@ -111,6 +111,7 @@ struct NearDev {
 // One N-nearest environment (cutoff, cap) and the Steinhardt degrees on it.
 struct EnvDev {
     double rc2;
+    float rcs;                // sqrt(rc2 (1 + kRcSlack)) rounded to FP32 (the scan's error bound)
     int32_t N;
     int32_t lmask;
     float* rec;               // n_local_total * rec_stride(lmask)

@@ -1255,6 +1255,7 @@ void fix_build(mdt_fix* fx) {
         EnvDev& d = e->dev;
         d = EnvDev{};
         d.rc2 = e->rc * e->rc; d.N = e->N; d.lmask = lm;
+        d.rcs = (float)std::sqrt(d.rc2 * (1.0 + kRcSlack));
         d.rec = e->rec.p; d.ql = e->ql.p; d.nsel = e->nsel.p; d.cnt_rc = e->cnt.p; d.selmask = e->selmask.p;
         int li = 0;
         for (int l : {3, 4, 6}) {

@@ -36,6 +36,7 @@ namespace {
 // c_lm(u) = N_lm (-1)^m d^m P_l(u)/du^m (orthonormal, Condon-Shortley phase:
 // the same functions as the reference closed forms).
 // ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));   // a 128-bit buffer load
 constexpr double kPi = 3.14159265358979323846;
 constexpr double cfact(int n) { double f = 1; for (int i = 2; i <= n; ++i) f *= i; return f; }
 constexpr double csqrt(double v) {
@@ -566,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     float margin;
     {
         constexpr float u = 5.9604645e-8f;
-        const float rcf = (float)__builtin_sqrt(rc2s);
+        const float rcf = e.rcs;
         const float X = fmaxf(fmaxf(fabsf(pif.x), fabsf(pif.y)), fabsf(pif.z)) + rcf + 1.0f;
         margin = 2.0f * (5.0f * u * (float)rc2s + 7.0f * u * X * rcf) + 1e-6f;
     }
@@ -574,16 +575,22 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
     M128 own{0, 0}, rcm{0, 0};
     int cnt = 0;                                     // group-uniform
     bool fbl = false;                                // this lane took the exact FP64 test
+    // the list and the packed positions as raw buffers addressed by 32-bit offsets
+    const __amdgpu_buffer_rsrc_t ridx = __builtin_amdgcn_make_buffer_rsrc((void*)g.idx, (short)0, 0x7FFFFFFF, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rx4f = __builtin_amdgcn_make_buffer_rsrc((void*)g.x4f, (short)0, 0x7FFFFFFF, 0x00020000);
     for (int base = 0; base < c; base += 4 * kLanes) {   // trip count uniform within the group
         int jj[4];
         #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int p = base + u * kLanes + lane;
-            jj[u] = g.idx[beg + (p < c ? p : c - 1)];
+            jj[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(ridx, 4u * (uint32_t)(beg + (p < c ? p : c - 1)), 0, 0);
         }
         float4 xx[4];
         #pragma unroll
-        for (int u = 0; u < 4; ++u) xx[u] = g.x4f[jj[u]];
+        for (int u = 0; u < 4; ++u) {
+            const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rx4f, 16u * (uint32_t)jj[u], 0, 0);
+            xx[u] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+        }
         uint32_t cin = 0, crc = 0;
         #pragma unroll
         for (int u = 0; u < 4; ++u) {
