@@ -297,3 +297,17 @@ def test_reference_q4_deck_full_size(mdt, orc):
     assert np.array_equal(cnt, env.cnt)
     fix.close()
     ctx.close()
+
+
+def test_more_than_128_within_the_cutoff_is_refused(mdt):
+    """A Steinhardt cutoff with more than 128 list entries inside it (FCC a = 3.26 A, rc 7.5 A:
+    ~190 atoms) cannot be held by the selection masks: the fix refuses it with a message at
+    init instead of selecting from a truncated list."""
+    s = fcc_system(mdt, 8, a=3.26, sigma=0.05, seed=49, rg=8.0)
+    ctx, fix = _one(mdt, s, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                            "CAL NAME Q4 STEINH Q 4 all cutoff_r 7.5 cutoff_Natoms 12 "
+                            "CV_dim 1 SYMBOL v1 Q4.AVE DIM 1 0 1 1000 v1 WT 0")
+    with pytest.raises(mdt.MdtError, match="more than 128 list entries within the Steinhardt cutoff"):
+        fix.init()
+    fix.close()
+    ctx.close()
