@@ -311,3 +311,40 @@ def test_more_than_128_within_the_cutoff_is_refused(mdt):
         fix.init()
     fix.close()
     ctx.close()
+
+
+def test_walkers_with_long_and_short_lists(mdt, orc):
+    """Two walkers in one context, one with the Q4 deck's ~240-entry lists (walked through
+    near lists) and one with ordinary 5.9 A lists: each walker's Q4.AVE, chain weight and
+    forces against the oracle on its own bound lists."""
+    s_long = fcc_system(mdt, 8, a=3.26, sigma=0.05, seed=61, rg=8.0)
+    s_short = fcc_system(mdt, 6, a=5.26, sigma=0.15, seed=62, rg=6.5)
+    systems = [s_long, s_short]
+    ctx = mdt.Context(0, 2)
+    for w, s in enumerate(systems):
+        ctx.bind_system(w, s)
+    fix = mdt.Fix(ctx, "GAUSSIAN 0.01 0.005 10 PACE 0 HILLS_FILE NONE "
+                       "CAL NAME Q4 STEINH Q 4 all cutoff_r 6.0 cutoff_Natoms 12 "
+                       "CV_dim 1 SYMBOL v1 Q4.AVE DIM 1 0 1 1000 v1 WT 0")
+    fix.init()
+    refs = []
+    for w, s in enumerate(systems):
+        x, off, idx, own = s.arrays()
+        env = orc.Env(x, off, idx, own, s.n_local, 6.0, 12)
+        q4 = env.q_ave(4)
+        fix.add_hills(np.array([[q4 + 0.004]]), np.array([0.005]), 0.01, walker=w)
+        refs.append((env, q4, np.diff(off).max()))
+    print(f"[walkers long/short] longest lists {[r[2] for r in refs]}", flush=True)
+    assert refs[0][2] > 128
+    ns = [s.n_local for s in systems]
+    f = np.zeros((sum(ns), 3))
+    fix.post_force(1, f)
+    a = 0
+    for w, (env, q4, _) in enumerate(refs):
+        assert abs(fix.cv("Q4.AVE", w) - q4) <= CV_TOL * q4, w
+        wt = fix.cv_weight("Q4.AVE", w)
+        assert wt != 0.0
+        assert rel_force_err(f[a:a + ns[w]], -env.dQ_dx(4, wt)) <= F_TOL, w
+        a += ns[w]
+    fix.close()
+    ctx.close()
