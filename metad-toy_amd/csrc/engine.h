@@ -25,6 +25,12 @@ constexpr int kMaxProg = 96;       // expression bytecode length per dimension
 constexpr int kStack = 16;
 constexpr int kRedArgWalkers = 64; // walkers whose tile ranges travel in the reduce kernel's arguments
 constexpr int64_t kMaxGridBins = 1ll << 28;   // bins per walker grid (2 GiB of FP64)
+// Sums over a hill log are spread over kHillBlocks workgroups per walker.  Per-walker
+// work buffer of that path: [kHillBlocks][4] partial sums (V, dV/ds), then the
+// chain-rule coefficients ds_d/dv_c as [kMaxDim][kMaxSlotCV].
+constexpr int kHillBlocks = 64;
+constexpr int kBwAc = 4 * kHillBlocks;
+constexpr int kBwStride = kBwAc + kMaxDim * kMaxSlotCV;
 
 // Bit set of Steinhardt degrees an environment computes.
 constexpr int kL3 = 1, kL4 = 2, kL6 = 4;
@@ -243,6 +249,7 @@ struct ReduceDev {
     const MlpNet* mlp;        // per walker live network, NULL unless kind == BIAS_MLP
     const int32_t* mlp_ok;    // per walker: 1 = network trusted, 0 = use exact sum
     const double* mlp_err;    // per walker: held-out max |dV| of the last fit
+    double* bwork;            // per walker [kBwStride]: hill-sum partials and ds/dv (hills kinds)
 };
 
 // state layout per walker

@@ -632,7 +632,7 @@ struct mdt_fix {
     uint64_t built_gen = 0;
     uint64_t builds = 0;           // fix_build calls (device buffers may move)
     int nslot_sum = 0;
-    DBuf<double> partial, direct, grid, hills, state, wcv, cv_inv;
+    DBuf<double> partial, direct, grid, hills, state, wcv, cv_inv, bwork;
     DBuf<int64_t> nhills, step;
     DBuf<int32_t> tile_begin, mlp_ok;
     DBuf<ExprProg> progs;
@@ -1444,6 +1444,8 @@ void fix_build(mdt_fix* fx) {
     HIPCHK(launch_fill(fx->state.p, (int64_t)nw * kStStride, 0.0, st));
     fx->wcv.alloc((size_t)nw * kWcvStride);
     HIPCHK(launch_fill(fx->wcv.p, (int64_t)nw * kWcvStride, 0.0, st));
+    fx->bwork.alloc((size_t)nw * kBwStride);
+    HIPCHK(launch_fill(fx->bwork.p, (int64_t)nw * kBwStride, 0.0, st));
     // 1 / group count per walker of every CV whose value is a sum over its group
     {
         std::vector<double> h((size_t)nw * kMaxSlotCV, 0.0);
@@ -1558,6 +1560,7 @@ void fix_build(mdt_fix* fx) {
     r.step = fx->step.p; r.state = fx->state.p; r.wcv = fx->wcv.p;
     r.epoch = ctx->near_on() ? ctx->nepoch.p : nullptr;
     r.mlp = fx->mlp_live.p; r.mlp_ok = fx->mlp_ok.p; r.mlp_err = fx->mlp_err.p;
+    r.bwork = fx->bwork.p;
     if (fx->hill_type == 3) {
         MlpDev& m = fx->mdev;
         m = MlpDev{};

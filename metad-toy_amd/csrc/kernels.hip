@@ -1358,17 +1358,15 @@ __device__ __forceinline__ double uniform_f64(double v) {
     return __longlong_as_double(((long long)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Exact FP64 hills sum at s, block-cooperative; V and dV valid on thread 0.
-template <int NW>
-__device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh,
-                                   int64_t h_first = 0) {
+// One thread's share of the exact FP64 hills sum at s: hills first, first + G,
+// first + 2G, ... below nh, in that order, added to acc = (V, dV/ds).
+__device__ __forceinline__ void hills_thread_sum(const ReduceDev& r, int w, const double* s, int64_t first,
+                                                 int64_t G, double* acc) {
     const double* H = r.hills + (int64_t)w * r.hills_cap * (r.D + 3);
     const int64_t nh = r.nhills[w];
-    double acc[4] = {0, 0, 0, 0};
-    // a thread's hills h, h + B, h + 2B, ... in that order, four records loaded ahead
-    // of their arithmetic (the loop is latency-bound one record at a time)
+    // four records loaded ahead of their arithmetic (the loop is latency-bound one
+    // record at a time)
     constexpr int HU = 4;
-    const int64_t B = blockDim.x;
     // 1/(2 sigma^2) per dimension, once (FP64 divisions cost more than the exp); twice
     // it is 1/sigma^2 exactly.  The values are uniform: held in scalar registers.
     double i2s2[kMaxDim] = {0, 0, 0};
@@ -1378,11 +1376,11 @@ __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, d
     // width (a hill replayed from a HILLS file keeps the file's sigma); deposits carry
     // the first width, so their scale is exactly 1 and costs no division.
     const double sig0 = uniform_f64(r.sigma[0]);
-    for (int64_t h = h_first + threadIdx.x; h < nh; h += HU * B) {
+    for (int64_t h = first; h < nh; h += HU * G) {
         double c[HU][kMaxDim], hgt[HU], hs[HU];
         #pragma unroll
         for (int u = 0; u < HU; ++u) {
-            const int64_t hu = h + u * B < nh ? h + u * B : h;
+            const int64_t hu = h + u * G < nh ? h + u * G : h;
             const double* rec = H + hu * (r.D + 3);
             #pragma unroll
             for (int d = 0; d < kMaxDim; ++d) c[u][d] = d < r.D ? rec[1 + d] : 0.0;
@@ -1391,7 +1389,7 @@ __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, d
         }
         #pragma unroll
         for (int u = 0; u < HU; ++u) {
-            if (h + u * B >= nh) break;
+            if (h + u * G >= nh) break;
             const double sc = hs[u] == sig0 ? 1.0 : (sig0 * sig0) / (hs[u] * hs[u]);
             double ex = 0, dd[kMaxDim] = {0, 0, 0};
             #pragma unroll
@@ -1405,6 +1403,14 @@ __device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, d
             for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acc[1 + d] -= t * dd[d] * (2.0 * i2s2[d] * sc);
         }
     }
+}
+
+// Exact FP64 hills sum at s, block-cooperative; V and dV valid on thread 0.
+template <int NW>
+__device__ double hills_eval_block(const ReduceDev& r, int w, const double* s, double* dV, double* sh,
+                                   int64_t h_first = 0) {
+    double acc[4] = {0, 0, 0, 0};
+    hills_thread_sum(r, w, s, h_first + threadIdx.x, blockDim.x, acc);
     block_sum<4, NW>(acc, sh);
     if (threadIdx.x == 0) {
         #pragma unroll
@@ -1591,7 +1597,9 @@ __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, 
 // k_reduce: one workgroup per walker.  Post-force order of
 // FixMetadynamics::post_force (src/core/fix_crystallize.cu:409-450):
 // CVs -> expression -> deposit on step % pace == 0 (WT height from the bias
-// before the deposit) -> V and dV/ds after the deposit -> chain rule.
+// before the deposit) -> V and dV/ds after the deposit -> chain rule.  A hill-log
+// bias (exact sum or learned) ends this kernel at s: k_hills_part and k_bias_tail
+// do the rest of the step.
 //
 // The kernel is one chain of dependent memory round trips, so it is laid out
 // to have few: the partial sums load in the first one (tile ranges come with
@@ -1752,11 +1760,28 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
             reduce_finish(r, w, c, cvv, acoef, sv, V, dVg, 0.0, step, mlp_err, mlp_ok, inv_c);
             return;
         }
-    } else if (r.kind == BIAS_GRID && !dep) {
+        if (r.kind != BIAS_GRID) {
+            // a hill log: its sum is spread over many workgroups (k_hills_part) and the
+            // step ends in k_bias_tail, which takes the slot values, s and ds/dv from here
+            double* st = r.state + (int64_t)w * kStStride;
+            double* bw = r.bwork + (int64_t)w * kBwStride;
+            if (c < r.ncv) st[kStCv + c] = cvv;
+            if (c < kMaxSlotCV) {
+                #pragma unroll
+                for (int d = 0; d < kMaxDim; ++d) bw[kBwAc + d * kMaxSlotCV + c] = acoef[d];
+            }
+            if (c == 0) {
+                #pragma unroll
+                for (int d = 0; d < kMaxDim; ++d) if (d < r.D) st[kStS + d] = sv[d];
+            }
+            return;
+        }
+    } else if (r.kind != BIAS_GRID || !dep) {
         return;
     }
+    // a grid bias on a deposit step from here on
     // keep the slot values and the chain-rule coefficients of wave 0 across the
-    // block-cooperative bias evaluation below
+    // block-cooperative deposit below
     __shared__ double acoef_s[kMaxDim][kMaxSlotCV];
     if (slot_lane) {
         #pragma unroll
@@ -1764,37 +1789,120 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     }
     __syncthreads();
     // 3. deposit
+    double* gw = r.grid + (int64_t)w * r.grid_size;
     double dV[kMaxDim] = {0, 0, 0};
-    if (dep) {
-        double Vb = 0.0;
-        if (r.wt) Vb = bias_eval_block<kRedThreads / 64>(r, w, s, dV, sh);
-        if (threadIdx.x == 0) bc[0] = r.h0 * exp(-(Vb) / (r.kT * (r.biasf - 1.0)));
-        __syncthreads();
-        const double hgt = bc[0];
-        if (r.kind == BIAS_GRID) grid_deposit_block(r, r.grid + (int64_t)w * r.grid_size, s, hgt, r.sigma[0]);
-        if (threadIdx.x == 0) {
-            int64_t k = nhill0;
-            if (k < r.hills_cap) {
-                double* rec = r.hills + ((int64_t)w * r.hills_cap + k) * (r.D + 3);
-                rec[0] = (double)step;
-                for (int d = 0; d < r.D; ++d) rec[1 + d] = s[d];
-                rec[1 + r.D] = hgt;
-                rec[2 + r.D] = r.sigma[0];
-                r.nhills[w] = k + 1;
-            }
-        }
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        const double Vb = r.wt ? grid_eval(r, gw, s, dV) : 0.0;
+        bc[0] = r.h0 * exp(-(Vb) / (r.kT * (r.biasf - 1.0)));
     }
+    __syncthreads();
+    const double hgt = bc[0];
+    grid_deposit_block(r, gw, s, hgt, r.sigma[0]);
+    if (threadIdx.x == 0) {
+        int64_t k = nhill0;
+        if (k < r.hills_cap) {
+            double* rec = r.hills + ((int64_t)w * r.hills_cap + k) * (r.D + 3);
+            rec[0] = (double)step;
+            for (int d = 0; d < r.D; ++d) rec[1 + d] = s[d];
+            rec[1 + r.D] = hgt;
+            rec[2 + r.D] = r.sigma[0];
+            r.nhills[w] = k + 1;
+        }
+    }
+    __syncthreads();
     // 4. bias value and gradient after the deposit (valid on thread 0)
-    const double V = bias_eval_block<kRedThreads / 64>(r, w, s, dV, sh);
+    double V = 0.0;
+    if (threadIdx.x == 0) V = grid_eval(r, gw, s, dV);
     if (wv == 0) {
         const int c = lane64;
         const double cvv = c < r.ncv ? cv[c] : 0.0;
         double ac[kMaxDim] = {0, 0, 0}, sv[kMaxDim] = {0, 0, 0};
         #pragma unroll
         for (int d = 0; d < kMaxDim; ++d) if (d < r.D) { ac[d] = c < kMaxSlotCV ? acoef_s[d][c] : 0.0; sv[d] = s[d]; }
-        reduce_finish(r, w, c, cvv, ac, sv, V, dV, dep ? bc[0] : 0.0, step, mlp_err, mlp_ok, inv_c);
+        reduce_finish(r, w, c, cvv, ac, sv, V, dV, hgt, step, mlp_err, mlp_ok, inv_c);
     }
+}
+
+// Partial sums of a walker's hill log at the walker's s of this step: workgroup b of
+// walker w (blockIdx.y) takes hills first + b*256 + t + k*kHillBlocks*256 and writes
+// (V, dV/ds) to partial slot b.  A trusted learned bias takes only the hills deposited
+// since its fit (first = n_fit); otherwise first = 0.
+constexpr int kHillThreads = 256;
+__global__ __launch_bounds__(kHillThreads) void k_hills_part(ReduceDev r) {
+    __shared__ double sh[4 * (kHillThreads / 64)];
+    const int w = blockIdx.y, b = blockIdx.x;
+    const double* st = r.state + (int64_t)w * kStStride;
+    double s[kMaxDim] = {0, 0, 0};
+    #pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) if (d < r.D) s[d] = st[kStS + d];
+    const int64_t first = (r.kind == BIAS_MLP && r.mlp_ok[w]) ? r.mlp[w].n_fit : 0;
+    double acc[4] = {0, 0, 0, 0};
+    hills_thread_sum(r, w, s, first + (int64_t)b * kHillThreads + threadIdx.x, (int64_t)kHillBlocks * kHillThreads, acc);
+    block_sum<4, kHillThreads / 64>(acc, sh);
+    if (threadIdx.x == 0) {
+        double* part = r.bwork + (int64_t)w * kBwStride + 4 * b;
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) part[k] = acc[k];
+    }
+}
+
+// The rest of a step on a hill-log bias, one wave per walker: the partial sums of
+// k_hills_part in slot order (a fixed tree), plus the network term of a trusted learned
+// bias, is the bias before the deposit; a deposit takes its well-tempered height from
+// it, and the new hill adds exactly its height to V at its own centre and nothing to
+// dV/ds there; then the chain-rule weights and the walker's state.
+__global__ __launch_bounds__(64) void k_bias_tail(ReduceDev r) {
+    const int w = blockIdx.x, c = threadIdx.x;
+    const int64_t step = r.step[w];
+    const int64_t nhill0 = r.nhills[w];
+    const double mlp_err = r.kind == BIAS_MLP ? r.mlp_err[w] : 0.0;
+    const int mlp_ok = r.kind == BIAS_MLP ? r.mlp_ok[w] : 0;
+    const double* st = r.state + (int64_t)w * kStStride;
+    const double* bw = r.bwork + (int64_t)w * kBwStride;
+    const double inv_c = c < kMaxSlotCV ? r.cv_inv[(int64_t)w * kMaxSlotCV + c] : 0.0;
+    const double cvv = c < r.ncv ? st[kStCv + c] : 0.0;
+    double sv[kMaxDim] = {0, 0, 0}, ac[kMaxDim] = {0, 0, 0};
+    #pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) if (d < r.D) {
+        sv[d] = st[kStS + d];
+        ac[d] = c < kMaxSlotCV ? bw[kBwAc + d * kMaxSlotCV + c] : 0.0;
+    }
+    double a[4] = {0, 0, 0, 0};
+    for (int b = c; b < kHillBlocks; b += 64) {
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] += bw[4 * b + k];
+    }
+    double Vb = wave_sum(a[0]);
+    double dV[kMaxDim] = {0, 0, 0};
+    #pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) if (d < r.D) dV[d] = wave_sum(a[1 + d]);
+    if (mlp_ok) {
+        double dn[kMaxDim] = {0, 0, 0};
+        const double vn = mlp_eval_block(r.mlp + w, sv, dn);
+        if (c == 0) {
+            Vb += vn;
+            #pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) if (d < r.D) dV[d] += dn[d];
+        }
+        Vb = readlane_f64(Vb, 0);
+    }
+    const bool dep = r.pace > 0 && (step % r.pace) == 0;
+    double hill = 0.0, V = Vb;
+    if (dep) {
+        hill = r.h0 * exp(-(r.wt ? Vb : 0.0) / (r.kT * (r.biasf - 1.0)));
+        if (nhill0 < r.hills_cap) {
+            if (c == 0) {
+                double* rec = r.hills + ((int64_t)w * r.hills_cap + nhill0) * (r.D + 3);
+                rec[0] = (double)step;
+                for (int d = 0; d < r.D; ++d) rec[1 + d] = sv[d];
+                rec[1 + r.D] = hill;
+                rec[2 + r.D] = r.sigma[0];
+                r.nhills[w] = nhill0 + 1;
+            }
+            V = Vb + hill;
+        }
+    }
+    reduce_finish(r, w, c, cvv, ac, sv, V, dV, hill, step, mlp_err, mlp_ok, inv_c);
 }
 
 // Values of the CVs whose partial sums come from the force pass (unbiased local-Q
@@ -2941,6 +3049,10 @@ hipError_t launch_cv_tail(const ReduceDev& r, hipStream_t st) {
 hipError_t launch_reduce(const ReduceDev& r, hipStream_t st) {
     if (r.n_walkers == 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce, dim3(r.n_walkers), dim3(kRedThreads), 0, st, r);
+    if (r.kind != BIAS_GRID) {
+        hipLaunchKernelGGL(k_hills_part, dim3(kHillBlocks, r.n_walkers), dim3(kHillThreads), 0, st, r);
+        hipLaunchKernelGGL(k_bias_tail, dim3(r.n_walkers), dim3(64), 0, st, r);
+    }
     return hipGetLastError();
 }
 
