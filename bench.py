@@ -60,6 +60,17 @@ def timed_run(fix, step0, steps, per_launch=None):
     return time.perf_counter() - t0
 
 
+def timed_plain(fix, step0, steps):
+    """Wall time of `steps` host-driven steps (plain launches, one post_force call and
+    state read per step)."""
+    fix.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        fix.post_force(step0 + k)
+    fix.sync()
+    return time.perf_counter() - t0
+
+
 def oracle_steps(fn, budget_s):
     """Run fn() until budget_s of CPU time is used (at least once); returns (count, seconds)."""
     k, t0 = 0, time.perf_counter()
@@ -345,10 +356,10 @@ def run_c5(mdt, args):
     kt = 300.0 * KB
     base = C5_BASE
     res = {}
-    steps = 2000
+    steps = args.steps if args.plain else 2000
     nh0 = 19800
     hist = None
-    for kind in ("hills", "mlp"):
+    for kind in args.c5_kinds.split(","):
         ctx = mdt.Context(0, 1)
         ctx.bind_system(0, system)
         extra = "Gaussian_Hill_type 2" if kind == "hills" else ("Gaussian_Hill_type 3 MLP_REFIT 500 "
@@ -369,8 +380,11 @@ def run_c5(mdt, args):
             fix.mlp_refit()
             fix.sync()
             t_fit = time.perf_counter() - t0
-        fix.run(1, 20)
-        dt = timed_run(fix, 21, steps)
+        if args.plain:
+            timed_plain(fix, 1, 20)
+        else:
+            fix.run(1, 20)
+        dt = timed_plain(fix, 21, steps) if args.plain else timed_run(fix, 21, steps)
         r = {"atom_steps_per_s": n * steps / dt, "us_per_step": dt * 1e6 / steps, "hills_end": int(fix.hills(3).shape[0])}
         if kind == "mlp":
             st = fix.mlp_status()
@@ -381,11 +395,15 @@ def run_c5(mdt, args):
         fix.close()
         ctx.close()
     cfg = {"workload": "config 5: 256,000-atom config-2 system, 3-D CV (Q4.AVE, Q6.AVE, S(k_111)), "
-                       "sigma 0.01/0.01/0.5, h 0.005, WT 10; 19,800 hills pre-deposited, 2,000 timed steps "
-                       "(+200 hills), learned bias refit every 500 steps on 1M points", "atoms": n, "results": res}
-    return line(res["mlp"]["atom_steps_per_s"], steps, 20, res["mlp"]["us_per_step"] * 1e-3,
-                "bf16 MFMA network (f32 accum), f64 labels", cfg, None, None,
-                metric="atom-steps/s, config 5 (learned bias; exact hills-sum result in config.results)")
+                       "sigma 0.01/0.01/0.5, h 0.005, WT 10; 19,800 hills pre-deposited, "
+                       f"{steps} timed steps (+{steps // 10} hills), learned bias refit every 500 steps on 1M points"
+                       + (", host-driven plain launches" if args.plain else ""), "atoms": n, "results": res}
+    if "mlp" in res:
+        return line(res["mlp"]["atom_steps_per_s"], steps, 20, res["mlp"]["us_per_step"] * 1e-3,
+                    "bf16 MFMA network (f32 accum), f64 labels", cfg, None, None,
+                    metric="atom-steps/s, config 5 (learned bias; exact hills-sum result in config.results)")
+    return line(res["hills"]["atom_steps_per_s"], steps, 20, res["hills"]["us_per_step"] * 1e-3, "f64", cfg,
+                None, None, metric="atom-steps/s, config 5 (exact hills sum)")
 
 
 def run_lq(mdt, args):
@@ -525,7 +543,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mlp-plain", action="store_true",
                     help="c5: the learned bias trains with plain launches instead of a replayed captured epoch")
+    ap.add_argument("--c5-kinds", default="hills,mlp",
+                    help="c5: bias kinds to run, comma-separated, from hills (exact sum) and mlp (learned)")
+    ap.add_argument("--plain", action="store_true",
+                    help="c5: time --steps host-driven steps (plain launches) instead of 2,000 replayed graph steps")
     args = ap.parse_args()
+    if any(k not in ("hills", "mlp") for k in args.c5_kinds.split(",")):
+        ap.error("--c5-kinds takes hills and/or mlp")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not built)", file=sys.stderr)

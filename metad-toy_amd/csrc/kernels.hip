@@ -1611,6 +1611,9 @@ __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, 
 // ---------------------------------------------------------------------------
 constexpr int kRedThreads = 1024;
 constexpr int kRedGridLds = 4096;     // grids up to this many bins are read from an LDS copy
+// AFFINE: every DIM program is affine (r.affine), so the expression interpreter, an
+// out-of-line call with a stack frame, is left out of that instance.
+template <bool AFFINE>
 __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     const int w = blockIdx.x;
     __shared__ double sh[kMaxSlotCV * (kRedThreads / 64)];
@@ -1644,11 +1647,11 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
     // lane c's 1/N, loaded with the first round trip
     const double inv_c = slot_lane ? r.cv_inv[(int64_t)w * kMaxSlotCV + lane64] : 0.0;
     double acoef[kMaxDim] = {0, 0, 0};
-    if (r.affine && slot_lane) {
+    if (AFFINE && slot_lane) {
         #pragma unroll
         for (int d = 0; d < kMaxDim; ++d) if (d < r.D) acoef[d] = r.aff[d * kMaxSlotCV + lane64];
     }
-    if (!r.affine) {
+    if (!AFFINE) {
         // expression programs to LDS for the interpreter (thread 0 below)
         const int nwords = r.D * (int)(sizeof(ExprProg) / 4);
         const int* src = reinterpret_cast<const int*>(r.prog);
@@ -1726,7 +1729,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         }
         // 2. dimensions
         double sv[kMaxDim] = {0, 0, 0};
-        if (r.affine) {
+        if constexpr (AFFINE) {
             #pragma unroll
             for (int d = 0; d < kMaxDim; ++d) if (d < r.D) sv[d] = r.aff0[d] + wave_sum(acoef[d] * cvv);
             if (c == 0) {
@@ -2732,15 +2735,23 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
         float run[kSkTile * 2];
         #pragma unroll
         for (int q = 0; q < kSkTile * 2; ++q) run[q] = 0.0f;
+        // the run's positions and group flags are loaded together, ahead of the arithmetic
+        double4 xv[kSkRun];
+        bool on[kSkRun];
+        #pragma unroll
         for (int r = 0; r < kSkRun; ++r) {
             const int a = a0 + r * blockDim.x;
-            if (a >= count) break;
-            const int i = start + a;
-            if (g.group && !g.group[i]) continue;
-            const double4 xv = g.x4[i];
+            on[r] = a < count;
+            const int i = start + (on[r] ? a : 0);
+            xv[r] = g.x4[i];
+            if (g.group) on[r] = on[r] && g.group[i];
+        }
+        #pragma unroll
+        for (int r = 0; r < kSkRun; ++r) {
+            if (!on[r]) continue;
             #pragma unroll
             for (int q = 0; q < kSkTile; ++q) {
-                const double ph = xv.x * kx[q] + xv.y * ky[q] + xv.z * kz[q];
+                const double ph = xv[r].x * kx[q] + xv[r].y * ky[q] + xv[r].z * kz[q];
                 float sn, cs;
                 sincos_rev(ph, sn, cs);
                 run[2 * q] += cs;
@@ -2750,13 +2761,21 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
         #pragma unroll
         for (int q = 0; q < kSkTile * 2; ++q) acc[q] += (double)run[q];
     }
-    block_sum<kSkTile * 2>(acc, sh);
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < kSkTile; ++q) {
-            if (kb + q >= sk.nk) break;
-            sk.apart[((int64_t)ch * sk.nk + kb + q) * 2] = acc[2 * q];
-            sk.apart[((int64_t)ch * sk.nk + kb + q) * 2 + 1] = acc[2 * q + 1];
-        }
+    // the block tree of block_sum, finished in parallel: wave sums, then thread q adds
+    // value q's four wave sums in wave order and writes it
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    #pragma unroll
+    for (int q = 0; q < kSkTile * 2; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) sh[q * (kBlock / 64) + wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kSkTile * 2) {
+        const int q = threadIdx.x;
+        double a = 0.0;
+        #pragma unroll
+        for (int k = 0; k < kBlock / 64; ++k) a += sh[q * (kBlock / 64) + k];
+        if (kb + q / 2 < sk.nk) sk.apart[((int64_t)ch * sk.nk + kb) * 2 + q] = a;
     }
 }
 
@@ -2789,6 +2808,76 @@ __global__ __launch_bounds__(kBlock) void k_sk_fold(SkDev sk) {
     sk.sterm[(int64_t)w * sk.nk + q] = (re * re + im * im) / N;
 }
 
+// k_sk_fold and k_sk_cv in one workgroup per walker (blockIdx.x) when there are at most
+// kSkFoldAll k vectors: every k's chunk partials in the order and block tree of
+// k_sk_fold, then SK.AVE in the order of k_sk_cv, so the results are bitwise those
+// of the two kernels.
+constexpr int kSkFoldAll = 8;
+__global__ __launch_bounds__(kBlock) void k_sk_fold_cv(SkDev sk) {
+    constexpr int NW = kBlock / 64;
+    __shared__ double sh[2 * kSkFoldAll * NW];
+    __shared__ double st[kSkFoldAll];
+    const int w = blockIdx.x;
+    const int nk = sk.nk;
+    const int c0 = sk.chunk_begin[w], c1 = sk.chunk_begin[w + 1];
+    double v2[2 * kSkFoldAll];
+    #pragma unroll
+    for (int q = 0; q < 2 * kSkFoldAll; ++q) v2[q] = 0.0;
+    // a thread's chunks c0 + t, c0 + t + 256, ... in that order, four chunks' loads in
+    // flight together
+    constexpr int U = 4;
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += U * kBlock) {
+        double2 v[U][kSkFoldAll];
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double2* pc = reinterpret_cast<const double2*>(sk.apart) + (int64_t)(c + u * kBlock) * nk;
+            #pragma unroll
+            for (int q = 0; q < kSkFoldAll; ++q) v[u][q] = (q < nk && c + u * kBlock < c1) ? pc[q] : make_double2(0.0, 0.0);
+        }
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (c + u * kBlock < c1) {
+                #pragma unroll
+                for (int q = 0; q < kSkFoldAll; ++q) { v2[2 * q] += v[u][q].x; v2[2 * q + 1] += v[u][q].y; }
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    #pragma unroll
+    for (int q = 0; q < 2 * kSkFoldAll; ++q) {
+        if (q < 2 * nk) {
+            const double v = wave_sum(v2[q]);
+            if (lane == 0) sh[q * NW + wv] = v;
+        }
+    }
+    __syncthreads();
+    const double N = sk.n_norm ? sk.n_norm[w] : (double)sk.n_atoms[w];
+    if (threadIdx.x < nk) {
+        const int q = threadIdx.x;
+        double re = 0.0, im = 0.0;
+        #pragma unroll
+        for (int k = 0; k < NW; ++k) re += sh[2 * q * NW + k];
+        #pragma unroll
+        for (int k = 0; k < NW; ++k) im += sh[(2 * q + 1) * NW + k];
+        sk.amp[((int64_t)w * nk + q) * 2] = re;
+        sk.amp[((int64_t)w * nk + q) * 2 + 1] = im;
+        SkK kp;
+        for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
+        kp.ar = (float)re; kp.ai = (float)im;
+        kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
+        sk.kpack[(int64_t)w * nk + q] = kp;
+        const double term = (re * re + im * im) / N;
+        sk.sterm[(int64_t)w * nk + q] = term;
+        st[q] = term;
+    }
+    __syncthreads();
+    if (wv == 0) {
+        // k_sk_cv's tree: lane q holds term q (the other waves of k_sk_cv add zeros)
+        const double v = wave_sum(lane < nk ? st[lane] : 0.0);
+        if (lane == 0) sk.direct[(int64_t)w * kMaxSlotCV + sk.slot] = v / nk;
+    }
+}
+
 // The gradient record and |A|^2 / N of amplitudes already in sk.amp (summed over the
 // contexts that each hold a share of the atoms): walker blockIdx.y, k = the thread.
 __global__ __launch_bounds__(kBlock) void k_sk_pack(SkDev sk) {
@@ -2816,25 +2905,30 @@ __global__ __launch_bounds__(kBlock) void k_sk_cv(SkDev sk) {
     if (threadIdx.x == 0) sk.direct[(int64_t)w * kMaxSlotCV + sk.slot] = v[0] / sk.nk;
 }
 
-// F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j}); one atom per thread, the k loop
-// index is uniform so the packed k data load through the scalar path
-// kSkAtoms atoms per thread (strided by the workgroup size): each wave-uniform k
-// record, loaded through the scalar cache, feeds kSkAtoms terms per lane, which
-// hides the load of the next record behind the arithmetic.
+// F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j}).  The k loop index is uniform,
+// so the packed k records load through the scalar path.  A thread takes AT atoms
+// (strided by the workgroup size): each k record feeds AT terms per lane, which hides
+// the load of the next record behind the arithmetic.  With few k vectors (at most
+// kSkFewK) a thread takes one atom, so that the grid has four times the workgroups to
+// cover the memory latency, and the k records are staged in LDS in one round trip
+// instead of one dependent load per record; an atom's terms are added in the same
+// order either way.
 constexpr int kSkAtoms = 4;
+constexpr int kSkFewK = 64;
+template <int AT>
 __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f) {
     const int w = blockIdx.y;
-    const int a0 = blockIdx.x * blockDim.x * kSkAtoms + threadIdx.x;
+    const int a0 = blockIdx.x * blockDim.x * AT + threadIdx.x;
     const int n = sk.n_atoms[w];
-    if ((int)(blockIdx.x * blockDim.x * kSkAtoms) >= n) return;
+    if ((int)(blockIdx.x * blockDim.x * AT) >= n) return;
     const double wt = sk.wcv[(int64_t)w * kWcvStride + sk.wslot];
     const double nn = sk.n_norm ? sk.n_norm[w] : (double)n;
     const float scale = (float)(wt * 2.0 / (nn * sk.nk));
-    int ii[kSkAtoms];
-    double px[kSkAtoms], py[kSkAtoms], pz[kSkAtoms];
-    float gx[kSkAtoms], gy[kSkAtoms], gz[kSkAtoms];
+    int ii[AT];
+    double px[AT], py[AT], pz[AT];
+    float gx[AT], gy[AT], gz[AT];
     #pragma unroll
-    for (int u = 0; u < kSkAtoms; ++u) {
+    for (int u = 0; u < AT; ++u) {
         const int a = a0 + u * (int)blockDim.x;
         ii[u] = sk.atom_begin[w] + (a < n ? a : 0);
         const double4 xv = g.x4[ii[u]];
@@ -2842,11 +2936,18 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
         gx[u] = gy[u] = gz[u] = 0.0f;
     }
     const SkK* kp = sk.kpack + (int64_t)w * sk.nk;
+    if constexpr (AT == 1) {
+        // few k vectors: all records to LDS with one round trip, read back as broadcasts
+        __shared__ SkK ks[kSkFewK];
+        if ((int)threadIdx.x < sk.nk) ks[threadIdx.x] = kp[threadIdx.x];
+        __syncthreads();
+        kp = ks;
+    }
     #pragma unroll 2
     for (int q = 0; q < sk.nk; ++q) {
         const SkK k = kp[q];
         #pragma unroll
-        for (int u = 0; u < kSkAtoms; ++u) {
+        for (int u = 0; u < AT; ++u) {
             const double ph = px[u] * k.kr[0] + py[u] * k.kr[1] + pz[u] * k.kr[2];
             float sn, cs;
             sincos_rev(ph, sn, cs);
@@ -2857,7 +2958,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
         }
     }
     #pragma unroll
-    for (int u = 0; u < kSkAtoms; ++u) {
+    for (int u = 0; u < AT; ++u) {
         const int a = a0 + u * (int)blockDim.x;
         const int i = ii[u];
         if (a < n && (g.group == nullptr || g.group[i])) {
@@ -3048,7 +3149,8 @@ hipError_t launch_cv_tail(const ReduceDev& r, hipStream_t st) {
 
 hipError_t launch_reduce(const ReduceDev& r, hipStream_t st) {
     if (r.n_walkers == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3(r.n_walkers), dim3(kRedThreads), 0, st, r);
+    if (r.affine) hipLaunchKernelGGL(k_reduce<true>, dim3(r.n_walkers), dim3(kRedThreads), 0, st, r);
+    else hipLaunchKernelGGL(k_reduce<false>, dim3(r.n_walkers), dim3(kRedThreads), 0, st, r);
     if (r.kind != BIAS_GRID) {
         hipLaunchKernelGGL(k_hills_part, dim3(kHillBlocks, r.n_walkers), dim3(kHillThreads), 0, st, r);
         hipLaunchKernelGGL(k_bias_tail, dim3(r.n_walkers), dim3(64), 0, st, r);
@@ -3097,6 +3199,10 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
     hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (s.nk <= kSkFoldAll) {
+        hipLaunchKernelGGL(k_sk_fold_cv, dim3(s.n_walkers), dim3(kBlock), 0, st, s);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_sk_fold, dim3(s.nk, s.n_walkers), dim3(kBlock), 0, st, s);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -3116,8 +3222,13 @@ hipError_t launch_cv_sums(const ReduceDev& r, const int32_t* psum, double* out, 
 
 hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t st) {
     // grid.y = walkers, grid.x covers the largest walker
-    dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
-    hipLaunchKernelGGL(k_sk_force, grid, dim3(kBlock), 0, st, g, s, f);
+    if (s.nk <= kSkFewK) {
+        dim3 grid((s.max_atoms + kBlock - 1) / kBlock, s.n_walkers);
+        hipLaunchKernelGGL(k_sk_force<1>, grid, dim3(kBlock), 0, st, g, s, f);
+    } else {
+        dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
+        hipLaunchKernelGGL(k_sk_force<kSkAtoms>, grid, dim3(kBlock), 0, st, g, s, f);
+    }
     return hipGetLastError();
 }
 
