@@ -258,15 +258,29 @@ __device__ __forceinline__ void tile_forward(const float* sW1, const float* sb1,
     }
 }
 
+// The network's parameters to LDS by a 256-thread workgroup, every load issued before
+// the first LDS store (one memory round trip); W2's BF16 halves move as 32-bit words
+// (sW2 and sW2l 4-byte aligned).
+constexpr int kNetThreads = 256;
 __device__ __forceinline__ void load_net(const MlpNet* net, float* sW1, float* sb1, short* sW2, short* sW2l, float* sb2,
                                          float* sW3) {
-    for (int k = threadIdx.x; k < H * 3; k += blockDim.x) sW1[k] = net->p[kW1 + k];
-    for (int k = threadIdx.x; k < H; k += blockDim.x) {
-        sb1[k] = net->p[kB1 + k];
-        sb2[k] = net->p[kB2 + k];
-        sW3[k] = net->p[kW3 + k];
-    }
-    for (int k = threadIdx.x; k < H * H; k += blockDim.x) { sW2[k] = net->w2bf[k]; sW2l[k] = net->w2lo[k]; }
+    static_assert(H * 3 <= kNetThreads && (H * H / 2) % kNetThreads == 0, "layout assumed below");
+    constexpr int NW2 = H * H / 2 / kNetThreads;
+    const int t = threadIdx.x;
+    const uint32_t* gh = reinterpret_cast<const uint32_t*>(net->w2bf);
+    const uint32_t* gl = reinterpret_cast<const uint32_t*>(net->w2lo);
+    uint32_t vh[NW2], vl[NW2];
+    #pragma unroll
+    for (int u = 0; u < NW2; ++u) { vh[u] = gh[t + u * kNetThreads]; vl[u] = gl[t + u * kNetThreads]; }
+    const float w1 = t < H * 3 ? net->p[kW1 + t] : 0.0f;
+    float b1 = 0.0f, b2 = 0.0f, w3 = 0.0f;
+    if (t < H) { b1 = net->p[kB1 + t]; b2 = net->p[kB2 + t]; w3 = net->p[kW3 + t]; }
+    uint32_t* dh = reinterpret_cast<uint32_t*>(sW2);
+    uint32_t* dl = reinterpret_cast<uint32_t*>(sW2l);
+    #pragma unroll
+    for (int u = 0; u < NW2; ++u) { dh[t + u * kNetThreads] = vh[u]; dl[t + u * kNetThreads] = vl[u]; }
+    if (t < H * 3) sW1[t] = w1;
+    if (t < H) { sb1[t] = b1; sb2[t] = b2; sW3[t] = w3; }
 }
 
 // ---------------------------------------------------------------------------
@@ -277,7 +291,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
                                                     const float4* __restrict__ s_in, const double* __restrict__ v_in,
                                                     float* __restrict__ gpart) {
     __shared__ float sW1[H * 3], sb1[H], sb2[H], sW3[H];
-    __shared__ short sW2[H * H], sW2l[H * H];
+    __shared__ __attribute__((aligned(16))) short sW2[H * H];
+    __shared__ __attribute__((aligned(16))) short sW2l[H * H];
     __shared__ float h1s[4][16 * H];
     __shared__ float d2s[4][16 * H];
     __shared__ float grad[kMlpP];
@@ -286,15 +301,24 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
     // one training step per launch: its Adam update (the next kernel) reads the advanced
     // step count, so a captured epoch replays with the right step and learning rate
     if (blockIdx.x == 0 && threadIdx.x == 0 && m.adam_t) m.adam_t[w] += 1;
-    load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
-    for (int k = threadIdx.x; k < kMlpP; k += blockDim.x) grad[k] = 0.0f;
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
     const int64_t p0 = first + (int64_t)blockIdx.x * 64 + wv * 16;
+    // the tile's points and the labels of the lane's four points are loaded together
+    // with the network
+    float4 sv = make_float4(0, 0, 0, 0);
+    if (l < 16 && p0 + l < first + count) sv = s_in[(p0 + l) % n_total];
+    double lab[4];
+    #pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int64_t p = p0 + 4 * g + t;
+        lab[t] = p < first + count ? v_in[p % n_total] : 0.0;
+    }
+    load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
+    for (int k = threadIdx.x; k < kMlpP; k += blockDim.x) grad[k] = 0.0f;
     // normalised inputs of the tile
     if (l < 16) {
         const int64_t p = p0 + l;
         const bool ok = p < first + count;
-        const float4 sv = ok ? s_in[p % n_total] : make_float4(0, 0, 0, 0);
         xs[wv][l][0] = ok ? (float)((sv.x - net->in_shift[0]) * net->in_scale[0]) : 0.0f;
         xs[wv][l][1] = ok ? (float)((sv.y - net->in_shift[1]) * net->in_scale[1]) : 0.0f;
         xs[wv][l][2] = ok ? (float)((sv.z - net->in_shift[2]) * net->in_scale[2]) : 0.0f;
@@ -311,9 +335,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
     #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int pl = 4 * g + t;
-        const int64_t p = p0 + pl;
         const float wt = xs[wv][pl][3];
-        const float tgt = wt > 0.0f ? (float)(v_in[p % n_total] * (double)osc) : 0.0f;
+        const float tgt = wt > 0.0f ? (float)(lab[t] * (double)osc) : 0.0f;
         dy[t] = wt * inv_b * (T.y[t] - tgt);
     }
     // delta2[p][n] = dy[p] W3[n] (1 - h2^2); dW3, db2, db3
@@ -480,16 +503,18 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const MlpNet* __restrict__ 
                                                       const float4* __restrict__ s_in, const double* __restrict__ v_ref,
                                                       double* __restrict__ v_out, double* __restrict__ blk_err) {
     __shared__ float sW1[H * 3], sb1[H], sb2[H], sW3[H];
-    __shared__ short sW2[H * H], sW2l[H * H];
+    __shared__ __attribute__((aligned(16))) short sW2[H * H];
+    __shared__ __attribute__((aligned(16))) short sW2l[H * H];
     __shared__ float h1s[4][16 * H];
     __shared__ float xs[4][16][4];
     __shared__ double red[256];
-    load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
     const int64_t p0 = (int64_t)blockIdx.x * 64 + wv * 16;
+    // the tile's points are loaded together with the network
+    float4 sv = make_float4(0, 0, 0, 0);
+    if (l < 16 && p0 + l < n) sv = s_in[p0 + l];
+    load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
     if (l < 16) {
-        const int64_t p = p0 + l;
-        const float4 sv = p < n ? s_in[p] : make_float4(0, 0, 0, 0);
         xs[wv][l][0] = (float)((sv.x - net->in_shift[0]) * net->in_scale[0]);
         xs[wv][l][1] = (float)((sv.y - net->in_shift[1]) * net->in_scale[1]);
         xs[wv][l][2] = (float)((sv.z - net->in_shift[2]) * net->in_scale[2]);
