@@ -174,6 +174,10 @@ enum BiasKind : int32_t { BIAS_GRID = 0, BIAS_HILLS = 2, BIAS_MLP = 3 };
 constexpr int kMlpH = 64;
 constexpr int kW1 = 0, kB1 = kW1 + kMlpH * 3, kW2 = kB1 + kMlpH, kB2 = kW2 + kMlpH * kMlpH,
               kW3 = kB2 + kMlpH, kB3 = kW3 + kMlpH, kMlpP = kB3 + 1;
+// training points per workgroup (one wave) of a training step; a step of `count` points
+// writes one gradient row of kMlpP floats per tile
+constexpr int kMlpTile = 16;
+inline int64_t mlp_grad_rows(int64_t count) { return (count + kMlpTile - 1) / kMlpTile; }
 struct MlpNet {
     float p[kMlpP + 3];
     short w2bf[kMlpH * kMlpH];      // BF16(W2)

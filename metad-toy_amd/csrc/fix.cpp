@@ -1501,7 +1501,7 @@ void fix_build(mdt_fix* fx) {
             const int64_t ntot = fx->mlp_points + fx->mlp_holdout;
             fx->mlp_sdata.alloc((size_t)ntot);
             fx->mlp_vdata.alloc((size_t)ntot);
-            fx->mlp_gpart.alloc((size_t)((fx->mlp_batch + 63) / 64) * kMlpP);
+            fx->mlp_gpart.alloc((size_t)mlp_grad_rows(fx->mlp_batch) * kMlpP);
             fx->mlp_blkerr.alloc((size_t)((fx->mlp_holdout + 63) / 64));
             for (int w = 0; w < nw; ++w) HIPCHK(mlp_init(fx->mlp_live.p + w, fx->mlp_seed + 977 * w, st));
             fx->mlp_has_fit.assign(nw, 0);
@@ -3001,9 +3001,8 @@ int mdt_mlp_loss_grad(mdt_ctx* ctx, const float* params, const double* norm, int
     DBuf<double> dv;
     dv.alloc((size_t)std::max<int64_t>(np, 1));
     dv.upload(v, (size_t)np, ctx->stream);
-    const int nb = (int)((np + 63) / 64);
     DBuf<float> gpart, gsum, am, av;
-    gpart.alloc((size_t)std::max(nb, 1) * kMlpP);
+    gpart.alloc((size_t)std::max<int64_t>(mlp_grad_rows(np), 1) * kMlpP);
     gsum.alloc(kMlpP); am.alloc(kMlpP); av.alloc(kMlpP);
     MlpDev m{};
     m.D = 3; m.stage = net.p; m.live = net.p; m.adam_m = am.p; m.adam_v = av.p;

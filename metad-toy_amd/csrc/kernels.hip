@@ -27,6 +27,7 @@
 #include <utility>
 
 #include "engine.h"
+#include "mlp_tanh.h"
 
 namespace mdt {
 namespace {
@@ -1433,13 +1434,13 @@ __device__ double mlp_eval_block(const MlpNet* net, const double* s, double* dV)
         float z = net->p[kB1 + a];
         #pragma unroll
         for (int d = 0; d < 3; ++d) z = __builtin_fmaf(net->p[kW1 + 3 * a + d], x[d], z);
-        h1s[a] = tanhf(z);
+        h1s[a] = mlp_tanh(z);
     }
     __syncthreads();
     if (a < kMlpH) {
         float z = net->p[kB2 + a];
         for (int b = 0; b < kMlpH; ++b) z = __builtin_fmaf(net->p[kW2 + a * kMlpH + b], h1s[b], z);
-        const float h = tanhf(z);
+        const float h = mlp_tanh(z);
         ys[a] = net->p[kW3 + a] * h;
         g2s[a] = net->p[kW3 + a] * (1.0f - h * h);
     }

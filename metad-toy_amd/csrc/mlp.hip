@@ -8,12 +8,12 @@
 //                  the exact FP64 sum of Gaussians (terms with exponent > 40,
 //                  i.e. below 4e-18 of a hill height, are skipped)
 //   k_mlp_stats    input normalisation and output scale of a training set
-//   k_mlp_train    fused forward + backward of a 16-point tile per wave:
-//                  hidden 64x64 layer and its two backward products on BF16
-//                  MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation), the
-//                  3-wide and 1-wide layers on FP32 VALU; deterministic block
-//                  gradient partials
-//   k_mlp_adam     gradient reduction over blocks + Adam update, BF16 copy of W2
+//   k_mlp_train    fused forward + backward of a 16-point tile per one-wave
+//                  workgroup: hidden 64x64 layer and its two backward products
+//                  on BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation),
+//                  the 3-wide and 1-wide layers on FP32 VALU; one gradient row
+//                  per tile
+//   k_mlp_adam     gradient rows summed in a fixed order + Adam update, BF16 copy of W2
 //                  (the training loop of a fit is replayed as one captured graph)
 //   k_mlp_forward  batched MFMA inference (held-out check, bias evaluation)
 //   k_mlp_commit   held-out gate: adopt the fitted network or fall back to the
@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "engine.h"
+#include "mlp_tanh.h"
 
 namespace mdt {
 namespace {
@@ -59,7 +60,7 @@ __device__ __forceinline__ double gauss(uint64_t& s) {
     return sqrt(-2.0 * log(a)) * cos(6.283185307179586 * b);
 }
 
-__device__ __forceinline__ float tanh_f(float z) { return tanhf(z); }
+__device__ __forceinline__ float tanh_f(float z) { return mlp_tanh(z); }
 
 // ---------------------------------------------------------------------------
 // Training and held-out points with exact FP64 labels (the sum of every hill).
@@ -258,51 +259,55 @@ __device__ __forceinline__ void tile_forward(const float* sW1, const float* sb1,
     }
 }
 
-// The network's parameters to LDS by a 256-thread workgroup, every load issued before
+// The network's parameters to LDS by an NT-thread workgroup, every load issued before
 // the first LDS store (one memory round trip); W2's BF16 halves move as 32-bit words
 // (sW2 and sW2l 4-byte aligned).
-constexpr int kNetThreads = 256;
+template <int NT>
 __device__ __forceinline__ void load_net(const MlpNet* net, float* sW1, float* sb1, short* sW2, short* sW2l, float* sb2,
                                          float* sW3) {
-    static_assert(H * 3 <= kNetThreads && (H * H / 2) % kNetThreads == 0, "layout assumed below");
-    constexpr int NW2 = H * H / 2 / kNetThreads;
+    static_assert(NT >= H && (H * H / 2) % NT == 0, "layout assumed below");
+    constexpr int NW2 = H * H / 2 / NT, NW1 = (H * 3 + NT - 1) / NT;
     const int t = threadIdx.x;
     const uint32_t* gh = reinterpret_cast<const uint32_t*>(net->w2bf);
     const uint32_t* gl = reinterpret_cast<const uint32_t*>(net->w2lo);
     uint32_t vh[NW2], vl[NW2];
     #pragma unroll
-    for (int u = 0; u < NW2; ++u) { vh[u] = gh[t + u * kNetThreads]; vl[u] = gl[t + u * kNetThreads]; }
-    const float w1 = t < H * 3 ? net->p[kW1 + t] : 0.0f;
+    for (int u = 0; u < NW2; ++u) { vh[u] = gh[t + u * NT]; vl[u] = gl[t + u * NT]; }
+    float w1[NW1];
+    #pragma unroll
+    for (int u = 0; u < NW1; ++u) w1[u] = t + u * NT < H * 3 ? net->p[kW1 + t + u * NT] : 0.0f;
     float b1 = 0.0f, b2 = 0.0f, w3 = 0.0f;
     if (t < H) { b1 = net->p[kB1 + t]; b2 = net->p[kB2 + t]; w3 = net->p[kW3 + t]; }
     uint32_t* dh = reinterpret_cast<uint32_t*>(sW2);
     uint32_t* dl = reinterpret_cast<uint32_t*>(sW2l);
     #pragma unroll
-    for (int u = 0; u < NW2; ++u) { dh[t + u * kNetThreads] = vh[u]; dl[t + u * kNetThreads] = vl[u]; }
-    if (t < H * 3) sW1[t] = w1;
+    for (int u = 0; u < NW2; ++u) { dh[t + u * NT] = vh[u]; dl[t + u * NT] = vl[u]; }
+    #pragma unroll
+    for (int u = 0; u < NW1; ++u) if (t + u * NT < H * 3) sW1[t + u * NT] = w1[u];
     if (t < H) { sb1[t] = b1; sb2[t] = b2; sW3[t] = w3; }
 }
 
 // ---------------------------------------------------------------------------
-// k_mlp_train: forward + backward of 64 points per block (16 per wave),
-// block partial of dL/dparams for L = (1/B) sum (y - t)^2
+// k_mlp_train: forward + backward of one 16-point tile per workgroup (one wave);
+// the tile's dL/dparams for L = (1/B) sum (y - t)^2 is written as gradient row
+// blockIdx.x (every parameter once), and the Adam kernel adds the rows in order.
+// One wave per tile gives the step ceil(B / 16) workgroups to spread over the CUs.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t first, int64_t count, int64_t n_total,
-                                                    const float4* __restrict__ s_in, const double* __restrict__ v_in,
-                                                    float* __restrict__ gpart) {
+__global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, int w, int64_t first, int64_t count, int64_t n_total,
+                                                   const float4* __restrict__ s_in, const double* __restrict__ v_in,
+                                                   float* __restrict__ gpart) {
     __shared__ float sW1[H * 3], sb1[H], sb2[H], sW3[H];
     __shared__ __attribute__((aligned(16))) short sW2[H * H];
     __shared__ __attribute__((aligned(16))) short sW2l[H * H];
-    __shared__ float h1s[4][16 * H];
-    __shared__ float d2s[4][16 * H];
-    __shared__ float grad[kMlpP];
-    __shared__ float xs[4][16][4];
+    __shared__ float h1s[16 * H];
+    __shared__ float d2s[16 * H];
+    __shared__ float xs[16][4];
     const MlpNet* net = m.stage + w;
     // one training step per launch: its Adam update (the next kernel) reads the advanced
     // step count, so a captured epoch replays with the right step and learning rate
     if (blockIdx.x == 0 && threadIdx.x == 0 && m.adam_t) m.adam_t[w] += 1;
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
-    const int64_t p0 = first + (int64_t)blockIdx.x * 64 + wv * 16;
+    const int l = threadIdx.x, r = l & 15, g = l >> 4;
+    const int64_t p0 = first + (int64_t)blockIdx.x * kMlpTile;
     // the tile's points and the labels of the lane's four points are loaded together
     // with the network
     float4 sv = make_float4(0, 0, 0, 0);
@@ -313,21 +318,19 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
         const int64_t p = p0 + 4 * g + t;
         lab[t] = p < first + count ? v_in[p % n_total] : 0.0;
     }
-    load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
-    for (int k = threadIdx.x; k < kMlpP; k += blockDim.x) grad[k] = 0.0f;
+    load_net<64>(net, sW1, sb1, sW2, sW2l, sb2, sW3);
     // normalised inputs of the tile
     if (l < 16) {
-        const int64_t p = p0 + l;
-        const bool ok = p < first + count;
-        xs[wv][l][0] = ok ? (float)((sv.x - net->in_shift[0]) * net->in_scale[0]) : 0.0f;
-        xs[wv][l][1] = ok ? (float)((sv.y - net->in_shift[1]) * net->in_scale[1]) : 0.0f;
-        xs[wv][l][2] = ok ? (float)((sv.z - net->in_shift[2]) * net->in_scale[2]) : 0.0f;
-        xs[wv][l][3] = ok ? 1.0f : 0.0f;   // point weight (0 pads the last tile)
+        const bool ok = p0 + l < first + count;
+        xs[l][0] = ok ? (float)((sv.x - net->in_shift[0]) * net->in_scale[0]) : 0.0f;
+        xs[l][1] = ok ? (float)((sv.y - net->in_shift[1]) * net->in_scale[1]) : 0.0f;
+        xs[l][2] = ok ? (float)((sv.z - net->in_shift[2]) * net->in_scale[2]) : 0.0f;
+        xs[l][3] = ok ? 1.0f : 0.0f;   // point weight (0 pads the last tile)
     }
     __syncthreads();
     TileFwd T;
-    tile_forward(sW1, sb1, sW2, sW2l, sb2, sW3, net->p[kB3], xs[wv][r], h1s[wv], T);
-    __syncthreads();   // h1s written by other lanes of the wave
+    tile_forward(sW1, sb1, sW2, sW2l, sb2, sW3, net->p[kB3], xs[r], h1s, T);
+    __syncthreads();   // h1s written by other lanes
     // dL/dy
     const float inv_b = 2.0f / (float)count;
     const float osc = (float)(1.0 / net->out_scale);
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
     #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int pl = 4 * g + t;
-        const float wt = xs[wv][pl][3];
+        const float wt = xs[pl][3];
         const float tgt = wt > 0.0f ? (float)(lab[t] * (double)osc) : 0.0f;
         dy[t] = wt * inv_b * (T.y[t] - tgt);
     }
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
             gW3[nb] = __builtin_fmaf(dy[t], h, gW3[nb]);
             d2[nb][t] = dy[t] * sW3[n] * (1.0f - h * h);
             gb2[nb] += d2[nb][t];
-            d2s[wv][(4 * g + t) * H + n] = d2[nb][t];
+            d2s[(4 * g + t) * H + n] = d2[nb][t];
         }
     #pragma unroll
     for (int t = 0; t < 4; ++t) gb3 += dy[t];
@@ -362,9 +365,9 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
         gb2[nb] += __shfl_xor(gb2[nb], 16, 64); gb2[nb] += __shfl_xor(gb2[nb], 32, 64);
     }
     gb3 += __shfl_xor(gb3, 16, 64); gb3 += __shfl_xor(gb3, 32, 64);
-    __syncthreads();   // d2s written by other lanes of the wave
+    __syncthreads();   // d2s written by other lanes
+    float* out = gpart + (int64_t)blockIdx.x * kMlpP;
     // dW2[n][k] = sum_p delta2[p][n] h1[p][k]: A[n][p] (lane: n = 16nb + r, p = 4g+t), B[p][k] = h1[p][k]
-    f32x4 gw2[4][4];
     #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         bf16x4 a;
@@ -374,12 +377,13 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
         for (int kb = 0; kb < 4; ++kb) {
             bf16x4 b;
             #pragma unroll
-            for (int t = 0; t < 4; ++t) b[t] = f2bf(h1s[wv][(4 * g + t) * H + 16 * kb + r]);
-            gw2[nb][kb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, f32x4{0, 0, 0, 0}, 0, 0, 0);
+            for (int t = 0; t < 4; ++t) b[t] = f2bf(h1s[(4 * g + t) * H + 16 * kb + r]);
+            const f32x4 gw = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, f32x4{0, 0, 0, 0}, 0, 0, 0);
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) out[kW2 + (16 * nb + 4 * g + t) * H + 16 * kb + r] = gw[t];
         }
     }
     // delta1[p][k] = (sum_n delta2[p][n] W2[n][k]) (1 - h1^2): A[p][n] from d2s, B[n][k] = W2[n][k]
-    float gW1[4][3], gb1[4];
     #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
         f32x4 acc = {0, 0, 0, 0};
@@ -388,86 +392,74 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev m, int w, int64_t firs
             bf16x4 a, b;
             #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                a[t] = f2bf(d2s[wv][r * H + 16 * ns + 4 * g + t]);
+                a[t] = f2bf(d2s[r * H + 16 * ns + 4 * g + t]);
                 b[t] = sW2[(16 * ns + 4 * g + t) * H + 16 * kb + r];
             }
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
         // lane holds S[p = 4g + t][k = 16 kb + r]
-        gb1[kb] = 0.0f;
-        gW1[kb][0] = gW1[kb][1] = gW1[kb][2] = 0.0f;
+        float gb1 = 0.0f, gW1[3] = {0.0f, 0.0f, 0.0f};
         #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int pl = 4 * g + t;
-            const float h = h1s[wv][pl * H + 16 * kb + r];
+            const float h = h1s[pl * H + 16 * kb + r];
             const float d1 = acc[t] * (1.0f - h * h);
-            gb1[kb] += d1;
-            gW1[kb][0] = __builtin_fmaf(d1, xs[wv][pl][0], gW1[kb][0]);
-            gW1[kb][1] = __builtin_fmaf(d1, xs[wv][pl][1], gW1[kb][1]);
-            gW1[kb][2] = __builtin_fmaf(d1, xs[wv][pl][2], gW1[kb][2]);
+            gb1 += d1;
+            gW1[0] = __builtin_fmaf(d1, xs[pl][0], gW1[0]);
+            gW1[1] = __builtin_fmaf(d1, xs[pl][1], gW1[1]);
+            gW1[2] = __builtin_fmaf(d1, xs[pl][2], gW1[2]);
         }
-        gb1[kb] += __shfl_xor(gb1[kb], 16, 64); gb1[kb] += __shfl_xor(gb1[kb], 32, 64);
-        for (int d = 0; d < 3; ++d) { gW1[kb][d] += __shfl_xor(gW1[kb][d], 16, 64); gW1[kb][d] += __shfl_xor(gW1[kb][d], 32, 64); }
-    }
-    // accumulate the four waves in a fixed order
-    for (int q = 0; q < 4; ++q) {
-        __syncthreads();
-        if (wv != q) continue;
-        #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
-            #pragma unroll
-            for (int kb = 0; kb < 4; ++kb)
-                #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    grad[kW2 + (16 * nb + 4 * g + t) * H + 16 * kb + r] += gw2[nb][kb][t];
+        gb1 += __shfl_xor(gb1, 16, 64); gb1 += __shfl_xor(gb1, 32, 64);
+        for (int d = 0; d < 3; ++d) { gW1[d] += __shfl_xor(gW1[d], 16, 64); gW1[d] += __shfl_xor(gW1[d], 32, 64); }
         if (g == 0) {
-            #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                grad[kW3 + 16 * nb + r] += gW3[nb];
-                grad[kB2 + 16 * nb + r] += gb2[nb];
-            }
-            #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
-                const int k = 16 * kb + r;
-                grad[kB1 + k] += gb1[kb];
-                grad[kW1 + 3 * k] += gW1[kb][0];
-                grad[kW1 + 3 * k + 1] += gW1[kb][1];
-                grad[kW1 + 3 * k + 2] += gW1[kb][2];
-            }
-            if (r == 0) grad[kB3] += gb3;
+            const int k = 16 * kb + r;
+            out[kB1 + k] = gb1;
+            out[kW1 + 3 * k] = gW1[0];
+            out[kW1 + 3 * k + 1] = gW1[1];
+            out[kW1 + 3 * k + 2] = gW1[2];
         }
     }
-    __syncthreads();
-    float* out = gpart + (int64_t)blockIdx.x * kMlpP;
-    for (int k = threadIdx.x; k < kMlpP; k += blockDim.x) out[k] = grad[k];
+    if (g == 0) {
+        #pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            out[kW3 + 16 * nb + r] = gW3[nb];
+            out[kB2 + 16 * nb + r] = gb2[nb];
+        }
+        if (r == 0) out[kB3] = gb3;
+    }
 }
 
-// sum of block partials (fixed order) + Adam.  A workgroup covers 64 parameters:
-// thread t sums the partials b = q, q+4, ... (q = t / 64) of parameter t % 64 with
-// independent loads, and the four group sums are added in order.
+// sum of the gradient rows (fixed order) + Adam.  A workgroup covers 64 parameters:
+// thread t sums the rows b = q, q+16, ... (q = t / 64) of parameter t % 64 with
+// independent loads (sixteen in flight), and the sixteen group sums are added in order.
 // step: 1-based Adam step within the current fit (the moments restart with each fit);
 // nit > 0: the step is the device counter m.adam_t[w] and the rate lr follows the cosine
 // schedule lr (0.01 + 0.99 (1 + cos(pi (step - 1) / nit)) / 2) over the nit steps of the fit
-__global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, int w, int nblocks, const float* __restrict__ gpart,
+constexpr int kAdamThreads = 1024;
+__global__ __launch_bounds__(kAdamThreads) void k_mlp_adam(MlpDev m, int w, int nblocks, const float* __restrict__ gpart,
                                                    float lr, float* __restrict__ gsum_out, int64_t step, int64_t nit) {
     if (nit > 0) {
         step = m.adam_t[w];
         lr = lr * (float)(0.01 + 0.99 * 0.5 * (1.0 + cos(3.141592653589793 * (double)(step - 1) / (double)nit)));
     }
-    __shared__ float part[4][64];
+    constexpr int NQ = kAdamThreads / 64;
+    __shared__ float part[NQ][64];
     MlpNet* net = m.stage + w;
     float* mo = m.adam_m + (int64_t)w * kMlpP;
     float* ve = m.adam_v + (int64_t)w * kMlpP;
     const int pl = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int k = blockIdx.x * 64 + pl;
+    // the moments and the parameter are loaded together with the gradient rows
+    float m0 = 0.0f, v0 = 0.0f, p0 = 0.0f;
+    if (q == 0 && k < kMlpP && lr > 0.0f) { m0 = mo[k]; v0 = ve[k]; p0 = net->p[k]; }
     float acc = 0.0f;
     if (k < kMlpP) {
-        constexpr int U = 8;
-        for (int b0 = q; b0 < nblocks; b0 += 4 * U) {
+        constexpr int U = 16;
+        for (int b0 = q; b0 < nblocks; b0 += NQ * U) {
             float x[U];
             #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int b = b0 + 4 * u;
+                const int b = b0 + NQ * u;
                 x[u] = b < nblocks ? gpart[(int64_t)b * kMlpP + k] : 0.0f;
             }
             #pragma unroll
@@ -477,17 +469,19 @@ __global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, int w, int nblocks, 
     part[q][pl] = acc;
     __syncthreads();
     if (q == 0 && k < kMlpP) {
-        const float gs = ((part[0][pl] + part[1][pl]) + part[2][pl]) + part[3][pl];
+        float gs = part[0][pl];
+        #pragma unroll
+        for (int u = 1; u < NQ; ++u) gs += part[u][pl];
         if (gsum_out) gsum_out[k] = gs;
         if (lr > 0.0f) {
             const float b1 = 0.9f, b2 = 0.999f;
-            float mm = b1 * mo[k] + (1.0f - b1) * gs;
-            float vv = b2 * ve[k] + (1.0f - b2) * gs * gs;
+            float mm = b1 * m0 + (1.0f - b1) * gs;
+            float vv = b2 * v0 + (1.0f - b2) * gs * gs;
             mo[k] = mm;
             ve[k] = vv;
             const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
             const float upd = lr * (mm / c1) / (sqrtf(vv / c2) + 1e-8f);
-            const float pn = net->p[k] - upd;
+            const float pn = p0 - upd;
             net->p[k] = pn;
             if (k >= kW2 && k < kW2 + H * H) {
                 const short hi = f2bf(pn);
@@ -513,7 +507,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const MlpNet* __restrict__ 
     // the tile's points are loaded together with the network
     float4 sv = make_float4(0, 0, 0, 0);
     if (l < 16 && p0 + l < n) sv = s_in[p0 + l];
-    load_net(net, sW1, sb1, sW2, sW2l, sb2, sW3);
+    load_net<256>(net, sW1, sb1, sW2, sW2l, sb2, sW3);
     if (l < 16) {
         xs[wv][l][0] = (float)((sv.x - net->in_shift[0]) * net->in_scale[0]);
         xs[wv][l][1] = (float)((sv.y - net->in_shift[1]) * net->in_scale[1]);
@@ -622,11 +616,11 @@ hipError_t mlp_stats(const MlpDev& m, int w, int64_t n, const double* v, hipStre
 hipError_t mlp_train_step(const MlpDev& m, int w, int64_t first, int64_t count, int64_t n_total, const float4* s,
                           const double* v, float* gpart, float lr, float* gsum, int64_t step, int64_t nit,
                           hipStream_t st) {
-    const int nb = (int)((count + 63) / 64);
-    hipLaunchKernelGGL(k_mlp_train, dim3(nb), dim3(256), 0, st, m, w, first, count, n_total, s, v, gpart);
+    const int nb = (int)mlp_grad_rows(count);
+    hipLaunchKernelGGL(k_mlp_train, dim3(nb), dim3(64), 0, st, m, w, first, count, n_total, s, v, gpart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpP + 63) / 64), dim3(256), 0, st, m, w, nb, gpart, lr, gsum, step, nit);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpP + 63) / 64), dim3(kAdamThreads), 0, st, m, w, nb, gpart, lr, gsum, step, nit);
     return hipGetLastError();
 }
 hipError_t mlp_forward(const MlpNet* net, int64_t n, const float4* s, const double* vref, double* vout,
