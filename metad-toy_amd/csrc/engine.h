@@ -432,9 +432,9 @@ hipError_t mlp_init(MlpNet* net, uint64_t seed, hipStream_t st);
 // Structure factor
 struct SkK {                  // per walker and k vector, rebuilt every step
     double kr[3];             // k / 2pi (revolutions per length)
+    double phi;               // arg A(k) / 2pi (revolutions)
     float kv[3];              // k
-    float ar, ai;             // A(k) (FP32 copy for the gradient)
-    float pad[3];
+    float amod;               // |A(k)|
 };
 struct SkDev {
     int32_t nk;

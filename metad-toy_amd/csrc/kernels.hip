@@ -2711,6 +2711,7 @@ __device__ __forceinline__ void sincos_rev(double ph, float& s, float& c) {
     s = __builtin_amdgcn_sinf(fr);
     c = __builtin_amdgcn_cosf(fr);
 }
+__device__ __forceinline__ float sin_rev(double ph) { return __builtin_amdgcn_sinf((float)(ph - rint(ph))); }
 
 // A(k) partial sums: workgroup (k tile, atom chunk); k vectors are uniform per
 // workgroup and live in scalar registers.
@@ -2780,6 +2781,16 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
     }
 }
 
+// The gradient record of k vector q with amplitude A = re + i im: the force pass takes
+// Im(conj(A) e^{i k.r}) = |A| sin(k.r - arg A), one sine per atom and k.
+__device__ __forceinline__ SkK sk_record(const SkDev& sk, int q, double re, double im) {
+    SkK kp;
+    for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
+    kp.phi = atan2(im, re) * 0.15915494309189535;     // 1 / 2pi
+    kp.amod = (float)sqrt(re * re + im * im);
+    return kp;
+}
+
 // per walker: A(k) = sum over chunks, S = |A|^2/N, value = mean S, packed k data
 // Amplitudes of walker w = blockIdx.y, k = blockIdx.x: a workgroup sums that k's
 // chunk partials, thread t taking chunks t, t+256, ... in order, then the
@@ -2801,11 +2812,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_fold(SkDev sk) {
     const double N = sk.n_norm ? sk.n_norm[w] : (double)sk.n_atoms[w];
     sk.amp[((int64_t)w * sk.nk + q) * 2] = re;
     sk.amp[((int64_t)w * sk.nk + q) * 2 + 1] = im;
-    SkK kp;
-    for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
-    kp.ar = (float)re; kp.ai = (float)im;
-    kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
-    sk.kpack[(int64_t)w * sk.nk + q] = kp;
+    sk.kpack[(int64_t)w * sk.nk + q] = sk_record(sk, q, re, im);
     sk.sterm[(int64_t)w * sk.nk + q] = (re * re + im * im) / N;
 }
 
@@ -2862,11 +2869,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_fold_cv(SkDev sk) {
         for (int k = 0; k < NW; ++k) im += sh[(2 * q + 1) * NW + k];
         sk.amp[((int64_t)w * nk + q) * 2] = re;
         sk.amp[((int64_t)w * nk + q) * 2 + 1] = im;
-        SkK kp;
-        for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
-        kp.ar = (float)re; kp.ai = (float)im;
-        kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
-        sk.kpack[(int64_t)w * nk + q] = kp;
+        sk.kpack[(int64_t)w * nk + q] = sk_record(sk, q, re, im);
         const double term = (re * re + im * im) / N;
         sk.sterm[(int64_t)w * nk + q] = term;
         st[q] = term;
@@ -2887,11 +2890,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_pack(SkDev sk) {
     if (q >= sk.nk) return;
     const double re = sk.amp[((int64_t)w * sk.nk + q) * 2], im = sk.amp[((int64_t)w * sk.nk + q) * 2 + 1];
     const double N = sk.n_norm ? sk.n_norm[w] : (double)sk.n_atoms[w];
-    SkK kp;
-    for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
-    kp.ar = (float)re; kp.ai = (float)im;
-    kp.pad[0] = kp.pad[1] = kp.pad[2] = 0.0f;
-    sk.kpack[(int64_t)w * sk.nk + q] = kp;
+    sk.kpack[(int64_t)w * sk.nk + q] = sk_record(sk, q, re, im);
     sk.sterm[(int64_t)w * sk.nk + q] = (re * re + im * im) / N;
 }
 
@@ -2906,8 +2905,9 @@ __global__ __launch_bounds__(kBlock) void k_sk_cv(SkDev sk) {
     if (threadIdx.x == 0) sk.direct[(int64_t)w * kMaxSlotCV + sk.slot] = v[0] / sk.nk;
 }
 
-// F_j = w (2/(N K)) sum_k k Im(conj(A_k) e^{i k.r_j}).  The k loop index is uniform,
-// so the packed k records load through the scalar path.  A thread takes AT atoms
+// F_j = w (2/(N K)) sum_k k |A_k| sin(k.r_j - arg A_k) (= k Im(conj(A_k) e^{i k.r_j})).
+// The k loop index is uniform, so the packed k records load through the scalar path.
+// A thread takes AT atoms
 // (strided by the workgroup size): each k record feeds AT terms per lane, which hides
 // the load of the next record behind the arithmetic.  With few k vectors (at most
 // kSkFewK) a thread takes one atom, so that the grid has four times the workgroups to
@@ -2949,10 +2949,8 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
         const SkK k = kp[q];
         #pragma unroll
         for (int u = 0; u < AT; ++u) {
-            const double ph = px[u] * k.kr[0] + py[u] * k.kr[1] + pz[u] * k.kr[2];
-            float sn, cs;
-            sincos_rev(ph, sn, cs);
-            const float imv = __builtin_fmaf(k.ar, sn, -k.ai * cs);
+            const double ph = px[u] * k.kr[0] + py[u] * k.kr[1] + pz[u] * k.kr[2] - k.phi;
+            const float imv = k.amod * sin_rev(ph);
             gx[u] = __builtin_fmaf(k.kv[0], imv, gx[u]);
             gy[u] = __builtin_fmaf(k.kv[1], imv, gy[u]);
             gz[u] = __builtin_fmaf(k.kv[2], imv, gz[u]);
