@@ -170,12 +170,20 @@ def run_c2(mdt, args):
     fix.init()
     fix.run(0, args.warmup, per_launch=1)
     step = args.warmup
-    # headline: one graph launch per step, what a caller that integrates between steps pays
-    dt, ts1 = windows(fix, step, args.steps, args.windows, 1)
+    # headline: one step at a time, what a caller that integrates between steps pays, as
+    # one graph launch per step or as the step's kernels launched one by one (the faster
+    # of the two is the headline; both are reported)
+    dtg, tsg = windows(fix, step, args.steps, args.windows, 1)
     step += args.windows * args.steps
+    log(f"c2, one graph launch per step: {dtg * 1e6 / args.steps:.1f} us/step (median of {args.windows}), "
+        f"{n * args.steps / dtg:.3e} atom-steps/s")
+    dtp, tsp = windows(fix, step, args.steps, args.windows, 0)
+    step += args.windows * args.steps
+    log(f"c2, plain kernel launches: {dtp * 1e6 / args.steps:.1f} us/step (median of {args.windows}), "
+        f"{n * args.steps / dtp:.3e} atom-steps/s")
+    plain = dtp < dtg
+    dt, ts1 = (dtp, tsp) if plain else (dtg, tsg)
     value = n * args.steps / dt
-    log(f"c2, one launch per step: {dt * 1e6 / args.steps:.1f} us/step (median of {args.windows}), "
-        f"{value:.3e} atom-steps/s")
     dt8, ts8 = windows(fix, step, args.steps, args.windows, 8)
     step += args.windows * args.steps
     log(f"c2, 8 steps per graph launch: {dt8 * 1e6 / args.steps:.1f} us/step, {n * args.steps / dt8:.3e} atom-steps/s")
@@ -207,8 +215,13 @@ def run_c2(mdt, args):
     cfg = {"workload": "config 2: 256,000-atom FCC Ar half melted, Q6.AVE+Q4.AVE+LD-LQ6 (N=18, rc=3.9), "
                        "DIM v1+v2 [0,2]x2000 WT", "atoms": n, "selected_pairs": pairs, "kernels_ms": kms,
            "step": "positions read from the engine's device buffer, selection, Q6, Q4, LQ6.AVE, FP64 sums, "
-                   "hill deposit and bias, chain rule, exact forces: one graph launch per step",
+                   "hill deposit and bias, chain rule, exact forces: one step at a time, "
+                   + ("its kernels launched one by one" if plain else "one graph launch per step"),
            "windows_s": ts1, "step_roofline": step_roof,
+           "one_graph_launch_per_step": {"value": n * args.steps / dtg, "ms_per_step": dtg * 1e3 / args.steps,
+                                         "windows_s": tsg},
+           "plain_kernel_launches": {"value": n * args.steps / dtp, "ms_per_step": dtp * 1e3 / args.steps,
+                                     "windows_s": tsp},
            "eight_steps_per_graph_launch": {"value": n * args.steps / dt8, "ms_per_step": dt8 * 1e3 / args.steps,
                                             "windows_s": ts8},
            "neighbor_build": neighbor_build_ms(mdt, system)}

@@ -184,7 +184,8 @@ int mdt_fix_resume(mdt_fix* fix, int64_t step, const double* reduced, double* f,
  * force buffer with no host synchronisation (standalone/batched driver); every
  * step is the whole path of mdt_fix_post_force.  Results are complete after
  * mdt_fix_sync.  mdt_fix_run_steps replays graphs of `steps_per_launch` (1..64)
- * captured steps; mdt_fix_run uses 8.  With a learned bias the launches stop at
+ * captured steps, or with 0 launches each step's kernels one by one (no graph);
+ * mdt_fix_run uses 8.  With a learned bias the launches stop at
  * every refit step.  No reference counterpart: the reference has no driver
  * outside the LAMMPS step loop. */
 int mdt_fix_run(mdt_fix* fix, int64_t step0, int64_t nsteps);

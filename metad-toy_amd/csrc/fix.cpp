@@ -1438,6 +1438,9 @@ void fix_build(mdt_fix* fx) {
     fx->nslot_sum = nsum;
     const int ntiles = (int)ctx->h_tiles.size();
     fx->partial.alloc((size_t)std::max(1, nsum) * std::max(1, ntiles));
+    // zero, so the first step's reduce kernel, which sums the fused local-Q partials of
+    // the step before, reads numbers
+    HIPCHK(launch_fill(fx->partial.p, (int64_t)std::max(1, nsum) * std::max(1, ntiles), 0.0, st));
     fx->direct.alloc((size_t)nw * kMaxSlotCV);
     HIPCHK(launch_fill(fx->direct.p, (int64_t)nw * kMaxSlotCV, 0.0, st));
     fx->state.alloc((size_t)nw * kStStride);
@@ -1989,11 +1992,15 @@ void enqueue_rest(mdt_fix* fx, double* f, Timer& T) {
             T.run("k_lq_force", st, [&] { HIPCHK(launch_lq(6, gs, d, f, st)); });
         }
     }
-    // The values of unbiased local-Q CVs fused into the force pass: the fixed-order
-    // sum of the tile partials the force pass wrote this step.
-    bool tail = false;
-    for (auto* c : fx->cvs) tail |= c->fused;
-    if (tail) T.run("k_cv_tail", st, [&] { HIPCHK(launch_cv_tail(fx->red, st)); });
+    // The values of unbiased local-Q CVs fused into the force pass are the fixed-order
+    // sums of the tile partials that pass writes: the next step's reduce kernel takes
+    // them, and fetch_state launches k_cv_tail for the last step's before a state read.
+}
+
+bool has_fused(const mdt_fix* fx) {
+    for (auto* c : fx->cvs)
+        if (c->fused) return true;
+    return false;
 }
 
 void enqueue_step(mdt_fix* fx, double* f, Timer* tm = nullptr) {
@@ -2025,6 +2032,8 @@ void set_steps(mdt_fix* fx, int64_t step) {
 void fetch_state(mdt_fix* fx) {
     const int nw = fx->nw();
     fx->h_state.resize((size_t)nw * kStStride);
+    // the fused local-Q values of the last step (a repeated read sums the same partials)
+    if (has_fused(fx)) HIPCHK(launch_cv_tail(fx->red, fx->ctx->stream));
     HIPCHK(hipMemcpyAsync(fx->h_state.data(), fx->state.p, sizeof(double) * fx->h_state.size(), hipMemcpyDeviceToHost, fx->ctx->stream));
     int32_t of = 0;
     if (fx->ctx->near_on())
@@ -2501,7 +2510,7 @@ static hipGraphExec_t run_graph(mdt_fix* fx, int k) {
 
 int mdt_fix_run_steps(mdt_fix* fx, int64_t step0, int64_t nsteps, int steps_per_launch) {
     FIX_BEGIN(fx)
-    if (steps_per_launch < 1 || steps_per_launch > 64) throw Err(MDT_EINVAL, "steps_per_launch must be in 1..64");
+    if (steps_per_launch < 0 || steps_per_launch > 64) throw Err(MDT_EINVAL, "steps_per_launch must be in 0..64");
     if (nsteps < 0) throw Err(MDT_EINVAL, "nsteps must be >= 0");
     mdt_ctx* ctx = fx->ctx;
     use_device(ctx);
@@ -2511,6 +2520,15 @@ int mdt_fix_run_steps(mdt_fix* fx, int64_t step0, int64_t nsteps, int steps_per_
     for (auto& b : fx->hills_bound) b += ndep;
     set_steps(fx, step0);
     const int K = steps_per_launch;
+    if (K == 0) {
+        // plain kernel launches, step by step, stopping at refit steps as below
+        for (int64_t s = 0; s < nsteps; ++s) {
+            enqueue_step(fx, ctx->f.p);
+            if (fx->hill_type == 3 && (step0 + s + 1) % fx->mlp_refit == 0) mlp_refit_all(fx);
+        }
+        fx->state_valid = false;
+        return MDT_OK;
+    }
     hipGraphExec_t g1 = run_graph(fx, 1);
     hipGraphExec_t gk = K > 1 ? run_graph(fx, K) : g1;
     // whole graphs of K steps, then single steps; with a learned bias each run of

@@ -1611,6 +1611,12 @@ __device__ __forceinline__ void reduce_finish(const ReduceDev& r, int w, int c, 
 // LDS reads remain on the usual path.
 // ---------------------------------------------------------------------------
 constexpr int kRedThreads = 1024;
+// Partial-sum slot of a CV source: a slot >= 0 is summed from this step's tile partials
+// (k_env, k_ld, k_lq_sum); -1 is a direct value; -2 - p is slot p of an unbiased local Q
+// fused into the force pass, which runs after this kernel, so the sum taken here is
+// the previous step's (the same fixed-order tree as k_cv_tail, which gives the current
+// step's value when the state is read).
+__device__ __forceinline__ int partial_slot(int src) { return src >= 0 ? src : (src <= -2 ? -2 - src : -1); }
 constexpr int kRedGridLds = 4096;     // grids up to this many bins are read from an LDS copy
 // AFFINE: every DIM program is affine (r.affine), so the expression interpreter, an
 // out-of-line call with a stack frame, is left out of that instance.
@@ -1687,7 +1693,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
             #pragma unroll
             for (int g = 0; g < SG; ++g) {
                 const int c = c0 + g;
-                const int src = c < r.ncv ? r.cv_src[c] : -1;
+                const int src = c < r.ncv ? partial_slot(r.cv_src[c]) : -1;
                 const double* pp = r.partial + (int64_t)(src >= 0 ? src : 0) * r.ntiles;
                 #pragma unroll
                 for (int u = 0; u < U; ++u)
@@ -1716,15 +1722,13 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(ReduceDev r) {
         const int c = lane64;
         double cvv = 0.0;
         if (c < r.ncv) {
-            if (src_c >= 0) {
+            if (partial_slot(src_c) >= 0) {
                 double a = 0.0;
                 #pragma unroll
                 for (int q = 0; q < NW; ++q) a += sh[c * NW + q];
                 cvv = a * inv_c;
             } else {
-                // direct values; CVs summed after the force pass (cv_src <= -2) are
-                // unbiased and reduced when the state is read
-                cvv = src_c == -1 ? direct_c : 0.0;
+                cvv = direct_c;                  // a direct value (distance, S(k))
             }
             cv[c] = cvv;
         }

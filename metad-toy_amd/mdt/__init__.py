@@ -395,7 +395,7 @@ class Fix:
 
     def run(self, step0, nsteps, per_launch=None):
         """nsteps steps on the device; per_launch = captured steps per graph launch
-        (default: the library's batch)."""
+        (default: the library's batch), 0 for plain kernel launches."""
         if per_launch is None:
             self._chk(lib().mdt_fix_run(self._h, step0, nsteps))
         else:

@@ -135,9 +135,10 @@ def test_walkers_graph_equals_single(mdt, orc):
 
 
 def test_run_batches_equal_single_launches(mdt):
-    """Steps replayed as graphs of 8 captured steps give the same bits as one launch per
-    step: forces, CVs, the bias state and the deposited hills (well-tempered, PACE 3,
-    23 steps: batches, a remainder, and deposits inside and across batches)."""
+    """Steps replayed as graphs of 8 captured steps, and steps whose kernels are launched
+    one by one, give the same bits as one graph launch per step: forces, CVs, the bias
+    state and the deposited hills (well-tempered, PACE 3, 23 steps: batches, a remainder,
+    and deposits inside and across batches)."""
     s = fcc_system(mdt, 6, sigma=0.15, seed=52)
     n = s.n_local
     args = ("GAUSSIAN 0.01 0.005 10 PACE 3 TEMP 300 HILLS_FILE NONE "
@@ -145,7 +146,7 @@ def test_run_batches_equal_single_launches(mdt):
             "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
             "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE DIM 1 0 2 2000 v1+v2 WT 1")
     out = []
-    for per_launch in (1, 8, None):
+    for per_launch in (1, 8, None, 0):
         ctx = mdt.Context(0, 1)
         ctx.bind_system(0, s)
         fix = mdt.Fix(ctx, args)

@@ -236,14 +236,15 @@ def test_mlp_gain_over_exact_sum(mdt, orc):
 
 def test_mlp_refits_in_batched_runs(mdt):
     """A run of 23 steps with refits every 10 gives the same bits with one step per
-    graph launch and with 8: the launches stop at each refit step, so the fits see
-    the same hills and the steps after them the same network."""
+    graph launch, with 8, and with plain kernel launches: the launches stop at each
+    refit step, so the fits see the same hills and the steps after them the same
+    network."""
     s = fcc_system(mdt, 5, sigma=0.1, seed=9)
     n = s.n_local
     args = (FIX3.replace("PACE 0", "PACE 2") +
             "MLP_REFIT 10 MLP_POINTS 30000 MLP_HOLDOUT 4096 MLP_EPOCHS_FIRST 20 MLP_EPOCHS 4 MLP_BATCH 4096")
     out = []
-    for per_launch in (1, 8):
+    for per_launch in (1, 8, 0):
         ctx = _ctx(mdt)
         ctx.bind_system(0, s)
         fix = mdt.Fix(ctx, args)
@@ -255,8 +256,9 @@ def test_mlp_refits_in_batched_runs(mdt):
         out.append((fix.forces(n), fix.scalar(), fix.hills(3), st["fits"], st["n_fit"], fix.mlp_params()))
         print(f"[mlp batched] per_launch={per_launch} fits={st['fits']} n_fit={st['n_fit']} V={out[-1][1]:.6g}",
               flush=True)
-    a, b = out
-    assert a[3] == b[3] == 2 and a[4] == b[4] == 10
-    assert np.array_equal(a[2], b[2]) and a[2].shape[0] == 12
-    assert all(np.array_equal(u, v) for u, v in zip(a[5], b[5]))
-    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    a = out[0]
+    for b in out[1:]:
+        assert a[3] == b[3] == 2 and a[4] == b[4] == 10
+        assert np.array_equal(a[2], b[2]) and a[2].shape[0] == 12
+        assert all(np.array_equal(u, v) for u, v in zip(a[5], b[5]))
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1]
