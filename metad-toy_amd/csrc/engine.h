@@ -79,6 +79,11 @@ struct Tile {
 // Geometry shared by all per-atom kernels (device pointers).
 struct Geo {
     const double4* x4;        // (n_local_total + n_ghost_total), xyz + pad, FP64
+    // the same FP64 positions as 3 doubles per atom in the x4 order, when the context's
+    // one walker reads them from an attached engine buffer (locals then ghosts, which is
+    // that order); NULL otherwise.  The selection's exact FP64 tests read it, so a step
+    // that has no other FP64 reader need not copy the positions into x4.
+    const double* x64;
     const float4* x4f;        // the same positions rounded to FP32 (list-scan prefilter)
     const int32_t* off;       // n_local_total + 1 (global CSR offsets)
     const int32_t* idx;       // global all-space indices

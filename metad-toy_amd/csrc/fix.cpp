@@ -283,6 +283,7 @@ struct mdt_ctx {
     Geo geo() const {
         Geo g{};
         g.x4 = x4.p; g.x4f = x4f.p; g.off = off.p; g.idx = idx.p; g.owner = owner.p;
+        g.x64 = engine_x64();
         g.eown = nullptr; g.revpos = nullptr; g.cnt = nullptr;
         g.group = nullptr;
         g.share = any_share ? dshare.p : nullptr;
@@ -360,9 +361,21 @@ struct mdt_ctx {
         return w[k];
     }
     void commit();
-    // pack walker k's positions x (n_all*3 device doubles, locals then ghosts)
-    void pack(int k, const double* xd) {
+    // the attached engine buffer of a one-walker context: its FP64 positions in the x4
+    // order (locals then ghosts), read in place by the selection's exact tests
+    const double* engine_x64() const { return w.size() == 1 ? w[0].x_src : nullptr; }
+    bool x4_stale = false;      // the last pack wrote only x4f (x4 holds older positions)
+    // pack walker k's positions x (n_all*3 device doubles, locals then ghosts); without
+    // with_x4 (one attached walker only) the FP64 copy x4 is not written
+    void pack(int k, const double* xd, bool with_x4 = true) {
         WalkerIn& a = w[k];
+        if (!with_x4 && !near_on() && engine_x64() == xd) {
+            HIPCHK(launch_pack_walker(xd, a.n_local, a.n_all, nullptr, x4f.p + a.atom_off,
+                                      nullptr, x4f.p + total_local + a.ghost_off, stream));
+            x4_stale = true;
+            return;
+        }
+        x4_stale = false;
         if (near_on())
             HIPCHK(launch_pack_walker_near(xd, a.n_local, a.n_all, x4.p + a.atom_off, x4f.p + a.atom_off,
                                            x4.p + total_local + a.ghost_off, x4f.p + total_local + a.ghost_off,
@@ -453,6 +466,7 @@ void mdt_ctx::commit() {
     x4.alloc((size_t)(tl + tg));
     x4f.alloc((size_t)(tl + tg));
     HIPCHK(launch_pack_x4(x.p, x4.p, x4f.p, tl + tg, stream));
+    x4_stale = false;
     any_share = false;
     for (auto& a : w) any_share |= !a.share.empty();
     if (any_share) {
@@ -1884,14 +1898,33 @@ LqDev lq_dev(mdt_fix* fx, CvH* c) {
     return d;
 }
 
+// Whether a step of this fix reads the FP64 positions x4 beyond the selection's exact
+// tests: the near-list build and the CVs whose kernels work on FP64 positions.
+bool fx_reads_x4(const mdt_fix* fx) {
+    if (fx->ctx->near_on()) return true;
+    for (auto* c : fx->cvs)
+        if (c->kind == CV_SF || c->kind == CV_LQ || c->kind == CV_SK || c->kind == CV_DIST) return true;
+    return false;
+}
+
+// x4 brought up to the attached positions after a step that packed the FP32 ones alone
+void ensure_x4(mdt_ctx* ctx) {
+    if (!ctx->x4_stale) return;
+    for (size_t k = 0; k < ctx->w.size(); ++k)
+        if (ctx->w[k].x_src) ctx->pack((int)k, ctx->w[k].x_src, true);
+}
+
 // The CV values of the step (through the per-atom sums and the direct values) on the
 // context stream: positions, near lists, selection and q_lm, and every value kernel.
 void enqueue_values(mdt_fix* fx, Timer& T) {
     mdt_ctx* ctx = fx->ctx;
     hipStream_t st = ctx->stream;
-    // positions of walkers attached to an engine buffer are read at the start of the step
+    // positions of walkers attached to an engine buffer are read at the start of the step;
+    // a step whose only FP64 reader is the selection (which then reads the one walker's
+    // engine buffer in place) packs the FP32 positions alone
+    const bool x4_readers = fx_reads_x4(fx);
     for (size_t k = 0; k < ctx->w.size(); ++k)
-        if (ctx->w[k].x_src) T.run("k_pack_walker", st, [&] { ctx->pack((int)k, ctx->w[k].x_src); });
+        if (ctx->w[k].x_src) T.run("k_pack_walker", st, [&] { ctx->pack((int)k, ctx->w[k].x_src, x4_readers); });
     if (!fx->envs.empty() && ctx->near_on()) {
         const NearDev nd = ctx->near_dev();
         T.run("k_prune", st, [&] { HIPCHK(launch_prune(ctx->geo(), nd, 0, st)); });
@@ -3043,6 +3076,7 @@ int mdt_fix_virial(mdt_fix* fx, int walker, const int32_t* image, double v[6]) {
     DBuf<int32_t> img;
     if (image) { img.alloc((size_t)std::max(n, 1) * 3); img.upload(image, (size_t)n * 3, ctx->stream); }
     const double L[3] = {a.hi[0] - a.lo[0], a.hi[1] - a.lo[1], a.hi[2] - a.lo[2]};
+    ensure_x4(ctx);
     HIPCHK(launch_virial(ctx->x4.p, ctx->f.p, image ? img.p : nullptr, (int)a.atom_off, n, L, part.p, ctx->stream));
     std::vector<double> h((size_t)std::max(nb, 1) * 6);
     HIPCHK(hipMemcpyAsync(h.data(), part.p, sizeof(double) * nb * 6, hipMemcpyDeviceToHost, ctx->stream));
@@ -3067,6 +3101,7 @@ int mdt_fix_virial_noalloc(mdt_fix* fx, int walker, const int32_t* image, double
     if (!part) throw Err(MDT_ESTATE, "device scratch was not set up (mdt_create)");
     const int nb = (int)((n + kBlock - 1) / kBlock);
     const double L[3] = {a.hi[0] - a.lo[0], a.hi[1] - a.lo[1], a.hi[2] - a.lo[2]};
+    ensure_x4(ctx);
     HIPCHK(launch_virial(ctx->x4.p, ctx->f.p, image, (int)a.atom_off, (int)n, L, part, ctx->stream));
     HIPCHK(launch_virial_sum(part, nb, v, ctx->stream));
     API_END

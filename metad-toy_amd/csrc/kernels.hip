@@ -220,13 +220,20 @@ __device__ __forceinline__ void block_sum(double* v, double* sh) {
 
 // FP64 pair vector and r^2 exactly as the reference evaluates it without
 // contraction: r2 = (dx*dx + dy*dy) + dz*dz.
-__device__ __forceinline__ double pair_r2(const double4* __restrict__ x4, int j, double xi0, double xi1,
-                                          double xi2, double& d0, double& d1, double& d2) {
+// FP64 position of atom j for the selection's exact tests: from the engine buffer when
+// the geometry carries one (x64), else from x4
+__device__ __forceinline__ double3 pos64(const Geo& g, int j) {
+    if (g.x64) {
+        const double* p = g.x64 + 3 * (int64_t)j;
+        return make_double3(p[0], p[1], p[2]);
+    }
+    const double4 v = g.x4[j];
+    return make_double3(v.x, v.y, v.z);
+}
+__device__ __forceinline__ double pair_r2(const Geo& g, int j, double xi0, double xi1, double xi2) {
 #pragma clang fp contract(off)
-    const double4 p = x4[j];
-    d0 = p.x - xi0;
-    d1 = p.y - xi1;
-    d2 = p.z - xi2;
+    const double3 p = pos64(g, j);
+    const double d0 = p.x - xi0, d1 = p.y - xi1, d2 = p.z - xi2;
     return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
@@ -350,7 +357,8 @@ __global__ void k_pack_x4(const double* __restrict__ x, double4* __restrict__ x4
 // read from an attached engine buffer, and in mdt_update_positions.
 // A workgroup takes 256 atoms at a time: their 768 coordinates are read with
 // unit-stride loads (each load instruction touches consecutive lines) into LDS,
-// then each thread packs its own atom.
+// then each thread packs its own atom.  With x4_loc and x4_gh NULL only the FP32
+// packing is written (the step's FP64 reads go to the engine buffer itself).
 __global__ __launch_bounds__(256) void k_pack_walker(const double* __restrict__ x, int64_t n_local, int64_t n_all,
                                                      double4* __restrict__ x4_loc, float4* __restrict__ x4f_loc,
                                                      double4* __restrict__ x4_gh, float4* __restrict__ x4f_gh) {
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(256) void k_pack_walker(const double* __restrict__ 
         const uint32_t w = pack_res1(a, af) | (pack_res1(b, bf) << 10) | (pack_res1(c, cf) << 20);
         const bool loc = i < n_local;
         const int64_t k = loc ? i : i - n_local;
-        (loc ? x4_loc : x4_gh)[k] = make_double4(a, b, c, 0.0);
+        if (x4_loc) (loc ? x4_loc : x4_gh)[k] = make_double4(a, b, c, 0.0);
         (loc ? x4f_loc : x4f_gh)[k] = make_float4(af, bf, cf, __uint_as_float(w));
     }
 }
@@ -494,10 +502,7 @@ __device__ __forceinline__ void literal_insert_group(const Geo& g, int beg, cons
     for (int rd = 0; rd < nr; ++rd) {
         const int p = mnext(it);
         double r2 = __builtin_inf();
-        if (p >= 0) {
-            double d0, d1, d2;
-            r2 = pair_r2(g.x4, g.idx[beg + p], xi0, xi1, xi2, d0, d1, d2);
-        }
+        if (p >= 0) r2 = pair_r2(g, g.idx[beg + p], xi0, xi1, xi2);
         #pragma unroll
         for (int k = 0; k < kLanes; ++k) {
             const int q0 = __shfl(p, src0 + k, 64);
@@ -604,9 +609,8 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
                     in = inrc = true;
                 } else if (!(r2f > r2hi)) {
                     fbl = true;
-                    double d0, d1, d2;
-                    const double4 xi = g.x4[i];     // the FP64 position only on this rare path
-                    const double r2 = pair_r2(g.x4, jj[u], xi.x, xi.y, xi.z, d0, d1, d2);
+                    const double3 xi = pos64(g, i);   // the FP64 position only on this rare path
+                    const double r2 = pair_r2(g, jj[u], xi.x, xi.y, xi.z);
                     inrc = r2 >= 1e-12 && r2 <= rc2s;
                     in = !((r2 > rc2) || (r2 < 1e-12));
                 }
@@ -660,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_env(Geo g, EnvDev e) {
         if (EXPORT || cnt > N) {                     // the cap binds: the literal insertion
             double* sr2 = reinterpret_cast<double*>(&kept[a][0]);
             int* sp = reinterpret_cast<int*>(sr2 + NMAX);
-            const double4 xi = g.x4[i];
+            const double3 xi = pos64(g, i);
             literal_insert_group<NMAX>(g, beg, own, xi.x, xi.y, xi.z, rc2, N, sr2, sp, lane);
             if (lane == 0) {
                 own = M128{0, 0};

@@ -134,3 +134,68 @@ def test_attached_positions_drive_near_list_rebuilds(mdt, orc):
         assert abs(c0[step][0] - q4) <= 1e-5 * q4, step
     moved = np.abs(traj[-1][:n] - traj[0][:n]).max()
     assert moved > 0.2, moved            # far past half the near-list margin (rc / 128)
+
+
+def test_attached_positions_exact_selection_paths(mdt, orc):
+    """A one-walker context reading an attached engine buffer packs only the FP32
+    positions, and its exact FP64 tests read the engine buffer in place.  On a perfect
+    lattice with the cutoff at the nearest-neighbour distance every candidate goes
+    through the FP64 band test, and with N = 4 the cap binds for most atoms (the
+    reference insertion, with exact ties on the unmoved lattice): forces, CVs, the kept sets and the
+    virial (which refreshes the FP64 copy first) equal those of host-bound positions
+    bit for bit, step after step, and the kept sets follow the oracle."""
+    a = 5.26
+    rc = a / np.sqrt(2.0)
+    s = fcc_system(mdt, 5, a=a, sigma=0.0, seed=4)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    mv = Mover(x, n, own)
+    rng = np.random.default_rng(8)
+    traj = [x.copy()]
+    xl = x[:n].copy()
+    for _ in range(2):
+        xl = xl + rng.normal(scale=1e-4, size=xl.shape)
+        traj.append(mv.set_local(xl).copy())
+    N = 4
+    fixargs = q46_fix(pace=1, wt=1, rc=repr(float(rc)), N=N)
+    print(f"[ingest exact] n={n} rc={rc!r} N={N} steps={len(traj)}", flush=True)
+    out = {}
+    for mode in ("host", "attached"):
+        ctx = mdt.Context(0, 1)
+        ctx.bind_system(0, s)
+        fix = mdt.Fix(ctx, fixargs)
+        buf = _Dev(traj[0])
+        if mode == "attached":
+            ctx.attach_positions(0, buf.p.value)
+        res = []
+        for step, xa in enumerate(traj):
+            fix.zero_forces()
+            if mode == "host":
+                ctx.update_positions(0, xa)
+            else:
+                buf.put(xa)
+            fix.post_force(step)
+            kept = fix.extract("Q6.kept", n, N).astype(np.int64)
+            sel, cnt, ns = fix.selection("Q6", n, N)
+            res.append((fix.forces(n), (fix.cv("Q6.AVE"), fix.cv("Q4.AVE"), fix.scalar()), sel, cnt,
+                        fix.virial(), kept))
+        out[mode] = res
+        buf.free()
+        fix.close()
+        ctx.close()
+    for (f0, c0, s0, n0, v0, k0), (f1, c1, s1, n1, v1, k1) in zip(out["host"], out["attached"]):
+        assert np.abs(f0).max() > 0.0
+        assert np.array_equal(f0, f1)
+        assert c0 == c1
+        assert np.array_equal(s0, s1) and np.array_equal(n0, n1) and np.array_equal(k0, k1)
+        assert np.array_equal(v0, v1)
+    # every step's kept set and slot order against the oracle's reference insertion
+    for xa, (f1, c1, sel, cnt, v1, kept) in zip(traj, out["attached"]):
+        env = orc.Env(xa, off, idx, own, n, float(rc), N)
+        want = np.full((n, N), -1, np.int64)
+        for i in range(n):
+            pos = np.sort(env.sel_pos[i][env.sel_pos[i] >= 0])
+            want[i, : pos.size] = idx[off[i] + pos]
+        assert np.array_equal(kept, want)
+        assert np.array_equal(cnt, env.cnt) and np.array_equal(sel, env.sel_idx)
+        assert (env.cnt > N).sum() > n // 2           # the cap binds for most atoms
