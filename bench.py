@@ -60,6 +60,21 @@ def timed_run(fix, step0, steps, per_launch=None):
     return time.perf_counter() - t0
 
 
+def timed_modes(fix, step0, steps, modes):
+    """Times `steps` steps in each run mode of `modes` (captured steps per graph launch, 0
+    for plain kernel launches), one after another from step0; returns (best seconds, its
+    mode, {mode: seconds})."""
+    ts = {}
+    for k, m in enumerate(modes):
+        ts[m] = timed_run(fix, step0 + k * steps, steps, m)
+    best = min(ts, key=ts.get)
+    return ts[best], best, ts
+
+
+def mode_name(m):
+    return "plain kernel launches" if m == 0 else f"{m} captured step(s) per graph launch"
+
+
 def timed_plain(fix, step0, steps):
     """Wall time of `steps` host-driven steps (plain launches, one post_force call and
     state read per step)."""
@@ -254,9 +269,9 @@ def run_c1(mdt, args):
     fix.init()
     fix.run(0, args.warmup)
     steps = 10000
-    dt = timed_run(fix, args.warmup, steps)
+    dt, mode, ts = timed_modes(fix, args.warmup, steps, (8, 0))
     value = n * steps / dt
-    kms = fix.profile(args.warmup + steps, args.profile_steps)
+    kms = fix.profile(args.warmup + 2 * steps, args.profile_steps)
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -272,7 +287,8 @@ def run_c1(mdt, args):
                "sample": f"{k} steps of the 4,000-atom Q6 case (selection, Q6.AVE, exact forces), FP64 oracle, "
                          f"OpenMP over atoms on {ci['cores']} threads, {t:.1f} s"}
     cfg = {"workload": "config 1: 4,000-atom FCC Ar (sigma 0.1 A), Q6.AVE on [0,1]x1000, PACE 10, WT 0, "
-                       "10,000 steps", "atoms": n, "kernels_ms": kms}
+                       "10,000 steps", "atoms": n, "kernels_ms": kms, "run_mode": mode_name(mode),
+           "us_per_step_by_mode": {mode_name(m): t * 1e6 / steps for m, t in ts.items()}}
     return line(value, steps, args.warmup, dt * 1e3 / steps, "f32 pair math, f64 sums/grid", cfg, None, cpu,
                 metric="atom-steps/s, config 1 (4,000 atoms, Q6.AVE)")
 
@@ -287,9 +303,9 @@ def run_c3(mdt, args):
     fix.init()
     fix.run(0, 2)
     steps = max(5, args.steps // 100)
-    dt = timed_run(fix, 2, steps)
+    dt, mode, ts = timed_modes(fix, 2, steps, (8, 0))
     value = n * steps / dt
-    kms = fix.profile(2 + steps, 3)
+    kms = fix.profile(2 + 2 * steps, 3)
     nk = 2056
     fl = FLOP_SK * n * nk
     tk = (kms.get("k_sk_amp", 0) + kms.get("k_sk_force", 0)) * 1e-3
@@ -321,7 +337,8 @@ def run_c3(mdt, args):
                          f"OpenMP, {cores} threads), {t:.1f} s, scaled by 2056/{done_k} to a full step"}
     cfg = {"workload": "config 3: 500,000-atom FCC/liquid slab, SK.AVE over 8 (111) + 2048 random k, "
                        "grid [0,50]x1000", "atoms": n, "k_vectors": nk, "cv_value": fix.cv("SK.AVE"),
-           "kernels_ms": kms}
+           "kernels_ms": kms, "run_mode": mode_name(mode),
+           "us_per_step_by_mode": {mode_name(m): t * 1e6 / steps for m, t in ts.items()}}
     return line(value, steps, 2, dt * 1e3 / steps, "f64 phase, f32 sincos, f64 sums", cfg, roof, cpu,
                 metric="atom-steps/s, config 3 (500k atoms, S(k) over 2,056 k)")
 
@@ -395,10 +412,15 @@ def run_c5(mdt, args):
             t_fit = time.perf_counter() - t0
         if args.plain:
             timed_plain(fix, 1, 20)
+            dt = timed_plain(fix, 21, steps)
+            r = {"run_mode": "host-driven post_force calls"}
         else:
             fix.run(1, 20)
-        dt = timed_plain(fix, 21, steps) if args.plain else timed_run(fix, 21, steps)
-        r = {"atom_steps_per_s": n * steps / dt, "us_per_step": dt * 1e6 / steps, "hills_end": int(fix.hills(3).shape[0])}
+            # the same steps (and refits) as 8 captured steps per graph launch and as plain launches
+            dt, mode, ts = timed_modes(fix, 21, steps, (8, 0))
+            r = {"run_mode": mode_name(mode), "us_per_step_by_mode": {mode_name(m): t * 1e6 / steps for m, t in ts.items()}}
+        r.update({"atom_steps_per_s": n * steps / dt, "us_per_step": dt * 1e6 / steps,
+                  "hills_end": int(fix.hills(3).shape[0])})
         if kind == "mlp":
             st = fix.mlp_status()
             r.update({"first_fit_s": t_fit, "heldout_max_dV_kT": st["err"] / kt, "network_in_use": st["ok"],
@@ -409,7 +431,7 @@ def run_c5(mdt, args):
         ctx.close()
     cfg = {"workload": "config 5: 256,000-atom config-2 system, 3-D CV (Q4.AVE, Q6.AVE, S(k_111)), "
                        "sigma 0.01/0.01/0.5, h 0.005, WT 10; 19,800 hills pre-deposited, "
-                       f"{steps} timed steps (+{steps // 10} hills), learned bias refit every 500 steps on 1M points"
+                       f"{steps} timed steps per run mode (+{steps // 10} hills each), learned bias refit every 500 steps on 1M points"
                        + (", host-driven plain launches" if args.plain else ""), "atoms": n, "results": res}
     if "mlp" in res:
         return line(res["mlp"]["atom_steps_per_s"], steps, 20, res["mlp"]["us_per_step"] * 1e-3,
@@ -429,10 +451,10 @@ def run_lq(mdt, args):
     fix.init()
     fix.run(0, 10)
     steps = max(20, args.steps // 20)
-    dt = timed_run(fix, 10, steps)
+    dt, mode, ts = timed_modes(fix, 10, steps, (8, 0))
     value = n * steps / dt
-    log(f"lq: {dt * 1e6 / steps:.1f} us/step, {value:.3e} atom-steps/s")
-    kms = fix.profile(10 + steps, 10)
+    log(f"lq: {dt * 1e6 / steps:.1f} us/step ({mode_name(mode)}), {value:.3e} atom-steps/s")
+    kms = fix.profile(10 + 2 * steps, 10)
     log("kernel mean ms: " + json.dumps(kms))
     x, off, idx, own = system.arrays()
     entries = float(off[-1])          # RATIONAL r_0 = 3 is above cutoff_eps on the whole 5.9 A list
@@ -459,6 +481,7 @@ def run_lq(mdt, args):
     cfg = {"workload": "STEINH L 6 MEAN_SOLID on the 256,000-atom half-melted FCC system (RATIONAL r_0 3.0 "
                        "weights on the 5.9 A list, FERMI r_0 0.35 alpha 10 on LQ_6), DIM [0,2]x2000 WT",
            "atoms": n, "list_entries": entries, "cv_value": fix.cv("LQ.MEAN_SOLID"), "kernels_ms": kms,
+           "run_mode": mode_name(mode), "us_per_step_by_mode": {mode_name(m): t * 1e6 / steps for m, t in ts.items()},
            "flop_per_entry": {"k_lq_q": FLOP_LQ_Q, "k_lq_lq": FLOP_LQ_LQ, "k_lq_h": FLOP_LQ_H, "k_lq_pair": FLOP_LQ_PAIR},
            "path_fp64_tflops": (FLOP_LQ_Q + FLOP_LQ_LQ + FLOP_LQ_H + FLOP_LQ_PAIR) * entries / (dt / steps) / 1e12}
     return line(value, steps, 10, dt * 1e3 / steps, "f64", cfg, roof, cpu,

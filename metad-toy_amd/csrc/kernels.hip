@@ -1026,12 +1026,19 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, EnvDev e, double* __res
     // lists of at most 64 entries (the usual case, uniform over the wave): the
     // lane's share of the low word is walked with plain 64-bit clears
     const bool wide = __ballot(rcm.hi != 0ull) != 0ull;
+    // candidates all within the first 32 list positions (uniform over the wave): 32-bit clears
+    const bool narrow = __ballot(rcm.hi != 0ull || (rcm.lo >> 32) != 0ull) == 0ull;
     M128 it = rcm;
     mdrop(it, lane);
     uint64_t it64 = it.lo;
+    uint32_t it32 = (uint32_t)it.lo;
     auto next_p = [&]() {
         int q;
-        if (!wide) {
+        if (narrow) {
+            q = __ffs(it32) - 1;
+            #pragma unroll
+            for (int k = 0; k < kLanes; ++k) it32 &= it32 - 1u;
+        } else if (!wide) {
             q = (int)__ffsll((unsigned long long)it64) - 1;
             #pragma unroll
             for (int k = 0; k < kLanes; ++k) it64 &= it64 - 1ull;
