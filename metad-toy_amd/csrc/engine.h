@@ -435,18 +435,32 @@ hipError_t mlp_copy_net(const MlpNet* src, MlpNet* dst, hipStream_t st);
 hipError_t mlp_init(MlpNet* net, uint64_t seed, hipStream_t st);
 
 // Structure factor
+// Above this many k vectors the structure-factor passes over (atom, k) take
+// tile-relative FP32 phases (SkDev.perm, tbeg, tref, dr): their per-step preparation
+// (one pass over the atoms in Hilbert order, and the force written through the
+// permutation) costs less than the FP64 phase work it saves.
+constexpr int kSkTiledK = 256;
 struct SkK {                  // per walker and k vector, rebuilt every step
     double kr[3];             // k / 2pi (revolutions per length)
     double phi;               // arg A(k) / 2pi (revolutions)
     float kv[3];              // k
     float amod;               // |A(k)|
+    float krf[3];             // k / 2pi rounded to FP32 (the tile-relative phase)
+    float pad;
 };
 struct SkDev {
     int32_t nk;
     const double* k;          // nk * 3 (k / 2pi, revolutions per length)
     const double* kvec;       // nk * 3 (k itself)
     int32_t nchunk;
-    const int32_t* chunk;     // per chunk: start, count, walker (3 ints)
+    const int32_t* chunk;     // per chunk: start, count, walker (3 ints), in slots
+    // slot -> local atom index: each walker's atoms ordered along a Hilbert curve, so
+    // that every run of 256 slots (a phase tile) is a compact set of atoms
+    const int32_t* perm;
+    const int32_t* tbeg;      // per walker: its first phase tile (n_walkers + 1)
+    double4* tref;            // per phase tile: the FP64 position of its first atom (this step)
+    float4* dr;               // per slot: FP32 offset of the atom from its tile's reference;
+                              // w = 1 when the atom adds to the sums (its group), else 0
     double* apart;            // [chunk][nk][2]
     double* amp;              // per walker [nk][2]
     SkK* kpack;               // per walker [nk]
@@ -463,6 +477,8 @@ struct SkDev {
     int32_t n_walkers;
     int32_t max_atoms;
 };
+// per-step slot data of the structure-factor passes (tile references and offsets), then
+// the amplitudes
 hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st);
 hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t st);
 // the gradient records and S terms of amplitudes written into s.amp (reduced over contexts)

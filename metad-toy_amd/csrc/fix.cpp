@@ -567,7 +567,9 @@ struct CvH {
     DBuf<double> kd, krev, apart, amp, sterm, nnorm;
     DBuf<SkK> kpack;
     DBuf<float> ampf;
-    DBuf<int32_t> chunk, chunk_begin, natoms, abegin;
+    DBuf<int32_t> chunk, chunk_begin, natoms, abegin, perm, tbeg;
+    DBuf<double4> tref;
+    DBuf<float4> dr;
     int nchunk = 0, max_atoms = 0;
     // DISTANCE: local indices of the two atoms, per-walker offsets, boxes and (d, r)
     int a1 = -1, a2 = -1;
@@ -1222,6 +1224,66 @@ bool prog_uses(const ExprProg& p, int sym) {
 void open_hills(mdt_fix* fx);
 void ensure_hills(mdt_fix* fx, int64_t extra);
 
+// Position of the cell (X[0], X[1], X[2]) (each below 2^b) along the 3-D Hilbert curve,
+// by Skilling's transform to the transposed index ("Programming the Hilbert curve",
+// AIP Conf. Proc. 707, 381 (2004)) and the interleaving of its bits.
+static uint64_t hilbert_key(uint32_t X[3], int b) {
+    const uint32_t M = 1u << (b - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint64_t key = 0;
+    for (int j = b - 1; j >= 0; --j)
+        for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> j) & 1u);
+    return key;
+}
+
+// A walker's local atoms (0 .. n_local-1) ordered along a Hilbert curve over its box
+// cut into 2^b cells per dimension, the cells about kSkCell length units long (ties in
+// atom order), so that every run of 256 consecutive atoms of the order lies within a
+// few neighbouring cells: the structure-factor passes take their phases relative to
+// the first atom of each run.  The grid covers the box exactly, so the curve never
+// leaves it between two occupied cells.
+constexpr double kSkCell = 8.0;
+static std::vector<int32_t> hilbert_order(const WalkerIn& a) {
+    const int64_t n = a.n_local;
+    int b = 1;
+    for (int d = 0; d < 3; ++d) {
+        const double L = a.hi[d] - a.lo[d];
+        while (b < 20 && (double)(1 << b) * kSkCell < L) ++b;
+    }
+    const int nc = 1 << b;
+    std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t X[3];
+        for (int d = 0; d < 3; ++d) {
+            const double L = a.hi[d] - a.lo[d];
+            const double u = L > 0.0 ? (a.x[3 * i + d] - a.lo[d]) / L : 0.0;
+            const int c = (int)std::floor(u * nc);
+            X[d] = (uint32_t)std::min(std::max(c, 0), nc - 1);
+        }
+        key[(size_t)i] = {hilbert_key(X, b), (int32_t)i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> ord((size_t)n);
+    for (int64_t i = 0; i < n; ++i) ord[(size_t)i] = key[(size_t)i].second;
+    return ord;
+}
+
 void fix_build(mdt_fix* fx) {
     mdt_ctx* ctx = fx->ctx;
     if (!ctx->committed) ctx->commit();
@@ -1361,6 +1423,25 @@ void fix_build(mdt_fix* fx) {
             c->sterm.alloc((size_t)nw * nk);
             c->kpack.alloc((size_t)nw * nk);
             if (fx->split) c->nnorm.alloc(nw); else c->nnorm.release();
+            if (nk > kSkTiledK) {
+                // tile-relative phases: slot -> atom, each walker's atoms in Hilbert order
+                // within its own range, and the per-step tile references and offsets
+                std::vector<int32_t> perm((size_t)std::max<int64_t>(ctx->total_local, 1), 0);
+                for (int k = 0; k < nw; ++k) {
+                    const WalkerIn& a = ctx->w[k];
+                    const std::vector<int32_t> ord = hilbert_order(a);
+                    for (int64_t s2 = 0; s2 < a.n_local; ++s2)
+                        perm[(size_t)(a.atom_off + s2)] = (int32_t)(a.atom_off + ord[(size_t)s2]);
+                }
+                c->perm.alloc(perm.size()); c->perm.upload(perm.data(), perm.size(), st);
+                std::vector<int32_t> tb(nw + 1, 0);
+                for (int k = 0; k < nw; ++k) tb[k + 1] = tb[k] + (int32_t)((ctx->w[k].n_local + 255) / 256);
+                c->tbeg.alloc(tb.size()); c->tbeg.upload(tb.data(), tb.size(), st);
+                c->tref.alloc((size_t)std::max(tb[nw], 1));
+                c->dr.alloc(perm.size());
+            } else {
+                c->perm.release(); c->tbeg.release(); c->tref.release(); c->dr.release();
+            }
         } else if (c->kind == CV_SF) {
             const std::vector<double> ngs = group_counts(c->sf_group);
             std::vector<double> inv(nw);
@@ -1951,7 +2032,7 @@ void enqueue_values(mdt_fix* fx, Timer& T) {
             s.nk = (int)(c->kvec.size() / 3);
             s.k = c->krev.p; s.kvec = c->kd.p; s.nchunk = c->nchunk; s.chunk = c->chunk.p;
             s.apart = c->apart.p; s.amp = c->amp.p; s.kpack = c->kpack.p; s.chunk_begin = c->chunk_begin.p;
-            s.sterm = c->sterm.p;
+            s.sterm = c->sterm.p; s.perm = c->perm.p; s.tbeg = c->tbeg.p; s.tref = c->tref.p; s.dr = c->dr.p;
             s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
             s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             s.n_norm = fx->split ? c->nnorm.p : nullptr;
@@ -1999,7 +2080,8 @@ void enqueue_rest(mdt_fix* fx, double* f, Timer& T) {
             s.nk = (int)(c->kvec.size() / 3);
             s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p; s.kpack = c->kpack.p;
             s.wslot = c->slot; s.wcv = fx->wcv.p; s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p;
-            s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
+            s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms; s.perm = c->perm.p;
+            s.tbeg = c->tbeg.p; s.tref = c->tref.p; s.dr = c->dr.p;
             s.n_norm = fx->split ? c->nnorm.p : nullptr;
             Geo gk = g;
             gk.group = g.share;             // a share pushes its own atoms

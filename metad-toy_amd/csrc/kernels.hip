@@ -2718,6 +2718,8 @@ __global__ void k_fill_f(float* p, int64_t n, float v) {
 // FP64 phase reduces to its fractional part before the FP32 sin/cos.
 // ---------------------------------------------------------------------------
 constexpr int kSkTile = 8;    // k vectors per workgroup in the amplitude pass
+constexpr int kSkSlots = 256; // slots of a phase tile (one reference position each)
+constexpr int kSkMaxTiles = 64;   // phase tiles of an amplitude chunk (chunks of <= 16,384 slots)
 
 // sin and cos of 2 pi ph (ph in revolutions): FP64 reduction to [-0.5, 0.5], then the
 // hardware sine/cosine, which take their argument in revolutions
@@ -2728,8 +2730,17 @@ __device__ __forceinline__ void sincos_rev(double ph, float& s, float& c) {
 }
 __device__ __forceinline__ float sin_rev(double ph) { return __builtin_amdgcn_sinf((float)(ph - rint(ph))); }
 
-// A(k) partial sums: workgroup (k tile, atom chunk); k vectors are uniform per
-// workgroup and live in scalar registers.
+// Tile-relative phases.  Each walker's atoms are ordered along a Hilbert curve (sk.perm),
+// so a run of kSkSlots consecutive slots is a compact set of atoms (within ~65 A of its
+// first atom on the benchmark systems).  The phase of atom j in tile t is
+//   k.r_j / 2pi = theta_t(k) + k.(r_j - r_t) / 2pi,   theta_t(k) = frac(k.r_t / 2pi),
+// theta from the first atom's FP64 position once per (tile, k), the rest in FP32 from
+// the FP32 offset r_j - r_t: a few revolutions, so the FP32 phase carries ~1e-5 rad.
+// frac() then keeps the hardware sine's argument in [0, 1) revolutions.
+__device__ __forceinline__ float frac_rev(double ph) { return (float)(ph - floor(ph)); }
+
+// A(k) partial sums with the phase in FP64 (at most kSkTiledK k vectors): workgroup (k
+// tile, atom chunk); k vectors are uniform per workgroup and live in scalar registers.
 __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
     __shared__ double sh[kSkTile * 2 * 4];
     const int kb = blockIdx.x * kSkTile;
@@ -2796,11 +2807,118 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp(Geo g, SkDev sk) {
     }
 }
 
+// Per-step slot data: workgroup (b, w) takes phase tile b of walker w: the tile's
+// reference (its first atom's FP64 position) and every slot's FP32 offset from it, in
+// slot order, so that the passes over (atom, k) read contiguous 16-byte records.
+__global__ __launch_bounds__(kBlock) void k_sk_prep(Geo g, SkDev sk) {
+    static_assert(kBlock == kSkSlots, "a workgroup is one phase tile");
+    const int w = blockIdx.y, b = blockIdx.x;
+    const int n = sk.n_atoms[w];
+    const int a0 = b * kSkSlots;
+    if (a0 >= n) return;
+    const int sb = sk.atom_begin[w];
+    const double4 r = g.x4[sk.perm[sb + a0]];
+    if (threadIdx.x == 0) sk.tref[sk.tbeg[w] + b] = r;
+    const int a = a0 + (int)threadIdx.x;
+    if (a < n) {
+        const int i = sk.perm[sb + a];
+        const double4 x = g.x4[i];
+        const float on = (g.group == nullptr || g.group[i]) ? 1.0f : 0.0f;
+        sk.dr[sb + a] = make_float4((float)(x.x - r.x), (float)(x.y - r.y), (float)(x.z - r.z), on);
+    }
+}
+
+// A(k) partial sums with tile-relative phases (more than kSkTiledK k vectors):
+// workgroup (k tile, atom chunk); a chunk starts on a phase tile of its walker.
+__global__ __launch_bounds__(kBlock) void k_sk_amp_tiled(Geo g, SkDev sk) {
+    static_assert(kBlock == kSkSlots, "a thread's r-th atom of a run lies in the chunk's tile r");
+    __shared__ double sh[kSkTile * 2 * 4];
+    __shared__ float th[kSkMaxTiles][kSkTile];
+    const int kb = blockIdx.x * kSkTile;
+    const int ch = blockIdx.y;
+    const int start = sk.chunk[3 * ch], count = sk.chunk[3 * ch + 1], w = sk.chunk[3 * ch + 2];
+    const int ntl = (count + kSkSlots - 1) / kSkSlots;
+    const double4* tref = sk.tref + sk.tbeg[w] + (start - sk.atom_begin[w]) / kSkSlots;
+    double kx[kSkTile], ky[kSkTile], kz[kSkTile];
+    #pragma unroll
+    for (int q = 0; q < kSkTile; ++q) {
+        const int qq = kb + q < sk.nk ? kb + q : sk.nk - 1;
+        kx[q] = sk.k[3 * qq]; ky[q] = sk.k[3 * qq + 1]; kz[q] = sk.k[3 * qq + 2];
+    }
+    // theta per (tile, k) from the tiles' reference positions
+    for (int e = threadIdx.x; e < ntl * kSkTile; e += blockDim.x) {
+        const int r = e / kSkTile, q = e % kSkTile;
+        const double4 p = tref[r];
+        th[r][q] = frac_rev(p.x * kx[q] + p.y * ky[q] + p.z * kz[q]);
+    }
+    __syncthreads();
+    float kxf[kSkTile], kyf[kSkTile], kzf[kSkTile];
+    #pragma unroll
+    for (int q = 0; q < kSkTile; ++q) { kxf[q] = (float)kx[q]; kyf[q] = (float)ky[q]; kzf[q] = (float)kz[q]; }
+    // FP32 sums over runs of kSkRun atoms of a thread, FP64 across runs and threads:
+    // a run's rounding error (<= kSkRun^2 * 2^-24 in units of |e^{ik.r}| = 1) stays
+    // below 1e-6 of |A| even at a Bragg peak
+    constexpr int kSkRun = 8;
+    double acc[kSkTile * 2];
+    #pragma unroll
+    for (int q = 0; q < kSkTile * 2; ++q) acc[q] = 0.0;
+    for (int a0 = threadIdx.x; a0 < count; a0 += kSkRun * blockDim.x) {
+        float run[kSkTile * 2];
+        #pragma unroll
+        for (int q = 0; q < kSkTile * 2; ++q) run[q] = 0.0f;
+        // the run's offsets and group flags are loaded together, ahead of the arithmetic
+        float4 d[kSkRun];
+        #pragma unroll
+        for (int r = 0; r < kSkRun; ++r) {
+            const int a = a0 + r * blockDim.x;
+            d[r] = a < count ? sk.dr[start + a] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        // slot a0 + r*256 lies in tile a0/256 + r, the same for the whole workgroup
+        const int t0 = a0 / kSkSlots;
+        #pragma unroll
+        for (int r = 0; r < kSkRun; ++r) {
+            if (d[r].w == 0.0f) continue;
+            const int t = t0 + r;
+            #pragma unroll
+            for (int q = 0; q < kSkTile; ++q) {
+                const float ph = __builtin_fmaf(d[r].z, kzf[q], __builtin_fmaf(d[r].y, kyf[q],
+                                                __builtin_fmaf(d[r].x, kxf[q], th[t][q])));
+                const float fr = __builtin_amdgcn_fractf(ph);
+                run[2 * q] += __builtin_amdgcn_cosf(fr);
+                run[2 * q + 1] += __builtin_amdgcn_sinf(fr);
+            }
+        }
+        #pragma unroll
+        for (int q = 0; q < kSkTile * 2; ++q) acc[q] += (double)run[q];
+    }
+    // the block tree of block_sum, finished in parallel: wave sums, then thread q adds
+    // value q's four wave sums in wave order and writes it
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    #pragma unroll
+    for (int q = 0; q < kSkTile * 2; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) sh[q * (kBlock / 64) + wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kSkTile * 2) {
+        const int q = threadIdx.x;
+        double a = 0.0;
+        #pragma unroll
+        for (int k = 0; k < kBlock / 64; ++k) a += sh[q * (kBlock / 64) + k];
+        if (kb + q / 2 < sk.nk) sk.apart[((int64_t)ch * sk.nk + kb) * 2 + q] = a;
+    }
+}
+
 // The gradient record of k vector q with amplitude A = re + i im: the force pass takes
 // Im(conj(A) e^{i k.r}) = |A| sin(k.r - arg A), one sine per atom and k.
 __device__ __forceinline__ SkK sk_record(const SkDev& sk, int q, double re, double im) {
     SkK kp;
-    for (int d = 0; d < 3; ++d) { kp.kr[d] = sk.k[3 * q + d]; kp.kv[d] = (float)sk.kvec[3 * q + d]; }
+    for (int d = 0; d < 3; ++d) {
+        kp.kr[d] = sk.k[3 * q + d];
+        kp.krf[d] = (float)sk.k[3 * q + d];
+        kp.kv[d] = (float)sk.kvec[3 * q + d];
+    }
+    kp.pad = 0.0f;
     kp.phi = atan2(im, re) * 0.15915494309189535;     // 1 / 2pi
     kp.amod = (float)sqrt(re * re + im * im);
     return kp;
@@ -2976,6 +3094,78 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
         const int a = a0 + u * (int)blockDim.x;
         const int i = ii[u];
         if (a < n && (g.group == nullptr || g.group[i])) {
+            f[3 * (int64_t)i] += (double)(scale * gx[u]);
+            f[3 * (int64_t)i + 1] += (double)(scale * gy[u]);
+            f[3 * (int64_t)i + 2] += (double)(scale * gz[u]);
+        }
+    }
+}
+
+// The same force with tile-relative phases (more than kSkTiledK k vectors).
+// A workgroup takes AT phase tiles of kSkSlots slots (thread t the t-th atom of each),
+// so the phase of atom j is theta'_u(k) + k.(r_j - r_u) / 2pi with
+// theta'_u(k) = frac(k.r_u / 2pi - arg A_k / 2pi) from the tile's first atom in FP64,
+// staged in LDS for a block of k vectors at a time; the rest is FP32 (see k_sk_amp).
+// The k loop index is uniform, so the packed k records load through the scalar path,
+// and each record feeds AT terms per lane, added in k order.
+constexpr int kSkForceKB = 1024;   // k vectors whose tile phases are staged at a time
+__global__ __launch_bounds__(kBlock) void k_sk_force_tiled(Geo g, SkDev sk, double* f) {
+    static_assert(kBlock == kSkSlots, "thread t takes slot t of each of the workgroup's tiles");
+    constexpr int AT = kSkAtoms, KB = kSkForceKB;
+    __shared__ float th[AT][KB];
+    const int w = blockIdx.y;
+    const int n = sk.n_atoms[w];
+    const int b0 = blockIdx.x * blockDim.x * AT;       // the workgroup's first slot in the walker
+    if (b0 >= n) return;
+    const int sb = sk.atom_begin[w];
+    const double wt = sk.wcv[(int64_t)w * kWcvStride + sk.wslot];
+    const double nn = sk.n_norm ? sk.n_norm[w] : (double)n;
+    const float scale = (float)(wt * 2.0 / (nn * sk.nk));
+    // the workgroup's tiles: b0 / kSkSlots + u of the walker (those past its end are empty)
+    const double4* tref = sk.tref + sk.tbeg[w] + b0 / kSkSlots;
+    const int ntl = (n - b0 + kSkSlots - 1) / kSkSlots < AT ? (n - b0 + kSkSlots - 1) / kSkSlots : AT;
+    int ii[AT];
+    bool ok[AT];
+    float dx[AT], dy[AT], dz[AT];
+    float gx[AT], gy[AT], gz[AT];
+    #pragma unroll
+    for (int u = 0; u < AT; ++u) {
+        const int a = b0 + u * kSkSlots + (int)threadIdx.x;
+        const float4 d = a < n ? sk.dr[sb + a] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ok[u] = d.w != 0.0f;
+        ii[u] = a < n ? sk.perm[sb + a] : 0;
+        dx[u] = d.x; dy[u] = d.y; dz[u] = d.z;
+        gx[u] = gy[u] = gz[u] = 0.0f;
+    }
+    const SkK* kp = sk.kpack + (int64_t)w * sk.nk;
+    for (int q0 = 0; q0 < sk.nk; q0 += KB) {
+        const int nq = sk.nk - q0 < KB ? sk.nk - q0 : KB;
+        __syncthreads();                               // the last block's phases are read
+        for (int e = threadIdx.x; e < ntl * nq; e += blockDim.x) {
+            const int u = e / nq, qq = e % nq;
+            const SkK& k = kp[q0 + qq];
+            const double4 p = tref[u];
+            th[u][qq] = frac_rev(p.x * k.kr[0] + p.y * k.kr[1] + p.z * k.kr[2] - k.phi);
+        }
+        __syncthreads();
+        #pragma unroll 2
+        for (int qq = 0; qq < nq; ++qq) {
+            const SkK k = kp[q0 + qq];
+            #pragma unroll
+            for (int u = 0; u < AT; ++u) {
+                const float ph = __builtin_fmaf(dz[u], k.krf[2], __builtin_fmaf(dy[u], k.krf[1],
+                                                __builtin_fmaf(dx[u], k.krf[0], th[u][qq])));
+                const float imv = k.amod * __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(ph));
+                gx[u] = __builtin_fmaf(k.kv[0], imv, gx[u]);
+                gy[u] = __builtin_fmaf(k.kv[1], imv, gy[u]);
+                gz[u] = __builtin_fmaf(k.kv[2], imv, gz[u]);
+            }
+        }
+    }
+    #pragma unroll
+    for (int u = 0; u < AT; ++u) {
+        const int i = ii[u];
+        if (ok[u]) {
             f[3 * (int64_t)i] += (double)(scale * gx[u]);
             f[3 * (int64_t)i + 1] += (double)(scale * gy[u]);
             f[3 * (int64_t)i + 2] += (double)(scale * gz[u]);
@@ -3210,7 +3400,17 @@ hipError_t launch_bias_add(const ReduceDev& r, int w, int64_t nh, const double* 
 
 hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
     dim3 grid((s.nk + kSkTile - 1) / kSkTile, s.nchunk);
-    hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
+    if (s.nk > kSkTiledK) {
+        if (s.max_atoms > 0) {
+            hipLaunchKernelGGL(k_sk_prep, dim3((s.max_atoms + kSkSlots - 1) / kSkSlots, s.n_walkers), dim3(kBlock), 0,
+                               st, g, s);
+            hipError_t e0 = hipGetLastError();
+            if (e0 != hipSuccess) return e0;
+        }
+        hipLaunchKernelGGL(k_sk_amp_tiled, grid, dim3(kBlock), 0, st, g, s);
+    } else {
+        hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (s.nk <= kSkFoldAll) {
@@ -3236,7 +3436,10 @@ hipError_t launch_cv_sums(const ReduceDev& r, const int32_t* psum, double* out, 
 
 hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t st) {
     // grid.y = walkers, grid.x covers the largest walker
-    if (s.nk <= kSkFewK) {
+    if (s.nk > kSkTiledK) {
+        dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
+        hipLaunchKernelGGL(k_sk_force_tiled, grid, dim3(kBlock), 0, st, g, s, f);
+    } else if (s.nk <= kSkFewK) {
         dim3 grid((s.max_atoms + kBlock - 1) / kBlock, s.n_walkers);
         hipLaunchKernelGGL(k_sk_force<1>, grid, dim3(kBlock), 0, st, g, s, f);
     } else {

@@ -21,11 +21,14 @@ def _kvecs(seed, nrand, a=5.26):
     return np.array(ks)
 
 
-def test_structure_factor_value_and_forces(mdt, orc):
+@pytest.mark.parametrize("nrand", [40, 300])
+def test_structure_factor_value_and_forces(mdt, orc, nrand):
+    """48 k vectors take the FP64 phase; 308 (above 256) the tile-relative FP32 phase
+    over atoms in Hilbert order (the force written through that order)."""
     s = fcc_system(mdt, 6, sigma=0.3, seed=12)
     x, off, idx, own = s.arrays()
     n = s.n_local
-    k = _kvecs(3, 40)
+    k = _kvecs(3, nrand)
     kv = " ".join(f"KVEC {a:.17g} {b:.17g} {c:.17g}" for a, b, c in k)
     ctx = mdt.Context(0, 1)
     ctx.bind_system(0, s)
@@ -44,20 +47,30 @@ def test_structure_factor_value_and_forces(mdt, orc):
     assert rel_force_err(f, F_ref) <= F_TOL
 
 
-def test_structure_factor_large_phase(mdt, orc):
-    """Phases of ~1e3 rad (500k-atom slab scale): FP64 phase reduction keeps S(k) at 1e-5."""
+@pytest.mark.parametrize("nrand", [56, 300])
+def test_structure_factor_large_phase(mdt, orc, nrand):
+    """Phases of ~1e3 rad (500k-atom slab scale): the FP64 phase reduction (64 k) and the
+    tile-relative phases (308 k) keep S(k) at 1e-5 and the forces at 1e-4."""
     L = 273.52
     s = mdt.System().set_box([0, 0, 0], [L, L, L]).add_uniform(20000, 0.0, L, seed=4).build(5.9)
     x = s.arrays()[0][: s.n_local]
-    k = _kvecs(9, 56)
+    n = s.n_local
+    k = _kvecs(9, nrand)
     kv = " ".join(f"KVEC {a:.17g} {b:.17g} {c:.17g}" for a, b, c in k)
     ctx = mdt.Context(0, 1)
     ctx.bind_system(0, s)
     fix = mdt.Fix(ctx, f"GAUSSIAN 0.5 0.005 10 PACE 0 HILLS_FILE NONE CAL NAME SK SK all {kv} "
                        "CV_dim 1 SYMBOL v1 SK.AVE DIM 1 0 50 1000 v1 WT 0")
-    fix.post_force(0)
     v_ref = orc.sk(x, k)[0]
+    print(f"[sk large phase] n={n} nk={len(k)} S.AVE={v_ref:.4f}", flush=True)
+    fix.init()
+    fix.add_hills(np.array([[v_ref + 0.3]]), np.array([0.005]), 0.5)
+    f = np.zeros((n, 3))
+    fix.post_force(0, f)
     assert abs(fix.cv("SK.AVE") - v_ref) <= CV_TOL * abs(v_ref)
+    w = fix.cv_weight("SK.AVE")
+    assert w != 0.0
+    assert rel_force_err(f, -orc.sk_grad(x, k, w)) <= F_TOL
 
 
 def _walker_systems(mdt):
