@@ -461,6 +461,7 @@ struct SkDev {
     double4* tref;            // per phase tile: the FP64 position of its first atom (this step)
     float4* dr;               // per slot: FP32 offset of the atom from its tile's reference;
                               // w = 1 when the atom adds to the sums (its group), else 0
+    int32_t wide;             // tile phases may exceed 128 revolutions: reduce them first
     double* apart;            // [chunk][nk][2]
     double* amp;              // per walker [nk][2]
     SkK* kpack;               // per walker [nk]

@@ -568,6 +568,7 @@ struct CvH {
     DBuf<SkK> kpack;
     DBuf<float> ampf;
     DBuf<int32_t> chunk, chunk_begin, natoms, abegin, perm, tbeg;
+    bool sk_wide = false;   // tile-relative phases may exceed 128 revolutions
     DBuf<double4> tref;
     DBuf<float4> dr;
     int nchunk = 0, max_atoms = 0;
@@ -1427,12 +1428,26 @@ void fix_build(mdt_fix* fx) {
                 // tile-relative phases: slot -> atom, each walker's atoms in Hilbert order
                 // within its own range, and the per-step tile references and offsets
                 std::vector<int32_t> perm((size_t)std::max<int64_t>(ctx->total_local, 1), 0);
+                double ext = 0.0, kmax = 0.0;   // largest |r_j - r_t| of a tile, largest |k| / 2pi
                 for (int k = 0; k < nw; ++k) {
                     const WalkerIn& a = ctx->w[k];
                     const std::vector<int32_t> ord = hilbert_order(a);
-                    for (int64_t s2 = 0; s2 < a.n_local; ++s2)
+                    for (int64_t s2 = 0; s2 < a.n_local; ++s2) {
                         perm[(size_t)(a.atom_off + s2)] = (int32_t)(a.atom_off + ord[(size_t)s2]);
+                        const int64_t t = ord[(size_t)(s2 / 256 * 256)], j = ord[(size_t)s2];
+                        double d2 = 0.0;
+                        for (int d = 0; d < 3; ++d) {
+                            const double dd = a.x[3 * j + d] - a.x[3 * t + d];
+                            d2 += dd * dd;
+                        }
+                        ext = std::max(ext, std::sqrt(d2));
+                    }
                 }
+                for (int q = 0; q < nk; ++q)
+                    kmax = std::max(kmax, std::sqrt(krev[3 * q] * krev[3 * q] + krev[3 * q + 1] * krev[3 * q + 1] +
+                                                    krev[3 * q + 2] * krev[3 * q + 2]));
+                // atoms move a few length units between list bindings: 16 of margin
+                c->sk_wide = (ext + 16.0) * kmax + 1.0 > 128.0;
                 c->perm.alloc(perm.size()); c->perm.upload(perm.data(), perm.size(), st);
                 std::vector<int32_t> tb(nw + 1, 0);
                 for (int k = 0; k < nw; ++k) tb[k + 1] = tb[k] + (int32_t)((ctx->w[k].n_local + 255) / 256);
@@ -2033,6 +2048,7 @@ void enqueue_values(mdt_fix* fx, Timer& T) {
             s.k = c->krev.p; s.kvec = c->kd.p; s.nchunk = c->nchunk; s.chunk = c->chunk.p;
             s.apart = c->apart.p; s.amp = c->amp.p; s.kpack = c->kpack.p; s.chunk_begin = c->chunk_begin.p;
             s.sterm = c->sterm.p; s.perm = c->perm.p; s.tbeg = c->tbeg.p; s.tref = c->tref.p; s.dr = c->dr.p;
+            s.wide = c->sk_wide ? 1 : 0;
             s.direct = fx->direct.p; s.slot = c->slot; s.wslot = c->slot; s.wcv = fx->wcv.p;
             s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p; s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms;
             s.n_norm = fx->split ? c->nnorm.p : nullptr;
@@ -2081,7 +2097,7 @@ void enqueue_rest(mdt_fix* fx, double* f, Timer& T) {
             s.k = c->krev.p; s.kvec = c->kd.p; s.amp = c->amp.p; s.kpack = c->kpack.p;
             s.wslot = c->slot; s.wcv = fx->wcv.p; s.n_atoms = c->natoms.p; s.atom_begin = c->abegin.p;
             s.n_walkers = fx->nw(); s.max_atoms = c->max_atoms; s.perm = c->perm.p;
-            s.tbeg = c->tbeg.p; s.tref = c->tref.p; s.dr = c->dr.p;
+            s.tbeg = c->tbeg.p; s.tref = c->tref.p; s.dr = c->dr.p; s.wide = c->sk_wide ? 1 : 0;
             s.n_norm = fx->split ? c->nnorm.p : nullptr;
             Geo gk = g;
             gk.group = g.share;             // a share pushes its own atoms

@@ -2736,7 +2736,10 @@ __device__ __forceinline__ float sin_rev(double ph) { return __builtin_amdgcn_si
 //   k.r_j / 2pi = theta_t(k) + k.(r_j - r_t) / 2pi,   theta_t(k) = frac(k.r_t / 2pi),
 // theta from the first atom's FP64 position once per (tile, k), the rest in FP32 from
 // the FP32 offset r_j - r_t: a few revolutions, so the FP32 phase carries ~1e-5 rad.
-// frac() then keeps the hardware sine's argument in [0, 1) revolutions.
+// The hardware sine and cosine take that argument as it is (on gfx950 they hold 1.2e-7
+// over |x| <= 300 revolutions, measured); when the tiles' extent at the list binding
+// times the largest |k| / 2pi exceeds 128 revolutions (sk.wide), frac() first brings the
+// argument to [0, 1).
 __device__ __forceinline__ float frac_rev(double ph) { return (float)(ph - floor(ph)); }
 
 // A(k) partial sums with the phase in FP64 (at most kSkTiledK k vectors): workgroup (k
@@ -2830,6 +2833,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_prep(Geo g, SkDev sk) {
 
 // A(k) partial sums with tile-relative phases (more than kSkTiledK k vectors):
 // workgroup (k tile, atom chunk); a chunk starts on a phase tile of its walker.
+template <bool FRACT>
 __global__ __launch_bounds__(kBlock) void k_sk_amp_tiled(Geo g, SkDev sk) {
     static_assert(kBlock == kSkSlots, "a thread's r-th atom of a run lies in the chunk's tile r");
     __shared__ double sh[kSkTile * 2 * 4];
@@ -2883,7 +2887,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_amp_tiled(Geo g, SkDev sk) {
             for (int q = 0; q < kSkTile; ++q) {
                 const float ph = __builtin_fmaf(d[r].z, kzf[q], __builtin_fmaf(d[r].y, kyf[q],
                                                 __builtin_fmaf(d[r].x, kxf[q], th[t][q])));
-                const float fr = __builtin_amdgcn_fractf(ph);
+                const float fr = FRACT ? __builtin_amdgcn_fractf(ph) : ph;
                 run[2 * q] += __builtin_amdgcn_cosf(fr);
                 run[2 * q + 1] += __builtin_amdgcn_sinf(fr);
             }
@@ -3109,6 +3113,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
 // The k loop index is uniform, so the packed k records load through the scalar path,
 // and each record feeds AT terms per lane, added in k order.
 constexpr int kSkForceKB = 1024;   // k vectors whose tile phases are staged at a time
+template <bool FRACT>
 __global__ __launch_bounds__(kBlock) void k_sk_force_tiled(Geo g, SkDev sk, double* f) {
     static_assert(kBlock == kSkSlots, "thread t takes slot t of each of the workgroup's tiles");
     constexpr int AT = kSkAtoms, KB = kSkForceKB;
@@ -3155,7 +3160,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_force_tiled(Geo g, SkDev sk, doub
             for (int u = 0; u < AT; ++u) {
                 const float ph = __builtin_fmaf(dz[u], k.krf[2], __builtin_fmaf(dy[u], k.krf[1],
                                                 __builtin_fmaf(dx[u], k.krf[0], th[u][qq])));
-                const float imv = k.amod * __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(ph));
+                const float imv = k.amod * __builtin_amdgcn_sinf(FRACT ? __builtin_amdgcn_fractf(ph) : ph);
                 gx[u] = __builtin_fmaf(k.kv[0], imv, gx[u]);
                 gy[u] = __builtin_fmaf(k.kv[1], imv, gy[u]);
                 gz[u] = __builtin_fmaf(k.kv[2], imv, gz[u]);
@@ -3407,7 +3412,8 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
             hipError_t e0 = hipGetLastError();
             if (e0 != hipSuccess) return e0;
         }
-        hipLaunchKernelGGL(k_sk_amp_tiled, grid, dim3(kBlock), 0, st, g, s);
+        if (s.wide) hipLaunchKernelGGL(k_sk_amp_tiled<true>, grid, dim3(kBlock), 0, st, g, s);
+        else hipLaunchKernelGGL(k_sk_amp_tiled<false>, grid, dim3(kBlock), 0, st, g, s);
     } else {
         hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
     }
@@ -3438,7 +3444,8 @@ hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t 
     // grid.y = walkers, grid.x covers the largest walker
     if (s.nk > kSkTiledK) {
         dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
-        hipLaunchKernelGGL(k_sk_force_tiled, grid, dim3(kBlock), 0, st, g, s, f);
+        if (s.wide) hipLaunchKernelGGL(k_sk_force_tiled<true>, grid, dim3(kBlock), 0, st, g, s, f);
+        else hipLaunchKernelGGL(k_sk_force_tiled<false>, grid, dim3(kBlock), 0, st, g, s, f);
     } else if (s.nk <= kSkFewK) {
         dim3 grid((s.max_atoms + kBlock - 1) / kBlock, s.n_walkers);
         hipLaunchKernelGGL(k_sk_force<1>, grid, dim3(kBlock), 0, st, g, s, f);

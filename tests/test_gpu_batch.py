@@ -21,14 +21,16 @@ def _kvecs(seed, nrand, a=5.26):
     return np.array(ks)
 
 
-@pytest.mark.parametrize("nrand", [40, 300])
-def test_structure_factor_value_and_forces(mdt, orc, nrand):
+@pytest.mark.parametrize("nrand,kscale", [(40, 1.0), (300, 1.0), (300, 6.0)])
+def test_structure_factor_value_and_forces(mdt, orc, nrand, kscale):
     """48 k vectors take the FP64 phase; 308 (above 256) the tile-relative FP32 phase
-    over atoms in Hilbert order (the force written through that order)."""
+    over atoms in Hilbert order (the force written through that order), with the
+    phases used as they are, or, with |k| six times larger (tile phases beyond 128
+    revolutions), reduced to [0, 1) first."""
     s = fcc_system(mdt, 6, sigma=0.3, seed=12)
     x, off, idx, own = s.arrays()
     n = s.n_local
-    k = _kvecs(3, nrand)
+    k = _kvecs(3, nrand) * kscale
     kv = " ".join(f"KVEC {a:.17g} {b:.17g} {c:.17g}" for a, b, c in k)
     ctx = mdt.Context(0, 1)
     ctx.bind_system(0, s)
