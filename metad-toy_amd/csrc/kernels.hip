@@ -2433,40 +2433,81 @@ __global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
     if (valid) d.S[i] = live ? S : 0.0;
 }
 
+// FP64 sum over an atom's lane group (quad butterfly through DPP; every lane gets it)
+__device__ __forceinline__ double grp_sum_f64(double v) {
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    return v;
+}
+
+// LQ_lm(i) = (q_lm(i) + sum_e sigma_e q_lm(owner(e))) / (S_i + 1), kLanes lanes per atom:
+// lane l gathers entries l, l + 4, ... of the atom's list, the group adds the shares
+// (four short gather chains per atom instead of one long one; splitting each row's
+// components over the lanes instead was slower).
 template <int L>
-__global__ __launch_bounds__(64) void k_lq_lq(Geo g, LqDev d) {
+__global__ __launch_bounds__(kBlock) void k_lq_lq(Geo g, LqDev d) {
     constexpr int LS = 2 * (L + 1);
+    __shared__ double sh[2 * (kBlock / 64)];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
-    const bool valid = (int)threadIdx.x < tl.count;
-    const int i = tl.start + (valid ? (int)threadIdx.x : 0);
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     const bool in_g = valid && (g.group == nullptr || g.group[i]);
+    const double Si = d.S[i];
+    const bool live = in_g && Si > 0.0;          // uniform within the group
+    const double* __restrict__ qa = d.q;
+    double acc[LS];
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) acc[k] = 0.0;
+    if (live) {
+        const int beg = g.off[i], end = g.off[i + 1];
+        const int64_t ap = d.apos[i];
+        int e = beg + lane;
+        for (; e + kLanes < end; e += 2 * kLanes) {
+            const int64_t e0 = ap + 64 * (int64_t)(e - beg), e1 = e0 + 64 * kLanes;
+            const double s0 = d.es[e0].x, s1 = d.es[e1].x;
+            const double* r0 = qa + (int64_t)d.eown[e0] * LS;
+            const double* r1 = qa + (int64_t)d.eown[e1] * LS;
+            double v0[LS], v1[LS];
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) { v0[k] = r0[k]; v1[k] = r1[k]; }
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) acc[k] += s0 * v0[k] + s1 * v1[k];
+        }
+        if (e < end) {
+            const int64_t e0 = ap + 64 * (int64_t)(e - beg);
+            const double s0 = d.es[e0].x;
+            const double* r0 = qa + (int64_t)d.eown[e0] * LS;
+            #pragma unroll
+            for (int k = 0; k < LS; ++k) acc[k] += s0 * r0[k];
+        }
+    }
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) acc[k] = grp_sum_f64(acc[k]);
     double v = 0.0;
-    if (in_g && d.S[i] > 0.0) {
-        const double* __restrict__ qa = d.q;
-        double acc[LS];
-        #pragma unroll
-        for (int k = 0; k < LS; ++k) acc[k] = qa[(int64_t)i * LS + k];
-        gather_weighted<LS>(d.eown, d.es, qa, g.off[i], g.off[i + 1], d.apos[i], acc);
-        const double inv = 1.0 / (d.S[i] + 1.0);
+    if (live) {
+        const double inv = 1.0 / (Si + 1.0);
         double sq = 0.0;
         #pragma unroll
         for (int k = 0; k < LS; ++k) {
-            acc[k] *= inv;
-            d.LQ[(int64_t)i * LS + k] = acc[k];
-            sq += (k < 2 ? 1.0 : 2.0) * acc[k] * acc[k];
+            const double lq = (acc[k] + qa[(int64_t)i * LS + k]) * inv;
+            if (k % kLanes == lane) d.LQ[(int64_t)i * LS + k] = lq;
+            sq += (k < 2 ? 1.0 : 2.0) * lq * lq;
         }
         v = sqrt(sq * (4.0 * kPi) / (double)(2 * L + 1));
     } else if (valid) {
         #pragma unroll
-        for (int k = 0; k < LS; ++k) d.LQ[(int64_t)i * LS + k] = 0.0;
+        for (int k = 0; k < LS; ++k)
+            if (k % kLanes == lane) d.LQ[(int64_t)i * LS + k] = 0.0;
     }
-    if (valid) d.ql[i] = v;
-    const double f = in_g ? sw_f(d.swcv, v) : 0.0;
-    const double sq = wave_sum(in_g ? v * f : 0.0);
-    const double sf = wave_sum(f);
-    if (threadIdx.x == 0) { d.part[2 * t] = sq; d.part[2 * t + 1] = sf; }
+    if (valid && lane == 0) d.ql[i] = v;
+    const bool one = in_g && lane == 0;            // each atom counted once
+    const double f = one ? sw_f(d.swcv, v) : 0.0;
+    double part[2] = {one ? v * f : 0.0, f};
+    block_sum<2>(part, sh);
+    if (threadIdx.x == 0) { d.part[2 * t] = part[0]; d.part[2 * t + 1] = part[1]; }
 }
 
 // per walker: sum q f and sum f over the tiles in a fixed order -> CV (one workgroup)
@@ -2625,25 +2666,30 @@ __global__ void k_lq_erev(Geo g, int64_t n, const int32_t* __restrict__ reown, c
     }
 }
 
-// F_a += sum over a's entries of g_e - sum of g at their reverse entries
-__global__ __launch_bounds__(64) void k_lq_force(Geo g, LqDev d, double* __restrict__ f) {
+// F_a += sum over a's entries of g_e - sum of g at their reverse entries, kLanes lanes
+// per atom (lane l takes entries l, l + 4, ...; the group adds the shares)
+__global__ __launch_bounds__(kBlock) void k_lq_force(Geo g, LqDev d, double* __restrict__ f) {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
-    if ((int)threadIdx.x >= tl.count) return;
-    const int i = tl.start + threadIdx.x;
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     const double* __restrict__ gg = d.g;
-    const int len = g.off[i + 1] - g.off[i];
+    const int len = valid ? g.off[i + 1] - g.off[i] : 0;
     const int64_t ap = d.apos[i];
     double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-    for (int k = 0; k < len; ++k) {
+    for (int k = lane; k < len; k += kLanes) {
         const int64_t el = ap + 64 * (int64_t)k;
         const int64_t er = d.erev[el];
         F0 += gg[3 * el]; F1 += gg[3 * el + 1]; F2 += gg[3 * el + 2];
         if (er < 0) continue;
         F0 -= gg[3 * er]; F1 -= gg[3 * er + 1]; F2 -= gg[3 * er + 2];
     }
-    f[3 * (int64_t)i] += F0; f[3 * (int64_t)i + 1] += F1; f[3 * (int64_t)i + 2] += F2;
+    F0 = grp_sum_f64(F0); F1 = grp_sum_f64(F1); F2 = grp_sum_f64(F2);
+    if (valid && lane == 0) {
+        f[3 * (int64_t)i] += F0; f[3 * (int64_t)i + 1] += F1; f[3 * (int64_t)i + 2] += F2;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3505,11 +3551,11 @@ hipError_t launch_lq_l(int stage, const Geo& g, const LqDev& d, double* f, hipSt
     if (g.ntiles == 0) return hipGetLastError();
     switch (stage) {
         case 0: hipLaunchKernelGGL(k_lq_q<L>, gr, bl, 0, st, g, d); break;
-        case 1: hipLaunchKernelGGL(k_lq_lq<L>, gr, bl, 0, st, g, d); break;
+        case 1: hipLaunchKernelGGL(k_lq_lq<L>, gr, dim3(kBlock), 0, st, g, d); break;
         case 3: hipLaunchKernelGGL(k_lq_k<L>, gr, bl, 0, st, g, d); break;
         case 4: hipLaunchKernelGGL(k_lq_h<L>, gr, bl, 0, st, g, d); break;
         case 5: hipLaunchKernelGGL(k_lq_pair<L>, gr, bl, 0, st, g, d); break;
-        case 6: hipLaunchKernelGGL(k_lq_force, gr, bl, 0, st, g, d, f); break;
+        case 6: hipLaunchKernelGGL(k_lq_force, gr, dim3(kBlock), 0, st, g, d, f); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
