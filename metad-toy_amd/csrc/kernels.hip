@@ -457,6 +457,13 @@ __device__ __forceinline__ float grp_sum(float v) {
     v += __uint_as_float(dpp_u32<0x4E>(__float_as_uint(v)));
     return v;
 }
+// FP64 sum over an atom's lane group (quad butterfly through DPP; every lane gets it)
+__device__ __forceinline__ double grp_sum_f64(double v) {
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    return v;
+}
+
 __device__ __forceinline__ M128 grp_bcast0(M128 m) {
     const int src = (int)(threadIdx.x & ~(unsigned)(kLanes - 1)) & 63;
     m.lo = (uint64_t)__shfl((long long)m.lo, src, 64);
@@ -2433,13 +2440,6 @@ __global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
     if (valid) d.S[i] = live ? S : 0.0;
 }
 
-// FP64 sum over an atom's lane group (quad butterfly through DPP; every lane gets it)
-__device__ __forceinline__ double grp_sum_f64(double v) {
-    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
-    return v;
-}
-
 // LQ_lm(i) = (q_lm(i) + sum_e sigma_e q_lm(owner(e))) / (S_i + 1), kLanes lanes per atom:
 // lane l gathers entries l, l + 4, ... of the atom's list, the group adds the shares
 // (four short gather chains per atom instead of one long one; splitting each row's
@@ -2560,29 +2560,27 @@ __global__ __launch_bounds__(64) void k_lq_k(Geo g, LqDev d) {
 // reverse entry of e needs (i = owner(j), j = a): the pair pass reads 8 bytes there instead
 // of gathering a neighbour's q row
 template <int L>
-__global__ __launch_bounds__(64) void k_lq_h(Geo g, LqDev d) {
+__global__ __launch_bounds__(kBlock) void k_lq_h(Geo g, LqDev d) {
+    // kLanes lanes per atom: lane l takes entries l, l + 4, ...; the group adds the shares of H
     constexpr int LS = 2 * (L + 1);
-    __shared__ double sh[64 * LS];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
-    const bool valid = (int)threadIdx.x < tl.count;
-    const int i = tl.start + (valid ? (int)threadIdx.x : 0);
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const bool valid = a < tl.count;
+    const int i = tl.start + (valid ? a : 0);
     const double* __restrict__ Ka = d.K;
     const double* __restrict__ qa = d.q;
     double H[LS], qi[LS];
-    double hq = 0.0, kl = 0.0;
     #pragma unroll
-    for (int k = 0; k < LS; ++k) { H[k] = Ka[(int64_t)i * LS + k]; qi[k] = qa[(int64_t)i * LS + k]; }
-    #pragma unroll
-    for (int k = 0; k < LS; ++k) kl += H[k] * d.LQ[(int64_t)i * LS + k];
+    for (int k = 0; k < LS; ++k) { H[k] = 0.0; qi[k] = qa[(int64_t)i * LS + k]; }
     if (valid) {
         // u on every entry, counted here or not: the reverse entry may count where this one
         // falls below cutoff_eps by rounding, and u does not depend on sigma
         const bool live = d.S[i] > 0.0;
         const int beg = g.off[i], end = g.off[i + 1];
         const int64_t ap = d.apos[i];
-        for (int e = beg; e < end; ++e) {
+        for (int e = beg + lane; e < end; e += kLanes) {
             const int64_t el = ap + 64 * (int64_t)(e - beg);
             const double s = live ? d.es[el].x : 0.0;
             const double* ko = Ka + (int64_t)d.eown[el] * LS;
@@ -2594,29 +2592,41 @@ __global__ __launch_bounds__(64) void k_lq_h(Geo g, LqDev d) {
             for (int k = 0; k < LS; ++k) { H[k] += s * v[k]; u += v[k] * qi[k]; }
             d.u[el] = u;
         }
-        #pragma unroll
-        for (int k = 0; k < LS; ++k) hq += H[k] * qi[k];
     }
-    if (valid) d.hk[i] = make_double2(hq, kl);
-    tile_store_rows<LS>(d.H + (int64_t)tl.start * LS, H, tl.count, sh);
+    double hq = 0.0, kl = 0.0;
+    #pragma unroll
+    for (int k = 0; k < LS; ++k) {
+        const double ki = Ka[(int64_t)i * LS + k];
+        H[k] = grp_sum_f64(H[k]) + ki;
+        kl += ki * d.LQ[(int64_t)i * LS + k];
+        hq += H[k] * qi[k];
+    }
+    if (valid) {
+        if (lane == 0) d.hk[i] = make_double2(hq, kl);
+        #pragma unroll
+        for (int k = 0; k < LS; ++k)
+            if (k % kLanes == lane) d.H[(int64_t)i * LS + k] = H[k];
+    }
 }
 
 // per entry of a: g_e = (1/S_a) [s' <H_a, Y - q_a> n + s grad_d <H_a, Y>] + s' (<K_a, q_j> - <K_a, LQ_a>) n,
 // <K_a, q_j> read at the reverse entry (u, formed by k_lq_h)
 template <int L>
-__global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
+__global__ __launch_bounds__(kBlock) void k_lq_pair(Geo g, LqDev d) {
+    // kLanes lanes per atom: lane l takes entries l, l + 4, ... (each entry's g is its own)
     constexpr int LS = 2 * (L + 1);
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= g.ntiles) return;
     const Tile tl = g.tiles[t];
-    if ((int)threadIdx.x >= tl.count) return;
-    const int i = tl.start + threadIdx.x;
+    const int a = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    if (a >= tl.count) return;
+    const int i = tl.start + a;
     const int beg = g.off[i], end = g.off[i + 1];
     const int64_t ap = d.apos[i];
     const double Si = d.S[i];
     double* __restrict__ go = d.g;
     if (!(Si > 0.0)) {
-        for (int e = beg; e < end; ++e) {
+        for (int e = beg + lane; e < end; e += kLanes) {
             const int64_t o = 3 * (ap + 64 * (int64_t)(e - beg));
             go[o] = 0.0; go[o + 1] = 0.0; go[o + 2] = 0.0;
         }
@@ -2628,7 +2638,7 @@ __global__ __launch_bounds__(64) void k_lq_pair(Geo g, LqDev d) {
     for (int k = 0; k < LS; ++k) H[k] = d.H[(int64_t)i * LS + k];
     const double2 hk = d.hk[i];
     const double iS = 1.0 / Si;
-    for (int e = beg; e < end; ++e) {
+    for (int e = beg + lane; e < end; e += kLanes) {
         const int64_t el = ap + 64 * (int64_t)(e - beg);
         const double2 sd = d.es[el];
         double G0 = 0.0, G1 = 0.0, G2 = 0.0;
@@ -3553,8 +3563,8 @@ hipError_t launch_lq_l(int stage, const Geo& g, const LqDev& d, double* f, hipSt
         case 0: hipLaunchKernelGGL(k_lq_q<L>, gr, bl, 0, st, g, d); break;
         case 1: hipLaunchKernelGGL(k_lq_lq<L>, gr, dim3(kBlock), 0, st, g, d); break;
         case 3: hipLaunchKernelGGL(k_lq_k<L>, gr, bl, 0, st, g, d); break;
-        case 4: hipLaunchKernelGGL(k_lq_h<L>, gr, bl, 0, st, g, d); break;
-        case 5: hipLaunchKernelGGL(k_lq_pair<L>, gr, bl, 0, st, g, d); break;
+        case 4: hipLaunchKernelGGL(k_lq_h<L>, gr, dim3(kBlock), 0, st, g, d); break;
+        case 5: hipLaunchKernelGGL(k_lq_pair<L>, gr, dim3(kBlock), 0, st, g, d); break;
         case 6: hipLaunchKernelGGL(k_lq_force, gr, dim3(kBlock), 0, st, g, d, f); break;
         default: return hipErrorInvalidValue;
     }
