@@ -45,7 +45,7 @@ def test_attached_and_device_positions_match_host_positions(mdt):
         xl = xl + rng.normal(scale=0.01, size=xl.shape)
     print(f"[ingest] n={n} steps={len(traj)}", flush=True)
     out = {}
-    for mode in ("host", "device", "attached", "attached_graph"):
+    for mode in ("host", "device", "attached", "attached_graph", "attached_plain"):
         ctx = mdt.Context(0, 1)
         ctx.bind_system(0, s)
         fix = mdt.Fix(ctx, q46_fix(pace=2, wt=1))
@@ -61,8 +61,8 @@ def test_attached_and_device_positions_match_host_positions(mdt):
                 buf.put(xa)
                 if mode == "device":
                     ctx.update_positions_device(0, buf.p.value)
-            if mode == "attached_graph":
-                fix.run(step, 1, per_launch=1)
+            if mode in ("attached_graph", "attached_plain"):
+                fix.run(step, 1, per_launch=1 if mode == "attached_graph" else 0)
                 fix.sync()
             else:
                 fix.post_force(step)
@@ -74,7 +74,7 @@ def test_attached_and_device_positions_match_host_positions(mdt):
         ctx.close()
     f0, c0 = out["host"]
     assert np.abs(f0).max() > 0.0
-    for mode in ("device", "attached", "attached_graph"):
+    for mode in ("device", "attached", "attached_graph", "attached_plain"):
         f1, c1 = out[mode]
         assert np.array_equal(f0, f1), mode
         assert c0 == c1, mode

@@ -189,6 +189,40 @@ def test_ld_unbiased_alongside_forces(mdt, orc):
     assert abs(fix.cv("LQ6.AVE") - lq.mean()) <= CV_TOL * lq.mean()
 
 
+def test_ld_unbiased_moving_atoms_every_step(mdt, orc):
+    """The fused local Q6 with atoms moving between steps: the next step's reduce kernel
+    sums a step's partials and a state read sums the last step's, so the value read after
+    each step (steps launched one by one, no graph) is that step's, against the oracle on
+    the step's positions, and the same bits as through post_force."""
+    s = fcc_system(mdt, 8, sigma=0.3, sigma_above=0.1, seed=19)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    mv = Mover(x, n, own)
+    rng = np.random.default_rng(6)
+    args = ("GAUSSIAN 0.01 0.005 10 PACE 2 HILLS_FILE NONE "
+            "CAL NAME Q6 STEINH Q 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME Q4 STEINH Q 4 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CAL NAME LQ6 STEINH LD 6 all cutoff_r 3.9 cutoff_Natoms 18 "
+            "CV_dim 1 SYMBOL v1 Q6.AVE SYMBOL v2 Q4.AVE SYMBOL v3 LQ6.AVE DIM 1 0 2 2000 v1+v2 WT 1")
+    ctx_a, ctx_b = _ctx(mdt, s), _ctx(mdt, s)
+    fa, fb = mdt.Fix(ctx_a, args), mdt.Fix(ctx_b, args)
+    fa.init(); fb.init()
+    xl = x[:n].copy()
+    for step in range(5):
+        xa = mv.set_local(xl)
+        ctx_a.update_positions(0, xa)
+        ctx_b.update_positions(0, xa)
+        fa.run(step, 1, 0)
+        fa.sync()
+        fb.post_force(step)
+        lq = orc.Env(xa, off, idx, own, n, 3.9, 18).ld(6)[1]
+        print(f"[ld moving] step={step} LQ6={fa.cv('LQ6.AVE'):.8f} ref={lq.mean():.8f}", flush=True)
+        assert abs(fa.cv("LQ6.AVE") - lq.mean()) <= CV_TOL * lq.mean(), step
+        assert fa.cv("LQ6.AVE") == fb.cv("LQ6.AVE")
+        xl = xl + rng.normal(scale=0.05, size=xl.shape)
+    fa.close(); fb.close(); ctx_a.close(); ctx_b.close()
+
+
 def test_lockstep_grid_c1(mdt, orc):
     """Lock-step replay: moving atoms, hills every 10 steps (PACE 10), grid, V, dV/ds, forces each step.
 
