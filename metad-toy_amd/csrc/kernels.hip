@@ -2090,7 +2090,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_min(double* v, int n) {
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
         __syncthreads();
     }
     const double mn = sm[0];
@@ -2890,7 +2890,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_prep(Geo g, SkDev sk) {
 // A(k) partial sums with tile-relative phases (more than kSkTiledK k vectors):
 // workgroup (k tile, atom chunk); a chunk starts on a phase tile of its walker.
 template <bool FRACT>
-__global__ __launch_bounds__(kBlock) void k_sk_amp_tiled(Geo g, SkDev sk) {
+__global__ __launch_bounds__(kBlock) void k_sk_amp_tiled(SkDev sk) {
     static_assert(kBlock == kSkSlots, "a thread's r-th atom of a run lies in the chunk's tile r");
     __shared__ double sh[kSkTile * 2 * 4];
     __shared__ float th[kSkMaxTiles][kSkTile];
@@ -3053,7 +3053,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_fold_cv(SkDev sk) {
     }
     __syncthreads();
     const double N = sk.n_norm ? sk.n_norm[w] : (double)sk.n_atoms[w];
-    if (threadIdx.x < nk) {
+    if ((int)threadIdx.x < nk) {
         const int q = threadIdx.x;
         double re = 0.0, im = 0.0;
         #pragma unroll
@@ -3170,7 +3170,7 @@ __global__ __launch_bounds__(kBlock) void k_sk_force(Geo g, SkDev sk, double* f)
 // and each record feeds AT terms per lane, added in k order.
 constexpr int kSkForceKB = 1024;   // k vectors whose tile phases are staged at a time
 template <bool FRACT>
-__global__ __launch_bounds__(kBlock) void k_sk_force_tiled(Geo g, SkDev sk, double* f) {
+__global__ __launch_bounds__(kBlock) void k_sk_force_tiled(SkDev sk, double* f) {
     static_assert(kBlock == kSkSlots, "thread t takes slot t of each of the workgroup's tiles");
     constexpr int AT = kSkAtoms, KB = kSkForceKB;
     __shared__ float th[AT][KB];
@@ -3468,8 +3468,8 @@ hipError_t launch_sk_amp(const Geo& g, const SkDev& s, hipStream_t st) {
             hipError_t e0 = hipGetLastError();
             if (e0 != hipSuccess) return e0;
         }
-        if (s.wide) hipLaunchKernelGGL(k_sk_amp_tiled<true>, grid, dim3(kBlock), 0, st, g, s);
-        else hipLaunchKernelGGL(k_sk_amp_tiled<false>, grid, dim3(kBlock), 0, st, g, s);
+        if (s.wide) hipLaunchKernelGGL(k_sk_amp_tiled<true>, grid, dim3(kBlock), 0, st, s);
+        else hipLaunchKernelGGL(k_sk_amp_tiled<false>, grid, dim3(kBlock), 0, st, s);
     } else {
         hipLaunchKernelGGL(k_sk_amp, grid, dim3(kBlock), 0, st, g, s);
     }
@@ -3500,8 +3500,8 @@ hipError_t launch_sk_force(const Geo& g, const SkDev& s, double* f, hipStream_t 
     // grid.y = walkers, grid.x covers the largest walker
     if (s.nk > kSkTiledK) {
         dim3 grid((s.max_atoms + kBlock * kSkAtoms - 1) / (kBlock * kSkAtoms), s.n_walkers);
-        if (s.wide) hipLaunchKernelGGL(k_sk_force_tiled<true>, grid, dim3(kBlock), 0, st, g, s, f);
-        else hipLaunchKernelGGL(k_sk_force_tiled<false>, grid, dim3(kBlock), 0, st, g, s, f);
+        if (s.wide) hipLaunchKernelGGL(k_sk_force_tiled<true>, grid, dim3(kBlock), 0, st, s, f);
+        else hipLaunchKernelGGL(k_sk_force_tiled<false>, grid, dim3(kBlock), 0, st, s, f);
     } else if (s.nk <= kSkFewK) {
         dim3 grid((s.max_atoms + kBlock - 1) / kBlock, s.n_walkers);
         hipLaunchKernelGGL(k_sk_force<1>, grid, dim3(kBlock), 0, st, g, s, f);
