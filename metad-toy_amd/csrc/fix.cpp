@@ -1396,11 +1396,13 @@ void fix_build(mdt_fix* fx) {
             c->krev.alloc(krev.size()); c->krev.upload(krev.data(), krev.size(), st);
             std::vector<int32_t> ch, cb(nw + 1, 0), na(nw), ab(nw);
             // atoms per chunk of the amplitude pass: about 2,048 workgroups (k tiles x chunks)
-            // whatever the k count, between 256 and 16,384 atoms
+            // whatever the k count, between 2,048 and 16,384 atoms (at least 8 atoms per
+            // thread, so that few k vectors do not leave every workgroup a block sum for
+            // one atom per thread, and the fold few chunk partials)
             int64_t tot_atoms = 0;
             for (int k = 0; k < nw; ++k) tot_atoms += ctx->w[k].n_local;
             const int64_t ktiles = (nk + 7) / 8;
-            int CH = 256;
+            int CH = 2048;
             while (CH < 16384 && (tot_atoms + CH - 1) / CH * ktiles > 2048) CH *= 2;
             c->max_atoms = 0;
             for (int k = 0; k < nw; ++k) {
