@@ -43,6 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 # 140); the weighted q average (k_lq_lq, 28); the weighted K sum and <K_j, q_a> (k_lq_h, 56);
 # the pair gradient <H, Y>, grad <H, Y> and its assembly (k_lq_pair, 282)
 FLOP_LQ_Q, FLOP_LQ_LQ, FLOP_LQ_H, FLOP_LQ_PAIR = 140.0, 28.0, 56.0, 282.0
+LQ_PAIR_ENTRY_B, LQ_PAIR_ATOM_B = 60.0, 180.0   # k_lq_pair algorithmic bytes per list entry, per atom
 METRIC = "atom-steps/s, Q6+Q4 CV+bias forces @256k atoms; NN bias vs hills @<=0.1 kT"
 
 
@@ -458,11 +459,23 @@ def run_lq(mdt, args):
     log("kernel mean ms: " + json.dumps(kms))
     x, off, idx, own = system.arrays()
     entries = float(off[-1])          # RATIONAL r_0 = 3 is above cutoff_eps on the whole 5.9 A list
+    n_ghost = x.shape[0] - n
     fl = FLOP_LQ_PAIR * entries
     t = kms["k_lq_pair"] * 1e-3
-    roof = {"kernel": "k_lq_pair", "bound": "fp64", "achieved": fl / t / 1e12, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": fl / t / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
-            "algorithmic": f"{FLOP_LQ_PAIR:.0f} FLOP x {entries:.0f} list entries per step", "mean_ms": kms["k_lq_pair"]}
+    # k_lq_pair does 282 FLOP per entry over ~69 bytes: below the FP64 balance point (78.6 TFLOP/s
+    # / 8 TB/s = 9.8 FLOP/B), so HBM bounds it.  Algorithmic bytes, each array once: per entry
+    # (sigma, sigma') 16, list index 4, reverse ELL index 8, u 8, gradient written 24; per atom
+    # offset 4, ELL base 8, S 8, position 32, H row 112, (<H,q>, <K,LQ>) 16; per ghost position 32.
+    lq_bytes = LQ_PAIR_ENTRY_B * entries + LQ_PAIR_ATOM_B * n + 32.0 * n_ghost
+    roof = {"kernel": "k_lq_pair", "bound": "hbm", "achieved": lq_bytes / t / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": lq_bytes / t / 1e9 / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic": f"{LQ_PAIR_ENTRY_B:.0f} B x {entries:.0f} list entries + {LQ_PAIR_ATOM_B:.0f} B x {n} atoms "
+                           f"+ 32 B x {n_ghost} ghosts per step",
+            "algorithmic_bytes": lq_bytes, "mean_ms": kms["k_lq_pair"],
+            "fp64_tflops": fl / t / 1e12, "fp64_frac": fl / t / 1e12 / FP64_PEAK_TFLOPS}
+    tp = os.path.join(ROOT, "profiles", "traffic_k_lq_pair_lq.json")
+    if os.path.exists(tp):
+        roof["traffic"] = json.load(open(tp)).get("bytes_per_launch")
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
