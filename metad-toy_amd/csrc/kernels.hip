@@ -2411,9 +2411,16 @@ __global__ __launch_bounds__(64) void k_lq_q(Geo g, LqDev d) {
     double S = 0.0;
     int cnt = 0;
     const int beg = g.off[i];
+    const int end = valid ? g.off[i + 1] : beg;
     const int64_t ap = d.apos[i];
-    for (int e = beg; e < (valid ? g.off[i + 1] : beg); ++e) {
-        const double4 q = g.x4[g.idx[e]];
+    // software pipeline: the next entry's position and the list index after it are
+    // loaded while the current entry is evaluated
+    int jn = beg + 1 < end ? g.idx[beg + 1] : i;
+    double4 qn = g.x4[beg < end ? g.idx[beg] : i];
+    for (int e = beg; e < end; ++e) {
+        const double4 q = qn;
+        qn = g.x4[jn];
+        jn = e + 2 < end ? g.idx[e + 2] : i;
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         double s = 0.0, ds = 0.0;
@@ -2638,14 +2645,36 @@ __global__ __launch_bounds__(kBlock) void k_lq_pair(Geo g, LqDev d) {
     for (int k = 0; k < LS; ++k) H[k] = d.H[(int64_t)i * LS + k];
     const double2 hk = d.hk[i];
     const double iS = 1.0 / Si;
+    // Two-stage software pipeline over the lane's entries: while entry e is evaluated, the
+    // neighbour position and reverse-entry u of e + 4 are in flight, and the (sigma, sigma'),
+    // list index and reverse index of e + 8.
+    auto words = [&](int e, double2& sd, int& j, int64_t& er) {
+        if (e < end) {
+            const int64_t el = ap + 64 * (int64_t)(e - beg);
+            sd = d.es[el]; j = d.eidx[el]; er = d.erev[el];
+        } else {
+            sd = make_double2(0.0, 0.0); j = i; er = -1;
+        }
+    };
+    double2 sd1, sd2;
+    int j1, j2;
+    int64_t er1, er2;
+    words(beg + lane, sd1, j1, er1);
+    words(beg + lane + kLanes, sd2, j2, er2);
+    double4 q1 = g.x4[j1];
+    double u1 = er1 >= 0 ? d.u[er1] : 0.0;
     for (int e = beg + lane; e < end; e += kLanes) {
         const int64_t el = ap + 64 * (int64_t)(e - beg);
-        const double2 sd = d.es[el];
+        const double2 sd = sd1;
+        const double4 q = q1;
+        const double ue = u1;
+        sd1 = sd2;
+        q1 = g.x4[j2];
+        u1 = er2 >= 0 ? d.u[er2] : 0.0;
+        words(e + 2 * kLanes, sd2, j2, er2);
         double G0 = 0.0, G1 = 0.0, G2 = 0.0;
         if (sd.x != 0.0) {
-            const double4 q = g.x4[d.eidx[el]];
-            const int64_t er = d.erev[el];
-            const double kq = (er >= 0 ? d.u[er] : 0.0) - hk.y;
+            const double kq = ue - hk.y;
             const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
             const double r = sqrt(dx * dx + dy * dy + dz * dz), ir = 1.0 / r;
             const double nx = dx * ir, ny = dy * ir, nz = dz * ir;
