@@ -204,3 +204,36 @@ def test_steinh_l_256k_half_melted_against_c_oracle(mdt, orc):
     assert rel_force_err(f, -w * G) <= 1e-9
     fix.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["MEAN_SOLID", "FRAC"])
+def test_steinh_l_dilute_gas_short_and_empty_lists(mdt, orc, mode):
+    """A dilute random gas (about 3 list entries per atom within 5.9 A): many atoms have empty,
+    one- or two-entry lists, so the gathers' software pipelines start, run and end on lists
+    shorter than their depth, and isolated atoms take no part.  CV and forces follow the
+    oracle to the FP64 bars."""
+    L = 83.0
+    s = mdt.System().set_box([0, 0, 0], [L, L, L]).add_uniform(2000, 0.0, L, seed=141).build(5.9)
+    x, off, idx, own = s.arrays()
+    n = s.n_local
+    lens = np.diff(off[: n + 1])
+    assert (lens == 0).sum() > 50 and (lens == 1).sum() > 100 and (lens == 2).sum() > 100, np.bincount(lens)
+    cv = orc.lq_cv(x, off, idx, n, own, 6, SWR[1], SWC[1], 1e-6, mode, None)
+    ctx = mdt.Context(0, 1)
+    ctx.bind_system(0, s)
+    fix = mdt.Fix(ctx, _deck(6, mode, "all"))
+    fix.init()
+    fix.add_hills(np.array([[cv + GRID[mode][2]]]), np.array([0.005]), float(GRID[mode][0]))
+    f = np.zeros((n, 3))
+    print(f"[steinh L dilute] mode={mode} n={n} mean list {lens.mean():.2f} cv={cv:.10f}", flush=True)
+    fix.post_force(0, f)
+    got = fix.cv(f"LQ.{mode}")
+    assert abs(got - cv) <= 1e-12 * abs(cv), (got, cv)
+    w = fix.cv_weight(f"LQ.{mode}")
+    assert w != 0.0
+    F_ref = -w * orc.lq_grad(x, off, idx, n, own, 6, SWR[1], SWC[1], 1e-6, mode, None)
+    assert np.abs(F_ref).max() > 0.0
+    assert rel_force_err(f, F_ref) <= 1e-9
+    assert np.abs(f[lens == 0]).max() == 0.0          # isolated atoms feel no bias force
+    fix.close()
+    ctx.close()
