@@ -122,6 +122,9 @@ def steinhardt_roofline(fix, system, kms, fused_ld):
     name = dom
     roof = {"kernel": name, "bound": "fp32", "achieved": rate, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": rate / FP32_PEAK_TFLOPS, "traffic": None,
+            # code built without packed FP32 instructions issues one wave64 FP32 op per clock per
+            # CU: half the listed peak
+            "peak_without_packed_fp32": FP32_PEAK_TFLOPS / 2, "frac_without_packed_fp32": rate / (FP32_PEAK_TFLOPS / 2),
             "algorithmic": f"{per:.0f} FLOP x {pairs:.0f} selected pairs per step", "mean_ms": tms[dom],
             "selection_hbm_GBs": bytes_sel / (kms["k_env"] * 1e-3) / 1e9 if "k_env" in kms else None,
             "selection_bytes": bytes_sel}
